@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""bench.py -- tiled SpGEMM throughput on one MI355X.
+
+Workload (default): C = A*A, A = 5-point Laplacian on a 2048 x 2048 grid
+(4,194,304 rows, 20,963,328 nonzeros, values 4/-1), FP64, in 16x16 tiles.
+A and B are built on the GPU (CSR -> tiles) before timing; a step is one full
+product C = A*B from tiles resident in HBM to C's tiles in HBM (Steps 1-3:
+structure, symbolic, numeric, with their scans), as the reference times it
+(src/spgemm_raw_v2.cu:111-240).
+
+value = GFLOP/s = 2 * products / step time (products = sum_{(i,k) in A} nnz(B[k,:])).
+The JSON line also carries the pipeline's algorithmic HBM GB/s, the roofline
+of the dominant kernel (HIP-event duration of the numeric launch on the
+library's stream) and the CPU baseline (the oracle's Gustavson loop, the
+reference's CPU check, on this host).
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def make_workload(g, name):
+    if name == "lap2d_2048":
+        A = g.gen_laplace2d(2048, 2048)
+        return A, A, {"workload": "laplace2d_5pt_2048x2048_AxA", "rows": A.m, "nnzA": A.nnz, "dtype": "f64"}
+    if name == "er4096":
+        A = g.gen_er(4096, 8, seed=0)
+        return A, A, {"workload": "erdos_renyi_4096_8perrow_AxA", "rows": A.m, "nnzA": A.nnz, "dtype": "f64"}
+    if name == "stencil27_AP":
+        A = g.gen_stencil27(128, 128, 128)
+        P = g.gen_aggregation(128, 128, 128)
+        return A, P, {"workload": "stencil27_128cube_AxP", "rows": A.m, "nnzA": A.nnz, "dtype": "f64"}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def alg_bytes(A, B, C, s):
+    """Bytes of A, B read once and C written once in the reference's layout
+    (4-byte offsets): Bytes(A)+Bytes(B)+Bytes(C)."""
+    ba = 4 * (A["tilem"] + 1) + 24 * A["numtile"] + 4 + (1 + s) * A["nnz"]
+    bb = 4 * (B["tilen"] + 1) + 4 * (B["tilem"] + 1) + 60 * B["numtile"] + 4 + (1 + s) * B["nnz"]
+    bc = 4 * (C["tilem"] + 1) + 60 * C["numtile"] + 4 + (1 + s) * C["nnz"]
+    return ba + bb + bc
+
+
+def numeric_bytes(A, B, C, s):
+    """Algorithmic bytes of one numeric launch in this library's layout: A and
+    B tile headers + payload read once, C's structure read once, C's col bytes
+    and values written once."""
+    a = 8 * (A["tilem"] + 1) + (4 + 8 + 16) * A["numtile"] + (1 + s) * A["nnz"]
+    b = 8 * (B["tilen"] + 1) + (4 + 8 + 16 + 32) * B["numtile"] + s * B["nnz"]
+    c = (4 + 4 + 8 + 32 + 16) * C["numtile"] + (1 + s) * C["nnz"]
+    return a + b + c
+
+
+def cpu_baseline(A, B, seconds):
+    from oracle import oracle as O
+    m = A.m
+    times = []
+    prod = 0
+    t_end = time.time() + seconds
+    while time.time() < t_end or len(times) < 3:
+        prod, gus, _ = O.cpu_baseline(A, B, 0, m, nthreads=1, convert=False)
+        times.append(gus)
+        if len(times) >= 7:
+            break
+    med = statistics.median(times)
+    return {"value": round(2.0 * prod / med / 1e9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"whole product, Gustavson FP64 loop of the reference's CPU check "
+                      f"(src/spgemm_raw_v2.cu:297-308), 1 thread, median of {len(times)} runs "
+                      f"({med:.3f} s each, {prod} products)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="lap2d_2048")
+    ap.add_argument("--scheduler", default="plain", choices=["plain", "learned"])
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
+              f"(the product is a single-GPU kernel path; sharding is described in DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import gpu_spgemm_amd as g
+    A, B, cfg = make_workload(g, args.workload)
+    products = A.products(B)
+    log(f"workload {cfg['workload']}: products={products}")
+    ctx = g.Context(0)
+    log("device", ctx.device_name())
+    tA = ctx.tiles_from_csr(A)
+    tB = ctx.tiles_from_csr(B, as_b=True)
+    sched = g.SCHED_LEARNED if args.scheduler == "learned" else g.SCHED_PLAIN
+    s = 8
+
+    for _ in range(args.warmup):
+        C, st = ctx.spgemm(tA, tB, scheduler=sched)
+        C.free()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        C, st = ctx.spgemm(tA, tB, scheduler=sched)
+        C.free()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall * 1e3 / args.steps
+    log(f"timed {args.steps} steps: {ms:.4f} ms/step")
+
+    # per-step HIP-event breakdown (events on the library stream)
+    br = []
+    for _ in range(max(5, min(args.steps, 20))):
+        C, st = ctx.spgemm(tA, tB, scheduler=sched, timing=True)
+        br.append(st)
+        Ad = {"tilem": tA.s.tilem, "tilen": tA.s.tilen, "numtile": tA.s.numtile, "nnz": tA.s.nnz}
+        Bd = {"tilem": tB.s.tilem, "tilen": tB.s.tilen, "numtile": tB.s.numtile, "nnz": tB.s.nnz}
+        Cd = {"tilem": C.s.tilem, "tilen": C.s.tilen, "numtile": C.s.numtile, "nnz": C.s.nnz}
+        C.free()
+    step3 = statistics.mean(x["ms_step3"] for x in br)
+    step1 = statistics.mean(x["ms_step1"] for x in br)
+    step2 = statistics.mean(x["ms_step2"] for x in br)
+    log(f"events: step1 {step1:.4f} ms, step2 {step2:.4f} ms, step3 {step3:.4f} ms")
+
+    gflops = 2.0 * products / (ms * 1e6)
+    pipe_bytes = alg_bytes(Ad, Bd, Cd, s)
+    nbytes = numeric_bytes(Ad, Bd, Cd, s)
+    achieved = nbytes / (step3 * 1e-3) / 1e9
+    out = {
+        "metric": "SpGEMM GFLOP/s + HBM GB/s, learned/batched vs plain tiled, 1 MI355X",
+        "value": round(gflops, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": cfg["dtype"],
+        "data": "synthetic (generated on the host, converted to tiles on the GPU before timing)",
+        "config": {**cfg, "scheduler": args.scheduler, "products": products,
+                   "numtileC": Cd["numtile"], "nnzC": Cd["nnz"], "parallelism": "single GPU"},
+        "pipeline_alg_GBps": round(pipe_bytes / (ms * 1e-3) / 1e9, 2),
+        "pipeline_alg_bytes": pipe_bytes,
+        "step_ms": {"structure": round(step1, 5), "symbolic": round(step2, 5), "numeric": round(step3, 5)},
+        "roofline": {"bound": "hbm", "kernel": "k_numeric_rows", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "alg_bytes_per_launch": nbytes},
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,379 @@
+"""gpu_spgemm_amd -- MI355X tiled SpGEMM (C = A*B in 16x16 tiles).
+
+Python mirror of the reference's host path (src/spgemm_raw_v2.cu:61-372) over
+the C ABI of libtsg.so (include/tsg.h).  Everything numeric runs in the HIP
+kernels of libtsg; this module only moves arrays and calls entry points.  The
+library must be built (gpu_spgemm_amd.build) -- there is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtsg.so")
+
+F64, F32 = 0, 1
+HOST, DEVICE = 0, 1
+SCHED_PLAIN, SCHED_LEARNED = 0, 1
+
+STATUS = {0: "OK", -1: "ERR_ARG", -2: "ERR_HIP", -3: "ERR_OOM", -4: "ERR_OVERFLOW",
+          -5: "ERR_UNSUPPORTED", -6: "ERR_IO", -7: "ERR_NODEVICE"}
+
+
+class TsgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class CSRStruct(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("nnz", C.c_int64), ("dtype", C.c_int32),
+                ("location", C.c_int32), ("rowptr", C.POINTER(C.c_int64)),
+                ("col", C.POINTER(C.c_int32)), ("val", C.c_void_p)]
+
+
+class TileStruct(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("tilem", C.c_int32), ("tilen", C.c_int32),
+                ("nnz", C.c_int64), ("numtile", C.c_int64), ("dtype", C.c_int32),
+                ("location", C.c_int32), ("is_csc", C.c_int32), ("reserved", C.c_int32),
+                ("tile_ptr", C.POINTER(C.c_int64)), ("tile_columnidx", C.POINTER(C.c_int32)),
+                ("tile_rowidx", C.POINTER(C.c_int32)), ("tile_nnz", C.POINTER(C.c_int64)),
+                ("tile_csr_Val", C.c_void_p), ("tile_csr_Col", C.POINTER(C.c_uint8)),
+                ("tile_csr_Ptr", C.POINTER(C.c_uint8)), ("mask", C.POINTER(C.c_uint16)),
+                ("csc_tile_ptr", C.POINTER(C.c_int64)), ("csc_tile_rowidx", C.POINTER(C.c_int32))]
+
+
+class Options(C.Structure):
+    _fields_ = [("scheduler", C.c_int32), ("numeric_variant", C.c_int32),
+                ("mfma_min_products", C.c_int32), ("timing", C.c_int32), ("pred_scale", C.c_double)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("numtileC", C.c_int64), ("nnzC", C.c_int64), ("products", C.c_uint64),
+                ("ms_step1", C.c_double), ("ms_step2", C.c_double), ("ms_step3", C.c_double),
+                ("ms_total", C.c_double), ("fallback_used", C.c_int32), ("host_syncs", C.c_int32),
+                ("pred_numtileC", C.c_int64), ("pred_nnzC", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Loads libtsg.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `python -m gpu_spgemm_amd.build` first")
+        _lib = C.CDLL(LIB_PATH)
+        _lib.tsg_last_error_string.restype = C.c_char_p
+        _lib.tsg_stream.restype = C.c_void_p
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise TsgError(rc, lib().tsg_last_error_string().decode(errors="replace"))
+
+
+def device_count():
+    n = C.c_int(0)
+    _check(lib().tsg_device_count(C.byref(n)))
+    return n.value
+
+
+def _np(ptr, count, dtype):
+    if count == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dtype))),
+                                 shape=(count,)).copy()
+
+
+class HostCSR:
+    """CSR on the host as numpy arrays (rowptr int64, col int32, val f64/f32)."""
+
+    def __init__(self, m, n, rowptr, col, val):
+        self.m, self.n = int(m), int(n)
+        self.rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+        self.col = np.ascontiguousarray(col, dtype=np.int32)
+        self.val = np.ascontiguousarray(val)
+        if self.val.dtype not in (np.float64, np.float32):
+            self.val = self.val.astype(np.float64)
+
+    @property
+    def nnz(self):
+        return int(self.rowptr[-1])
+
+    @property
+    def dtype(self):
+        return F32 if self.val.dtype == np.float32 else F64
+
+    def struct(self):
+        s = CSRStruct()
+        s.m, s.n, s.nnz, s.dtype, s.location = self.m, self.n, self.nnz, self.dtype, HOST
+        s.rowptr = self.rowptr.ctypes.data_as(C.POINTER(C.c_int64))
+        s.col = self.col.ctypes.data_as(C.POINTER(C.c_int32))
+        s.val = self.val.ctypes.data_as(C.c_void_p)
+        return s
+
+    @classmethod
+    def _take(cls, s):
+        """Copies a library-allocated host CSR into numpy and frees it."""
+        vdt = np.float32 if s.dtype == F32 else np.float64
+        out = cls(s.m, s.n, _np(s.rowptr, s.m + 1, np.int64), _np(s.col, s.nnz, np.int32),
+                  _np(s.val, s.nnz, vdt))
+        _check(lib().tsg_csr_free_host(C.byref(s)))
+        return out
+
+    def astype(self, dtype):
+        vdt = np.float32 if dtype == F32 else np.float64
+        return HostCSR(self.m, self.n, self.rowptr, self.col, self.val.astype(vdt))
+
+    def transpose(self):
+        """B = A^T, the reference's --aat path (src/spgemm_raw_v2.cu:78-89)."""
+        s, o = self.struct(), CSRStruct()
+        _check(lib().tsg_csr_transpose_host(C.byref(s), C.byref(o)))
+        return HostCSR._take(o)
+
+    def products(self, B):
+        """flops = sum_{(i,k) in A} nnz(B[k,:]) (src/spgemm_raw_v2.cu:95-99)."""
+        a, b, f = self.struct(), B.struct(), C.c_uint64(0)
+        _check(lib().tsg_csr_products(C.byref(a), C.byref(b), C.byref(f)))
+        return f.value
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col, self.rowptr), shape=(self.m, self.n))
+
+
+def _gen(fn, *args):
+    o = CSRStruct()
+    _check(fn(*args, C.byref(o)))
+    return HostCSR._take(o)
+
+
+def gen_er(n, per_row, seed=0):
+    return _gen(lib().tsg_gen_er, C.c_int32(n), C.c_int32(per_row), C.c_uint64(seed))
+
+
+def gen_er_var(n, lo, hi, seed=0):
+    return _gen(lib().tsg_gen_er_var, C.c_int32(n), C.c_int32(lo), C.c_int32(hi), C.c_uint64(seed))
+
+
+def gen_laplace2d(nx, ny):
+    return _gen(lib().tsg_gen_laplace2d, C.c_int32(nx), C.c_int32(ny))
+
+
+def gen_laplace3d(nx, ny, nz):
+    return _gen(lib().tsg_gen_laplace3d, C.c_int32(nx), C.c_int32(ny), C.c_int32(nz))
+
+
+def gen_stencil27(nx, ny, nz):
+    return _gen(lib().tsg_gen_stencil27, C.c_int32(nx), C.c_int32(ny), C.c_int32(nz))
+
+
+def gen_aggregation(nx, ny, nz):
+    return _gen(lib().tsg_gen_aggregation, C.c_int32(nx), C.c_int32(ny), C.c_int32(nz))
+
+
+def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, dtype=F32):
+    return _gen(lib().tsg_gen_rmat, C.c_int32(scale), C.c_int32(edge_factor), C.c_double(a),
+                C.c_double(b), C.c_double(c), C.c_uint64(seed), C.c_int32(dtype))
+
+
+def load_mtx(path):
+    """MTX reader with load_mtx semantics (src/tile_format.h:51-90)."""
+    return _gen(lib().tsg_load_mtx, os.fsencode(path))
+
+
+def write_mtx(path, csr):
+    s = csr.struct()
+    _check(lib().tsg_write_mtx(os.fsencode(path), C.byref(s)))
+
+
+class Context:
+    """One GPU, one HIP stream, one device memory pool (tsg_create)."""
+
+    def __init__(self, device=0):
+        self._p = C.c_void_p()
+        _check(lib().tsg_create(C.c_int(device), C.byref(self._p)))
+
+    def close(self):
+        if self._p:
+            lib().tsg_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._p
+
+    def synchronize(self):
+        _check(lib().tsg_synchronize(self._p))
+
+    def device_name(self):
+        buf = C.create_string_buffer(256)
+        _check(lib().tsg_device_name(self._p, buf, C.c_size_t(256)))
+        return buf.value.decode()
+
+    def pool_stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        _check(lib().tsg_pool_stats(self._p, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    # ---- device objects
+    def upload_csr(self, h):
+        return DeviceCSR(self, h)
+
+    def csr2tile(self, dcsr):
+        """csr2tile (src/tile_format.h:93-222) on the GPU."""
+        t = DeviceTiles(self)
+        _check(lib().tsg_csr2tile(self._p, C.byref(dcsr.s), C.byref(t.s)))
+        return t
+
+    def tiles_from_csr(self, h, as_b=False):
+        d = self.upload_csr(h)
+        try:
+            t = self.csr2tile(d)
+        finally:
+            d.free()
+        if as_b:
+            t.build_csc_tiles()
+        return t
+
+    def spgemm(self, A, B, scheduler=SCHED_PLAIN, timing=False, variant=0, pred_scale=1.0,
+               mfma_min_products=0):
+        """C = A*B (Steps 1-3 of src/spgemm_raw_v2.cu:111-240); returns (C, stats dict)."""
+        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale)
+        st = Stats()
+        Cm = DeviceTiles(self)
+        _check(lib().tsg_spgemm(self._p, C.byref(A.s), C.byref(B.s), C.byref(Cm.s), C.byref(o),
+                                C.byref(st)))
+        return Cm, st.as_dict()
+
+
+class DeviceCSR:
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self._host = h
+        self.s = CSRStruct()
+        hs = h.struct()
+        _check(lib().tsg_csr_upload(ctx.handle, C.byref(hs), C.byref(self.s)))
+
+    def free(self):
+        if self.s.rowptr:
+            _check(lib().tsg_csr_free(self.ctx.handle, C.byref(self.s)))
+
+    def __del__(self):
+        try:
+            if self.ctx.handle:
+                self.free()
+        except Exception:
+            pass
+
+
+class DeviceTiles:
+    """A tile matrix in device memory (owned by a Context's pool)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.s = TileStruct()
+
+    def __getattr__(self, k):
+        if k in ("m", "n", "tilem", "tilen", "nnz", "numtile", "dtype", "is_csc"):
+            return getattr(self.s, k)
+        raise AttributeError(k)
+
+    def build_csc_tiles(self):
+        """build_csc_tiles (src/tile_format.h:424-546) on the GPU, in place."""
+        _check(lib().tsg_build_csc_tiles(self.ctx.handle, C.byref(self.s)))
+        return self
+
+    def download(self):
+        """Returns a dict of numpy arrays in the reference's field names."""
+        h = TileStruct()
+        _check(lib().tsg_tile_download(self.ctx.handle, C.byref(self.s), C.byref(h)))
+        nt, nnz = h.numtile, h.nnz
+        vdt = np.float32 if h.dtype == F32 else np.float64
+        d = dict(m=h.m, n=h.n, tilem=h.tilem, tilen=h.tilen, nnz=nnz, numtile=nt, is_csc=h.is_csc,
+                 dtype=h.dtype,
+                 tile_ptr=_np(h.tile_ptr, h.tilem + 1, np.int64),
+                 tile_columnidx=_np(h.tile_columnidx, nt, np.int32),
+                 tile_rowidx=_np(h.tile_rowidx, nt, np.int32) if h.tile_rowidx else None,
+                 tile_nnz=_np(h.tile_nnz, nt + 1, np.int64),
+                 tile_csr_Val=_np(h.tile_csr_Val, nnz, vdt),
+                 tile_csr_Col=_np(h.tile_csr_Col, nnz, np.uint8),
+                 tile_csr_Ptr=_np(h.tile_csr_Ptr, nt * 16, np.uint8),
+                 mask=_np(h.mask, nt * 16, np.uint16))
+        if h.is_csc:
+            d["csc_tile_ptr"] = _np(h.csc_tile_ptr, h.tilen + 1, np.int64)
+            d["csc_tile_rowidx"] = _np(h.csc_tile_rowidx, nt, np.int32)
+        _check(lib().tsg_tile_free_host(C.byref(h)))
+        return d
+
+    def to_csr(self):
+        """Row tiles -> host CSR through the GPU tile2csr kernels."""
+        o, h = CSRStruct(), CSRStruct()
+        _check(lib().tsg_tile2csr(self.ctx.handle, C.byref(self.s), C.byref(o)))
+        try:
+            _check(lib().tsg_csr_download(self.ctx.handle, C.byref(o), C.byref(h)))
+        finally:
+            lib().tsg_csr_free(self.ctx.handle, C.byref(o))
+        return HostCSR._take(h)
+
+    def free(self):
+        if self.s.tile_ptr:
+            _check(lib().tsg_tile_free(self.ctx.handle, C.byref(self.s)))
+
+    def __del__(self):
+        try:
+            if self.ctx.handle:
+                self.free()
+        except Exception:
+            pass
+
+
+def upload_tiles(ctx, d):
+    """Uploads a host tile dict (download() layout) -- upload_tiles, src/spgemm_raw_v2.cu:25-48."""
+    keep = {}
+
+    def ptr(name, ctype):
+        a = d.get(name)
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a)
+        keep[name] = a
+        return a.ctypes.data_as(C.POINTER(ctype))
+
+    h = TileStruct()
+    h.m, h.n, h.tilem, h.tilen = d["m"], d["n"], d["tilem"], d["tilen"]
+    h.nnz, h.numtile, h.dtype, h.location, h.is_csc = d["nnz"], d["numtile"], d["dtype"], HOST, d["is_csc"]
+    h.tile_ptr = ptr("tile_ptr", C.c_int64)
+    h.tile_columnidx = ptr("tile_columnidx", C.c_int32)
+    h.tile_rowidx = ptr("tile_rowidx", C.c_int32)
+    h.tile_nnz = ptr("tile_nnz", C.c_int64)
+    v = np.ascontiguousarray(d["tile_csr_Val"])
+    keep["v"] = v
+    h.tile_csr_Val = v.ctypes.data_as(C.c_void_p)
+    h.tile_csr_Col = ptr("tile_csr_Col", C.c_uint8)
+    h.tile_csr_Ptr = ptr("tile_csr_Ptr", C.c_uint8)
+    h.mask = ptr("mask", C.c_uint16)
+    h.csc_tile_ptr = ptr("csc_tile_ptr", C.c_int64)
+    h.csc_tile_rowidx = ptr("csc_tile_rowidx", C.c_int32)
+    t = DeviceTiles(ctx)
+    _check(lib().tsg_tile_upload(ctx.handle, C.byref(h), C.byref(t.s)))
+    return t

@@ -1,0 +1,406 @@
+// api.hip -- C ABI: context, device pool, transfers and the plain SpGEMM
+// pipeline (Steps 1-3 of src/spgemm_raw_v2.cu:111-240).
+#include <cstdarg>
+#include <cstdlib>
+#include <cstring>
+
+#include "tsg_internal.h"
+
+namespace tsg {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int fail(int code, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int pool_alloc(Context *ctx, void **p, size_t bytes) {
+    size_t r = Pool::round(bytes);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->pool.free_blocks.find(r);
+    if (it != ctx->pool.free_blocks.end()) {
+        *p = it->second;
+        ctx->pool.free_blocks.erase(it);
+        ctx->pool.cached -= (int64_t)r;
+    } else {
+        hipError_t e = hipMalloc(p, r);
+        if (e != hipSuccess) {
+            // release cached blocks and retry once
+            for (auto &kv : ctx->pool.free_blocks) (void)hipFree(kv.second);
+            ctx->pool.free_blocks.clear();
+            ctx->pool.cached = 0;
+            (void)hipGetLastError();
+            e = hipMalloc(p, r);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                *p = nullptr;
+                return fail(TSG_ERR_OOM, "device allocation of %zu bytes failed", r);
+            }
+        }
+    }
+    ctx->pool.live[*p] = r;
+    ctx->pool.in_use += (int64_t)r;
+    return TSG_OK;
+}
+
+int pool_free(Context *ctx, void *p) {
+    if (!p) return TSG_OK;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->pool.live.find(p);
+    if (it == ctx->pool.live.end()) return fail(TSG_ERR_ARG, "pointer %p not owned by this context", p);
+    size_t r = it->second;
+    ctx->pool.live.erase(it);
+    ctx->pool.in_use -= (int64_t)r;
+    ctx->pool.free_blocks.emplace(r, p);
+    ctx->pool.cached += (int64_t)r;
+    return TSG_OK;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() {
+        int cur;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+template <typename T>
+static int up(Context *ctx, const T *src, size_t n, T **dst) {
+    *dst = nullptr;
+    TSG_CHECK(palloc(ctx, dst, n ? n : 1));
+    if (n) TSG_HIP(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream));
+    return TSG_OK;
+}
+
+template <typename T>
+static int down(Context *ctx, const T *src, size_t n, T **dst) {
+    *dst = (T *)malloc(n ? n * sizeof(T) : 1);
+    if (!*dst) return fail(TSG_ERR_OOM, "host allocation failed");
+    if (n) TSG_HIP(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    return TSG_OK;
+}
+
+}  // namespace tsg
+
+using namespace tsg;
+
+extern "C" {
+
+const char *tsg_last_error_string(void) { return g_err.c_str(); }
+
+int tsg_device_count(int *count) {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        c = 0;
+    }
+    *count = c;
+    return TSG_OK;
+}
+
+int tsg_create(int device, tsg_context **out) {
+    if (!out) return fail(TSG_ERR_ARG, "null context pointer");
+    *out = nullptr;
+    int cnt = 0;
+    tsg_device_count(&cnt);
+    if (cnt <= 0) return fail(TSG_ERR_NODEVICE, "no HIP device visible");
+    if (device < 0 || device >= cnt) return fail(TSG_ERR_ARG, "device %d out of range (%d devices)", device, cnt);
+    Context *ctx = new Context();
+    ctx->device = device;
+    TSG_HIP(hipSetDevice(device));
+    TSG_HIP(hipGetDeviceProperties(&ctx->prop, device));
+    TSG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    TSG_HIP(hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault));
+    *out = reinterpret_cast<tsg_context *>(ctx);
+    return TSG_OK;
+}
+
+int tsg_destroy(tsg_context *c) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx) return TSG_OK;
+    DeviceGuard g(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    learned_release(ctx);
+    for (auto &kv : ctx->pool.free_blocks) (void)hipFree(kv.second);
+    for (auto &kv : ctx->pool.live) (void)hipFree(kv.first);
+    (void)hipHostFree(ctx->pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return TSG_OK;
+}
+
+int tsg_synchronize(tsg_context *c) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx) return fail(TSG_ERR_ARG, "null context");
+    DeviceGuard g(ctx->device);
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    return TSG_OK;
+}
+
+int tsg_device_name(tsg_context *c, char *buf, size_t len) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !buf || !len) return fail(TSG_ERR_ARG, "bad arguments");
+    snprintf(buf, len, "%s (%s, %d CUs)", ctx->prop.name, ctx->prop.gcnArchName, ctx->prop.multiProcessorCount);
+    return TSG_OK;
+}
+
+void *tsg_stream(tsg_context *c) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    return ctx ? (void *)ctx->stream : nullptr;
+}
+
+int tsg_pool_stats(tsg_context *c, int64_t *in_use, int64_t *cached) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx) return fail(TSG_ERR_ARG, "null context");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (in_use) *in_use = ctx->pool.in_use;
+    if (cached) *cached = ctx->pool.cached;
+    return TSG_OK;
+}
+
+int tsg_pool_trim(tsg_context *c) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx) return fail(TSG_ERR_ARG, "null context");
+    DeviceGuard g(ctx->device);
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    for (auto &kv : ctx->pool.free_blocks) (void)hipFree(kv.second);
+    ctx->pool.free_blocks.clear();
+    ctx->pool.cached = 0;
+    return TSG_OK;
+}
+
+// ------------------------------------------------------------------ CSR transfers
+int tsg_csr_upload(tsg_context *c, const tsg_csr *h, tsg_csr *d) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !h || !d) return fail(TSG_ERR_ARG, "null argument");
+    if (h->location != TSG_HOST) return fail(TSG_ERR_ARG, "source CSR is not on the host");
+    DeviceGuard g(ctx->device);
+    *d = *h;
+    d->location = TSG_DEVICE;
+    d->rowptr = nullptr; d->col = nullptr; d->val = nullptr;
+    int rc = up(ctx, h->rowptr, (size_t)h->m + 1, &d->rowptr);
+    if (!rc) rc = up(ctx, h->col, (size_t)h->nnz, &d->col);
+    if (!rc) rc = up(ctx, (const uint8_t *)h->val, (size_t)h->nnz * vsize(h->dtype), (uint8_t **)&d->val);
+    if (!rc) { hipError_t e = hipStreamSynchronize(ctx->stream); if (e != hipSuccess) rc = fail(TSG_ERR_HIP, "%s", hipGetErrorString(e)); }
+    return rc;
+}
+
+int tsg_csr_download(tsg_context *c, const tsg_csr *d, tsg_csr *h) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !h || !d) return fail(TSG_ERR_ARG, "null argument");
+    DeviceGuard g(ctx->device);
+    *h = *d;
+    h->location = TSG_HOST;
+    TSG_CHECK(down(ctx, d->rowptr, (size_t)d->m + 1, &h->rowptr));
+    TSG_CHECK(down(ctx, d->col, (size_t)d->nnz, &h->col));
+    TSG_CHECK(down(ctx, (const uint8_t *)d->val, (size_t)d->nnz * vsize(d->dtype), (uint8_t **)&h->val));
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    return TSG_OK;
+}
+
+int tsg_csr_free(tsg_context *c, tsg_csr *d) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !d) return fail(TSG_ERR_ARG, "null argument");
+    if (d->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "CSR is not on the device");
+    pool_free(ctx, d->rowptr);
+    pool_free(ctx, d->col);
+    pool_free(ctx, d->val);
+    memset(d, 0, sizeof *d);
+    return TSG_OK;
+}
+
+// ------------------------------------------------------------------ tile transfers
+int tsg_tile_upload(tsg_context *c, const tsg_tile_matrix *h, tsg_tile_matrix *d) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !h || !d) return fail(TSG_ERR_ARG, "null argument");
+    if (h->location != TSG_HOST) return fail(TSG_ERR_ARG, "source tiles are not on the host");
+    DeviceGuard g(ctx->device);
+    tsg_tile_matrix t = *h;
+    t.location = TSG_DEVICE;
+    size_t nt = (size_t)h->numtile;
+    t.tile_rowidx = nullptr; t.csc_tile_ptr = nullptr; t.csc_tile_rowidx = nullptr;
+    TSG_CHECK(up(ctx, h->tile_ptr, (size_t)h->tilem + 1, &t.tile_ptr));
+    TSG_CHECK(up(ctx, h->tile_columnidx, nt, &t.tile_columnidx));
+    if (h->tile_rowidx) TSG_CHECK(up(ctx, h->tile_rowidx, nt, &t.tile_rowidx));
+    TSG_CHECK(up(ctx, h->tile_nnz, nt + 1, &t.tile_nnz));
+    TSG_CHECK(up(ctx, (const uint8_t *)h->tile_csr_Val, (size_t)h->nnz * vsize(h->dtype), (uint8_t **)&t.tile_csr_Val));
+    TSG_CHECK(up(ctx, h->tile_csr_Col, (size_t)h->nnz, &t.tile_csr_Col));
+    TSG_CHECK(up(ctx, h->tile_csr_Ptr, nt * 16, &t.tile_csr_Ptr));
+    TSG_CHECK(up(ctx, h->mask, nt * 16, &t.mask));
+    if (h->is_csc) {
+        if (!h->csc_tile_ptr || !h->csc_tile_rowidx) return fail(TSG_ERR_ARG, "B operand lacks CSC tile arrays");
+        TSG_CHECK(up(ctx, h->csc_tile_ptr, (size_t)h->tilen + 1, &t.csc_tile_ptr));
+        TSG_CHECK(up(ctx, h->csc_tile_rowidx, nt, &t.csc_tile_rowidx));
+    }
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    *d = t;
+    return TSG_OK;
+}
+
+int tsg_tile_download(tsg_context *c, const tsg_tile_matrix *d, tsg_tile_matrix *h) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !h || !d) return fail(TSG_ERR_ARG, "null argument");
+    if (d->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles are not on the device");
+    DeviceGuard g(ctx->device);
+    tsg_tile_matrix t = *d;
+    t.location = TSG_HOST;
+    size_t nt = (size_t)d->numtile;
+    t.tile_rowidx = nullptr; t.csc_tile_ptr = nullptr; t.csc_tile_rowidx = nullptr;
+    TSG_CHECK(down(ctx, d->tile_ptr, (size_t)d->tilem + 1, &t.tile_ptr));
+    TSG_CHECK(down(ctx, d->tile_columnidx, nt, &t.tile_columnidx));
+    if (d->tile_rowidx) TSG_CHECK(down(ctx, d->tile_rowidx, nt, &t.tile_rowidx));
+    TSG_CHECK(down(ctx, d->tile_nnz, nt + 1, &t.tile_nnz));
+    TSG_CHECK(down(ctx, (const uint8_t *)d->tile_csr_Val, (size_t)d->nnz * vsize(d->dtype), (uint8_t **)&t.tile_csr_Val));
+    TSG_CHECK(down(ctx, d->tile_csr_Col, (size_t)d->nnz, &t.tile_csr_Col));
+    TSG_CHECK(down(ctx, d->tile_csr_Ptr, nt * 16, &t.tile_csr_Ptr));
+    TSG_CHECK(down(ctx, d->mask, nt * 16, &t.mask));
+    if (d->is_csc) {
+        TSG_CHECK(down(ctx, d->csc_tile_ptr, (size_t)d->tilen + 1, &t.csc_tile_ptr));
+        TSG_CHECK(down(ctx, d->csc_tile_rowidx, nt, &t.csc_tile_rowidx));
+    }
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    *h = t;
+    return TSG_OK;
+}
+
+int tsg_tile_free(tsg_context *c, tsg_tile_matrix *d) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !d) return fail(TSG_ERR_ARG, "null argument");
+    if (d->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles are not on the device");
+    void *ps[] = {d->tile_ptr, d->tile_columnidx, d->tile_rowidx, d->tile_nnz, d->tile_csr_Val,
+                  d->tile_csr_Col, d->tile_csr_Ptr, d->mask, d->csc_tile_ptr, d->csc_tile_rowidx};
+    for (void *p : ps) pool_free(ctx, p);
+    memset(d, 0, sizeof *d);
+    return TSG_OK;
+}
+
+int tsg_tile_free_host(tsg_tile_matrix *h) {
+    if (!h) return TSG_OK;
+    if (h->location != TSG_HOST) return fail(TSG_ERR_ARG, "tiles are not on the host");
+    void *ps[] = {h->tile_ptr, h->tile_columnidx, h->tile_rowidx, h->tile_nnz, h->tile_csr_Val,
+                  h->tile_csr_Col, h->tile_csr_Ptr, h->mask, h->csc_tile_ptr, h->csc_tile_rowidx};
+    for (void *p : ps) free(p);
+    memset(h, 0, sizeof *h);
+    return TSG_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ SpGEMM
+namespace tsg {
+
+int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B) {
+    if (!A || !B) return fail(TSG_ERR_ARG, "null operand");
+    if (A->location != TSG_DEVICE || B->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "operands must be on the device");
+    if (!B->is_csc) return fail(TSG_ERR_ARG, "B must be laid out by tsg_build_csc_tiles (is_csc = 0)");
+    if (A->is_csc) return fail(TSG_ERR_ARG, "A must be in row-tile order (is_csc = 1)");
+    if (A->n != B->m) return fail(TSG_ERR_ARG, "inner dimensions differ: A is %dx%d, B is %dx%d", A->m, A->n, B->m, B->n);
+    if (A->dtype != B->dtype) return fail(TSG_ERR_ARG, "A and B value types differ");
+    if (!A->mask) return fail(TSG_ERR_ARG, "A lacks row masks");
+    return TSG_OK;
+}
+
+// Plain method: the reference's three steps with exact counts; the host reads
+// numtileC and nnzC to size C (the reference's own host syncs,
+// src/spgemm_raw_v2.cu:141, :211; its third, the pair-cache count at :192,
+// does not exist here because no pair cache is kept).
+int spgemm_plain(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                 const tsg_options *opts, tsg_stats *st) {
+    const bool timing = opts && opts->timing;
+    hipEvent_t ev[4] = {};
+    if (timing)
+        for (auto &e : ev) TSG_HIP(hipEventCreate(&e));
+    memset(C, 0, sizeof *C);
+    C->m = A->m; C->n = B->n; C->tilem = A->tilem; C->tilen = B->tilen;
+    C->dtype = A->dtype; C->location = TSG_DEVICE;
+    int rc = TSG_OK;
+    int syncs = 0;
+    if (timing) (void)hipEventRecord(ev[0], ctx->stream);
+    // Step 1
+    rc = palloc(ctx, &C->tile_ptr, (size_t)C->tilem + 1);
+    if (!rc) rc = launch_structure_count(ctx, A, B, C->tile_ptr);
+    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_ptr, (int64_t)C->tilem + 1);
+    if (!rc) { rc = read_i64(ctx, C->tile_ptr + C->tilem, &C->numtile); syncs++; }
+    if (!rc && C->numtile > INT32_MAX) rc = fail(TSG_ERR_OVERFLOW, "numtileC %lld exceeds int32 tile ids", (long long)C->numtile);
+    if (!rc) rc = palloc(ctx, &C->tile_rowidx, (size_t)C->numtile);
+    if (!rc) rc = palloc(ctx, &C->tile_columnidx, (size_t)C->numtile);
+    if (!rc) rc = launch_structure_fill(ctx, A, B, C->tile_ptr, C->tile_rowidx, C->tile_columnidx);
+    if (timing && !rc) (void)hipEventRecord(ev[1], ctx->stream);
+    // Step 2
+    if (!rc) rc = palloc(ctx, &C->mask, (size_t)C->numtile * 16);
+    if (!rc) rc = palloc(ctx, &C->tile_csr_Ptr, (size_t)C->numtile * 16);
+    if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)C->numtile + 1);
+    if (!rc) rc = launch_symbolic(ctx, A, B, C->numtile, C->tile_rowidx, C->tile_columnidx, C->mask,
+                                  C->tile_csr_Ptr, C->tile_nnz);
+    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1);
+    if (!rc) { rc = read_i64(ctx, C->tile_nnz + C->numtile, &C->nnz); syncs++; }
+    if (timing && !rc) (void)hipEventRecord(ev[2], ctx->stream);
+    // Step 3
+    if (!rc) rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)C->nnz * vsize(C->dtype));
+    if (!rc) rc = palloc(ctx, &C->tile_csr_Col, (size_t)C->nnz);
+    if (!rc) rc = launch_numeric(ctx, A, B, C, opts);
+    if (timing && !rc) (void)hipEventRecord(ev[3], ctx->stream);
+    if (st) {
+        st->numtileC = C->numtile;
+        st->nnzC = C->nnz;
+        st->host_syncs = syncs;
+        st->fallback_used = 0;
+        st->pred_numtileC = st->pred_nnzC = 0;
+    }
+    if (timing) {
+        if (!rc && st) {
+            float a = 0, b = 0, c3 = 0;
+            (void)hipEventSynchronize(ev[3]);
+            (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+            (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+            (void)hipEventElapsedTime(&c3, ev[2], ev[3]);
+            st->ms_step1 = a; st->ms_step2 = b; st->ms_step3 = c3; st->ms_total = a + b + c3;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    if (rc) {
+        tsg_tile_matrix tmp = *C;
+        void *ps[] = {tmp.tile_ptr, tmp.tile_columnidx, tmp.tile_rowidx, tmp.tile_nnz, tmp.tile_csr_Val,
+                      tmp.tile_csr_Col, tmp.tile_csr_Ptr, tmp.mask};
+        for (void *p : ps) pool_free(ctx, p);
+        memset(C, 0, sizeof *C);
+    }
+    return rc;
+}
+
+int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                   const tsg_options *opts, tsg_stats *st);  // learned.hip
+
+}  // namespace tsg
+
+extern "C" int tsg_spgemm(tsg_context *c, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                          tsg_tile_matrix *C, const tsg_options *opts, tsg_stats *st) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !C) return fail(TSG_ERR_ARG, "null argument");
+    TSG_CHECK(check_operands(A, B));
+    DeviceGuard g(ctx->device);
+    if (st) memset(st, 0, sizeof *st);
+    if (opts && opts->scheduler == TSG_SCHED_LEARNED) return spgemm_learned(ctx, A, B, C, opts, st);
+    return spgemm_plain(ctx, A, B, C, opts, st);
+}

@@ -1,0 +1,544 @@
+// convert.hip -- tile-format conversion on the GPU.
+//
+// tsg_csr2tile replaces the host csr2tile (src/tile_format.h:93-222) and
+// tsg_build_csc_tiles replaces the host build_csc_tiles
+// (src/tile_format.h:424-546).  Output arrays are identical to what those host
+// functions produce for CSR input whose rows are sorted by column (the form
+// load_mtx, the --aat transpose and every generator give); duplicate columns
+// are kept in input order like the reference.
+//
+// csr2tile, one wave64 per 16-row tile-row, entries read coalesced from the
+// contiguous CSR range [rowptr[16bi], rowptr[16bi+16]):
+//   K1 count   -- LDS bitmap over the tile-row's tile-column window; popcount
+//   K2 fill    -- same bitmap: ascending tile_columnidx; the first entry of each
+//                 (tile,row) segment walks its segment (at most 16 distinct
+//                 columns) to produce that row's count and bitmask, so no
+//                 atomics are needed
+//   K3 rowscan -- per tile, 16-lane exclusive scan of row counts -> tile_csr_Ptr
+//                 and the tile's nnz; one int64 scan -> tile_nnz
+//   K4 scatter -- each entry's slot = tile_nnz + tile_csr_Ptr[row] + position in
+//                 its segment; col byte (row<<4)|col
+// build_csc_tiles: tile-level transpose.  Tiles are counted per tile column,
+// dropped into unordered column slots, each column's tile rows sorted with the
+// same LDS-bitmap pass, then every tile's 16-row payload (row pointers, masks,
+// values, local col bytes) is gathered into its CSC position.
+#include <climits>
+#include <cstring>
+
+#include "tsg_internal.h"
+
+namespace tsg {
+
+constexpr int CV_WAVES = 4;
+constexpr int CV_WIN_WORDS = 1024;  // 32,768 tile columns per pass
+
+// ---------------------------------------------------------------- csr2tile
+template <int MODE>  // 0 = count, 1 = fill
+__global__ void __launch_bounds__(CV_WAVES *WAVE)
+k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m, int32_t tilem,
+      int64_t *__restrict__ tile_ptr, int32_t *__restrict__ tile_colidx, uint8_t *__restrict__ rowcnt,
+      uint16_t *__restrict__ mask, int32_t *__restrict__ flags) {
+    __shared__ uint32_t s_bm[CV_WAVES][CV_WIN_WORDS];
+    __shared__ int32_t s_wpre[CV_WAVES][CV_WIN_WORDS];
+    __shared__ int64_t s_rp[CV_WAVES][17];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int32_t bi = blockIdx.x * CV_WAVES + w;
+    if (bi >= tilem) return;
+    const int32_t rs = bi * 16, re = min(rs + 16, m);
+    if (lane <= 16) s_rp[w][lane] = rp[min(rs + lane, re)];  // rows past m repeat the end
+    wave_sync();
+    const int64_t js = s_rp[w][0], je = s_rp[w][re - rs];
+    int32_t cmin = INT_MAX, cmax = -1;
+    if (lane < re - rs) {
+        int64_t a = s_rp[w][lane], b = s_rp[w][lane + 1];
+        if (b > a) { cmin = col[a] >> 4; cmax = col[b - 1] >> 4; }
+    }
+    cmin = wave_min(cmin);
+    cmax = wave_max(cmax);
+    uint32_t *bm = s_bm[w];
+    int64_t count = 0;
+    int64_t tbase = MODE ? tile_ptr[bi] : 0;
+    bool bad = false;
+    for (int64_t w0 = cmin; w0 <= cmax; w0 += (int64_t)CV_WIN_WORDS * 32) {
+        const int32_t wbits = (int32_t)min<int64_t>((int64_t)CV_WIN_WORDS * 32, cmax - w0 + 1);
+        const int nwords = (wbits + 31) >> 5;
+        for (int i = lane; i < nwords; i += WAVE) bm[i] = 0;
+        wave_sync();
+        for (int64_t j = js + lane; j < je; j += WAVE) {
+            int64_t off = (int64_t)(col[j] >> 4) - w0;
+            if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
+        }
+        wave_sync();
+        if (MODE == 0) {
+            int32_t cnt = 0;
+            for (int i = lane; i < nwords; i += WAVE) cnt += __popc(bm[i]);
+            count += wave_sum(cnt);
+        } else {
+            // per-word exclusive tile ranks and ascending tile_columnidx
+            const int wpl = (nwords + WAVE - 1) / WAVE;
+            const int i0 = min(nwords, lane * wpl), i1 = min(nwords, i0 + wpl);
+            int32_t cnt = 0;
+            for (int i = i0; i < i1; i++) cnt += __popc(bm[i]);
+            int32_t incl = wave_incl_sum(cnt);
+            int32_t tot = __shfl(incl, WAVE - 1, WAVE);
+            int32_t run = incl - cnt;
+            for (int i = i0; i < i1; i++) {
+                s_wpre[w][i] = run;
+                uint32_t b = bm[i];
+                while (b) {
+                    int bit = __ffs(b) - 1;
+                    b &= b - 1;
+                    tile_colidx[tbase + run] = (int32_t)(w0 + i * 32 + bit);
+                    run++;
+                }
+            }
+            wave_sync();
+            // segment heads: (tile,row) counts and masks
+            for (int64_t j = js + lane; j < je; j += WAVE) {
+                const int32_t cj = col[j];
+                const int32_t tc = cj >> 4;
+                int64_t off = (int64_t)tc - w0;
+                if (off < 0 || off >= wbits) continue;
+                int r = 0;
+                while (r < 15 && s_rp[w][r + 1] <= j) r++;
+                const int64_t rowbeg = s_rp[w][r], rowend = s_rp[w][r + 1];
+                if (j > rowbeg && (col[j - 1] >> 4) == tc) continue;  // not a segment head
+                const int wi = (int)(off >> 5);
+                const int32_t t = s_wpre[w][wi] + __popc(bm[wi] & ((1u << (off & 31)) - 1u));
+                const int64_t T = tbase + t;
+                uint32_t mk = 0;
+                int32_t n = 0, prev = -1;
+                for (int64_t q = j; q < rowend; q++) {
+                    const int32_t cq = col[q];
+                    if ((cq >> 4) != tc) break;
+                    if (cq < prev) bad = true;
+                    prev = cq;
+                    mk |= 0x8000u >> (cq & 15);
+                    n++;
+                }
+                rowcnt[T * 16 + r] = (uint8_t)n;
+                mask[T * 16 + r] = (uint16_t)mk;
+            }
+            tbase += tot;
+            wave_sync();
+        }
+    }
+    if (MODE == 0) {
+        // rows must be sorted by column: a descending step is rejected
+        for (int64_t j = js + lane; j < je; j += WAVE) {
+            if (j == js) continue;
+            int r = 0;
+            while (r < 15 && s_rp[w][r + 1] <= j) r++;
+            if (j > s_rp[w][r] && col[j - 1] > col[j]) bad = true;
+        }
+        if (lane == 0) {
+            tile_ptr[bi + 1] = count;
+            if (bi == 0) tile_ptr[0] = 0;
+        }
+    }
+    if (__ballot(bad) && lane == 0) atomicOr(flags, 1);
+}
+
+// Per tile: exclusive scan of 16 row counts (in place) and the tile's nnz.
+__global__ void __launch_bounds__(256) k_tile_rowscan(int64_t numtile, uint8_t *__restrict__ ptr,
+                                                      int64_t *__restrict__ tile_nnz) {
+    const int64_t T = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
+    if (T >= numtile) return;
+    const int c = ptr[T * 16 + l];
+    const int incl = group16_incl_sum(c);
+    ptr[T * 16 + l] = (uint8_t)(incl - c);
+    if (l == 15) tile_nnz[T + 1] = incl;
+    if (T == 0 && l == 0) tile_nnz[0] = 0;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(CV_WAVES *WAVE)
+k_c2t_scatter(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, const V *__restrict__ val,
+              int32_t m, int32_t tilem, const int64_t *__restrict__ tile_ptr,
+              const int32_t *__restrict__ tile_colidx, const int64_t *__restrict__ tile_nnz,
+              const uint8_t *__restrict__ ptr, V *__restrict__ tval, uint8_t *__restrict__ tcol) {
+    __shared__ int64_t s_rp[CV_WAVES][17];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int32_t bi = blockIdx.x * CV_WAVES + w;
+    if (bi >= tilem) return;
+    const int32_t rs = bi * 16, re = min(rs + 16, m);
+    if (lane <= 16) s_rp[w][lane] = rp[min(rs + lane, re)];  // rows past m repeat the end
+    wave_sync();
+    const int64_t js = s_rp[w][0], je = s_rp[w][re - rs];
+    const int64_t tb = tile_ptr[bi], te = tile_ptr[bi + 1];
+    for (int64_t j = js + lane; j < je; j += WAVE) {
+        const int32_t cj = col[j];
+        const int32_t tc = cj >> 4;
+        int r = 0;
+        while (r < 15 && s_rp[w][r + 1] <= j) r++;
+        const int64_t T = lower_bound<int64_t, int32_t>(tile_colidx, tb, te, tc);
+        const int64_t seg = lower_bound<int64_t, int32_t>(col, s_rp[w][r], j, tc << 4);
+        const int64_t pos = tile_nnz[T] + ptr[T * 16 + r] + (j - seg);
+        tval[pos] = val[j];
+        tcol[pos] = (uint8_t)((r << 4) | (cj & 15));
+    }
+}
+
+// ---------------------------------------------------------------- helpers
+__global__ void __launch_bounds__(256) k_fill_rowidx(const int64_t *__restrict__ tile_ptr, int32_t tilem,
+                                                     int32_t *__restrict__ rowidx) {
+    const int32_t bi = (int32_t)(((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE);
+    if (bi >= tilem) return;
+    for (int64_t t = tile_ptr[bi] + lane_id(); t < tile_ptr[bi + 1]; t += WAVE) rowidx[t] = bi;
+}
+
+// ---------------------------------------------------------------- build_csc_tiles
+__global__ void __launch_bounds__(256) k_csc_count(int64_t numtile, const int32_t *__restrict__ colidx,
+                                                   int64_t *__restrict__ cptr) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < numtile) atomicAdd((unsigned long long *)&cptr[colidx[t] + 1], 1ull);
+}
+
+__global__ void __launch_bounds__(256) k_csc_slot(int64_t numtile, const int32_t *__restrict__ colidx,
+                                                  const int32_t *__restrict__ rowidx,
+                                                  int64_t *__restrict__ cursor, int32_t *__restrict__ tmp) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= numtile) return;
+    const int64_t s = (int64_t)atomicAdd((unsigned long long *)&cursor[colidx[t]], 1ull);
+    tmp[s] = rowidx[t];
+}
+
+// One wave per tile column: sort its tile rows with an LDS bitmap.
+__global__ void __launch_bounds__(CV_WAVES *WAVE)
+k_csc_sort(int32_t tilen, const int64_t *__restrict__ cptr, const int32_t *__restrict__ tmp,
+           int32_t *__restrict__ crow) {
+    __shared__ uint32_t s_bm[CV_WAVES][CV_WIN_WORDS];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int32_t bt = blockIdx.x * CV_WAVES + w;
+    if (bt >= tilen) return;
+    const int64_t s = cptr[bt], e = cptr[bt + 1];
+    int32_t vmin = INT_MAX, vmax = -1;
+    for (int64_t i = s + lane; i < e; i += WAVE) { vmin = min(vmin, tmp[i]); vmax = max(vmax, tmp[i]); }
+    vmin = wave_min(vmin);
+    vmax = wave_max(vmax);
+    uint32_t *bm = s_bm[w];
+    int64_t out = s;
+    for (int64_t w0 = vmin; w0 <= vmax; w0 += (int64_t)CV_WIN_WORDS * 32) {
+        const int32_t wbits = (int32_t)min<int64_t>((int64_t)CV_WIN_WORDS * 32, vmax - w0 + 1);
+        const int nwords = (wbits + 31) >> 5;
+        for (int i = lane; i < nwords; i += WAVE) bm[i] = 0;
+        wave_sync();
+        for (int64_t i = s + lane; i < e; i += WAVE) {
+            int64_t off = (int64_t)tmp[i] - w0;
+            if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
+        }
+        wave_sync();
+        const int wpl = (nwords + WAVE - 1) / WAVE;
+        const int i0 = min(nwords, lane * wpl), i1 = min(nwords, i0 + wpl);
+        int32_t cnt = 0;
+        for (int i = i0; i < i1; i++) cnt += __popc(bm[i]);
+        int32_t incl = wave_incl_sum(cnt);
+        int32_t tot = __shfl(incl, WAVE - 1, WAVE);
+        int64_t pos = out + incl - cnt;
+        for (int i = i0; i < i1; i++) {
+            uint32_t b = bm[i];
+            while (b) {
+                int bit = __ffs(b) - 1;
+                b &= b - 1;
+                crow[pos++] = (int32_t)(w0 + i * 32 + bit);
+            }
+        }
+        out += tot;
+        wave_sync();
+    }
+}
+
+// inv[p] = CSR tile id of CSC position p; per-tile header copied.
+__global__ void __launch_bounds__(256)
+k_csc_perm(int64_t numtile, const int32_t *__restrict__ colidx, const int32_t *__restrict__ rowidx,
+           const int64_t *__restrict__ cptr, const int32_t *__restrict__ crow,
+           const int64_t *__restrict__ nnz_old, int64_t *__restrict__ inv, int64_t *__restrict__ nnz_new) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= numtile) return;
+    const int32_t tc = colidx[t];
+    const int64_t p = lower_bound<int64_t, int32_t>(crow, cptr[tc], cptr[tc + 1], rowidx[t]);
+    inv[p] = t;
+    nnz_new[p + 1] = nnz_old[t + 1] - nnz_old[t];
+    if (t == 0) nnz_new[0] = 0;
+}
+
+// 16 lanes per CSC tile: row pointers, masks, then values and local cols.
+template <typename V>
+__global__ void __launch_bounds__(256)
+k_csc_gather(int64_t numtile, const int64_t *__restrict__ inv, const int64_t *__restrict__ nnz_old,
+             const int64_t *__restrict__ nnz_new, const uint8_t *__restrict__ ptr_old,
+             const uint16_t *__restrict__ mask_old, const uint8_t *__restrict__ col_old,
+             const V *__restrict__ val_old, uint8_t *__restrict__ ptr_new, uint16_t *__restrict__ mask_new,
+             uint8_t *__restrict__ col_new, V *__restrict__ val_new) {
+    const int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+    const int l = threadIdx.x & 15;
+    if (p >= numtile) return;
+    const int64_t t = inv[p];
+    ptr_new[p * 16 + l] = ptr_old[t * 16 + l];
+    mask_new[p * 16 + l] = mask_old[t * 16 + l];
+    const int64_t so = nnz_old[t], sn = nnz_new[p], n = nnz_old[t + 1] - so;
+    for (int64_t e = l; e < n; e += 16) {
+        val_new[sn + e] = val_old[so + e];
+        col_new[sn + e] = col_old[so + e] & 15;
+    }
+}
+
+static unsigned blocks_for(int64_t threads, int per) { return (unsigned)((threads + per - 1) / per); }
+
+}  // namespace tsg
+
+using namespace tsg;
+
+extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !A || !T) return fail(TSG_ERR_ARG, "null argument");
+    if (A->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "CSR must be on the device");
+    if (A->m < 0 || A->n < 0 || A->nnz < 0) return fail(TSG_ERR_ARG, "negative sizes");
+    (void)hipSetDevice(ctx->device);
+    memset(T, 0, sizeof *T);
+    T->m = A->m; T->n = A->n; T->nnz = A->nnz; T->dtype = A->dtype; T->location = TSG_DEVICE;
+    T->tilem = (A->m + 15) / 16; T->tilen = (A->n + 15) / 16;
+    hipStream_t s = ctx->stream;
+    int32_t *flags = nullptr;
+    int rc = palloc(ctx, &flags, 1);
+    if (!rc) rc = palloc(ctx, &T->tile_ptr, (size_t)T->tilem + 1);
+    if (rc) { pool_free(ctx, flags); return rc; }
+    auto cleanup = [&](int code) {
+        pool_free(ctx, flags);
+        void *ps[] = {T->tile_ptr, T->tile_columnidx, T->tile_nnz, T->tile_csr_Val, T->tile_csr_Col,
+                      T->tile_csr_Ptr, T->mask};
+        for (void *p : ps) pool_free(ctx, p);
+        memset(T, 0, sizeof *T);
+        return code;
+    };
+    if (hipMemsetAsync(flags, 0, sizeof(int32_t), s) != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "memset"));
+    if (T->tilem == 0) {
+        (void)hipMemsetAsync(T->tile_ptr, 0, sizeof(int64_t), s);
+    } else {
+        k_c2t<0><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
+            A->rowptr, A->col, A->m, T->tilem, T->tile_ptr, nullptr, nullptr, nullptr, flags);
+        if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "k_c2t count launch"));
+    }
+    if ((rc = scan_inclusive_i64(ctx, T->tile_ptr, (int64_t)T->tilem + 1))) return cleanup(rc);
+    if ((rc = read_i64(ctx, T->tile_ptr + T->tilem, &T->numtile))) return cleanup(rc);
+    int64_t fl = 0;
+    {
+        int32_t hf = 0;
+        if (hipMemcpy(&hf, flags, sizeof hf, hipMemcpyDeviceToHost) != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "flag read"));
+        fl = hf;
+    }
+    if (fl) return cleanup(fail(TSG_ERR_UNSUPPORTED, "CSR rows must be sorted by column"));
+    if (T->numtile > INT32_MAX) return cleanup(fail(TSG_ERR_OVERFLOW, "numtile exceeds int32 tile ids"));
+    const int64_t nt = T->numtile;
+    if ((rc = palloc(ctx, &T->tile_columnidx, (size_t)nt))) return cleanup(rc);
+    if ((rc = palloc(ctx, &T->tile_nnz, (size_t)nt + 1))) return cleanup(rc);
+    if ((rc = palloc(ctx, &T->tile_csr_Ptr, (size_t)nt * 16))) return cleanup(rc);
+    if ((rc = palloc(ctx, &T->mask, (size_t)nt * 16))) return cleanup(rc);
+    if ((rc = pool_alloc(ctx, &T->tile_csr_Val, (size_t)T->nnz * vsize(T->dtype)))) return cleanup(rc);
+    if ((rc = palloc(ctx, &T->tile_csr_Col, (size_t)T->nnz))) return cleanup(rc);
+    if (hipMemsetAsync(T->tile_csr_Ptr, 0, (size_t)nt * 16, s) != hipSuccess ||
+        hipMemsetAsync(T->mask, 0, (size_t)nt * 32, s) != hipSuccess)
+        return cleanup(fail(TSG_ERR_HIP, "memset"));
+    if (T->tilem > 0) {
+        k_c2t<1><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
+            A->rowptr, A->col, A->m, T->tilem, T->tile_ptr, T->tile_columnidx, T->tile_csr_Ptr, T->mask, flags);
+        if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "k_c2t fill launch"));
+    }
+    if (nt > 0) {
+        k_tile_rowscan<<<blocks_for(nt * 16, 256), 256, 0, s>>>(nt, T->tile_csr_Ptr, T->tile_nnz);
+        if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "rowscan launch"));
+    } else {
+        (void)hipMemsetAsync(T->tile_nnz, 0, sizeof(int64_t), s);
+    }
+    if ((rc = scan_inclusive_i64(ctx, T->tile_nnz, nt + 1))) return cleanup(rc);
+    if (T->tilem > 0 && T->nnz > 0) {
+        if (T->dtype == TSG_F32)
+            k_c2t_scatter<float><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
+                A->rowptr, A->col, (const float *)A->val, A->m, T->tilem, T->tile_ptr, T->tile_columnidx,
+                T->tile_nnz, T->tile_csr_Ptr, (float *)T->tile_csr_Val, T->tile_csr_Col);
+        else
+            k_c2t_scatter<double><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
+                A->rowptr, A->col, (const double *)A->val, A->m, T->tilem, T->tile_ptr, T->tile_columnidx,
+                T->tile_nnz, T->tile_csr_Ptr, (double *)T->tile_csr_Val, T->tile_csr_Col);
+        if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "scatter launch"));
+    }
+    pool_free(ctx, flags);
+    if (hipStreamSynchronize(s) != hipSuccess) return fail(TSG_ERR_HIP, "csr2tile: %s", hipGetErrorString(hipGetLastError()));
+    return TSG_OK;
+}
+
+extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !B) return fail(TSG_ERR_ARG, "null argument");
+    if (B->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles must be on the device");
+    if (B->is_csc) return fail(TSG_ERR_ARG, "already in CSC-tile order");
+    (void)hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const int64_t nt = B->numtile;
+    int32_t *rowidx = B->tile_rowidx, *tmp = nullptr, *crow = nullptr;
+    int64_t *cptr = nullptr, *cursor = nullptr, *inv = nullptr, *nnz_new = nullptr;
+    uint8_t *ptr_new = nullptr, *col_new = nullptr;
+    uint16_t *mask_new = nullptr;
+    void *val_new = nullptr;
+    bool own_rowidx = false;
+    int rc = TSG_OK;
+    if (!rowidx) {
+        rc = palloc(ctx, &rowidx, (size_t)nt);
+        own_rowidx = true;
+        if (!rc && B->tilem > 0) {
+            k_fill_rowidx<<<blocks_for((int64_t)B->tilem * WAVE, 256), 256, 0, s>>>(B->tile_ptr, B->tilem, rowidx);
+            if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "rowidx launch");
+        }
+    }
+    if (!rc) rc = palloc(ctx, &cptr, (size_t)B->tilen + 1);
+    if (!rc) rc = palloc(ctx, &cursor, (size_t)B->tilen + 1);
+    if (!rc) rc = palloc(ctx, &tmp, (size_t)nt);
+    if (!rc) rc = palloc(ctx, &crow, (size_t)nt);
+    if (!rc) rc = palloc(ctx, &inv, (size_t)nt);
+    if (!rc) rc = palloc(ctx, &nnz_new, (size_t)nt + 1);
+    if (!rc) rc = palloc(ctx, &ptr_new, (size_t)nt * 16);
+    if (!rc) rc = palloc(ctx, &mask_new, (size_t)nt * 16);
+    if (!rc) rc = palloc(ctx, &col_new, (size_t)B->nnz);
+    if (!rc) rc = pool_alloc(ctx, &val_new, (size_t)B->nnz * vsize(B->dtype));
+    if (!rc && hipMemsetAsync(cptr, 0, sizeof(int64_t) * ((size_t)B->tilen + 1), s) != hipSuccess)
+        rc = fail(TSG_ERR_HIP, "memset");
+    if (!rc && nt > 0) {
+        k_csc_count<<<blocks_for(nt, 256), 256, 0, s>>>(nt, B->tile_columnidx, cptr);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "csc count launch");
+    }
+    if (!rc) rc = scan_inclusive_i64(ctx, cptr, (int64_t)B->tilen + 1);
+    if (!rc && hipMemcpyAsync(cursor, cptr, sizeof(int64_t) * ((size_t)B->tilen + 1), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        rc = fail(TSG_ERR_HIP, "copy");
+    if (!rc && nt > 0) {
+        k_csc_slot<<<blocks_for(nt, 256), 256, 0, s>>>(nt, B->tile_columnidx, rowidx, cursor, tmp);
+        k_csc_sort<<<blocks_for(B->tilen, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(B->tilen, cptr, tmp, crow);
+        k_csc_perm<<<blocks_for(nt, 256), 256, 0, s>>>(nt, B->tile_columnidx, rowidx, cptr, crow, B->tile_nnz, inv, nnz_new);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "csc perm launches");
+    } else if (!rc) {
+        (void)hipMemsetAsync(nnz_new, 0, sizeof(int64_t), s);
+    }
+    if (!rc) rc = scan_inclusive_i64(ctx, nnz_new, nt + 1);
+    if (!rc && nt > 0) {
+        if (B->dtype == TSG_F32)
+            k_csc_gather<float><<<blocks_for(nt * 16, 256), 256, 0, s>>>(
+                nt, inv, B->tile_nnz, nnz_new, B->tile_csr_Ptr, B->mask, B->tile_csr_Col,
+                (const float *)B->tile_csr_Val, ptr_new, mask_new, col_new, (float *)val_new);
+        else
+            k_csc_gather<double><<<blocks_for(nt * 16, 256), 256, 0, s>>>(
+                nt, inv, B->tile_nnz, nnz_new, B->tile_csr_Ptr, B->mask, B->tile_csr_Col,
+                (const double *)B->tile_csr_Val, ptr_new, mask_new, col_new, (double *)val_new);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "csc gather launch");
+    }
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(TSG_ERR_HIP, "build_csc_tiles: %s", hipGetErrorString(hipGetLastError()));
+    pool_free(ctx, cursor);
+    pool_free(ctx, tmp);
+    pool_free(ctx, inv);
+    if (rc) {
+        void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new};
+        for (void *p : ps) pool_free(ctx, p);
+        if (own_rowidx) pool_free(ctx, rowidx);
+        return rc;
+    }
+    pool_free(ctx, B->tile_nnz);
+    pool_free(ctx, B->tile_csr_Ptr);
+    pool_free(ctx, B->mask);
+    pool_free(ctx, B->tile_csr_Col);
+    pool_free(ctx, B->tile_csr_Val);
+    B->tile_nnz = nnz_new;
+    B->tile_csr_Ptr = ptr_new;
+    B->mask = mask_new;
+    B->tile_csr_Col = col_new;
+    B->tile_csr_Val = val_new;
+    B->csc_tile_ptr = cptr;
+    B->csc_tile_rowidx = crow;
+    B->tile_rowidx = rowidx;
+    B->is_csc = 1;
+    return TSG_OK;
+}
+
+// ---------------------------------------------------------------- tile2csr
+namespace tsg {
+
+// One wave per tile-row.  For each of the 16 rows the wave scans the row
+// popcounts of the tile-row's tiles (ascending tile column) to place each
+// tile's segment, so CSR rows come out sorted by column.
+template <bool FILL, typename V>
+__global__ void __launch_bounds__(CV_WAVES *WAVE)
+k_t2c(int32_t m, int32_t tilem, const int64_t *__restrict__ tile_ptr, const int32_t *__restrict__ colidx,
+      const int64_t *__restrict__ tile_nnz, const uint8_t *__restrict__ ptr, const uint16_t *__restrict__ mask,
+      const uint8_t *__restrict__ tcol, const V *__restrict__ tval, int64_t *__restrict__ rowptr,
+      int32_t *__restrict__ col, V *__restrict__ val) {
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int32_t bi = blockIdx.x * CV_WAVES + w;
+    if (bi >= tilem) return;
+    const int64_t tb = tile_ptr[bi], te = tile_ptr[bi + 1];
+    for (int r = 0; r < 16 && bi * 16 + r < m; r++) {
+        const int32_t row = bi * 16 + r;
+        int64_t base = FILL ? rowptr[row] : 0;
+        for (int64_t t0 = tb; t0 < te; t0 += WAVE) {
+            const int64_t T = t0 + lane;
+            int pc = 0;
+            if (T < te) pc = __popc((uint32_t)mask[T * 16 + r]);
+            const int incl = wave_incl_sum(pc);
+            if (FILL && pc) {
+                const int64_t src = tile_nnz[T] + ptr[T * 16 + r];
+                const int64_t dst = base + incl - pc;
+                const int32_t cb = colidx[T] * 16;
+                for (int q = 0; q < pc; q++) {
+                    col[dst + q] = cb + (tcol[src + q] & 15);
+                    val[dst + q] = tval[src + q];
+                }
+            }
+            base += __shfl(incl, WAVE - 1, WAVE);
+        }
+        if (!FILL && lane == 0) {
+            rowptr[row + 1] = base;
+            if (row == 0) rowptr[0] = 0;
+        }
+    }
+}
+
+template <typename V>
+static int tile2csr_t(Context *ctx, const tsg_tile_matrix *T, tsg_csr *o) {
+    hipStream_t s = ctx->stream;
+    const unsigned grid = blocks_for(T->tilem, CV_WAVES);
+    if (T->tilem > 0) {
+        k_t2c<false, V><<<grid, CV_WAVES * WAVE, 0, s>>>(T->m, T->tilem, T->tile_ptr, T->tile_columnidx,
+                                                         T->tile_nnz, T->tile_csr_Ptr, T->mask, T->tile_csr_Col,
+                                                         (const V *)T->tile_csr_Val, o->rowptr, nullptr, nullptr);
+        TSG_HIP(hipGetLastError());
+    } else {
+        TSG_HIP(hipMemsetAsync(o->rowptr, 0, sizeof(int64_t), s));
+    }
+    TSG_CHECK(scan_inclusive_i64(ctx, o->rowptr, (int64_t)T->m + 1));
+    if (T->tilem > 0 && T->nnz > 0) {
+        k_t2c<true, V><<<grid, CV_WAVES * WAVE, 0, s>>>(T->m, T->tilem, T->tile_ptr, T->tile_columnidx,
+                                                        T->tile_nnz, T->tile_csr_Ptr, T->mask, T->tile_csr_Col,
+                                                        (const V *)T->tile_csr_Val, o->rowptr, o->col, (V *)o->val);
+        TSG_HIP(hipGetLastError());
+    }
+    TSG_HIP(hipStreamSynchronize(s));
+    return TSG_OK;
+}
+
+}  // namespace tsg
+
+extern "C" int tsg_tile2csr(tsg_context *c, const tsg_tile_matrix *T, tsg_csr *o) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !T || !o) return fail(TSG_ERR_ARG, "null argument");
+    if (T->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles must be on the device");
+    if (T->is_csc) return fail(TSG_ERR_ARG, "tile2csr needs row-tile order (is_csc = 0)");
+    (void)hipSetDevice(ctx->device);
+    memset(o, 0, sizeof *o);
+    o->m = T->m; o->n = T->n; o->nnz = T->nnz; o->dtype = T->dtype; o->location = TSG_DEVICE;
+    int rc = palloc(ctx, &o->rowptr, (size_t)T->m + 1);
+    if (!rc) rc = palloc(ctx, &o->col, (size_t)T->nnz);
+    if (!rc) rc = pool_alloc(ctx, &o->val, (size_t)T->nnz * vsize(T->dtype));
+    if (!rc) rc = T->dtype == TSG_F32 ? tile2csr_t<float>(ctx, T, o) : tile2csr_t<double>(ctx, T, o);
+    if (rc) {
+        pool_free(ctx, o->rowptr); pool_free(ctx, o->col); pool_free(ctx, o->val);
+        memset(o, 0, sizeof *o);
+    }
+    return rc;
+}

@@ -1,0 +1,107 @@
+// scan.hip -- device-wide int64 inclusive scan (reduce, scan partials, apply).
+//
+// Replaces the reference's cub::DeviceScan::ExclusiveSum calls
+// (src/spgemm_raw_v2.cu:131-138 and :202-208).  Callers store 0 in data[0] and
+// the per-item counts in data[1..n-1]; the inclusive scan then leaves the
+// exclusive offsets plus the total in data[n-1].  64-bit throughout, and the
+// temporary partial sums come from the context pool (no hipMalloc/hipFree per
+// call as in the reference).
+#include "tsg_internal.h"
+
+namespace tsg {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 8;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const int64_t *__restrict__ data,
+                                                               int64_t n, int64_t *__restrict__ partial) {
+    __shared__ int64_t s_w[SCAN_THREADS / WAVE];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int64_t s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
+        if (idx < n) s += data[idx];
+    }
+    s = wave_sum(s);
+    if (lane_id() == 0) s_w[threadIdx.x / WAVE] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int w = 0; w < SCAN_THREADS / WAVE; w++) t += s_w[w];
+        partial[blockIdx.x] = t;
+    }
+}
+
+// Inclusive scan of one SCAN_TILE chunk per block, plus the carried-in offset
+// (inclusive partial of the previous block).
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict__ data, int64_t n,
+                                                              const int64_t *__restrict__ partial_incl) {
+    __shared__ int64_t s_d[SCAN_TILE + SCAN_TILE / 32];
+    __shared__ int64_t s_w[SCAN_THREADS / WAVE];
+    int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    auto pad = [](int i) { return i + (i >> 5); };
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        int li = i * SCAN_THREADS + threadIdx.x;
+        int64_t idx = base + li;
+        s_d[pad(li)] = idx < n ? data[idx] : 0;
+    }
+    __syncthreads();
+    int64_t v[SCAN_ITEMS];
+    int64_t run = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        run += s_d[pad(threadIdx.x * SCAN_ITEMS + i)];
+        v[i] = run;
+    }
+    int64_t incl = wave_incl_sum(run);
+    int w = threadIdx.x / WAVE;
+    if (lane_id() == WAVE - 1) s_w[w] = incl;
+    __syncthreads();
+    int64_t off = (blockIdx.x > 0 && partial_incl) ? partial_incl[blockIdx.x - 1] : 0;
+    for (int q = 0; q < w; q++) off += s_w[q];
+    off += incl - run;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) s_d[pad(threadIdx.x * SCAN_ITEMS + i)] = v[i] + off;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        int li = i * SCAN_THREADS + threadIdx.x;
+        int64_t idx = base + li;
+        if (idx < n) data[idx] = s_d[pad(li)];
+    }
+}
+
+int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n) {
+    if (n <= 0) return TSG_OK;
+    int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "scan of %lld items exceeds grid limit", (long long)n);
+    if (nb == 1) {
+        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, nullptr);
+        TSG_HIP(hipGetLastError());
+        return TSG_OK;
+    }
+    int64_t *partial = nullptr;
+    TSG_CHECK(palloc(ctx, &partial, (size_t)nb));
+    k_scan_reduce<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, partial);
+    int rc = TSG_OK;
+    if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_reduce launch failed");
+    if (rc == TSG_OK) rc = scan_inclusive_i64(ctx, partial, nb);
+    if (rc == TSG_OK) {
+        k_scan_apply<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, partial);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_apply launch failed");
+    }
+    pool_free(ctx, partial);
+    return rc;
+}
+
+int read_i64(Context *ctx, const int64_t *dptr, int64_t *out) {
+    TSG_HIP(hipMemcpyAsync(ctx->pinned, dptr, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->pinned[0];
+    return TSG_OK;
+}
+
+}  // namespace tsg

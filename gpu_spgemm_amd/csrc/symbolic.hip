@@ -1,0 +1,72 @@
+// symbolic.hip -- Step 2: per-row bitmasks of every C tile.
+//
+// Replaces step2_symbolic_kernel and step2_build_ptrRowC_kernel
+// (src/spgemm_raw_kernels.cuh:105-215):
+//   maskC[r]  = OR over matched pairs (a,b), OR over the columns kk present in
+//               A-tile row r, of B-tile row mask kk
+//   tile_nnzC = sum_r popcount(maskC[r])   (count stored at [ti+1] for the scan)
+//   ptrRowC[r]= exclusive prefix of the row popcounts (uint8)
+//
+// MI355X design: a 16-lane group per C tile (four C tiles per wave64), lane =
+// tile row.  Each pair stages B's 16 row masks (32 B, one coalesced load) in
+// LDS; each lane walks the set bits of its A row mask (bit 15-kk) with clz and
+// ORs the staged rows -- no per-entry column bytes are read.  The per-tile
+// exclusive scan of popcounts is a 16-lane shuffle scan in the same kernel,
+// so ptrRowC costs no extra launch.  No pair cache is written: the numeric
+// step recomputes the (cheap, L2-resident) intersection, so there is no
+// capacity to overflow and no host read of a pair count.
+#include "pairs.h"
+
+namespace tsg {
+
+constexpr int SY_THREADS = 256;
+constexpr int SY_GROUPS = SY_THREADS / 16;
+
+__global__ void __launch_bounds__(SY_THREADS)
+k_symbolic(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ colidxA,
+           const uint16_t *__restrict__ maskA, const int64_t *__restrict__ csc_ptrB,
+           const int32_t *__restrict__ csc_rowB, const uint16_t *__restrict__ maskB,
+           int64_t numtileC, const int32_t *__restrict__ rowidxC, const int32_t *__restrict__ colidxC,
+           uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC) {
+    __shared__ uint16_t s_mb[SY_GROUPS][16];
+    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
+    const int64_t ti = (int64_t)blockIdx.x * SY_GROUPS + g;
+    if (ti >= numtileC) return;  // whole group leaves; only wave-level syncs below
+    const int32_t bi = rowidxC[ti], bj = colidxC[ti];
+    uint32_t rm = 0;
+    for_each_pair(tile_ptrA, colidxA, csc_ptrB, csc_rowB, bi, bj, [&](int64_t a, int64_t b) {
+        s_mb[g][l] = maskB[b * 16 + l];
+        wave_sync();
+        uint32_t ma = maskA[a * 16 + l];
+        while (ma) {
+            const int kk = __clz(ma) - 16;
+            rm |= s_mb[g][kk];
+            ma ^= 0x8000u >> kk;
+        }
+        wave_sync();
+    });
+    maskC[ti * 16 + l] = (uint16_t)rm;
+    const int pc = __popc(rm);
+    const int incl = group16_incl_sum(pc);
+    ptrRowC[ti * 16 + l] = (uint8_t)(incl - pc);
+    if (l == 15) tile_nnzC[ti + 1] = incl;
+    if (ti == 0 && l == 0) tile_nnzC[0] = 0;
+}
+
+int launch_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, int64_t numtileC,
+                    const int32_t *rowidxC, const int32_t *colidxC, uint16_t *maskC, uint8_t *ptrRowC,
+                    int64_t *tile_nnzC) {
+    if (numtileC == 0) {
+        TSG_HIP(hipMemsetAsync(tile_nnzC, 0, sizeof(int64_t), ctx->stream));
+        return TSG_OK;
+    }
+    int64_t grid = (numtileC + SY_GROUPS - 1) / SY_GROUPS;
+    if (grid > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "symbolic grid too large");
+    k_symbolic<<<(unsigned)grid, SY_THREADS, 0, ctx->stream>>>(
+        A->tile_ptr, A->tile_columnidx, A->mask, B->csc_tile_ptr, B->csc_tile_rowidx, B->mask, numtileC,
+        rowidxC, colidxC, maskC, ptrRowC, tile_nnzC);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
+}  // namespace tsg

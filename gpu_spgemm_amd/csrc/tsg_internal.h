@@ -1,0 +1,178 @@
+// tsg_internal.h -- shared internals of libtsg: context, device pool, error
+// reporting and wave64 device helpers for gfx950.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tsg.h"
+
+namespace tsg {
+
+constexpr int TS = 16;
+constexpr int WAVE = 64;
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define TSG_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::tsg::fail(TSG_ERR_HIP, "%s failed: %s (%s:%d)", #call,           \
+                               hipGetErrorString(e_), __FILE__, __LINE__);             \
+    } while (0)
+
+#define TSG_CHECK(expr)                 \
+    do {                                \
+        int rc_ = (expr);               \
+        if (rc_ != TSG_OK) return rc_;  \
+    } while (0)
+
+// ---------------------------------------------------------------- device pool
+// Caching allocator: freed blocks are kept per exact rounded size and reused,
+// so repeated products of one shape make no hipMalloc after the first.
+struct Pool {
+    std::multimap<size_t, void *> free_blocks;
+    std::map<void *, size_t> live;
+    int64_t in_use = 0, cached = 0;
+    static size_t round(size_t b) {
+        if (b < 256) return 256;
+        // round to 1/8 of the power of two so reuse across close sizes works
+        size_t p = 1;
+        while (p < b) p <<= 1;
+        size_t step = p >> 3;
+        return (b + step - 1) / step * step;
+    }
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipDeviceProp_t prop{};
+    Pool pool;
+    std::mutex mu;
+    // small pinned host staging for scalar reads
+    int64_t *pinned = nullptr;
+    // learned scheduler state lives in learned.hip
+    void *learned = nullptr;
+};
+
+int pool_alloc(Context *ctx, void **p, size_t bytes);
+int pool_free(Context *ctx, void *p);
+template <typename T>
+inline int palloc(Context *ctx, T **p, size_t count) {
+    return pool_alloc(ctx, reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+inline size_t vsize(int dtype) { return dtype == TSG_F32 ? 4 : 8; }
+
+// ---------------------------------------------------------------- scans (scan.hip)
+// In-place inclusive scan of n int64 values.  With data[0] = 0 and counts in
+// data[1..n-1], this leaves exclusive offsets (the reference's CUB
+// ExclusiveSum, src/spgemm_raw_v2.cu:131-138, 202-208, widened to int64).
+int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n);
+// Reads one int64 from device memory (blocking; counts as a host sync).
+int read_i64(Context *ctx, const int64_t *dptr, int64_t *out);
+
+// ---------------------------------------------------------------- kernels entry (host side)
+int launch_structure_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                           int64_t *tile_ptrC);
+int launch_structure_fill(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                          const int64_t *tile_ptrC, int32_t *rowidxC, int32_t *colidxC);
+int launch_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                    int64_t numtileC, const int32_t *rowidxC, const int32_t *colidxC,
+                    uint16_t *maskC, uint8_t *ptrRowC, int64_t *tile_nnzC);
+void learned_release(Context *ctx);  // learned.hip
+int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                   tsg_tile_matrix *C, const tsg_options *opts);
+
+}  // namespace tsg
+
+// ---------------------------------------------------------------- device helpers
+#if defined(__HIPCC__)
+namespace tsg {
+
+// Orders LDS traffic between lanes of one wavefront (no hardware barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// Number of set bits of a 64-bit ballot below this lane.
+__device__ __forceinline__ int mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+// Wave-wide inclusive sum (64 lanes).
+template <typename T>
+__device__ __forceinline__ T wave_incl_sum(T v) {
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        T o = __shfl_up(v, d, WAVE);
+        if (lane_id() >= d) v += o;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, WAVE);
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) { T o = __shfl_xor(v, d, WAVE); v = o < v ? o : v; }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) { T o = __shfl_xor(v, d, WAVE); v = o > v ? o : v; }
+    return v;
+}
+
+// 16-lane group inclusive sum (groups are aligned lane ranges of 16).
+template <typename T>
+__device__ __forceinline__ T group16_incl_sum(T v) {
+    int l = lane_id() & 15;
+#pragma unroll
+    for (int d = 1; d < 16; d <<= 1) {
+        T o = __shfl_up(v, d, 16);
+        if (l >= d) v += o;
+    }
+    return v;
+}
+
+template <typename T>
+__device__ __forceinline__ T group16_sum(T v) {
+#pragma unroll
+    for (int d = 8; d > 0; d >>= 1) v += __shfl_xor(v, d, 16);
+    return v;
+}
+
+// First index in [lo,hi) of sorted arr with arr[i] >= key.
+template <typename I, typename T>
+__device__ __forceinline__ I lower_bound(const T *__restrict__ arr, I lo, I hi, T key) {
+    while (lo < hi) {
+        I mid = lo + ((hi - lo) >> 1);
+        if (arr[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+}  // namespace tsg
+#endif

@@ -1,0 +1,189 @@
+/*
+ * tsg.h -- C ABI of the MI355X tiled SpGEMM library (libtsg.so).
+ *
+ * C = A*B in the 16x16 tile format of the reference (src/tile_format.h:15-42),
+ * computed by hand-written HIP kernels for gfx950.  Plain pointers and sizes
+ * only; every call returns TSG_OK (0) or a negative tsg_status, and
+ * tsg_last_error_string() describes the last failure of the calling thread.
+ * No exception crosses this boundary.
+ *
+ * The reference has no library API: its path is the host program
+ * src/spgemm_raw_v2.cu:61-372 (main) driving host conversion
+ * (src/tile_format.h:93-222, :424-546), upload (src/spgemm_raw_v2.cu:25-48)
+ * and three kernel steps.  Each entry point below names the reference code it
+ * replaces.
+ *
+ * Layout differences from the reference, all widening only:
+ *   - prefix arrays (tile_ptr, csc_tile_ptr, tile_nnz, rowptr) are int64, so
+ *     nnz(C) above 2^31 is representable (R-MAT scale 20 reaches 9.7e9);
+ *   - values are double (TSG_F64) or float (TSG_F32), chosen at run time
+ *     instead of the compile-time TVAL typedef (src/tile_format.h:13).
+ */
+#ifndef TSG_H
+#define TSG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSG_TILE 16  /* src/tile_format.h:11 TILE_SIZE */
+
+typedef enum tsg_status {
+    TSG_OK = 0,
+    TSG_ERR_ARG = -1,         /* invalid argument / inconsistent shapes      */
+    TSG_ERR_HIP = -2,         /* HIP runtime error                          */
+    TSG_ERR_OOM = -3,         /* device or host allocation failed            */
+    TSG_ERR_OVERFLOW = -4,    /* size beyond a documented limit              */
+    TSG_ERR_UNSUPPORTED = -5, /* unsupported input (e.g. unsorted CSR rows)   */
+    TSG_ERR_IO = -6,          /* file open / parse failure                   */
+    TSG_ERR_NODEVICE = -7     /* no GPU visible                               */
+} tsg_status;
+
+typedef enum tsg_dtype { TSG_F64 = 0, TSG_F32 = 1 } tsg_dtype;
+typedef enum tsg_location { TSG_HOST = 0, TSG_DEVICE = 1 } tsg_location;
+
+/* Compressed sparse rows.  Column indices sorted ascending within each row
+ * (what load_mtx, the transposes and every generator produce). */
+typedef struct tsg_csr {
+    int32_t m, n;
+    int64_t nnz;
+    int32_t dtype;     /* tsg_dtype */
+    int32_t location;  /* tsg_location */
+    int64_t *rowptr;   /* [m+1] */
+    int32_t *col;      /* [nnz] */
+    void *val;         /* [nnz] double or float */
+} tsg_csr;
+
+/* Tile matrix: the reference TileMatrix (src/tile_format.h:15-42) and the
+ * device mirror TileDevPtrs (src/spgemm_raw_v2.cu:15-23) in one struct.
+ * For a B operand (is_csc = 1) tile_ptr/tile_columnidx keep the CSR-tile
+ * order while tile_nnz, tile_csr_Ptr, mask, tile_csr_Col and tile_csr_Val are
+ * in CSC-tile order (csc_tile_ptr / csc_tile_rowidx), and tile_csr_Col holds
+ * the local column only (src/tile_format.h:535-542).  For A and C the col
+ * byte is (row<<4)|col (src/tile_format.h:208). */
+typedef struct tsg_tile_matrix {
+    int32_t m, n;
+    int32_t tilem, tilen;
+    int64_t nnz;
+    int64_t numtile;
+    int32_t dtype;            /* tsg_dtype */
+    int32_t location;         /* tsg_location */
+    int32_t is_csc;           /* 1 for a B operand laid out by tsg_build_csc_tiles */
+    int32_t reserved;
+    int64_t *tile_ptr;        /* [tilem+1] */
+    int32_t *tile_columnidx;  /* [numtile] */
+    int32_t *tile_rowidx;     /* [numtile] row of each tile (C output), may be NULL for A/B */
+    int64_t *tile_nnz;        /* [nt+1] exclusive prefix; nt = numtile */
+    void *tile_csr_Val;       /* [nnz] */
+    uint8_t *tile_csr_Col;    /* [nnz] */
+    uint8_t *tile_csr_Ptr;    /* [nt*16] exclusive row offsets inside each tile */
+    uint16_t *mask;           /* [nt*16] bit (15-c) = local column c present */
+    int64_t *csc_tile_ptr;    /* [tilen+1] (B only) */
+    int32_t *csc_tile_rowidx; /* [numtile] (B only) */
+} tsg_tile_matrix;
+
+typedef struct tsg_context tsg_context;
+
+/* Scheduler choice for tsg_spgemm. */
+typedef enum tsg_scheduler {
+    TSG_SCHED_PLAIN = 0,    /* the reference's 3-step method: exact counts, host reads of numtileC / nnzC */
+    TSG_SCHED_LEARNED = 1   /* MLP-predicted sizes and bins, no host sync; exact fallback on overflow */
+} tsg_scheduler;
+
+typedef struct tsg_options {
+    int32_t scheduler;      /* tsg_scheduler */
+    int32_t numeric_variant;/* 0 = auto (binned), 1 = row lanes, 2 = staged dense B, 3 = MFMA */
+    int32_t mfma_min_products; /* tile pairs with >= this many products may use MFMA (0 = default) */
+    int32_t timing;         /* 1 = record per-step hipEvent times in tsg_stats */
+    double  pred_scale;     /* learned: multiply predicted sizes (1.0 normal; 0 forces fallback) */
+} tsg_options;
+
+typedef struct tsg_stats {
+    int64_t numtileC;       /* tiles of C, including structurally empty ones */
+    int64_t nnzC;
+    uint64_t products;      /* sum_{(i,k) in A} nnz(B[k,:]) (src/spgemm_raw_v2.cu:95-99) */
+    double ms_step1, ms_step2, ms_step3, ms_total;
+    int32_t fallback_used;  /* learned: 1 when the exact path re-ran after an overflow */
+    int32_t host_syncs;     /* blocking device-to-host reads made inside the call */
+    int64_t pred_numtileC, pred_nnzC;  /* learned: allocated capacities */
+} tsg_stats;
+
+/* ---- context ---- */
+int tsg_create(int device, tsg_context **ctx);
+int tsg_destroy(tsg_context *ctx);
+const char *tsg_last_error_string(void);
+int tsg_device_count(int *count);
+int tsg_synchronize(tsg_context *ctx);
+/* Device name (src/spgemm_raw_v2.cu:70-71). */
+int tsg_device_name(tsg_context *ctx, char *buf, size_t len);
+/* The HIP stream every call of this context runs on (hipStream_t). */
+void *tsg_stream(tsg_context *ctx);
+/* Bytes held by the context's device pool (in use, cached). */
+int tsg_pool_stats(tsg_context *ctx, int64_t *in_use, int64_t *cached);
+int tsg_pool_trim(tsg_context *ctx);
+
+/* ---- host matrices (generators, MTX I/O) ---- */
+/* MTX reader, replaces load_mtx (src/tile_format.h:51-90): symmetric mirrored,
+ * pattern -> 1.0, complex -> real part, sorted by (row,col), duplicates kept. */
+int tsg_load_mtx(const char *path, tsg_csr *out_host);
+int tsg_write_mtx(const char *path, const tsg_csr *host);
+/* B = A^T (src/spgemm_raw_v2.cu:78-89, the --aat path). */
+int tsg_csr_transpose_host(const tsg_csr *in, tsg_csr *out_host);
+/* flops of A*B (src/spgemm_raw_v2.cu:95-99). */
+int tsg_csr_products(const tsg_csr *A_host, const tsg_csr *B_host, uint64_t *products);
+int tsg_csr_free_host(tsg_csr *h);
+int tsg_csr_convert_dtype_host(tsg_csr *h, int32_t dtype);
+
+/* Synthetic inputs (no reference counterpart: the reference reads .mtx files).
+ * All use a counter-based SplitMix64 stream, so results do not depend on threads. */
+int tsg_gen_er(int32_t n, int32_t per_row, uint64_t seed, tsg_csr *out);          /* exactly per_row distinct uniform columns, U[-1,1] */
+int tsg_gen_er_var(int32_t n, int32_t lo, int32_t hi, uint64_t seed, tsg_csr *out); /* nnz/row uniform in [lo,hi] */
+int tsg_gen_laplace2d(int32_t nx, int32_t ny, tsg_csr *out);                      /* 5-point, 4/-1 */
+int tsg_gen_laplace3d(int32_t nx, int32_t ny, int32_t nz, tsg_csr *out);          /* 7-point, 6/-1 */
+int tsg_gen_stencil27(int32_t nx, int32_t ny, int32_t nz, tsg_csr *out);          /* 27-point, 26/-1 */
+int tsg_gen_aggregation(int32_t nx, int32_t ny, int32_t nz, tsg_csr *out);        /* 2x2x2 piecewise-constant P */
+int tsg_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c, uint64_t seed,
+                 int32_t dtype, tsg_csr *out);                                     /* duplicates merged, U(0,1] */
+
+/* ---- device memory ---- */
+int tsg_csr_upload(tsg_context *ctx, const tsg_csr *host, tsg_csr *dev);
+int tsg_csr_download(tsg_context *ctx, const tsg_csr *dev, tsg_csr *host);
+int tsg_csr_free(tsg_context *ctx, tsg_csr *dev);
+/* Host<->device copies of a tile matrix; replaces upload_tiles
+ * (src/spgemm_raw_v2.cu:25-48).  Download allocates host arrays with malloc. */
+int tsg_tile_upload(tsg_context *ctx, const tsg_tile_matrix *host, tsg_tile_matrix *dev);
+int tsg_tile_download(tsg_context *ctx, const tsg_tile_matrix *dev, tsg_tile_matrix *host);
+/* Frees device arrays (free_dev, src/spgemm_raw_v2.cu:50-59). */
+int tsg_tile_free(tsg_context *ctx, tsg_tile_matrix *dev);
+/* Frees host arrays (free_tile_matrix, src/tile_format.h:647-659). */
+int tsg_tile_free_host(tsg_tile_matrix *host);
+
+/* ---- conversion on the GPU ---- */
+/* CSR -> row tiles; replaces csr2tile (src/tile_format.h:93-222). */
+int tsg_csr2tile(tsg_context *ctx, const tsg_csr *dev_csr, tsg_tile_matrix *out_dev);
+/* Re-lays a row-tiled matrix's data arrays in CSC-tile order, making it a B
+ * operand; replaces build_csc_tiles (src/tile_format.h:424-546).  Works on any
+ * row-tiled matrix, including a C output (so products chain without host work). */
+int tsg_build_csc_tiles(tsg_context *ctx, tsg_tile_matrix *B_dev);
+/* Row tiles -> CSR (the reference's verification unpacking, src/spgemm_raw_v2.cu:268-284). */
+int tsg_tile2csr(tsg_context *ctx, const tsg_tile_matrix *dev, tsg_csr *out_dev);
+
+/* ---- SpGEMM ---- */
+/* C = A*B; replaces Steps 1-3 of src/spgemm_raw_v2.cu:111-240 (kernels in
+ * src/spgemm_raw_kernels.cuh:23-215 and src/spgemm_raw_kernels_v2.cuh:10-139).
+ * A: row tiles; B: CSC-laid tiles (tsg_build_csc_tiles); C allocated by the
+ * library in the context pool (free with tsg_tile_free).  opts/stats may be NULL. */
+int tsg_spgemm(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+               tsg_tile_matrix *C, const tsg_options *opts, tsg_stats *stats);
+
+/* Many independent products in one persistent launch (no reference counterpart). */
+int tsg_spgemm_batched(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A,
+                       const tsg_tile_matrix *B, tsg_tile_matrix *C, const tsg_options *opts,
+                       tsg_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSG_H */

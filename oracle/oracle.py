@@ -1,0 +1,270 @@
+"""ctypes wrapper of the oracle (liboracle.so) and of the reference-compiled
+checker (_ref/libref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product.  Results come back as dicts
+with the reference's field names (tile_ptr, tile_columnidx, tile_nnz,
+tile_csr_Ptr, mask, tile_csr_Col, tile_csr_Val, csc_tile_ptr, csc_tile_rowidx),
+the same layout gpu_spgemm_amd.DeviceTiles.download() returns.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libref.so")
+
+
+class OraTile(C.Structure):
+    _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("tilem", C.c_int32), ("tilen", C.c_int32),
+                ("nnz", C.c_int64), ("numtile", C.c_int64),
+                ("tile_ptr", C.POINTER(C.c_int64)), ("tile_colidx", C.POINTER(C.c_int32)),
+                ("tile_rowidx", C.POINTER(C.c_int32)), ("tile_nnz", C.POINTER(C.c_int64)),
+                ("tile_ptrRow", C.POINTER(C.c_uint8)), ("mask", C.POINTER(C.c_uint16)),
+                ("tile_col", C.POINTER(C.c_uint8)), ("val64", C.POINTER(C.c_double)),
+                ("val32", C.POINTER(C.c_float)), ("csc_tile_ptr", C.POINTER(C.c_int64)),
+                ("csc_tile_rowidx", C.POINTER(C.c_int32)), ("numtile_csc", C.c_int64),
+                ("is_csc", C.c_int32), ("is_f32", C.c_int32)]
+
+
+class OraCheck(C.Structure):
+    _fields_ = [("total_ref", C.c_int64), ("total_gpu", C.c_int64), ("missing", C.c_int64),
+                ("extra", C.c_int64), ("rel_frob", C.c_double), ("max_scaled", C.c_double),
+                ("max_abs", C.c_double), ("pass_", C.c_int32)]
+
+
+_lib = None
+_ref = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True, capture_output=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _lib.ora_flops.restype = C.c_uint64
+        _lib.ora_cpu_baseline.restype = C.c_int64
+    return _lib
+
+
+def ref_available():
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        _ref = C.CDLL(REF_LIB)
+    return _ref
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _csr_args(csr):
+    rp = np.ascontiguousarray(csr.rowptr, dtype=np.int64)
+    col = np.ascontiguousarray(csr.col, dtype=np.int32)
+    val = np.ascontiguousarray(csr.val, dtype=np.float64)
+    return rp, col, val
+
+
+def _arr(ptr, n, dt):
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype=dt)
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(n,)).copy()
+
+
+def tile_to_dict(t, free_fn):
+    nt = t.numtile_csc if t.is_csc else t.numtile
+    d = dict(m=t.m, n=t.n, tilem=t.tilem, tilen=t.tilen, nnz=t.nnz, numtile=t.numtile,
+             is_csc=t.is_csc, dtype=1 if t.is_f32 else 0,
+             tile_ptr=_arr(t.tile_ptr, t.tilem + 1, np.int64),
+             tile_columnidx=_arr(t.tile_colidx, t.numtile, np.int32),
+             tile_rowidx=_arr(t.tile_rowidx, t.numtile, np.int32) if t.tile_rowidx else None,
+             tile_nnz=_arr(t.tile_nnz, nt + 1, np.int64),
+             tile_csr_Ptr=_arr(t.tile_ptrRow, nt * 16, np.uint8),
+             mask=_arr(t.mask, nt * 16, np.uint16),
+             tile_csr_Col=_arr(t.tile_col, t.nnz, np.uint8),
+             tile_csr_Val=(_arr(t.val32, t.nnz, np.float32) if t.is_f32 else _arr(t.val64, t.nnz, np.float64)))
+    if t.is_csc:
+        d["csc_tile_ptr"] = _arr(t.csc_tile_ptr, t.tilen + 1, np.int64)
+        d["csc_tile_rowidx"] = _arr(t.csc_tile_rowidx, t.numtile_csc, np.int32)
+    free_fn(C.byref(t))
+    return d
+
+
+def _dict_to_tile(d, keep):
+    """Builds an OraTile view over numpy arrays of a dict (no ownership)."""
+    t = OraTile()
+    t.m, t.n, t.tilem, t.tilen = d["m"], d["n"], d["tilem"], d["tilen"]
+    t.nnz, t.numtile, t.is_csc = d["nnz"], d["numtile"], d.get("is_csc", 0)
+    t.is_f32 = 1 if d["tile_csr_Val"].dtype == np.float32 else 0
+
+    def put(name, key, ct, dt):
+        a = d.get(key)
+        if a is None:
+            return
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        setattr(t, name, _p(a, ct))
+
+    put("tile_ptr", "tile_ptr", C.c_int64, np.int64)
+    put("tile_colidx", "tile_columnidx", C.c_int32, np.int32)
+    put("tile_rowidx", "tile_rowidx", C.c_int32, np.int32)
+    put("tile_nnz", "tile_nnz", C.c_int64, np.int64)
+    put("tile_ptrRow", "tile_csr_Ptr", C.c_uint8, np.uint8)
+    put("mask", "mask", C.c_uint16, np.uint16)
+    put("tile_col", "tile_csr_Col", C.c_uint8, np.uint8)
+    if t.is_f32:
+        put("val32", "tile_csr_Val", C.c_float, np.float32)
+    else:
+        put("val64", "tile_csr_Val", C.c_double, np.float64)
+    if t.is_csc:
+        put("csc_tile_ptr", "csc_tile_ptr", C.c_int64, np.int64)
+        put("csc_tile_rowidx", "csc_tile_rowidx", C.c_int32, np.int32)
+        t.numtile_csc = d["numtile"]
+    return t
+
+
+# ---------------------------------------------------------------- oracle calls
+def csr2tile(csr, f32=False):
+    """tile_format.h:93-222 restated."""
+    rp, col, val = _csr_args(csr)
+    t = OraTile()
+    lib().ora_csr2tile(csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double),
+                       1 if f32 else 0, C.byref(t))
+    return tile_to_dict(t, lib().ora_tile_free)
+
+
+def csr2tile_b(csr, f32=False):
+    """csr2tile + build_csc_tiles (tile_format.h:424-546) restated."""
+    rp, col, val = _csr_args(csr)
+    t = OraTile()
+    lib().ora_csr2tile(csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double),
+                       1 if f32 else 0, C.byref(t))
+    lib().ora_build_csc_tiles(C.byref(t), csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32),
+                              _p(val, C.c_double))
+    return tile_to_dict(t, lib().ora_tile_free)
+
+
+def spgemm_tiled(A, B, f32=False, steps=3):
+    """Steps 1..steps of the tiled method on the CPU; returns the C dict."""
+    keep = []
+    tA = csr2tile_raw(A, f32, False, keep)
+    tB = csr2tile_raw(B, f32, True, keep)
+    tC = OraTile()
+    lib().ora_step1(C.byref(tA), C.byref(tB), C.byref(tC))
+    if steps >= 2:
+        lib().ora_step2(C.byref(tA), C.byref(tB), C.byref(tC))
+    if steps >= 3:
+        lib().ora_step3(C.byref(tA), C.byref(tB), C.byref(tC))
+    else:
+        tC.nnz = tC.nnz if steps >= 2 else 0
+    d = tile_to_dict(tC, lib().ora_tile_free)
+    lib().ora_tile_free(C.byref(tA))
+    lib().ora_tile_free(C.byref(tB))
+    return d
+
+
+def csr2tile_raw(csr, f32, as_b, keep):
+    rp, col, val = _csr_args(csr)
+    keep += [rp, col, val]
+    t = OraTile()
+    lib().ora_csr2tile(csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double),
+                       1 if f32 else 0, C.byref(t))
+    if as_b:
+        lib().ora_build_csc_tiles(C.byref(t), csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32),
+                                  _p(val, C.c_double))
+    return t
+
+
+def gustavson_check(A, B, Cd, row_lo=0, row_hi=None):
+    """spgemm_raw_v2.cu:253-359 restated: compare a C tile dict with FP64 Gustavson."""
+    if row_hi is None:
+        row_hi = A.m
+    ra, ca, va = _csr_args(A)
+    rb, cb, vb = _csr_args(B)
+    keep = []
+    tC = _dict_to_tile(Cd, keep)
+    out = OraCheck()
+    lib().ora_gustavson_check(A.m, B.n, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(va, C.c_double),
+                              _p(rb, C.c_int64), _p(cb, C.c_int32), _p(vb, C.c_double), C.byref(tC),
+                              C.c_int32(row_lo), C.c_int32(row_hi), C.byref(out))
+    return {k: getattr(out, k) for k, _ in out._fields_}
+
+
+def gustavson_csr(A, B):
+    """Gustavson product with structural union; returns (rowptr, col, val)."""
+    ra, ca, va = _csr_args(A)
+    rb, cb, vb = _csr_args(B)
+    rpc = np.zeros(A.m + 1, dtype=np.int64)
+    lib().ora_gustavson_csr(A.m, B.n, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(va, C.c_double),
+                            _p(rb, C.c_int64), _p(cb, C.c_int32), _p(vb, C.c_double),
+                            _p(rpc, C.c_int64), None, None)
+    nnz = int(rpc[-1])
+    colc = np.zeros(max(nnz, 1), dtype=np.int32)
+    valc = np.zeros(max(nnz, 1), dtype=np.float64)
+    lib().ora_gustavson_csr(A.m, B.n, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(va, C.c_double),
+                            _p(rb, C.c_int64), _p(cb, C.c_int32), _p(vb, C.c_double),
+                            _p(rpc, C.c_int64), _p(colc, C.c_int32), _p(valc, C.c_double))
+    return rpc, colc[:nnz], valc[:nnz]
+
+
+def flops(A, B):
+    ra, ca, _ = _csr_args(A)
+    rb, _, _ = _csr_args(B)
+    return int(lib().ora_flops(A.m, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(rb, C.c_int64)))
+
+
+def cpu_baseline(A, B, row_lo, row_hi, nthreads=1, convert=False):
+    """Times the reference's CPU path on rows [row_lo,row_hi): returns
+    (products, gustavson_seconds, conversion_seconds)."""
+    ra, ca, va = _csr_args(A)
+    rb, cb, vb = _csr_args(B)
+    conv, gus = C.c_double(), C.c_double()
+    prod = lib().ora_cpu_baseline(A.m, B.n, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(va, C.c_double),
+                                  _p(rb, C.c_int64), _p(cb, C.c_int32), _p(vb, C.c_double),
+                                  C.c_int32(row_lo), C.c_int32(row_hi), C.c_int32(nthreads),
+                                  C.c_int32(1 if convert else 0), C.byref(conv), C.byref(gus))
+    return int(prod), gus.value, conv.value
+
+
+def load_mtx(path):
+    m, n, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+    lib().ora_load_mtx(os.fsencode(path), C.byref(m), C.byref(n), C.byref(nnz), None, None, None)
+    rp = np.zeros(m.value + 1, dtype=np.int64)
+    col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+    val = np.zeros(max(nnz.value, 1), dtype=np.float64)
+    lib().ora_load_mtx(os.fsencode(path), C.byref(m), C.byref(n), C.byref(nnz), _p(rp, C.c_int64),
+                       _p(col, C.c_int32), _p(val, C.c_double))
+    return m.value, n.value, rp, col[:nnz.value], val[:nnz.value]
+
+
+# ---------------------------------------------------------------- reference-compiled checker
+def ref_csr2tile(csr, as_b=False):
+    """The reference's own csr2tile (+ build_csc_tiles) compiled from its sources."""
+    rp, col, val = _csr_args(csr)
+    t = OraTile()
+    fn = ref().ref_csr2tile_csc if as_b else ref().ref_csr2tile
+    fn(csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double), C.byref(t))
+    return tile_to_dict(t, ref().ref_tile_free)
+
+
+def ref_load_mtx(path):
+    m, n, nnz = C.c_int32(), C.c_int32(), C.c_int64()
+    ref().ref_load_mtx(os.fsencode(path), C.byref(m), C.byref(n), C.byref(nnz), None, None, None)
+    rp = np.zeros(m.value + 1, dtype=np.int64)
+    col = np.zeros(max(nnz.value, 1), dtype=np.int32)
+    val = np.zeros(max(nnz.value, 1), dtype=np.float64)
+    ref().ref_load_mtx(os.fsencode(path), C.byref(m), C.byref(n), C.byref(nnz), _p(rp, C.c_int64),
+                       _p(col, C.c_int32), _p(val, C.c_double))
+    return m.value, n.value, rp, col[:nnz.value], val[:nnz.value]

@@ -1,0 +1,91 @@
+"""GPU tile-format conversion (tsg_csr2tile, tsg_build_csc_tiles, tsg_tile2csr)
+against the reference's own conversion output (golden fixtures written by
+src/tile_format.h) and against the oracle, bit for bit."""
+import os
+
+import numpy as np
+import pytest
+
+import gpu_spgemm_amd as g
+from oracle import oracle as O
+from tests.conftest import assert_tiles_equal
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "tile_format_golden.npz")
+CASES = ["er200", "lap20x13", "rect77x300", "dense40", "sparse50"]
+
+
+def gold(name):
+    z = np.load(GOLD, allow_pickle=False)
+    A = g.HostCSR(int(z[f"{name}/m"]), int(z[f"{name}/n"]), z[f"{name}/rowptr"], z[f"{name}/col"],
+                  z[f"{name}/val"])
+    exp = {}
+    for tag in ("A", "B"):
+        exp[tag] = {k.split("/")[-1]: (z[k] if z[k].ndim else z[k].item())
+                    for k in z.files if k.startswith(f"{name}/{tag}/")}
+    return A, exp
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_csr2tile_matches_reference_golden(ctx, name):
+    print(f"[case] csr2tile golden {name}", flush=True)
+    A, exp = gold(name)
+    assert_tiles_equal(ctx.tiles_from_csr(A).download(), exp["A"], label="A ")
+    assert_tiles_equal(ctx.tiles_from_csr(A, as_b=True).download(), exp["B"], label="B ")
+
+
+def _matrices():
+    return {
+        "er4096": lambda: g.gen_er(4096, 8, seed=0),
+        "lap2d_64x48": lambda: g.gen_laplace2d(64, 48),
+        "stencil27_9": lambda: g.gen_stencil27(9, 9, 9),
+        "aggregation": lambda: g.gen_aggregation(10, 6, 4),
+        "rmat10": lambda: g.gen_rmat(10, 16, seed=1, dtype=g.F64),
+        "er_var": lambda: g.gen_er_var(1000, 0, 40, seed=2),
+    }
+
+
+@pytest.mark.parametrize("name", list(_matrices()))
+@pytest.mark.parametrize("f32", [False, True])
+def test_gpu_conversion_matches_oracle(ctx, name, f32):
+    print(f"[case] conversion {name} f32={f32}", flush=True)
+    A = _matrices()[name]()
+    h = A.astype(g.F32) if f32 else A
+    assert_tiles_equal(ctx.tiles_from_csr(h).download(), O.csr2tile(A, f32=f32), label="A ")
+    assert_tiles_equal(ctx.tiles_from_csr(h, as_b=True).download(), O.csr2tile_b(A, f32=f32), label="B ")
+
+
+def test_gpu_conversion_wide_rows_take_several_windows(ctx):
+    # tile columns span more than one LDS window (32,768 tile columns per pass)
+    n = 16 * 70000 + 5
+    rows = np.repeat(np.arange(40), 3)
+    cols = np.array([0, 16 * 33000 + 3, n - 1] * 40)
+    rp = np.arange(0, 121, 3, dtype=np.int64)
+    A = g.HostCSR(40, n, rp, cols, np.linspace(-1, 1, 120))
+    print("[case] wide rows", flush=True)
+    assert_tiles_equal(ctx.tiles_from_csr(A).download(), O.csr2tile(A), label="A ")
+    assert_tiles_equal(ctx.tiles_from_csr(A, as_b=True).download(), O.csr2tile_b(A), label="B ")
+
+
+def test_gpu_conversion_empty(ctx):
+    A = g.HostCSR(33, 20, np.zeros(34, np.int64), np.zeros(0, np.int32), np.zeros(0))
+    d = ctx.tiles_from_csr(A).download()
+    assert d["numtile"] == 0 and d["nnz"] == 0 and np.all(d["tile_ptr"] == 0)
+    b = ctx.tiles_from_csr(A, as_b=True).download()
+    assert b["numtile"] == 0 and np.all(b["csc_tile_ptr"] == 0)
+
+
+def test_gpu_conversion_rejects_unsorted_rows(ctx):
+    A = g.HostCSR(2, 40, [0, 2, 3], [30, 5, 7], [1.0, 2.0, 3.0])
+    with pytest.raises(g.TsgError) as e:
+        ctx.tiles_from_csr(A)
+    assert e.value.code == -5
+
+
+@pytest.mark.parametrize("name", ["er4096", "rmat10", "aggregation"])
+def test_gpu_tile2csr_roundtrip(ctx, name):
+    A = _matrices()[name]()
+    t = ctx.tiles_from_csr(A)
+    B = t.to_csr()
+    assert np.array_equal(B.rowptr, A.rowptr) and np.array_equal(B.col, A.col)
+    assert np.array_equal(B.val, A.val)

@@ -1,0 +1,242 @@
+"""Pins the oracle (CPU restatement of the reference path) before it is trusted.
+
+- conversion (csr2tile / build_csc_tiles / load_mtx) against golden fixtures
+  written by the reference's own src/tile_format.h (tests/golden/), and live
+  against oracle/_ref when it is built;
+- Step 1 against the closed form: the C tile set is the boolean product of
+  the tile patterns of A and B;
+- Step 2 against the closed form: maskC is the element pattern of the boolean
+  product A_pattern * B_pattern, tile_nnzC its per-tile popcount;
+- Step 3 against brute-force dense products and the Gustavson recompute, and
+  bit-exactly on integer stencils;
+- planted mistakes (a transposed operand, a dropped pair, a flipped mask bit)
+  must make these checks fail.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+from tests.conftest import assert_tiles_equal
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "tile_format_golden.npz")
+CASES = ["er200", "lap20x13", "rect77x300", "dense40", "sparse50"]
+
+
+class CSR:
+    def __init__(self, m, n, rowptr, col, val):
+        self.m, self.n = int(m), int(n)
+        self.rowptr = np.asarray(rowptr, dtype=np.int64)
+        self.col = np.asarray(col, dtype=np.int32)
+        self.val = np.asarray(val, dtype=np.float64)
+
+    def to_scipy(self):
+        return sp.csr_matrix((self.val, self.col, self.rowptr), shape=(self.m, self.n))
+
+
+def from_scipy(S):
+    S = sp.csr_matrix(S)
+    S.sort_indices()
+    return CSR(S.shape[0], S.shape[1], S.indptr, S.indices, S.data)
+
+
+def golden():
+    return np.load(GOLD, allow_pickle=False)
+
+
+def gold_case(z, name):
+    A = CSR(z[f"{name}/m"], z[f"{name}/n"], z[f"{name}/rowptr"], z[f"{name}/col"], z[f"{name}/val"])
+    exp = {}
+    for tag in ("A", "B"):
+        d = {}
+        for k in z.files:
+            if k.startswith(f"{name}/{tag}/"):
+                v = z[k]
+                d[k.split("/")[-1]] = v if v.ndim else v.item()
+        exp[tag] = d
+    return A, exp
+
+
+def rand_csr(m, n, density, seed, vals="uniform"):
+    rng = np.random.default_rng(seed)
+    S = sp.random(m, n, density=density, format="csr", random_state=rng)
+    if vals == "int":
+        S.data = rng.integers(-3, 4, S.nnz).astype(np.float64)
+        S.data[S.data == 0] = 1.0
+    else:
+        S.data = rng.uniform(-1, 1, S.nnz)
+    return from_scipy(S)
+
+
+def lap2d(nx, ny):
+    ex = sp.diags([-1, 2, -1], [-1, 0, 1], shape=(nx, nx))
+    ey = sp.diags([-1, 2, -1], [-1, 0, 1], shape=(ny, ny))
+    L = sp.kron(sp.identity(ny), ex) + sp.kron(ey, sp.identity(nx))
+    return from_scipy(L)
+
+
+# ------------------------------------------------------------------ conversion
+@pytest.mark.parametrize("name", CASES)
+def test_csr2tile_matches_reference_golden(name):
+    z = golden()
+    A, exp = gold_case(z, name)
+    assert_tiles_equal(O.csr2tile(A), exp["A"], label=f"{name} A ")
+    assert_tiles_equal(O.csr2tile_b(A), exp["B"], label=f"{name} B ")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_csr2tile_matches_reference_live(seed):
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    rng = np.random.default_rng(seed)
+    m, n = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+    A = rand_csr(m, n, float(rng.uniform(0.005, 0.2)), seed)
+    assert_tiles_equal(O.csr2tile(A), O.ref_csr2tile(A), label="A ")
+    assert_tiles_equal(O.csr2tile_b(A), O.ref_csr2tile(A, as_b=True), label="B ")
+
+
+@pytest.mark.parametrize("name", ["sym", "pattern", "complex"])
+def test_load_mtx_matches_reference_golden(name):
+    z = golden()
+    p = os.path.join(os.path.dirname(GOLD), f"{name}.mtx")
+    m, n, rp, col, val = O.load_mtx(p)
+    assert (m, n) == (int(z[f"mtx_{name}/m"]), int(z[f"mtx_{name}/n"]))
+    np.testing.assert_array_equal(rp, z[f"mtx_{name}/rowptr"])
+    np.testing.assert_array_equal(col, z[f"mtx_{name}/col"])
+    np.testing.assert_array_equal(val, z[f"mtx_{name}/val"])
+
+
+# ------------------------------------------------------------------ closed forms
+def tile_pattern(S):
+    S = sp.coo_matrix(S)
+    tm, tn = (S.shape[0] + 15) // 16, (S.shape[1] + 15) // 16
+    P = sp.coo_matrix((np.ones(S.nnz), (S.row // 16, S.col // 16)), shape=(tm, tn)).tocsr()
+    P.data[:] = 1
+    return P
+
+
+def closed_form_structure(A, B):
+    """C tile set = bool(tilepat(A) @ tilepat(B)); masks = element pattern of bool(A) @ bool(B)."""
+    SA, SB = A.to_scipy(), B.to_scipy()
+    T = (tile_pattern(SA) @ tile_pattern(SB)).tocsr()
+    T.sort_indices()
+    PA, PB = SA.copy(), SB.copy()
+    PA.data[:] = 1
+    PB.data[:] = 1
+    E = sp.coo_matrix(PA @ PB)
+    return T, E
+
+
+def check_structure(Cd, A, B):
+    T, E = closed_form_structure(A, B)
+    np.testing.assert_array_equal(Cd["tile_ptr"], T.indptr)
+    np.testing.assert_array_equal(Cd["tile_columnidx"], T.indices)
+    # masks from the element pattern
+    tid = {}
+    for t, (r, c) in enumerate(zip(Cd["tile_rowidx"], Cd["tile_columnidx"])):
+        tid[(int(r), int(c))] = t
+    mask = np.zeros(len(Cd["tile_columnidx"]) * 16, dtype=np.uint16)
+    for r, c in zip(E.row, E.col):
+        t = tid[(r // 16, c // 16)]
+        mask[t * 16 + r % 16] |= np.uint16(1 << (15 - c % 16))
+    np.testing.assert_array_equal(Cd["mask"], mask)
+    pc = np.array([bin(int(x)).count("1") for x in mask]).reshape(-1, 16)
+    np.testing.assert_array_equal(np.diff(Cd["tile_nnz"]), pc.sum(1))
+    excl = np.concatenate([np.zeros((pc.shape[0], 1), int), np.cumsum(pc, 1)[:, :-1]], 1)
+    np.testing.assert_array_equal(Cd["tile_csr_Ptr"].reshape(-1, 16), excl)
+
+
+def tiles_to_dense(Cd):
+    M = np.zeros((Cd["m"], Cd["n"]))
+    for t in range(len(Cd["tile_columnidx"])):
+        tr, tc = Cd["tile_rowidx"][t], Cd["tile_columnidx"][t]
+        for e in range(Cd["tile_nnz"][t], Cd["tile_nnz"][t + 1]):
+            b = int(Cd["tile_csr_Col"][e])
+            M[tr * 16 + (b >> 4), tc * 16 + (b & 15)] += Cd["tile_csr_Val"][e]
+    return M
+
+
+@pytest.mark.parametrize("shape", [(40, 40, 40, 0.2), (33, 70, 50, 0.1), (100, 100, 100, 0.03), (17, 9, 31, 0.5)])
+def test_tiled_oracle_closed_forms(shape):
+    m, k, n, d = shape
+    A, B = rand_csr(m, k, d, 1), rand_csr(k, n, d, 2)
+    Cd = O.spgemm_tiled(A, B)
+    check_structure(Cd, A, B)
+    dense = A.to_scipy().toarray() @ B.to_scipy().toarray()
+    np.testing.assert_allclose(tiles_to_dense(Cd), dense, rtol=1e-13, atol=1e-13)
+    chk = O.gustavson_check(A, B, Cd)
+    assert chk["missing"] == 0 and chk["extra"] == 0 and chk["pass_"] == 1
+    assert chk["max_scaled"] <= 1e-15
+
+
+def test_tiled_oracle_stencil_exact():
+    A = lap2d(37, 29)
+    Cd = O.spgemm_tiled(A, A)
+    check_structure(Cd, A, A)
+    exact = (A.to_scipy() @ A.to_scipy()).toarray()
+    assert np.array_equal(tiles_to_dense(Cd), exact)   # integer arithmetic: bit-exact
+
+
+def test_tiled_oracle_f32_matches_f64_within_tolerance():
+    A = rand_csr(120, 120, 0.05, 7)
+    c64 = O.spgemm_tiled(A, A)
+    c32 = O.spgemm_tiled(A, A, f32=True)
+    for k in ("tile_ptr", "tile_columnidx", "tile_nnz", "mask", "tile_csr_Col"):
+        np.testing.assert_array_equal(c64[k], c32[k])
+    chk = O.gustavson_check(A, A, c32)
+    assert chk["max_scaled"] < 1e-6 and chk["pass_"] == 1
+
+
+def test_empty_and_degenerate():
+    Z = CSR(20, 20, np.zeros(21), np.zeros(0), np.zeros(0))
+    Cd = O.spgemm_tiled(Z, Z)
+    assert Cd["numtile"] == 0 and Cd["nnz"] == 0
+    A = rand_csr(1, 1, 1.0, 3)
+    Cd = O.spgemm_tiled(A, A)
+    assert Cd["numtile"] == 1 and Cd["nnz"] == 1
+    assert Cd["tile_csr_Val"][0] == A.val[0] * A.val[0]
+
+
+def test_gustavson_matches_scipy_and_flops():
+    A = rand_csr(150, 150, 0.04, 9)
+    rp, col, val = O.gustavson_csr(A, A)
+    S = A.to_scipy()
+    ref = (S @ S).toarray()
+    D = sp.csr_matrix((val, col, rp), shape=(150, 150)).toarray()
+    np.testing.assert_allclose(D, ref, rtol=1e-14, atol=1e-14)
+    P = S.copy()
+    P.data[:] = 1
+    assert len(col) == (P @ P).nnz        # structural union, no cancellation drop
+    expected = sum(int(S.indptr[c + 1] - S.indptr[c]) for c in S.indices)
+    assert O.flops(A, A) == expected
+
+
+# ------------------------------------------------------------------ planted mistakes
+def test_planted_transposed_operand_is_caught():
+    A, B = rand_csr(60, 60, 0.08, 4), rand_csr(60, 60, 0.08, 5)
+    Bt = from_scipy(B.to_scipy().T)          # a transposed operand: same shape, wrong product
+    check_structure(O.spgemm_tiled(A, B), A, B)
+    with pytest.raises(AssertionError):
+        check_structure(O.spgemm_tiled(A, Bt), A, B)
+
+
+def test_planted_dropped_pair_is_caught():
+    A = rand_csr(64, 64, 0.1, 6)
+    Cd = O.spgemm_tiled(A, A)
+    bad = dict(Cd)
+    bad["tile_csr_Val"] = Cd["tile_csr_Val"].copy()
+    bad["tile_csr_Val"][len(bad["tile_csr_Val"]) // 2] += 0.25   # one contribution lost/added
+    chk = O.gustavson_check(A, A, bad)
+    assert chk["max_scaled"] > 1e-3
+
+
+def test_planted_mask_bit_is_caught():
+    A = rand_csr(48, 48, 0.1, 8)
+    Cd = O.spgemm_tiled(A, A)
+    bad = dict(Cd)
+    bad["mask"] = Cd["mask"].copy()
+    bad["mask"][np.nonzero(bad["mask"])[0][0]] ^= 1
+    with pytest.raises(AssertionError):
+        check_structure(bad, A, A)
