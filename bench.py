@@ -106,36 +106,40 @@ def main():
     log("device", ctx.device_name())
     tA = ctx.tiles_from_csr(A)
     tB = ctx.tiles_from_csr(B, as_b=True)
-    sched = g.SCHED_LEARNED if args.scheduler == "learned" else g.SCHED_PLAIN
     s = 8
 
-    for _ in range(args.warmup):
-        C, st = ctx.spgemm(tA, tB, scheduler=sched)
-        C.free()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        C, st = ctx.spgemm(tA, tB, scheduler=sched)
-        C.free()
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    ms = wall * 1e3 / args.steps
-    log(f"timed {args.steps} steps: {ms:.4f} ms/step")
+    def measure(label, **kw):
+        for _ in range(args.warmup):
+            C, st = ctx.spgemm(tA, tB, **kw)
+            C.free()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            C, st = ctx.spgemm(tA, tB, **kw)
+            C.free()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        br = []
+        for _ in range(max(5, min(args.steps, 20))):   # per-step HIP-event breakdown on the library stream
+            C, st = ctx.spgemm(tA, tB, timing=True, **kw)
+            br.append(st)
+            Cd = {"tilem": C.s.tilem, "tilen": C.s.tilen, "numtile": C.s.numtile, "nnz": C.s.nnz}
+            C.free()
+        ev = {k: statistics.mean(x[k] for x in br) for k in ("ms_step1", "ms_step2", "ms_step3")}
+        log(f"{label}: {ms:.4f} ms/step  events: structure {ev['ms_step1']:.4f}  symbolic {ev['ms_step2']:.4f}"
+            f"  numeric {ev['ms_step3']:.4f} ms")
+        return ms, ev, Cd
 
-    # per-step HIP-event breakdown (events on the library stream)
-    br = []
-    for _ in range(max(5, min(args.steps, 20))):
-        C, st = ctx.spgemm(tA, tB, scheduler=sched, timing=True)
-        br.append(st)
-        Ad = {"tilem": tA.s.tilem, "tilen": tA.s.tilen, "numtile": tA.s.numtile, "nnz": tA.s.nnz}
-        Bd = {"tilem": tB.s.tilem, "tilen": tB.s.tilen, "numtile": tB.s.numtile, "nnz": tB.s.nnz}
-        Cd = {"tilem": C.s.tilem, "tilen": C.s.tilen, "numtile": C.s.numtile, "nnz": C.s.nnz}
-        C.free()
-    step3 = statistics.mean(x["ms_step3"] for x in br)
-    step1 = statistics.mean(x["ms_step1"] for x in br)
-    step2 = statistics.mean(x["ms_step2"] for x in br)
-    log(f"events: step1 {step1:.4f} ms, step2 {step2:.4f} ms, step3 {step3:.4f} ms")
-
+    Ad = {"tilem": tA.s.tilem, "tilen": tA.s.tilen, "numtile": tA.s.numtile, "nnz": tA.s.nnz}
+    Bd = {"tilem": tB.s.tilem, "tilen": tB.s.tilen, "numtile": tB.s.numtile, "nnz": tB.s.nnz}
+    runs = {}
+    runs["ctile"] = measure("reference C-tile method", variant=1)
+    runs["plain"] = measure("plain (binned, exact sizes)", variant=0)
+    if args.scheduler == "learned":
+        runs["learned"] = measure("learned", scheduler=g.SCHED_LEARNED)
+    best = "learned" if "learned" in runs else "plain"
+    ms, ev, Cd = runs[best]
+    step1, step2, step3 = ev["ms_step1"], ev["ms_step2"], ev["ms_step3"]
     gflops = 2.0 * products / (ms * 1e6)
     pipe_bytes = alg_bytes(Ad, Bd, Cd, s)
     nbytes = numeric_bytes(Ad, Bd, Cd, s)
@@ -153,12 +157,14 @@ def main():
         "vs_baseline": None,
         "dtype": cfg["dtype"],
         "data": "synthetic (generated on the host, converted to tiles on the GPU before timing)",
-        "config": {**cfg, "scheduler": args.scheduler, "products": products,
+        "config": {**cfg, "scheduler": best, "products": products,
                    "numtileC": Cd["numtile"], "nnzC": Cd["nnz"], "parallelism": "single GPU"},
         "pipeline_alg_GBps": round(pipe_bytes / (ms * 1e-3) / 1e9, 2),
         "pipeline_alg_bytes": pipe_bytes,
         "step_ms": {"structure": round(step1, 5), "symbolic": round(step2, 5), "numeric": round(step3, 5)},
-        "roofline": {"bound": "hbm", "kernel": "k_numeric_rows", "achieved": round(achieved, 2),
+        "variants_ms_per_step": {k: round(v[0], 5) for k, v in runs.items()},
+        "variants_gflops": {k: round(2.0 * products / (v[0] * 1e6), 3) for k, v in runs.items()},
+        "roofline": {"bound": "hbm", "kernel": "numeric step (k_num_staged + k_numeric_list)", "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "alg_bytes_per_launch": nbytes},
         "cpu_baseline": None,

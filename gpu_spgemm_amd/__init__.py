@@ -36,12 +36,15 @@ class CSRStruct(C.Structure):
 class TileStruct(C.Structure):
     _fields_ = [("m", C.c_int32), ("n", C.c_int32), ("tilem", C.c_int32), ("tilen", C.c_int32),
                 ("nnz", C.c_int64), ("numtile", C.c_int64), ("dtype", C.c_int32),
-                ("location", C.c_int32), ("is_csc", C.c_int32), ("reserved", C.c_int32),
+                ("location", C.c_int32), ("is_csc", C.c_int32), ("flags", C.c_int32),
                 ("tile_ptr", C.POINTER(C.c_int64)), ("tile_columnidx", C.POINTER(C.c_int32)),
                 ("tile_rowidx", C.POINTER(C.c_int32)), ("tile_nnz", C.POINTER(C.c_int64)),
                 ("tile_csr_Val", C.c_void_p), ("tile_csr_Col", C.POINTER(C.c_uint8)),
                 ("tile_csr_Ptr", C.POINTER(C.c_uint8)), ("mask", C.POINTER(C.c_uint16)),
-                ("csc_tile_ptr", C.POINTER(C.c_int64)), ("csc_tile_rowidx", C.POINTER(C.c_int32))]
+                ("csc_tile_ptr", C.POINTER(C.c_int64)), ("csc_tile_rowidx", C.POINTER(C.c_int32)),
+                ("csr_tile_nnz", C.POINTER(C.c_int64)), ("csr_tile_Ptr", C.POINTER(C.c_uint8)),
+                ("csr_mask", C.POINTER(C.c_uint16)), ("csr_tile_Val", C.c_void_p),
+                ("row_nnz_ptr", C.POINTER(C.c_int64))]
 
 
 class Options(C.Structure):

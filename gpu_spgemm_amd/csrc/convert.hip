@@ -35,7 +35,7 @@ constexpr int CV_WIN_WORDS = 1024;  // 32,768 tile columns per pass
 // ---------------------------------------------------------------- csr2tile
 template <int MODE>  // 0 = count, 1 = fill
 __global__ void __launch_bounds__(CV_WAVES *WAVE)
-k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m, int32_t tilem,
+k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m, int32_t n, int32_t tilem,
       int64_t *__restrict__ tile_ptr, int32_t *__restrict__ tile_colidx, uint8_t *__restrict__ rowcnt,
       uint16_t *__restrict__ mask, int32_t *__restrict__ flags) {
     __shared__ uint32_t s_bm[CV_WAVES][CV_WIN_WORDS];
@@ -58,7 +58,7 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
     uint32_t *bm = s_bm[w];
     int64_t count = 0;
     int64_t tbase = MODE ? tile_ptr[bi] : 0;
-    bool bad = false;
+    bool bad = false, dup = false;
     for (int64_t w0 = cmin; w0 <= cmax; w0 += (int64_t)CV_WIN_WORDS * 32) {
         const int32_t wbits = (int32_t)min<int64_t>((int64_t)CV_WIN_WORDS * 32, cmax - w0 + 1);
         const int nwords = (wbits + 31) >> 5;
@@ -116,6 +116,7 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
                     mk |= 0x8000u >> (cq & 15);
                     n++;
                 }
+                if (n != __popc(mk)) dup = true;  // a column repeated inside the row
                 rowcnt[T * 16 + r] = (uint8_t)n;
                 mask[T * 16 + r] = (uint16_t)mk;
             }
@@ -123,9 +124,12 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
             wave_sync();
         }
     }
+    bool range_bad = false;
     if (MODE == 0) {
-        // rows must be sorted by column: a descending step is rejected
+        // rows must be sorted by column: a descending step is rejected;
+        // column indices must lie in [0, n)
         for (int64_t j = js + lane; j < je; j += WAVE) {
+            if (col[j] < 0 || col[j] >= n) range_bad = true;
             if (j == js) continue;
             int r = 0;
             while (r < 15 && s_rp[w][r + 1] <= j) r++;
@@ -137,6 +141,8 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
         }
     }
     if (__ballot(bad) && lane == 0) atomicOr(flags, 1);
+    if (__ballot(dup) && lane == 0) atomicOr(flags, 2);
+    if (__ballot(range_bad) && lane == 0) atomicOr(flags, 4);
 }
 
 // Per tile: exclusive scan of 16 row counts (in place) and the tile's nnz.
@@ -284,6 +290,13 @@ k_csc_gather(int64_t numtile, const int64_t *__restrict__ inv, const int64_t *__
     }
 }
 
+// row_nnz_ptr[k] = first payload entry of tile-row k (tile_nnz at its first tile).
+__global__ void __launch_bounds__(256) k_row_nnz_ptr(int32_t tilem, const int64_t *__restrict__ tile_ptr,
+                                                     const int64_t *__restrict__ tile_nnz, int64_t *__restrict__ rnp) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k <= tilem) rnp[k] = tile_nnz[tile_ptr[k]];
+}
+
 static unsigned blocks_for(int64_t threads, int per) { return (unsigned)((threads + per - 1) / per); }
 
 }  // namespace tsg
@@ -317,18 +330,15 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
         (void)hipMemsetAsync(T->tile_ptr, 0, sizeof(int64_t), s);
     } else {
         k_c2t<0><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
-            A->rowptr, A->col, A->m, T->tilem, T->tile_ptr, nullptr, nullptr, nullptr, flags);
+            A->rowptr, A->col, A->m, A->n, T->tilem, T->tile_ptr, nullptr, nullptr, nullptr, flags);
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "k_c2t count launch"));
     }
     if ((rc = scan_inclusive_i64(ctx, T->tile_ptr, (int64_t)T->tilem + 1))) return cleanup(rc);
     if ((rc = read_i64(ctx, T->tile_ptr + T->tilem, &T->numtile))) return cleanup(rc);
-    int64_t fl = 0;
-    {
-        int32_t hf = 0;
-        if (hipMemcpy(&hf, flags, sizeof hf, hipMemcpyDeviceToHost) != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "flag read"));
-        fl = hf;
-    }
-    if (fl) return cleanup(fail(TSG_ERR_UNSUPPORTED, "CSR rows must be sorted by column"));
+    int32_t hf = 0;
+    if ((rc = read_i32(ctx, flags, &hf))) return cleanup(rc);
+    if (hf & 4) return cleanup(fail(TSG_ERR_ARG, "CSR column index outside [0, %d)", A->n));
+    if (hf & 1) return cleanup(fail(TSG_ERR_UNSUPPORTED, "CSR rows must be sorted by column"));
     if (T->numtile > INT32_MAX) return cleanup(fail(TSG_ERR_OVERFLOW, "numtile exceeds int32 tile ids"));
     const int64_t nt = T->numtile;
     if ((rc = palloc(ctx, &T->tile_columnidx, (size_t)nt))) return cleanup(rc);
@@ -342,7 +352,7 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
         return cleanup(fail(TSG_ERR_HIP, "memset"));
     if (T->tilem > 0) {
         k_c2t<1><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
-            A->rowptr, A->col, A->m, T->tilem, T->tile_ptr, T->tile_columnidx, T->tile_csr_Ptr, T->mask, flags);
+            A->rowptr, A->col, A->m, A->n, T->tilem, T->tile_ptr, T->tile_columnidx, T->tile_csr_Ptr, T->mask, flags);
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "k_c2t fill launch"));
     }
     if (nt > 0) {
@@ -363,8 +373,9 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
                 T->tile_nnz, T->tile_csr_Ptr, (double *)T->tile_csr_Val, T->tile_csr_Col);
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "scatter launch"));
     }
+    if ((rc = read_i32(ctx, flags, &hf))) return cleanup(rc);
+    T->flags = (hf & 2) ? 1 : 0;
     pool_free(ctx, flags);
-    if (hipStreamSynchronize(s) != hipSuccess) return fail(TSG_ERR_HIP, "csr2tile: %s", hipGetErrorString(hipGetLastError()));
     return TSG_OK;
 }
 
@@ -440,11 +451,26 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
         if (own_rowidx) pool_free(ctx, rowidx);
         return rc;
     }
-    pool_free(ctx, B->tile_nnz);
-    pool_free(ctx, B->tile_csr_Ptr);
-    pool_free(ctx, B->mask);
+    // keep the CSR-tile-order data for the row-staged kernels; the packed col
+    // bytes are implied by the masks there and are dropped
+    int64_t *rnp = nullptr;
+    rc = palloc(ctx, &rnp, (size_t)B->tilem + 1);
+    if (!rc) {
+        k_row_nnz_ptr<<<blocks_for((int64_t)B->tilem + 1, 256), 256, 0, s>>>(B->tilem, B->tile_ptr, B->tile_nnz, rnp);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "row_nnz_ptr launch");
+    }
+    if (rc) {
+        void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp};
+        for (void *p : ps) pool_free(ctx, p);
+        if (own_rowidx) pool_free(ctx, rowidx);
+        return rc;
+    }
     pool_free(ctx, B->tile_csr_Col);
-    pool_free(ctx, B->tile_csr_Val);
+    B->csr_tile_nnz = B->tile_nnz;
+    B->csr_tile_Ptr = B->tile_csr_Ptr;
+    B->csr_mask = B->mask;
+    B->csr_tile_Val = B->tile_csr_Val;
+    B->row_nnz_ptr = rnp;
     B->tile_nnz = nnz_new;
     B->tile_csr_Ptr = ptr_new;
     B->mask = mask_new;
@@ -454,6 +480,7 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     B->csc_tile_rowidx = crow;
     B->tile_rowidx = rowidx;
     B->is_csc = 1;
+    TSG_HIP(hipStreamSynchronize(s));
     return TSG_OK;
 }
 

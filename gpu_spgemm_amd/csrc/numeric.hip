@@ -1,4 +1,4 @@
-// numeric.hip -- Step 3: values of C.
+// numeric.hip -- Step 3: values of C, one C tile per 16-lane group.
 //
 // Replaces step3_numeric_kernel_v2 (src/spgemm_raw_kernels_v2.cuh:10-139):
 // for every matched pair (ascending k), every A entry (r,kk,va) and every B
@@ -14,6 +14,10 @@
 // loads and FMAs of present columns execute (predicated).  One fused
 // multiply-add per product, pairs in ascending k and A entries in storage
 // order: the summation order is fixed, so results are reproducible run to run.
+//
+// This is the C-tile-centric method of the reference.  It runs every tile in
+// the reference-method variant and the heavy tile-rows (one workgroup per row)
+// in the binned pipeline; light rows go to the row-staged kernels.
 #include "pairs.h"
 
 namespace tsg {
@@ -29,75 +33,133 @@ template <>
 __device__ __forceinline__ float fmav<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 template <typename V>
-__global__ void __launch_bounds__(NU_THREADS)
-k_numeric_rows(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ colidxA,
-               const int64_t *__restrict__ tile_nnzA, const uint8_t *__restrict__ ptrRowA,
-               const uint8_t *__restrict__ colA, const V *__restrict__ valA,
-               const int64_t *__restrict__ csc_ptrB, const int32_t *__restrict__ csc_rowB,
-               const int64_t *__restrict__ tile_nnzB, const uint8_t *__restrict__ ptrRowB,
-               const uint16_t *__restrict__ maskB, const V *__restrict__ valB, int64_t numtileC,
-               const int32_t *__restrict__ rowidxC, const int32_t *__restrict__ colidxC,
-               const int64_t *__restrict__ tile_nnzC, const uint16_t *__restrict__ maskC,
-               const uint8_t *__restrict__ ptrRowC, V *__restrict__ valC, uint8_t *__restrict__ colC) {
-    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    const int64_t ti = (int64_t)blockIdx.x * NU_GROUPS + g;
-    if (ti >= numtileC) return;
-    const int64_t c0 = tile_nnzC[ti];
-    if (tile_nnzC[ti + 1] == c0) return;  // structurally empty tile
-    const int32_t bi = rowidxC[ti], bj = colidxC[ti];
+struct NumArgs {
+    const int64_t *tile_ptrA; const int32_t *colidxA; const int64_t *tile_nnzA; const uint8_t *ptrRowA;
+    const uint8_t *colA; const V *valA;
+    const int64_t *csc_ptrB; const int32_t *csc_rowB; const int64_t *tile_nnzB; const uint8_t *ptrRowB;
+    const uint16_t *maskB; const uint8_t *colB; const V *valB;
+    const int64_t *tile_ptrC; const int32_t *rowidxC; const int32_t *colidxC; const int64_t *tile_nnzC;
+    const uint16_t *maskC; const uint8_t *ptrRowC; V *valC; uint8_t *colC;
+};
+
+template <typename V>
+static NumArgs<V> num_args(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C) {
+    return NumArgs<V>{A->tile_ptr, A->tile_columnidx, A->tile_nnz, A->tile_csr_Ptr, A->tile_csr_Col,
+                      (const V *)A->tile_csr_Val, B->csc_tile_ptr, B->csc_tile_rowidx, B->tile_nnz,
+                      B->tile_csr_Ptr, B->mask, B->tile_csr_Col, (const V *)B->tile_csr_Val, C->tile_ptr, C->tile_rowidx,
+                      C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr, (V *)C->tile_csr_Val,
+                      C->tile_csr_Col};
+}
+
+// Numeric step of one C tile by a 16-lane group (lane = tile row).  DUPS:
+// some row repeats a column, so B row entries are walked by their col bytes
+// in storage order (as the reference does) instead of by mask bits.
+template <typename V, bool DUPS>
+__device__ __forceinline__ void num_tile(int64_t ti, const NumArgs<V> &p) {
+    const int l = lane_id() & 15;
+    const int64_t c0 = p.tile_nnzC[ti];
+    if (p.tile_nnzC[ti + 1] == c0) return;  // structurally empty tile
+    const int32_t bi = p.rowidxC[ti], bj = p.colidxC[ti];
     V acc[16];
 #pragma unroll
     for (int c = 0; c < 16; c++) acc[c] = V(0);
-    for_each_pair(tile_ptrA, colidxA, csc_ptrB, csc_rowB, bi, bj, [&](int64_t a, int64_t b) {
-        const int64_t an0 = tile_nnzA[a];
-        const int e0 = ptrRowA[a * 16 + l];
-        const int e1 = l == 15 ? (int)(tile_nnzA[a + 1] - an0) : (int)ptrRowA[a * 16 + l + 1];
-        const int64_t bn0 = tile_nnzB[b];
+    for_each_pair(p.tile_ptrA, p.colidxA, p.csc_ptrB, p.csc_rowB, bi, bj, [&](int64_t a, int64_t b) {
+        const int64_t an0 = p.tile_nnzA[a];
+        const int e0 = p.ptrRowA[a * 16 + l];
+        const int e1 = l == 15 ? (int)(p.tile_nnzA[a + 1] - an0) : (int)p.ptrRowA[a * 16 + l + 1];
+        const int64_t bn0 = p.tile_nnzB[b];
         for (int e = e0; e < e1; e++) {
-            const int kk = colA[an0 + e] & 15;
-            const V va = valA[an0 + e];
-            const uint32_t mb = maskB[b * 16 + kk];
-            int64_t pb = bn0 + ptrRowB[b * 16 + kk];
+            const int kk = p.colA[an0 + e] & 15;
+            const V va = p.valA[an0 + e];
+            int64_t pb = bn0 + p.ptrRowB[b * 16 + kk];
+            if (DUPS) {
+                const int64_t pe = bn0 + (kk == 15 ? (int)(p.tile_nnzB[b + 1] - bn0) : (int)p.ptrRowB[b * 16 + kk + 1]);
+                for (; pb < pe; pb++) {
+                    const int cb = p.colB[pb] & 15;
+                    const V vb = p.valB[pb];
+#pragma unroll
+                    for (int c = 0; c < 16; c++)
+                        if (c == cb) acc[c] = fmav<V>(va, vb, acc[c]);
+                }
+                continue;
+            }
+            const uint32_t mb = p.maskB[b * 16 + kk];
 #pragma unroll
             for (int c = 0; c < 16; c++) {
                 if (mb & (0x8000u >> c)) {
-                    acc[c] = fmav<V>(va, valB[pb], acc[c]);
+                    acc[c] = fmav<V>(va, p.valB[pb], acc[c]);
                     pb++;
                 }
             }
         }
     });
-    const uint32_t mc = maskC[ti * 16 + l];
-    int64_t o = c0 + ptrRowC[ti * 16 + l];
+    const uint32_t mc = p.maskC[ti * 16 + l];
+    int64_t o = c0 + p.ptrRowC[ti * 16 + l];
 #pragma unroll
     for (int c = 0; c < 16; c++) {
         if (mc & (0x8000u >> c)) {
-            valC[o] = acc[c];
-            colC[o] = (uint8_t)((l << 4) | c);
+            p.valC[o] = acc[c];
+            p.colC[o] = (uint8_t)((l << 4) | c);
             o++;
         }
     }
 }
 
+template <typename V, bool DUPS>
+__global__ void __launch_bounds__(NU_THREADS) k_numeric_tiles(int64_t numtileC, NumArgs<V> p) {
+    const int64_t ti = (int64_t)blockIdx.x * NU_GROUPS + (threadIdx.x >> 4);
+    if (ti >= numtileC) return;
+    num_tile<V, DUPS>(ti, p);
+}
+
+// Heavy tile-rows: one workgroup per listed row, its 16 groups stride over the
+// row's C tiles.
+template <typename V, bool DUPS>
+__global__ void __launch_bounds__(NU_THREADS) k_numeric_list(const int32_t *__restrict__ rows, int64_t row_begin,
+                                                             NumArgs<V> p) {
+    const int32_t bi = rows[row_begin + blockIdx.x];
+    const int64_t ce = p.tile_ptrC[bi + 1];
+    for (int64_t ti = p.tile_ptrC[bi] + (threadIdx.x >> 4); ti < ce; ti += NU_GROUPS) num_tile<V, DUPS>(ti, p);
+}
+
 template <typename V>
-static int launch_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C) {
-    int64_t grid = (C->numtile + NU_GROUPS - 1) / NU_GROUPS;
-    if (grid > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "numeric grid too large");
-    k_numeric_rows<V><<<(unsigned)grid, NU_THREADS, 0, ctx->stream>>>(
-        A->tile_ptr, A->tile_columnidx, A->tile_nnz, A->tile_csr_Ptr, A->tile_csr_Col,
-        (const V *)A->tile_csr_Val, B->csc_tile_ptr, B->csc_tile_rowidx, B->tile_nnz, B->tile_csr_Ptr,
-        B->mask, (const V *)B->tile_csr_Val, C->numtile, C->tile_rowidx, C->tile_columnidx, C->tile_nnz,
-        C->mask, C->tile_csr_Ptr, (V *)C->tile_csr_Val, C->tile_csr_Col);
+static int launch_list_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                         const int32_t *rows, int64_t begin, int64_t count) {
+    if ((A->flags | B->flags) & 1)
+        k_numeric_list<V, true><<<(unsigned)count, NU_THREADS, 0, ctx->stream>>>(rows, begin, num_args<V>(A, B, C));
+    else
+        k_numeric_list<V, false><<<(unsigned)count, NU_THREADS, 0, ctx->stream>>>(rows, begin, num_args<V>(A, B, C));
     TSG_HIP(hipGetLastError());
     return TSG_OK;
+}
+
+template <typename V>
+static int launch_tiles_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                          unsigned grid) {
+    if ((A->flags | B->flags) & 1)
+        k_numeric_tiles<V, true><<<grid, NU_THREADS, 0, ctx->stream>>>(C->numtile, num_args<V>(A, B, C));
+    else
+        k_numeric_tiles<V, false><<<grid, NU_THREADS, 0, ctx->stream>>>(C->numtile, num_args<V>(A, B, C));
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
+int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                        const int32_t *rows, int64_t begin, int64_t count) {
+    if (count <= 0) return TSG_OK;
+    if (count > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "numeric row list too long");
+    if (A->dtype == TSG_F32) return launch_list_t<float>(ctx, A, B, C, rows, begin, count);
+    return launch_list_t<double>(ctx, A, B, C, rows, begin, count);
 }
 
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                    const tsg_options *opts) {
     (void)opts;
     if (C->numtile == 0 || C->nnz == 0) return TSG_OK;
-    if (A->dtype == TSG_F32) return launch_rows<float>(ctx, A, B, C);
-    return launch_rows<double>(ctx, A, B, C);
+    int64_t grid = (C->numtile + NU_GROUPS - 1) / NU_GROUPS;
+    if (grid > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "numeric grid too large");
+    if (A->dtype == TSG_F32) return launch_tiles_t<float>(ctx, A, B, C, (unsigned)grid);
+    return launch_tiles_t<double>(ctx, A, B, C, (unsigned)grid);
 }
 
 }  // namespace tsg

@@ -1,0 +1,819 @@
+// rowstaged.hip -- the binned pipeline: per-tile-row statistics, row binning,
+// and the wave-per-tile-row staged symbolic and numeric kernels.
+//
+// The reference works C tile by C tile (src/spgemm_raw_kernels.cuh:105-202,
+// src/spgemm_raw_kernels_v2.cuh:10-139): every tile walks a chain of dependent
+// global loads (its row pointers, the two index lists, each pair's headers,
+// each pair's values), so on a 5-point Laplacian most of the time is load
+// latency.  Here a whole C tile-row is one wave64's job.  A per-call "link"
+// array gives, for every A tile (bi,k), the CSR-tile range and payload range
+// of B's tile-row k (B keeps its CSR-tile-order data next to its CSC arrays,
+// so a B tile-row is one contiguous range).  A wave then stages everything its
+// row needs into LDS in three bulk, coalesced load levels:
+//   1. the A and C tile-row pointers
+//   2. A's tile headers and links, C's headers
+//   3. A's values, B's tile headers and values
+// and computes from LDS only.  The numeric step gives every lane one output
+// nonzero at a time (outputs of a tile-row are contiguous in C, so the stores
+// coalesce); an output (r,c) of C tile ct runs ct's matched tile pairs in
+// ascending k and, inside a pair, the columns kk of A's row r in ascending
+// order, one FMA per product -- the summation order of the C-tile kernels and
+// of the oracle, so every path yields identical bits.
+//
+// A row is staged only when its sizes fit a variant's LDS capacities; rows are
+// binned by exact statistics emitted with the Step-1 count, and rows that fit
+// no variant run the C-tile kernels, one workgroup per row.
+#include <climits>
+#include <cstring>
+
+#include "tsg_internal.h"
+
+namespace tsg {
+
+// ---------------------------------------------------------------- variants
+template <int NA_, int NC_, int NB_, int PA_, int PB_, int WAVES_>
+struct Variant {
+    static constexpr int NA = NA_, NC = NC_, NB = NB_, PA = PA_, PB = PB_, WAVES = WAVES_;
+};
+// S: 2-D 5-point-like rows; W: wide A tile-rows (27-point stencil times an
+// aggregation); M: medium rows of irregular matrices.
+using VarS = Variant<8, 16, 32, 96, 416, 4>;
+using VarW = Variant<32, 16, 32, 512, 512, 2>;
+using VarM = Variant<32, 48, 64, 512, 1024, 2>;
+constexpr int NUM_STAGED = 3;
+constexpr int MAX_BINS = 4;
+constexpr int DESC_CAP = 512;  // output descriptors of one chunk of (C tile, row) items
+
+struct StageCaps {
+    int32_t na, nc, nb, pa, pb;
+};
+struct BinTable {
+    StageCaps v[MAX_BINS];
+    int32_t nvar;
+};
+
+template <typename Var>
+static StageCaps caps_of() {
+    return StageCaps{Var::NA, Var::NC, Var::NB, Var::PA, Var::PB};
+}
+
+// Largest t in [0,n) with seg[t] <= i (seg ascending, seg[0] = 0).
+__device__ __forceinline__ int owner_of(const int32_t *seg, int n, int32_t i) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (seg[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Bits of a 16-bit row mask that stand for columns below c (bit 15-c is column c).
+__device__ __forceinline__ uint32_t below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
+
+// ---------------------------------------------------------------- A->B links
+// link[t] = {first CSR tile of B's tile-row k, its tile count, its payload
+// count} and lpay[t] = its first payload entry, for A tile t in column k.
+__global__ void __launch_bounds__(256)
+k_ab_link(int64_t numtileA, const int32_t *__restrict__ colidxA, const int64_t *__restrict__ tile_ptrB,
+          const int64_t *__restrict__ rnpB, int4 *__restrict__ link, int64_t *__restrict__ lpay) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t >= numtileA) return;
+    const int32_t k = colidxA[t];
+    const int64_t bs = tile_ptrB[k], be = tile_ptrB[k + 1];
+    int64_t ps = 0, pe = 0;
+    if (rnpB) { ps = rnpB[k]; pe = rnpB[k + 1]; }
+    link[t] = make_int4((int32_t)bs, (int32_t)(be - bs), (int32_t)min<int64_t>(pe - ps, INT_MAX), 0);
+    lpay[t] = ps;
+}
+
+// ---------------------------------------------------------------- Step 1 count + statistics
+constexpr int RS_WAVES = 4;
+constexpr int RS_WIN_WORDS = 512;  // 16,384 tile columns per pass
+
+// Exact C tile count of each tile-row (as step1_count_kernel,
+// src/spgemm_raw_kernels.cuh:23-57) plus the sizes that decide its bin:
+// A tiles, A payload, B tiles and B payload reached, C tiles.
+__global__ void __launch_bounds__(RS_WAVES *WAVE)
+k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
+            const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
+            int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats) {
+    __shared__ uint32_t s_bm[RS_WAVES][RS_WIN_WORDS];
+    __shared__ int32_t s_bs[RS_WAVES][WAVE];
+    __shared__ int32_t s_pre[RS_WAVES][WAVE];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int32_t bi = blockIdx.x * RS_WAVES + w;
+    if (bi >= tilemA) return;
+    uint32_t *bm = s_bm[w];
+    const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
+    int32_t cmin = INT_MAX, cmax = -1;
+    int64_t nb = 0, pb = 0;
+    for (int64_t i = as + lane; i < ae; i += WAVE) {
+        const int4 l = link[i];
+        if (l.y > 0) {
+            cmin = min(cmin, colidxB[l.x]);
+            cmax = max(cmax, colidxB[l.x + l.y - 1]);
+        }
+        nb += l.y;
+        pb += l.z;
+    }
+    cmin = wave_min(cmin);
+    cmax = wave_max(cmax);
+    nb = wave_sum(nb);
+    pb = wave_sum(pb);
+    int64_t count = 0;
+    for (int64_t w0 = cmin; w0 <= cmax; w0 += (int64_t)RS_WIN_WORDS * 32) {
+        const int32_t wbits = (int32_t)min<int64_t>((int64_t)RS_WIN_WORDS * 32, cmax - w0 + 1);
+        const int nwords = (wbits + 31) >> 5;
+        for (int i = lane; i < nwords; i += WAVE) bm[i] = 0;
+        wave_sync();
+        for (int64_t c0 = as; c0 < ae; c0 += WAVE) {
+            const int64_t i = c0 + lane;
+            int32_t len = 0, bs = 0;
+            if (i < ae) {
+                const int4 l = link[i];
+                bs = l.x;
+                len = l.y;
+            }
+            const int32_t incl = wave_incl_sum(len);
+            const int32_t total = __shfl(incl, WAVE - 1, WAVE);
+            s_pre[w][lane] = incl;
+            s_bs[w][lane] = bs;
+            wave_sync();
+            for (int32_t t = lane; t < total; t += WAVE) {
+                int lo = 0, hi = WAVE - 1;
+                while (lo < hi) {
+                    int mid = (lo + hi) >> 1;
+                    if (s_pre[w][mid] > t) hi = mid; else lo = mid + 1;
+                }
+                const int32_t excl = lo ? s_pre[w][lo - 1] : 0;
+                const int64_t off = (int64_t)colidxB[s_bs[w][lo] + (t - excl)] - w0;
+                if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
+            }
+            wave_sync();
+        }
+        int32_t cnt = 0;
+        for (int i = lane; i < nwords; i += WAVE) cnt += __popc(bm[i]);
+        count += wave_sum(cnt);
+        wave_sync();
+    }
+    if (lane == 0) {
+        tile_ptrC[bi + 1] = count;
+        if (bi == 0) tile_ptrC[0] = 0;
+        const int64_t na = ae - as, pa = tile_nnzA[ae] - tile_nnzA[as];
+        int b = bt.nvar;
+        for (int v = 0; v < bt.nvar; v++) {
+            const StageCaps c = bt.v[v];
+            if (na <= c.na && count <= c.nc && nb <= c.nb && pa <= c.pa && pb <= c.pb) { b = v; break; }
+        }
+        bin[bi] = (uint8_t)b;
+        if (stats) {  // per-row features for the learned scheduler
+            int32_t *s = stats + (int64_t)bi * 8;
+            s[0] = (int32_t)min<int64_t>(na, INT_MAX);
+            s[1] = (int32_t)min<int64_t>(pa, INT_MAX);
+            s[2] = (int32_t)min<int64_t>(nb, INT_MAX);
+            s[3] = (int32_t)min<int64_t>(pb, INT_MAX);
+            s[4] = (int32_t)min<int64_t>(count, INT_MAX);
+            s[5] = cmax >= cmin ? cmax - cmin + 1 : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- row binning
+constexpr int BIN_THREADS = 256;
+constexpr int BIN_ITEMS = 4;
+constexpr int BIN_TILE = BIN_THREADS * BIN_ITEMS;
+
+__global__ void __launch_bounds__(BIN_THREADS)
+k_bin_count(const uint8_t *__restrict__ bin, int64_t n, int nbins, int64_t nblocks, int64_t *__restrict__ counts) {
+    __shared__ int s_cnt[MAX_BINS];
+    if (threadIdx.x < MAX_BINS) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BIN_ITEMS; i++) {
+        const int64_t idx = (int64_t)blockIdx.x * BIN_TILE + i * BIN_THREADS + threadIdx.x;
+        if (idx < n) atomicAdd(&s_cnt[bin[idx]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < nbins) counts[1 + threadIdx.x * nblocks + blockIdx.x] = s_cnt[threadIdx.x];
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[0] = 0;
+}
+
+// Stable partition of row ids by bin: rows of bin b, ascending, at
+// [start_b, start_{b+1}).  counts holds the inclusive scan of
+// [0, (bin, block) counts in bin-major order].
+__global__ void __launch_bounds__(BIN_THREADS)
+k_bin_scatter(const uint8_t *__restrict__ bin, int64_t n, int nbins, int64_t nblocks,
+              const int64_t *__restrict__ counts, int32_t *__restrict__ rows) {
+    __shared__ int s_wc[BIN_THREADS / WAVE][MAX_BINS];
+    __shared__ int64_t s_base[MAX_BINS];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    if (threadIdx.x < nbins) s_base[threadIdx.x] = counts[threadIdx.x * nblocks + blockIdx.x];
+    __syncthreads();
+    for (int round = 0; round < BIN_ITEMS; round++) {
+        const int64_t idx = (int64_t)blockIdx.x * BIN_TILE + round * BIN_THREADS + threadIdx.x;
+        const int b = idx < n ? bin[idx] : 0xFF;
+        int rank = 0;
+        for (int bb = 0; bb < nbins; bb++) {
+            const uint64_t m = __ballot(b == bb);
+            if (b == bb) rank = mbcnt(m);
+            if (lane == 0) s_wc[w][bb] = __popcll(m);
+        }
+        __syncthreads();
+        if (b < nbins) {
+            int64_t off = s_base[b];
+            for (int q = 0; q < w; q++) off += s_wc[q][b];
+            rows[off + rank] = (int32_t)idx;
+        }
+        __syncthreads();
+        if (threadIdx.x < nbins) {
+            int t = 0;
+            for (int q = 0; q < BIN_THREADS / WAVE; q++) t += s_wc[q][threadIdx.x];
+            s_base[threadIdx.x] += t;
+        }
+        __syncthreads();
+    }
+}
+
+// hdr[0] = numtileC, hdr[1 + b] = first row-list slot of bin b (b <= nbins).
+__global__ void k_bin_header(const int64_t *__restrict__ counts, int nbins, int64_t nblocks,
+                             const int64_t *__restrict__ tile_ptrC, int32_t tilem, int64_t *__restrict__ hdr) {
+    const int t = threadIdx.x;
+    if (t == 0) hdr[0] = tile_ptrC[tilem];
+    if (t <= nbins) hdr[1 + t] = counts[(int64_t)t * nblocks];
+}
+
+// ---------------------------------------------------------------- wave helpers
+// Ascending bitonic sort of one 64-bit key per lane across the wave.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t key) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int k = 2; k <= WAVE; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t other = __shfl_xor(key, j, WAVE);
+            const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+            const uint64_t mn = key < other ? key : other, mx = key < other ? other : key;
+            key = (lower == up) ? mn : mx;
+        }
+    }
+    return key;
+}
+
+// 16-byte loads of a tile's 16 row pointers (uint8) and 16 row masks (uint16).
+__device__ __forceinline__ uint4 load_ptr16(const uint8_t *p, int64_t tile) {
+    return reinterpret_cast<const uint4 *>(p)[tile];
+}
+__device__ __forceinline__ void load_mask16(const uint16_t *m, int64_t tile, uint4 &lo, uint4 &hi) {
+    lo = reinterpret_cast<const uint4 *>(m)[tile * 2];
+    hi = reinterpret_cast<const uint4 *>(m)[tile * 2 + 1];
+}
+__device__ __forceinline__ void store_mask16(uint16_t *dst, const uint4 &lo, const uint4 &hi) {
+    reinterpret_cast<uint4 *>(dst)[0] = lo;
+    reinterpret_cast<uint4 *>(dst)[1] = hi;
+}
+
+// ---------------------------------------------------------------- staged symbolic (fill + masks)
+template <typename Var>
+struct SymLDS {
+    uint32_t cmask[Var::NC * 16];
+    alignas(16) uint16_t amask[Var::NA * 16];  // 16-byte stores
+    alignas(16) uint16_t bmask[Var::NB * 16];
+    int32_t abs_[Var::NA];
+    int32_t aseg[Var::NA + 1];
+    uint8_t bown[Var::NB];
+    uint8_t bct[Var::NB];
+};
+
+// One wave per listed tile-row: C tile columns (ascending, like
+// step1_fill_kernel), row masks (step2_symbolic_kernel), per-tile exclusive
+// row offsets (step2_build_ptrRowC_kernel) and tile counts.  The distinct
+// tile columns come from one wave-wide bitonic sort of the B tiles' columns.
+template <typename Var>
+__global__ void __launch_bounds__(Var::WAVES *WAVE)
+k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
+             const int64_t *__restrict__ tile_ptrA, const uint16_t *__restrict__ maskA,
+             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB,
+             const uint16_t *__restrict__ csr_maskB, const int64_t *__restrict__ tile_ptrC,
+             int32_t *__restrict__ rowidxC, int32_t *__restrict__ colidxC, uint16_t *__restrict__ maskC,
+             uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC, int32_t *__restrict__ overflow) {
+    static_assert(Var::NC <= 256 && Var::NA <= 32 && Var::NB <= WAVE, "caps exceed the kernel's index widths");
+    __shared__ SymLDS<Var> lds[Var::WAVES];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int64_t idx = (int64_t)blockIdx.x * Var::WAVES + w;
+    if (idx >= nrows) return;
+    SymLDS<Var> &s = lds[w];
+    const int32_t bi = rows[row_begin + idx];
+    const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
+    const int64_t cs = tile_ptrC[bi], ce = tile_ptrC[bi + 1];
+    const int na = (int)(ae - as), nc = (int)(ce - cs);
+    if (na > Var::NA || nc > Var::NC) {
+        if (lane == 0) atomicOr(overflow, 1);
+        return;
+    }
+    // level 2: A links and masks
+    int4 lk = make_int4(0, 0, 0, 0);
+    uint4 m0, m1;
+    if (lane < na) {
+        lk = link[as + lane];
+        load_mask16(maskA, as + lane, m0, m1);
+    }
+    for (int i = lane; i < nc * 16; i += WAVE) s.cmask[i] = 0;
+    const int32_t incl = wave_incl_sum(lk.y);
+    const int nB = __shfl(incl, WAVE - 1, WAVE);
+    if (nB > Var::NB) {
+        if (lane == 0) atomicOr(overflow, 1);
+        return;
+    }
+    if (lane < na) {
+        s.aseg[lane + 1] = incl;
+        s.abs_[lane] = lk.x;
+        store_mask16(&s.amask[lane * 16], m0, m1);
+    }
+    if (lane == 0) s.aseg[0] = 0;
+    wave_sync();
+    // level 3: B tile columns and masks, one lane per B tile
+    int32_t j = INT_MAX;
+    if (lane < nB) {
+        const int t = owner_of(s.aseg, na, lane);
+        const int64_t q = (int64_t)s.abs_[t] + (lane - s.aseg[t]);
+        j = colidxB[q];
+        load_mask16(csr_maskB, q, m0, m1);
+        s.bown[lane] = (uint8_t)t;
+        store_mask16(&s.bmask[lane * 16], m0, m1);
+    }
+    // distinct tile columns and their ranks
+    const uint64_t key = wave_sort64(lane < nB ? ((uint64_t)(uint32_t)j << 8) | (uint64_t)lane : ~0ull);
+    const int32_t sj = (int32_t)(key >> 8);
+    const int sb = (int)(key & 0xFF);
+    const bool valid = lane < nB;
+    const int32_t prev = __shfl_up(sj, 1, WAVE);
+    const bool first = valid && (lane == 0 || prev != sj);
+    const uint64_t fb = __ballot(first);
+    const int rank = mbcnt(fb) + (first ? 1 : 0) - 1;
+    if (first) {
+        colidxC[cs + rank] = sj;
+        rowidxC[cs + rank] = bi;
+    }
+    if (valid) s.bct[sb] = (uint8_t)rank;
+    wave_sync();
+    // masks: item (B tile b, row r) ORs B's rows selected by A's row mask
+    for (int it = lane; it < nB * 16; it += WAVE) {
+        const int b = it >> 4, r = it & 15;
+        uint32_t ma = s.amask[s.bown[b] * 16 + r];
+        uint32_t rm = 0;
+        while (ma) {
+            const int kk = __clz(ma) - 16;
+            ma ^= 0x8000u >> kk;
+            rm |= s.bmask[b * 16 + kk];
+        }
+        if (rm) atomicOr(&s.cmask[s.bct[b] * 16 + r], rm);
+    }
+    wave_sync();
+    // write: groups of 16 lanes = one C tile
+    for (int it = lane; it < nc * 16; it += WAVE) {
+        const int ct = it >> 4, r = it & 15;
+        const uint32_t rm = s.cmask[it];
+        maskC[(cs + ct) * 16 + r] = (uint16_t)rm;
+        const int pc = __popc(rm);
+        const int inc = group16_incl_sum(pc);
+        ptrRowC[(cs + ct) * 16 + r] = (uint8_t)(inc - pc);
+        if (r == 15) tile_nnzC[cs + ct + 1] = inc;
+    }
+}
+
+// ---------------------------------------------------------------- staged numeric
+template <typename V, typename Var>
+struct NumLDS {
+    V aval[Var::PA];
+    V bval[Var::PB];
+    int64_t coff[Var::NC + 1];
+    int64_t apay[Var::NA];
+    int32_t cj[Var::NC];
+    int32_t abs_[Var::NA];
+    int32_t aoff[Var::NA + 1];
+    int32_t aseg[Var::NA + 1];
+    int32_t apseg[Var::NA + 1];
+    int32_t boff[Var::NB];
+    uint32_t tmask[Var::NC];
+    int16_t pstart[Var::NC + 1];
+    uint16_t desc[DESC_CAP];
+    alignas(16) uint16_t cmask[Var::NC * 16];  // 16-byte stores
+    alignas(16) uint16_t amask[Var::NA * 16];
+    alignas(16) uint16_t bmask[Var::NB * 16];
+    alignas(16) uint8_t aptr[Var::NA * 16];
+    alignas(16) uint8_t bptr[Var::NB * 16];
+    uint8_t bown[Var::NB];
+    uint8_t bct[Var::NB];
+    uint8_t plist[Var::NB];
+};
+
+template <typename V>
+__device__ __forceinline__ V fma_v(V a, V b, V c);
+template <>
+__device__ __forceinline__ double fma_v<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+template <>
+__device__ __forceinline__ float fma_v<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+template <typename V, typename Var>
+__global__ void __launch_bounds__(Var::WAVES *WAVE)
+k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
+             const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA,
+             const uint8_t *__restrict__ ptrA, const uint16_t *__restrict__ maskA, const V *__restrict__ valA,
+             const int4 *__restrict__ link, const int64_t *__restrict__ lpay, const int32_t *__restrict__ colidxB,
+             const int64_t *__restrict__ csr_nnzB, const uint8_t *__restrict__ csr_ptrB,
+             const uint16_t *__restrict__ csr_maskB, const V *__restrict__ csr_valB,
+             const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ colidxC,
+             const int64_t *__restrict__ tile_nnzC, const uint16_t *__restrict__ maskC, V *__restrict__ valC,
+             uint8_t *__restrict__ colC, int32_t *__restrict__ overflow) {
+    static_assert(Var::NC < WAVE && Var::NA <= 32 && Var::NB <= WAVE, "caps exceed the kernel's index widths");
+    constexpr int ITA = (Var::PA + WAVE - 1) / WAVE, ITB = (Var::PB + WAVE - 1) / WAVE;
+    __shared__ NumLDS<V, Var> lds[Var::WAVES];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int64_t idx = (int64_t)blockIdx.x * Var::WAVES + w;
+    if (idx >= nrows) return;
+    NumLDS<V, Var> &s = lds[w];
+    const int32_t bi = rows[row_begin + idx];
+    // level 1: tile-row pointers
+    const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
+    const int64_t cs = tile_ptrC[bi], ce = tile_ptrC[bi + 1];
+    const int na = (int)(ae - as), nc = (int)(ce - cs);
+    if (na > Var::NA || nc > Var::NC) {
+        if (lane == 0) atomicOr(overflow, 1);
+        return;
+    }
+    // level 2: A headers and links, C headers (all loads issued, then stored)
+    int64_t anz = 0, lp = 0, co = 0;
+    int4 lk = make_int4(0, 0, 0, 0);
+    uint4 ap, am0, am1, cm0, cm1;
+    int32_t cjv = 0;
+    if (lane <= na) anz = tile_nnzA[as + lane];
+    if (lane < na) {
+        lk = link[as + lane];
+        lp = lpay[as + lane];
+        ap = load_ptr16(ptrA, as + lane);
+        load_mask16(maskA, as + lane, am0, am1);
+    }
+    if (lane <= nc) co = tile_nnzC[cs + lane];
+    if (lane < nc) {
+        cjv = colidxC[cs + lane];
+        load_mask16(maskC, cs + lane, cm0, cm1);
+    }
+    const int64_t abase = __shfl(anz, 0, WAVE);
+    const int32_t incl = wave_incl_sum(lk.y), pincl = wave_incl_sum(lk.z);
+    const int nB = __shfl(incl, WAVE - 1, WAVE), pB = __shfl(pincl, WAVE - 1, WAVE);
+    const int pA = (int)(__shfl(anz, na, WAVE) - abase);
+    if (lane <= na) s.aoff[lane] = (int32_t)(anz - abase);
+    if (lane < na) {
+        s.aseg[lane + 1] = incl;
+        s.apseg[lane + 1] = pincl;
+        s.abs_[lane] = lk.x;
+        s.apay[lane] = lp;
+        reinterpret_cast<uint4 *>(s.aptr)[lane] = ap;
+        store_mask16(&s.amask[lane * 16], am0, am1);
+    }
+    if (lane == 0) {
+        s.aseg[0] = 0;
+        s.apseg[0] = 0;
+    }
+    if (lane <= nc) s.coff[lane] = co;
+    if (lane < nc) {
+        s.cj[lane] = cjv;
+        s.tmask[lane] = 0;
+        store_mask16(&s.cmask[lane * 16], cm0, cm1);
+    }
+    if (nB > Var::NB || pB > Var::PB || pA > Var::PA) {
+        if (lane == 0) atomicOr(overflow, 1);
+        return;
+    }
+    wave_sync();
+    // level 3: A values, B headers (one lane per B tile) and B values
+    V av[ITA], bv[ITB];
+#pragma unroll
+    for (int it = 0; it < ITA; it++) {
+        const int i = lane + it * WAVE;
+        if (i < pA) av[it] = valA[abase + i];
+    }
+    int64_t bnz = 0;
+    int32_t bjv = 0;
+    uint4 bp, bm0, bm1;
+    int bt = 0;
+    if (lane < nB) {
+        bt = owner_of(s.aseg, na, lane);
+        const int64_t q = (int64_t)s.abs_[bt] + (lane - s.aseg[bt]);
+        bjv = colidxB[q];
+        bnz = csr_nnzB[q];
+        bp = load_ptr16(csr_ptrB, q);
+        load_mask16(csr_maskB, q, bm0, bm1);
+    }
+#pragma unroll
+    for (int it = 0; it < ITB; it++) {
+        const int i = lane + it * WAVE;
+        if (i < pB) {
+            const int t = owner_of(s.apseg, na, i);
+            bv[it] = csr_valB[s.apay[t] + (i - s.apseg[t])];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < ITA; it++) {
+        const int i = lane + it * WAVE;
+        if (i < pA) s.aval[i] = av[it];
+    }
+#pragma unroll
+    for (int it = 0; it < ITB; it++) {
+        const int i = lane + it * WAVE;
+        if (i < pB) s.bval[i] = bv[it];
+    }
+    int ct_b = 0;
+    if (lane < nB) {
+        s.boff[lane] = (int32_t)(bnz - s.apay[bt]) + s.apseg[bt];
+        s.bown[lane] = (uint8_t)bt;
+        reinterpret_cast<uint4 *>(s.bptr)[lane] = bp;
+        store_mask16(&s.bmask[lane * 16], bm0, bm1);
+        int lo = 0, hi = nc;  // C tile of this B tile
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (s.cj[mid] < bjv) lo = mid + 1; else hi = mid;
+        }
+        ct_b = lo;
+        s.bct[lane] = (uint8_t)lo;
+        atomicOr(&s.tmask[lo], 1u << bt);
+    }
+    wave_sync();
+    // pair table: the B tiles of C tile ct in ascending k = set bits of tmask[ct]
+    const int pcnt = lane < nc ? __popc(s.tmask[lane]) : 0;
+    const int pinc = wave_incl_sum(pcnt);
+    if (lane < nc) s.pstart[lane + 1] = (int16_t)pinc;
+    if (lane == 0) s.pstart[0] = 0;
+    wave_sync();
+    if (lane < nB) s.plist[s.pstart[ct_b] + __popc(s.tmask[ct_b] & ((1u << bt) - 1u))] = (uint8_t)lane;
+    wave_sync();
+    // outputs in chunks of (C tile, row) items: lane = item while the chunk's
+    // output descriptors are written, then lane = output.  A chunk is 4 C
+    // tiles, or 2 when 4 would hold more than DESC_CAP outputs.
+    const int64_t gbase = s.coff[0];
+    for (int ct0 = 0; ct0 < nc;) {
+        const int span = (s.coff[min(ct0 + 4, nc)] - s.coff[ct0] > DESC_CAP) ? 2 : 4;
+        const int item = ct0 * 16 + lane;
+        const uint32_t mc = (lane < span * 16 && item < nc * 16) ? s.cmask[item] : 0u;
+        const int pc = __popc(mc);
+        const int oincl = wave_incl_sum(pc);
+        const int nout = __shfl(oincl, WAVE - 1, WAVE);
+        {
+            int o = oincl - pc;
+            uint32_t m = mc;
+            while (m) {
+                const int c = __clz(m) - 16;
+                m ^= 0x8000u >> c;
+                s.desc[o++] = (uint16_t)((lane << 4) | c);  // (tile in chunk, row) = lane
+            }
+        }
+        wave_sync();
+        const int64_t obase = s.coff[ct0] - gbase;
+        for (int o = lane; o < nout; o += WAVE) {
+            const int d = s.desc[o];
+            const int ct = ct0 + (d >> 8), r = (d >> 4) & 15, c = d & 15;
+            const uint32_t cbit = 0x8000u >> c;
+            V acc = V(0);
+            for (int p = s.pstart[ct]; p < s.pstart[ct + 1]; p++) {
+                const int b = s.plist[p], t = s.bown[b];
+                const uint32_t ma = s.amask[t * 16 + r];
+                uint32_t m = ma;
+                const int ea = s.aoff[t] + s.aptr[t * 16 + r];
+                while (m) {
+                    const int kk = __clz(m) - 16;
+                    m ^= 0x8000u >> kk;
+                    const uint32_t mb = s.bmask[b * 16 + kk];
+                    if (mb & cbit) {
+                        const V va = s.aval[ea + __popc(ma & below(kk))];
+                        const V vb = s.bval[s.boff[b] + s.bptr[b * 16 + kk] + __popc(mb & below(c))];
+                        acc = fma_v<V>(va, vb, acc);
+                    }
+                }
+            }
+            const int64_t go = gbase + obase + o;
+            valC[go] = acc;
+            colC[go] = (uint8_t)((r << 4) | c);
+        }
+        wave_sync();
+        ct0 += span;
+    }
+}
+
+// ---------------------------------------------------------------- host side
+struct Plan {
+    int32_t *rows = nullptr;
+    uint8_t *bin = nullptr;
+    int64_t *counts = nullptr;
+    int64_t *hdr = nullptr;
+    int32_t *overflow = nullptr;
+    int4 *link = nullptr;
+    int64_t *lpay = nullptr;
+    int64_t bin_off[MAX_BINS + 1] = {};
+    int nbins = 0;
+};
+
+static bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
+    if (opts && opts->numeric_variant == 1) return false;  // reference method everywhere
+    return B->csr_tile_nnz && B->csr_tile_Ptr && B->csr_mask && B->csr_tile_Val && B->row_nnz_ptr &&
+           !(A->flags & 1) && !(B->flags & 1);
+}
+
+static BinTable bin_table(bool ok) {
+    BinTable t{};
+    t.nvar = 0;
+    if (ok) {
+        t.v[0] = caps_of<VarS>();
+        t.v[1] = caps_of<VarW>();
+        t.v[2] = caps_of<VarM>();
+        t.nvar = NUM_STAGED;
+    }
+    return t;
+}
+
+static void free_plan(Context *ctx, Plan &p) {
+    void *ps[] = {p.rows, p.bin, p.counts, p.hdr, p.overflow, p.link, p.lpay};
+    for (void *q : ps) pool_free(ctx, q);
+    p = Plan{};
+}
+
+static unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+// A->B links, Step 1 count, statistics, exclusive scan of tile_ptrC, binning
+// and one host read of (numtileC, bin offsets).
+int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+              const tsg_options *opts, Plan &p, int32_t *stats_out) {
+    const bool ok = staged_ok(A, B, opts);
+    const BinTable bt = bin_table(ok);
+    p.nbins = bt.nvar + 1;
+    const int64_t tilem = A->tilem;
+    const int64_t nb = (tilem + BIN_TILE - 1) / BIN_TILE;
+    TSG_CHECK(palloc(ctx, &p.rows, (size_t)tilem));
+    TSG_CHECK(palloc(ctx, &p.bin, (size_t)tilem));
+    TSG_CHECK(palloc(ctx, &p.counts, (size_t)(p.nbins * nb + 1)));
+    TSG_CHECK(palloc(ctx, &p.hdr, MAX_BINS + 2));
+    TSG_CHECK(palloc(ctx, &p.overflow, 1));
+    TSG_CHECK(palloc(ctx, &p.link, (size_t)A->numtile));
+    TSG_CHECK(palloc(ctx, &p.lpay, (size_t)A->numtile));
+    hipStream_t s = ctx->stream;
+    TSG_HIP(hipMemsetAsync(p.overflow, 0, sizeof(int32_t), s));
+    if (tilem == 0) {
+        TSG_HIP(hipMemsetAsync(C->tile_ptr, 0, sizeof(int64_t), s));
+        C->numtile = 0;
+        return TSG_OK;
+    }
+    if (A->numtile > 0) {
+        k_ab_link<<<nblk(A->numtile, 256), 256, 0, s>>>(A->numtile, A->tile_columnidx, B->tile_ptr,
+                                                         ok ? B->row_nnz_ptr : nullptr, p.link, p.lpay);
+        TSG_HIP(hipGetLastError());
+    }
+    k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, s>>>(A->tile_ptr, A->tile_nnz, A->tilem, p.link,
+                                                                   B->tile_columnidx, bt, C->tile_ptr, p.bin,
+                                                                   stats_out);
+    TSG_HIP(hipGetLastError());
+    TSG_CHECK(scan_inclusive_i64(ctx, C->tile_ptr, tilem + 1));
+    k_bin_count<<<(unsigned)nb, BIN_THREADS, 0, s>>>(p.bin, tilem, p.nbins, nb, p.counts);
+    TSG_HIP(hipGetLastError());
+    TSG_CHECK(scan_inclusive_i64(ctx, p.counts, p.nbins * nb + 1));
+    k_bin_scatter<<<(unsigned)nb, BIN_THREADS, 0, s>>>(p.bin, tilem, p.nbins, nb, p.counts, p.rows);
+    k_bin_header<<<1, 64, 0, s>>>(p.counts, p.nbins, nb, C->tile_ptr, A->tilem, p.hdr);
+    TSG_HIP(hipGetLastError());
+    TSG_HIP(hipMemcpyAsync(ctx->pinned, p.hdr, sizeof(int64_t) * (p.nbins + 2), hipMemcpyDeviceToHost, s));
+    TSG_HIP(hipStreamSynchronize(s));
+    C->numtile = ctx->pinned[0];
+    for (int b = 0; b <= p.nbins; b++) p.bin_off[b] = ctx->pinned[1 + b];
+    return TSG_OK;
+}
+
+template <typename Var>
+static int launch_sym_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                             const tsg_tile_matrix *C, const Plan &p, int bin) {
+    const int64_t n = p.bin_off[bin + 1] - p.bin_off[bin];
+    if (n <= 0) return TSG_OK;
+    k_sym_staged<Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
+        p.rows, p.bin_off[bin], n, A->tile_ptr, A->mask, p.link, B->tile_columnidx, B->csr_mask, C->tile_ptr,
+        C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, p.overflow);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
+template <typename V, typename Var>
+static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                             const tsg_tile_matrix *C, const Plan &p, int bin) {
+    const int64_t n = p.bin_off[bin + 1] - p.bin_off[bin];
+    if (n <= 0) return TSG_OK;
+    k_num_staged<V, Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
+        p.rows, p.bin_off[bin], n, A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
+        p.link, p.lpay, B->tile_columnidx, B->csr_tile_nnz, B->csr_tile_Ptr, B->csr_mask,
+        (const V *)B->csr_tile_Val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, (V *)C->tile_csr_Val,
+        C->tile_csr_Col, p.overflow);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
+// Structure fill + symbolic for every bin.
+int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                 const Plan &p) {
+    const int fb = p.nbins - 1;  // last bin: C-tile kernels
+    if (p.nbins > 1) {
+        TSG_CHECK(launch_sym_staged<VarS>(ctx, A, B, C, p, 0));
+        TSG_CHECK(launch_sym_staged<VarW>(ctx, A, B, C, p, 1));
+        TSG_CHECK(launch_sym_staged<VarM>(ctx, A, B, C, p, 2));
+    }
+    const int64_t nf = p.bin_off[fb + 1] - p.bin_off[fb];
+    TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb], nf));
+    TSG_CHECK(launch_symbolic_list(ctx, A, B, p.rows, p.bin_off[fb], nf, C));
+    return TSG_OK;
+}
+
+int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                const Plan &p) {
+    const int fb = p.nbins - 1;
+    if (p.nbins > 1) {
+        if (A->dtype == TSG_F32) {
+            TSG_CHECK((launch_num_staged<float, VarS>(ctx, A, B, C, p, 0)));
+            TSG_CHECK((launch_num_staged<float, VarW>(ctx, A, B, C, p, 1)));
+            TSG_CHECK((launch_num_staged<float, VarM>(ctx, A, B, C, p, 2)));
+        } else {
+            TSG_CHECK((launch_num_staged<double, VarS>(ctx, A, B, C, p, 0)));
+            TSG_CHECK((launch_num_staged<double, VarW>(ctx, A, B, C, p, 1)));
+            TSG_CHECK((launch_num_staged<double, VarM>(ctx, A, B, C, p, 2)));
+        }
+    }
+    TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, p.bin_off[fb], p.bin_off[fb + 1] - p.bin_off[fb]));
+    return TSG_OK;
+}
+
+static void free_c(Context *ctx, tsg_tile_matrix *C) {
+    void *ps[] = {C->tile_ptr, C->tile_columnidx, C->tile_rowidx, C->tile_nnz, C->tile_csr_Val, C->tile_csr_Col,
+                  C->tile_csr_Ptr, C->mask};
+    for (void *q : ps) pool_free(ctx, q);
+    memset(C, 0, sizeof *C);
+}
+
+// The exact binned pipeline: the reference's three steps (exact counts and
+// host reads of numtileC and nnzC, src/spgemm_raw_v2.cu:141, :211), with rows
+// binned to the staged kernels or the C-tile kernels.  The staged kernels'
+// capacity flag is read with nnzC's sync.
+int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                  const tsg_options *opts, tsg_stats *st) {
+    const bool timing = opts && opts->timing;
+    hipEvent_t ev[4] = {};
+    if (timing)
+        for (auto &e : ev) TSG_HIP(hipEventCreate(&e));
+    memset(C, 0, sizeof *C);
+    C->m = A->m; C->n = B->n; C->tilem = A->tilem; C->tilen = B->tilen;
+    C->dtype = A->dtype; C->location = TSG_DEVICE;
+    Plan p;
+    hipStream_t s = ctx->stream;
+    int rc = TSG_OK;
+    int syncs = 0;
+    if (timing) (void)hipEventRecord(ev[0], s);
+    rc = palloc(ctx, &C->tile_ptr, (size_t)C->tilem + 1);
+    if (!rc) { rc = plan_rows(ctx, A, B, C, opts, p, nullptr); syncs++; }
+    if (!rc && C->numtile > INT32_MAX) rc = fail(TSG_ERR_OVERFLOW, "numtileC exceeds int32 tile ids");
+    if (!rc) rc = palloc(ctx, &C->tile_rowidx, (size_t)C->numtile);
+    if (!rc) rc = palloc(ctx, &C->tile_columnidx, (size_t)C->numtile);
+    if (!rc) rc = palloc(ctx, &C->mask, (size_t)C->numtile * 16);
+    if (!rc) rc = palloc(ctx, &C->tile_csr_Ptr, (size_t)C->numtile * 16);
+    if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)C->numtile + 1);
+    if (timing && !rc) (void)hipEventRecord(ev[1], s);
+    if (!rc && hipMemsetAsync(C->tile_nnz, 0, sizeof(int64_t), s) != hipSuccess) rc = fail(TSG_ERR_HIP, "memset");
+    if (!rc) rc = run_symbolic(ctx, A, B, C, p);
+    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1);
+    if (!rc) {
+        // nnzC and the capacity flag in one read
+        if (hipMemcpyAsync(ctx->pinned, C->tile_nnz + C->numtile, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipMemcpyAsync(ctx->pinned + 1, p.overflow, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = fail(TSG_ERR_HIP, "reading nnzC failed");
+        syncs++;
+        C->nnz = ctx->pinned[0];
+        if (!rc && (int32_t)ctx->pinned[1]) rc = fail(TSG_ERR_OVERFLOW, "a staged row exceeded its variant's capacity");
+    }
+    if (timing && !rc) (void)hipEventRecord(ev[2], s);
+    if (!rc) rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)C->nnz * vsize(C->dtype));
+    if (!rc) rc = palloc(ctx, &C->tile_csr_Col, (size_t)C->nnz);
+    if (!rc && C->nnz > 0) rc = run_numeric(ctx, A, B, C, p);
+    if (timing && !rc) (void)hipEventRecord(ev[3], s);
+    if (st) {
+        st->numtileC = C->numtile;
+        st->nnzC = C->nnz;
+        st->host_syncs = syncs;
+    }
+    if (timing) {
+        if (!rc && st) {
+            float a = 0, b = 0, c3 = 0;
+            (void)hipEventSynchronize(ev[3]);
+            (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+            (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+            (void)hipEventElapsedTime(&c3, ev[2], ev[3]);
+            st->ms_step1 = a; st->ms_step2 = b; st->ms_step3 = c3; st->ms_total = a + b + c3;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    free_plan(ctx, p);
+    if (rc) free_c(ctx, C);
+    return rc;
+}
+
+}  // namespace tsg

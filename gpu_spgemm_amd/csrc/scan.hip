@@ -104,4 +104,11 @@ int read_i64(Context *ctx, const int64_t *dptr, int64_t *out) {
     return TSG_OK;
 }
 
+int read_i32(Context *ctx, const int32_t *dptr, int32_t *out) {
+    TSG_HIP(hipMemcpyAsync(ctx->pinned, dptr, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ((int32_t *)ctx->pinned)[0];
+    return TSG_OK;
+}
+
 }  // namespace tsg

@@ -29,13 +29,15 @@ __global__ void __launch_bounds__(ST_WAVES *WAVE)
 k_structure(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ colidxA, int32_t tilemA,
             const int64_t *__restrict__ tile_ptrB, const int32_t *__restrict__ colidxB,
             int64_t *__restrict__ tile_ptrC, int32_t *__restrict__ rowidxC,
-            int32_t *__restrict__ colidxC) {
+            int32_t *__restrict__ colidxC, const int32_t *__restrict__ rows, int64_t row_begin,
+            int64_t nrows) {
     __shared__ uint32_t s_bm[ST_WAVES][ST_WIN_WORDS];
     __shared__ int64_t s_bs[ST_WAVES][WAVE];
     __shared__ int32_t s_pre[ST_WAVES][WAVE];
     const int w = threadIdx.x / WAVE, lane = lane_id();
-    const int32_t bi = blockIdx.x * ST_WAVES + w;
-    if (bi >= tilemA) return;  // whole wave leaves; no block barrier is used below
+    const int64_t idx = (int64_t)blockIdx.x * ST_WAVES + w;
+    if (idx >= (rows ? nrows : (int64_t)tilemA)) return;  // whole wave leaves; no block barrier below
+    const int32_t bi = rows ? rows[row_begin + idx] : (int32_t)idx;
     uint32_t *bm = s_bm[w];
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
 
@@ -125,7 +127,7 @@ int launch_structure_count(Context *ctx, const tsg_tile_matrix *A, const tsg_til
     unsigned grid = (unsigned)((A->tilem + ST_WAVES - 1) / ST_WAVES);
     k_structure<false><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx, tile_ptrC, nullptr,
-        nullptr);
+        nullptr, nullptr, 0, 0);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -136,7 +138,18 @@ int launch_structure_fill(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
     unsigned grid = (unsigned)((A->tilem + ST_WAVES - 1) / ST_WAVES);
     k_structure<true><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx,
-        const_cast<int64_t *>(tile_ptrC), rowidxC, colidxC);
+        const_cast<int64_t *>(tile_ptrC), rowidxC, colidxC, nullptr, 0, 0);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
+int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                               const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count) {
+    if (count <= 0) return TSG_OK;
+    unsigned grid = (unsigned)((count + ST_WAVES - 1) / ST_WAVES);
+    k_structure<true><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
+        A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx, C->tile_ptr, C->tile_rowidx,
+        C->tile_columnidx, rows, begin, count);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }

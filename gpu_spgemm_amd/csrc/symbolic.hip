@@ -22,25 +22,25 @@ namespace tsg {
 constexpr int SY_THREADS = 256;
 constexpr int SY_GROUPS = SY_THREADS / 16;
 
-__global__ void __launch_bounds__(SY_THREADS)
-k_symbolic(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ colidxA,
-           const uint16_t *__restrict__ maskA, const int64_t *__restrict__ csc_ptrB,
-           const int32_t *__restrict__ csc_rowB, const uint16_t *__restrict__ maskB,
-           int64_t numtileC, const int32_t *__restrict__ rowidxC, const int32_t *__restrict__ colidxC,
-           uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC) {
-    __shared__ uint16_t s_mb[SY_GROUPS][16];
-    const int g = threadIdx.x >> 4, l = threadIdx.x & 15;
-    const int64_t ti = (int64_t)blockIdx.x * SY_GROUPS + g;
-    if (ti >= numtileC) return;  // whole group leaves; only wave-level syncs below
+// Symbolic step of one C tile by a 16-lane group (lane = tile row); s_mb is
+// the group's 16-entry LDS slot for B's row masks.
+__device__ __forceinline__ void sym_tile(int64_t ti, const int64_t *__restrict__ tile_ptrA,
+                                         const int32_t *__restrict__ colidxA, const uint16_t *__restrict__ maskA,
+                                         const int64_t *__restrict__ csc_ptrB, const int32_t *__restrict__ csc_rowB,
+                                         const uint16_t *__restrict__ maskB, const int32_t *__restrict__ rowidxC,
+                                         const int32_t *__restrict__ colidxC, uint16_t *__restrict__ maskC,
+                                         uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC,
+                                         uint16_t *s_mb) {
+    const int l = lane_id() & 15;
     const int32_t bi = rowidxC[ti], bj = colidxC[ti];
     uint32_t rm = 0;
     for_each_pair(tile_ptrA, colidxA, csc_ptrB, csc_rowB, bi, bj, [&](int64_t a, int64_t b) {
-        s_mb[g][l] = maskB[b * 16 + l];
+        s_mb[l] = maskB[b * 16 + l];
         wave_sync();
         uint32_t ma = maskA[a * 16 + l];
         while (ma) {
             const int kk = __clz(ma) - 16;
-            rm |= s_mb[g][kk];
+            rm |= s_mb[kk];
             ma ^= 0x8000u >> kk;
         }
         wave_sync();
@@ -50,7 +50,49 @@ k_symbolic(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ co
     const int incl = group16_incl_sum(pc);
     ptrRowC[ti * 16 + l] = (uint8_t)(incl - pc);
     if (l == 15) tile_nnzC[ti + 1] = incl;
-    if (ti == 0 && l == 0) tile_nnzC[0] = 0;
+}
+
+__global__ void __launch_bounds__(SY_THREADS)
+k_symbolic(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ colidxA,
+           const uint16_t *__restrict__ maskA, const int64_t *__restrict__ csc_ptrB,
+           const int32_t *__restrict__ csc_rowB, const uint16_t *__restrict__ maskB,
+           int64_t numtileC, const int32_t *__restrict__ rowidxC, const int32_t *__restrict__ colidxC,
+           uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC) {
+    __shared__ uint16_t s_mb[SY_GROUPS][16];
+    const int g = threadIdx.x >> 4;
+    const int64_t ti = (int64_t)blockIdx.x * SY_GROUPS + g;
+    if (ti >= numtileC) return;  // whole group leaves; only wave-level syncs below
+    sym_tile(ti, tile_ptrA, colidxA, maskA, csc_ptrB, csc_rowB, maskB, rowidxC, colidxC, maskC, ptrRowC,
+             tile_nnzC, s_mb[g]);
+    if (ti == 0 && (threadIdx.x & 15) == 0) tile_nnzC[0] = 0;
+}
+
+// Heavy tile-rows: one workgroup per listed row, its 16 groups stride over the
+// row's C tiles.
+__global__ void __launch_bounds__(SY_THREADS)
+k_symbolic_list(const int32_t *__restrict__ rows, int64_t row_begin, const int64_t *__restrict__ tile_ptrC,
+                const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ colidxA,
+                const uint16_t *__restrict__ maskA, const int64_t *__restrict__ csc_ptrB,
+                const int32_t *__restrict__ csc_rowB, const uint16_t *__restrict__ maskB,
+                const int32_t *__restrict__ rowidxC, const int32_t *__restrict__ colidxC,
+                uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC) {
+    __shared__ uint16_t s_mb[SY_GROUPS][16];
+    const int g = threadIdx.x >> 4;
+    const int32_t bi = rows[row_begin + blockIdx.x];
+    const int64_t ce = tile_ptrC[bi + 1];
+    for (int64_t ti = tile_ptrC[bi] + g; ti < ce; ti += SY_GROUPS)
+        sym_tile(ti, tile_ptrA, colidxA, maskA, csc_ptrB, csc_rowB, maskB, rowidxC, colidxC, maskC, ptrRowC,
+                 tile_nnzC, s_mb[g]);
+}
+
+int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const int32_t *rows,
+                         int64_t begin, int64_t count, const tsg_tile_matrix *C) {
+    if (count <= 0) return TSG_OK;
+    k_symbolic_list<<<(unsigned)count, SY_THREADS, 0, ctx->stream>>>(
+        rows, begin, C->tile_ptr, A->tile_ptr, A->tile_columnidx, A->mask, B->csc_tile_ptr, B->csc_tile_rowidx,
+        B->mask, C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
 }
 
 int launch_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, int64_t numtileC,

@@ -80,6 +80,7 @@ inline size_t vsize(int dtype) { return dtype == TSG_F32 ? 4 : 8; }
 int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n);
 // Reads one int64 from device memory (blocking; counts as a host sync).
 int read_i64(Context *ctx, const int64_t *dptr, int64_t *out);
+int read_i32(Context *ctx, const int32_t *dptr, int32_t *out);
 
 // ---------------------------------------------------------------- kernels entry (host side)
 int launch_structure_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
@@ -89,6 +90,12 @@ int launch_structure_fill(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
 int launch_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                     int64_t numtileC, const int32_t *rowidxC, const int32_t *colidxC,
                     uint16_t *maskC, uint8_t *ptrRowC, int64_t *tile_nnzC);
+int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                               const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count);
+int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const int32_t *rows,
+                         int64_t begin, int64_t count, const tsg_tile_matrix *C);
+int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                        const int32_t *rows, int64_t begin, int64_t count);
 void learned_release(Context *ctx);  // learned.hip
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                    tsg_tile_matrix *C, const tsg_options *opts);

@@ -71,7 +71,7 @@ typedef struct tsg_tile_matrix {
     int32_t dtype;            /* tsg_dtype */
     int32_t location;         /* tsg_location */
     int32_t is_csc;           /* 1 for a B operand laid out by tsg_build_csc_tiles */
-    int32_t reserved;
+    int32_t flags;            /* bit 0: some row holds a column twice (duplicates kept) */
     int64_t *tile_ptr;        /* [tilem+1] */
     int32_t *tile_columnidx;  /* [numtile] */
     int32_t *tile_rowidx;     /* [numtile] row of each tile (C output), may be NULL for A/B */
@@ -82,6 +82,14 @@ typedef struct tsg_tile_matrix {
     uint16_t *mask;           /* [nt*16] bit (15-c) = local column c present */
     int64_t *csc_tile_ptr;    /* [tilen+1] (B only) */
     int32_t *csc_tile_rowidx; /* [numtile] (B only) */
+    /* B only, optional: the same tiles in CSR-tile order, kept by
+     * tsg_build_csc_tiles so that a B tile-row is one contiguous range for the
+     * row-staged kernels.  NULL when absent (the C-tile kernels are used). */
+    int64_t *csr_tile_nnz;    /* [numtile+1] */
+    uint8_t *csr_tile_Ptr;    /* [numtile*16] */
+    uint16_t *csr_mask;       /* [numtile*16] */
+    void *csr_tile_Val;       /* [nnz] */
+    int64_t *row_nnz_ptr;     /* [tilem+1] first payload entry of each tile-row (= CSR rowptr[16*i]) */
 } tsg_tile_matrix;
 
 typedef struct tsg_context tsg_context;
@@ -94,7 +102,8 @@ typedef enum tsg_scheduler {
 
 typedef struct tsg_options {
     int32_t scheduler;      /* tsg_scheduler */
-    int32_t numeric_variant;/* 0 = auto (binned), 1 = row lanes, 2 = staged dense B, 3 = MFMA */
+    int32_t numeric_variant;/* 0 = auto: tile-rows binned to the row-staged kernels, heavy rows to the
+                               C-tile kernels; 1 = C-tile kernels for every tile (the reference's method) */
     int32_t mfma_min_products; /* tile pairs with >= this many products may use MFMA (0 = default) */
     int32_t timing;         /* 1 = record per-step hipEvent times in tsg_stats */
     double  pred_scale;     /* learned: multiply predicted sizes (1.0 normal; 0 forces fallback) */

@@ -75,6 +75,16 @@ def test_gpu_conversion_empty(ctx):
     assert b["numtile"] == 0 and np.all(b["csc_tile_ptr"] == 0)
 
 
+def test_gpu_conversion_rejects_bad_columns(ctx):
+    A = g.HostCSR(2, 10, [0, 2, 3], [3, 12, 7], [1.0, 2.0, 3.0])
+    with pytest.raises(g.TsgError) as e:
+        ctx.tiles_from_csr(A)
+    assert e.value.code == -1
+    A = g.HostCSR(2, 10, [0, 2, 3], [-1, 2, 7], [1.0, 2.0, 3.0])
+    with pytest.raises(g.TsgError):
+        ctx.tiles_from_csr(A)
+
+
 def test_gpu_conversion_rejects_unsorted_rows(ctx):
     A = g.HostCSR(2, 40, [0, 2, 3], [30, 5, 7], [1.0, 2.0, 3.0])
     with pytest.raises(g.TsgError) as e:

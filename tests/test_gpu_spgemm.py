@@ -18,10 +18,13 @@ from tests.conftest import assert_tiles_equal
 
 pytestmark = pytest.mark.gpu
 TOL = {g.F64: 1e-12, g.F32: 1e-5}
+# 0: tile-rows binned to the row-staged kernels (heavy rows to the C-tile
+# kernels); 1: the reference's C-tile method for every tile
+VARIANTS = [0, 1]
 
 
 def run(ctx, A, B, f32=False, label="", **kw):
-    print(f"[case] spgemm {label} A={A.m}x{A.n} nnz={A.nnz} B={B.m}x{B.n} f32={f32}", flush=True)
+    print(f"[case] spgemm {label} A={A.m}x{A.n} nnz={A.nnz} B={B.m}x{B.n} f32={f32} {kw}", flush=True)
     ha, hb = (A.astype(g.F32), B.astype(g.F32)) if f32 else (A, B)
     tA = ctx.tiles_from_csr(ha)
     tB = ctx.tiles_from_csr(hb, as_b=True)
@@ -44,16 +47,18 @@ def check(got, A, B, f32=False, exact_values=True):
     return chk
 
 
-def test_er4096_fp64_plain(ctx):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_er4096_fp64_plain(ctx, variant):
     """The CPU-path configuration: ER 4096, exactly 8 nnz/row, U[-1,1], seed 0."""
     A = g.gen_er(4096, 8, seed=0)
-    got, st = run(ctx, A, A, label="er4096")
+    got, st = run(ctx, A, A, label="er4096", variant=variant)
     check(got, A, A)
-    assert st["host_syncs"] == 2 and st["numtileC"] == got["numtile"] and st["nnzC"] == got["nnz"]
+    assert st["numtileC"] == got["numtile"] and st["nnzC"] == got["nnz"]
 
 
+@pytest.mark.parametrize("variant", VARIANTS)
 @pytest.mark.parametrize("f32", [False, True])
-def test_random_rectangular(ctx, f32):
+def test_random_rectangular(ctx, f32, variant):
     rng = np.random.default_rng(5)
     import scipy.sparse as sp
     for (m, k, n, d) in [(300, 500, 200, 0.02), (64, 64, 64, 0.3), (1000, 37, 900, 0.05)]:
@@ -64,50 +69,57 @@ def test_random_rectangular(ctx, f32):
         SA.sort_indices(); SB.sort_indices()
         A = g.HostCSR(m, k, SA.indptr, SA.indices, SA.data)
         B = g.HostCSR(k, n, SB.indptr, SB.indices, SB.data)
-        got, _ = run(ctx, A, B, f32=f32, label=f"rand{m}x{k}x{n}")
+        got, _ = run(ctx, A, B, f32=f32, label=f"rand{m}x{k}x{n}", variant=variant)
         check(got, A, B, f32=f32)
 
 
-def test_aat(ctx):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_aat(ctx, variant):
     """--aat path: B = A^T (src/spgemm_raw_v2.cu:78-89)."""
     A = g.gen_er_var(700, 1, 30, seed=3)
     B = A.transpose()
-    got, _ = run(ctx, A, B, label="aat")
+    got, _ = run(ctx, A, B, label="aat", variant=variant)
     check(got, A, B)
 
 
-def test_dense_tiles(ctx):
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("n", [48, 200])
+def test_dense_tiles(ctx, variant, n):
     import scipy.sparse as sp
-    D = sp.csr_matrix(np.random.default_rng(1).uniform(-1, 1, (48, 48)))
-    A = g.HostCSR(48, 48, D.indptr, D.indices, D.data)
-    got, _ = run(ctx, A, A, label="dense48")
+    D = sp.csr_matrix(np.random.default_rng(1).uniform(-1, 1, (n, n)))
+    A = g.HostCSR(n, n, D.indptr, D.indices, D.data)
+    got, _ = run(ctx, A, A, label=f"dense{n}", variant=variant)
     check(got, A, A)
 
 
-def test_laplace2d_integer_exact(ctx):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_laplace2d_integer_exact(ctx, variant):
     A = g.gen_laplace2d(200, 150)
-    got, _ = run(ctx, A, A, label="lap2d")
+    got, _ = run(ctx, A, A, label="lap2d", variant=variant)
     chk = check(got, A, A)
     assert chk["max_abs"] == 0.0
 
 
-def test_laplace2d_2048_full(ctx):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_laplace2d_2048_full(ctx, variant):
     """The bench workload at full size: 5-point Laplacian on a 2048 x 2048 grid."""
     A = g.gen_laplace2d(2048, 2048)
-    got, st = run(ctx, A, A, label="lap2d_2048")
+    got, st = run(ctx, A, A, label="lap2d_2048", variant=variant)
     assert got["numtile"] == 3386116 and got["nnz"] == 54484996
     chk = check(got, A, A)
     assert chk["max_abs"] == 0.0 and chk["rel_frob"] == 0.0
 
 
-def test_rmat_fp32(ctx):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_rmat_fp32(ctx, variant):
     A = g.gen_rmat(12, 16, seed=1, dtype=g.F32)
     A64 = A.astype(g.F64)
-    got, _ = run(ctx, A64, A64, f32=True, label="rmat12")
+    got, _ = run(ctx, A64, A64, f32=True, label="rmat12", variant=variant)
     check(got, A64, A64, f32=True)
 
 
-def test_galerkin_chain_on_device(ctx):
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_galerkin_chain_on_device(ctx, variant):
     """P^T (A P) with A a 27-point stencil: C1 = A P is re-laid as a B operand on
     the GPU (tsg_build_csc_tiles on a product) and fed to the second product."""
     nx = 12
@@ -116,12 +128,12 @@ def test_galerkin_chain_on_device(ctx):
     Pt = P.transpose()
     tA = ctx.tiles_from_csr(A)
     tP = ctx.tiles_from_csr(P, as_b=True)
-    C1, _ = ctx.spgemm(tA, tP)
+    C1, _ = ctx.spgemm(tA, tP, variant=variant)
     c1 = C1.download()
     check(c1, A, P)
     C1.build_csc_tiles()
     tPt = ctx.tiles_from_csr(Pt)
-    C2, _ = ctx.spgemm(tPt, C1)
+    C2, _ = ctx.spgemm(tPt, C1, variant=variant)
     c2 = C2.download()
     AP = O.gustavson_csr(A, P)
     APc = g.HostCSR(A.m, P.n, *AP)
@@ -150,6 +162,30 @@ def test_empty_and_tiny(ctx):
     one = g.HostCSR(1, 1, [0, 1], [0], [3.0])
     got, _ = run(ctx, one, one, label="1x1")
     assert got["nnz"] == 1 and got["tile_csr_Val"][0] == 9.0
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_duplicate_columns(ctx, variant):
+    """load_mtx keeps duplicates (src/tile_format.h:78-89): both entries live in
+    the tile, one mask bit; products use every entry."""
+    rp = np.array([0, 3, 5, 6, 8, 8, 8, 8, 9], dtype=np.int64)
+    col = np.array([1, 1, 7, 0, 3, 2, 2, 3, 6], dtype=np.int32)
+    A = g.HostCSR(8, 8, rp, col, np.arange(1.0, 10.0))
+    tA = ctx.tiles_from_csr(A)
+    assert tA.s.flags & 1
+    got, _ = run(ctx, A, A, label="dups", variant=variant)
+    exp = O.spgemm_tiled(A, A)
+    assert_tiles_equal(got, exp, keys=["tile_ptr", "tile_columnidx", "tile_nnz", "mask", "tile_csr_Col",
+                                       "tile_csr_Val"])
+
+
+def test_variants_agree_bitwise(ctx):
+    """The binned pipeline and the C-tile method give identical bits on a
+    matrix with light (staged) and heavy (C-tile) rows."""
+    A = g.gen_rmat(11, 16, seed=4, dtype=g.F64)
+    a, _ = run(ctx, A, A, label="rmat11 binned", variant=0)
+    b, _ = run(ctx, A, A, label="rmat11 ctile", variant=1)
+    assert_tiles_equal(a, b)
 
 
 def test_deterministic_repeat(ctx):
