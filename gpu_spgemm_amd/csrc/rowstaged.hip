@@ -37,9 +37,9 @@ struct Variant {
 };
 // S: 2-D 5-point-like rows; W: wide A tile-rows (27-point stencil times an
 // aggregation); M: medium rows of irregular matrices.
-using VarS = Variant<8, 16, 32, 96, 416, 4>;
-using VarW = Variant<32, 16, 32, 512, 512, 2>;
-using VarM = Variant<32, 48, 64, 512, 1024, 2>;
+using VarS = Variant<8, 16, 32, 128, 1 << 20, 4>;
+using VarW = Variant<32, 16, 32, 512, 1 << 20, 2>;
+using VarM = Variant<32, 48, 64, 512, 1 << 20, 2>;
 constexpr int NUM_STAGED = 3;
 constexpr int MAX_BINS = 4;
 constexpr int DESC_CAP = 512;  // output descriptors of one chunk of (C tile, row) items
@@ -90,76 +90,128 @@ k_ab_link(int64_t numtileA, const int32_t *__restrict__ colidxA, const int64_t *
 constexpr int RS_WAVES = 4;
 constexpr int RS_WIN_WORDS = 512;  // 16,384 tile columns per pass
 
+// Ascending bitonic sort of one 64-bit key per lane across the wave.
+__device__ __forceinline__ uint64_t wave_sort64(uint64_t key) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int k = 2; k <= WAVE; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t other = __shfl_xor(key, j, WAVE);
+            const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+            const uint64_t mn = key < other ? key : other, mx = key < other ? other : key;
+            key = (lower == up) ? mn : mx;
+        }
+    }
+    return key;
+}
+
 // Exact C tile count of each tile-row (as step1_count_kernel,
 // src/spgemm_raw_kernels.cuh:23-57) plus the sizes that decide its bin:
-// A tiles, A payload, B tiles and B payload reached, C tiles.
+// A tiles, A payload, B tiles and B payload reached, C tiles.  Rows reaching
+// at most 64 B tiles count distinct columns with one wave-wide sort; larger
+// rows use an LDS bitmap over their column window.
 __global__ void __launch_bounds__(RS_WAVES *WAVE)
 k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
             int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats) {
     __shared__ uint32_t s_bm[RS_WAVES][RS_WIN_WORDS];
     __shared__ int32_t s_bs[RS_WAVES][WAVE];
-    __shared__ int32_t s_pre[RS_WAVES][WAVE];
+    __shared__ int32_t s_pre[RS_WAVES][WAVE + 1];
     const int w = threadIdx.x / WAVE, lane = lane_id();
     const int32_t bi = blockIdx.x * RS_WAVES + w;
     if (bi >= tilemA) return;
     uint32_t *bm = s_bm[w];
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
+    const int64_t na = ae - as;
     int32_t cmin = INT_MAX, cmax = -1;
-    int64_t nb = 0, pb = 0;
-    for (int64_t i = as + lane; i < ae; i += WAVE) {
-        const int4 l = link[i];
-        if (l.y > 0) {
-            cmin = min(cmin, colidxB[l.x]);
-            cmax = max(cmax, colidxB[l.x + l.y - 1]);
+    int64_t nb = 0, pb = 0, count = 0;
+    bool done = false;
+    if (na <= WAVE) {
+        const int4 l = lane < na ? link[as + lane] : make_int4(0, 0, 0, 0);
+        const int incl = dpp_incl_sum(l.y);
+        nb = last_lane(incl);
+        pb = last_lane(dpp_incl_sum(l.z));
+        if (nb <= WAVE) {
+            if (lane < na) {
+                s_pre[w][lane + 1] = incl;
+                s_bs[w][lane] = l.x;
+            }
+            if (lane == 0) s_pre[w][0] = 0;
+            wave_sync();
+            int32_t j = 0;
+            if (lane < nb) {
+                const int t = owner_of(s_pre[w], (int)na, lane);
+                j = colidxB[s_bs[w][t] + (lane - s_pre[w][t])];
+            }
+            const uint64_t key = wave_sort64(lane < nb ? (uint64_t)(uint32_t)j : ~0ull);
+            const int32_t sj = (int32_t)(uint32_t)key;
+            const int32_t prev = __shfl_up(sj, 1, WAVE);
+            const bool first = lane < nb && (lane == 0 || prev != sj);
+            count = __popcll(__ballot(first));
+            if (nb > 0) {
+                cmin = __builtin_amdgcn_readlane(sj, 0);
+                cmax = __builtin_amdgcn_readlane(sj, (int)nb - 1);
+            }
+            done = true;
         }
-        nb += l.y;
-        pb += l.z;
     }
-    cmin = wave_min(cmin);
-    cmax = wave_max(cmax);
-    nb = wave_sum(nb);
-    pb = wave_sum(pb);
-    int64_t count = 0;
-    for (int64_t w0 = cmin; w0 <= cmax; w0 += (int64_t)RS_WIN_WORDS * 32) {
-        const int32_t wbits = (int32_t)min<int64_t>((int64_t)RS_WIN_WORDS * 32, cmax - w0 + 1);
-        const int nwords = (wbits + 31) >> 5;
-        for (int i = lane; i < nwords; i += WAVE) bm[i] = 0;
-        wave_sync();
-        for (int64_t c0 = as; c0 < ae; c0 += WAVE) {
-            const int64_t i = c0 + lane;
-            int32_t len = 0, bs = 0;
-            if (i < ae) {
-                const int4 l = link[i];
-                bs = l.x;
-                len = l.y;
+    if (!done) {
+        nb = 0;
+        pb = 0;
+        for (int64_t i = as + lane; i < ae; i += WAVE) {
+            const int4 l = link[i];
+            if (l.y > 0) {
+                cmin = min(cmin, colidxB[l.x]);
+                cmax = max(cmax, colidxB[l.x + l.y - 1]);
             }
-            const int32_t incl = wave_incl_sum(len);
-            const int32_t total = __shfl(incl, WAVE - 1, WAVE);
-            s_pre[w][lane] = incl;
-            s_bs[w][lane] = bs;
+            nb += l.y;
+            pb += l.z;
+        }
+        cmin = wave_min(cmin);
+        cmax = wave_max(cmax);
+        nb = wave_sum(nb);
+        pb = wave_sum(pb);
+        for (int64_t w0 = cmin; w0 <= cmax; w0 += (int64_t)RS_WIN_WORDS * 32) {
+            const int32_t wbits = (int32_t)min<int64_t>((int64_t)RS_WIN_WORDS * 32, cmax - w0 + 1);
+            const int nwords = (wbits + 31) >> 5;
+            for (int i = lane; i < nwords; i += WAVE) bm[i] = 0;
             wave_sync();
-            for (int32_t t = lane; t < total; t += WAVE) {
-                int lo = 0, hi = WAVE - 1;
-                while (lo < hi) {
-                    int mid = (lo + hi) >> 1;
-                    if (s_pre[w][mid] > t) hi = mid; else lo = mid + 1;
+            for (int64_t c0 = as; c0 < ae; c0 += WAVE) {
+                const int64_t i = c0 + lane;
+                int32_t len = 0, bs = 0;
+                if (i < ae) {
+                    const int4 l = link[i];
+                    bs = l.x;
+                    len = l.y;
                 }
-                const int32_t excl = lo ? s_pre[w][lo - 1] : 0;
-                const int64_t off = (int64_t)colidxB[s_bs[w][lo] + (t - excl)] - w0;
-                if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
+                const int32_t incl = wave_incl_sum(len);
+                const int32_t total = __shfl(incl, WAVE - 1, WAVE);
+                s_pre[w][lane] = incl;
+                s_bs[w][lane] = bs;
+                wave_sync();
+                for (int32_t t = lane; t < total; t += WAVE) {
+                    int lo = 0, hi = WAVE - 1;
+                    while (lo < hi) {
+                        int mid = (lo + hi) >> 1;
+                        if (s_pre[w][mid] > t) hi = mid; else lo = mid + 1;
+                    }
+                    const int32_t excl = lo ? s_pre[w][lo - 1] : 0;
+                    const int64_t off = (int64_t)colidxB[s_bs[w][lo] + (t - excl)] - w0;
+                    if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
+                }
+                wave_sync();
             }
+            int32_t cnt = 0;
+            for (int i = lane; i < nwords; i += WAVE) cnt += __popc(bm[i]);
+            count += wave_sum(cnt);
             wave_sync();
         }
-        int32_t cnt = 0;
-        for (int i = lane; i < nwords; i += WAVE) cnt += __popc(bm[i]);
-        count += wave_sum(cnt);
-        wave_sync();
     }
     if (lane == 0) {
         tile_ptrC[bi + 1] = count;
         if (bi == 0) tile_ptrC[0] = 0;
-        const int64_t na = ae - as, pa = tile_nnzA[ae] - tile_nnzA[as];
+        const int64_t pa = tile_nnzA[ae] - tile_nnzA[as];
         int b = bt.nvar;
         for (int v = 0; v < bt.nvar; v++) {
             const StageCaps c = bt.v[v];
@@ -167,13 +219,13 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
         }
         bin[bi] = (uint8_t)b;
         if (stats) {  // per-row features for the learned scheduler
-            int32_t *s = stats + (int64_t)bi * 8;
-            s[0] = (int32_t)min<int64_t>(na, INT_MAX);
-            s[1] = (int32_t)min<int64_t>(pa, INT_MAX);
-            s[2] = (int32_t)min<int64_t>(nb, INT_MAX);
-            s[3] = (int32_t)min<int64_t>(pb, INT_MAX);
-            s[4] = (int32_t)min<int64_t>(count, INT_MAX);
-            s[5] = cmax >= cmin ? cmax - cmin + 1 : 0;
+            int32_t *st = stats + (int64_t)bi * 8;
+            st[0] = (int32_t)min<int64_t>(na, INT_MAX);
+            st[1] = (int32_t)min<int64_t>(pa, INT_MAX);
+            st[2] = (int32_t)min<int64_t>(nb, INT_MAX);
+            st[3] = (int32_t)min<int64_t>(pb, INT_MAX);
+            st[4] = (int32_t)min<int64_t>(count, INT_MAX);
+            st[5] = cmax >= cmin ? cmax - cmin + 1 : 0;
         }
     }
 }
@@ -243,22 +295,6 @@ __global__ void k_bin_header(const int64_t *__restrict__ counts, int nbins, int6
 }
 
 // ---------------------------------------------------------------- wave helpers
-// Ascending bitonic sort of one 64-bit key per lane across the wave.
-__device__ __forceinline__ uint64_t wave_sort64(uint64_t key) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int k = 2; k <= WAVE; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const uint64_t other = __shfl_xor(key, j, WAVE);
-            const bool up = (lane & k) == 0, lower = (lane & j) == 0;
-            const uint64_t mn = key < other ? key : other, mx = key < other ? other : key;
-            key = (lower == up) ? mn : mx;
-        }
-    }
-    return key;
-}
-
 // 16-byte loads of a tile's 16 row pointers (uint8) and 16 row masks (uint16).
 __device__ __forceinline__ uint4 load_ptr16(const uint8_t *p, int64_t tile) {
     return reinterpret_cast<const uint4 *>(p)[tile];
@@ -318,8 +354,8 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         load_mask16(maskA, as + lane, m0, m1);
     }
     for (int i = lane; i < nc * 16; i += WAVE) s.cmask[i] = 0;
-    const int32_t incl = wave_incl_sum(lk.y);
-    const int nB = __shfl(incl, WAVE - 1, WAVE);
+    const int incl = dpp_incl_sum(lk.y);
+    const int nB = last_lane(incl);
     if (nB > Var::NB) {
         if (lane == 0) atomicOr(overflow, 1);
         return;
@@ -332,7 +368,7 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     if (lane == 0) s.aseg[0] = 0;
     wave_sync();
     // level 3: B tile columns and masks, one lane per B tile
-    int32_t j = INT_MAX;
+    int32_t j = 0;
     if (lane < nB) {
         const int t = owner_of(s.aseg, na, lane);
         const int64_t q = (int64_t)s.abs_[t] + (lane - s.aseg[t]);
@@ -341,10 +377,10 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         s.bown[lane] = (uint8_t)t;
         store_mask16(&s.bmask[lane * 16], m0, m1);
     }
-    // distinct tile columns and their ranks
-    const uint64_t key = wave_sort64(lane < nB ? ((uint64_t)(uint32_t)j << 8) | (uint64_t)lane : ~0ull);
-    const int32_t sj = (int32_t)(key >> 8);
-    const int sb = (int)(key & 0xFF);
+    // distinct tile columns and their ranks (tile columns are below 2^27)
+    const uint64_t key = wave_sort64(lane < nB ? ((uint64_t)(uint32_t)j << 6) | (uint64_t)lane : ~0ull);
+    const int32_t sj = (int32_t)(key >> 6);
+    const int sb = (int)(key & 63);
     const bool valid = lane < nB;
     const int32_t prev = __shfl_up(sj, 1, WAVE);
     const bool first = valid && (lane == 0 || prev != sj);
@@ -385,15 +421,12 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
 template <typename V, typename Var>
 struct NumLDS {
     V aval[Var::PA];
-    V bval[Var::PB];
+    int64_t bgoff[Var::NB];
     int64_t coff[Var::NC + 1];
-    int64_t apay[Var::NA];
     int32_t cj[Var::NC];
     int32_t abs_[Var::NA];
     int32_t aoff[Var::NA + 1];
     int32_t aseg[Var::NA + 1];
-    int32_t apseg[Var::NA + 1];
-    int32_t boff[Var::NB];
     uint32_t tmask[Var::NC];
     int16_t pstart[Var::NC + 1];
     uint16_t desc[DESC_CAP];
@@ -403,7 +436,6 @@ struct NumLDS {
     alignas(16) uint8_t aptr[Var::NA * 16];
     alignas(16) uint8_t bptr[Var::NB * 16];
     uint8_t bown[Var::NB];
-    uint8_t bct[Var::NB];
     uint8_t plist[Var::NB];
 };
 
@@ -414,26 +446,29 @@ __device__ __forceinline__ double fma_v<double>(double a, double b, double c) { 
 template <>
 __device__ __forceinline__ float fma_v<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// A's tile headers and values and B's tile headers are staged in LDS; B's
+// values (used about once each per row) are read from memory in the output
+// loop, which keeps LDS per wave small and occupancy high.
 template <typename V, typename Var>
 __global__ void __launch_bounds__(Var::WAVES *WAVE)
 k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
              const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA,
              const uint8_t *__restrict__ ptrA, const uint16_t *__restrict__ maskA, const V *__restrict__ valA,
-             const int4 *__restrict__ link, const int64_t *__restrict__ lpay, const int32_t *__restrict__ colidxB,
+             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB,
              const int64_t *__restrict__ csr_nnzB, const uint8_t *__restrict__ csr_ptrB,
              const uint16_t *__restrict__ csr_maskB, const V *__restrict__ csr_valB,
              const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ colidxC,
              const int64_t *__restrict__ tile_nnzC, const uint16_t *__restrict__ maskC, V *__restrict__ valC,
              uint8_t *__restrict__ colC, int32_t *__restrict__ overflow) {
     static_assert(Var::NC < WAVE && Var::NA <= 32 && Var::NB <= WAVE, "caps exceed the kernel's index widths");
-    constexpr int ITA = (Var::PA + WAVE - 1) / WAVE, ITB = (Var::PB + WAVE - 1) / WAVE;
+    constexpr int ITA = (Var::PA + WAVE - 1) / WAVE;
     __shared__ NumLDS<V, Var> lds[Var::WAVES];
     const int w = threadIdx.x / WAVE, lane = lane_id();
     const int64_t idx = (int64_t)blockIdx.x * Var::WAVES + w;
     if (idx >= nrows) return;
     NumLDS<V, Var> &s = lds[w];
     const int32_t bi = rows[row_begin + idx];
-    // level 1: tile-row pointers
+    // level 1: tile-row pointers (wave-uniform scalars)
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
     const int64_t cs = tile_ptrC[bi], ce = tile_ptrC[bi + 1];
     const int na = (int)(ae - as), nc = (int)(ce - cs);
@@ -442,14 +477,15 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         return;
     }
     // level 2: A headers and links, C headers (all loads issued, then stored)
-    int64_t anz = 0, lp = 0, co = 0;
+    const int64_t abase = tile_nnzA[as];
+    const int pA = (int)(tile_nnzA[ae] - abase);
+    int32_t anz = 0, cjv = 0;
+    int64_t co = 0;
     int4 lk = make_int4(0, 0, 0, 0);
     uint4 ap, am0, am1, cm0, cm1;
-    int32_t cjv = 0;
-    if (lane <= na) anz = tile_nnzA[as + lane];
     if (lane < na) {
+        anz = (int32_t)(tile_nnzA[as + lane] - abase);
         lk = link[as + lane];
-        lp = lpay[as + lane];
         ap = load_ptr16(ptrA, as + lane);
         load_mask16(maskA, as + lane, am0, am1);
     }
@@ -458,22 +494,29 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         cjv = colidxC[cs + lane];
         load_mask16(maskC, cs + lane, cm0, cm1);
     }
-    const int64_t abase = __shfl(anz, 0, WAVE);
-    const int32_t incl = wave_incl_sum(lk.y), pincl = wave_incl_sum(lk.z);
-    const int nB = __shfl(incl, WAVE - 1, WAVE), pB = __shfl(pincl, WAVE - 1, WAVE);
-    const int pA = (int)(__shfl(anz, na, WAVE) - abase);
-    if (lane <= na) s.aoff[lane] = (int32_t)(anz - abase);
+    const int incl = dpp_incl_sum(lk.y);
+    const int nB = last_lane(incl);
+    if (nB > Var::NB || pA > Var::PA) {
+        if (lane == 0) atomicOr(overflow, 1);
+        return;
+    }
+    // A values (one contiguous range)
+    V av[ITA];
+#pragma unroll
+    for (int it = 0; it < ITA; it++) {
+        const int i = lane + it * WAVE;
+        if (i < pA) av[it] = valA[abase + i];
+    }
     if (lane < na) {
+        s.aoff[lane] = anz;
         s.aseg[lane + 1] = incl;
-        s.apseg[lane + 1] = pincl;
         s.abs_[lane] = lk.x;
-        s.apay[lane] = lp;
         reinterpret_cast<uint4 *>(s.aptr)[lane] = ap;
         store_mask16(&s.amask[lane * 16], am0, am1);
     }
     if (lane == 0) {
+        s.aoff[na] = pA;
         s.aseg[0] = 0;
-        s.apseg[0] = 0;
     }
     if (lane <= nc) s.coff[lane] = co;
     if (lane < nc) {
@@ -481,67 +524,37 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         s.tmask[lane] = 0;
         store_mask16(&s.cmask[lane * 16], cm0, cm1);
     }
-    if (nB > Var::NB || pB > Var::PB || pA > Var::PA) {
-        if (lane == 0) atomicOr(overflow, 1);
-        return;
-    }
     wave_sync();
-    // level 3: A values, B headers (one lane per B tile) and B values
-    V av[ITA], bv[ITB];
-#pragma unroll
-    for (int it = 0; it < ITA; it++) {
-        const int i = lane + it * WAVE;
-        if (i < pA) av[it] = valA[abase + i];
-    }
-    int64_t bnz = 0;
-    int32_t bjv = 0;
-    uint4 bp, bm0, bm1;
-    int bt = 0;
+    // level 3: B headers, one lane per B tile
+    int bt = 0, ct_b = 0;
     if (lane < nB) {
         bt = owner_of(s.aseg, na, lane);
         const int64_t q = (int64_t)s.abs_[bt] + (lane - s.aseg[bt]);
-        bjv = colidxB[q];
-        bnz = csr_nnzB[q];
-        bp = load_ptr16(csr_ptrB, q);
+        const int32_t bjv = colidxB[q];
+        const int64_t bg = csr_nnzB[q];
+        uint4 bm0, bm1;
+        const uint4 bp = load_ptr16(csr_ptrB, q);
         load_mask16(csr_maskB, q, bm0, bm1);
-    }
-#pragma unroll
-    for (int it = 0; it < ITB; it++) {
-        const int i = lane + it * WAVE;
-        if (i < pB) {
-            const int t = owner_of(s.apseg, na, i);
-            bv[it] = csr_valB[s.apay[t] + (i - s.apseg[t])];
-        }
-    }
-#pragma unroll
-    for (int it = 0; it < ITA; it++) {
-        const int i = lane + it * WAVE;
-        if (i < pA) s.aval[i] = av[it];
-    }
-#pragma unroll
-    for (int it = 0; it < ITB; it++) {
-        const int i = lane + it * WAVE;
-        if (i < pB) s.bval[i] = bv[it];
-    }
-    int ct_b = 0;
-    if (lane < nB) {
-        s.boff[lane] = (int32_t)(bnz - s.apay[bt]) + s.apseg[bt];
-        s.bown[lane] = (uint8_t)bt;
-        reinterpret_cast<uint4 *>(s.bptr)[lane] = bp;
-        store_mask16(&s.bmask[lane * 16], bm0, bm1);
         int lo = 0, hi = nc;  // C tile of this B tile
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (s.cj[mid] < bjv) lo = mid + 1; else hi = mid;
         }
         ct_b = lo;
-        s.bct[lane] = (uint8_t)lo;
-        atomicOr(&s.tmask[lo], 1u << bt);
+        s.bgoff[lane] = bg;
+        s.bown[lane] = (uint8_t)bt;
+        reinterpret_cast<uint4 *>(s.bptr)[lane] = bp;
+        store_mask16(&s.bmask[lane * 16], bm0, bm1);
+        atomicOr(&s.tmask[ct_b], 1u << bt);
+    }
+#pragma unroll
+    for (int it = 0; it < ITA; it++) {
+        const int i = lane + it * WAVE;
+        if (i < pA) s.aval[i] = av[it];
     }
     wave_sync();
     // pair table: the B tiles of C tile ct in ascending k = set bits of tmask[ct]
-    const int pcnt = lane < nc ? __popc(s.tmask[lane]) : 0;
-    const int pinc = wave_incl_sum(pcnt);
+    const int pinc = dpp_incl_sum(lane < nc ? __popc(s.tmask[lane]) : 0);
     if (lane < nc) s.pstart[lane + 1] = (int16_t)pinc;
     if (lane == 0) s.pstart[0] = 0;
     wave_sync();
@@ -556,8 +569,8 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         const int item = ct0 * 16 + lane;
         const uint32_t mc = (lane < span * 16 && item < nc * 16) ? s.cmask[item] : 0u;
         const int pc = __popc(mc);
-        const int oincl = wave_incl_sum(pc);
-        const int nout = __shfl(oincl, WAVE - 1, WAVE);
+        const int oincl = dpp_incl_sum(pc);
+        const int nout = last_lane(oincl);
         {
             int o = oincl - pc;
             uint32_t m = mc;
@@ -568,31 +581,32 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
             }
         }
         wave_sync();
-        const int64_t obase = s.coff[ct0] - gbase;
+        const int64_t obase = s.coff[ct0];
         for (int o = lane; o < nout; o += WAVE) {
             const int d = s.desc[o];
             const int ct = ct0 + (d >> 8), r = (d >> 4) & 15, c = d & 15;
-            const uint32_t cbit = 0x8000u >> c;
+            const uint32_t cbit = 0x8000u >> c, cbelow = below(c);
             V acc = V(0);
-            for (int p = s.pstart[ct]; p < s.pstart[ct + 1]; p++) {
+            const int pe = s.pstart[ct + 1];
+            for (int p = s.pstart[ct]; p < pe; p++) {
                 const int b = s.plist[p], t = s.bown[b];
                 const uint32_t ma = s.amask[t * 16 + r];
                 uint32_t m = ma;
                 const int ea = s.aoff[t] + s.aptr[t * 16 + r];
+                const int64_t bb = s.bgoff[b];
                 while (m) {
                     const int kk = __clz(m) - 16;
                     m ^= 0x8000u >> kk;
                     const uint32_t mb = s.bmask[b * 16 + kk];
                     if (mb & cbit) {
                         const V va = s.aval[ea + __popc(ma & below(kk))];
-                        const V vb = s.bval[s.boff[b] + s.bptr[b * 16 + kk] + __popc(mb & below(c))];
+                        const V vb = csr_valB[bb + s.bptr[b * 16 + kk] + __popc(mb & cbelow)];
                         acc = fma_v<V>(va, vb, acc);
                     }
                 }
             }
-            const int64_t go = gbase + obase + o;
-            valC[go] = acc;
-            colC[go] = (uint8_t)((r << 4) | c);
+            valC[obase + o] = acc;
+            colC[obase + o] = (uint8_t)((r << 4) | c);
         }
         wave_sync();
         ct0 += span;
@@ -703,7 +717,7 @@ static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
     if (n <= 0) return TSG_OK;
     k_num_staged<V, Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
         p.rows, p.bin_off[bin], n, A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
-        p.link, p.lpay, B->tile_columnidx, B->csr_tile_nnz, B->csr_tile_Ptr, B->csr_mask,
+        p.link, B->tile_columnidx, B->csr_tile_nnz, B->csr_tile_Ptr, B->csr_mask,
         (const V *)B->csr_tile_Val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, (V *)C->tile_csr_Val,
         C->tile_csr_Col, p.overflow);
     TSG_HIP(hipGetLastError());

@@ -152,6 +152,22 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
+// Wave64 inclusive sum of 32-bit integers with DPP lane moves (no LDS
+// traffic): Hillis-Steele over rows of 16 (row_shr 1,2,4,8, zero shifted in),
+// then row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.
+__device__ __forceinline__ int dpp_incl_sum(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);
+    return v;
+}
+
+// Value of lane 63 (a wave-uniform scalar).
+__device__ __forceinline__ int last_lane(int v) { return __builtin_amdgcn_readlane(v, WAVE - 1); }
+
 // 16-lane group inclusive sum (groups are aligned lane ranges of 16).
 template <typename T>
 __device__ __forceinline__ T group16_incl_sum(T v) {
