@@ -268,6 +268,72 @@ class Context:
                                 C.byref(st)))
         return Cm, st.as_dict()
 
+    # ---- learned scheduler (MLP 8 -> 32 -> 32 -> 2 on the GPU)
+    def learned_collect(self, A, B):
+        """Adds A*B's per-tile-row (features, exact targets) to the training set."""
+        _check(lib().tsg_learned_collect(self._p, C.byref(A.s), C.byref(B.s)))
+
+    def learned_clear(self):
+        _check(lib().tsg_learned_clear(self._p))
+
+    def learned_train(self, epochs=200, batch=4096, lr=3e-3, seed=0):
+        """Adam on the collected set; returns the last epoch's mean loss."""
+        loss = C.c_float(0)
+        _check(lib().tsg_learned_train(self._p, C.c_int32(epochs), C.c_int32(batch), C.c_float(lr),
+                                       C.c_uint64(seed), C.byref(loss)))
+        return loss.value
+
+    @staticmethod
+    def learned_num_params():
+        return int(lib().tsg_learned_num_params())
+
+    def learned_weights(self):
+        n = self.learned_num_params()
+        w = np.zeros(n, np.float32)
+        _check(lib().tsg_learned_get_weights(self._p, w.ctypes.data_as(C.c_void_p), C.c_int32(n)))
+        return w
+
+    def learned_set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        _check(lib().tsg_learned_set_weights(self._p, w.ctypes.data_as(C.c_void_p), C.c_int32(w.size)))
+
+    def learned_calibration(self, set_to=None):
+        """Log-space offsets (tiles, nonzeros) added to the predictions; optionally sets them."""
+        get = (C.c_float * 2)()
+        st = None
+        if set_to is not None:
+            st = (C.c_float * 2)(*[float(x) for x in set_to])
+        _check(lib().tsg_learned_calibration(self._p, get, st))
+        return np.array(list(get), np.float32)
+
+    def learned_features(self, A, B):
+        X = np.zeros((A.s.tilem, 8), np.float32)
+        _check(lib().tsg_learned_features(self._p, C.byref(A.s), C.byref(B.s), X.ctypes.data_as(C.c_void_p)))
+        return X
+
+    def learned_predict(self, A, B):
+        """Per tile-row (predicted tiles, predicted nonzeros)."""
+        Y = np.zeros((A.s.tilem, 2), np.float32)
+        _check(lib().tsg_learned_predict(self._p, C.byref(A.s), C.byref(B.s), Y.ctypes.data_as(C.c_void_p)))
+        return Y
+
+    def learned_forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((X.shape[0], 2), np.float32)
+        _check(lib().tsg_learned_forward(self._p, X.ctypes.data_as(C.c_void_p), C.c_int64(X.shape[0]),
+                                         Y.ctypes.data_as(C.c_void_p)))
+        return Y
+
+    def learned_gradient(self, X, T):
+        """(gradient of the mean squared error over the batch, loss)."""
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(self.learned_num_params(), np.float32)
+        loss = C.c_float(0)
+        _check(lib().tsg_learned_gradient(self._p, X.ctypes.data_as(C.c_void_p), T.ctypes.data_as(C.c_void_p),
+                                          C.c_int64(X.shape[0]), g.ctypes.data_as(C.c_void_p), C.byref(loss)))
+        return g, loss.value
+
 
 class DeviceCSR:
     def __init__(self, ctx, h):

@@ -40,6 +40,7 @@ struct NumArgs {
     const uint16_t *maskB; const uint8_t *colB; const V *valB;
     const int64_t *tile_ptrC; const int32_t *rowidxC; const int32_t *colidxC; const int64_t *tile_nnzC;
     const uint16_t *maskC; const uint8_t *ptrRowC; V *valC; uint8_t *colC;
+    const int32_t *row_nc; int32_t *overflow; int64_t val_cap;  // slot layout (learned scheduler)
 };
 
 template <typename V>
@@ -48,7 +49,7 @@ static NumArgs<V> num_args(const tsg_tile_matrix *A, const tsg_tile_matrix *B, c
                       (const V *)A->tile_csr_Val, B->csc_tile_ptr, B->csc_tile_rowidx, B->tile_nnz,
                       B->tile_csr_Ptr, B->mask, B->tile_csr_Col, (const V *)B->tile_csr_Val, C->tile_ptr, C->tile_rowidx,
                       C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr, (V *)C->tile_csr_Val,
-                      C->tile_csr_Col};
+                      C->tile_csr_Col, nullptr, nullptr, INT64_MAX};
 }
 
 // Numeric step of one C tile by a 16-lane group (lane = tile row).  DUPS:
@@ -59,6 +60,10 @@ __device__ __forceinline__ void num_tile(int64_t ti, const NumArgs<V> &p) {
     const int l = lane_id() & 15;
     const int64_t c0 = p.tile_nnzC[ti];
     if (p.tile_nnzC[ti + 1] == c0) return;  // structurally empty tile
+    if (p.tile_nnzC[ti + 1] > p.val_cap) {   // slot layout: C's values outgrew their capacity
+        if ((lane_id() & 15) == 0) atomicOr(p.overflow, 1);
+        return;
+    }
     const int32_t bi = p.rowidxC[ti], bj = p.colidxC[ti];
     V acc[16];
 #pragma unroll
@@ -118,17 +123,22 @@ template <typename V, bool DUPS>
 __global__ void __launch_bounds__(NU_THREADS) k_numeric_list(const int32_t *__restrict__ rows, int64_t row_begin,
                                                              NumArgs<V> p) {
     const int32_t bi = rows[row_begin + blockIdx.x];
-    const int64_t ce = p.tile_ptrC[bi + 1];
-    for (int64_t ti = p.tile_ptrC[bi] + (threadIdx.x >> 4); ti < ce; ti += NU_GROUPS) num_tile<V, DUPS>(ti, p);
+    int64_t cs, ce;
+    c_row_range(p.tile_ptrC, p.row_nc, bi, cs, ce);
+    for (int64_t ti = cs + (threadIdx.x >> 4); ti < ce; ti += NU_GROUPS) num_tile<V, DUPS>(ti, p);
 }
 
 template <typename V>
 static int launch_list_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                         const int32_t *rows, int64_t begin, int64_t count) {
+                         const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
+    NumArgs<V> a = num_args<V>(A, B, C);
+    a.row_nc = sl.row_nc;
+    a.overflow = sl.overflow;
+    a.val_cap = sl.val_cap;
     if ((A->flags | B->flags) & 1)
-        k_numeric_list<V, true><<<(unsigned)count, NU_THREADS, 0, ctx->stream>>>(rows, begin, num_args<V>(A, B, C));
+        k_numeric_list<V, true><<<(unsigned)count, NU_THREADS, 0, ctx->stream>>>(rows, begin, a);
     else
-        k_numeric_list<V, false><<<(unsigned)count, NU_THREADS, 0, ctx->stream>>>(rows, begin, num_args<V>(A, B, C));
+        k_numeric_list<V, false><<<(unsigned)count, NU_THREADS, 0, ctx->stream>>>(rows, begin, a);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -145,11 +155,11 @@ static int launch_tiles_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
 }
 
 int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                        const int32_t *rows, int64_t begin, int64_t count) {
+                        const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
     if (count <= 0) return TSG_OK;
     if (count > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "numeric row list too long");
-    if (A->dtype == TSG_F32) return launch_list_t<float>(ctx, A, B, C, rows, begin, count);
-    return launch_list_t<double>(ctx, A, B, C, rows, begin, count);
+    if (A->dtype == TSG_F32) return launch_list_t<float>(ctx, A, B, C, rows, begin, count, sl);
+    return launch_list_t<double>(ctx, A, B, C, rows, begin, count, sl);
 }
 
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,

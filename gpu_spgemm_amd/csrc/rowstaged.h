@@ -1,0 +1,50 @@
+// rowstaged.h -- host-side interface of the binned pipeline (rowstaged.hip),
+// shared with the learned scheduler (learned.hip).
+#pragma once
+#include "tsg_internal.h"
+
+namespace tsg {
+
+constexpr int MAX_BINS = 4;
+
+// LDS capacities of one staged kernel variant; a tile-row is staged by the
+// first variant whose capacities hold it, else it runs the C-tile kernels.
+struct StageCaps {
+    int32_t na, nc, nb, pa, pb;
+};
+struct BinTable {
+    StageCaps v[MAX_BINS];
+    int32_t nvar;
+};
+
+// Per-call working set: A->B links, per-row bins, bin-ordered row lists.
+struct Plan {
+    int32_t *rows = nullptr;     // row ids grouped by bin, ascending in each bin
+    uint8_t *bin = nullptr;      // bin of each tile-row
+    int64_t *counts = nullptr;   // (bin, block) counts, scanned
+    int64_t *hdr = nullptr;      // [total0, total1, bin offsets...] for one host read
+    int32_t *overflow = nullptr;
+    int4 *link = nullptr;        // per A tile: B tile-row start, tiles, payload count
+    int64_t *lpay = nullptr;     // per A tile: B tile-row payload start
+    int64_t bin_off[MAX_BINS + 1] = {};
+    int nbins = 0;
+};
+
+bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
+BinTable bin_table(bool ok);
+void free_plan(Context *ctx, Plan &p);
+// Allocates the plan's arrays and computes the A->B links.
+int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, bool ok, Plan &p);
+// Partitions rows by p.bin and gathers hdr = [*total0, *total1 (or 0), bin offsets].
+int plan_bin(Context *ctx, Plan &p, int32_t tilem, const int64_t *total0, const int64_t *total1);
+// Reads hdr to the host: totals and bin offsets (one blocking read).
+int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1);
+int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                 const Plan &p, const SlotLayout &sl);
+int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                const Plan &p, const SlotLayout &sl);
+void free_c(Context *ctx, tsg_tile_matrix *C);
+int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                  const tsg_options *opts, tsg_stats *st);
+
+}  // namespace tsg

@@ -26,7 +26,7 @@
 #include <climits>
 #include <cstring>
 
-#include "tsg_internal.h"
+#include "rowstaged.h"
 
 namespace tsg {
 
@@ -41,16 +41,7 @@ using VarS = Variant<8, 16, 32, 128, 1 << 20, 4>;
 using VarW = Variant<32, 16, 32, 512, 1 << 20, 2>;
 using VarM = Variant<32, 48, 64, 512, 1 << 20, 2>;
 constexpr int NUM_STAGED = 3;
-constexpr int MAX_BINS = 4;
 constexpr int DESC_CAP = 512;  // output descriptors of one chunk of (C tile, row) items
-
-struct StageCaps {
-    int32_t na, nc, nb, pa, pb;
-};
-struct BinTable {
-    StageCaps v[MAX_BINS];
-    int32_t nvar;
-};
 
 template <typename Var>
 static StageCaps caps_of() {
@@ -286,12 +277,15 @@ k_bin_scatter(const uint8_t *__restrict__ bin, int64_t n, int nbins, int64_t nbl
     }
 }
 
-// hdr[0] = numtileC, hdr[1 + b] = first row-list slot of bin b (b <= nbins).
+// hdr[0] = *total0, hdr[1] = *total1 (0 when absent), hdr[2 + b] = first
+// row-list slot of bin b (b <= nbins).
 __global__ void k_bin_header(const int64_t *__restrict__ counts, int nbins, int64_t nblocks,
-                             const int64_t *__restrict__ tile_ptrC, int32_t tilem, int64_t *__restrict__ hdr) {
+                             const int64_t *__restrict__ total0, const int64_t *__restrict__ total1,
+                             int64_t *__restrict__ hdr) {
     const int t = threadIdx.x;
-    if (t == 0) hdr[0] = tile_ptrC[tilem];
-    if (t <= nbins) hdr[1 + t] = counts[(int64_t)t * nblocks];
+    if (t == 0) hdr[0] = total0 ? *total0 : 0;
+    if (t == 1) hdr[1] = total1 ? *total1 : 0;
+    if (t <= nbins) hdr[2 + t] = counts[(int64_t)t * nblocks];
 }
 
 // ---------------------------------------------------------------- wave helpers
@@ -331,7 +325,8 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
              const int4 *__restrict__ link, const int32_t *__restrict__ colidxB,
              const uint16_t *__restrict__ csr_maskB, const int64_t *__restrict__ tile_ptrC,
              int32_t *__restrict__ rowidxC, int32_t *__restrict__ colidxC, uint16_t *__restrict__ maskC,
-             uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC, int32_t *__restrict__ overflow) {
+             uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC, int32_t *__restrict__ overflow,
+             int32_t *__restrict__ row_nc) {
     static_assert(Var::NC <= 256 && Var::NA <= 32 && Var::NB <= WAVE, "caps exceed the kernel's index widths");
     __shared__ SymLDS<Var> lds[Var::WAVES];
     const int w = threadIdx.x / WAVE, lane = lane_id();
@@ -340,10 +335,14 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     SymLDS<Var> &s = lds[w];
     const int32_t bi = rows[row_begin + idx];
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
+    // exact layout: [cs, ce) are the row's tiles; slot layout: its slots
     const int64_t cs = tile_ptrC[bi], ce = tile_ptrC[bi + 1];
-    const int na = (int)(ae - as), nc = (int)(ce - cs);
-    if (na > Var::NA || nc > Var::NC) {
-        if (lane == 0) atomicOr(overflow, 1);
+    const int na = (int)(ae - as), cap = (int)(ce - cs);
+    if (na > Var::NA || cap > Var::NC) {
+        if (lane == 0) {
+            atomicOr(overflow, 1);
+            if (row_nc) row_nc[bi] = 0;
+        }
         return;
     }
     // level 2: A links and masks
@@ -353,11 +352,14 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         lk = link[as + lane];
         load_mask16(maskA, as + lane, m0, m1);
     }
-    for (int i = lane; i < nc * 16; i += WAVE) s.cmask[i] = 0;
+    for (int i = lane; i < cap * 16; i += WAVE) s.cmask[i] = 0;
     const int incl = dpp_incl_sum(lk.y);
     const int nB = last_lane(incl);
     if (nB > Var::NB) {
-        if (lane == 0) atomicOr(overflow, 1);
+        if (lane == 0) {
+            atomicOr(overflow, 1);
+            if (row_nc) row_nc[bi] = 0;
+        }
         return;
     }
     if (lane < na) {
@@ -385,6 +387,15 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     const int32_t prev = __shfl_up(sj, 1, WAVE);
     const bool first = valid && (lane == 0 || prev != sj);
     const uint64_t fb = __ballot(first);
+    const int nc = __popcll(fb);
+    if (nc > cap) {  // slot layout: the prediction was short for this row
+        if (lane == 0) {
+            atomicOr(overflow, 1);
+            if (row_nc) row_nc[bi] = 0;
+        }
+        return;
+    }
+    if (row_nc && lane == 0) row_nc[bi] = nc;
     const int rank = mbcnt(fb) + (first ? 1 : 0) - 1;
     if (first) {
         colidxC[cs + rank] = sj;
@@ -459,7 +470,8 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
              const uint16_t *__restrict__ csr_maskB, const V *__restrict__ csr_valB,
              const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ colidxC,
              const int64_t *__restrict__ tile_nnzC, const uint16_t *__restrict__ maskC, V *__restrict__ valC,
-             uint8_t *__restrict__ colC, int32_t *__restrict__ overflow) {
+             uint8_t *__restrict__ colC, int32_t *__restrict__ overflow, const int32_t *__restrict__ row_nc,
+             int64_t val_cap) {
     static_assert(Var::NC < WAVE && Var::NA <= 32 && Var::NB <= WAVE, "caps exceed the kernel's index widths");
     constexpr int ITA = (Var::PA + WAVE - 1) / WAVE;
     __shared__ NumLDS<V, Var> lds[Var::WAVES];
@@ -470,7 +482,8 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     const int32_t bi = rows[row_begin + idx];
     // level 1: tile-row pointers (wave-uniform scalars)
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
-    const int64_t cs = tile_ptrC[bi], ce = tile_ptrC[bi + 1];
+    int64_t cs, ce;
+    c_row_range(tile_ptrC, row_nc, bi, cs, ce);
     const int na = (int)(ae - as), nc = (int)(ce - cs);
     if (na > Var::NA || nc > Var::NC) {
         if (lane == 0) atomicOr(overflow, 1);
@@ -496,7 +509,9 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     }
     const int incl = dpp_incl_sum(lk.y);
     const int nB = last_lane(incl);
-    if (nB > Var::NB || pA > Var::PA) {
+    const int64_t cend = __builtin_amdgcn_readlane((int)(co >> 32), nc) * 4294967296ll +
+                         (uint32_t)__builtin_amdgcn_readlane((int)co, nc);  // end of the row's values
+    if (nB > Var::NB || pA > Var::PA || cend > val_cap) {
         if (lane == 0) atomicOr(overflow, 1);
         return;
     }
@@ -614,25 +629,13 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
 }
 
 // ---------------------------------------------------------------- host side
-struct Plan {
-    int32_t *rows = nullptr;
-    uint8_t *bin = nullptr;
-    int64_t *counts = nullptr;
-    int64_t *hdr = nullptr;
-    int32_t *overflow = nullptr;
-    int4 *link = nullptr;
-    int64_t *lpay = nullptr;
-    int64_t bin_off[MAX_BINS + 1] = {};
-    int nbins = 0;
-};
-
-static bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
+bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
     if (opts && opts->numeric_variant == 1) return false;  // reference method everywhere
     return B->csr_tile_nnz && B->csr_tile_Ptr && B->csr_mask && B->csr_tile_Val && B->row_nnz_ptr &&
            !(A->flags & 1) && !(B->flags & 1);
 }
 
-static BinTable bin_table(bool ok) {
+BinTable bin_table(bool ok) {
     BinTable t{};
     t.nvar = 0;
     if (ok) {
@@ -644,7 +647,7 @@ static BinTable bin_table(bool ok) {
     return t;
 }
 
-static void free_plan(Context *ctx, Plan &p) {
+void free_plan(Context *ctx, Plan &p) {
     void *ps[] = {p.rows, p.bin, p.counts, p.hdr, p.overflow, p.link, p.lpay};
     for (void *q : ps) pool_free(ctx, q);
     p = Plan{};
@@ -652,112 +655,131 @@ static void free_plan(Context *ctx, Plan &p) {
 
 static unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-// A->B links, Step 1 count, statistics, exclusive scan of tile_ptrC, binning
-// and one host read of (numtileC, bin offsets).
-int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-              const tsg_options *opts, Plan &p, int32_t *stats_out) {
-    const bool ok = staged_ok(A, B, opts);
-    const BinTable bt = bin_table(ok);
-    p.nbins = bt.nvar + 1;
+int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, bool ok, Plan &p) {
     const int64_t tilem = A->tilem;
     const int64_t nb = (tilem + BIN_TILE - 1) / BIN_TILE;
+    p.nbins = bin_table(ok).nvar + 1;
     TSG_CHECK(palloc(ctx, &p.rows, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.bin, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.counts, (size_t)(p.nbins * nb + 1)));
-    TSG_CHECK(palloc(ctx, &p.hdr, MAX_BINS + 2));
+    TSG_CHECK(palloc(ctx, &p.hdr, MAX_BINS + 3));
     TSG_CHECK(palloc(ctx, &p.overflow, 1));
     TSG_CHECK(palloc(ctx, &p.link, (size_t)A->numtile));
     TSG_CHECK(palloc(ctx, &p.lpay, (size_t)A->numtile));
-    hipStream_t s = ctx->stream;
-    TSG_HIP(hipMemsetAsync(p.overflow, 0, sizeof(int32_t), s));
-    if (tilem == 0) {
-        TSG_HIP(hipMemsetAsync(C->tile_ptr, 0, sizeof(int64_t), s));
-        C->numtile = 0;
-        return TSG_OK;
-    }
+    TSG_HIP(hipMemsetAsync(p.overflow, 0, sizeof(int32_t), ctx->stream));
     if (A->numtile > 0) {
-        k_ab_link<<<nblk(A->numtile, 256), 256, 0, s>>>(A->numtile, A->tile_columnidx, B->tile_ptr,
-                                                         ok ? B->row_nnz_ptr : nullptr, p.link, p.lpay);
+        k_ab_link<<<nblk(A->numtile, 256), 256, 0, ctx->stream>>>(A->numtile, A->tile_columnidx, B->tile_ptr,
+                                                                   ok ? B->row_nnz_ptr : nullptr, p.link, p.lpay);
         TSG_HIP(hipGetLastError());
     }
-    k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, s>>>(A->tile_ptr, A->tile_nnz, A->tilem, p.link,
-                                                                   B->tile_columnidx, bt, C->tile_ptr, p.bin,
-                                                                   stats_out);
-    TSG_HIP(hipGetLastError());
-    TSG_CHECK(scan_inclusive_i64(ctx, C->tile_ptr, tilem + 1));
-    k_bin_count<<<(unsigned)nb, BIN_THREADS, 0, s>>>(p.bin, tilem, p.nbins, nb, p.counts);
-    TSG_HIP(hipGetLastError());
-    TSG_CHECK(scan_inclusive_i64(ctx, p.counts, p.nbins * nb + 1));
-    k_bin_scatter<<<(unsigned)nb, BIN_THREADS, 0, s>>>(p.bin, tilem, p.nbins, nb, p.counts, p.rows);
-    k_bin_header<<<1, 64, 0, s>>>(p.counts, p.nbins, nb, C->tile_ptr, A->tilem, p.hdr);
-    TSG_HIP(hipGetLastError());
-    TSG_HIP(hipMemcpyAsync(ctx->pinned, p.hdr, sizeof(int64_t) * (p.nbins + 2), hipMemcpyDeviceToHost, s));
-    TSG_HIP(hipStreamSynchronize(s));
-    C->numtile = ctx->pinned[0];
-    for (int b = 0; b <= p.nbins; b++) p.bin_off[b] = ctx->pinned[1 + b];
     return TSG_OK;
+}
+
+int plan_bin(Context *ctx, Plan &p, int32_t tilem, const int64_t *total0, const int64_t *total1) {
+    hipStream_t s = ctx->stream;
+    const int64_t nb = ((int64_t)tilem + BIN_TILE - 1) / BIN_TILE;
+    if (tilem > 0) {
+        k_bin_count<<<(unsigned)nb, BIN_THREADS, 0, s>>>(p.bin, tilem, p.nbins, nb, p.counts);
+        TSG_HIP(hipGetLastError());
+        TSG_CHECK(scan_inclusive_i64(ctx, p.counts, p.nbins * nb + 1));
+        k_bin_scatter<<<(unsigned)nb, BIN_THREADS, 0, s>>>(p.bin, tilem, p.nbins, nb, p.counts, p.rows);
+    } else {
+        TSG_HIP(hipMemsetAsync(p.counts, 0, sizeof(int64_t), s));
+    }
+    k_bin_header<<<1, 64, 0, s>>>(p.counts, p.nbins, nb, total0, total1, p.hdr);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
+int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1) {
+    TSG_HIP(hipMemcpyAsync(ctx->pinned, p.hdr, sizeof(int64_t) * (p.nbins + 3), hipMemcpyDeviceToHost, ctx->stream));
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    if (total0) *total0 = ctx->pinned[0];
+    if (total1) *total1 = ctx->pinned[1];
+    for (int b = 0; b <= p.nbins; b++) p.bin_off[b] = ctx->pinned[2 + b];
+    return TSG_OK;
+}
+
+// A->B links, Step 1 count, statistics, exclusive scan of tile_ptrC, binning
+// and one host read of (numtileC, bin offsets).
+static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                     const tsg_options *opts, Plan &p, int32_t *stats_out) {
+    const bool ok = staged_ok(A, B, opts);
+    const BinTable bt = bin_table(ok);
+    TSG_CHECK(plan_setup(ctx, A, B, ok, p));
+    const int32_t tilem = A->tilem;
+    if (tilem == 0) {
+        TSG_HIP(hipMemsetAsync(C->tile_ptr, 0, sizeof(int64_t), ctx->stream));
+    } else {
+        k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
+            A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out);
+        TSG_HIP(hipGetLastError());
+        TSG_CHECK(scan_inclusive_i64(ctx, C->tile_ptr, (int64_t)tilem + 1));
+    }
+    TSG_CHECK(plan_bin(ctx, p, tilem, C->tile_ptr + tilem, nullptr));
+    return plan_read(ctx, p, &C->numtile, nullptr);
 }
 
 template <typename Var>
 static int launch_sym_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
-                             const tsg_tile_matrix *C, const Plan &p, int bin) {
+                             const tsg_tile_matrix *C, const Plan &p, int bin, const SlotLayout &sl) {
     const int64_t n = p.bin_off[bin + 1] - p.bin_off[bin];
     if (n <= 0) return TSG_OK;
     k_sym_staged<Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
         p.rows, p.bin_off[bin], n, A->tile_ptr, A->mask, p.link, B->tile_columnidx, B->csr_mask, C->tile_ptr,
-        C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, p.overflow);
+        C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, p.overflow, sl.row_nc);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
 
 template <typename V, typename Var>
 static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
-                             const tsg_tile_matrix *C, const Plan &p, int bin) {
+                             const tsg_tile_matrix *C, const Plan &p, int bin, const SlotLayout &sl) {
     const int64_t n = p.bin_off[bin + 1] - p.bin_off[bin];
     if (n <= 0) return TSG_OK;
     k_num_staged<V, Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
         p.rows, p.bin_off[bin], n, A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
         p.link, B->tile_columnidx, B->csr_tile_nnz, B->csr_tile_Ptr, B->csr_mask,
         (const V *)B->csr_tile_Val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, (V *)C->tile_csr_Val,
-        C->tile_csr_Col, p.overflow);
+        C->tile_csr_Col, p.overflow, sl.row_nc, sl.val_cap);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
 
 // Structure fill + symbolic for every bin.
 int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                 const Plan &p) {
+                 const Plan &p, const SlotLayout &sl) {
     const int fb = p.nbins - 1;  // last bin: C-tile kernels
     if (p.nbins > 1) {
-        TSG_CHECK(launch_sym_staged<VarS>(ctx, A, B, C, p, 0));
-        TSG_CHECK(launch_sym_staged<VarW>(ctx, A, B, C, p, 1));
-        TSG_CHECK(launch_sym_staged<VarM>(ctx, A, B, C, p, 2));
+        TSG_CHECK(launch_sym_staged<VarS>(ctx, A, B, C, p, 0, sl));
+        TSG_CHECK(launch_sym_staged<VarW>(ctx, A, B, C, p, 1, sl));
+        TSG_CHECK(launch_sym_staged<VarM>(ctx, A, B, C, p, 2, sl));
     }
     const int64_t nf = p.bin_off[fb + 1] - p.bin_off[fb];
-    TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb], nf));
-    TSG_CHECK(launch_symbolic_list(ctx, A, B, p.rows, p.bin_off[fb], nf, C));
+    TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb], nf, sl));
+    TSG_CHECK(launch_symbolic_list(ctx, A, B, p.rows, p.bin_off[fb], nf, C, sl));
     return TSG_OK;
 }
 
 int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                const Plan &p) {
+                const Plan &p, const SlotLayout &sl) {
     const int fb = p.nbins - 1;
     if (p.nbins > 1) {
         if (A->dtype == TSG_F32) {
-            TSG_CHECK((launch_num_staged<float, VarS>(ctx, A, B, C, p, 0)));
-            TSG_CHECK((launch_num_staged<float, VarW>(ctx, A, B, C, p, 1)));
-            TSG_CHECK((launch_num_staged<float, VarM>(ctx, A, B, C, p, 2)));
+            TSG_CHECK((launch_num_staged<float, VarS>(ctx, A, B, C, p, 0, sl)));
+            TSG_CHECK((launch_num_staged<float, VarW>(ctx, A, B, C, p, 1, sl)));
+            TSG_CHECK((launch_num_staged<float, VarM>(ctx, A, B, C, p, 2, sl)));
         } else {
-            TSG_CHECK((launch_num_staged<double, VarS>(ctx, A, B, C, p, 0)));
-            TSG_CHECK((launch_num_staged<double, VarW>(ctx, A, B, C, p, 1)));
-            TSG_CHECK((launch_num_staged<double, VarM>(ctx, A, B, C, p, 2)));
+            TSG_CHECK((launch_num_staged<double, VarS>(ctx, A, B, C, p, 0, sl)));
+            TSG_CHECK((launch_num_staged<double, VarW>(ctx, A, B, C, p, 1, sl)));
+            TSG_CHECK((launch_num_staged<double, VarM>(ctx, A, B, C, p, 2, sl)));
         }
     }
-    TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, p.bin_off[fb], p.bin_off[fb + 1] - p.bin_off[fb]));
+    TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, p.bin_off[fb], p.bin_off[fb + 1] - p.bin_off[fb], sl));
     return TSG_OK;
 }
 
-static void free_c(Context *ctx, tsg_tile_matrix *C) {
+void free_c(Context *ctx, tsg_tile_matrix *C) {
     void *ps[] = {C->tile_ptr, C->tile_columnidx, C->tile_rowidx, C->tile_nnz, C->tile_csr_Val, C->tile_csr_Col,
                   C->tile_csr_Ptr, C->mask};
     for (void *q : ps) pool_free(ctx, q);
@@ -792,7 +814,9 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)C->numtile + 1);
     if (timing && !rc) (void)hipEventRecord(ev[1], s);
     if (!rc && hipMemsetAsync(C->tile_nnz, 0, sizeof(int64_t), s) != hipSuccess) rc = fail(TSG_ERR_HIP, "memset");
-    if (!rc) rc = run_symbolic(ctx, A, B, C, p);
+    SlotLayout exact;
+    exact.overflow = p.overflow;
+    if (!rc) rc = run_symbolic(ctx, A, B, C, p, exact);
     if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1);
     if (!rc) {
         // nnzC and the capacity flag in one read
@@ -807,7 +831,7 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     if (timing && !rc) (void)hipEventRecord(ev[2], s);
     if (!rc) rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)C->nnz * vsize(C->dtype));
     if (!rc) rc = palloc(ctx, &C->tile_csr_Col, (size_t)C->nnz);
-    if (!rc && C->nnz > 0) rc = run_numeric(ctx, A, B, C, p);
+    if (!rc && C->nnz > 0) rc = run_numeric(ctx, A, B, C, p, exact);
     if (timing && !rc) (void)hipEventRecord(ev[3], s);
     if (st) {
         st->numtileC = C->numtile;

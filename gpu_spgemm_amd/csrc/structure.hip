@@ -30,7 +30,7 @@ k_structure(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ c
             const int64_t *__restrict__ tile_ptrB, const int32_t *__restrict__ colidxB,
             int64_t *__restrict__ tile_ptrC, int32_t *__restrict__ rowidxC,
             int32_t *__restrict__ colidxC, const int32_t *__restrict__ rows, int64_t row_begin,
-            int64_t nrows) {
+            int64_t nrows, int32_t *__restrict__ row_nc, int32_t *__restrict__ overflow) {
     __shared__ uint32_t s_bm[ST_WAVES][ST_WIN_WORDS];
     __shared__ int64_t s_bs[ST_WAVES][WAVE];
     __shared__ int32_t s_pre[ST_WAVES][WAVE];
@@ -55,6 +55,9 @@ k_structure(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ c
 
     int64_t count = 0;
     int64_t out = FILL ? tile_ptrC[bi] : 0;
+    // slot layout: the row's slots end at tile_ptrC[bi + 1]; an overfull row
+    // writes nothing past them and raises the overflow flag
+    const int64_t slot_end = (FILL && row_nc) ? tile_ptrC[bi + 1] : INT64_MAX;
     for (int64_t w0 = cmin; w0 <= cmax; w0 += (int64_t)ST_WIN_WORDS * 32) {
         const int32_t wbits = (int32_t)min<int64_t>((int64_t)ST_WIN_WORDS * 32, cmax - w0 + 1);
         const int nwords = (wbits + 31) >> 5;
@@ -103,8 +106,10 @@ k_structure(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ c
                 while (b) {
                     int bit = __ffs(b) - 1;
                     b &= b - 1;
-                    rowidxC[pos] = bi;
-                    colidxC[pos] = (int32_t)(w0 + i * 32 + bit);
+                    if (pos < slot_end) {
+                        rowidxC[pos] = bi;
+                        colidxC[pos] = (int32_t)(w0 + i * 32 + bit);
+                    }
                     pos++;
                 }
             }
@@ -115,6 +120,12 @@ k_structure(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ c
     if (!FILL && lane == 0) {
         tile_ptrC[bi + 1] = count;
         if (bi == 0) tile_ptrC[0] = 0;
+    }
+    if (FILL && row_nc && lane == 0) {
+        // an overfull row keeps only its slots' worth (its results are discarded)
+        const int64_t n = min<int64_t>(out, slot_end) - tile_ptrC[bi];
+        row_nc[bi] = (int32_t)min<int64_t>(n, INT32_MAX);
+        if (out > slot_end) atomicOr(overflow, 1);
     }
 }
 
@@ -127,7 +138,7 @@ int launch_structure_count(Context *ctx, const tsg_tile_matrix *A, const tsg_til
     unsigned grid = (unsigned)((A->tilem + ST_WAVES - 1) / ST_WAVES);
     k_structure<false><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx, tile_ptrC, nullptr,
-        nullptr, nullptr, 0, 0);
+        nullptr, nullptr, 0, 0, nullptr, nullptr);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -138,18 +149,19 @@ int launch_structure_fill(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
     unsigned grid = (unsigned)((A->tilem + ST_WAVES - 1) / ST_WAVES);
     k_structure<true><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx,
-        const_cast<int64_t *>(tile_ptrC), rowidxC, colidxC, nullptr, 0, 0);
+        const_cast<int64_t *>(tile_ptrC), rowidxC, colidxC, nullptr, 0, 0, nullptr, nullptr);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
 
 int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
-                               const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count) {
+                               const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
+                               const SlotLayout &sl) {
     if (count <= 0) return TSG_OK;
     unsigned grid = (unsigned)((count + ST_WAVES - 1) / ST_WAVES);
     k_structure<true><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx, C->tile_ptr, C->tile_rowidx,
-        C->tile_columnidx, rows, begin, count);
+        C->tile_columnidx, rows, begin, count, sl.row_nc, sl.overflow);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }

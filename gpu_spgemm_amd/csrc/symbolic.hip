@@ -75,22 +75,24 @@ k_symbolic_list(const int32_t *__restrict__ rows, int64_t row_begin, const int64
                 const uint16_t *__restrict__ maskA, const int64_t *__restrict__ csc_ptrB,
                 const int32_t *__restrict__ csc_rowB, const uint16_t *__restrict__ maskB,
                 const int32_t *__restrict__ rowidxC, const int32_t *__restrict__ colidxC,
-                uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC) {
+                uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC,
+                const int32_t *__restrict__ row_nc) {
     __shared__ uint16_t s_mb[SY_GROUPS][16];
     const int g = threadIdx.x >> 4;
     const int32_t bi = rows[row_begin + blockIdx.x];
-    const int64_t ce = tile_ptrC[bi + 1];
-    for (int64_t ti = tile_ptrC[bi] + g; ti < ce; ti += SY_GROUPS)
+    int64_t cs, ce;
+    c_row_range(tile_ptrC, row_nc, bi, cs, ce);
+    for (int64_t ti = cs + g; ti < ce; ti += SY_GROUPS)
         sym_tile(ti, tile_ptrA, colidxA, maskA, csc_ptrB, csc_rowB, maskB, rowidxC, colidxC, maskC, ptrRowC,
                  tile_nnzC, s_mb[g]);
 }
 
 int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const int32_t *rows,
-                         int64_t begin, int64_t count, const tsg_tile_matrix *C) {
+                         int64_t begin, int64_t count, const tsg_tile_matrix *C, const SlotLayout &sl) {
     if (count <= 0) return TSG_OK;
     k_symbolic_list<<<(unsigned)count, SY_THREADS, 0, ctx->stream>>>(
         rows, begin, C->tile_ptr, A->tile_ptr, A->tile_columnidx, A->mask, B->csc_tile_ptr, B->csc_tile_rowidx,
-        B->mask, C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz);
+        B->mask, C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, sl.row_nc);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -90,13 +91,21 @@ int launch_structure_fill(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
 int launch_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                     int64_t numtileC, const int32_t *rowidxC, const int32_t *colidxC,
                     uint16_t *maskC, uint8_t *ptrRowC, int64_t *tile_nnzC);
+// Slot layout of C used by the learned scheduler (null members: exact layout).
+struct SlotLayout {
+    int32_t *row_nc = nullptr;     // actual tiles of each tile-row
+    int32_t *overflow = nullptr;   // set when a row or C's values outgrow their capacity
+    int64_t val_cap = INT64_MAX;   // capacity of C's value arrays
+};
 int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
-                               const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count);
+                               const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
+                               const SlotLayout &sl);
 int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const int32_t *rows,
-                         int64_t begin, int64_t count, const tsg_tile_matrix *C);
+                         int64_t begin, int64_t count, const tsg_tile_matrix *C, const SlotLayout &sl);
 int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                        const int32_t *rows, int64_t begin, int64_t count);
+                        const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
 void learned_release(Context *ctx);  // learned.hip
+int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B);  // api.hip
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                    tsg_tile_matrix *C, const tsg_options *opts);
 
@@ -185,6 +194,15 @@ __device__ __forceinline__ T group16_sum(T v) {
 #pragma unroll
     for (int d = 8; d > 0; d >>= 1) v += __shfl_xor(v, d, 16);
     return v;
+}
+
+// Tile range [s,e) of C tile-row bi.  Exact layout: ptr is tile_ptrC.  Slot
+// layout (learned scheduler): ptr holds each row's padded slot offsets and
+// row_nc the row's actual tile count.
+__device__ __forceinline__ void c_row_range(const int64_t *ptr, const int32_t *row_nc, int32_t bi, int64_t &s,
+                                            int64_t &e) {
+    s = ptr[bi];
+    e = row_nc ? s + row_nc[bi] : ptr[bi + 1];
 }
 
 // First index in [lo,hi) of sorted arr with arr[i] >= key.

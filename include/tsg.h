@@ -106,7 +106,8 @@ typedef struct tsg_options {
                                C-tile kernels; 1 = C-tile kernels for every tile (the reference's method) */
     int32_t mfma_min_products; /* tile pairs with >= this many products may use MFMA (0 = default) */
     int32_t timing;         /* 1 = record per-step hipEvent times in tsg_stats */
-    double  pred_scale;     /* learned: multiply predicted sizes (1.0 normal; 0 forces fallback) */
+    double  pred_scale;     /* learned: scale of the predicted capacities (0 or 1.0 = normal; below 1 drops the
+                               margin, a test hook that forces the exact fallback) */
 } tsg_options;
 
 typedef struct tsg_stats {
@@ -186,6 +187,43 @@ int tsg_tile2csr(tsg_context *ctx, const tsg_tile_matrix *dev, tsg_csr *out_dev)
  * library in the context pool (free with tsg_tile_free).  opts/stats may be NULL. */
 int tsg_spgemm(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                tsg_tile_matrix *C, const tsg_options *opts, tsg_stats *stats);
+
+/* ---- Learned scheduler (no reference counterpart: the reference sizes C with
+ * exact count kernels and host reads, src/spgemm_raw_v2.cu:119-141, :191-211) ----
+ * An MLP (8 features -> 32 -> 32 -> 2, ReLU, FP32; tsg_learned_num_params()
+ * weights: W1[8][32], b1[32], W2[32][32], b2[32], W3[32][2], b3[2], row-major,
+ * x @ W) predicts log1p of each C tile-row's tile count and nonzero count from
+ * per-tile-row features of A*B: log1p of (A tiles, B tiles reached, A nonzeros,
+ * B nonzeros reached, span of B tile columns, longest B tile-row reached, summed
+ * B tile-row spans, nonempty B tile-rows reached).  With scheduler
+ * TSG_SCHED_LEARNED, tsg_spgemm sizes C from the prediction and falls back to the
+ * exact pipeline when any row outgrows it (stats->fallback_used = 1): the result
+ * is always the exact product. */
+int tsg_learned_num_params(void);
+/* Appends (features, exact targets) of every tile-row of A*B to the context's
+ * training set (runs the exact product once). */
+int tsg_learned_collect(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B);
+int tsg_learned_clear(tsg_context *ctx);
+/* Adam (beta 0.9/0.999, eps 1e-8) on the mean squared error over the training
+ * set, shuffled minibatches of `batch`; weights start Glorot-uniform from `seed`
+ * when none are set.  final_loss: mean loss of the last epoch (may be NULL). */
+int tsg_learned_train(tsg_context *ctx, int32_t epochs, int32_t batch, float lr, uint64_t seed,
+                      float *final_loss);
+int tsg_learned_get_weights(tsg_context *ctx, float *w, int32_t n);
+/* Log-space offsets (tiles, nonzeros) added to the predictions before C is
+ * sized; training sets them to max(target - prediction) over the training set,
+ * clamped to [log 1.05, log 16].  get and/or set: 2 floats each (may be NULL). */
+int tsg_learned_calibration(tsg_context *ctx, float *get, const float *set);
+int tsg_learned_set_weights(tsg_context *ctx, const float *w, int32_t n);
+/* Host arrays: feats tilem x 8, pred tilem x 2 (expm1 of the network's outputs:
+ * predicted tiles, nonzeros). */
+int tsg_learned_features(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, float *feats);
+int tsg_learned_predict(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, float *pred);
+/* The network's GPU forward pass and mean-squared-error gradient on host arrays
+ * X (n x 8), T (n x 2 log1p targets): Y (n x 2 raw outputs), grad (num_params). */
+int tsg_learned_forward(tsg_context *ctx, const float *X, int64_t n, float *Y);
+int tsg_learned_gradient(tsg_context *ctx, const float *X, const float *T, int64_t n, float *grad,
+                         float *loss);
 
 /* Many independent products in one persistent launch (no reference counterpart). */
 int tsg_spgemm_batched(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A,
