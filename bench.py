@@ -83,6 +83,29 @@ def train_learned(g, ctx, log):
         f"loss {loss:.4f}, calibration {ctx.learned_calibration().tolist()}")
 
 
+def copy_probe(mib):
+    import torch
+    n = mib << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n, "copies": 5}), flush=True)
+
+
+def profiled_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 --pmc summaries (lines 'TRAFFIC <workload> <kernel> <bytes>'), or None."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
+        for line in open(f):
+            p = line.split()
+            if len(p) == 4 and p[0] == "TRAFFIC" and p[1] == workload and p[2] == kernel:
+                return int(p[3])
+    return None
+
+
 def cpu_baseline(A, B, seconds):
     from oracle import oracle as O
     m = A.m
@@ -110,7 +133,13 @@ def main():
     ap.add_argument("--scheduler", default="plain", choices=["plain", "learned"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device-to-device copy of this many MiB 5 times (the known-size kernel "
+                         "that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
               f"(the product is a single-GPU kernel path; sharding is described in DESIGN.md)",
@@ -167,6 +196,9 @@ def main():
     pipe_bytes = alg_bytes(Ad, Bd, Cd, s)
     nbytes = numeric_bytes(Ad, Bd, Cd, s)
     achieved = nbytes / (step3 * 1e-3) / 1e9
+    # the numeric step of this workload is one launch of the staged numeric
+    # kernel (every tile-row fits the small variant); its HIP-event time is step3
+    kname = "k_num_staged<double,VarS>" if best == "plain" else "numeric step (learned)"
     out = {
         "metric": "SpGEMM GFLOP/s + HBM GB/s, learned/batched vs plain tiled, 1 MI355X",
         "value": round(gflops, 3),
@@ -187,9 +219,10 @@ def main():
         "step_ms": {"structure": round(step1, 5), "symbolic": round(step2, 5), "numeric": round(step3, 5)},
         "variants_ms_per_step": {k: round(v[0], 5) for k, v in runs.items()},
         "variants_gflops": {k: round(2.0 * products / (v[0] * 1e6), 3) for k, v in runs.items()},
-        "roofline": {"bound": "hbm", "kernel": "numeric step (k_num_staged + k_numeric_list)", "achieved": round(achieved, 2),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "alg_bytes_per_launch": nbytes},
+                     "traffic": profiled_traffic(cfg["workload"], kname), "alg_bytes_per_launch": nbytes,
+                     "launch_ms": round(step3, 5)},
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
