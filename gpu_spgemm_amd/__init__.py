@@ -42,9 +42,9 @@ class TileStruct(C.Structure):
                 ("tile_csr_Val", C.c_void_p), ("tile_csr_Col", C.POINTER(C.c_uint8)),
                 ("tile_csr_Ptr", C.POINTER(C.c_uint8)), ("mask", C.POINTER(C.c_uint16)),
                 ("csc_tile_ptr", C.POINTER(C.c_int64)), ("csc_tile_rowidx", C.POINTER(C.c_int32)),
-                ("csr_tile_nnz", C.POINTER(C.c_int64)), ("csr_tile_Ptr", C.POINTER(C.c_uint8)),
-                ("csr_mask", C.POINTER(C.c_uint16)), ("csr_tile_Val", C.c_void_p),
-                ("row_nnz_ptr", C.POINTER(C.c_int64))]
+                ("csr_tile_nnz", C.POINTER(C.c_int64)), ("csr_mask", C.POINTER(C.c_uint16)),
+                ("csr_col_mask", C.POINTER(C.c_uint16)), ("csr_col_Ptr", C.POINTER(C.c_uint8)),
+                ("csr_col_Val", C.c_void_p), ("row_nnz_ptr", C.POINTER(C.c_int64))]
 
 
 class Options(C.Structure):
@@ -391,6 +391,13 @@ class DeviceTiles:
         if h.is_csc:
             d["csc_tile_ptr"] = _np(h.csc_tile_ptr, h.tilen + 1, np.int64)
             d["csc_tile_rowidx"] = _np(h.csc_tile_rowidx, nt, np.int32)
+            if h.csr_tile_nnz:      # the CSR-order, column-major copy for the staged kernels
+                d["csr_tile_nnz"] = _np(h.csr_tile_nnz, nt + 1, np.int64)
+                d["csr_mask"] = _np(h.csr_mask, nt * 16, np.uint16)
+                d["csr_col_mask"] = _np(h.csr_col_mask, nt * 16, np.uint16)
+                d["csr_col_Ptr"] = _np(h.csr_col_Ptr, nt * 16, np.uint8)
+                d["csr_col_Val"] = _np(h.csr_col_Val, nnz, vdt)
+                d["row_nnz_ptr"] = _np(h.row_nnz_ptr, h.tilem + 1, np.int64)
         _check(lib().tsg_tile_free_host(C.byref(h)))
         return d
 

@@ -248,17 +248,18 @@ int tsg_tile_upload(tsg_context *c, const tsg_tile_matrix *h, tsg_tile_matrix *d
     TSG_CHECK(up(ctx, h->tile_csr_Col, (size_t)h->nnz, &t.tile_csr_Col));
     TSG_CHECK(up(ctx, h->tile_csr_Ptr, nt * 16, &t.tile_csr_Ptr));
     TSG_CHECK(up(ctx, h->mask, nt * 16, &t.mask));
-    t.csr_tile_nnz = nullptr; t.csr_tile_Ptr = nullptr; t.csr_mask = nullptr; t.csr_tile_Val = nullptr;
-    t.row_nnz_ptr = nullptr;
+    t.csr_tile_nnz = nullptr; t.csr_mask = nullptr; t.csr_col_mask = nullptr; t.csr_col_Ptr = nullptr;
+    t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
     if (h->is_csc) {
         if (!h->csc_tile_ptr || !h->csc_tile_rowidx) return fail(TSG_ERR_ARG, "B operand lacks CSC tile arrays");
         TSG_CHECK(up(ctx, h->csc_tile_ptr, (size_t)h->tilen + 1, &t.csc_tile_ptr));
         TSG_CHECK(up(ctx, h->csc_tile_rowidx, nt, &t.csc_tile_rowidx));
-        if (h->csr_tile_nnz && h->csr_tile_Ptr && h->csr_mask && h->csr_tile_Val && h->row_nnz_ptr) {
+        if (h->csr_tile_nnz && h->csr_mask && h->csr_col_mask && h->csr_col_Ptr && h->csr_col_Val && h->row_nnz_ptr) {
             TSG_CHECK(up(ctx, h->csr_tile_nnz, nt + 1, &t.csr_tile_nnz));
-            TSG_CHECK(up(ctx, h->csr_tile_Ptr, nt * 16, &t.csr_tile_Ptr));
             TSG_CHECK(up(ctx, h->csr_mask, nt * 16, &t.csr_mask));
-            TSG_CHECK(up(ctx, (const uint8_t *)h->csr_tile_Val, (size_t)h->nnz * vsize(h->dtype), (uint8_t **)&t.csr_tile_Val));
+            TSG_CHECK(up(ctx, h->csr_col_mask, nt * 16, &t.csr_col_mask));
+            TSG_CHECK(up(ctx, h->csr_col_Ptr, nt * 16, &t.csr_col_Ptr));
+            TSG_CHECK(up(ctx, (const uint8_t *)h->csr_col_Val, (size_t)h->nnz * vsize(h->dtype), (uint8_t **)&t.csr_col_Val));
             TSG_CHECK(up(ctx, h->row_nnz_ptr, (size_t)h->tilem + 1, &t.row_nnz_ptr));
         }
     }
@@ -284,16 +285,17 @@ int tsg_tile_download(tsg_context *c, const tsg_tile_matrix *d, tsg_tile_matrix 
     TSG_CHECK(down(ctx, d->tile_csr_Col, (size_t)d->nnz, &t.tile_csr_Col));
     TSG_CHECK(down(ctx, d->tile_csr_Ptr, nt * 16, &t.tile_csr_Ptr));
     TSG_CHECK(down(ctx, d->mask, nt * 16, &t.mask));
-    t.csr_tile_nnz = nullptr; t.csr_tile_Ptr = nullptr; t.csr_mask = nullptr; t.csr_tile_Val = nullptr;
-    t.row_nnz_ptr = nullptr;
+    t.csr_tile_nnz = nullptr; t.csr_mask = nullptr; t.csr_col_mask = nullptr; t.csr_col_Ptr = nullptr;
+    t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
     if (d->is_csc) {
         TSG_CHECK(down(ctx, d->csc_tile_ptr, (size_t)d->tilen + 1, &t.csc_tile_ptr));
         TSG_CHECK(down(ctx, d->csc_tile_rowidx, nt, &t.csc_tile_rowidx));
-        if (d->csr_tile_nnz) {
+        if (d->csr_tile_nnz && d->csr_mask && d->csr_col_mask && d->csr_col_Ptr && d->csr_col_Val && d->row_nnz_ptr) {
             TSG_CHECK(down(ctx, d->csr_tile_nnz, nt + 1, &t.csr_tile_nnz));
-            TSG_CHECK(down(ctx, d->csr_tile_Ptr, nt * 16, &t.csr_tile_Ptr));
             TSG_CHECK(down(ctx, d->csr_mask, nt * 16, &t.csr_mask));
-            TSG_CHECK(down(ctx, (const uint8_t *)d->csr_tile_Val, (size_t)d->nnz * vsize(d->dtype), (uint8_t **)&t.csr_tile_Val));
+            TSG_CHECK(down(ctx, d->csr_col_mask, nt * 16, &t.csr_col_mask));
+            TSG_CHECK(down(ctx, d->csr_col_Ptr, nt * 16, &t.csr_col_Ptr));
+            TSG_CHECK(down(ctx, (const uint8_t *)d->csr_col_Val, (size_t)d->nnz * vsize(d->dtype), (uint8_t **)&t.csr_col_Val));
             TSG_CHECK(down(ctx, d->row_nnz_ptr, (size_t)d->tilem + 1, &t.row_nnz_ptr));
         }
     }
@@ -308,7 +310,7 @@ int tsg_tile_free(tsg_context *c, tsg_tile_matrix *d) {
     if (d->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles are not on the device");
     void *ps[] = {d->tile_ptr, d->tile_columnidx, d->tile_rowidx, d->tile_nnz, d->tile_csr_Val,
                   d->tile_csr_Col, d->tile_csr_Ptr, d->mask, d->csc_tile_ptr, d->csc_tile_rowidx,
-                  d->csr_tile_nnz, d->csr_tile_Ptr, d->csr_mask, d->csr_tile_Val, d->row_nnz_ptr};
+                  d->csr_tile_nnz, d->csr_mask, d->csr_col_mask, d->csr_col_Ptr, d->csr_col_Val, d->row_nnz_ptr};
     for (void *p : ps) pool_free(ctx, p);
     memset(d, 0, sizeof *d);
     return TSG_OK;
@@ -319,7 +321,7 @@ int tsg_tile_free_host(tsg_tile_matrix *h) {
     if (h->location != TSG_HOST) return fail(TSG_ERR_ARG, "tiles are not on the host");
     void *ps[] = {h->tile_ptr, h->tile_columnidx, h->tile_rowidx, h->tile_nnz, h->tile_csr_Val,
                   h->tile_csr_Col, h->tile_csr_Ptr, h->mask, h->csc_tile_ptr, h->csc_tile_rowidx,
-                  h->csr_tile_nnz, h->csr_tile_Ptr, h->csr_mask, h->csr_tile_Val, h->row_nnz_ptr};
+                  h->csr_tile_nnz, h->csr_mask, h->csr_col_mask, h->csr_col_Ptr, h->csr_col_Val, h->row_nnz_ptr};
     for (void *p : ps) free(p);
     memset(h, 0, sizeof *h);
     return TSG_OK;

@@ -297,6 +297,42 @@ __global__ void __launch_bounds__(256) k_row_nnz_ptr(int32_t tilem, const int64_
     if (k <= tilem) rnp[k] = tile_nnz[tile_ptr[k]];
 }
 
+// Column-major copy of a row-tiled matrix's tiles (16 lanes per tile, lane =
+// column c): column masks (bit 15-r = row r), exclusive column offsets, and
+// the values of each column in ascending row order.  Rows hold each column at
+// most once (no duplicates).
+template <typename V>
+__global__ void __launch_bounds__(256)
+k_colmajor(int64_t numtile, const int64_t *__restrict__ tile_nnz, const uint8_t *__restrict__ ptr,
+           const uint16_t *__restrict__ mask, const V *__restrict__ val, uint16_t *__restrict__ cmask,
+           uint8_t *__restrict__ cptr, V *__restrict__ cval) {
+    const int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
+    const int c = threadIdx.x & 15;
+    if (p >= numtile) return;  // whole 16-lane groups leave together
+    const uint32_t rm = mask[p * 16 + c];  // lane c holds row c's mask and offset
+    const int ro = ptr[p * 16 + c];
+    const uint32_t cbit = 0x8000u >> c;
+    uint32_t cm = 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        if (__shfl(rm, r, 16) & cbit) cm |= 0x8000u >> r;
+    const int pc = __popc(cm);
+    const int co = group16_incl_sum(pc) - pc;
+    cmask[p * 16 + c] = (uint16_t)cm;
+    cptr[p * 16 + c] = (uint8_t)co;
+    const int64_t base = tile_nnz[p];
+    int j = 0;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t m = __shfl(rm, r, 16);
+        const int o = __shfl(ro, r, 16);
+        if (m & cbit) {
+            cval[base + co + j] = val[base + o + __popc(m & ~((cbit << 1) - 1u))];
+            j++;
+        }
+    }
+}
+
 static unsigned blocks_for(int64_t threads, int per) { return (unsigned)((threads + per - 1) / per); }
 
 }  // namespace tsg
@@ -465,11 +501,41 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
         if (own_rowidx) pool_free(ctx, rowidx);
         return rc;
     }
+    // column-major values of the CSR-order tiles for the staged numeric kernel
+    // (only without duplicate columns: the staged kernels are not used otherwise)
+    uint16_t *ccm = nullptr;
+    uint8_t *ccp = nullptr;
+    void *ccv = nullptr;
+    if (!(B->flags & 1)) {
+        rc = palloc(ctx, &ccm, (size_t)nt * 16);
+        if (!rc) rc = palloc(ctx, &ccp, (size_t)nt * 16);
+        if (!rc) rc = pool_alloc(ctx, &ccv, (size_t)B->nnz * vsize(B->dtype));
+        if (!rc && nt > 0) {
+            if (B->dtype == TSG_F32)
+                k_colmajor<float><<<blocks_for(nt * 16, 256), 256, 0, s>>>(
+                    nt, B->tile_nnz, B->tile_csr_Ptr, B->mask, (const float *)B->tile_csr_Val, ccm, ccp, (float *)ccv);
+            else
+                k_colmajor<double><<<blocks_for(nt * 16, 256), 256, 0, s>>>(
+                    nt, B->tile_nnz, B->tile_csr_Ptr, B->mask, (const double *)B->tile_csr_Val, ccm, ccp,
+                    (double *)ccv);
+            if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "colmajor launch");
+        }
+        if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(TSG_ERR_HIP, "colmajor failed");
+        if (rc) {
+            void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp, ccm, ccp, ccv};
+            for (void *p : ps) pool_free(ctx, p);
+            if (own_rowidx) pool_free(ctx, rowidx);
+            return rc;
+        }
+    }
     pool_free(ctx, B->tile_csr_Col);
+    pool_free(ctx, B->tile_csr_Ptr);
+    pool_free(ctx, B->tile_csr_Val);
     B->csr_tile_nnz = B->tile_nnz;
-    B->csr_tile_Ptr = B->tile_csr_Ptr;
     B->csr_mask = B->mask;
-    B->csr_tile_Val = B->tile_csr_Val;
+    B->csr_col_mask = ccm;
+    B->csr_col_Ptr = ccp;
+    B->csr_col_Val = ccv;
     B->row_nnz_ptr = rnp;
     B->tile_nnz = nnz_new;
     B->tile_csr_Ptr = ptr_new;

@@ -443,9 +443,9 @@ struct NumLDS {
     uint16_t desc[DESC_CAP];
     alignas(16) uint16_t cmask[Var::NC * 16];  // 16-byte stores
     alignas(16) uint16_t amask[Var::NA * 16];
-    alignas(16) uint16_t bmask[Var::NB * 16];
+    alignas(16) uint16_t bcmask[Var::NB * 16];  // B column masks
     alignas(16) uint8_t aptr[Var::NA * 16];
-    alignas(16) uint8_t bptr[Var::NB * 16];
+    alignas(16) uint8_t bcptr[Var::NB * 16];    // B column offsets
     uint8_t bown[Var::NB];
     uint8_t plist[Var::NB];
 };
@@ -459,15 +459,18 @@ __device__ __forceinline__ float fma_v<float>(float a, float b, float c) { retur
 
 // A's tile headers and values and B's tile headers are staged in LDS; B's
 // values (used about once each per row) are read from memory in the output
-// loop, which keeps LDS per wave small and occupancy high.
+// loop, which keeps LDS per wave small and occupancy high.  B's tiles come
+// column by column (column masks, offsets, values), so for output (r, c) and
+// pair (A tile, B tile) the contributing k are exactly the set bits of
+// (A row mask r) & (B column mask c): the loop runs once per product.
 template <typename V, typename Var>
 __global__ void __launch_bounds__(Var::WAVES *WAVE)
 k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
              const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA,
              const uint8_t *__restrict__ ptrA, const uint16_t *__restrict__ maskA, const V *__restrict__ valA,
              const int4 *__restrict__ link, const int32_t *__restrict__ colidxB,
-             const int64_t *__restrict__ csr_nnzB, const uint8_t *__restrict__ csr_ptrB,
-             const uint16_t *__restrict__ csr_maskB, const V *__restrict__ csr_valB,
+             const int64_t *__restrict__ csr_nnzB, const uint8_t *__restrict__ cptrB,
+             const uint16_t *__restrict__ cmaskB, const V *__restrict__ cvalB,
              const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ colidxC,
              const int64_t *__restrict__ tile_nnzC, const uint16_t *__restrict__ maskC, V *__restrict__ valC,
              uint8_t *__restrict__ colC, int32_t *__restrict__ overflow, const int32_t *__restrict__ row_nc,
@@ -548,8 +551,8 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         const int32_t bjv = colidxB[q];
         const int64_t bg = csr_nnzB[q];
         uint4 bm0, bm1;
-        const uint4 bp = load_ptr16(csr_ptrB, q);
-        load_mask16(csr_maskB, q, bm0, bm1);
+        const uint4 bp = load_ptr16(cptrB, q);
+        load_mask16(cmaskB, q, bm0, bm1);
         int lo = 0, hi = nc;  // C tile of this B tile
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
@@ -558,8 +561,8 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         ct_b = lo;
         s.bgoff[lane] = bg;
         s.bown[lane] = (uint8_t)bt;
-        reinterpret_cast<uint4 *>(s.bptr)[lane] = bp;
-        store_mask16(&s.bmask[lane * 16], bm0, bm1);
+        reinterpret_cast<uint4 *>(s.bcptr)[lane] = bp;
+        store_mask16(&s.bcmask[lane * 16], bm0, bm1);
         atomicOr(&s.tmask[ct_b], 1u << bt);
     }
 #pragma unroll
@@ -600,25 +603,22 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         for (int o = lane; o < nout; o += WAVE) {
             const int d = s.desc[o];
             const int ct = ct0 + (d >> 8), r = (d >> 4) & 15, c = d & 15;
-            const uint32_t cbit = 0x8000u >> c, cbelow = below(c);
             V acc = V(0);
             const int pe = s.pstart[ct + 1];
             for (int p = s.pstart[ct]; p < pe; p++) {
                 const int b = s.plist[p], t = s.bown[b];
                 const uint32_t ma = s.amask[t * 16 + r];
-                uint32_t m = ma;
+                const uint32_t cm = s.bcmask[b * 16 + c];
+                uint32_t m = ma & cm;  // k of the products, ascending from the top bit
+                if (!m) continue;
                 const int ea = s.aoff[t] + s.aptr[t * 16 + r];
-                const int64_t bb = s.bgoff[b];
-                while (m) {
+                const V *bv = cvalB + (s.bgoff[b] + s.bcptr[b * 16 + c]);
+                do {
                     const int kk = __clz(m) - 16;
                     m ^= 0x8000u >> kk;
-                    const uint32_t mb = s.bmask[b * 16 + kk];
-                    if (mb & cbit) {
-                        const V va = s.aval[ea + __popc(ma & below(kk))];
-                        const V vb = csr_valB[bb + s.bptr[b * 16 + kk] + __popc(mb & cbelow)];
-                        acc = fma_v<V>(va, vb, acc);
-                    }
-                }
+                    const uint32_t up = below(kk);
+                    acc = fma_v<V>(s.aval[ea + __popc(ma & up)], bv[__popc(cm & up)], acc);
+                } while (m);
             }
             valC[obase + o] = acc;
             colC[obase + o] = (uint8_t)((r << 4) | c);
@@ -631,7 +631,7 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
 // ---------------------------------------------------------------- host side
 bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
     if (opts && opts->numeric_variant == 1) return false;  // reference method everywhere
-    return B->csr_tile_nnz && B->csr_tile_Ptr && B->csr_mask && B->csr_tile_Val && B->row_nnz_ptr &&
+    return B->csr_tile_nnz && B->csr_mask && B->csr_col_mask && B->csr_col_Ptr && B->csr_col_Val && B->row_nnz_ptr &&
            !(A->flags & 1) && !(B->flags & 1);
 }
 
@@ -739,8 +739,8 @@ static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
     if (n <= 0) return TSG_OK;
     k_num_staged<V, Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
         p.rows, p.bin_off[bin], n, A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
-        p.link, B->tile_columnidx, B->csr_tile_nnz, B->csr_tile_Ptr, B->csr_mask,
-        (const V *)B->csr_tile_Val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, (V *)C->tile_csr_Val,
+        p.link, B->tile_columnidx, B->csr_tile_nnz, B->csr_col_Ptr, B->csr_col_mask,
+        (const V *)B->csr_col_Val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, (V *)C->tile_csr_Val,
         C->tile_csr_Col, p.overflow, sl.row_nc, sl.val_cap);
     TSG_HIP(hipGetLastError());
     return TSG_OK;

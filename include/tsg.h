@@ -84,11 +84,13 @@ typedef struct tsg_tile_matrix {
     int32_t *csc_tile_rowidx; /* [numtile] (B only) */
     /* B only, optional: the same tiles in CSR-tile order, kept by
      * tsg_build_csc_tiles so that a B tile-row is one contiguous range for the
-     * row-staged kernels.  NULL when absent (the C-tile kernels are used). */
-    int64_t *csr_tile_nnz;    /* [numtile+1] */
-    uint8_t *csr_tile_Ptr;    /* [numtile*16] */
-    uint16_t *csr_mask;       /* [numtile*16] */
-    void *csr_tile_Val;       /* [nnz] */
+     * row-staged kernels, with each tile's values stored column by column.
+     * NULL when absent (the C-tile kernels are used). */
+    int64_t *csr_tile_nnz;    /* [numtile+1] exclusive prefix (CSR-tile order) */
+    uint16_t *csr_mask;       /* [numtile*16] row masks, bit (15-c) = column c */
+    uint16_t *csr_col_mask;   /* [numtile*16] column masks, bit (15-r) = row r */
+    uint8_t *csr_col_Ptr;     /* [numtile*16] exclusive column offsets inside each tile */
+    void *csr_col_Val;        /* [nnz] values, column-major inside each tile */
     int64_t *row_nnz_ptr;     /* [tilem+1] first payload entry of each tile-row (= CSR rowptr[16*i]) */
 } tsg_tile_matrix;
 

@@ -99,3 +99,37 @@ def test_gpu_tile2csr_roundtrip(ctx, name):
     B = t.to_csr()
     assert np.array_equal(B.rowptr, A.rowptr) and np.array_equal(B.col, A.col)
     assert np.array_equal(B.val, A.val)
+
+
+def colmajor_expected(t):
+    """Column-major copy of each row-ordered tile from the oracle's csr2tile
+    output: entries re-sorted by (tile, column, row)."""
+    nt, nnz = t["numtile"], t["nnz"]
+    tn, ptr = np.asarray(t["tile_nnz"], np.int64), np.asarray(t["tile_csr_Ptr"], np.int64).reshape(nt, 16)
+    tile = np.repeat(np.arange(nt), np.diff(tn))
+    local = np.arange(nnz) - tn[tile]
+    # row of each entry: last r with ptr[tile, r] <= local
+    row = (ptr[tile] <= local[:, None]).sum(1) - 1
+    col = np.asarray(t["tile_csr_Col"], np.int64) & 15
+    order = np.lexsort((row, col, tile))
+    cv = np.asarray(t["tile_csr_Val"])[order]
+    cm = np.zeros(nt * 16, np.uint16)
+    np.bitwise_or.at(cm, tile * 16 + col, (0x8000 >> row).astype(np.uint16))
+    cnt = np.bincount(tile * 16 + col, minlength=nt * 16).reshape(nt, 16)
+    cp = (np.cumsum(cnt, 1) - cnt).astype(np.uint8).ravel()
+    return cm, cp, cv
+
+
+@pytest.mark.parametrize("name", ["er4096", "lap2d_64x48", "rmat10"])
+def test_gpu_b_column_major_copy(ctx, name):
+    A = _matrices()[name]()
+    print(f"[case] column-major B {name}", flush=True)
+    b = ctx.tiles_from_csr(A, as_b=True).download()
+    t = O.csr2tile(A)
+    assert np.array_equal(b["csr_tile_nnz"], t["tile_nnz"]) and np.array_equal(b["csr_mask"], t["mask"])
+    cm, cp, cv = colmajor_expected(t)
+    assert np.array_equal(b["csr_col_mask"], cm)
+    assert np.array_equal(b["csr_col_Ptr"], cp)
+    assert np.array_equal(b["csr_col_Val"], cv)
+    rp = np.asarray(A.rowptr)
+    assert np.array_equal(b["row_nnz_ptr"], rp[np.minimum(np.arange(t["tilem"] + 1) * 16, A.m)])
