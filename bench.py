@@ -106,6 +106,56 @@ def profiled_traffic(workload, kernel):
     return None
 
 
+def bench_batch(g, args):
+    """512 uneven independent products A_i*A_i (FP64): one batched call vs a
+    loop of single calls over the same resident operands."""
+    t0 = time.time()
+    probs = g.gen_batch_uneven(512, seed=4)
+    products = sum(A.products(A) for _, A in probs)
+    ctx = g.Context(0)
+    log(f"batch512: {len(probs)} problems, rows {sum(A.m for _, A in probs)}, "
+        f"products {products}, generated in {time.time() - t0:.1f} s")
+    tA = [ctx.tiles_from_csr(A) for _, A in probs]
+    tB = [ctx.tiles_from_csr(A, as_b=True) for _, A in probs]
+
+    def loop():
+        for a, b in zip(tA, tB):
+            C, _ = ctx.spgemm(a, b)
+            C.free()
+
+    def batched():
+        Cs, _ = ctx.spgemm_batched(tA, tB)
+        for C in Cs:
+            C.free()
+
+    res = {}
+    for label, fn in (("loop", loop), ("batched", batched)):
+        for _ in range(args.warmup):
+            fn()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        ctx.synchronize()
+        res[label] = (time.perf_counter() - t1) * 1e3 / args.steps
+        log(f"{label}: {res[label]:.3f} ms per batch")
+    ms = res["batched"]
+    out = {
+        "metric": "SpGEMM GFLOP/s + HBM GB/s, learned/batched vs plain tiled, 1 MI355X",
+        "value": round(2.0 * products / (ms * 1e6), 3), "unit": "GFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (generated on the host, converted to tiles on the GPU before timing)",
+        "config": {"workload": "batch512_uneven_AixAi", "problems": len(probs), "products": products,
+                   "seed": 4, "scheduler": "batched", "parallelism": "single GPU"},
+        "variants_ms_per_step": {k: round(v, 4) for k, v in res.items()},
+        "batched_speedup_vs_loop": round(res["loop"] / res["batched"], 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def cpu_baseline(A, B, seconds):
     from oracle import oracle as O
     m = A.m
@@ -147,6 +197,8 @@ def main():
         sys.exit(2)
 
     import gpu_spgemm_amd as g
+    if args.workload == "batch512":
+        return bench_batch(g, args)
     A, B, cfg = make_workload(g, args.workload)
     products = A.products(B)
     log(f"workload {cfg['workload']}: products={products}")

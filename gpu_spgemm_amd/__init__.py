@@ -44,7 +44,8 @@ class TileStruct(C.Structure):
                 ("csc_tile_ptr", C.POINTER(C.c_int64)), ("csc_tile_rowidx", C.POINTER(C.c_int32)),
                 ("csr_tile_nnz", C.POINTER(C.c_int64)), ("csr_mask", C.POINTER(C.c_uint16)),
                 ("csr_col_mask", C.POINTER(C.c_uint16)), ("csr_col_Ptr", C.POINTER(C.c_uint8)),
-                ("csr_col_Val", C.c_void_p), ("row_nnz_ptr", C.POINTER(C.c_int64))]
+                ("csr_col_Val", C.c_void_p), ("row_nnz_ptr", C.POINTER(C.c_int64)),
+                ("row_ptr", C.POINTER(C.c_int64)), ("row_col", C.POINTER(C.c_int32)), ("row_val", C.c_void_p)]
 
 
 class Options(C.Structure):
@@ -187,6 +188,27 @@ def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, dtype=F32):
                 C.c_double(b), C.c_double(c), C.c_uint64(seed), C.c_int32(dtype))
 
 
+def gen_batch_uneven(count=512, seed=4, lo=256, hi=65536):
+    """The uneven batch: count problems, n_i = round(exp(U(ln lo, ln hi)));
+    even i: 7-point 3D Laplacian on a near-cubic nx*ny*nz grid (values 6/-1),
+    odd i: Erdos-Renyi with nnz/row drawn uniformly from {2..32} per matrix
+    (values U[-1,1]); sizes and draws from numpy's default_rng(seed).
+    Returns [(name, HostCSR)]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+        if i % 2 == 0:
+            nz = max(1, int(round(n ** (1.0 / 3.0))))
+            ny = max(1, int(round((n / nz) ** 0.5)))
+            nx = max(1, int(round(n / (nz * ny))))
+            out.append((f"lap3d_{nx}x{ny}x{nz}", gen_laplace3d(nx, ny, nz)))
+        else:
+            per = int(rng.integers(2, 33))
+            out.append((f"er_{n}_{per}", gen_er(n, per, seed=1000 + i)))
+    return out
+
+
 def load_mtx(path):
     """MTX reader with load_mtx semantics (src/tile_format.h:51-90)."""
     return _gen(lib().tsg_load_mtx, os.fsencode(path))
@@ -267,6 +289,24 @@ class Context:
         _check(lib().tsg_spgemm(self._p, C.byref(A.s), C.byref(B.s), C.byref(Cm.s), C.byref(o),
                                 C.byref(st)))
         return Cm, st.as_dict()
+
+    def spgemm_batched(self, As, Bs, scheduler=SCHED_PLAIN, timing=False, variant=0, pred_scale=1.0,
+                       mfma_min_products=0):
+        """C_i = A_i*B_i for every pair in one pipeline run; returns ([C_i], [stats_i])."""
+        n = len(As)
+        assert len(Bs) == n
+        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale)
+        a = (TileStruct * max(n, 1))(*[x.s for x in As])
+        b = (TileStruct * max(n, 1))(*[x.s for x in Bs])
+        c = (TileStruct * max(n, 1))()
+        st = (Stats * max(n, 1))()
+        _check(lib().tsg_spgemm_batched(self._p, C.c_int32(n), a, b, c, C.byref(o), st))
+        out = []
+        for i in range(n):
+            t = DeviceTiles(self)
+            t.s = TileStruct.from_buffer_copy(c[i])
+            out.append(t)
+        return out, [st[i].as_dict() for i in range(n)]
 
     # ---- learned scheduler (MLP 8 -> 32 -> 32 -> 2 on the GPU)
     def learned_collect(self, A, B):

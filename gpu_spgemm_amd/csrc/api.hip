@@ -250,6 +250,12 @@ int tsg_tile_upload(tsg_context *c, const tsg_tile_matrix *h, tsg_tile_matrix *d
     TSG_CHECK(up(ctx, h->mask, nt * 16, &t.mask));
     t.csr_tile_nnz = nullptr; t.csr_mask = nullptr; t.csr_col_mask = nullptr; t.csr_col_Ptr = nullptr;
     t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
+    t.row_ptr = nullptr; t.row_col = nullptr; t.row_val = nullptr;
+    if (h->row_ptr && h->row_col && h->row_val) {
+        TSG_CHECK(up(ctx, h->row_ptr, (size_t)h->m + 1, &t.row_ptr));
+        TSG_CHECK(up(ctx, h->row_col, (size_t)h->nnz, &t.row_col));
+        TSG_CHECK(up(ctx, (const uint8_t *)h->row_val, (size_t)h->nnz * vsize(h->dtype), (uint8_t **)&t.row_val));
+    }
     if (h->is_csc) {
         if (!h->csc_tile_ptr || !h->csc_tile_rowidx) return fail(TSG_ERR_ARG, "B operand lacks CSC tile arrays");
         TSG_CHECK(up(ctx, h->csc_tile_ptr, (size_t)h->tilen + 1, &t.csc_tile_ptr));
@@ -287,6 +293,12 @@ int tsg_tile_download(tsg_context *c, const tsg_tile_matrix *d, tsg_tile_matrix 
     TSG_CHECK(down(ctx, d->mask, nt * 16, &t.mask));
     t.csr_tile_nnz = nullptr; t.csr_mask = nullptr; t.csr_col_mask = nullptr; t.csr_col_Ptr = nullptr;
     t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
+    t.row_ptr = nullptr; t.row_col = nullptr; t.row_val = nullptr;
+    if (d->row_ptr && d->row_col && d->row_val) {
+        TSG_CHECK(down(ctx, d->row_ptr, (size_t)d->m + 1, &t.row_ptr));
+        TSG_CHECK(down(ctx, d->row_col, (size_t)d->nnz, &t.row_col));
+        TSG_CHECK(down(ctx, (const uint8_t *)d->row_val, (size_t)d->nnz * vsize(d->dtype), (uint8_t **)&t.row_val));
+    }
     if (d->is_csc) {
         TSG_CHECK(down(ctx, d->csc_tile_ptr, (size_t)d->tilen + 1, &t.csc_tile_ptr));
         TSG_CHECK(down(ctx, d->csc_tile_rowidx, nt, &t.csc_tile_rowidx));
@@ -310,7 +322,8 @@ int tsg_tile_free(tsg_context *c, tsg_tile_matrix *d) {
     if (d->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles are not on the device");
     void *ps[] = {d->tile_ptr, d->tile_columnidx, d->tile_rowidx, d->tile_nnz, d->tile_csr_Val,
                   d->tile_csr_Col, d->tile_csr_Ptr, d->mask, d->csc_tile_ptr, d->csc_tile_rowidx,
-                  d->csr_tile_nnz, d->csr_mask, d->csr_col_mask, d->csr_col_Ptr, d->csr_col_Val, d->row_nnz_ptr};
+                  d->csr_tile_nnz, d->csr_mask, d->csr_col_mask, d->csr_col_Ptr, d->csr_col_Val, d->row_nnz_ptr,
+                  d->row_ptr, d->row_col, d->row_val};
     for (void *p : ps) pool_free(ctx, p);
     memset(d, 0, sizeof *d);
     return TSG_OK;
@@ -321,7 +334,8 @@ int tsg_tile_free_host(tsg_tile_matrix *h) {
     if (h->location != TSG_HOST) return fail(TSG_ERR_ARG, "tiles are not on the host");
     void *ps[] = {h->tile_ptr, h->tile_columnidx, h->tile_rowidx, h->tile_nnz, h->tile_csr_Val,
                   h->tile_csr_Col, h->tile_csr_Ptr, h->mask, h->csc_tile_ptr, h->csc_tile_rowidx,
-                  h->csr_tile_nnz, h->csr_mask, h->csr_col_mask, h->csr_col_Ptr, h->csr_col_Val, h->row_nnz_ptr};
+                  h->csr_tile_nnz, h->csr_mask, h->csr_col_mask, h->csr_col_Ptr, h->csr_col_Val, h->row_nnz_ptr,
+                  h->row_ptr, h->row_col, h->row_val};
     for (void *p : ps) free(p);
     memset(h, 0, sizeof *h);
     return TSG_OK;
@@ -419,14 +433,21 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
 
 }  // namespace tsg
 
+namespace tsg {
+int spgemm_dispatch(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                    const tsg_options *opts, tsg_stats *st) {
+    if (st) memset(st, 0, sizeof *st);
+    if (opts && opts->scheduler == TSG_SCHED_LEARNED) return spgemm_learned(ctx, A, B, C, opts, st);
+    if (opts && opts->numeric_variant == 1) return spgemm_plain(ctx, A, B, C, opts, st);
+    return spgemm_binned(ctx, A, B, C, opts, st);
+}
+}  // namespace tsg
+
 extern "C" int tsg_spgemm(tsg_context *c, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                           tsg_tile_matrix *C, const tsg_options *opts, tsg_stats *st) {
     Context *ctx = reinterpret_cast<Context *>(c);
     if (!ctx || !C) return fail(TSG_ERR_ARG, "null argument");
     TSG_CHECK(check_operands(A, B));
     DeviceGuard g(ctx->device);
-    if (st) memset(st, 0, sizeof *st);
-    if (opts && opts->scheduler == TSG_SCHED_LEARNED) return spgemm_learned(ctx, A, B, C, opts, st);
-    if (opts && opts->numeric_variant == 1) return spgemm_plain(ctx, A, B, C, opts, st);
-    return spgemm_binned(ctx, A, B, C, opts, st);
+    return spgemm_dispatch(ctx, A, B, C, opts, st);
 }

@@ -356,7 +356,7 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
     auto cleanup = [&](int code) {
         pool_free(ctx, flags);
         void *ps[] = {T->tile_ptr, T->tile_columnidx, T->tile_nnz, T->tile_csr_Val, T->tile_csr_Col,
-                      T->tile_csr_Ptr, T->mask};
+                      T->tile_csr_Ptr, T->mask, T->row_ptr, T->row_col, T->row_val};
         for (void *p : ps) pool_free(ctx, p);
         memset(T, 0, sizeof *T);
         return code;
@@ -409,6 +409,18 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
                 T->tile_nnz, T->tile_csr_Ptr, (double *)T->tile_csr_Val, T->tile_csr_Col);
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "scatter launch"));
     }
+    // keep the scalar CSR beside the tiles (row-by-row product of sparse heavy rows)
+    const size_t vb = (size_t)T->nnz * vsize(T->dtype);
+    if ((rc = palloc(ctx, &T->row_ptr, (size_t)T->m + 1))) return cleanup(rc);
+    if ((rc = palloc(ctx, &T->row_col, (size_t)T->nnz))) return cleanup(rc);
+    if ((rc = pool_alloc(ctx, &T->row_val, vb))) return cleanup(rc);
+    if (hipMemcpyAsync(T->row_ptr, A->rowptr, sizeof(int64_t) * ((size_t)T->m + 1), hipMemcpyDeviceToDevice, s) !=
+            hipSuccess ||
+        (T->nnz > 0 &&
+         (hipMemcpyAsync(T->row_col, A->col, sizeof(int32_t) * (size_t)T->nnz, hipMemcpyDeviceToDevice, s) !=
+              hipSuccess ||
+          hipMemcpyAsync(T->row_val, A->val, vb, hipMemcpyDeviceToDevice, s) != hipSuccess)))
+        return cleanup(fail(TSG_ERR_HIP, "copy of the scalar CSR failed"));
     if ((rc = read_i32(ctx, flags, &hf))) return cleanup(rc);
     T->flags = (hf & 2) ? 1 : 0;
     pool_free(ctx, flags);

@@ -288,6 +288,7 @@ k_learned_plan(int32_t tilem, const int64_t *__restrict__ tile_ptrA, const int64
         const StageCaps c = bt.v[v];
         if (na <= c.na && cap <= c.nc && nb <= c.nb && pa <= c.pa && pb <= c.pb) { b = v; break; }
     }
+    if (b == bt.nvar && bt.rg && pa <= (int64_t)bt.rg_density * na) b = bt.nvar + 1;
     bin[bi] = (uint8_t)b;
 }
 
@@ -388,8 +389,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     C->m = A->m; C->n = B->n; C->tilem = A->tilem; C->tilen = B->tilen;
     C->dtype = A->dtype; C->location = TSG_DEVICE;
     const int32_t tilem = A->tilem;
-    const bool ok = staged_ok(A, B, opts);
-    const BinTable bt = bin_table(ok);
+    const BinTable bt = bin_table_for(A, B, opts);
     Plan p;
     int64_t *slotp = nullptr, *valc = nullptr, *expect = nullptr, *res = nullptr;
     int32_t *row_nc = nullptr, *pcol = nullptr, *prow = nullptr;
@@ -403,7 +403,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
         for (void *q : ps) pool_free(ctx, q);
         free_plan(ctx, p);
     };
-    int rc = plan_setup(ctx, A, B, ok, p);
+    int rc = plan_setup(ctx, A, B, bt, p);
     if (!rc) rc = palloc(ctx, &slotp, (size_t)tilem + 1);
     if (!rc) rc = palloc(ctx, &valc, (size_t)tilem + 1);
     if (!rc) rc = palloc(ctx, &expect, MAX_BINS + 3);
@@ -659,7 +659,8 @@ static int features_or_predict(tsg_context *c, const tsg_tile_matrix *A, const t
     Plan p;
     float *X = nullptr, *Y = nullptr;
     int64_t *slotp = nullptr, *valc = nullptr;
-    int rc = plan_setup(ctx, A, B, staged_ok(A, B, nullptr), p);
+    const BinTable bt = bin_table_for(A, B, nullptr);
+    int rc = plan_setup(ctx, A, B, bt, p);
     if (!rc) rc = palloc(ctx, &X, (size_t)std::max(tilem, 1) * NF);
     if (!rc) rc = palloc(ctx, &Y, (size_t)std::max(tilem, 1) * NO);
     if (!rc) rc = palloc(ctx, &slotp, (size_t)tilem + 1);
@@ -667,7 +668,7 @@ static int features_or_predict(tsg_context *c, const tsg_tile_matrix *A, const t
     if (!rc && tilem > 0) {
         k_learned_plan<<<nblk(tilem, 256), 256, 0, ctx->stream>>>(
             tilem, A->tile_ptr, A->tile_nnz, p.link, B->tile_columnidx, pred ? L->w : nullptr,
-            bin_table(staged_ok(A, B, nullptr)), L->off_nc, L->off_nnz, 1.0f, 1, X, Y, slotp, valc, p.bin);
+            bt, L->off_nc, L->off_nnz, 1.0f, 1, X, Y, slotp, valc, p.bin);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_learned_plan launch");
     }
     if (!rc && feats && tilem > 0 &&
@@ -704,7 +705,7 @@ int tsg_learned_collect(tsg_context *c, const tsg_tile_matrix *A, const tsg_tile
     Plan p;
     float *X = nullptr, *T = nullptr;
     int64_t *slotp = nullptr, *valc = nullptr;
-    int rc = plan_setup(ctx, A, B, staged_ok(A, B, nullptr), p);
+    int rc = plan_setup(ctx, A, B, bin_table_for(A, B, nullptr), p);
     if (!rc) rc = palloc(ctx, &X, (size_t)tilem * NF);
     if (!rc) rc = palloc(ctx, &T, (size_t)tilem * NO);
     if (!rc) rc = palloc(ctx, &slotp, (size_t)tilem + 1);
@@ -896,10 +897,5 @@ int tsg_learned_gradient(tsg_context *c, const float *X, const float *T, int64_t
     return rc;
 }
 
-int tsg_spgemm_batched(tsg_context *c, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
-                       tsg_tile_matrix *C, const tsg_options *opts, tsg_stats *stats) {
-    (void)c; (void)count; (void)A; (void)B; (void)C; (void)opts; (void)stats;
-    return tsg::fail(TSG_ERR_UNSUPPORTED, "batched launch not available in this build");
-}
 
 }  // extern "C"

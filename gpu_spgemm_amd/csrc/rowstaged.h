@@ -5,17 +5,24 @@
 
 namespace tsg {
 
-constexpr int MAX_BINS = 4;
+constexpr int MAX_VAR = 4;               // staged variants
+constexpr int MAX_BINS = MAX_VAR + 2;    // + C-tile kernels + row-by-row kernels
 
 // LDS capacities of one staged kernel variant; a tile-row is staged by the
-// first variant whose capacities hold it, else it runs the C-tile kernels.
+// first variant whose capacities hold it.  Other rows run the row-by-row
+// kernels when enabled and their A tiles hold at most rg_density entries on
+// average, else the C-tile kernels.  Bins: 0..nvar-1 staged, nvar C-tile,
+// nvar+1 row-by-row.
 struct StageCaps {
     int32_t na, nc, nb, pa, pb;
 };
 struct BinTable {
-    StageCaps v[MAX_BINS];
+    StageCaps v[MAX_VAR];
     int32_t nvar;
+    int32_t rg;          // 1: the row-by-row bin exists
+    int32_t rg_density;  // A entries per A tile at or below which a heavy row goes row-by-row
 };
+constexpr int RG_DENSITY_DEFAULT = 8;
 
 // Per-call working set: A->B links, per-row bins, bin-ordered row lists.
 struct Plan {
@@ -28,13 +35,19 @@ struct Plan {
     int64_t *lpay = nullptr;     // per A tile: B tile-row payload start
     int64_t bin_off[MAX_BINS + 1] = {};
     int nbins = 0;
+    int nvar = 0;     // staged bins; bin nvar = C-tile kernels
+    bool rg = false;  // bin nvar + 1 = row-by-row kernels
 };
 
 bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
-BinTable bin_table(bool ok);
+// The row-by-row kernels need the scalar CSR of A and B and no duplicate columns.
+bool rg_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
+BinTable bin_table(bool ok, bool rg = false, int rg_density = RG_DENSITY_DEFAULT);
+// The bin table of a product under its options.
+BinTable bin_table_for(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
 void free_plan(Context *ctx, Plan &p);
 // Allocates the plan's arrays and computes the A->B links.
-int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, bool ok, Plan &p);
+int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const BinTable &bt, Plan &p);
 // Partitions rows by p.bin and gathers hdr = [*total0, *total1 (or 0), bin offsets].
 int plan_bin(Context *ctx, Plan &p, int32_t tilem, const int64_t *total0, const int64_t *total1);
 // Reads hdr to the host: totals and bin offsets (one blocking read).

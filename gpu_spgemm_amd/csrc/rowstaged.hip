@@ -208,6 +208,7 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
             const StageCaps c = bt.v[v];
             if (na <= c.na && count <= c.nc && nb <= c.nb && pa <= c.pa && pb <= c.pb) { b = v; break; }
         }
+        if (b == bt.nvar && bt.rg && pa <= (int64_t)bt.rg_density * na) b = bt.nvar + 1;
         bin[bi] = (uint8_t)b;
         if (stats) {  // per-row features for the learned scheduler
             int32_t *st = stats + (int64_t)bi * 8;
@@ -635,9 +636,21 @@ bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_opt
            !(A->flags & 1) && !(B->flags & 1);
 }
 
-BinTable bin_table(bool ok) {
+bool rg_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
+    if (opts && opts->numeric_variant == 1) return false;  // reference method everywhere
+    return A->row_ptr && A->row_col && A->row_val && B->row_ptr && B->row_col && B->row_val && !(A->flags & 1) &&
+           !(B->flags & 1);
+}
+
+BinTable bin_table_for(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
+    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts));
+}
+
+BinTable bin_table(bool ok, bool rg, int rg_density) {
     BinTable t{};
     t.nvar = 0;
+    t.rg = rg ? 1 : 0;
+    t.rg_density = rg_density;
     if (ok) {
         t.v[0] = caps_of<VarS>();
         t.v[1] = caps_of<VarW>();
@@ -655,10 +668,13 @@ void free_plan(Context *ctx, Plan &p) {
 
 static unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, bool ok, Plan &p) {
+int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const BinTable &bt, Plan &p) {
     const int64_t tilem = A->tilem;
     const int64_t nb = (tilem + BIN_TILE - 1) / BIN_TILE;
-    p.nbins = bin_table(ok).nvar + 1;
+    const bool ok = bt.nvar > 0;
+    p.nvar = bt.nvar;
+    p.rg = bt.rg != 0;
+    p.nbins = bt.nvar + 1 + (p.rg ? 1 : 0);
     TSG_CHECK(palloc(ctx, &p.rows, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.bin, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.counts, (size_t)(p.nbins * nb + 1)));
@@ -704,9 +720,8 @@ int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1) {
 // and one host read of (numtileC, bin offsets).
 static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                      const tsg_options *opts, Plan &p, int32_t *stats_out) {
-    const bool ok = staged_ok(A, B, opts);
-    const BinTable bt = bin_table(ok);
-    TSG_CHECK(plan_setup(ctx, A, B, ok, p));
+    const BinTable bt = bin_table_for(A, B, opts);
+    TSG_CHECK(plan_setup(ctx, A, B, bt, p));
     const int32_t tilem = A->tilem;
     if (tilem == 0) {
         TSG_HIP(hipMemsetAsync(C->tile_ptr, 0, sizeof(int64_t), ctx->stream));
@@ -749,8 +764,8 @@ static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
 // Structure fill + symbolic for every bin.
 int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                  const Plan &p, const SlotLayout &sl) {
-    const int fb = p.nbins - 1;  // last bin: C-tile kernels
-    if (p.nbins > 1) {
+    const int fb = p.nvar;  // C-tile kernels
+    if (p.nvar > 0) {
         TSG_CHECK(launch_sym_staged<VarS>(ctx, A, B, C, p, 0, sl));
         TSG_CHECK(launch_sym_staged<VarW>(ctx, A, B, C, p, 1, sl));
         TSG_CHECK(launch_sym_staged<VarM>(ctx, A, B, C, p, 2, sl));
@@ -758,13 +773,18 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
     const int64_t nf = p.bin_off[fb + 1] - p.bin_off[fb];
     TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb], nf, sl));
     TSG_CHECK(launch_symbolic_list(ctx, A, B, p.rows, p.bin_off[fb], nf, C, sl));
+    if (p.rg) {  // row-by-row kernels: Step 1 fill as the C-tile rows, then masks from the products
+        const int64_t nr = p.bin_off[fb + 2] - p.bin_off[fb + 1];
+        TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, sl));
+        TSG_CHECK(launch_rg_symbolic(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, sl));
+    }
     return TSG_OK;
 }
 
 int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                 const Plan &p, const SlotLayout &sl) {
-    const int fb = p.nbins - 1;
-    if (p.nbins > 1) {
+    const int fb = p.nvar;
+    if (p.nvar > 0) {
         if (A->dtype == TSG_F32) {
             TSG_CHECK((launch_num_staged<float, VarS>(ctx, A, B, C, p, 0, sl)));
             TSG_CHECK((launch_num_staged<float, VarW>(ctx, A, B, C, p, 1, sl)));
@@ -776,6 +796,8 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
         }
     }
     TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, p.bin_off[fb], p.bin_off[fb + 1] - p.bin_off[fb], sl));
+    if (p.rg)
+        TSG_CHECK(launch_rg_numeric(ctx, A, B, C, p.rows, p.bin_off[fb + 1], p.bin_off[fb + 2] - p.bin_off[fb + 1], sl));
     return TSG_OK;
 }
 

@@ -105,6 +105,11 @@ int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
 int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                         const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
 void learned_release(Context *ctx);  // learned.hip
+// Row-by-row kernels for sparse heavy tile-rows (heavy.hip).
+int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                       const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
+int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                      const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
 int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B);  // api.hip
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                    tsg_tile_matrix *C, const tsg_options *opts);

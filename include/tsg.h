@@ -92,6 +92,12 @@ typedef struct tsg_tile_matrix {
     uint8_t *csr_col_Ptr;     /* [numtile*16] exclusive column offsets inside each tile */
     void *csr_col_Val;        /* [nnz] values, column-major inside each tile */
     int64_t *row_nnz_ptr;     /* [tilem+1] first payload entry of each tile-row (= CSR rowptr[16*i]) */
+    /* Optional: the same matrix in scalar CSR (columns ascending in each row),
+     * kept by tsg_csr2tile.  Heavy tile-rows whose tiles are sparse run a
+     * row-by-row product on it; NULL when absent (C outputs). */
+    int64_t *row_ptr;         /* [m+1] */
+    int32_t *row_col;         /* [nnz] */
+    void *row_val;            /* [nnz] */
 } tsg_tile_matrix;
 
 typedef struct tsg_context tsg_context;

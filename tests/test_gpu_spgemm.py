@@ -206,3 +206,41 @@ def test_shape_mismatch_is_an_error(ctx):
     tB2 = ctx.tiles_from_csr(A)  # not laid out as B
     with pytest.raises(g.TsgError):
         ctx.spgemm(tA, tB2)
+
+
+def banded(n, half, seed=0):
+    """Dense band |i-j| <= half: heavy tile-rows with full tiles."""
+    rows, cols = [], []
+    for i in range(n):
+        lo, hi = max(0, i - half), min(n, i + half + 1)
+        rows.append(np.arange(lo, hi))
+    col = np.concatenate(rows).astype(np.int32)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    val = np.random.default_rng(seed).uniform(-1, 1, len(col))
+    return g.HostCSR(n, n, rp, col, val)
+
+
+@pytest.mark.parametrize("name", ["er20000x20", "rmat12", "er_rect", "band"])
+@pytest.mark.parametrize("f32", [False, True])
+def test_heavy_rows_paths_agree(ctx, name, f32):
+    """Heavy tile-rows: sparse tiles go row-by-row (scalar CSR walk), full
+    tiles to the C-tile kernels; both bit-identical to the reference method
+    and the oracle."""
+    if name == "er20000x20":
+        A = g.gen_er(20000, 20, seed=5); B = A
+    elif name == "rmat12":
+        A = g.gen_rmat(12, 16, seed=3, dtype=g.F64); B = A
+    elif name == "er_rect":
+        A = g.gen_er(3000, 24, seed=6)
+        Bt = g.gen_er(5000, 30, seed=7)             # 5000 x 5000 -> take a 3000 x 5000 B
+        S = Bt.to_scipy()[:3000].tocsr()
+        B = g.HostCSR(3000, 5000, S.indptr, S.indices, S.data)
+    else:
+        A = banded(1500, 300); B = A
+    got0, s0 = run(ctx, A, B, f32=f32, label=f"{name} binned", variant=0)
+    got1, _ = run(ctx, A, B, f32=f32, label=f"{name} ctile", variant=1)
+    assert_tiles_equal(got0, got1, keys=["tile_ptr", "tile_columnidx", "tile_rowidx", "tile_nnz", "tile_csr_Ptr",
+                                          "mask", "tile_csr_Col", "tile_csr_Val"], label="C ")
+    if name != "er20000x20":
+        check(got0, A, B, f32=f32)
