@@ -128,6 +128,11 @@ def bench_batch(g, args):
         for C in Cs:
             C.free()
 
+    t1 = time.time()
+    best, rep = ctx.autotune(tA, tB, reps=1)
+    log(f"autotune ({len(rep)} configurations, {time.time() - t1:.1f} s): best var_mask {best['var_mask']} "
+        f"rg_density {best['rg_density']} {best['ms']:.1f} ms; " +
+        " ".join(f"{r['var_mask']}/{r['rg_density']}:{r['ms']:.0f}" for r in rep))
     res = {}
     for label, fn in (("loop", loop), ("batched", batched)):
         for _ in range(args.warmup):
@@ -150,6 +155,8 @@ def bench_batch(g, args):
                    "seed": 4, "scheduler": "batched", "parallelism": "single GPU"},
         "variants_ms_per_step": {k: round(v, 4) for k, v in res.items()},
         "batched_speedup_vs_loop": round(res["loop"] / res["batched"], 3),
+        "autotuned": {"var_mask": best["var_mask"], "rg_density": best["rg_density"],
+                      "candidates": len(rep)},
         "roofline": None, "cpu_baseline": None,
     }
     ctx.close()

@@ -50,7 +50,15 @@ class TileStruct(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("scheduler", C.c_int32), ("numeric_variant", C.c_int32),
-                ("mfma_min_products", C.c_int32), ("timing", C.c_int32), ("pred_scale", C.c_double)]
+                ("mfma_min_products", C.c_int32), ("timing", C.c_int32), ("pred_scale", C.c_double),
+                ("tuning", C.c_int32), ("reserved", C.c_int32)]
+
+
+class TuneEntry(C.Structure):
+    _fields_ = [("var_mask", C.c_int32), ("rg_density", C.c_int32), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class Stats(C.Structure):
@@ -281,9 +289,9 @@ class Context:
         return t
 
     def spgemm(self, A, B, scheduler=SCHED_PLAIN, timing=False, variant=0, pred_scale=1.0,
-               mfma_min_products=0):
+               mfma_min_products=0, tuning=0):
         """C = A*B (Steps 1-3 of src/spgemm_raw_v2.cu:111-240); returns (C, stats dict)."""
-        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale)
+        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale, tuning)
         st = Stats()
         Cm = DeviceTiles(self)
         _check(lib().tsg_spgemm(self._p, C.byref(A.s), C.byref(B.s), C.byref(Cm.s), C.byref(o),
@@ -291,11 +299,11 @@ class Context:
         return Cm, st.as_dict()
 
     def spgemm_batched(self, As, Bs, scheduler=SCHED_PLAIN, timing=False, variant=0, pred_scale=1.0,
-                       mfma_min_products=0):
+                       mfma_min_products=0, tuning=0):
         """C_i = A_i*B_i for every pair in one pipeline run; returns ([C_i], [stats_i])."""
         n = len(As)
         assert len(Bs) == n
-        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale)
+        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale, tuning)
         a = (TileStruct * max(n, 1))(*[x.s for x in As])
         b = (TileStruct * max(n, 1))(*[x.s for x in Bs])
         c = (TileStruct * max(n, 1))()
@@ -307,6 +315,44 @@ class Context:
             t.s = TileStruct.from_buffer_copy(c[i])
             out.append(t)
         return out, [st[i].as_dict() for i in range(n)]
+
+    # ---- autotuner
+    def autotune(self, As, Bs, reps=3):
+        """Times every kernel-variant configuration on the product (or batch, for
+        lists) and stores the fastest for the operands' class; returns (best, report)."""
+        As = As if isinstance(As, (list, tuple)) else [As]
+        Bs = Bs if isinstance(Bs, (list, tuple)) else [Bs]
+        n = len(As)
+        a = (TileStruct * n)(*[x.s for x in As])
+        b = (TileStruct * n)(*[x.s for x in Bs])
+        rep = (TuneEntry * 64)()
+        nrep = C.c_int32(0)
+        best = TuneEntry()
+        _check(lib().tsg_autotune(self._p, C.c_int32(n), a, b, C.c_int32(reps), rep, C.c_int32(64),
+                                  C.byref(nrep), C.byref(best)))
+        return best.as_dict(), [rep[i].as_dict() for i in range(nrep.value)]
+
+    def autotune_query(self, As, Bs):
+        As = As if isinstance(As, (list, tuple)) else [As]
+        Bs = Bs if isinstance(Bs, (list, tuple)) else [Bs]
+        n = len(As)
+        a = (TileStruct * n)(*[x.s for x in As])
+        b = (TileStruct * n)(*[x.s for x in Bs])
+        cfg = TuneEntry()
+        found = lib().tsg_autotune_query(self._p, C.c_int32(n), a, b, C.byref(cfg))
+        return cfg.as_dict() if found == 1 else None
+
+    def autotune_set(self, As, Bs, var_mask, rg_density):
+        As = As if isinstance(As, (list, tuple)) else [As]
+        Bs = Bs if isinstance(Bs, (list, tuple)) else [Bs]
+        n = len(As)
+        a = (TileStruct * n)(*[x.s for x in As])
+        b = (TileStruct * n)(*[x.s for x in Bs])
+        cfg = TuneEntry(var_mask, rg_density, 0.0)
+        _check(lib().tsg_autotune_set(self._p, C.c_int32(n), a, b, C.byref(cfg)))
+
+    def autotune_clear(self):
+        _check(lib().tsg_autotune_clear(self._p))
 
     # ---- learned scheduler (MLP 8 -> 32 -> 32 -> 2 on the GPU)
     def learned_collect(self, A, B):

@@ -141,6 +141,7 @@ int tsg_destroy(tsg_context *c) {
     DeviceGuard g(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     learned_release(ctx);
+    tuner_release(ctx);
     for (auto &kv : ctx->pool.free_blocks) (void)hipFree(kv.second);
     for (auto &kv : ctx->pool.live) (void)hipFree(kv.first);
     (void)hipHostFree(ctx->pinned);

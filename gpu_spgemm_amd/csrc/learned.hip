@@ -389,7 +389,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     C->m = A->m; C->n = B->n; C->tilem = A->tilem; C->tilen = B->tilen;
     C->dtype = A->dtype; C->location = TSG_DEVICE;
     const int32_t tilem = A->tilem;
-    const BinTable bt = bin_table_for(A, B, opts);
+    const BinTable bt = bin_table_for(ctx, A, B, opts);
     Plan p;
     int64_t *slotp = nullptr, *valc = nullptr, *expect = nullptr, *res = nullptr;
     int32_t *row_nc = nullptr, *pcol = nullptr, *prow = nullptr;
@@ -659,7 +659,7 @@ static int features_or_predict(tsg_context *c, const tsg_tile_matrix *A, const t
     Plan p;
     float *X = nullptr, *Y = nullptr;
     int64_t *slotp = nullptr, *valc = nullptr;
-    const BinTable bt = bin_table_for(A, B, nullptr);
+    const BinTable bt = bin_table_for(ctx, A, B, nullptr);
     int rc = plan_setup(ctx, A, B, bt, p);
     if (!rc) rc = palloc(ctx, &X, (size_t)std::max(tilem, 1) * NF);
     if (!rc) rc = palloc(ctx, &Y, (size_t)std::max(tilem, 1) * NO);
@@ -705,7 +705,7 @@ int tsg_learned_collect(tsg_context *c, const tsg_tile_matrix *A, const tsg_tile
     Plan p;
     float *X = nullptr, *T = nullptr;
     int64_t *slotp = nullptr, *valc = nullptr;
-    int rc = plan_setup(ctx, A, B, bin_table_for(A, B, nullptr), p);
+    int rc = plan_setup(ctx, A, B, bin_table_for(ctx, A, B, nullptr), p);
     if (!rc) rc = palloc(ctx, &X, (size_t)tilem * NF);
     if (!rc) rc = palloc(ctx, &T, (size_t)tilem * NO);
     if (!rc) rc = palloc(ctx, &slotp, (size_t)tilem + 1);

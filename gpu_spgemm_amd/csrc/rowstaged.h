@@ -42,9 +42,23 @@ struct Plan {
 bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
 // The row-by-row kernels need the scalar CSR of A and B and no duplicate columns.
 bool rg_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
-BinTable bin_table(bool ok, bool rg = false, int rg_density = RG_DENSITY_DEFAULT);
-// The bin table of a product under its options.
-BinTable bin_table_for(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
+// var_mask bit v admits staged variant v; rg_density 0 disables the row-by-row bin.
+BinTable bin_table(bool ok, bool rg = false, int rg_density = RG_DENSITY_DEFAULT, int var_mask = 7);
+// The bin table of a product under its options and the context's tuned configuration.
+BinTable bin_table_for(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
+
+// Autotuned configuration (autotune.hip).
+struct TuneCfg {
+    int32_t var_mask = 7;
+    int32_t rg_density = RG_DENSITY_DEFAULT;
+};
+// Class signature of a product from its operands' sizes.
+uint64_t tune_signature(int64_t tilemA, int64_t numtileA, int64_t nnzA, int64_t tilemB, int64_t numtileB,
+                        int32_t dtype);
+// The configuration to use: a forced one (while tuning), else the stored one
+// for the signature unless opts->tuning == 1; false = defaults.
+bool tuned_config(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts,
+                  TuneCfg *cfg);
 void free_plan(Context *ctx, Plan &p);
 // Allocates the plan's arrays and computes the A->B links.
 int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const BinTable &bt, Plan &p);

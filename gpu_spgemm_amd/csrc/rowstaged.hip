@@ -642,19 +642,23 @@ bool rg_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options
            !(B->flags & 1);
 }
 
-BinTable bin_table_for(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
-    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts));
+BinTable bin_table_for(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
+    TuneCfg cfg;
+    tuned_config(ctx, A, B, opts, &cfg);
+    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts), cfg.rg_density, cfg.var_mask);
 }
 
-BinTable bin_table(bool ok, bool rg, int rg_density) {
+BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask) {
     BinTable t{};
     t.nvar = 0;
-    t.rg = rg ? 1 : 0;
+    t.rg = (rg && rg_density > 0) ? 1 : 0;
     t.rg_density = rg_density;
     if (ok) {
         t.v[0] = caps_of<VarS>();
         t.v[1] = caps_of<VarW>();
         t.v[2] = caps_of<VarM>();
+        for (int v = 0; v < NUM_STAGED; v++)  // a variant left out takes no row
+            if (!(var_mask & (1 << v))) t.v[v] = StageCaps{-1, -1, -1, -1, -1};
         t.nvar = NUM_STAGED;
     }
     return t;
@@ -720,7 +724,7 @@ int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1) {
 // and one host read of (numtileC, bin offsets).
 static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                      const tsg_options *opts, Plan &p, int32_t *stats_out) {
-    const BinTable bt = bin_table_for(A, B, opts);
+    const BinTable bt = bin_table_for(ctx, A, B, opts);
     TSG_CHECK(plan_setup(ctx, A, B, bt, p));
     const int32_t tilem = A->tilem;
     if (tilem == 0) {

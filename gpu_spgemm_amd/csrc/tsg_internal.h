@@ -63,6 +63,8 @@ struct Context {
     int64_t *pinned = nullptr;
     // learned scheduler state lives in learned.hip
     void *learned = nullptr;
+    // autotuner state lives in autotune.hip
+    void *tuner = nullptr;
 };
 
 int pool_alloc(Context *ctx, void **p, size_t bytes);
@@ -105,6 +107,7 @@ int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
 int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                         const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
 void learned_release(Context *ctx);  // learned.hip
+void tuner_release(Context *ctx);    // autotune.hip
 // Row-by-row kernels for sparse heavy tile-rows (heavy.hip).
 int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                        const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);

@@ -116,6 +116,9 @@ typedef struct tsg_options {
     int32_t timing;         /* 1 = record per-step hipEvent times in tsg_stats */
     double  pred_scale;     /* learned: scale of the predicted capacities (0 or 1.0 = normal; below 1 drops the
                                margin, a test hook that forces the exact fallback) */
+    int32_t tuning;         /* 0 = use the context's autotuned configuration for this operand class when
+                               tsg_autotune stored one; 1 = built-in defaults */
+    int32_t reserved;
 } tsg_options;
 
 typedef struct tsg_stats {
@@ -232,6 +235,33 @@ int tsg_learned_predict(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_ti
 int tsg_learned_forward(tsg_context *ctx, const float *X, int64_t n, float *Y);
 int tsg_learned_gradient(tsg_context *ctx, const float *X, const float *T, int64_t n, float *grad,
                          float *loss);
+
+/* ---- Autotuner (no reference counterpart) ----
+ * A configuration chooses which row-staged kernel variants may take rows
+ * (var_mask bit v: variant v, v = 0 small, 1 wide, 2 many-tiles) and the A
+ * tile density (entries per A tile) at or below which a heavy tile-row runs
+ * the row-by-row kernels (rg_density; 0 = never).  tsg_autotune times every
+ * configuration on the given product (count = 1) or batch (count > 1, as
+ * tsg_spgemm_batched), median of `reps` runs, and stores the fastest for the
+ * operands' class (log2 buckets of tile-rows, A tiles per tile-row, A entries
+ * per tile, B tiles per tile-row, value type); later products of that class
+ * use it unless opts->tuning = 1.  Results are identical under every
+ * configuration: only speed changes. */
+typedef struct tsg_tune_entry {
+    int32_t var_mask;
+    int32_t rg_density;
+    double ms;               /* median wall time of one product (or batch), host syncs included */
+} tsg_tune_entry;
+int tsg_autotune(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                 int32_t reps, tsg_tune_entry *report, int32_t cap, int32_t *nreport, tsg_tune_entry *best);
+/* The stored configuration for the operands' class: returns 1 and fills *cfg
+ * when one exists, 0 when not (count as in tsg_autotune). */
+int tsg_autotune_query(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                       tsg_tune_entry *cfg);
+/* Stores cfg (var_mask, rg_density; ms ignored) for the operands' class. */
+int tsg_autotune_set(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                     const tsg_tune_entry *cfg);
+int tsg_autotune_clear(tsg_context *ctx);
 
 /* Many independent products in one persistent launch (no reference counterpart). */
 int tsg_spgemm_batched(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A,
