@@ -38,6 +38,9 @@ def make_workload(g, name):
     if name == "er4096":
         A = g.gen_er(4096, 8, seed=0)
         return A, A, {"workload": "erdos_renyi_4096_8perrow_AxA", "rows": A.m, "nnzA": A.nnz, "dtype": "f64"}
+    if name == "band":
+        A = g.gen_band(65536, 48, seed=0)
+        return A, A, {"workload": "dense_band_65536_halfwidth48_AxA", "rows": A.m, "nnzA": A.nnz, "dtype": "f64"}
     if name == "stencil27_AP":
         A = g.gen_stencil27(128, 128, 128)
         P = g.gen_aggregation(128, 128, 128)
@@ -242,6 +245,8 @@ def main():
     runs = {}
     runs["ctile"] = measure("reference C-tile method", variant=1)
     runs["plain"] = measure("plain (binned, exact sizes)", variant=0)
+    if args.workload == "band":     # full tiles: matrix cores for the dense pairs
+        runs["mfma"] = measure("C-tile + matrix cores", variant=2)
     if args.scheduler == "learned":
         train_learned(g, ctx, log)
         _, st = ctx.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
@@ -257,7 +262,8 @@ def main():
     achieved = nbytes / (step3 * 1e-3) / 1e9
     # the numeric step of this workload is one launch of the staged numeric
     # kernel (every tile-row fits the small variant); its HIP-event time is step3
-    kname = "k_num_staged<double,VarS>" if best == "plain" else "numeric step (learned)"
+    kname = {"plain": "k_num_staged<double,VarS>", "mfma": "k_numeric_mfma<double>",
+             "ctile": "k_numeric_tiles<double>"}.get(best, "numeric step (learned)")
     out = {
         "metric": "SpGEMM GFLOP/s + HBM GB/s, learned/batched vs plain tiled, 1 MI355X",
         "value": round(gflops, 3),

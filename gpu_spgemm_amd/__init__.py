@@ -55,7 +55,8 @@ class Options(C.Structure):
 
 
 class TuneEntry(C.Structure):
-    _fields_ = [("var_mask", C.c_int32), ("rg_density", C.c_int32), ("ms", C.c_double)]
+    _fields_ = [("var_mask", C.c_int32), ("rg_density", C.c_int32), ("mfma_min_products", C.c_int32),
+                ("reserved", C.c_int32), ("ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -194,6 +195,19 @@ def gen_aggregation(nx, ny, nz):
 def gen_rmat(scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=1, dtype=F32):
     return _gen(lib().tsg_gen_rmat, C.c_int32(scale), C.c_int32(edge_factor), C.c_double(a),
                 C.c_double(b), C.c_double(c), C.c_uint64(seed), C.c_int32(dtype))
+
+
+def gen_band(n, half, seed=0):
+    """Dense band |i - j| <= half (values U[-1,1] from numpy's default_rng(seed)):
+    tile-rows of full 16x16 tiles, the case for matrix cores."""
+    lo = np.maximum(np.arange(n) - half, 0)
+    hi = np.minimum(np.arange(n) + half + 1, n)
+    cnt = hi - lo
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(cnt)
+    col = (np.arange(rp[-1]) - np.repeat(rp[:-1], cnt) + np.repeat(lo, cnt)).astype(np.int32)
+    val = np.random.default_rng(seed).uniform(-1, 1, int(rp[-1]))
+    return HostCSR(n, n, rp, col, val)
 
 
 def gen_batch_uneven(count=512, seed=4, lo=256, hi=65536):
@@ -342,13 +356,13 @@ class Context:
         found = lib().tsg_autotune_query(self._p, C.c_int32(n), a, b, C.byref(cfg))
         return cfg.as_dict() if found == 1 else None
 
-    def autotune_set(self, As, Bs, var_mask, rg_density):
+    def autotune_set(self, As, Bs, var_mask, rg_density, mfma_min_products=0):
         As = As if isinstance(As, (list, tuple)) else [As]
         Bs = Bs if isinstance(Bs, (list, tuple)) else [Bs]
         n = len(As)
         a = (TileStruct * n)(*[x.s for x in As])
         b = (TileStruct * n)(*[x.s for x in Bs])
-        cfg = TuneEntry(var_mask, rg_density, 0.0)
+        cfg = TuneEntry(var_mask, rg_density, mfma_min_products, 0, 0.0)
         _check(lib().tsg_autotune_set(self._p, C.c_int32(n), a, b, C.byref(cfg)))
 
     def autotune_clear(self):

@@ -4,7 +4,7 @@
 #include <cstdlib>
 #include <cstring>
 
-#include "tsg_internal.h"
+#include "rowstaged.h"
 
 namespace tsg {
 
@@ -397,7 +397,8 @@ int spgemm_plain(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
     // Step 3
     if (!rc) rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)C->nnz * vsize(C->dtype));
     if (!rc) rc = palloc(ctx, &C->tile_csr_Col, (size_t)C->nnz);
-    if (!rc) rc = launch_numeric(ctx, A, B, C, opts);
+    if (!rc) rc = (opts && opts->numeric_variant == 2) ? launch_numeric_mfma(ctx, A, B, C, opts)
+                                                       : launch_numeric(ctx, A, B, C, opts);
     if (timing && !rc) (void)hipEventRecord(ev[3], ctx->stream);
     if (st) {
         st->numtileC = C->numtile;
@@ -438,8 +439,16 @@ namespace tsg {
 int spgemm_dispatch(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                     const tsg_options *opts, tsg_stats *st) {
     if (st) memset(st, 0, sizeof *st);
+    TuneCfg cfg;  // a tuned configuration may select the matrix-core kernels
+    if ((!opts || (opts->numeric_variant == 0 && opts->scheduler != TSG_SCHED_LEARNED)) &&
+        tuned_config(ctx, A, B, opts, &cfg) && cfg.mfma > 0 && !((A->flags | B->flags) & 1)) {
+        tsg_options o = opts ? *opts : tsg_options{};
+        o.numeric_variant = 2;
+        o.mfma_min_products = cfg.mfma;
+        return spgemm_plain(ctx, A, B, C, &o, st);
+    }
     if (opts && opts->scheduler == TSG_SCHED_LEARNED) return spgemm_learned(ctx, A, B, C, opts, st);
-    if (opts && opts->numeric_variant == 1) return spgemm_plain(ctx, A, B, C, opts, st);
+    if (opts && (opts->numeric_variant == 1 || opts->numeric_variant == 2)) return spgemm_plain(ctx, A, B, C, opts, st);
     return spgemm_binned(ctx, A, B, C, opts, st);
 }
 }  // namespace tsg

@@ -109,16 +109,33 @@ int tsg_autotune(tsg_context *c, int32_t count, const tsg_tile_matrix *A, const 
         staged = staged && staged_ok(&A[i], &B[i], nullptr);
         rg = rg && rg_ok(&A[i], &B[i], nullptr);
     }
+    bool dups = false;
+    for (int32_t i = 0; i < count; i++) dups = dups || ((A[i].flags | B[i].flags) & 1);
     std::vector<int> masks = staged ? std::vector<int>{7, 5, 1, 4, 0} : std::vector<int>{0};
     std::vector<int> dens = rg ? std::vector<int>{0, 2, 8, 32} : std::vector<int>{0};
+    std::vector<TuneCfg> cands;
+    for (int mk : masks)
+        for (int d : dens) {
+            TuneCfg c;
+            c.var_mask = mk;
+            c.rg_density = d;
+            cands.push_back(c);
+        }
+    if (!dups)
+        for (int mp : {64, 512}) {  // matrix-core C-tile kernels
+            TuneCfg c;
+            c.var_mask = 0;
+            c.rg_density = 0;
+            c.mfma = mp;
+            cands.push_back(c);
+        }
     Tuner *t = tuner(ctx);
     std::vector<tsg_tune_entry> rep;
     int rc = TSG_OK;
-    for (int mk : masks) {
-        for (int d : dens) {
+    {
+        for (const TuneCfg &cand : cands) {
             t->forcing = true;
-            t->forced.var_mask = mk;
-            t->forced.rg_density = d;
+            t->forced = cand;
             double w = 0;
             rc = run_once(ctx, count, A, B, &w);  // warm-up (pool sizes, caches)
             std::vector<double> ms;
@@ -129,7 +146,7 @@ int tsg_autotune(tsg_context *c, int32_t count, const tsg_tile_matrix *A, const 
             t->forcing = false;
             if (rc) return rc;
             std::sort(ms.begin(), ms.end());
-            rep.push_back(tsg_tune_entry{mk, d, ms[ms.size() / 2]});
+            rep.push_back(tsg_tune_entry{cand.var_mask, cand.rg_density, cand.mfma, 0, ms[ms.size() / 2]});
         }
     }
     const auto it = std::min_element(rep.begin(), rep.end(),
@@ -137,6 +154,7 @@ int tsg_autotune(tsg_context *c, int32_t count, const tsg_tile_matrix *A, const 
     TuneCfg cfg;
     cfg.var_mask = it->var_mask;
     cfg.rg_density = it->rg_density;
+    cfg.mfma = it->mfma_min_products;
     t->best[call_signature(count, A, B)] = cfg;
     if (best_out) *best_out = *it;
     if (nreport) *nreport = (int32_t)rep.size();
@@ -153,7 +171,7 @@ int tsg_autotune_query(tsg_context *c, int32_t count, const tsg_tile_matrix *A, 
     if (!t) return 0;
     auto it = t->best.find(call_signature(count, A, B));
     if (it == t->best.end()) return 0;
-    if (cfg) *cfg = tsg_tune_entry{it->second.var_mask, it->second.rg_density, 0.0};
+    if (cfg) *cfg = tsg_tune_entry{it->second.var_mask, it->second.rg_density, it->second.mfma, 0, 0.0};
     return 1;
 }
 
@@ -161,11 +179,12 @@ int tsg_autotune_set(tsg_context *c, int32_t count, const tsg_tile_matrix *A, co
                      const tsg_tune_entry *cfg) {
     Context *ctx = reinterpret_cast<Context *>(c);
     if (!ctx || count < 1 || !A || !B || !cfg) return fail(TSG_ERR_ARG, "bad arguments");
-    if (cfg->var_mask < 0 || cfg->var_mask > 7 || cfg->rg_density < 0)
-        return fail(TSG_ERR_ARG, "var_mask must lie in [0, 7] and rg_density be >= 0");
+    if (cfg->var_mask < 0 || cfg->var_mask > 7 || cfg->rg_density < 0 || cfg->mfma_min_products < 0)
+        return fail(TSG_ERR_ARG, "var_mask must lie in [0, 7], rg_density and mfma_min_products be >= 0");
     TuneCfg t;
     t.var_mask = cfg->var_mask;
     t.rg_density = cfg->rg_density;
+    t.mfma = cfg->mfma_min_products;
     tuner(ctx)->best[call_signature(count, A, B)] = t;
     return TSG_OK;
 }

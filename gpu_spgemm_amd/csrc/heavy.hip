@@ -16,10 +16,13 @@
 //     reference's OR over tile pairs: a column is present exactly when some
 //     product reaches it.
 //   k_rg_numeric: the row's values are zeroed and their col bytes written;
-//     then one lane per scalar row walks its products in ascending k and
-//     accumulates each with one FMA into the output's slot.  Each output thus
-//     sums its products in ascending k -- the order of the tiled method (tile
-//     pairs ascending, entries ascending within a pair) -- so values are
+//     then thread (r, s) takes scalar row r and slice s of the row's C tiles
+//     (16 slices of consecutive tiles, hence of consecutive columns): it walks
+//     the row's A entries in ascending k, finds the slice's column range in
+//     B row k by binary search, and accumulates each product there with one
+//     FMA into the output's slot.  Every output belongs to one thread and sums
+//     its products in ascending k -- the order of the tiled method (tile pairs
+//     ascending, entries ascending within a pair) -- so values are
 //     bit-identical to the other kernels.
 //
 // A C tile column is mapped to its position in the row through an LDS table
@@ -29,8 +32,7 @@
 
 namespace tsg {
 
-constexpr int RG_THREADS = 256;   // symbolic: 16 threads per scalar row
-constexpr int RGN_THREADS = 64;   // numeric: one lane per scalar row, the rest zero the outputs
+constexpr int RG_THREADS = 256;   // 16 threads per scalar row
 constexpr int RG_MAP = 8192;      // tile columns mapped through LDS (16 KB)
 
 __device__ __forceinline__ uint32_t rg_below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
@@ -126,7 +128,7 @@ k_rg_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int32_t m,
 }
 
 template <typename V>
-__global__ void __launch_bounds__(RGN_THREADS)
+__global__ void __launch_bounds__(RG_THREADS)
 k_rg_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int32_t m, const int64_t *__restrict__ rpA,
              const int32_t *__restrict__ colA, const V *__restrict__ valA, const int64_t *__restrict__ rpB,
              const int32_t *__restrict__ colB, const V *__restrict__ valB, const int64_t *__restrict__ tile_ptrC,
@@ -144,7 +146,7 @@ k_rg_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int32_t m, con
         return;
     }
     // zero the row's values and write their col bytes (16 threads per tile)
-    for (int64_t it = threadIdx.x; it < (ce - cs) * 16; it += RGN_THREADS) {
+    for (int64_t it = threadIdx.x; it < (ce - cs) * 16; it += RG_THREADS) {
         const int64_t t = cs + (it >> 4);
         const int rr = (int)(it & 15);
         uint32_t rm = maskC[t * 16 + rr];
@@ -162,16 +164,26 @@ k_rg_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int32_t m, con
     const bool verify = row_nc != nullptr;
     __threadfence();
     __syncthreads();
-    // one lane per scalar row, products in ascending k, one FMA each
-    if (threadIdx.x < 16) {
-        const int r = threadIdx.x;
-        const int32_t i = bi * 16 + r;
-        if (i < m) {
+    // thread (r, s): scalar row r, C tiles [t_lo, t_hi) of the row; products in ascending k
+    const int r = threadIdx.x >> 4, sl = threadIdx.x & 15;
+    const int32_t i = bi * 16 + r;
+    const int64_t nc = ce - cs;
+    const int64_t t_lo = cs + nc * sl / 16, t_hi = cs + nc * (sl + 1) / 16;
+    if (i < m && t_lo < t_hi) {
+        const int32_t col_lo = colidxC[t_lo] * 16, col_hi = colidxC[t_hi - 1] * 16 + 15;
+        {
             for (int64_t e = rpA[i]; e < rpA[i + 1]; e++) {
                 const int32_t k = colA[e];
                 const V a = valA[e];
-                for (int64_t f = rpB[k]; f < rpB[k + 1]; f++) {
+                int64_t lo = rpB[k], hi = rpB[k + 1];
+                const int64_t fe = hi;
+                while (lo < hi) {  // first entry of B row k at or after col_lo
+                    const int64_t mid = (lo + hi) >> 1;
+                    if (colB[mid] < col_lo) lo = mid + 1; else hi = mid;
+                }
+                for (int64_t f = lo; f < fe; f++) {
                     const int32_t j = colB[f];
+                    if (j > col_hi) break;
                     const int c = j & 15;
                     const int64_t t = rg_tile(j >> 4, use_map, map, cmin, cmax, colidxC, cs, ce, verify);
                     if (t < 0) continue;
@@ -199,12 +211,12 @@ int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_mat
     if (count <= 0) return TSG_OK;
     if (count > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "row list too long");
     if (A->dtype == TSG_F32)
-        k_rg_numeric<float><<<(unsigned)count, RGN_THREADS, 0, ctx->stream>>>(
+        k_rg_numeric<float><<<(unsigned)count, RG_THREADS, 0, ctx->stream>>>(
             rows, begin, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr, B->row_col,
             (const float *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr,
             C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
     else
-        k_rg_numeric<double><<<(unsigned)count, RGN_THREADS, 0, ctx->stream>>>(
+        k_rg_numeric<double><<<(unsigned)count, RG_THREADS, 0, ctx->stream>>>(
             rows, begin, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr, B->row_col,
             (const double *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr,
             C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);

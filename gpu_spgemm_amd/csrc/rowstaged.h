@@ -51,6 +51,7 @@ BinTable bin_table_for(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
 struct TuneCfg {
     int32_t var_mask = 7;
     int32_t rg_density = RG_DENSITY_DEFAULT;
+    int32_t mfma = 0;  // > 0: matrix-core C-tile kernels with this density gate
 };
 // Class signature of a product from its operands' sizes.
 uint64_t tune_signature(int64_t tilemA, int64_t numtileA, int64_t nnzA, int64_t tilemB, int64_t numtileB,

@@ -631,13 +631,13 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
 
 // ---------------------------------------------------------------- host side
 bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
-    if (opts && opts->numeric_variant == 1) return false;  // reference method everywhere
+    if (opts && opts->numeric_variant != 0) return false;  // C-tile kernels everywhere
     return B->csr_tile_nnz && B->csr_mask && B->csr_col_mask && B->csr_col_Ptr && B->csr_col_Val && B->row_nnz_ptr &&
            !(A->flags & 1) && !(B->flags & 1);
 }
 
 bool rg_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
-    if (opts && opts->numeric_variant == 1) return false;  // reference method everywhere
+    if (opts && opts->numeric_variant != 0) return false;  // C-tile kernels everywhere
     return A->row_ptr && A->row_col && A->row_val && B->row_ptr && B->row_col && B->row_val && !(A->flags & 1) &&
            !(B->flags & 1);
 }

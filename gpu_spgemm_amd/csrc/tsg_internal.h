@@ -116,6 +116,9 @@ int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_mat
 int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B);  // api.hip
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                    tsg_tile_matrix *C, const tsg_options *opts);
+// Matrix-core variant of launch_numeric (mfma.hip).
+int launch_numeric_mfma(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                        const tsg_options *opts);
 
 }  // namespace tsg
 

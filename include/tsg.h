@@ -111,8 +111,11 @@ typedef enum tsg_scheduler {
 typedef struct tsg_options {
     int32_t scheduler;      /* tsg_scheduler */
     int32_t numeric_variant;/* 0 = auto: tile-rows binned to the row-staged kernels, heavy rows to the
-                               C-tile kernels; 1 = C-tile kernels for every tile (the reference's method) */
-    int32_t mfma_min_products; /* tile pairs with >= this many products may use MFMA (0 = default) */
+                               row-by-row or C-tile kernels; 1 = C-tile kernels for every tile (the
+                               reference's method); 2 = C-tile kernels with matrix cores (MFMA) for dense
+                               tile pairs */
+    int32_t mfma_min_products; /* variant 2: a pair goes to the matrix cores when nnz(A tile) * nnz(B tile)
+                                  / 16 (its expected product count) >= this (0 = 256) */
     int32_t timing;         /* 1 = record per-step hipEvent times in tsg_stats */
     double  pred_scale;     /* learned: scale of the predicted capacities (0 or 1.0 = normal; below 1 drops the
                                margin, a test hook that forces the exact fallback) */
@@ -240,7 +243,9 @@ int tsg_learned_gradient(tsg_context *ctx, const float *X, const float *T, int64
  * A configuration chooses which row-staged kernel variants may take rows
  * (var_mask bit v: variant v, v = 0 small, 1 wide, 2 many-tiles) and the A
  * tile density (entries per A tile) at or below which a heavy tile-row runs
- * the row-by-row kernels (rg_density; 0 = never).  tsg_autotune times every
+ * the row-by-row kernels (rg_density; 0 = never), or, with
+ * mfma_min_products > 0, the C-tile kernels with matrix cores for every tile
+ * (numeric_variant 2 with that density gate).  tsg_autotune times every
  * configuration on the given product (count = 1) or batch (count > 1, as
  * tsg_spgemm_batched), median of `reps` runs, and stores the fastest for the
  * operands' class (log2 buckets of tile-rows, A tiles per tile-row, A entries
@@ -250,6 +255,8 @@ int tsg_learned_gradient(tsg_context *ctx, const float *X, const float *T, int64
 typedef struct tsg_tune_entry {
     int32_t var_mask;
     int32_t rg_density;
+    int32_t mfma_min_products;  /* > 0: matrix-core C-tile kernels with this gate */
+    int32_t reserved;
     double ms;               /* median wall time of one product (or batch), host syncs included */
 } tsg_tune_entry;
 int tsg_autotune(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
@@ -258,7 +265,7 @@ int tsg_autotune(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, cons
  * when one exists, 0 when not (count as in tsg_autotune). */
 int tsg_autotune_query(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                        tsg_tune_entry *cfg);
-/* Stores cfg (var_mask, rg_density; ms ignored) for the operands' class. */
+/* Stores cfg (var_mask, rg_density, mfma_min_products; ms ignored) for the operands' class. */
 int tsg_autotune_set(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                      const tsg_tune_entry *cfg);
 int tsg_autotune_clear(tsg_context *ctx);

@@ -38,7 +38,7 @@ def test_autotune_picks_the_fastest_and_stores_it(ctx):
     tA, tB = ctx.tiles_from_csr(A), ctx.tiles_from_csr(A, as_b=True)
     best, rep = ctx.autotune(tA, tB, reps=2)
     print("    report:", [(r["var_mask"], r["rg_density"], round(r["ms"], 3)) for r in rep], flush=True)
-    assert len(rep) == 20
+    assert len(rep) == 22
     assert best["ms"] == min(r["ms"] for r in rep)
     q = ctx.autotune_query(tA, tB)
     assert q["var_mask"] == best["var_mask"] and q["rg_density"] == best["rg_density"]
@@ -52,12 +52,34 @@ def test_autotune_picks_the_fastest_and_stores_it(ctx):
     ctx.autotune_clear()
 
 
+@pytest.mark.parametrize("mfma", [64, 512])
+def test_tuned_matrix_core_configuration(ctx, mfma):
+    A = g.gen_band(900, 40, seed=5)
+    tA, tB = ctx.tiles_from_csr(A), ctx.tiles_from_csr(A, as_b=True)
+    ref, _ = ctx.spgemm(tA, tB, variant=1)
+    ctx.autotune_set(tA, tB, 0, 0, mfma)
+    assert ctx.autotune_query(tA, tB)["mfma_min_products"] == mfma
+    got, _ = ctx.spgemm(tA, tB)
+    assert_tiles_equal(got.download(), ref.download(), keys=CKEYS, label=f"mfma {mfma} ")
+    ctx.autotune_clear()
+
+
+def test_autotune_prefers_matrix_cores_on_full_tiles(ctx):
+    A = g.gen_band(16384, 48, seed=6)
+    tA, tB = ctx.tiles_from_csr(A), ctx.tiles_from_csr(A, as_b=True)
+    best, rep = ctx.autotune(tA, tB, reps=2)
+    print("    band report:", [(r["var_mask"], r["rg_density"], r["mfma_min_products"], round(r["ms"], 3))
+                               for r in rep], flush=True)
+    assert best["mfma_min_products"] > 0
+    ctx.autotune_clear()
+
+
 def test_autotune_batch(ctx):
     ps = [g.gen_er(int(n), int(k), seed=int(n)) for n, k in [(500, 4), (3000, 20), (1200, 9), (64, 2)]]
     tA = [ctx.tiles_from_csr(a) for a in ps]
     tB = [ctx.tiles_from_csr(a, as_b=True) for a in ps]
     best, rep = ctx.autotune(tA, tB, reps=1)
-    assert len(rep) == 20 and ctx.autotune_query(tA, tB) is not None
+    assert len(rep) == 22 and ctx.autotune_query(tA, tB) is not None
     Cs, _ = ctx.spgemm_batched(tA, tB)
     for i in range(len(ps)):
         C1, _ = ctx.spgemm(tA[i], tB[i], variant=1)
