@@ -45,7 +45,8 @@ class TileStruct(C.Structure):
                 ("csr_tile_nnz", C.POINTER(C.c_int64)), ("csr_mask", C.POINTER(C.c_uint16)),
                 ("csr_col_mask", C.POINTER(C.c_uint16)), ("csr_col_Ptr", C.POINTER(C.c_uint8)),
                 ("csr_col_Val", C.c_void_p), ("row_nnz_ptr", C.POINTER(C.c_int64)),
-                ("row_ptr", C.POINTER(C.c_int64)), ("row_col", C.POINTER(C.c_int32)), ("row_val", C.c_void_p)]
+                ("row_ptr", C.POINTER(C.c_int64)), ("row_col", C.POINTER(C.c_int32)), ("row_val", C.c_void_p),
+                ("val_perm", C.POINTER(C.c_int64)), ("val_perm_col", C.POINTER(C.c_int64))]
 
 
 class Options(C.Structure):
@@ -84,6 +85,7 @@ def lib():
         _lib = C.CDLL(LIB_PATH)
         _lib.tsg_last_error_string.restype = C.c_char_p
         _lib.tsg_stream.restype = C.c_void_p
+        _lib.tsg_tile_set_values.argtypes = [C.c_void_p, C.POINTER(TileStruct), C.c_void_p]
     return _lib
 
 
@@ -330,6 +332,23 @@ class Context:
             out.append(t)
         return out, [st[i].as_dict() for i in range(n)]
 
+    # ---- new values, same structure
+    def set_values(self, T, host_csr):
+        """Rewrites every value copy of T (made by csr2tile / tiles_from_csr)
+        from a host CSR with T's structure (its values in CSR order)."""
+        d = self.upload_csr(host_csr)
+        try:
+            # a device pointer: pass it as void*, never as a default (32-bit) int
+            _check(lib().tsg_tile_set_values(self._p, C.byref(T.s), C.c_void_p(d.s.val)))
+        finally:
+            self.synchronize()
+            d.free()
+
+    def plan(self, As, Bs, variant=0, mfma_min_products=0, tuning=0):
+        """Product (or batch, for lists) plus a plan that recomputes the values
+        from the operands' current values as one captured hipGraph."""
+        return Plan(self, As, Bs, Options(SCHED_PLAIN, variant, mfma_min_products, 0, 1.0, tuning))
+
     # ---- autotuner
     def autotune(self, As, Bs, reps=3):
         """Times every kernel-variant configuration on the product (or batch, for
@@ -433,6 +452,42 @@ class Context:
         _check(lib().tsg_learned_gradient(self._p, X.ctypes.data_as(C.c_void_p), T.ctypes.data_as(C.c_void_p),
                                           C.c_int64(X.shape[0]), g.ctypes.data_as(C.c_void_p), C.byref(loss)))
         return g, loss.value
+
+
+class Plan:
+    """tsg_plan: outputs[i] = As[i]*Bs[i]; execute() recomputes their values."""
+
+    def __init__(self, ctx, As, Bs, opts):
+        self.ctx = ctx
+        single = not isinstance(As, (list, tuple))
+        As = [As] if single else list(As)
+        Bs = [Bs] if single else list(Bs)
+        n = len(As)
+        a = (TileStruct * n)(*[x.s for x in As])
+        b = (TileStruct * n)(*[x.s for x in Bs])
+        c = (TileStruct * n)()
+        self._p = C.c_void_p()
+        _check(lib().tsg_plan_create(ctx.handle, C.c_int32(n), a, b, c, C.byref(opts), C.byref(self._p)))
+        self.outputs = []
+        for i in range(n):
+            t = DeviceTiles(ctx)
+            t.s = TileStruct.from_buffer_copy(c[i])
+            self.outputs.append(t)
+        self._keep = (As, Bs)
+
+    def execute(self):
+        _check(lib().tsg_plan_execute(self._p))
+
+    def destroy(self):
+        if self._p:
+            _check(lib().tsg_plan_destroy(self._p))
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class DeviceCSR:

@@ -252,6 +252,7 @@ int tsg_tile_upload(tsg_context *c, const tsg_tile_matrix *h, tsg_tile_matrix *d
     t.csr_tile_nnz = nullptr; t.csr_mask = nullptr; t.csr_col_mask = nullptr; t.csr_col_Ptr = nullptr;
     t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
     t.row_ptr = nullptr; t.row_col = nullptr; t.row_val = nullptr;
+    t.val_perm = nullptr; t.val_perm_col = nullptr;
     if (h->row_ptr && h->row_col && h->row_val) {
         TSG_CHECK(up(ctx, h->row_ptr, (size_t)h->m + 1, &t.row_ptr));
         TSG_CHECK(up(ctx, h->row_col, (size_t)h->nnz, &t.row_col));
@@ -295,6 +296,7 @@ int tsg_tile_download(tsg_context *c, const tsg_tile_matrix *d, tsg_tile_matrix 
     t.csr_tile_nnz = nullptr; t.csr_mask = nullptr; t.csr_col_mask = nullptr; t.csr_col_Ptr = nullptr;
     t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
     t.row_ptr = nullptr; t.row_col = nullptr; t.row_val = nullptr;
+    t.val_perm = nullptr; t.val_perm_col = nullptr;
     if (d->row_ptr && d->row_col && d->row_val) {
         TSG_CHECK(down(ctx, d->row_ptr, (size_t)d->m + 1, &t.row_ptr));
         TSG_CHECK(down(ctx, d->row_col, (size_t)d->nnz, &t.row_col));
@@ -324,7 +326,7 @@ int tsg_tile_free(tsg_context *c, tsg_tile_matrix *d) {
     void *ps[] = {d->tile_ptr, d->tile_columnidx, d->tile_rowidx, d->tile_nnz, d->tile_csr_Val,
                   d->tile_csr_Col, d->tile_csr_Ptr, d->mask, d->csc_tile_ptr, d->csc_tile_rowidx,
                   d->csr_tile_nnz, d->csr_mask, d->csr_col_mask, d->csr_col_Ptr, d->csr_col_Val, d->row_nnz_ptr,
-                  d->row_ptr, d->row_col, d->row_val};
+                  d->row_ptr, d->row_col, d->row_val, d->val_perm, d->val_perm_col};
     for (void *p : ps) pool_free(ctx, p);
     memset(d, 0, sizeof *d);
     return TSG_OK;
@@ -336,7 +338,7 @@ int tsg_tile_free_host(tsg_tile_matrix *h) {
     void *ps[] = {h->tile_ptr, h->tile_columnidx, h->tile_rowidx, h->tile_nnz, h->tile_csr_Val,
                   h->tile_csr_Col, h->tile_csr_Ptr, h->mask, h->csc_tile_ptr, h->csc_tile_rowidx,
                   h->csr_tile_nnz, h->csr_mask, h->csr_col_mask, h->csr_col_Ptr, h->csr_col_Val, h->row_nnz_ptr,
-                  h->row_ptr, h->row_col, h->row_val};
+                  h->row_ptr, h->row_col, h->row_val, h->val_perm, h->val_perm_col};
     for (void *p : ps) free(p);
     memset(h, 0, sizeof *h);
     return TSG_OK;
@@ -430,25 +432,39 @@ int spgemm_plain(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
 
 int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                    const tsg_options *opts, tsg_stats *st);  // learned.hip
-int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                  const tsg_options *opts, tsg_stats *st);   // rowstaged.hip
 
 }  // namespace tsg
 
 namespace tsg {
 int spgemm_dispatch(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                    const tsg_options *opts, tsg_stats *st) {
+                    const tsg_options *opts, tsg_stats *st, NumericReplay *keep) {
     if (st) memset(st, 0, sizeof *st);
     TuneCfg cfg;  // a tuned configuration may select the matrix-core kernels
-    if ((!opts || (opts->numeric_variant == 0 && opts->scheduler != TSG_SCHED_LEARNED)) &&
+    if ((!opts || (opts->numeric_variant == 0 && (opts->scheduler != TSG_SCHED_LEARNED || keep))) &&
         tuned_config(ctx, A, B, opts, &cfg) && cfg.mfma > 0 && !((A->flags | B->flags) & 1)) {
         tsg_options o = opts ? *opts : tsg_options{};
         o.numeric_variant = 2;
         o.mfma_min_products = cfg.mfma;
+        if (keep) {
+            keep->path = 2;
+            keep->o = o;
+        }
         return spgemm_plain(ctx, A, B, C, &o, st);
     }
-    if (opts && opts->scheduler == TSG_SCHED_LEARNED) return spgemm_learned(ctx, A, B, C, opts, st);
-    if (opts && (opts->numeric_variant == 1 || opts->numeric_variant == 2)) return spgemm_plain(ctx, A, B, C, opts, st);
+    // a kept plan needs the exact pipeline (the learned one re-plans every call)
+    if (opts && opts->scheduler == TSG_SCHED_LEARNED && !keep) return spgemm_learned(ctx, A, B, C, opts, st);
+    if (opts && (opts->numeric_variant == 1 || opts->numeric_variant == 2)) {
+        if (keep) {
+            keep->path = opts->numeric_variant;
+            keep->o = *opts;
+        }
+        return spgemm_plain(ctx, A, B, C, opts, st);
+    }
+    if (keep) {
+        keep->path = 0;
+        if (opts) keep->o = *opts;
+        return spgemm_binned(ctx, A, B, C, opts, st, &keep->p);
+    }
     return spgemm_binned(ctx, A, B, C, opts, st);
 }
 }  // namespace tsg

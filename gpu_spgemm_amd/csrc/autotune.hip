@@ -62,10 +62,6 @@ bool tuned_config(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     return true;
 }
 
-int spgemm_dispatch(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                    const tsg_options *opts, tsg_stats *st);  // api.hip
-int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                   const tsg_options *opts, tsg_stats *stats);  // batch.hip
 
 // Signature of the product a call with these operands runs (for a batch:
 // the block-diagonal product's totals).

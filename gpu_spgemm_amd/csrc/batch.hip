@@ -22,19 +22,6 @@
 
 namespace tsg {
 
-// One contiguous copy: count elements of `kind` from src to dst, adding `add`
-// (integer kinds).  kind: 0 = bytes, 1 = 16-byte units, 2 = int32 + add,
-// 3 = int64 + add, 4 = 8-byte units, 5 = 4-byte units, 6 = int64 value `add`
-// (no source).
-struct CatJob {
-    const void *src;
-    void *dst;
-    int64_t count;
-    int64_t add;
-    int32_t kind;
-    int32_t pad;
-};
-
 __global__ void __launch_bounds__(256) k_cat(const CatJob *__restrict__ jobs) {
     const CatJob j = jobs[blockIdx.y];
     const int64_t stride = (int64_t)gridDim.x * 256;
@@ -76,23 +63,28 @@ struct JobList {
     }
 };
 
-int run_jobs(Context *ctx, const JobList &jl) {
-    if (jl.jobs.empty()) return TSG_OK;
+int64_t max_count(const JobList &jl) {
     int64_t maxc = 0;
     for (const CatJob &j : jl.jobs) maxc = std::max(maxc, j.count);
-    CatJob *dj = nullptr;
-    TSG_CHECK(palloc(ctx, &dj, jl.jobs.size()));
-    int rc = TSG_OK;
-    // pageable copy: the runtime stages it before the call returns
-    if (hipMemcpyAsync(dj, jl.jobs.data(), sizeof(CatJob) * jl.jobs.size(), hipMemcpyHostToDevice, ctx->stream) !=
+    return maxc;
+}
+
+// Device copy of a job table (the host vector must live until the stream syncs).
+int upload_jobs(Context *ctx, const JobList &jl, CatJob **out) {
+    *out = nullptr;
+    if (jl.jobs.empty()) return TSG_OK;
+    TSG_CHECK(palloc(ctx, out, jl.jobs.size()));
+    if (hipMemcpyAsync(*out, jl.jobs.data(), sizeof(CatJob) * jl.jobs.size(), hipMemcpyHostToDevice, ctx->stream) !=
         hipSuccess)
-        rc = fail(TSG_ERR_HIP, "job table upload failed");
-    for (size_t j0 = 0; !rc && j0 < jl.jobs.size(); j0 += 65535) {
-        const unsigned ny = (unsigned)std::min<size_t>(65535, jl.jobs.size() - j0);
-        const unsigned nx = (unsigned)std::min<int64_t>(std::max<int64_t>((maxc + 255) / 256, 1), 64);
-        k_cat<<<dim3(nx, ny), 256, 0, ctx->stream>>>(dj + j0);
-        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_cat launch");
-    }
+        return fail(TSG_ERR_HIP, "job table upload failed");
+    return TSG_OK;
+}
+
+int run_jobs(Context *ctx, const JobList &jl) {
+    if (jl.jobs.empty()) return TSG_OK;
+    CatJob *dj = nullptr;
+    int rc = upload_jobs(ctx, jl, &dj);
+    if (!rc) rc = launch_jobs(ctx, dj, (int64_t)jl.jobs.size(), max_count(jl));
     // the table must outlive the copies that read it
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(TSG_ERR_HIP, "batch copy failed");
     pool_free(ctx, dj);
@@ -101,11 +93,33 @@ int run_jobs(Context *ctx, const JobList &jl) {
 
 }  // namespace
 
-int spgemm_dispatch(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                    const tsg_options *opts, tsg_stats *st);  // api.hip
+int launch_jobs(Context *ctx, const CatJob *dj, int64_t n, int64_t maxc) {
+    for (int64_t j0 = 0; j0 < n; j0 += 65535) {
+        const unsigned ny = (unsigned)std::min<int64_t>(65535, n - j0);
+        const unsigned nx = (unsigned)std::min<int64_t>(std::max<int64_t>((maxc + 255) / 256, 1), 64);
+        k_cat<<<dim3(nx, ny), 256, 0, ctx->stream>>>(dj + j0);
+        TSG_HIP(hipGetLastError());
+    }
+    return TSG_OK;
+}
+
+void free_batch_keep(Context *ctx, BatchKeep &k) {
+    tsg_tile_matrix *ms[] = {&k.BA, &k.BB, &k.BC};
+    for (tsg_tile_matrix *M : ms) {
+        void *ps[] = {M->tile_ptr, M->tile_columnidx, M->tile_rowidx, M->tile_nnz, M->tile_csr_Val, M->tile_csr_Col,
+                      M->tile_csr_Ptr, M->mask, M->csc_tile_ptr, M->csc_tile_rowidx, M->csr_tile_nnz, M->csr_mask,
+                      M->csr_col_mask, M->csr_col_Ptr, M->csr_col_Val, M->row_nnz_ptr, M->row_ptr, M->row_col,
+                      M->row_val, M->val_perm, M->val_perm_col};
+        for (void *p : ps) pool_free(ctx, p);
+    }
+    free_plan(ctx, k.nr.p);
+    pool_free(ctx, k.vin);
+    pool_free(ctx, k.vout);
+    k = BatchKeep{};
+}
 
 int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                   const tsg_options *opts, tsg_stats *stats) {
+                   const tsg_options *opts, tsg_stats *stats, BatchKeep *keep) {
     if (n < 0) return fail(TSG_ERR_ARG, "negative batch size");
     if (n == 0) return TSG_OK;
     const int32_t dtype = A[0].dtype;
@@ -186,7 +200,7 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
         return rc;
     }
     // concatenation jobs; the final entry of every prefix array is written once
-    JobList jl;
+    JobList jl, jv;  // every array; the value arrays alone (re-run of the numeric step)
     const int vk = vs == 8 ? 4 : 5;
     for (int32_t i = 0; i < n; i++) {
         const tsg_tile_matrix &a = A[i], &b = B[i];
@@ -197,6 +211,7 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
         jl.add(a.mask, BA.mask + at[i] * 16, a.numtile * 2, 1);
         jl.add(a.tile_csr_Col, BA.tile_csr_Col + an[i], a.nnz, 0);
         jl.add(a.tile_csr_Val, (uint8_t *)BA.tile_csr_Val + an[i] * vs, a.nnz, vk);
+        jv.add(a.tile_csr_Val, (uint8_t *)BA.tile_csr_Val + an[i] * vs, a.nnz, vk);
         jl.add(b.tile_ptr, BB.tile_ptr + K[i], b.tilem, 3, bt[i]);
         jl.add(b.tile_columnidx, BB.tile_columnidx + bt[i], b.numtile, 2, Cc[i]);
         if (brow) jl.add(b.tile_rowidx, BB.tile_rowidx + bt[i], b.numtile, 2, K[i]);
@@ -205,6 +220,7 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
         jl.add(b.mask, BB.mask + bt[i] * 16, b.numtile * 2, 1);
         jl.add(b.tile_csr_Col, BB.tile_csr_Col + bn[i], b.nnz, 0);
         jl.add(b.tile_csr_Val, (uint8_t *)BB.tile_csr_Val + bn[i] * vs, b.nnz, vk);
+        jv.add(b.tile_csr_Val, (uint8_t *)BB.tile_csr_Val + bn[i] * vs, b.nnz, vk);
         jl.add(b.csc_tile_ptr, BB.csc_tile_ptr + Cc[i], b.tilen, 3, bt[i]);
         jl.add(b.csc_tile_rowidx, BB.csc_tile_rowidx + bt[i], b.numtile, 2, K[i]);
         if (scal) {  // rows: the problem's m rows, then its padding rows (empty: the end offset repeated)
@@ -212,10 +228,12 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
             jl.fill(BA.row_ptr + R[i] * 16 + a.m, (int64_t)a.tilem * 16 - a.m, an[i + 1]);
             jl.add(a.row_col, BA.row_col + an[i], a.nnz, 2, K[i] * 16);
             jl.add(a.row_val, (uint8_t *)BA.row_val + an[i] * vs, a.nnz, vk);
+            jv.add(a.row_val, (uint8_t *)BA.row_val + an[i] * vs, a.nnz, vk);
             jl.add(b.row_ptr, BB.row_ptr + K[i] * 16, b.m, 3, bn[i]);
             jl.fill(BB.row_ptr + K[i] * 16 + b.m, (int64_t)b.tilem * 16 - b.m, bn[i + 1]);
             jl.add(b.row_col, BB.row_col + bn[i], b.nnz, 2, Cc[i] * 16);
             jl.add(b.row_val, (uint8_t *)BB.row_val + bn[i] * vs, b.nnz, vk);
+            jv.add(b.row_val, (uint8_t *)BB.row_val + bn[i] * vs, b.nnz, vk);
         }
         if (staged) {
             jl.add(b.csr_tile_nnz, BB.csr_tile_nnz + bt[i], b.numtile, 3, bn[i]);
@@ -223,6 +241,7 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
             jl.add(b.csr_col_mask, BB.csr_col_mask + bt[i] * 16, b.numtile * 2, 1);
             jl.add(b.csr_col_Ptr, BB.csr_col_Ptr + bt[i] * 16, b.numtile, 1);
             jl.add(b.csr_col_Val, (uint8_t *)BB.csr_col_Val + bn[i] * vs, b.nnz, vk);
+            jv.add(b.csr_col_Val, (uint8_t *)BB.csr_col_Val + bn[i] * vs, b.nnz, vk);
             jl.add(b.row_nnz_ptr, BB.row_nnz_ptr + K[i], b.tilem, 3, bn[i]);
         }
     }
@@ -244,9 +263,12 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
         rc = fail(TSG_ERR_HIP, "batch tail upload failed");
     if (!rc) rc = run_jobs(ctx, jl);
     tsg_stats bst{};
-    if (!rc) rc = spgemm_dispatch(ctx, &BA, &BB, &BC, opts, &bst);
-    free_m(BA);
-    free_m(BB);
+    if (!rc) rc = spgemm_dispatch(ctx, &BA, &BB, &BC, opts, &bst, keep ? &keep->nr : nullptr);
+    if (!keep || rc) {
+        free_m(BA);
+        free_m(BB);
+        if (keep) free_plan(ctx, keep->nr.p);
+    }
     if (rc) return rc;
     // per-problem boundaries of C (one read)
     int64_t *drow = nullptr, *dbounds = nullptr;
@@ -297,7 +319,29 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
         }
         rc = run_jobs(ctx, sj);
     }
-    free_c(ctx, &BC);
+    if (!rc && keep) {  // keep the operands, the product and the value copies for re-runs
+        JobList so;
+        for (int32_t i = 0; i < n; i++)
+            so.add((uint8_t *)BC.tile_csr_Val + hb[2 * i + 1] * vs, C[i].tile_csr_Val, C[i].nnz, vk);
+        rc = upload_jobs(ctx, jv, &keep->vin);
+        if (!rc) rc = upload_jobs(ctx, so, &keep->vout);
+        if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(TSG_ERR_HIP, "job table upload failed");
+        keep->nvin = (int64_t)jv.jobs.size();
+        keep->vin_max = max_count(jv);
+        keep->nvout = (int64_t)so.jobs.size();
+        keep->vout_max = max_count(so);
+        keep->BA = BA;
+        keep->BB = BB;
+        keep->BC = BC;
+        if (rc) free_batch_keep(ctx, *keep);
+    } else {
+        if (keep) {  // failed before the hand-over
+            free_m(BA);
+            free_m(BB);
+            free_plan(ctx, keep->nr.p);
+        }
+        free_c(ctx, &BC);
+    }
     if (rc) {
         for (int32_t i = 0; i < n; i++) free_c(ctx, &C[i]);
         return rc;

@@ -163,7 +163,8 @@ __global__ void __launch_bounds__(CV_WAVES *WAVE)
 k_c2t_scatter(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, const V *__restrict__ val,
               int32_t m, int32_t tilem, const int64_t *__restrict__ tile_ptr,
               const int32_t *__restrict__ tile_colidx, const int64_t *__restrict__ tile_nnz,
-              const uint8_t *__restrict__ ptr, V *__restrict__ tval, uint8_t *__restrict__ tcol) {
+              const uint8_t *__restrict__ ptr, V *__restrict__ tval, uint8_t *__restrict__ tcol,
+              int64_t *__restrict__ perm) {
     __shared__ int64_t s_rp[CV_WAVES][17];
     const int w = threadIdx.x / WAVE, lane = lane_id();
     const int32_t bi = blockIdx.x * CV_WAVES + w;
@@ -183,6 +184,7 @@ k_c2t_scatter(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, c
         const int64_t pos = tile_nnz[T] + ptr[T * 16 + r] + (j - seg);
         tval[pos] = val[j];
         tcol[pos] = (uint8_t)((r << 4) | (cj & 15));
+        perm[j] = pos;
     }
 }
 
@@ -259,12 +261,14 @@ k_csc_sort(int32_t tilen, const int64_t *__restrict__ cptr, const int32_t *__res
 __global__ void __launch_bounds__(256)
 k_csc_perm(int64_t numtile, const int32_t *__restrict__ colidx, const int32_t *__restrict__ rowidx,
            const int64_t *__restrict__ cptr, const int32_t *__restrict__ crow,
-           const int64_t *__restrict__ nnz_old, int64_t *__restrict__ inv, int64_t *__restrict__ nnz_new) {
+           const int64_t *__restrict__ nnz_old, int64_t *__restrict__ inv, int64_t *__restrict__ nnz_new,
+           int64_t *__restrict__ pos_of) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (t >= numtile) return;
     const int32_t tc = colidx[t];
     const int64_t p = lower_bound<int64_t, int32_t>(crow, cptr[tc], cptr[tc + 1], rowidx[t]);
     inv[p] = t;
+    pos_of[t] = p;
     nnz_new[p + 1] = nnz_old[t + 1] - nnz_old[t];
     if (t == 0) nnz_new[0] = 0;
 }
@@ -333,6 +337,44 @@ k_colmajor(int64_t numtile, const int64_t *__restrict__ tile_nnz, const uint8_t 
     }
 }
 
+// Value positions after the re-layout: perm[e] (a position in the CSR-order
+// tiles) becomes the position in the CSC-order tiles; perm_col[e] the position
+// in the column-major copy.  The entry's row and column inside its tile come
+// from its CSR-order col byte (r<<4)|c.
+__global__ void __launch_bounds__(256)
+k_val_perm_b(int64_t nnz, int64_t numtile, const int64_t *__restrict__ nnz_old, const int64_t *__restrict__ pos_of,
+             const int64_t *__restrict__ nnz_new, const uint8_t *__restrict__ col_old,
+             const uint16_t *__restrict__ cmask, const uint8_t *__restrict__ cptr, int64_t *__restrict__ perm,
+             int64_t *__restrict__ perm_col) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nnz) return;
+    const int64_t old = perm[e];
+    int64_t lo = 0, hi = numtile - 1;  // last tile whose first entry is at or before `old`
+    while (lo < hi) {
+        const int64_t mid = (lo + hi + 1) >> 1;
+        if (nnz_old[mid] <= old) lo = mid; else hi = mid - 1;
+    }
+    const int64_t t = lo, off = old - nnz_old[t];
+    perm[e] = nnz_new[pos_of[t]] + off;
+    if (perm_col) {
+        const int rc = col_old[old], r = rc >> 4, c = rc & 15;
+        perm_col[e] = nnz_old[t] + cptr[t * 16 + c] + __popc(cmask[t * 16 + c] & ((0xFFFF0000u >> r) & 0xFFFFu));
+    }
+}
+
+// New values in CSR order into every copy.
+template <typename V>
+__global__ void __launch_bounds__(256)
+k_set_values(int64_t nnz, const V *__restrict__ vals, const int64_t *__restrict__ perm,
+             const int64_t *__restrict__ perm_col, V *__restrict__ tval, V *__restrict__ cval, V *__restrict__ rval) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= nnz) return;
+    const V v = vals[e];
+    tval[perm[e]] = v;
+    if (perm_col) cval[perm_col[e]] = v;
+    if (rval) rval[e] = v;
+}
+
 static unsigned blocks_for(int64_t threads, int per) { return (unsigned)((threads + per - 1) / per); }
 
 }  // namespace tsg
@@ -356,7 +398,7 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
     auto cleanup = [&](int code) {
         pool_free(ctx, flags);
         void *ps[] = {T->tile_ptr, T->tile_columnidx, T->tile_nnz, T->tile_csr_Val, T->tile_csr_Col,
-                      T->tile_csr_Ptr, T->mask, T->row_ptr, T->row_col, T->row_val};
+                      T->tile_csr_Ptr, T->mask, T->row_ptr, T->row_col, T->row_val, T->val_perm};
         for (void *p : ps) pool_free(ctx, p);
         memset(T, 0, sizeof *T);
         return code;
@@ -383,6 +425,7 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
     if ((rc = palloc(ctx, &T->mask, (size_t)nt * 16))) return cleanup(rc);
     if ((rc = pool_alloc(ctx, &T->tile_csr_Val, (size_t)T->nnz * vsize(T->dtype)))) return cleanup(rc);
     if ((rc = palloc(ctx, &T->tile_csr_Col, (size_t)T->nnz))) return cleanup(rc);
+    if ((rc = palloc(ctx, &T->val_perm, (size_t)T->nnz))) return cleanup(rc);
     if (hipMemsetAsync(T->tile_csr_Ptr, 0, (size_t)nt * 16, s) != hipSuccess ||
         hipMemsetAsync(T->mask, 0, (size_t)nt * 32, s) != hipSuccess)
         return cleanup(fail(TSG_ERR_HIP, "memset"));
@@ -402,11 +445,11 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
         if (T->dtype == TSG_F32)
             k_c2t_scatter<float><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
                 A->rowptr, A->col, (const float *)A->val, A->m, T->tilem, T->tile_ptr, T->tile_columnidx,
-                T->tile_nnz, T->tile_csr_Ptr, (float *)T->tile_csr_Val, T->tile_csr_Col);
+                T->tile_nnz, T->tile_csr_Ptr, (float *)T->tile_csr_Val, T->tile_csr_Col, T->val_perm);
         else
             k_c2t_scatter<double><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
                 A->rowptr, A->col, (const double *)A->val, A->m, T->tilem, T->tile_ptr, T->tile_columnidx,
-                T->tile_nnz, T->tile_csr_Ptr, (double *)T->tile_csr_Val, T->tile_csr_Col);
+                T->tile_nnz, T->tile_csr_Ptr, (double *)T->tile_csr_Val, T->tile_csr_Col, T->val_perm);
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "scatter launch"));
     }
     // keep the scalar CSR beside the tiles (row-by-row product of sparse heavy rows)
@@ -436,7 +479,7 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     hipStream_t s = ctx->stream;
     const int64_t nt = B->numtile;
     int32_t *rowidx = B->tile_rowidx, *tmp = nullptr, *crow = nullptr;
-    int64_t *cptr = nullptr, *cursor = nullptr, *inv = nullptr, *nnz_new = nullptr;
+    int64_t *cptr = nullptr, *cursor = nullptr, *inv = nullptr, *nnz_new = nullptr, *pos_of = nullptr;
     uint8_t *ptr_new = nullptr, *col_new = nullptr;
     uint16_t *mask_new = nullptr;
     void *val_new = nullptr;
@@ -455,6 +498,7 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     if (!rc) rc = palloc(ctx, &tmp, (size_t)nt);
     if (!rc) rc = palloc(ctx, &crow, (size_t)nt);
     if (!rc) rc = palloc(ctx, &inv, (size_t)nt);
+    if (!rc) rc = palloc(ctx, &pos_of, (size_t)nt);
     if (!rc) rc = palloc(ctx, &nnz_new, (size_t)nt + 1);
     if (!rc) rc = palloc(ctx, &ptr_new, (size_t)nt * 16);
     if (!rc) rc = palloc(ctx, &mask_new, (size_t)nt * 16);
@@ -472,7 +516,8 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     if (!rc && nt > 0) {
         k_csc_slot<<<blocks_for(nt, 256), 256, 0, s>>>(nt, B->tile_columnidx, rowidx, cursor, tmp);
         k_csc_sort<<<blocks_for(B->tilen, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(B->tilen, cptr, tmp, crow);
-        k_csc_perm<<<blocks_for(nt, 256), 256, 0, s>>>(nt, B->tile_columnidx, rowidx, cptr, crow, B->tile_nnz, inv, nnz_new);
+        k_csc_perm<<<blocks_for(nt, 256), 256, 0, s>>>(nt, B->tile_columnidx, rowidx, cptr, crow, B->tile_nnz, inv, nnz_new,
+                                                        pos_of);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "csc perm launches");
     } else if (!rc) {
         (void)hipMemsetAsync(nnz_new, 0, sizeof(int64_t), s);
@@ -494,7 +539,7 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     pool_free(ctx, tmp);
     pool_free(ctx, inv);
     if (rc) {
-        void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new};
+        void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, pos_of};
         for (void *p : ps) pool_free(ctx, p);
         if (own_rowidx) pool_free(ctx, rowidx);
         return rc;
@@ -508,7 +553,7 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "row_nnz_ptr launch");
     }
     if (rc) {
-        void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp};
+        void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp, pos_of};
         for (void *p : ps) pool_free(ctx, p);
         if (own_rowidx) pool_free(ctx, rowidx);
         return rc;
@@ -534,12 +579,31 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
         }
         if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(TSG_ERR_HIP, "colmajor failed");
         if (rc) {
-            void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp, ccm, ccp, ccv};
+            void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp, ccm, ccp, ccv, pos_of};
             for (void *p : ps) pool_free(ctx, p);
             if (own_rowidx) pool_free(ctx, rowidx);
             return rc;
         }
     }
+    // value positions of the CSR entries in the CSC-order tiles and in the column-major copy
+    int64_t *vpc = nullptr;
+    if (B->val_perm && B->nnz > 0) {
+        if (ccm) rc = palloc(ctx, &vpc, (size_t)B->nnz);
+        if (!rc) {
+            k_val_perm_b<<<blocks_for(B->nnz, 256), 256, 0, s>>>(B->nnz, nt, B->tile_nnz, pos_of, nnz_new,
+                                                                  B->tile_csr_Col, ccm, ccp, B->val_perm, vpc);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+                rc = fail(TSG_ERR_HIP, "value positions failed");
+        }
+        if (rc) {
+            void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp, ccm, ccp, ccv, vpc, pos_of};
+            for (void *p : ps) pool_free(ctx, p);
+            if (own_rowidx) pool_free(ctx, rowidx);
+            return rc;
+        }
+    }
+    pool_free(ctx, pos_of);
+    B->val_perm_col = vpc;
     pool_free(ctx, B->tile_csr_Col);
     pool_free(ctx, B->tile_csr_Ptr);
     pool_free(ctx, B->tile_csr_Val);
@@ -646,4 +710,26 @@ extern "C" int tsg_tile2csr(tsg_context *c, const tsg_tile_matrix *T, tsg_csr *o
         memset(o, 0, sizeof *o);
     }
     return rc;
+}
+
+extern "C" int tsg_tile_set_values(tsg_context *c, tsg_tile_matrix *T, const void *vals) {
+    using namespace tsg;
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !T || (!vals && T->nnz > 0)) return fail(TSG_ERR_ARG, "null argument");
+    if (T->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles are not on the device");
+    if (!T->val_perm) return fail(TSG_ERR_ARG, "no value positions (a matrix made by tsg_csr2tile keeps them)");
+    if (T->csr_col_Val && !T->val_perm_col) return fail(TSG_ERR_ARG, "column-major copy without value positions");
+    (void)hipSetDevice(ctx->device);
+    if (T->nnz == 0) return TSG_OK;
+    const unsigned grid = blocks_for(T->nnz, 256);
+    if (T->dtype == TSG_F32)
+        k_set_values<float><<<grid, 256, 0, ctx->stream>>>(T->nnz, (const float *)vals, T->val_perm, T->val_perm_col,
+                                                            (float *)T->tile_csr_Val, (float *)T->csr_col_Val,
+                                                            (float *)T->row_val);
+    else
+        k_set_values<double><<<grid, 256, 0, ctx->stream>>>(T->nnz, (const double *)vals, T->val_perm,
+                                                             T->val_perm_col, (double *)T->tile_csr_Val,
+                                                             (double *)T->csr_col_Val, (double *)T->row_val);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
 }

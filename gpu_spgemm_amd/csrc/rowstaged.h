@@ -72,7 +72,49 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
 int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                 const Plan &p, const SlotLayout &sl);
 void free_c(Context *ctx, tsg_tile_matrix *C);
+// keep != nullptr: the plan (row bins, links) is handed over instead of freed,
+// for re-running the numeric step.
 int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                  const tsg_options *opts, tsg_stats *st);
+                  const tsg_options *opts, tsg_stats *st, Plan *keep = nullptr);
+
+// What re-running a product's numeric step needs (plan.hip): the kernels the
+// product used and, for the binned pipeline, its plan.
+struct NumericReplay {
+    int path = 0;  // 0 binned pipeline, 1 C-tile kernels, 2 C-tile kernels with matrix cores
+    Plan p;
+    tsg_options o{};
+};
+int spgemm_dispatch(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                    const tsg_options *opts, tsg_stats *st, NumericReplay *keep = nullptr);
+// One contiguous copy of the batch kernels (batch.hip): count elements of
+// `kind` from src to dst, adding `add` to integer kinds.  kind: 0 = bytes,
+// 1 = 16-byte units, 2 = int32 + add, 3 = int64 + add, 4 = 8-byte units,
+// 5 = 4-byte units, 6 = int64 value `add` (no source).
+struct CatJob {
+    const void *src;
+    void *dst;
+    int64_t count;
+    int64_t add;
+    int32_t kind;
+    int32_t pad;
+};
+
+// What re-running a batch's numeric step needs: the block-diagonal operands
+// and product, and the value copies into and out of them (device job tables).
+struct BatchKeep {
+    tsg_tile_matrix BA{}, BB{}, BC{};
+    NumericReplay nr;
+    CatJob *vin = nullptr, *vout = nullptr;
+    int64_t nvin = 0, nvout = 0, vin_max = 0, vout_max = 0;
+};
+int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                   const tsg_options *opts, tsg_stats *stats, BatchKeep *keep = nullptr);
+void free_batch_keep(Context *ctx, BatchKeep &k);
+// Launches the copies of a job table (kernels only).
+int launch_jobs(Context *ctx, const CatJob *dj, int64_t n, int64_t maxc);
+
+// Launches the numeric step again (kernels only: capturable in a graph).
+int replay_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                   NumericReplay &r);
 
 }  // namespace tsg

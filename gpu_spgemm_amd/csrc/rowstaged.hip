@@ -817,7 +817,7 @@ void free_c(Context *ctx, tsg_tile_matrix *C) {
 // binned to the staged kernels or the C-tile kernels.  The staged kernels'
 // capacity flag is read with nnzC's sync.
 int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                  const tsg_options *opts, tsg_stats *st) {
+                  const tsg_options *opts, tsg_stats *st, Plan *keep) {
     const bool timing = opts && opts->timing;
     hipEvent_t ev[4] = {};
     if (timing)
@@ -875,9 +875,26 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
         }
         for (auto &e : ev) (void)hipEventDestroy(e);
     }
+    if (keep && !rc) {
+        *keep = p;
+        p = Plan{};
+    }
     free_plan(ctx, p);
     if (rc) free_c(ctx, C);
     return rc;
+}
+
+int replay_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                   NumericReplay &r) {
+    if (C->nnz == 0) return TSG_OK;
+    if (r.path == 0) {
+        SlotLayout exact;
+        exact.overflow = r.p.overflow;
+        return run_numeric(ctx, A, B, C, r.p, exact);
+    }
+    tsg_tile_matrix Cm = *C;
+    if (r.path == 2) return launch_numeric_mfma(ctx, A, B, &Cm, &r.o);
+    return launch_numeric(ctx, A, B, &Cm, &r.o);
 }
 
 }  // namespace tsg

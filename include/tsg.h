@@ -98,6 +98,10 @@ typedef struct tsg_tile_matrix {
     int64_t *row_ptr;         /* [m+1] */
     int32_t *row_col;         /* [nnz] */
     void *row_val;            /* [nnz] */
+    /* Optional (kept by tsg_csr2tile / tsg_build_csc_tiles): where each CSR
+     * entry's value lives, so tsg_tile_set_values can refresh every copy. */
+    int64_t *val_perm;        /* [nnz] position in tile_csr_Val */
+    int64_t *val_perm_col;    /* [nnz] B only: position in csr_col_Val */
 } tsg_tile_matrix;
 
 typedef struct tsg_context tsg_context;
@@ -191,6 +195,11 @@ int tsg_csr2tile(tsg_context *ctx, const tsg_csr *dev_csr, tsg_tile_matrix *out_
  * operand; replaces build_csc_tiles (src/tile_format.h:424-546).  Works on any
  * row-tiled matrix, including a C output (so products chain without host work). */
 int tsg_build_csc_tiles(tsg_context *ctx, tsg_tile_matrix *B_dev);
+/* New values for a matrix made by tsg_csr2tile (and tsg_build_csc_tiles), same
+ * structure: vals = nnz device values in the CSR order of the original input;
+ * every value copy (tiles, column-major copy, scalar CSR) is rewritten.  No
+ * reference counterpart (the reference rebuilds the tiles on the host). */
+int tsg_tile_set_values(tsg_context *ctx, tsg_tile_matrix *dev, const void *vals);
 /* Row tiles -> CSR (the reference's verification unpacking, src/spgemm_raw_v2.cu:268-284). */
 int tsg_tile2csr(tsg_context *ctx, const tsg_tile_matrix *dev, tsg_csr *out_dev);
 
@@ -270,7 +279,26 @@ int tsg_autotune_set(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, 
                      const tsg_tune_entry *cfg);
 int tsg_autotune_clear(tsg_context *ctx);
 
-/* Many independent products in one persistent launch (no reference counterpart). */
+/* ---- Plans: same structure, new values (no reference counterpart) ----
+ * tsg_plan_create computes C_i = A_i * B_i (count = 1: as tsg_spgemm; count >
+ * 1: as tsg_spgemm_batched, C an array of count outputs) and keeps what the
+ * numeric step needs.  tsg_plan_execute recomputes every C_i's values from the
+ * operands' current values (change them with tsg_tile_set_values) without
+ * touching C's structure: the first call captures the launches -- value copies
+ * into the batch's block-diagonal operands, the numeric kernels, copies out --
+ * into a hipGraph, later calls replay it (one graph launch, no host sync, no
+ * allocation; asynchronous on the context's stream).  Operands and outputs must
+ * keep their arrays while the plan lives; C_i stay the caller's (tsg_tile_free).
+ * Plans use the exact pipeline (the learned scheduler re-plans every call). */
+typedef struct tsg_plan tsg_plan;
+int tsg_plan_create(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                    tsg_tile_matrix *C, const tsg_options *opts, tsg_plan **plan);
+int tsg_plan_execute(tsg_plan *plan);
+int tsg_plan_destroy(tsg_plan *plan);
+
+/* Many independent products in one pipeline run (no reference counterpart):
+ * the batch is one product of block-diagonal operands; C is an array of count
+ * outputs, stats (may be NULL) an array of count entries. */
 int tsg_spgemm_batched(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A,
                        const tsg_tile_matrix *B, tsg_tile_matrix *C, const tsg_options *opts,
                        tsg_stats *stats);
