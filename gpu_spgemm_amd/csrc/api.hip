@@ -357,6 +357,8 @@ int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B) {
     if (A->n != B->m) return fail(TSG_ERR_ARG, "inner dimensions differ: A is %dx%d, B is %dx%d", A->m, A->n, B->m, B->n);
     if (A->dtype != B->dtype) return fail(TSG_ERR_ARG, "A and B value types differ");
     if (!A->mask) return fail(TSG_ERR_ARG, "A lacks row masks");
+    if (A->tilem > MAX_TILE_ROWS || B->tilen > MAX_TILE_ROWS)
+        return fail(TSG_ERR_OVERFLOW, "more than %lld tile-rows or tile-columns", (long long)MAX_TILE_ROWS);
     return TSG_OK;
 }
 

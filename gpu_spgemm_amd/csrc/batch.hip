@@ -144,8 +144,8 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
         bt[i + 1] = bt[i] + B[i].numtile;
         bn[i + 1] = bn[i] + B[i].nnz;
     }
-    if (R[n] * 16 > INT32_MAX || K[n] * 16 > INT32_MAX || Cc[n] * 16 > INT32_MAX)
-        return fail(TSG_ERR_OVERFLOW, "batch dimensions exceed int32 rows/columns");
+    if (R[n] > MAX_TILE_ROWS || K[n] > MAX_TILE_ROWS || Cc[n] > MAX_TILE_ROWS)
+        return fail(TSG_ERR_OVERFLOW, "batch dimensions exceed %lld tile-rows", (long long)MAX_TILE_ROWS);
     const size_t vs = vsize(dtype);
     tsg_tile_matrix BA{}, BB{}, BC{};
     BA.m = (int32_t)(R[n] * 16); BA.n = (int32_t)(K[n] * 16); BA.tilem = (int32_t)R[n]; BA.tilen = (int32_t)K[n];

@@ -148,14 +148,14 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
 // Per tile: exclusive scan of 16 row counts (in place) and the tile's nnz.
 __global__ void __launch_bounds__(256) k_tile_rowscan(int64_t numtile, uint8_t *__restrict__ ptr,
                                                       int64_t *__restrict__ tile_nnz) {
-    const int64_t T = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
-    if (T >= numtile) return;
-    const int c = ptr[T * 16 + l];
-    const int incl = group16_incl_sum(c);
-    ptr[T * 16 + l] = (uint8_t)(incl - c);
-    if (l == 15) tile_nnz[T + 1] = incl;
-    if (T == 0 && l == 0) tile_nnz[0] = 0;
+    for (int64_t T = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4; T < numtile; T += (int64_t)gridDim.x * 16) {
+        const int c = ptr[T * 16 + l];
+        const int incl = group16_incl_sum(c);
+        ptr[T * 16 + l] = (uint8_t)(incl - c);
+        if (l == 15) tile_nnz[T + 1] = incl;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) tile_nnz[0] = 0;
 }
 
 template <typename V>
@@ -191,9 +191,8 @@ k_c2t_scatter(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, c
 // ---------------------------------------------------------------- helpers
 __global__ void __launch_bounds__(256) k_fill_rowidx(const int64_t *__restrict__ tile_ptr, int32_t tilem,
                                                      int32_t *__restrict__ rowidx) {
-    const int32_t bi = (int32_t)(((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE);
-    if (bi >= tilem) return;
-    for (int64_t t = tile_ptr[bi] + lane_id(); t < tile_ptr[bi + 1]; t += WAVE) rowidx[t] = bi;
+    for (int64_t bi = ((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE; bi < tilem; bi += (int64_t)gridDim.x * 4)
+        for (int64_t t = tile_ptr[bi] + lane_id(); t < tile_ptr[bi + 1]; t += WAVE) rowidx[t] = (int32_t)bi;
 }
 
 // ---------------------------------------------------------------- build_csc_tiles
@@ -281,16 +280,16 @@ k_csc_gather(int64_t numtile, const int64_t *__restrict__ inv, const int64_t *__
              const uint16_t *__restrict__ mask_old, const uint8_t *__restrict__ col_old,
              const V *__restrict__ val_old, uint8_t *__restrict__ ptr_new, uint16_t *__restrict__ mask_new,
              uint8_t *__restrict__ col_new, V *__restrict__ val_new) {
-    const int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
     const int l = threadIdx.x & 15;
-    if (p >= numtile) return;
-    const int64_t t = inv[p];
-    ptr_new[p * 16 + l] = ptr_old[t * 16 + l];
-    mask_new[p * 16 + l] = mask_old[t * 16 + l];
-    const int64_t so = nnz_old[t], sn = nnz_new[p], n = nnz_old[t + 1] - so;
-    for (int64_t e = l; e < n; e += 16) {
-        val_new[sn + e] = val_old[so + e];
-        col_new[sn + e] = col_old[so + e] & 15;
+    for (int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4; p < numtile; p += (int64_t)gridDim.x * 16) {
+        const int64_t t = inv[p];
+        ptr_new[p * 16 + l] = ptr_old[t * 16 + l];
+        mask_new[p * 16 + l] = mask_old[t * 16 + l];
+        const int64_t so = nnz_old[t], sn = nnz_new[p], n = nnz_old[t + 1] - so;
+        for (int64_t e = l; e < n; e += 16) {
+            val_new[sn + e] = val_old[so + e];
+            col_new[sn + e] = col_old[so + e] & 15;
+        }
     }
 }
 
@@ -310,9 +309,9 @@ __global__ void __launch_bounds__(256)
 k_colmajor(int64_t numtile, const int64_t *__restrict__ tile_nnz, const uint8_t *__restrict__ ptr,
            const uint16_t *__restrict__ mask, const V *__restrict__ val, uint16_t *__restrict__ cmask,
            uint8_t *__restrict__ cptr, V *__restrict__ cval) {
-    const int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4;
     const int c = threadIdx.x & 15;
-    if (p >= numtile) return;  // whole 16-lane groups leave together
+    // grid-stride over tiles; whole 16-lane groups move together
+    for (int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4; p < numtile; p += (int64_t)gridDim.x * 16) {
     const uint32_t rm = mask[p * 16 + c];  // lane c holds row c's mask and offset
     const int ro = ptr[p * 16 + c];
     const uint32_t cbit = 0x8000u >> c;
@@ -335,6 +334,7 @@ k_colmajor(int64_t numtile, const int64_t *__restrict__ tile_nnz, const uint8_t 
             j++;
         }
     }
+    }
 }
 
 // Value positions after the re-layout: perm[e] (a position in the CSR-order
@@ -346,8 +346,7 @@ k_val_perm_b(int64_t nnz, int64_t numtile, const int64_t *__restrict__ nnz_old, 
              const int64_t *__restrict__ nnz_new, const uint8_t *__restrict__ col_old,
              const uint16_t *__restrict__ cmask, const uint8_t *__restrict__ cptr, int64_t *__restrict__ perm,
              int64_t *__restrict__ perm_col) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= nnz) return;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * 256) {
     const int64_t old = perm[e];
     int64_t lo = 0, hi = numtile - 1;  // last tile whose first entry is at or before `old`
     while (lo < hi) {
@@ -360,6 +359,7 @@ k_val_perm_b(int64_t nnz, int64_t numtile, const int64_t *__restrict__ nnz_old, 
         const int rc = col_old[old], r = rc >> 4, c = rc & 15;
         perm_col[e] = nnz_old[t] + cptr[t * 16 + c] + __popc(cmask[t * 16 + c] & ((0xFFFF0000u >> r) & 0xFFFFu));
     }
+    }
 }
 
 // New values in CSR order into every copy.
@@ -367,12 +367,12 @@ template <typename V>
 __global__ void __launch_bounds__(256)
 k_set_values(int64_t nnz, const V *__restrict__ vals, const int64_t *__restrict__ perm,
              const int64_t *__restrict__ perm_col, V *__restrict__ tval, V *__restrict__ cval, V *__restrict__ rval) {
-    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= nnz) return;
-    const V v = vals[e];
-    tval[perm[e]] = v;
-    if (perm_col) cval[perm_col[e]] = v;
-    if (rval) rval[e] = v;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nnz; e += (int64_t)gridDim.x * 256) {
+        const V v = vals[e];
+        tval[perm[e]] = v;
+        if (perm_col) cval[perm_col[e]] = v;
+        if (rval) rval[e] = v;
+    }
 }
 
 static unsigned blocks_for(int64_t threads, int per) { return (unsigned)((threads + per - 1) / per); }
@@ -390,6 +390,12 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
     memset(T, 0, sizeof *T);
     T->m = A->m; T->n = A->n; T->nnz = A->nnz; T->dtype = A->dtype; T->location = TSG_DEVICE;
     T->tilem = (A->m + 15) / 16; T->tilen = (A->n + 15) / 16;
+    if (T->tilem > MAX_TILE_ROWS || T->tilen > MAX_TILE_ROWS) {
+        const int32_t m = A->m, n = A->n;
+        memset(T, 0, sizeof *T);
+        return fail(TSG_ERR_OVERFLOW, "%d x %d: more than %lld tile-rows or tile-columns", m, n,
+                    (long long)MAX_TILE_ROWS);
+    }
     hipStream_t s = ctx->stream;
     int32_t *flags = nullptr;
     int rc = palloc(ctx, &flags, 1);
@@ -435,7 +441,7 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "k_c2t fill launch"));
     }
     if (nt > 0) {
-        k_tile_rowscan<<<blocks_for(nt * 16, 256), 256, 0, s>>>(nt, T->tile_csr_Ptr, T->tile_nnz);
+        k_tile_rowscan<<<grid_capped(nt, 16), 256, 0, s>>>(nt, T->tile_csr_Ptr, T->tile_nnz);
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "rowscan launch"));
     } else {
         (void)hipMemsetAsync(T->tile_nnz, 0, sizeof(int64_t), s);
@@ -489,7 +495,7 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
         rc = palloc(ctx, &rowidx, (size_t)nt);
         own_rowidx = true;
         if (!rc && B->tilem > 0) {
-            k_fill_rowidx<<<blocks_for((int64_t)B->tilem * WAVE, 256), 256, 0, s>>>(B->tile_ptr, B->tilem, rowidx);
+            k_fill_rowidx<<<grid_capped(B->tilem, 4), 256, 0, s>>>(B->tile_ptr, B->tilem, rowidx);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "rowidx launch");
         }
     }
@@ -525,11 +531,11 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     if (!rc) rc = scan_inclusive_i64(ctx, nnz_new, nt + 1);
     if (!rc && nt > 0) {
         if (B->dtype == TSG_F32)
-            k_csc_gather<float><<<blocks_for(nt * 16, 256), 256, 0, s>>>(
+            k_csc_gather<float><<<grid_capped(nt, 16), 256, 0, s>>>(
                 nt, inv, B->tile_nnz, nnz_new, B->tile_csr_Ptr, B->mask, B->tile_csr_Col,
                 (const float *)B->tile_csr_Val, ptr_new, mask_new, col_new, (float *)val_new);
         else
-            k_csc_gather<double><<<blocks_for(nt * 16, 256), 256, 0, s>>>(
+            k_csc_gather<double><<<grid_capped(nt, 16), 256, 0, s>>>(
                 nt, inv, B->tile_nnz, nnz_new, B->tile_csr_Ptr, B->mask, B->tile_csr_Col,
                 (const double *)B->tile_csr_Val, ptr_new, mask_new, col_new, (double *)val_new);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "csc gather launch");
@@ -569,10 +575,10 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
         if (!rc) rc = pool_alloc(ctx, &ccv, (size_t)B->nnz * vsize(B->dtype));
         if (!rc && nt > 0) {
             if (B->dtype == TSG_F32)
-                k_colmajor<float><<<blocks_for(nt * 16, 256), 256, 0, s>>>(
+                k_colmajor<float><<<grid_capped(nt, 16), 256, 0, s>>>(
                     nt, B->tile_nnz, B->tile_csr_Ptr, B->mask, (const float *)B->tile_csr_Val, ccm, ccp, (float *)ccv);
             else
-                k_colmajor<double><<<blocks_for(nt * 16, 256), 256, 0, s>>>(
+                k_colmajor<double><<<grid_capped(nt, 16), 256, 0, s>>>(
                     nt, B->tile_nnz, B->tile_csr_Ptr, B->mask, (const double *)B->tile_csr_Val, ccm, ccp,
                     (double *)ccv);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "colmajor launch");
@@ -590,7 +596,7 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     if (B->val_perm && B->nnz > 0) {
         if (ccm) rc = palloc(ctx, &vpc, (size_t)B->nnz);
         if (!rc) {
-            k_val_perm_b<<<blocks_for(B->nnz, 256), 256, 0, s>>>(B->nnz, nt, B->tile_nnz, pos_of, nnz_new,
+            k_val_perm_b<<<grid_capped(B->nnz, 256), 256, 0, s>>>(B->nnz, nt, B->tile_nnz, pos_of, nnz_new,
                                                                   B->tile_csr_Col, ccm, ccp, B->val_perm, vpc);
             if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
                 rc = fail(TSG_ERR_HIP, "value positions failed");
@@ -721,7 +727,7 @@ extern "C" int tsg_tile_set_values(tsg_context *c, tsg_tile_matrix *T, const voi
     if (T->csr_col_Val && !T->val_perm_col) return fail(TSG_ERR_ARG, "column-major copy without value positions");
     (void)hipSetDevice(ctx->device);
     if (T->nnz == 0) return TSG_OK;
-    const unsigned grid = blocks_for(T->nnz, 256);
+    const unsigned grid = grid_capped(T->nnz, 256);
     if (T->dtype == TSG_F32)
         k_set_values<float><<<grid, 256, 0, ctx->stream>>>(T->nnz, (const float *)vals, T->val_perm, T->val_perm_col,
                                                             (float *)T->tile_csr_Val, (float *)T->csr_col_Val,

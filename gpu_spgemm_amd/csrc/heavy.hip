@@ -28,6 +28,8 @@
 // A C tile column is mapped to its position in the row through an LDS table
 // over the row's tile-column window (RG_MAP entries), or by binary search in
 // the row's sorted tile columns when the window is wider.
+#include <algorithm>
+
 #include "tsg_internal.h"
 
 namespace tsg {
@@ -197,30 +199,32 @@ k_rg_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int32_t m, con
 
 int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                        const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
-    if (count <= 0) return TSG_OK;
-    if (count > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "row list too long");
-    k_rg_symbolic<<<(unsigned)count, RG_THREADS, 0, ctx->stream>>>(
-        rows, begin, A->m, A->row_ptr, A->row_col, B->row_ptr, B->row_col, C->tile_ptr, sl.row_nc,
-        C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz);
-    TSG_HIP(hipGetLastError());
+    for (int64_t c0 = 0; c0 < count; c0 += ROW_CHUNK) {  // work-items per launch stay below 2^32
+        const unsigned n = (unsigned)std::min<int64_t>(ROW_CHUNK, count - c0);
+        k_rg_symbolic<<<n, RG_THREADS, 0, ctx->stream>>>(
+            rows, begin + c0, A->m, A->row_ptr, A->row_col, B->row_ptr, B->row_col, C->tile_ptr, sl.row_nc,
+            C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz);
+        TSG_HIP(hipGetLastError());
+    }
     return TSG_OK;
 }
 
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                       const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
-    if (count <= 0) return TSG_OK;
-    if (count > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "row list too long");
+    for (int64_t c0 = 0; c0 < count; c0 += ROW_CHUNK) {  // work-items per launch stay below 2^32
+    const unsigned n = (unsigned)std::min<int64_t>(ROW_CHUNK, count - c0);
     if (A->dtype == TSG_F32)
-        k_rg_numeric<float><<<(unsigned)count, RG_THREADS, 0, ctx->stream>>>(
-            rows, begin, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr, B->row_col,
+        k_rg_numeric<float><<<n, RG_THREADS, 0, ctx->stream>>>(
+            rows, begin + c0, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr, B->row_col,
             (const float *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr,
             C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
     else
-        k_rg_numeric<double><<<(unsigned)count, RG_THREADS, 0, ctx->stream>>>(
-            rows, begin, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr, B->row_col,
+        k_rg_numeric<double><<<n, RG_THREADS, 0, ctx->stream>>>(
+            rows, begin + c0, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr, B->row_col,
             (const double *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr,
             C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
     TSG_HIP(hipGetLastError());
+    }
     return TSG_OK;
 }
 

@@ -62,10 +62,11 @@ k_numeric_mfma(int64_t numtileC, int64_t min_prod16, const int64_t *__restrict__
                const uint16_t *__restrict__ maskC, const uint8_t *__restrict__ ptrRowC, V *__restrict__ valC,
                uint8_t *__restrict__ colC) {
     typedef typename MfAcc<V>::T Acc;
-    const int64_t ti = (int64_t)blockIdx.x * MF_WAVES + threadIdx.x / WAVE;
-    if (ti >= numtileC) return;  // whole wave leaves together
+    // grid-stride over tiles, one wave per tile (the matrix cores need the whole wave)
+    for (int64_t ti = (int64_t)blockIdx.x * MF_WAVES + threadIdx.x / WAVE; ti < numtileC;
+         ti += (int64_t)gridDim.x * MF_WAVES) {
     const int64_t c0 = tile_nnzC[ti];
-    if (tile_nnzC[ti + 1] == c0) return;  // structurally empty tile
+    if (tile_nnzC[ti + 1] == c0) continue;  // structurally empty tile
     const int l = lane_id();
     const int col = l & 15, quad = l >> 4;  // output column; operand k within a 4-step, output row block
     const int32_t bi = rowidxC[ti], bj = colidxC[ti];
@@ -116,6 +117,7 @@ k_numeric_mfma(int64_t numtileC, int64_t min_prod16, const int64_t *__restrict__
             colC[o] = (uint8_t)((r << 4) | col);
         }
     }
+    }
 }
 
 int launch_numeric_mfma(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
@@ -123,16 +125,15 @@ int launch_numeric_mfma(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_m
     if (C->numtile == 0 || C->nnz == 0) return TSG_OK;
     if ((A->flags | B->flags) & 1) return fail(TSG_ERR_UNSUPPORTED, "the matrix-core kernel needs rows without duplicate columns");
     const int64_t minp = (opts && opts->mfma_min_products > 0) ? opts->mfma_min_products : 256;
-    const int64_t grid = (C->numtile + MF_WAVES - 1) / MF_WAVES;
-    if (grid > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "numeric grid too large");
+    const unsigned grid = grid_capped(C->numtile, MF_WAVES);
     if (A->dtype == TSG_F32)
-        k_numeric_mfma<float><<<(unsigned)grid, MF_WAVES * WAVE, 0, ctx->stream>>>(
+        k_numeric_mfma<float><<<grid, MF_WAVES * WAVE, 0, ctx->stream>>>(
             C->numtile, minp * 16, A->tile_ptr, A->tile_columnidx, A->tile_nnz, A->tile_csr_Ptr, A->mask,
             (const float *)A->tile_csr_Val, B->csc_tile_ptr, B->csc_tile_rowidx, B->tile_nnz, B->tile_csr_Ptr,
             B->mask, (const float *)B->tile_csr_Val, C->tile_rowidx, C->tile_columnidx, C->tile_nnz, C->mask,
             C->tile_csr_Ptr, (float *)C->tile_csr_Val, C->tile_csr_Col);
     else
-        k_numeric_mfma<double><<<(unsigned)grid, MF_WAVES * WAVE, 0, ctx->stream>>>(
+        k_numeric_mfma<double><<<grid, MF_WAVES * WAVE, 0, ctx->stream>>>(
             C->numtile, minp * 16, A->tile_ptr, A->tile_columnidx, A->tile_nnz, A->tile_csr_Ptr, A->mask,
             (const double *)A->tile_csr_Val, B->csc_tile_ptr, B->csc_tile_rowidx, B->tile_nnz, B->tile_csr_Ptr,
             B->mask, (const double *)B->tile_csr_Val, C->tile_rowidx, C->tile_columnidx, C->tile_nnz, C->mask,

@@ -112,9 +112,9 @@ __device__ __forceinline__ void num_tile(int64_t ti, const NumArgs<V> &p) {
 
 template <typename V, bool DUPS>
 __global__ void __launch_bounds__(NU_THREADS) k_numeric_tiles(int64_t numtileC, NumArgs<V> p) {
-    const int64_t ti = (int64_t)blockIdx.x * NU_GROUPS + (threadIdx.x >> 4);
-    if (ti >= numtileC) return;
-    num_tile<V, DUPS>(ti, p);
+    for (int64_t ti = (int64_t)blockIdx.x * NU_GROUPS + (threadIdx.x >> 4); ti < numtileC;
+         ti += (int64_t)gridDim.x * NU_GROUPS)  // grid-stride over tiles
+        num_tile<V, DUPS>(ti, p);
 }
 
 // Heavy tile-rows: one workgroup per listed row, its 16 groups stride over the
@@ -135,11 +135,14 @@ static int launch_list_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
     a.row_nc = sl.row_nc;
     a.overflow = sl.overflow;
     a.val_cap = sl.val_cap;
-    if ((A->flags | B->flags) & 1)
-        k_numeric_list<V, true><<<(unsigned)count, NU_THREADS, 0, ctx->stream>>>(rows, begin, a);
-    else
-        k_numeric_list<V, false><<<(unsigned)count, NU_THREADS, 0, ctx->stream>>>(rows, begin, a);
-    TSG_HIP(hipGetLastError());
+    for (int64_t c0 = 0; c0 < count; c0 += ROW_CHUNK) {  // work-items per launch stay below 2^32
+        const unsigned n = (unsigned)(count - c0 < ROW_CHUNK ? count - c0 : ROW_CHUNK);
+        if ((A->flags | B->flags) & 1)
+            k_numeric_list<V, true><<<n, NU_THREADS, 0, ctx->stream>>>(rows, begin + c0, a);
+        else
+            k_numeric_list<V, false><<<n, NU_THREADS, 0, ctx->stream>>>(rows, begin + c0, a);
+        TSG_HIP(hipGetLastError());
+    }
     return TSG_OK;
 }
 
@@ -157,7 +160,6 @@ static int launch_tiles_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
 int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                         const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
     if (count <= 0) return TSG_OK;
-    if (count > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "numeric row list too long");
     if (A->dtype == TSG_F32) return launch_list_t<float>(ctx, A, B, C, rows, begin, count, sl);
     return launch_list_t<double>(ctx, A, B, C, rows, begin, count, sl);
 }
@@ -166,10 +168,9 @@ int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
                    const tsg_options *opts) {
     (void)opts;
     if (C->numtile == 0 || C->nnz == 0) return TSG_OK;
-    int64_t grid = (C->numtile + NU_GROUPS - 1) / NU_GROUPS;
-    if (grid > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "numeric grid too large");
-    if (A->dtype == TSG_F32) return launch_tiles_t<float>(ctx, A, B, C, (unsigned)grid);
-    return launch_tiles_t<double>(ctx, A, B, C, (unsigned)grid);
+    const unsigned grid = grid_capped(C->numtile, NU_GROUPS);
+    if (A->dtype == TSG_F32) return launch_tiles_t<float>(ctx, A, B, C, grid);
+    return launch_tiles_t<double>(ctx, A, B, C, grid);
 }
 
 }  // namespace tsg

@@ -60,11 +60,11 @@ k_symbolic(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ co
            uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC) {
     __shared__ uint16_t s_mb[SY_GROUPS][16];
     const int g = threadIdx.x >> 4;
-    const int64_t ti = (int64_t)blockIdx.x * SY_GROUPS + g;
-    if (ti >= numtileC) return;  // whole group leaves; only wave-level syncs below
-    sym_tile(ti, tile_ptrA, colidxA, maskA, csc_ptrB, csc_rowB, maskB, rowidxC, colidxC, maskC, ptrRowC,
-             tile_nnzC, s_mb[g]);
-    if (ti == 0 && (threadIdx.x & 15) == 0) tile_nnzC[0] = 0;
+    // grid-stride over tiles (groups leave independently; only wave-level syncs below)
+    for (int64_t ti = (int64_t)blockIdx.x * SY_GROUPS + g; ti < numtileC; ti += (int64_t)gridDim.x * SY_GROUPS)
+        sym_tile(ti, tile_ptrA, colidxA, maskA, csc_ptrB, csc_rowB, maskB, rowidxC, colidxC, maskC, ptrRowC,
+                 tile_nnzC, s_mb[g]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) tile_nnzC[0] = 0;
 }
 
 // Heavy tile-rows: one workgroup per listed row, its 16 groups stride over the
@@ -89,11 +89,14 @@ k_symbolic_list(const int32_t *__restrict__ rows, int64_t row_begin, const int64
 
 int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const int32_t *rows,
                          int64_t begin, int64_t count, const tsg_tile_matrix *C, const SlotLayout &sl) {
-    if (count <= 0) return TSG_OK;
-    k_symbolic_list<<<(unsigned)count, SY_THREADS, 0, ctx->stream>>>(
-        rows, begin, C->tile_ptr, A->tile_ptr, A->tile_columnidx, A->mask, B->csc_tile_ptr, B->csc_tile_rowidx,
-        B->mask, C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, sl.row_nc);
-    TSG_HIP(hipGetLastError());
+    for (int64_t c0 = 0; c0 < count; c0 += ROW_CHUNK) {  // work-items per launch stay below 2^32
+        const unsigned n = (unsigned)(count - c0 < ROW_CHUNK ? count - c0 : ROW_CHUNK);
+        k_symbolic_list<<<n, SY_THREADS, 0, ctx->stream>>>(
+            rows, begin + c0, C->tile_ptr, A->tile_ptr, A->tile_columnidx, A->mask, B->csc_tile_ptr,
+            B->csc_tile_rowidx, B->mask, C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz,
+            sl.row_nc);
+        TSG_HIP(hipGetLastError());
+    }
     return TSG_OK;
 }
 
@@ -104,9 +107,7 @@ int launch_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matri
         TSG_HIP(hipMemsetAsync(tile_nnzC, 0, sizeof(int64_t), ctx->stream));
         return TSG_OK;
     }
-    int64_t grid = (numtileC + SY_GROUPS - 1) / SY_GROUPS;
-    if (grid > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "symbolic grid too large");
-    k_symbolic<<<(unsigned)grid, SY_THREADS, 0, ctx->stream>>>(
+    k_symbolic<<<grid_capped(numtileC, SY_GROUPS), SY_THREADS, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->mask, B->csc_tile_ptr, B->csc_tile_rowidx, B->mask, numtileC,
         rowidxC, colidxC, maskC, ptrRowC, tile_nnzC);
     TSG_HIP(hipGetLastError());

@@ -76,6 +76,19 @@ inline int palloc(Context *ctx, T **p, size_t count) {
 
 inline size_t vsize(int dtype) { return dtype == TSG_F32 ? 4 : 8; }
 
+// A dispatch's work-item count (blocks x threads) is a 32-bit quantity: kernels
+// over index spaces that can exceed it run a capped grid and stride.
+constexpr int64_t MAX_GRID_BLOCKS = int64_t(1) << 22;
+inline unsigned grid_capped(int64_t items, int per_block) {
+    int64_t b = (items + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    return (unsigned)(b < MAX_GRID_BLOCKS ? b : MAX_GRID_BLOCKS);
+}
+// Tile-rows at or above this would make one wave per tile-row reach 2^32
+// work-items; launches of one workgroup per row go in chunks of ROW_CHUNK.
+constexpr int64_t MAX_TILE_ROWS = (int64_t(1) << 26) - 64;
+constexpr int64_t ROW_CHUNK = int64_t(1) << 23;
+
 // ---------------------------------------------------------------- scans (scan.hip)
 // In-place inclusive scan of n int64 values.  With data[0] = 0 and counts in
 // data[1..n-1], this leaves exclusive offsets (the reference's CUB
