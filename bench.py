@@ -41,6 +41,10 @@ def make_workload(g, name):
     if name == "band":
         A = g.gen_band(65536, 48, seed=0)
         return A, A, {"workload": "dense_band_65536_halfwidth48_AxA", "rows": A.m, "nnzA": A.nnz, "dtype": "f64"}
+    if name.startswith("rmat"):  # R-MAT, edge factor 16, (0.57, 0.19, 0.19, 0.05), FP32, seed 1
+        sc = int(name[4:])
+        A = g.gen_rmat(sc, 16, seed=1, dtype=g.F32)
+        return A, A, {"workload": f"rmat_scale{sc}_ef16_AxA", "rows": A.m, "nnzA": A.nnz, "dtype": "f32"}
     if name == "stencil27_AP":
         A = g.gen_stencil27(128, 128, 128)
         P = g.gen_aggregation(128, 128, 128)
@@ -67,13 +71,16 @@ def numeric_bytes(A, B, C, s):
     return a + b + c
 
 
-def train_learned(g, ctx, log):
+def train_learned(g, ctx, log, rmat_scale=0):
     """Trains the learned scheduler on a family of smaller matrices (never the
-    benchmarked product itself), outside any timed region."""
+    benchmarked product itself), outside any timed region; for an R-MAT
+    workload the family adds R-MATs 4-16x smaller with other seeds."""
     fam = [g.gen_laplace2d(256, 256), g.gen_laplace2d(512, 384), g.gen_laplace2d(300, 700),
            g.gen_laplace3d(40, 40, 40), g.gen_stencil27(32, 32, 32), g.gen_er(20000, 6, seed=11),
            g.gen_er(20000, 12, seed=12), g.gen_er_var(20000, 0, 24, seed=14),
            g.gen_rmat(13, 8, seed=15, dtype=g.F64)]
+    if rmat_scale:
+        fam += [g.gen_rmat(sc, 16, seed=100 + sc, dtype=g.F32) for sc in range(rmat_scale - 4, rmat_scale - 1)]
     ctx.learned_clear()
     t0 = time.time()
     for M in fam:
@@ -226,21 +233,46 @@ def bench_galerkin(g, args):
 
 
 def cpu_baseline(A, B, seconds):
+    """The oracle's Gustavson loop (the reference's CPU check), 1 thread: the
+    whole product when it finishes within `seconds`, else row blocks at 8
+    evenly spaced starts holding about `seconds` worth of products in total."""
+    import numpy as np
     from oracle import oracle as O
     m = A.m
-    times = []
-    prod = 0
-    t_end = time.time() + seconds
-    while time.time() < t_end or len(times) < 3:
-        prod, gus, _ = O.cpu_baseline(A, B, 0, m, nthreads=1, convert=False)
-        times.append(gus)
-        if len(times) >= 7:
-            break
-    med = statistics.median(times)
-    return {"value": round(2.0 * prod / med / 1e9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
-            "sample": f"whole product, Gustavson FP64 loop of the reference's CPU check "
-                      f"(src/spgemm_raw_v2.cu:297-308), 1 thread, median of {len(times)} runs "
-                      f"({med:.3f} s each, {prod} products)"}
+    cum = np.concatenate([[0], np.cumsum(np.diff(B.rowptr)[A.col], dtype=np.int64)])[A.rowptr]
+    total = int(cum[-1])
+
+    def block(start, want, stop):  # rows [start, end <= stop) holding about `want` products
+        end = int(np.searchsorted(cum, cum[start] + want, side="left"))
+        return start, max(start + 1, min(stop, end))
+
+    r0, r1 = block(m // 2, min(total, 20_000_000), m)
+    p0, g0, _ = O.cpu_baseline(A, B, r0, r1, nthreads=1, convert=False)
+    rate = p0 / max(g0, 1e-6)
+    if total / rate <= seconds:
+        times = []
+        prod = 0
+        t_end = time.time() + seconds
+        while time.time() < t_end or len(times) < 3:
+            prod, gus, _ = O.cpu_baseline(A, B, 0, m, nthreads=1, convert=False)
+            times.append(gus)
+            if len(times) >= 7:
+                break
+        med = statistics.median(times)
+        return {"value": round(2.0 * prod / med / 1e9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+                "sample": f"whole product, Gustavson FP64 loop of the reference's CPU check "
+                          f"(src/spgemm_raw_v2.cu:297-308), 1 thread, median of {len(times)} runs "
+                          f"({med:.3f} s each, {prod} products)"}
+    want = rate * seconds / 8
+    prod, gus, rows = 0, 0.0, 0
+    for i in range(8):
+        a, b = block(i * m // 8, want, (i + 1) * m // 8)
+        p, t, _ = O.cpu_baseline(A, B, a, b, nthreads=1, convert=False)
+        prod, gus, rows = prod + p, gus + t, rows + (b - a)
+    return {"value": round(2.0 * prod / gus / 1e9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"{rows} of {m} rows in 8 blocks at evenly spaced starts, {prod} of {total} products "
+                      f"({gus:.2f} s), Gustavson FP64 loop of the reference's CPU check "
+                      f"(src/spgemm_raw_v2.cu:297-308), 1 thread"}
 
 
 def main():
@@ -277,7 +309,8 @@ def main():
     log("device", ctx.device_name())
     tA = ctx.tiles_from_csr(A)
     tB = ctx.tiles_from_csr(B, as_b=True)
-    s = 8
+    s = 4 if cfg["dtype"] == "f32" else 8
+    rmat = args.workload.startswith("rmat")
 
     def measure(label, **kw):
         for _ in range(args.warmup):
@@ -304,15 +337,17 @@ def main():
     Ad = {"tilem": tA.s.tilem, "tilen": tA.s.tilen, "numtile": tA.s.numtile, "nnz": tA.s.nnz}
     Bd = {"tilem": tB.s.tilem, "tilen": tB.s.tilen, "numtile": tB.s.numtile, "nnz": tB.s.nnz}
     runs = {}
-    runs["ctile"] = measure("reference C-tile method", variant=1)
+    if not rmat:   # R-MAT compares the learned scheduler with the exact binned pipeline
+        runs["ctile"] = measure("reference C-tile method", variant=1)
     runs["plain"] = measure("plain (binned, exact sizes)", variant=0)
     if args.workload == "band":     # full tiles: matrix cores for the dense pairs
         runs["mfma"] = measure("C-tile + matrix cores", variant=2)
-    if args.scheduler == "learned":
-        train_learned(g, ctx, log)
+    if args.scheduler == "learned" or rmat:
+        train_learned(g, ctx, log, int(args.workload[4:]) if rmat else 0)
         _, st = ctx.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
         log(f"learned: fallback {st['fallback_used']} slots {st['pred_numtileC']} (tiles {st['numtileC']}) "
             f"values {st['pred_nnzC']} (nnz {st['nnzC']})")
+        learned_fallback = st["fallback_used"]
         runs["learned"] = measure("learned", scheduler=g.SCHED_LEARNED)
     best = min(runs, key=lambda k: runs[k][0])      # the faster exact-result scheduler is the headline
     ms, ev, Cd = runs[best]
@@ -325,6 +360,8 @@ def main():
     # kernel (every tile-row fits the small variant); its HIP-event time is step3
     kname = {"plain": "k_num_staged<double,VarS>", "mfma": "k_numeric_mfma<double>",
              "ctile": "k_numeric_tiles<double>"}.get(best, "numeric step (learned)")
+    if rmat:   # heavy rows: several numeric launches (row-by-row and C-tile kernels)
+        kname = "numeric step (all bins)"
     out = {
         "metric": "SpGEMM GFLOP/s + HBM GB/s, learned/batched vs plain tiled, 1 MI355X",
         "value": round(gflops, 3),
@@ -351,6 +388,9 @@ def main():
                      "launch_ms": round(step3, 5)},
         "cpu_baseline": None,
     }
+    if "learned" in runs:
+        out["learned_vs_plain"] = round(runs["plain"][0] / runs["learned"][0], 3)
+        out["learned_fallback"] = learned_fallback
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds)

@@ -9,222 +9,389 @@
 // scalar CSR copies of A and B: for each row i of the tile-row, each A entry
 // (i, k) in ascending k, each B entry (k, j).
 //
-//   k_rg_symbolic: C's tile columns of the row are already written (Step 1
-//     fill); each product sets bit c = j % 16 of row mask (i % 16) in C tile
-//     j / 16 (atomic OR on the row's own tiles, zeroed first), then the row's
-//     tiles get their row offsets and value counts.  The masks equal the
-//     reference's OR over tile pairs: a column is present exactly when some
-//     product reaches it.
-//   k_rg_numeric: the row's values are zeroed and their col bytes written;
-//     then thread (r, s) takes scalar row r and slice s of the row's C tiles
-//     (16 slices of consecutive tiles, hence of consecutive columns): it walks
-//     the row's A entries in ascending k, finds the slice's column range in
-//     B row k by binary search, and accumulates each product there with one
-//     FMA into the output's slot.  Every output belongs to one thread and sums
-//     its products in ascending k -- the order of the tiled method (tile pairs
-//     ascending, entries ascending within a pair) -- so values are
-//     bit-identical to the other kernels.
+// Work is split into windows of RGW_COLS consecutive C tile columns, counted
+// from the tile-row's first C tile column (its C tile columns are already
+// written by the Step-1 fill).  A work item is a run of consecutive windows
+// of one tile-row -- about RGW_ITEM_TILES C tiles' worth, at most RGW_GMAX
+// windows -- and a persistent grid takes items from a device counter, so the
+// few very heavy tile-rows of a power-law matrix spread over many workgroups.
+// Inside a window everything lives in LDS:
 //
-// A C tile column is mapped to its position in the row through an LDS table
-// over the row's tile-column window (RG_MAP entries), or by binary search in
-// the row's sorted tile columns when the window is wider.
+//   k_rgw_symbolic: the window's row masks (RGW_COLS tiles x 16 rows, dense
+//     by tile column), set by an LDS atomic OR per product; then the window's
+//     C tiles get their masks, row offsets and value counts.  A column is
+//     present exactly when some product reaches it, which is the reference's
+//     OR over tile pairs.  Long B row segments are walked by the whole
+//     workgroup.
+//   k_rgw_numeric: the window's values (in sub-windows of at most RGW_TILES
+//     tiles and RGW_VBYTES bytes of values), a map tile column -> sub-window
+//     tile, and each tile's packed (row offset, row mask) words.  Scalar row r
+//     of the tile-row belongs to lanes r*16..r*16+15; they take the row's A
+//     entries in ascending k, one at a time, and spread that entry's B
+//     segment over the 16 lanes.  The outputs of one A entry are distinct and
+//     the entries follow in k order, so each output sums its products in
+//     ascending k with one FMA each -- the order of the tiled method (tile
+//     pairs ascending, entries ascending within a pair) -- and values are
+//     bit-identical to the other kernels.
 #include <algorithm>
 
-#include "tsg_internal.h"
+#include "rowstaged.h"
 
 namespace tsg {
 
-constexpr int RG_THREADS = 256;   // 16 threads per scalar row
-constexpr int RG_MAP = 8192;      // tile columns mapped through LDS (16 KB)
+constexpr int RGW_THREADS = 256;       // 16 scalar rows x 16 lanes
+constexpr int RGW_COLS = 1024;         // C tile columns per window
+constexpr int RGW_ITEM_TILES = 4096;   // C tiles per work item (aim)
+constexpr int RGW_GMAX = 64;           // windows per work item (cap)
+constexpr int RGW_TILES = 256;         // tiles per numeric sub-window
+constexpr int RGW_VBYTES = 32768;      // value bytes per numeric sub-window
+constexpr int RGW_LONG = 64;           // B segments longer than this are walked by the workgroup
 
-__device__ __forceinline__ uint32_t rg_below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
+__device__ __forceinline__ uint32_t rgw_below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
 
 template <typename V>
-__device__ __forceinline__ V rg_fma(V a, V b, V c);
+__device__ __forceinline__ V rgw_fma(V a, V b, V c);
 template <>
-__device__ __forceinline__ double rg_fma<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ double rgw_fma<double>(double a, double b, double c) { return __builtin_fma(a, b, c); }
 template <>
-__device__ __forceinline__ float rg_fma<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ float rgw_fma<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-// Position of C tile column bj in [cs, ce): LDS map when the window fits,
-// else binary search in the sorted global list.  In the slot layout a row
-// whose tiles overflowed their slots holds only some of its columns (its
-// result is discarded); `verify` then returns -1 for a column not found.
-__device__ __forceinline__ int64_t rg_tile(int32_t bj, bool use_map, const uint16_t *map, int32_t cmin,
-                                           int32_t cmax, const int32_t *__restrict__ colidxC, int64_t cs,
-                                           int64_t ce, bool verify) {
-    int64_t t;
-    if (use_map) {
-        if (verify && (bj < cmin || bj > cmax)) return -1;
-        t = cs + map[bj - cmin];
-    } else {
-        int64_t lo = cs, hi = ce - 1;
+// Windows of a row with C tiles [cs, ce) and how many go in one work item.
+__device__ __forceinline__ void rgw_split(const int32_t *__restrict__ colidxC, int64_t cs, int64_t ce,
+                                          int64_t &nwin, int64_t &per_item) {
+    nwin = ce > cs ? (int64_t)(colidxC[ce - 1] - colidxC[cs]) / RGW_COLS + 1 : 0;
+    const int64_t items = max<int64_t>(max<int64_t>(1, (ce - cs + RGW_ITEM_TILES - 1) / RGW_ITEM_TILES),
+                                       (nwin + RGW_GMAX - 1) / RGW_GMAX);
+    per_item = nwin > 0 ? (nwin + min(items, nwin) - 1) / min(items, nwin) : 1;
+}
+
+// Work items of each listed row: woff[q + 1] = item count of row q, woff[0] = 0
+// (scanned afterwards into item offsets).
+__global__ void __launch_bounds__(256)
+k_rgw_count(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ tile_ptrC,
+            const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC, int64_t *__restrict__ woff) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q == 0) woff[0] = 0;
+    if (q >= nr) return;
+    int64_t cs, ce, nwin, per;
+    c_row_range(tile_ptrC, row_nc, rows[row_begin + q], cs, ce);
+    rgw_split(colidxC, cs, ce, nwin, per);
+    woff[q + 1] = (nwin + per - 1) / per;
+}
+
+// Clears the two work counters (a kernel, so the numeric launch can be captured in a graph).
+__global__ void k_rgw_zero(unsigned long long *ctr) {
+    if (threadIdx.x < 2) ctr[threadIdx.x] = 0ull;
+}
+
+struct RgwItem {
+    int32_t bi, c0, nw;  // tile-row, first tile column of the item's first window, windows
+    int64_t cs, ce;      // the tile-row's C tiles
+};
+
+// Set-up of one work item, by every thread of the block: the row and windows
+// of `item`, the tile-row's scalar row pointers s_rp[0..16] and the first C
+// tile of each window s_pb[0..nw] (window w covers tiles [s_pb[w], s_pb[w+1])).
+__device__ __forceinline__ void rgw_setup(int64_t item, const int32_t *__restrict__ rows, int64_t row_begin,
+                                          int64_t nr, const int64_t *__restrict__ woff, int32_t m,
+                                          const int64_t *__restrict__ rpA, const int64_t *__restrict__ tile_ptrC,
+                                          const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC,
+                                          RgwItem &s_it, int64_t *s_rp, int64_t *s_pb) {
+    if (threadIdx.x == 0) {
+        int64_t lo = 0, hi = nr - 1;  // largest q with woff[q] <= item
         while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (colidxC[mid] < bj) lo = mid + 1; else hi = mid;
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (woff[mid] <= item) lo = mid; else hi = mid - 1;
         }
-        t = lo;
+        const int32_t bi = rows[row_begin + lo];
+        int64_t cs, ce, nwin, per;
+        c_row_range(tile_ptrC, row_nc, bi, cs, ce);
+        rgw_split(colidxC, cs, ce, nwin, per);
+        const int64_t w0 = (item - woff[lo]) * per;
+        s_it.bi = bi;
+        s_it.c0 = colidxC[cs] + (int32_t)w0 * RGW_COLS;
+        s_it.nw = (int32_t)min<int64_t>(per, nwin - w0);
+        s_it.cs = cs;
+        s_it.ce = ce;
     }
-    if (verify && (t < cs || t >= ce || colidxC[t] != bj)) return -1;
-    return t;
-}
-
-// Builds the map (every thread of the block calls it; map entries of absent
-// columns stay unset and are only read under `verify`).
-__device__ __forceinline__ bool rg_build_map(uint16_t *map, const int32_t *__restrict__ colidxC, int64_t cs,
-                                             int64_t ce, int32_t &cmin, int32_t &cmax) {
-    const int64_t nc = ce - cs;
-    cmin = colidxC[cs];
-    cmax = colidxC[ce - 1];
-    const bool use = (int64_t)cmax - cmin < RG_MAP && nc <= 65535;
-    if (use)
-        for (int64_t q = threadIdx.x; q < nc; q += blockDim.x) map[colidxC[cs + q] - cmin] = (uint16_t)q;
-    return use;
-}
-
-__global__ void __launch_bounds__(RG_THREADS)
-k_rg_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int32_t m,
-              const int64_t *__restrict__ rpA, const int32_t *__restrict__ colA, const int64_t *__restrict__ rpB,
-              const int32_t *__restrict__ colB, const int64_t *__restrict__ tile_ptrC,
-              const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC, uint16_t *__restrict__ maskC,
-              uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC) {
-    __shared__ uint16_t map[RG_MAP];
-    const int32_t bi = rows[row_begin + blockIdx.x];
-    int64_t cs, ce;
-    c_row_range(tile_ptrC, row_nc, bi, cs, ce);
-    if (ce <= cs) return;  // whole block leaves: no C tiles, no products
-    uint4 *mw = reinterpret_cast<uint4 *>(maskC);
-    for (int64_t q = threadIdx.x; q < (ce - cs) * 2; q += RG_THREADS) mw[cs * 2 + q] = make_uint4(0, 0, 0, 0);
-    int32_t cmin = 0, cmax = 0;
-    const bool use_map = rg_build_map(map, colidxC, cs, ce, cmin, cmax);
-    const bool verify = row_nc != nullptr;
-    __threadfence();
     __syncthreads();
-    // products: 16 threads per scalar row, each a strided share of the row's A entries
-    const int r = threadIdx.x >> 4, q = threadIdx.x & 15;
-    const int32_t i = bi * 16 + r;
-    uint32_t *mw32 = reinterpret_cast<uint32_t *>(maskC);
-    if (i < m) {
-        for (int64_t e = rpA[i] + q; e < rpA[i + 1]; e += 16) {
-            const int32_t k = colA[e];
-            for (int64_t f = rpB[k]; f < rpB[k + 1]; f++) {
-                const int32_t j = colB[f];
-                const int64_t t = rg_tile(j >> 4, use_map, map, cmin, cmax, colidxC, cs, ce, verify);
-                if (t >= 0) atomicOr(&mw32[t * 8 + (r >> 1)], (0x8000u >> (j & 15)) << ((r & 1) * 16));
+    if (threadIdx.x <= 16) {
+        const int64_t ri = (int64_t)s_it.bi * 16 + (int64_t)threadIdx.x;
+        s_rp[threadIdx.x] = rpA[ri < m ? ri : (int64_t)m];
+    }
+    if ((int)threadIdx.x <= s_it.nw)
+        s_pb[threadIdx.x] = lower_bound<int64_t, int32_t>(colidxC, s_it.cs, s_it.ce,
+                                                          s_it.c0 + (int32_t)threadIdx.x * RGW_COLS);
+    __syncthreads();
+}
+
+// First entry of B row segment [b0, b1) at or after scalar column col (b1 when none).
+__device__ __forceinline__ int64_t rgw_seek(const int32_t *__restrict__ colB, int64_t b0, int64_t b1, int32_t col) {
+    if (b0 >= b1 || colB[b0] >= col) return b0;
+    if (colB[b1 - 1] < col) return b1;
+    return lower_bound<int64_t, int32_t>(colB, b0, b1, col);
+}
+
+struct RgwSeg {
+    int64_t lo, hi;
+    int32_t r;
+};
+
+__global__ void __launch_bounds__(RGW_THREADS)
+k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ woff,
+               unsigned long long *__restrict__ ctr, int32_t m, const int64_t *__restrict__ rpA,
+               const int32_t *__restrict__ colA, const int64_t *__restrict__ rpB, const int32_t *__restrict__ colB,
+               const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ row_nc,
+               const int32_t *__restrict__ colidxC, uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC,
+               int64_t *__restrict__ tile_nnzC) {
+    __shared__ uint32_t s_m[RGW_COLS * 8];  // two 16-bit row masks per word
+    __shared__ RgwSeg s_q[RGW_THREADS];     // long segments of one pass
+    __shared__ int64_t s_rp[17];
+    __shared__ int64_t s_pb[RGW_GMAX + 1];
+    __shared__ RgwItem s_it;
+    __shared__ int64_t s_item;
+    __shared__ int s_nq;
+    const int64_t total = woff[nr];
+    const int tid = threadIdx.x;
+    for (;;) {
+        if (tid == 0) {
+            s_item = (int64_t)atomicAdd(&ctr[0], 1ull);
+            s_nq = 0;
+        }
+        __syncthreads();
+        const int64_t item = s_item;
+        if (item >= total) break;  // uniform: every thread read the same item
+        rgw_setup(item, rows, row_begin, nr, woff, m, rpA, tile_ptrC, row_nc, colidxC, s_it, s_rp, s_pb);
+        const int64_t e0 = s_rp[0], e1 = s_rp[16];
+        for (int w = 0; w < s_it.nw; w++) {
+            const int32_t c0 = s_it.c0 + w * RGW_COLS;
+            const int32_t lo_col = c0 * 16;
+            const int64_t hi_col = (int64_t)(c0 + RGW_COLS) * 16;
+            const int64_t p0 = s_pb[w], np = s_pb[w + 1] - p0;
+            if (np == 0) continue;  // no C tile in this window (uniform)
+            for (int i = tid; i < RGW_COLS * 8; i += RGW_THREADS) s_m[i] = 0u;
+            __syncthreads();
+            for (int64_t base = e0; base < e1; base += RGW_THREADS) {  // uniform trip count
+                const int64_t e = base + tid;
+                if (e < e1) {
+                    int r = 0;
+#pragma unroll
+                    for (int st = 8; st > 0; st >>= 1)
+                        if (r + st <= 15 && s_rp[r + st] <= e) r += st;
+                    const int32_t k = colA[e];
+                    const int64_t b1 = rpB[k + 1];
+                    const int64_t lo = rgw_seek(colB, rpB[k], b1, lo_col);
+                    if (lo < b1) {
+                        if (b1 - lo > RGW_LONG) {
+                            const int qi = atomicAdd(&s_nq, 1);
+                            s_q[qi] = RgwSeg{lo, b1, r};
+                        } else {
+                            for (int64_t f = lo; f < b1; f++) {
+                                const int32_t j = colB[f];
+                                if (j >= hi_col) break;
+                                const int tc = (j >> 4) - c0;
+                                if ((unsigned)tc < (unsigned)RGW_COLS)
+                                    atomicOr(&s_m[tc * 8 + (r >> 1)], (0x8000u >> (j & 15)) << ((r & 1) * 16));
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                const int nq = s_nq;
+                for (int qi = 0; qi < nq; qi++) {  // long segments: the whole block walks each
+                    const RgwSeg sg = s_q[qi];
+                    for (int64_t f = sg.lo + tid; f < sg.hi; f += RGW_THREADS) {
+                        const int32_t j = colB[f];
+                        if (j >= hi_col) break;
+                        const int tc = (j >> 4) - c0;
+                        if ((unsigned)tc < (unsigned)RGW_COLS)
+                            atomicOr(&s_m[tc * 8 + (sg.r >> 1)], (0x8000u >> (j & 15)) << ((sg.r & 1) * 16));
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) s_nq = 0;
+                __syncthreads();
             }
+            // the window's C tiles: masks, row offsets, value counts (16 threads per tile)
+            for (int64_t it = tid; it < np * 16; it += RGW_THREADS) {
+                const int64_t t = p0 + (it >> 4);
+                const int rr = (int)(it & 15);
+                const int tc = colidxC[t] - c0;
+                const uint32_t rm = (s_m[tc * 8 + (rr >> 1)] >> ((rr & 1) * 16)) & 0xFFFFu;
+                maskC[t * 16 + rr] = (uint16_t)rm;
+                const int pc = __popc(rm);
+                const int inc = group16_incl_sum(pc);
+                ptrRowC[t * 16 + rr] = (uint8_t)(inc - pc);
+                if (rr == 15) tile_nnzC[t + 1] = inc;
+            }
+            __syncthreads();
         }
-    }
-    __threadfence();
-    __syncthreads();
-    // row offsets and value counts of every tile of the row (16 threads per tile)
-    for (int64_t it = threadIdx.x; it < (ce - cs) * 16; it += RG_THREADS) {
-        const int64_t t = cs + (it >> 4);
-        const int rr = (int)(it & 15);
-        const uint32_t w = __hip_atomic_load(&mw32[t * 8 + (rr >> 1)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t rm = (w >> ((rr & 1) * 16)) & 0xFFFFu;
-        const int pc = __popc(rm);
-        const int inc = group16_incl_sum(pc);
-        ptrRowC[t * 16 + rr] = (uint8_t)(inc - pc);
-        if (rr == 15) tile_nnzC[t + 1] = inc;
     }
 }
 
 template <typename V>
-__global__ void __launch_bounds__(RG_THREADS)
-k_rg_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int32_t m, const int64_t *__restrict__ rpA,
-             const int32_t *__restrict__ colA, const V *__restrict__ valA, const int64_t *__restrict__ rpB,
-             const int32_t *__restrict__ colB, const V *__restrict__ valB, const int64_t *__restrict__ tile_ptrC,
-             const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC,
-             const uint16_t *__restrict__ maskC, const uint8_t *__restrict__ ptrRowC,
-             const int64_t *__restrict__ tile_nnzC, V *__restrict__ valC, uint8_t *__restrict__ colC,
-             int32_t *__restrict__ overflow, int64_t val_cap) {
-    __shared__ uint16_t map[RG_MAP];
-    const int32_t bi = rows[row_begin + blockIdx.x];
-    int64_t cs, ce;
-    c_row_range(tile_ptrC, row_nc, bi, cs, ce);
-    if (ce <= cs) return;
-    if (tile_nnzC[ce] > val_cap) {  // slot layout: C's values outgrew their capacity
-        if (threadIdx.x == 0) atomicOr(overflow, 1);
-        return;
-    }
-    // zero the row's values and write their col bytes (16 threads per tile)
-    for (int64_t it = threadIdx.x; it < (ce - cs) * 16; it += RG_THREADS) {
-        const int64_t t = cs + (it >> 4);
-        const int rr = (int)(it & 15);
-        uint32_t rm = maskC[t * 16 + rr];
-        int64_t o = tile_nnzC[t] + ptrRowC[t * 16 + rr];
-        while (rm) {
-            const int c = __clz(rm) - 16;
-            rm ^= 0x8000u >> c;
-            valC[o] = V(0);
-            colC[o] = (uint8_t)((rr << 4) | c);
-            o++;
+struct RgwNumLDS {
+    V val[RGW_VBYTES / sizeof(V)];
+    uint32_t tw[RGW_TILES * 16];  // (row offset << 16) | row mask
+    int32_t tb[RGW_TILES];        // value offset of each tile in the sub-window
+    uint16_t map[RGW_COLS];       // tile column - c0 -> sub-window tile (0xFFFF: none)
+};
+
+template <typename V>
+__global__ void __launch_bounds__(RGW_THREADS)
+k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ woff,
+              unsigned long long *__restrict__ ctr, int32_t m, const int64_t *__restrict__ rpA,
+              const int32_t *__restrict__ colA, const V *__restrict__ valA, const int64_t *__restrict__ rpB,
+              const int32_t *__restrict__ colB, const V *__restrict__ valB, const int64_t *__restrict__ tile_ptrC,
+              const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC,
+              const uint16_t *__restrict__ maskC, const uint8_t *__restrict__ ptrRowC,
+              const int64_t *__restrict__ tile_nnzC, V *__restrict__ valC, uint8_t *__restrict__ colC,
+              int32_t *__restrict__ overflow, int64_t val_cap) {
+    constexpr int VCAP = RGW_VBYTES / (int)sizeof(V);
+    __shared__ RgwNumLDS<V> s;
+    __shared__ int64_t s_rp[17];
+    __shared__ int64_t s_pb[RGW_GMAX + 1];
+    __shared__ RgwItem s_it;
+    __shared__ int64_t s_item, s_sp1;
+    const int64_t total = woff[nr];
+    const int tid = threadIdx.x, r = tid >> 4, l = tid & 15;
+    const int gshift = (tid & (WAVE - 1)) & ~15;  // this group's first lane in the wave
+    for (;;) {
+        if (tid == 0) s_item = (int64_t)atomicAdd(&ctr[1], 1ull);
+        __syncthreads();
+        const int64_t item = s_item;
+        if (item >= total) break;
+        rgw_setup(item, rows, row_begin, nr, woff, m, rpA, tile_ptrC, row_nc, colidxC, s_it, s_rp, s_pb);
+        if (tile_nnzC[s_it.ce] > val_cap) {  // slot layout: C's values outgrew their capacity (uniform)
+            if (tid == 0) atomicOr(overflow, 1);
+            __syncthreads();
+            continue;
         }
-    }
-    int32_t cmin = 0, cmax = 0;
-    const bool use_map = rg_build_map(map, colidxC, cs, ce, cmin, cmax);
-    const bool verify = row_nc != nullptr;
-    __threadfence();
-    __syncthreads();
-    // thread (r, s): scalar row r, C tiles [t_lo, t_hi) of the row; products in ascending k
-    const int r = threadIdx.x >> 4, sl = threadIdx.x & 15;
-    const int32_t i = bi * 16 + r;
-    const int64_t nc = ce - cs;
-    const int64_t t_lo = cs + nc * sl / 16, t_hi = cs + nc * (sl + 1) / 16;
-    if (i < m && t_lo < t_hi) {
-        const int32_t col_lo = colidxC[t_lo] * 16, col_hi = colidxC[t_hi - 1] * 16 + 15;
-        {
-            for (int64_t e = rpA[i]; e < rpA[i + 1]; e++) {
-                const int32_t k = colA[e];
-                const V a = valA[e];
-                int64_t lo = rpB[k], hi = rpB[k + 1];
-                const int64_t fe = hi;
-                while (lo < hi) {  // first entry of B row k at or after col_lo
-                    const int64_t mid = (lo + hi) >> 1;
-                    if (colB[mid] < col_lo) lo = mid + 1; else hi = mid;
+        const int32_t i = s_it.bi * 16 + r;
+        const int64_t ea = s_rp[r], eb = s_rp[r + 1];
+        for (int w = 0; w < s_it.nw; w++) {
+            const int32_t c0 = s_it.c0 + w * RGW_COLS;
+            const int64_t pend = s_pb[w + 1];
+            for (int64_t sp0 = s_pb[w]; sp0 < pend;) {  // uniform sub-window loop
+                if (tid == 0) {  // largest end with at most RGW_TILES tiles and VCAP values
+                    int64_t lo = sp0 + 1, hi = min<int64_t>(pend, sp0 + RGW_TILES);
+                    const int64_t v0 = tile_nnzC[sp0];
+                    while (lo < hi) {
+                        const int64_t mid = (lo + hi + 1) >> 1;
+                        if (tile_nnzC[mid] - v0 <= VCAP) lo = mid; else hi = mid - 1;
+                    }
+                    s_sp1 = lo;
                 }
-                for (int64_t f = lo; f < fe; f++) {
-                    const int32_t j = colB[f];
-                    if (j > col_hi) break;
-                    const int c = j & 15;
-                    const int64_t t = rg_tile(j >> 4, use_map, map, cmin, cmax, colidxC, cs, ce, verify);
-                    if (t < 0) continue;
-                    const int64_t o = tile_nnzC[t] + ptrRowC[t * 16 + r] + __popc(maskC[t * 16 + r] & rg_below(c));
-                    valC[o] = rg_fma<V>(a, valB[f], valC[o]);
+                for (int q = tid; q < RGW_COLS; q += RGW_THREADS) s.map[q] = 0xFFFF;
+                __syncthreads();
+                const int64_t sp1 = s_sp1;
+                const int nt = (int)(sp1 - sp0);
+                const int64_t vbase = tile_nnzC[sp0];
+                const int nv = (int)(tile_nnzC[sp1] - vbase);
+                for (int q = tid; q < nt; q += RGW_THREADS) {
+                    s.map[colidxC[sp0 + q] - c0] = (uint16_t)q;
+                    s.tb[q] = (int32_t)(tile_nnzC[sp0 + q] - vbase);
                 }
+                for (int q = tid; q < nt * 16; q += RGW_THREADS)
+                    s.tw[q] = ((uint32_t)ptrRowC[sp0 * 16 + q] << 16) | maskC[sp0 * 16 + q];
+                for (int q = tid; q < nv; q += RGW_THREADS) s.val[q] = V(0);
+                __syncthreads();
+                const int32_t lo_col = colidxC[sp0] * 16;
+                const int64_t hi_col = (int64_t)colidxC[sp1 - 1] * 16 + 16;
+                if (i < m) {
+                    for (int64_t e0 = ea; e0 < eb; e0 += 16) {  // 16 A entries of row r at a time
+                        const int64_t e = e0 + l;
+                        int64_t lo = 0, hi = 0;
+                        V a = V(0);
+                        if (e < eb) {
+                            const int32_t k = colA[e];
+                            a = valA[e];
+                            hi = rpB[k + 1];
+                            lo = rgw_seek(colB, rpB[k], hi, lo_col);
+                        }
+                        uint32_t mk = (uint32_t)(__ballot(lo < hi) >> gshift) & 0xFFFFu;
+                        while (mk) {  // entries in ascending k; one entry's outputs are distinct
+                            const int sgi = __ffs(mk) - 1;
+                            mk &= mk - 1;
+                            const int64_t slo = __shfl(lo, sgi, 16), shi = __shfl(hi, sgi, 16);
+                            const V sa = __shfl(a, sgi, 16);
+                            for (int64_t f = slo + l; f < shi; f += 16) {
+                                const int32_t j = colB[f];
+                                if (j >= hi_col) break;
+                                const int tc = (j >> 4) - c0;
+                                if ((unsigned)tc >= (unsigned)RGW_COLS) continue;
+                                const int lt = s.map[tc];
+                                if (lt >= nt) continue;
+                                const uint32_t tw = s.tw[lt * 16 + r];
+                                const int c = j & 15;
+                                const uint32_t mrow = tw & 0xFFFFu;
+                                if (!(mrow & (0x8000u >> c))) continue;
+                                const int off = s.tb[lt] + (int)(tw >> 16) + __popc(mrow & rgw_below(c));
+                                if (off < nv) s.val[off] = rgw_fma<V>(sa, valB[f], s.val[off]);
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+                // values and col bytes of the sub-window (16 threads per tile)
+                for (int q = tid; q < nt * 16; q += RGW_THREADS) {
+                    const int lt = q >> 4, rr = q & 15;
+                    const uint32_t tw = s.tw[q];
+                    uint32_t mrow = tw & 0xFFFFu;
+                    int o = s.tb[lt] + (int)(tw >> 16);
+                    while (mrow) {
+                        const int c = __clz(mrow) - 16;
+                        mrow ^= 0x8000u >> c;
+                        if (o < nv) {
+                            valC[vbase + o] = s.val[o];
+                            colC[vbase + o] = (uint8_t)((rr << 4) | c);
+                        }
+                        o++;
+                    }
+                }
+                __syncthreads();
+                sp0 = sp1;
             }
         }
     }
+}
+
+static unsigned rgw_grid(Context *ctx, int per_cu) {
+    return (unsigned)std::max(1, ctx->prop.multiProcessorCount * per_cu);
 }
 
 int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                       const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
-    for (int64_t c0 = 0; c0 < count; c0 += ROW_CHUNK) {  // work-items per launch stay below 2^32
-        const unsigned n = (unsigned)std::min<int64_t>(ROW_CHUNK, count - c0);
-        k_rg_symbolic<<<n, RG_THREADS, 0, ctx->stream>>>(
-            rows, begin + c0, A->m, A->row_ptr, A->row_col, B->row_ptr, B->row_col, C->tile_ptr, sl.row_nc,
-            C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz);
-        TSG_HIP(hipGetLastError());
-    }
+                       const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const SlotLayout &sl) {
+    if (count <= 0) return TSG_OK;
+    unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
+    k_rgw_count<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, C->tile_ptr,
+                                                                         sl.row_nc, C->tile_columnidx, woff);
+    TSG_HIP(hipGetLastError());
+    TSG_CHECK(scan_inclusive_i64(ctx, woff, count + 1));
+    k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);
+    k_rgw_symbolic<<<rgw_grid(ctx, 4), RGW_THREADS, 0, ctx->stream>>>(
+        rows, begin, count, woff, ctr, A->m, A->row_ptr, A->row_col, B->row_ptr, B->row_col, C->tile_ptr,
+        sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz);
+    TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
 
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                      const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
-    for (int64_t c0 = 0; c0 < count; c0 += ROW_CHUNK) {  // work-items per launch stay below 2^32
-    const unsigned n = (unsigned)std::min<int64_t>(ROW_CHUNK, count - c0);
+                      const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const SlotLayout &sl) {
+    if (count <= 0) return TSG_OK;
+    unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
+    k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);
     if (A->dtype == TSG_F32)
-        k_rg_numeric<float><<<n, RG_THREADS, 0, ctx->stream>>>(
-            rows, begin + c0, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr, B->row_col,
-            (const float *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr,
-            C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
+        k_rgw_numeric<float><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
+            rows, begin, count, woff, ctr, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr,
+            B->row_col, (const float *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
+            C->tile_csr_Ptr, C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
     else
-        k_rg_numeric<double><<<n, RG_THREADS, 0, ctx->stream>>>(
-            rows, begin + c0, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr, B->row_col,
-            (const double *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr,
-            C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
+        k_rgw_numeric<double><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
+            rows, begin, count, woff, ctr, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr,
+            B->row_col, (const double *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
+            C->tile_csr_Ptr, C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
     TSG_HIP(hipGetLastError());
-    }
     return TSG_OK;
 }
 

@@ -459,7 +459,30 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_plan_guard launch");
         }
     }
-    if (!rc && slots > INT32_MAX) rc = fail(TSG_ERR_OVERFLOW, "predicted C tiles exceed int32 tile ids");
+    // The slot layout and the compact C both hold `slots` tiles (120 bytes a
+    // tile) next to the values: when that exceeds int32 tile ids or the free
+    // device memory (with the pool's cached blocks), the exact pipeline runs
+    // instead (same result).
+    if (!rc) {
+        size_t freeb = 0, totalb = 0;
+        const double need = (double)slots * 120.0 + (double)vcap * (double)(vsize(A->dtype) + 1);
+        if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) rc = fail(TSG_ERR_HIP, "hipMemGetInfo failed");
+        if (!rc && (slots > INT32_MAX || need > 0.9 * ((double)freeb + (double)ctx->pool.cached))) {
+            cleanup();
+            tsg_options o = opts ? *opts : tsg_options{};
+            o.scheduler = TSG_SCHED_PLAIN;
+            TSG_CHECK(spgemm_binned(ctx, A, B, C, &o, st));
+            if (st) {
+                st->fallback_used = 2;
+                st->host_syncs += syncs;
+                st->pred_numtileC = slots;
+                st->pred_nnzC = vcap;
+            }
+            if (timing)
+                for (auto &e : ev) (void)hipEventDestroy(e);
+            return TSG_OK;
+        }
+    }
     // slot-layout structure and the compact C
     if (!rc) rc = palloc(ctx, &row_nc, (size_t)tilem);
     if (!rc) rc = palloc(ctx, &prow, (size_t)slots);

@@ -33,6 +33,7 @@ struct Plan {
     int32_t *overflow = nullptr;
     int4 *link = nullptr;        // per A tile: B tile-row start, tiles, payload count
     int64_t *lpay = nullptr;     // per A tile: B tile-row payload start
+    int64_t *rgw = nullptr;      // row-by-row bin: work-item offsets + counters (tilem + 3)
     int64_t bin_off[MAX_BINS + 1] = {};
     int nbins = 0;
     int nvar = 0;     // staged bins; bin nvar = C-tile kernels

@@ -100,8 +100,9 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t key) {
 // Exact C tile count of each tile-row (as step1_count_kernel,
 // src/spgemm_raw_kernels.cuh:23-57) plus the sizes that decide its bin:
 // A tiles, A payload, B tiles and B payload reached, C tiles.  Rows reaching
-// at most 64 B tiles count distinct columns with one wave-wide sort; larger
-// rows use an LDS bitmap over their column window.
+// at most 64 B tiles count distinct columns with a 2048-column LDS bitmap when
+// the columns span no more, else with one wave-wide sort; larger rows use an
+// LDS bitmap over successive column windows.
 __global__ void __launch_bounds__(RS_WAVES *WAVE)
 k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
@@ -135,14 +136,23 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
                 const int t = owner_of(s_pre[w], (int)na, lane);
                 j = colidxB[s_bs[w][t] + (lane - s_pre[w][t])];
             }
-            const uint64_t key = wave_sort64(lane < nb ? (uint64_t)(uint32_t)j : ~0ull);
-            const int32_t sj = (int32_t)(uint32_t)key;
-            const int32_t prev = __shfl_up(sj, 1, WAVE);
-            const bool first = lane < nb && (lane == 0 || prev != sj);
-            count = __popcll(__ballot(first));
             if (nb > 0) {
-                cmin = __builtin_amdgcn_readlane(sj, 0);
-                cmax = __builtin_amdgcn_readlane(sj, (int)nb - 1);
+                cmin = -last_lane(dpp_incl_max(lane < nb ? -j : -INT_MAX));
+                cmax = last_lane(dpp_incl_max(lane < nb ? j : -1));
+            }
+            if (nb > 0 && cmax - cmin < WAVE * 32) {
+                // distinct columns = set bits of a bitmap over [cmin, cmin + 2048)
+                bm[lane] = 0u;
+                wave_sync();
+                if (lane < nb) atomicOr(&bm[(j - cmin) >> 5], 1u << ((j - cmin) & 31));
+                wave_sync();
+                count = last_lane(dpp_incl_sum(__popc(bm[lane])));
+            } else {
+                const uint64_t key = wave_sort64(lane < nb ? (uint64_t)(uint32_t)j : ~0ull);
+                const int32_t sj = (int32_t)(uint32_t)key;
+                const int32_t prev = __shfl_up(sj, 1, WAVE);
+                const bool first = lane < nb && (lane == 0 || prev != sj);
+                count = __popcll(__ballot(first));
             }
             done = true;
         }
@@ -311,6 +321,8 @@ struct SymLDS {
     alignas(16) uint16_t bmask[Var::NB * 16];
     int32_t abs_[Var::NA];
     int32_t aseg[Var::NA + 1];
+    uint32_t bm[WAVE];   // column bitmap
+    int32_t wpre[WAVE];  // set bits before each bitmap word
     uint8_t bown[Var::NB];
     uint8_t bct[Var::NB];
 };
@@ -380,15 +392,41 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         s.bown[lane] = (uint8_t)t;
         store_mask16(&s.bmask[lane * 16], m0, m1);
     }
-    // distinct tile columns and their ranks (tile columns are below 2^27)
-    const uint64_t key = wave_sort64(lane < nB ? ((uint64_t)(uint32_t)j << 6) | (uint64_t)lane : ~0ull);
-    const int32_t sj = (int32_t)(key >> 6);
-    const int sb = (int)(key & 63);
-    const bool valid = lane < nB;
-    const int32_t prev = __shfl_up(sj, 1, WAVE);
-    const bool first = valid && (lane == 0 || prev != sj);
-    const uint64_t fb = __ballot(first);
-    const int nc = __popcll(fb);
+    // distinct tile columns and their ranks: set bits of a bitmap over the
+    // columns' range when it spans at most 2048 columns, else one wave-wide
+    // bitonic sort of (column, lane) keys (tile columns are below 2^27)
+    int32_t jmin = 0, jmax = -1;
+    if (nB > 0) {
+        jmin = -last_lane(dpp_incl_max(lane < nB ? -j : -INT_MAX));
+        jmax = last_lane(dpp_incl_max(lane < nB ? j : -1));
+    }
+    const bool bitmap = nB > 0 && jmax - jmin < WAVE * 32;
+    int nc, rank = 0, sb = lane;
+    bool first = false;
+    if (bitmap) {
+        s.bm[lane] = 0u;
+        wave_sync();
+        const int off = j - jmin;
+        if (lane < nB) atomicOr(&s.bm[off >> 5], 1u << (off & 31));
+        wave_sync();
+        const uint32_t word = s.bm[lane];
+        const int pc = __popc(word);
+        const int incl = dpp_incl_sum(pc);
+        nc = last_lane(incl);
+        s.wpre[lane] = incl - pc;
+        wave_sync();
+        if (lane < nB) rank = s.wpre[off >> 5] + __popc(s.bm[off >> 5] & ((1u << (off & 31)) - 1u));
+        first = lane < nB;  // every B tile writes its column at its rank (duplicates agree)
+    } else {
+        const uint64_t key = wave_sort64(lane < nB ? ((uint64_t)(uint32_t)j << 6) | (uint64_t)lane : ~0ull);
+        j = (int32_t)(key >> 6);
+        sb = (int)(key & 63);
+        const int32_t prev = __shfl_up(j, 1, WAVE);
+        first = lane < nB && (lane == 0 || prev != j);
+        const uint64_t fb = __ballot(first);
+        nc = __popcll(fb);
+        rank = mbcnt(fb) + (first ? 1 : 0) - 1;
+    }
     if (nc > cap) {  // slot layout: the prediction was short for this row
         if (lane == 0) {
             atomicOr(overflow, 1);
@@ -397,12 +435,11 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         return;
     }
     if (row_nc && lane == 0) row_nc[bi] = nc;
-    const int rank = mbcnt(fb) + (first ? 1 : 0) - 1;
     if (first) {
-        colidxC[cs + rank] = sj;
+        colidxC[cs + rank] = j;
         rowidxC[cs + rank] = bi;
     }
-    if (valid) s.bct[sb] = (uint8_t)rank;
+    if (lane < nB) s.bct[sb] = (uint8_t)rank;
     wave_sync();
     // masks: item (B tile b, row r) ORs B's rows selected by A's row mask
     for (int it = lane; it < nB * 16; it += WAVE) {
@@ -448,7 +485,7 @@ struct NumLDS {
     alignas(16) uint8_t aptr[Var::NA * 16];
     alignas(16) uint8_t bcptr[Var::NB * 16];    // B column offsets
     uint8_t bown[Var::NB];
-    uint8_t plist[Var::NB];
+    uint16_t plist[Var::NB];
 };
 
 template <typename V>
@@ -572,42 +609,51 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         if (i < pA) s.aval[i] = av[it];
     }
     wave_sync();
-    // pair table: the B tiles of C tile ct in ascending k = set bits of tmask[ct]
+    // pair table: the B tiles of C tile ct in ascending k = set bits of tmask[ct];
+    // an entry holds the B tile and its A tile (b | t << 8)
     const int pinc = dpp_incl_sum(lane < nc ? __popc(s.tmask[lane]) : 0);
     if (lane < nc) s.pstart[lane + 1] = (int16_t)pinc;
     if (lane == 0) s.pstart[0] = 0;
     wave_sync();
-    if (lane < nB) s.plist[s.pstart[ct_b] + __popc(s.tmask[ct_b] & ((1u << bt) - 1u))] = (uint8_t)lane;
+    if (lane < nB) s.plist[s.pstart[ct_b] + __popc(s.tmask[ct_b] & ((1u << bt) - 1u))] = (uint16_t)(lane | (bt << 8));
     wave_sync();
-    // outputs in chunks of (C tile, row) items: lane = item while the chunk's
-    // output descriptors are written, then lane = output.  A chunk is 4 C
-    // tiles, or 2 when 4 would hold more than DESC_CAP outputs.
-    const int64_t gbase = s.coff[0];
+    // outputs in chunks of at most DESC_CAP: the descriptors (C tile, row,
+    // column) of a chunk are written 4 C tiles (64 items, lane = item) at a
+    // time, then lane = output.  A whole tile-row usually fits one chunk.
     for (int ct0 = 0; ct0 < nc;) {
-        const int span = (s.coff[min(ct0 + 4, nc)] - s.coff[ct0] > DESC_CAP) ? 2 : 4;
-        const int item = ct0 * 16 + lane;
-        const uint32_t mc = (lane < span * 16 && item < nc * 16) ? s.cmask[item] : 0u;
-        const int pc = __popc(mc);
-        const int oincl = dpp_incl_sum(pc);
-        const int nout = last_lane(oincl);
-        {
-            int o = oincl - pc;
+        int ct1 = ct0, nout = 0;
+        while (ct1 < nc) {
+            int span = min(4, nc - ct1);
+            int blk = (int)(s.coff[ct1 + span] - s.coff[ct1]);
+            if (nout + blk > DESC_CAP) {
+                if (nout > 0) break;
+                span = 2;  // 2 tiles hold at most 512 = DESC_CAP outputs
+                blk = (int)(s.coff[ct1 + span] - s.coff[ct1]);
+            }
+            const int item = ct1 * 16 + lane;
+            const uint32_t mc = (lane < span * 16) ? s.cmask[item] : 0u;
+            const int pc = __popc(mc);
+            const int oincl = dpp_incl_sum(pc);
+            int o = nout + oincl - pc;
             uint32_t m = mc;
             while (m) {
                 const int c = __clz(m) - 16;
                 m ^= 0x8000u >> c;
-                s.desc[o++] = (uint16_t)((lane << 4) | c);  // (tile in chunk, row) = lane
+                s.desc[o++] = (uint16_t)((item << 4) | c);  // (C tile, row, column)
             }
+            nout += blk;
+            ct1 += span;
         }
         wave_sync();
         const int64_t obase = s.coff[ct0];
         for (int o = lane; o < nout; o += WAVE) {
             const int d = s.desc[o];
-            const int ct = ct0 + (d >> 8), r = (d >> 4) & 15, c = d & 15;
+            const int ct = d >> 8, r = (d >> 4) & 15, c = d & 15;
             V acc = V(0);
             const int pe = s.pstart[ct + 1];
             for (int p = s.pstart[ct]; p < pe; p++) {
-                const int b = s.plist[p], t = s.bown[b];
+                const int e = s.plist[p];
+                const int b = e & 0xFF, t = e >> 8;
                 const uint32_t ma = s.amask[t * 16 + r];
                 const uint32_t cm = s.bcmask[b * 16 + c];
                 uint32_t m = ma & cm;  // k of the products, ascending from the top bit
@@ -625,7 +671,7 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
             colC[obase + o] = (uint8_t)((r << 4) | c);
         }
         wave_sync();
-        ct0 += span;
+        ct0 = ct1;
     }
 }
 
@@ -665,7 +711,7 @@ BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask) {
 }
 
 void free_plan(Context *ctx, Plan &p) {
-    void *ps[] = {p.rows, p.bin, p.counts, p.hdr, p.overflow, p.link, p.lpay};
+    void *ps[] = {p.rows, p.bin, p.counts, p.hdr, p.overflow, p.link, p.lpay, p.rgw};
     for (void *q : ps) pool_free(ctx, q);
     p = Plan{};
 }
@@ -686,6 +732,7 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
     TSG_CHECK(palloc(ctx, &p.overflow, 1));
     TSG_CHECK(palloc(ctx, &p.link, (size_t)A->numtile));
     TSG_CHECK(palloc(ctx, &p.lpay, (size_t)A->numtile));
+    if (p.rg) TSG_CHECK(palloc(ctx, &p.rgw, (size_t)tilem + 3));
     TSG_HIP(hipMemsetAsync(p.overflow, 0, sizeof(int32_t), ctx->stream));
     if (A->numtile > 0) {
         k_ab_link<<<nblk(A->numtile, 256), 256, 0, ctx->stream>>>(A->numtile, A->tile_columnidx, B->tile_ptr,
@@ -780,7 +827,7 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
     if (p.rg) {  // row-by-row kernels: Step 1 fill as the C-tile rows, then masks from the products
         const int64_t nr = p.bin_off[fb + 2] - p.bin_off[fb + 1];
         TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, sl));
-        TSG_CHECK(launch_rg_symbolic(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, sl));
+        TSG_CHECK(launch_rg_symbolic(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, p.rgw, sl));
     }
     return TSG_OK;
 }
@@ -801,7 +848,8 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
     }
     TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, p.bin_off[fb], p.bin_off[fb + 1] - p.bin_off[fb], sl));
     if (p.rg)
-        TSG_CHECK(launch_rg_numeric(ctx, A, B, C, p.rows, p.bin_off[fb + 1], p.bin_off[fb + 2] - p.bin_off[fb + 1], sl));
+        TSG_CHECK(launch_rg_numeric(ctx, A, B, C, p.rows, p.bin_off[fb + 1], p.bin_off[fb + 2] - p.bin_off[fb + 1],
+                                    p.rgw, sl));
     return TSG_OK;
 }
 

@@ -121,11 +121,13 @@ int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_m
                         const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
 void learned_release(Context *ctx);  // learned.hip
 void tuner_release(Context *ctx);    // autotune.hip
-// Row-by-row kernels for sparse heavy tile-rows (heavy.hip).
+// Row-by-row kernels for sparse heavy tile-rows (heavy.hip).  woff holds
+// count + 3 int64: the rows' work-item offsets (written by the symbolic
+// launch, read by the numeric one) and two work counters.
 int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                       const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
+                       const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const SlotLayout &sl);
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                      const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
+                      const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const SlotLayout &sl);
 int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B);  // api.hip
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                    tsg_tile_matrix *C, const tsg_options *opts);
@@ -195,6 +197,18 @@ __device__ __forceinline__ int dpp_incl_sum(int v) {
     v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);
     v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);
     v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);
+    return v;
+}
+
+// Wave64 inclusive maximum with the same DPP pattern (a lane with no source
+// keeps its own value).
+__device__ __forceinline__ int dpp_incl_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
     return v;
 }
 

@@ -133,7 +133,8 @@ typedef struct tsg_stats {
     int64_t nnzC;
     uint64_t products;      /* sum_{(i,k) in A} nnz(B[k,:]) (src/spgemm_raw_v2.cu:95-99) */
     double ms_step1, ms_step2, ms_step3, ms_total;
-    int32_t fallback_used;  /* learned: 1 when the exact path re-ran after an overflow */
+    int32_t fallback_used;  /* learned: 1 when the exact path re-ran after an overflow, 2 when the
+                               predicted layout exceeded int32 tile ids or free device memory */
     int32_t host_syncs;     /* blocking device-to-host reads made inside the call */
     int64_t pred_numtileC, pred_nnzC;  /* learned: allocated capacities */
 } tsg_stats;
@@ -220,8 +221,9 @@ int tsg_spgemm(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
  * B nonzeros reached, span of B tile columns, longest B tile-row reached, summed
  * B tile-row spans, nonempty B tile-rows reached).  With scheduler
  * TSG_SCHED_LEARNED, tsg_spgemm sizes C from the prediction and falls back to the
- * exact pipeline when any row outgrows it (stats->fallback_used = 1): the result
- * is always the exact product. */
+ * exact pipeline when any row outgrows it (stats->fallback_used = 1) or when the
+ * predicted layout does not fit int32 tile ids or the free device memory
+ * (stats->fallback_used = 2): the result is always the exact product. */
 int tsg_learned_num_params(void);
 /* Appends (features, exact targets) of every tile-row of A*B to the context's
  * training set (runs the exact product once). */
