@@ -25,15 +25,21 @@
 //     workgroup.
 //   k_rgw_numeric: the window's values (in sub-windows of at most RGW_TILES
 //     tiles and RGW_VBYTES bytes of values), a map tile column -> sub-window
-//     tile, and each tile's packed (row offset, row mask) words.  Scalar row r
-//     of the tile-row belongs to lanes r*16..r*16+15; they take the row's A
-//     entries in ascending k, one at a time, and spread that entry's B
-//     segment over the 16 lanes.  The outputs of one A entry are distinct and
-//     the entries follow in k order, so each output sums its products in
-//     ascending k with one FMA each -- the order of the tiled method (tile
-//     pairs ascending, entries ascending within a pair) -- and values are
-//     bit-identical to the other kernels.
+//     tile, and each tile's packed (row offset, row mask) words.  The waves
+//     take the tile-row's 16 scalar rows in turn; for its row a wave loads 64
+//     A entries at a time (a lane per entry: the entry's B segment in the
+//     sub-window, its first products' B values and output slots), then adds
+//     the entries' products one entry after the other in ascending k -- the
+//     owning lane its staged products, the whole wave the rest of a long
+//     segment.  The outputs of one A entry are distinct and the entries follow
+//     in k order, so each output sums its products in ascending k with one
+//     FMA each -- the order of the tiled method (tile pairs ascending, entries
+//     ascending within a pair) -- and values are bit-identical to the other
+//     kernels.  Numeric work items are ranges of a tile-row's values (about
+//     one LDS load of values each), so a heavy tile-row spreads over many
+//     workgroups.
 #include <algorithm>
+#include <climits>
 
 #include "rowstaged.h"
 
@@ -46,6 +52,8 @@ constexpr int RGW_GMAX = 64;           // windows per work item (cap)
 constexpr int RGW_TILES = 256;         // tiles per numeric sub-window
 constexpr int RGW_VBYTES = 32768;      // value bytes per numeric sub-window
 constexpr int RGW_LONG = 64;           // B segments longer than this are walked by the workgroup
+constexpr int RGW_ILP = 4;             // A entries per thread per symbolic pass (independent loads)
+constexpr int RGW_PF = 16;             // B entries loaded at once by one lane
 
 __device__ __forceinline__ uint32_t rgw_below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
 
@@ -171,33 +179,58 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
             if (np == 0) continue;  // no C tile in this window (uniform)
             for (int i = tid; i < RGW_COLS * 8; i += RGW_THREADS) s_m[i] = 0u;
             __syncthreads();
-            for (int64_t base = e0; base < e1; base += RGW_THREADS) {  // uniform trip count
-                const int64_t e = base + tid;
-                if (e < e1) {
+            for (int64_t base = e0; base < e1; base += RGW_THREADS * RGW_ILP) {  // uniform trip count
+                // thread t takes entries base + t + 256u (u < RGW_ILP), loads of all u issued together
+                int32_t k[RGW_ILP], jf[RGW_ILP], jl[RGW_ILP];
+                int64_t b0[RGW_ILP], b1[RGW_ILP];
+#pragma unroll
+                for (int u = 0; u < RGW_ILP; u++) {
+                    const int64_t e = base + tid + RGW_THREADS * u;
+                    k[u] = colA[e < e1 ? e : e0];  // clamped: the load stays in A
+                }
+#pragma unroll
+                for (int u = 0; u < RGW_ILP; u++) {
+                    const int64_t e = base + tid + RGW_THREADS * u;
+                    b0[u] = rpB[k[u]];
+                    b1[u] = e < e1 ? rpB[k[u] + 1] : b0[u];  // an entry past the tile-row: empty segment
+                }
+#pragma unroll
+                for (int u = 0; u < RGW_ILP; u++) {
+                    jf[u] = b1[u] > b0[u] ? colB[b0[u]] : INT_MAX;
+                    jl[u] = b1[u] > b0[u] ? colB[b1[u] - 1] : INT_MIN;
+                }
+#pragma unroll
+                for (int u = 0; u < RGW_ILP; u++) {
+                    if (jl[u] < lo_col || jf[u] >= hi_col) continue;  // no entry in the window
+                    const int64_t e = base + tid + RGW_THREADS * u;
                     int r = 0;
 #pragma unroll
                     for (int st = 8; st > 0; st >>= 1)
                         if (r + st <= 15 && s_rp[r + st] <= e) r += st;
-                    const int32_t k = colA[e];
-                    const int64_t b1 = rpB[k + 1];
-                    const int64_t lo = rgw_seek(colB, rpB[k], b1, lo_col);
-                    if (lo < b1) {
-                        if (b1 - lo > RGW_LONG) {
-                            const int qi = atomicAdd(&s_nq, 1);
-                            s_q[qi] = RgwSeg{lo, b1, r};
-                        } else {
-                            for (int64_t f = lo; f < b1; f++) {
-                                const int32_t j = colB[f];
-                                if (j >= hi_col) break;
-                                const int tc = (j >> 4) - c0;
-                                if ((unsigned)tc < (unsigned)RGW_COLS)
-                                    atomicOr(&s_m[tc * 8 + (r >> 1)], (0x8000u >> (j & 15)) << ((r & 1) * 16));
+                    const int64_t lo = jf[u] >= lo_col ? b0[u] : lower_bound<int64_t, int32_t>(colB, b0[u], b1[u], lo_col);
+                    int qi = RGW_THREADS;
+                    if (b1[u] - lo > RGW_LONG) qi = atomicAdd(&s_nq, 1);
+                    if (qi < RGW_THREADS) {
+                        s_q[qi] = RgwSeg{lo, b1[u], r};
+                    } else {
+                        for (int64_t f0 = lo; f0 < b1[u]; f0 += RGW_PF) {  // RGW_PF loads issued together
+                            int32_t jj[RGW_PF];
+#pragma unroll
+                            for (int t = 0; t < RGW_PF; t++) jj[t] = colB[min<int64_t>(f0 + t, b1[u] - 1)];
+                            bool past = false;
+#pragma unroll
+                            for (int t = 0; t < RGW_PF; t++) {
+                                const int tc = (jj[t] >> 4) - c0;
+                                if (f0 + t < b1[u] && jj[t] < hi_col && (unsigned)tc < (unsigned)RGW_COLS)
+                                    atomicOr(&s_m[tc * 8 + (r >> 1)], (0x8000u >> (jj[t] & 15)) << ((r & 1) * 16));
+                                past = past || jj[t] >= hi_col;
                             }
+                            if (past) break;
                         }
                     }
                 }
                 __syncthreads();
-                const int nq = s_nq;
+                const int nq = min(s_nq, RGW_THREADS);
                 for (int qi = 0; qi < nq; qi++) {  // long segments: the whole block walks each
                     const RgwSeg sg = s_q[qi];
                     for (int64_t f = sg.lo + tid; f < sg.hi; f += RGW_THREADS) {
@@ -237,9 +270,49 @@ struct RgwNumLDS {
     uint16_t map[RGW_COLS];       // tile column - c0 -> sub-window tile (0xFFFF: none)
 };
 
+// Values of one numeric work item (at most): an item holds the tiles of one
+// tile-row whose value offsets start in one block of RGW_VBYTES / sizeof(V) -
+// 256 values, so with the last tile's (at most 256) values it fits LDS.
+template <typename V>
+constexpr int rgw_item_values() { return RGW_VBYTES / (int)sizeof(V) - 256; }
+
+// Numeric work items of each listed row: woffn[q + 1] = ceil(row values / item values).
+__global__ void __launch_bounds__(256)
+k_rgw_count_num(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr,
+                const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ row_nc,
+                const int64_t *__restrict__ tile_nnzC, int64_t item_values, int64_t *__restrict__ woffn) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q == 0) woffn[0] = 0;
+    if (q >= nr) return;
+    int64_t cs, ce;
+    c_row_range(tile_ptrC, row_nc, rows[row_begin + q], cs, ce);
+    const int64_t nv = ce > cs ? tile_nnzC[ce] - tile_nnzC[cs] : 0;
+    woffn[q + 1] = ce > cs ? max<int64_t>(1, (nv + item_values - 1) / item_values) : 0;
+}
+
+struct RgwNumItem {
+    int32_t bi;
+    int64_t ce, ta, tb;  // row's end tile; the item's tiles [ta, tb)
+};
+
+// Output slot of product column j for scalar row r in the staged sub-window, or -1.
+template <typename V>
+__device__ __forceinline__ int rgw_slot(const RgwNumLDS<V> &s, int32_t j, int32_t c0, int nt, int nv, int r) {
+    const int tc = (j >> 4) - c0;
+    if ((unsigned)tc >= (unsigned)RGW_COLS) return -1;
+    const int lt = s.map[tc];
+    if (lt >= nt) return -1;
+    const uint32_t tw = s.tw[lt * 16 + r];
+    const int c = j & 15;
+    const uint32_t mrow = tw & 0xFFFFu;
+    if (!(mrow & (0x8000u >> c))) return -1;
+    const int off = s.tb[lt] + (int)(tw >> 16) + __popc(mrow & rgw_below(c));
+    return off < nv ? off : -1;
+}
+
 template <typename V>
 __global__ void __launch_bounds__(RGW_THREADS)
-k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ woff,
+k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ woffn,
               unsigned long long *__restrict__ ctr, int32_t m, const int64_t *__restrict__ rpA,
               const int32_t *__restrict__ colA, const V *__restrict__ valA, const int64_t *__restrict__ rpB,
               const int32_t *__restrict__ colB, const V *__restrict__ valB, const int64_t *__restrict__ tile_ptrC,
@@ -247,111 +320,183 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
               const uint16_t *__restrict__ maskC, const uint8_t *__restrict__ ptrRowC,
               const int64_t *__restrict__ tile_nnzC, V *__restrict__ valC, uint8_t *__restrict__ colC,
               int32_t *__restrict__ overflow, int64_t val_cap) {
-    constexpr int VCAP = RGW_VBYTES / (int)sizeof(V);
+    constexpr int64_t H = rgw_item_values<V>();
     __shared__ RgwNumLDS<V> s;
     __shared__ int64_t s_rp[17];
-    __shared__ int64_t s_pb[RGW_GMAX + 1];
-    __shared__ RgwItem s_it;
+    __shared__ RgwNumItem s_it;
     __shared__ int64_t s_item, s_sp1;
-    const int64_t total = woff[nr];
-    const int tid = threadIdx.x, r = tid >> 4, l = tid & 15;
-    const int gshift = (tid & (WAVE - 1)) & ~15;  // this group's first lane in the wave
+    __shared__ int s_row;
+    const int64_t total = woffn[nr];
+    const int tid = threadIdx.x, l = lane_id();
     for (;;) {
         if (tid == 0) s_item = (int64_t)atomicAdd(&ctr[1], 1ull);
         __syncthreads();
         const int64_t item = s_item;
-        if (item >= total) break;
-        rgw_setup(item, rows, row_begin, nr, woff, m, rpA, tile_ptrC, row_nc, colidxC, s_it, s_rp, s_pb);
+        if (item >= total) break;  // uniform
+        if (tid == 0) {
+            int64_t lo = 0, hi = nr - 1;  // largest q with woffn[q] <= item
+            while (lo < hi) {
+                const int64_t mid = (lo + hi + 1) >> 1;
+                if (woffn[mid] <= item) lo = mid; else hi = mid - 1;
+            }
+            const int32_t bi = rows[row_begin + lo];
+            int64_t cs, ce;
+            c_row_range(tile_ptrC, row_nc, bi, cs, ce);
+            const int64_t j = item - woffn[lo], vb = tile_nnzC[cs];
+            s_it.bi = bi;
+            s_it.ce = ce;
+            s_it.ta = lower_bound<int64_t, int64_t>(tile_nnzC, cs, ce, vb + j * H);
+            s_it.tb = (item + 1 == woffn[lo + 1]) ? ce
+                                                   : lower_bound<int64_t, int64_t>(tile_nnzC, s_it.ta, ce, vb + (j + 1) * H);
+        }
+        __syncthreads();
+        if (tid <= 16) {
+            const int64_t ri = (int64_t)s_it.bi * 16 + tid;
+            s_rp[tid] = rpA[ri < m ? ri : (int64_t)m];
+        }
+        __syncthreads();
         if (tile_nnzC[s_it.ce] > val_cap) {  // slot layout: C's values outgrew their capacity (uniform)
             if (tid == 0) atomicOr(overflow, 1);
             __syncthreads();
             continue;
         }
-        const int32_t i = s_it.bi * 16 + r;
-        const int64_t ea = s_rp[r], eb = s_rp[r + 1];
-        for (int w = 0; w < s_it.nw; w++) {
-            const int32_t c0 = s_it.c0 + w * RGW_COLS;
-            const int64_t pend = s_pb[w + 1];
-            for (int64_t sp0 = s_pb[w]; sp0 < pend;) {  // uniform sub-window loop
-                if (tid == 0) {  // largest end with at most RGW_TILES tiles and VCAP values
-                    int64_t lo = sp0 + 1, hi = min<int64_t>(pend, sp0 + RGW_TILES);
-                    const int64_t v0 = tile_nnzC[sp0];
-                    while (lo < hi) {
-                        const int64_t mid = (lo + hi + 1) >> 1;
-                        if (tile_nnzC[mid] - v0 <= VCAP) lo = mid; else hi = mid - 1;
+        const int64_t tb = s_it.tb;
+        for (int64_t sp0 = s_it.ta; sp0 < tb;) {  // uniform sub-window loop
+            if (tid == 0) {  // at most RGW_TILES tiles within RGW_COLS tile columns
+                const int64_t hi = min<int64_t>(tb, sp0 + RGW_TILES);
+                s_sp1 = lower_bound<int64_t, int32_t>(colidxC, sp0 + 1, hi, colidxC[sp0] + RGW_COLS);
+                s_row = 0;
+            }
+            for (int q = tid; q < RGW_COLS; q += RGW_THREADS) s.map[q] = 0xFFFF;
+            __syncthreads();
+            const int64_t sp1 = s_sp1;
+            const int nt = (int)(sp1 - sp0);
+            const int64_t vbase = tile_nnzC[sp0];
+            const int nv = (int)min<int64_t>(tile_nnzC[sp1] - vbase, RGW_VBYTES / (int)sizeof(V));
+            const int32_t c0 = colidxC[sp0];
+            for (int q = tid; q < nt; q += RGW_THREADS) {
+                s.map[colidxC[sp0 + q] - c0] = (uint16_t)q;
+                s.tb[q] = (int32_t)(tile_nnzC[sp0 + q] - vbase);
+            }
+            for (int q = tid; q < nt * 16; q += RGW_THREADS)
+                s.tw[q] = ((uint32_t)ptrRowC[sp0 * 16 + q] << 16) | maskC[sp0 * 16 + q];
+            for (int q = tid; q < nv; q += RGW_THREADS) s.val[q] = V(0);
+            __syncthreads();
+            const int32_t lo_col = c0 * 16;
+            const int64_t hi_col = (int64_t)colidxC[sp1 - 1] * 16 + 16;
+            // one wave per scalar row, rows taken in turn from s_row
+            for (;;) {
+                int r = 0;
+                if (l == 0) r = atomicAdd(&s_row, 1);
+                r = __shfl(r, 0, WAVE);
+                if (r >= 16) break;  // wave-uniform
+                const int32_t i = s_it.bi * 16 + r;
+                if (i >= m) continue;
+                const int64_t ea = s_rp[r], eb = s_rp[r + 1];
+                for (int64_t e0 = ea; e0 < eb; e0 += WAVE) {  // 64 A entries of row r at a time
+                    // lane l owns entry e0 + l: its B segment in the sub-window,
+                    // the first RGW_PF products' B values and output slots (each
+                    // stage's loads issued together, all lanes at once)
+                    const int64_t e = e0 + l;
+                    int64_t lo = 0, hi = 0;
+                    V a = V(0);
+                    if (e < eb) {
+                        const int32_t k = colA[e];
+                        a = valA[e];
+                        hi = rpB[k + 1];
+                        lo = rgw_seek(colB, rpB[k], hi, lo_col);
                     }
-                    s_sp1 = lo;
-                }
-                for (int q = tid; q < RGW_COLS; q += RGW_THREADS) s.map[q] = 0xFFFF;
-                __syncthreads();
-                const int64_t sp1 = s_sp1;
-                const int nt = (int)(sp1 - sp0);
-                const int64_t vbase = tile_nnzC[sp0];
-                const int nv = (int)(tile_nnzC[sp1] - vbase);
-                for (int q = tid; q < nt; q += RGW_THREADS) {
-                    s.map[colidxC[sp0 + q] - c0] = (uint16_t)q;
-                    s.tb[q] = (int32_t)(tile_nnzC[sp0 + q] - vbase);
-                }
-                for (int q = tid; q < nt * 16; q += RGW_THREADS)
-                    s.tw[q] = ((uint32_t)ptrRowC[sp0 * 16 + q] << 16) | maskC[sp0 * 16 + q];
-                for (int q = tid; q < nv; q += RGW_THREADS) s.val[q] = V(0);
-                __syncthreads();
-                const int32_t lo_col = colidxC[sp0] * 16;
-                const int64_t hi_col = (int64_t)colidxC[sp1 - 1] * 16 + 16;
-                if (i < m) {
-                    for (int64_t e0 = ea; e0 < eb; e0 += 16) {  // 16 A entries of row r at a time
-                        const int64_t e = e0 + l;
-                        int64_t lo = 0, hi = 0;
-                        V a = V(0);
-                        if (e < eb) {
-                            const int32_t k = colA[e];
-                            a = valA[e];
-                            hi = rpB[k + 1];
-                            lo = rgw_seek(colB, rpB[k], hi, lo_col);
+                    int sl[RGW_PF];
+                    V bv[RGW_PF];
+                    bool more = false, any = false;
+#pragma unroll
+                    for (int t = 0; t < RGW_PF; t++) sl[t] = -1;
+                    if (lo < hi) {
+                        int32_t jj[RGW_PF], tc[RGW_PF], lt[RGW_PF], tbo[RGW_PF];
+                        uint32_t tw[RGW_PF];
+#pragma unroll
+                        for (int t = 0; t < RGW_PF; t++) {
+                            const int64_t f = min<int64_t>(lo + t, hi - 1);  // clamped: in B row k
+                            jj[t] = colB[f];
+                            bv[t] = valB[f];
                         }
-                        uint32_t mk = (uint32_t)(__ballot(lo < hi) >> gshift) & 0xFFFFu;
-                        while (mk) {  // entries in ascending k; one entry's outputs are distinct
-                            const int sgi = __ffs(mk) - 1;
-                            mk &= mk - 1;
-                            const int64_t slo = __shfl(lo, sgi, 16), shi = __shfl(hi, sgi, 16);
-                            const V sa = __shfl(a, sgi, 16);
-                            for (int64_t f = slo + l; f < shi; f += 16) {
-                                const int32_t j = colB[f];
-                                if (j >= hi_col) break;
-                                const int tc = (j >> 4) - c0;
-                                if ((unsigned)tc >= (unsigned)RGW_COLS) continue;
-                                const int lt = s.map[tc];
-                                if (lt >= nt) continue;
-                                const uint32_t tw = s.tw[lt * 16 + r];
-                                const int c = j & 15;
-                                const uint32_t mrow = tw & 0xFFFFu;
-                                if (!(mrow & (0x8000u >> c))) continue;
-                                const int off = s.tb[lt] + (int)(tw >> 16) + __popc(mrow & rgw_below(c));
-                                if (off < nv) s.val[off] = rgw_fma<V>(sa, valB[f], s.val[off]);
+                        uint32_t ok = 0;
+#pragma unroll
+                        for (int t = 0; t < RGW_PF; t++) {
+                            tc[t] = (jj[t] >> 4) - c0;
+                            const bool v = lo + t < hi && jj[t] < hi_col && (unsigned)tc[t] < (unsigned)RGW_COLS;
+                            ok |= (uint32_t)v << t;
+                            tc[t] = v ? tc[t] : 0;
+                        }
+#pragma unroll
+                        for (int t = 0; t < RGW_PF; t++) lt[t] = s.map[tc[t]];
+#pragma unroll
+                        for (int t = 0; t < RGW_PF; t++) {
+                            if (lt[t] >= nt) ok &= ~(1u << t);
+                            lt[t] = lt[t] < nt ? lt[t] : 0;
+                        }
+#pragma unroll
+                        for (int t = 0; t < RGW_PF; t++) {
+                            tw[t] = s.tw[lt[t] * 16 + r];
+                            tbo[t] = s.tb[lt[t]];
+                        }
+#pragma unroll
+                        for (int t = 0; t < RGW_PF; t++) {
+                            const int c = jj[t] & 15;
+                            const uint32_t mrow = tw[t] & 0xFFFFu;
+                            const int off = tbo[t] + (int)(tw[t] >> 16) + __popc(mrow & rgw_below(c));
+                            sl[t] = ((ok >> t) & 1u) && (mrow & (0x8000u >> c)) && off < nv ? off : -1;
+                        }
+                        any = jj[0] < hi_col;
+                        more = lo + RGW_PF < hi && jj[RGW_PF - 1] < hi_col;
+                    }
+                    uint64_t mk = __ballot(any);
+                    const uint64_t mm = __ballot(more);
+                    while (mk) {  // entries in ascending k; one entry's outputs are distinct
+                        const int sgi = __ffsll((unsigned long long)mk) - 1;
+                        mk &= mk - 1;
+                        if (l == sgi) {  // the owner adds its staged products (all reads, then all writes)
+                            V old[RGW_PF];
+#pragma unroll
+                            for (int t = 0; t < RGW_PF; t++) old[t] = s.val[sl[t] >= 0 ? sl[t] : 0];
+#pragma unroll
+                            for (int t = 0; t < RGW_PF; t++)
+                                if (sl[t] >= 0) s.val[sl[t]] = rgw_fma<V>(a, bv[t], old[t]);
+                        }
+                        if ((mm >> sgi) & 1) {  // the rest of a long segment: the whole wave, 64 at a time
+                            const V sa = __shfl(a, sgi, WAVE);
+                            const int64_t slo = __shfl(lo, sgi, WAVE) + RGW_PF, shi = __shfl(hi, sgi, WAVE);
+                            for (int64_t f0 = slo; f0 < shi; f0 += WAVE) {  // wave-uniform trip
+                                const int64_t f = f0 + l;
+                                const int32_t j = f < shi ? colB[f] : INT_MAX;
+                                const V b = f < shi ? valB[f] : V(0);
+                                const int off = j < hi_col ? rgw_slot(s, j, c0, nt, nv, r) : -1;
+                                if (off >= 0) s.val[off] = rgw_fma<V>(sa, b, s.val[off]);
+                                if (__ballot(j >= hi_col)) break;  // the segment left the sub-window
                             }
                         }
                     }
                 }
-                __syncthreads();
-                // values and col bytes of the sub-window (16 threads per tile)
-                for (int q = tid; q < nt * 16; q += RGW_THREADS) {
-                    const int lt = q >> 4, rr = q & 15;
-                    const uint32_t tw = s.tw[q];
-                    uint32_t mrow = tw & 0xFFFFu;
-                    int o = s.tb[lt] + (int)(tw >> 16);
-                    while (mrow) {
-                        const int c = __clz(mrow) - 16;
-                        mrow ^= 0x8000u >> c;
-                        if (o < nv) {
-                            valC[vbase + o] = s.val[o];
-                            colC[vbase + o] = (uint8_t)((rr << 4) | c);
-                        }
-                        o++;
-                    }
-                }
-                __syncthreads();
-                sp0 = sp1;
             }
+            __syncthreads();
+            // values and col bytes of the sub-window (16 threads per tile)
+            for (int q = tid; q < nt * 16; q += RGW_THREADS) {
+                const int lt = q >> 4, rr = q & 15;
+                const uint32_t tw = s.tw[q];
+                uint32_t mrow = tw & 0xFFFFu;
+                int o = s.tb[lt] + (int)(tw >> 16);
+                while (mrow) {
+                    const int c = __clz(mrow) - 16;
+                    mrow ^= 0x8000u >> c;
+                    if (o < nv) {
+                        valC[vbase + o] = s.val[o];
+                        colC[vbase + o] = (uint8_t)((rr << 4) | c);
+                    }
+                    o++;
+                }
+            }
+            __syncthreads();
+            sp0 = sp1;
         }
     }
 }
@@ -376,19 +521,30 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
     return TSG_OK;
 }
 
+int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
+                       int64_t *woffn, const SlotLayout &sl) {
+    if (count <= 0) return TSG_OK;
+    const int64_t h = C->dtype == TSG_F32 ? rgw_item_values<float>() : rgw_item_values<double>();
+    k_rgw_count_num<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, C->tile_ptr,
+                                                                             sl.row_nc, C->tile_nnz, h, woffn);
+    TSG_HIP(hipGetLastError());
+    return scan_inclusive_i64(ctx, woffn, count + 1);
+}
+
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                      const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const SlotLayout &sl) {
+                      const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const int64_t *woffn,
+                      const SlotLayout &sl) {
     if (count <= 0) return TSG_OK;
     unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
     k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);
     if (A->dtype == TSG_F32)
         k_rgw_numeric<float><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
-            rows, begin, count, woff, ctr, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr,
+            rows, begin, count, woffn, ctr, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr,
             B->row_col, (const float *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
             C->tile_csr_Ptr, C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
     else
         k_rgw_numeric<double><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
-            rows, begin, count, woff, ctr, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr,
+            rows, begin, count, woffn, ctr, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr,
             B->row_col, (const double *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
             C->tile_csr_Ptr, C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
     TSG_HIP(hipGetLastError());
