@@ -106,7 +106,8 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t key) {
 __global__ void __launch_bounds__(RS_WAVES *WAVE)
 k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
-            int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats) {
+            int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats,
+            int32_t *__restrict__ hlist, int32_t *__restrict__ hcount) {
     __shared__ uint32_t s_bm[RS_WAVES][RS_WIN_WORDS];
     __shared__ int32_t s_bs[RS_WAVES][WAVE];
     __shared__ int32_t s_pre[RS_WAVES][WAVE + 1];
@@ -173,7 +174,9 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
         cmax = wave_max(cmax);
         nb = wave_sum(nb);
         pb = wave_sum(pb);
-        for (int64_t w0 = cmin; w0 <= cmax; w0 += (int64_t)RS_WIN_WORDS * 32) {
+        const bool heavy = hlist && nb > HEAVY_NB;  // counted by a workgroup afterwards
+        if (heavy && lane == 0) hlist[atomicAdd(hcount, 1)] = bi;
+        for (int64_t w0 = cmin; !heavy && w0 <= cmax; w0 += (int64_t)RS_WIN_WORDS * 32) {
             const int32_t wbits = (int32_t)min<int64_t>((int64_t)RS_WIN_WORDS * 32, cmax - w0 + 1);
             const int nwords = (wbits + 31) >> 5;
             for (int i = lane; i < nwords; i += WAVE) bm[i] = 0;
@@ -780,9 +783,20 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
     if (tilem == 0) {
         TSG_HIP(hipMemsetAsync(C->tile_ptr, 0, sizeof(int64_t), ctx->stream));
     } else {
-        k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
-            A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out);
-        TSG_HIP(hipGetLastError());
+        int32_t *hlist = nullptr, *hc = nullptr;  // heavy rows (counted by a workgroup each)
+        TSG_CHECK(palloc(ctx, &hlist, (size_t)tilem));
+        int rc = palloc(ctx, &hc, 4);
+        if (!rc && hipMemsetAsync(hc, 0, 4 * sizeof(int32_t), ctx->stream) != hipSuccess) rc = fail(TSG_ERR_HIP, "memset");
+        if (!rc) {
+            k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
+                A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
+                hlist, hc);
+            if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_stats launch");
+        }
+        if (!rc) rc = launch_heavy_count(ctx, A, B, C->tile_ptr, hlist, hc, stats_out);
+        pool_free(ctx, hlist);
+        pool_free(ctx, hc);
+        TSG_CHECK(rc);
         TSG_CHECK(scan_inclusive_i64(ctx, C->tile_ptr, (int64_t)tilem + 1));
     }
     TSG_CHECK(plan_bin(ctx, p, tilem, C->tile_ptr + tilem, nullptr));

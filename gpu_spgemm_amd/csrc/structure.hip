@@ -15,6 +15,7 @@
 // per block on the 2048x2048 Laplacian; this window needs 17 words there.
 // Count = popcount; fill = wave prefix over per-lane word chunks, so each lane
 // writes its columns in ascending order at its own offset.
+#include <algorithm>
 #include <climits>
 
 #include "tsg_internal.h"
@@ -30,7 +31,8 @@ k_structure(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ c
             const int64_t *__restrict__ tile_ptrB, const int32_t *__restrict__ colidxB,
             int64_t *__restrict__ tile_ptrC, int32_t *__restrict__ rowidxC,
             int32_t *__restrict__ colidxC, const int32_t *__restrict__ rows, int64_t row_begin,
-            int64_t nrows, int32_t *__restrict__ row_nc, int32_t *__restrict__ overflow) {
+            int64_t nrows, int32_t *__restrict__ row_nc, int32_t *__restrict__ overflow,
+            int32_t *__restrict__ hlist, int32_t *__restrict__ hcount) {
     __shared__ uint32_t s_bm[ST_WAVES][ST_WIN_WORDS];
     __shared__ int64_t s_bs[ST_WAVES][WAVE];
     __shared__ int32_t s_pre[ST_WAVES][WAVE];
@@ -42,9 +44,11 @@ k_structure(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ c
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
 
     int32_t cmin = INT_MAX, cmax = -1;
+    int64_t nb = 0;
     for (int64_t i = as + lane; i < ae; i += WAVE) {
         int32_t k = colidxA[i];
         int64_t bs = tile_ptrB[k], be = tile_ptrB[k + 1];
+        nb += be - bs;
         if (be > bs) {
             cmin = min(cmin, colidxB[bs]);
             cmax = max(cmax, colidxB[be - 1]);
@@ -52,6 +56,10 @@ k_structure(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ c
     }
     cmin = wave_min(cmin);
     cmax = wave_max(cmax);
+    if (hlist && wave_sum(nb) > HEAVY_NB) {  // a workgroup kernel takes this row
+        if (lane == 0) hlist[atomicAdd(hcount, 1)] = bi;
+        return;
+    }
 
     int64_t count = 0;
     int64_t out = FILL ? tile_ptrC[bi] : 0;
@@ -138,7 +146,7 @@ int launch_structure_count(Context *ctx, const tsg_tile_matrix *A, const tsg_til
     unsigned grid = (unsigned)((A->tilem + ST_WAVES - 1) / ST_WAVES);
     k_structure<false><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx, tile_ptrC, nullptr,
-        nullptr, nullptr, 0, 0, nullptr, nullptr);
+        nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -149,7 +157,173 @@ int launch_structure_fill(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
     unsigned grid = (unsigned)((A->tilem + ST_WAVES - 1) / ST_WAVES);
     k_structure<true><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx,
-        const_cast<int64_t *>(tile_ptrC), rowidxC, colidxC, nullptr, 0, 0, nullptr, nullptr);
+        const_cast<int64_t *>(tile_ptrC), rowidxC, colidxC, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
+// ---------------------------------------------------------------- heavy tile-rows
+// A tile-row reaching more than HEAVY_NB B tiles gets a whole workgroup: its
+// A tiles in blocks of 256 (a thread per A tile walks a short B tile-row list
+// itself; longer lists go to a queue the whole block walks, coalesced), into
+// an LDS bitmap of HS_WIN_WORDS * 32 tile columns per pass.  Rows come from a
+// device list; a persistent grid takes them from a counter.
+constexpr int HS_THREADS = 256;
+constexpr int HS_WIN_WORDS = 2048;  // 65,536 tile columns per pass
+constexpr int HS_SHORT = 64;        // B tile-row lists walked by one thread
+
+__device__ __forceinline__ int hs_block_sum(int v, int *s_w) {
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane_id() == 0) s_w[threadIdx.x / WAVE] = v;
+    __syncthreads();
+    int t = 0;
+    for (int w = 0; w < HS_THREADS / WAVE; w++) t += s_w[w];
+    return t;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(HS_THREADS)
+k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__ hcount,
+                  unsigned long long *__restrict__ ctr, const int64_t *__restrict__ tile_ptrA,
+                  const int32_t *__restrict__ colidxA, const int64_t *__restrict__ tile_ptrB,
+                  const int32_t *__restrict__ colidxB, int64_t *__restrict__ tile_ptrC,
+                  int32_t *__restrict__ rowidxC, int32_t *__restrict__ colidxC, int32_t *__restrict__ row_nc,
+                  int32_t *__restrict__ overflow, int32_t *__restrict__ stats) {
+    __shared__ uint32_t bm[HS_WIN_WORDS];
+    __shared__ int64_t s_qbs[HS_THREADS];
+    __shared__ int32_t s_qlen[HS_THREADS];
+    __shared__ int s_w[HS_THREADS / WAVE];
+    __shared__ int s_nq, s_h;
+    __shared__ int32_t s_min, s_max;
+    const int tid = threadIdx.x;
+    const int nh = *hcount;
+    for (;;) {
+        if (tid == 0) {
+            s_h = (int)atomicAdd(ctr, 1ull);
+            s_nq = 0;
+            s_min = INT_MAX;
+            s_max = -1;
+        }
+        __syncthreads();
+        const int h = s_h;
+        if (h >= nh) break;  // uniform
+        const int32_t bi = hlist[h];
+        const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
+        {
+            int32_t mn = INT_MAX, mx = -1;
+            for (int64_t i = as + tid; i < ae; i += HS_THREADS) {
+                const int32_t k = colidxA[i];
+                const int64_t bs = tile_ptrB[k], be = tile_ptrB[k + 1];
+                if (be > bs) {
+                    mn = min(mn, colidxB[bs]);
+                    mx = max(mx, colidxB[be - 1]);
+                }
+            }
+            mn = wave_min(mn);
+            mx = wave_max(mx);
+            if (lane_id() == 0) {
+                atomicMin(&s_min, mn);
+                atomicMax(&s_max, mx);
+            }
+        }
+        __syncthreads();
+        const int32_t cmin = s_min, cmax = s_max;
+        int64_t out = FILL ? tile_ptrC[bi] : 0;
+        const int64_t slot_end = (FILL && row_nc) ? tile_ptrC[bi + 1] : INT64_MAX;
+        int64_t count = 0;
+        for (int64_t w0 = cmin; w0 <= cmax; w0 += (int64_t)HS_WIN_WORDS * 32) {  // uniform
+            const int64_t wbits = min<int64_t>((int64_t)HS_WIN_WORDS * 32, cmax - w0 + 1);
+            for (int i = tid; i < HS_WIN_WORDS; i += HS_THREADS) bm[i] = 0u;
+            __syncthreads();
+            for (int64_t base = as; base < ae; base += HS_THREADS) {  // uniform trip count
+                const int64_t i = base + tid;
+                if (i < ae) {
+                    const int32_t k = colidxA[i];
+                    const int64_t bs = tile_ptrB[k];
+                    const int32_t len = (int32_t)(tile_ptrB[k + 1] - bs);
+                    if (len > HS_SHORT) {
+                        const int qi = atomicAdd(&s_nq, 1);
+                        s_qbs[qi] = bs;
+                        s_qlen[qi] = len;
+                    } else {
+                        for (int32_t q = 0; q < len; q++) {
+                            const int64_t off = (int64_t)colidxB[bs + q] - w0;
+                            if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
+                        }
+                    }
+                }
+                __syncthreads();
+                const int nq = s_nq;
+                for (int qi = 0; qi < nq; qi++) {  // long lists: the whole block, coalesced
+                    const int64_t bs = s_qbs[qi];
+                    const int32_t len = s_qlen[qi];
+                    for (int32_t q = tid; q < len; q += HS_THREADS) {
+                        const int64_t off = (int64_t)colidxB[bs + q] - w0;
+                        if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
+                    }
+                }
+                __syncthreads();
+                if (tid == 0) s_nq = 0;
+                __syncthreads();
+            }
+            // each thread owns HS_WIN_WORDS / HS_THREADS consecutive words
+            constexpr int WPT = HS_WIN_WORDS / HS_THREADS;
+            int cnt = 0;
+#pragma unroll
+            for (int q = 0; q < WPT; q++) cnt += __popc(bm[tid * WPT + q]);
+            if (!FILL) {
+                count += hs_block_sum(cnt, s_w);
+            } else {
+                // exclusive scan of the per-thread counts: waves, then wave totals
+                const int incl = wave_incl_sum(cnt);
+                __syncthreads();
+                if (lane_id() == WAVE - 1) s_w[tid / WAVE] = incl;
+                __syncthreads();
+                int before = 0, tot = 0;
+                for (int w = 0; w < HS_THREADS / WAVE; w++) {
+                    if (w < tid / WAVE) before += s_w[w];
+                    tot += s_w[w];
+                }
+                int64_t pos = out + before + incl - cnt;
+#pragma unroll
+                for (int q = 0; q < WPT; q++) {
+                    uint32_t b = bm[tid * WPT + q];
+                    while (b) {
+                        const int bit = __ffs(b) - 1;
+                        b &= b - 1;
+                        if (pos < slot_end) {
+                            rowidxC[pos] = bi;
+                            colidxC[pos] = (int32_t)(w0 + (tid * WPT + q) * 32 + bit);
+                        }
+                        pos++;
+                    }
+                }
+                out += tot;
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            if (!FILL) {
+                tile_ptrC[bi + 1] = count;
+                if (stats) stats[(int64_t)bi * 8 + 4] = (int32_t)min<int64_t>(count, INT_MAX);
+            } else if (row_nc) {
+                const int64_t n = min<int64_t>(out, slot_end) - tile_ptrC[bi];
+                row_nc[bi] = (int32_t)min<int64_t>(n, INT32_MAX);
+                if (out > slot_end) atomicOr(overflow, 1);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Exact C tile counts of the heavy rows listed by k_row_stats (written into
+// tile_ptrC[bi + 1] before its scan).  hc = {row count, work counter}.
+int launch_heavy_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, int64_t *tile_ptrC,
+                       const int32_t *hlist, int32_t *hc, int32_t *stats) {
+    k_heavy_structure<false><<<(unsigned)std::max(1, ctx->prop.multiProcessorCount * 8), HS_THREADS, 0, ctx->stream>>>(
+        hlist, hc, reinterpret_cast<unsigned long long *>(hc + 2), A->tile_ptr, A->tile_columnidx, B->tile_ptr,
+        B->tile_columnidx, tile_ptrC, nullptr, nullptr, nullptr, nullptr, stats);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -158,12 +332,27 @@ int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg
                                const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
                                const SlotLayout &sl) {
     if (count <= 0) return TSG_OK;
-    unsigned grid = (unsigned)((count + ST_WAVES - 1) / ST_WAVES);
-    k_structure<true><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
-        A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx, C->tile_ptr, C->tile_rowidx,
-        C->tile_columnidx, rows, begin, count, sl.row_nc, sl.overflow);
-    TSG_HIP(hipGetLastError());
-    return TSG_OK;
+    int32_t *hlist = nullptr, *hc = nullptr;  // heavy rows; hc = {count, pad, work counter (8 bytes)}
+    TSG_CHECK(palloc(ctx, &hlist, (size_t)count));
+    int rc = palloc(ctx, &hc, 4);
+    if (!rc && hipMemsetAsync(hc, 0, 4 * sizeof(int32_t), ctx->stream) != hipSuccess) rc = fail(TSG_ERR_HIP, "memset");
+    if (!rc) {
+        unsigned grid = (unsigned)((count + ST_WAVES - 1) / ST_WAVES);
+        k_structure<true><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
+            A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx, C->tile_ptr, C->tile_rowidx,
+            C->tile_columnidx, rows, begin, count, sl.row_nc, sl.overflow, hlist, hc);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_structure launch");
+    }
+    if (!rc) {
+        k_heavy_structure<true><<<(unsigned)std::max(1, ctx->prop.multiProcessorCount * 8), HS_THREADS, 0,
+                                   ctx->stream>>>(
+            hlist, hc, reinterpret_cast<unsigned long long *>(hc + 2), A->tile_ptr, A->tile_columnidx, B->tile_ptr,
+            B->tile_columnidx, C->tile_ptr, C->tile_rowidx, C->tile_columnidx, sl.row_nc, sl.overflow, nullptr);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_heavy_structure launch");
+    }
+    pool_free(ctx, hlist);
+    pool_free(ctx, hc);
+    return rc;
 }
 
 }  // namespace tsg

@@ -112,6 +112,13 @@ struct SlotLayout {
     int32_t *overflow = nullptr;   // set when a row or C's values outgrow their capacity
     int64_t val_cap = INT64_MAX;   // capacity of C's value arrays
 };
+// Tile-rows reaching more than HEAVY_NB B tiles are counted and filled by a
+// workgroup each (structure.hip) instead of one wave.
+constexpr int64_t HEAVY_NB = 8192;
+// C tile counts of the heavy rows in hlist (hc = {row count, pad, 64-bit work
+// counter}, zeroed before the list is built), written to tile_ptrC[bi + 1].
+int launch_heavy_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, int64_t *tile_ptrC,
+                       const int32_t *hlist, int32_t *hc, int32_t *stats);
 int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                                const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
                                const SlotLayout &sl);
