@@ -23,9 +23,10 @@
 //     present exactly when some product reaches it, which is the reference's
 //     OR over tile pairs.  Long B row segments are walked by the whole
 //     workgroup.
-//   k_rgw_numeric: the window's values (in sub-windows of at most RGW_TILES
-//     tiles and RGW_VBYTES bytes of values), a map tile column -> sub-window
-//     tile, and each tile's packed (row offset, row mask) words.  The waves
+//   k_rgw_numeric: a sub-window of C tiles (as many as fit RGN_BYTES of LDS
+//     with their values, within RGN_COLS tile columns): the values, a map tile
+//     column -> sub-window tile, and each tile's packed (row offset, row mask)
+//     words.  The waves
 //     take the tile-row's 16 scalar rows in turn; for its row a wave loads 64
 //     A entries at a time (a lane per entry: the entry's B segment in the
 //     sub-window, its first products' B values and output slots), then adds
@@ -49,8 +50,10 @@ constexpr int RGW_THREADS = 256;       // 16 scalar rows x 16 lanes
 constexpr int RGW_COLS = 1024;         // C tile columns per window
 constexpr int RGW_ITEM_TILES = 4096;   // C tiles per work item (aim)
 constexpr int RGW_GMAX = 64;           // windows per work item (cap)
-constexpr int RGW_TILES = 256;         // tiles per numeric sub-window
-constexpr int RGW_VBYTES = 32768;      // value bytes per numeric sub-window
+constexpr int RGN_COLS = 4096;         // tile-column span of a numeric sub-window (LDS map)
+constexpr int RGN_BYTES = 36864;       // LDS bytes of a numeric sub-window: RGN_TILE_BYTES a tile + its values
+constexpr int RGN_CUR = 512;           // A entries of a tile-row whose B positions carry over sub-windows
+constexpr int RGN_TILE_BYTES = 68;     // 16 packed row words + the tile's value offset
 constexpr int RGW_LONG = 64;           // B segments longer than this are walked by the workgroup
 constexpr int RGW_ILP = 4;             // A entries per thread per symbolic pass (independent loads)
 constexpr int RGW_PF = 16;             // B entries loaded at once by one lane
@@ -208,8 +211,8 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
                     for (int st = 8; st > 0; st >>= 1)
                         if (r + st <= 15 && s_rp[r + st] <= e) r += st;
                     const int64_t lo = jf[u] >= lo_col ? b0[u] : lower_bound<int64_t, int32_t>(colB, b0[u], b1[u], lo_col);
-                    int qi = RGW_THREADS;
-                    if (b1[u] - lo > RGW_LONG) qi = atomicAdd(&s_nq, 1);
+                    int qi = RGW_THREADS;  // long: more than RGW_LONG entries inside the window
+                    if (b1[u] - lo > RGW_LONG && colB[lo + RGW_LONG] < hi_col) qi = atomicAdd(&s_nq, 1);
                     if (qi < RGW_THREADS) {
                         s_q[qi] = RgwSeg{lo, b1[u], r};
                     } else {
@@ -262,19 +265,21 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
     }
 }
 
-template <typename V>
+// A sub-window's LDS: nt tiles' packed row words tw[nt * 16] ((row offset <<
+// 16) | row mask), their value offsets tb[nt], then the values, in buf; and
+// the map tile column - c0 -> sub-window tile (0xFFFF: none).
 struct RgwNumLDS {
-    V val[RGW_VBYTES / sizeof(V)];
-    uint32_t tw[RGW_TILES * 16];  // (row offset << 16) | row mask
-    int32_t tb[RGW_TILES];        // value offset of each tile in the sub-window
-    uint16_t map[RGW_COLS];       // tile column - c0 -> sub-window tile (0xFFFF: none)
+    alignas(16) unsigned char buf[RGN_BYTES];
+    int64_t cur[RGN_CUR];  // per A entry (first RGN_CUR of the tile-row): next B position
+    int64_t hi[RGN_CUR];   // and the end of its B row
+    uint16_t map[RGN_COLS];
 };
+__host__ __device__ constexpr int rgn_val_offset(int nt) { return (nt * RGN_TILE_BYTES + 15) & ~15; }
 
-// Values of one numeric work item (at most): an item holds the tiles of one
-// tile-row whose value offsets start in one block of RGW_VBYTES / sizeof(V) -
-// 256 values, so with the last tile's (at most 256) values it fits LDS.
+// Values of one numeric work item: the tiles of one tile-row whose value
+// offsets start in one block of this many values (a few sub-windows' worth).
 template <typename V>
-constexpr int rgw_item_values() { return RGW_VBYTES / (int)sizeof(V) - 256; }
+constexpr int rgw_item_values() { return 2 * RGN_BYTES / (int)sizeof(V); }
 
 // Numeric work items of each listed row: woffn[q + 1] = ceil(row values / item values).
 __global__ void __launch_bounds__(256)
@@ -296,17 +301,17 @@ struct RgwNumItem {
 };
 
 // Output slot of product column j for scalar row r in the staged sub-window, or -1.
-template <typename V>
-__device__ __forceinline__ int rgw_slot(const RgwNumLDS<V> &s, int32_t j, int32_t c0, int nt, int nv, int r) {
+__device__ __forceinline__ int rgw_slot(const uint16_t *map, const uint32_t *tws, const int32_t *tbs, int32_t j,
+                                        int32_t c0, int nt, int nv, int r) {
     const int tc = (j >> 4) - c0;
-    if ((unsigned)tc >= (unsigned)RGW_COLS) return -1;
-    const int lt = s.map[tc];
+    if ((unsigned)tc >= (unsigned)RGN_COLS) return -1;
+    const int lt = map[tc];
     if (lt >= nt) return -1;
-    const uint32_t tw = s.tw[lt * 16 + r];
+    const uint32_t tw = tws[lt * 16 + r];
     const int c = j & 15;
     const uint32_t mrow = tw & 0xFFFFu;
     if (!(mrow & (0x8000u >> c))) return -1;
-    const int off = s.tb[lt] + (int)(tw >> 16) + __popc(mrow & rgw_below(c));
+    const int off = tbs[lt] + (int)(tw >> 16) + __popc(mrow & rgw_below(c));
     return off < nv ? off : -1;
 }
 
@@ -321,7 +326,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
               const int64_t *__restrict__ tile_nnzC, V *__restrict__ valC, uint8_t *__restrict__ colC,
               int32_t *__restrict__ overflow, int64_t val_cap) {
     constexpr int64_t H = rgw_item_values<V>();
-    __shared__ RgwNumLDS<V> s;
+    __shared__ RgwNumLDS s;
     __shared__ int64_t s_rp[17];
     __shared__ RgwNumItem s_it;
     __shared__ int64_t s_item, s_sp1;
@@ -362,25 +367,37 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
         }
         const int64_t tb = s_it.tb;
         for (int64_t sp0 = s_it.ta; sp0 < tb;) {  // uniform sub-window loop
-            if (tid == 0) {  // at most RGW_TILES tiles within RGW_COLS tile columns
-                const int64_t hi = min<int64_t>(tb, sp0 + RGW_TILES);
-                s_sp1 = lower_bound<int64_t, int32_t>(colidxC, sp0 + 1, hi, colidxC[sp0] + RGW_COLS);
+            const bool first = sp0 == s_it.ta;
+            if (tid == 0) {  // largest end whose tiles and values fit RGN_BYTES, within RGN_COLS columns
+                const int64_t v0 = tile_nnzC[sp0];
+                int64_t lo = sp0 + 1, hi = min<int64_t>(tb, sp0 + RGN_BYTES / RGN_TILE_BYTES);
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi + 1) >> 1;
+                    if (rgn_val_offset((int)(mid - sp0)) + (tile_nnzC[mid] - v0) * (int64_t)sizeof(V) <= RGN_BYTES)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                s_sp1 = lower_bound<int64_t, int32_t>(colidxC, sp0 + 1, lo, colidxC[sp0] + RGN_COLS);
                 s_row = 0;
             }
-            for (int q = tid; q < RGW_COLS; q += RGW_THREADS) s.map[q] = 0xFFFF;
+            for (int q = tid; q < RGN_COLS; q += RGW_THREADS) s.map[q] = 0xFFFF;
             __syncthreads();
             const int64_t sp1 = s_sp1;
             const int nt = (int)(sp1 - sp0);
+            uint32_t *const tws = reinterpret_cast<uint32_t *>(s.buf);
+            int32_t *const tbs = reinterpret_cast<int32_t *>(s.buf + nt * 64);
+            V *const vals = reinterpret_cast<V *>(s.buf + rgn_val_offset(nt));
             const int64_t vbase = tile_nnzC[sp0];
-            const int nv = (int)min<int64_t>(tile_nnzC[sp1] - vbase, RGW_VBYTES / (int)sizeof(V));
+            const int nv = (int)min<int64_t>(tile_nnzC[sp1] - vbase, (RGN_BYTES - rgn_val_offset(nt)) / (int)sizeof(V));
             const int32_t c0 = colidxC[sp0];
             for (int q = tid; q < nt; q += RGW_THREADS) {
                 s.map[colidxC[sp0 + q] - c0] = (uint16_t)q;
-                s.tb[q] = (int32_t)(tile_nnzC[sp0 + q] - vbase);
+                tbs[q] = (int32_t)(tile_nnzC[sp0 + q] - vbase);
             }
             for (int q = tid; q < nt * 16; q += RGW_THREADS)
-                s.tw[q] = ((uint32_t)ptrRowC[sp0 * 16 + q] << 16) | maskC[sp0 * 16 + q];
-            for (int q = tid; q < nv; q += RGW_THREADS) s.val[q] = V(0);
+                tws[q] = ((uint32_t)ptrRowC[sp0 * 16 + q] << 16) | maskC[sp0 * 16 + q];
+            for (int q = tid; q < nv; q += RGW_THREADS) vals[q] = V(0);
             __syncthreads();
             const int32_t lo_col = c0 * 16;
             const int64_t hi_col = (int64_t)colidxC[sp1 - 1] * 16 + 16;
@@ -397,15 +414,28 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                     // lane l owns entry e0 + l: its B segment in the sub-window,
                     // the first RGW_PF products' B values and output slots (each
                     // stage's loads issued together, all lanes at once)
+                    // Within an item the sub-windows follow in column order and no B
+                    // entry falls between two of them (it would have a C tile), so
+                    // an entry's segment starts where its previous one ended: the
+                    // first RGN_CUR entries of the tile-row keep that position in
+                    // LDS and search only in the item's first sub-window.
                     const int64_t e = e0 + l;
+                    const int64_t ci = e - s_rp[0];
+                    const bool cached = ci < RGN_CUR;
                     int64_t lo = 0, hi = 0;
                     V a = V(0);
                     if (e < eb) {
-                        const int32_t k = colA[e];
                         a = valA[e];
-                        hi = rpB[k + 1];
-                        lo = rgw_seek(colB, rpB[k], hi, lo_col);
+                        if (cached && !first) {
+                            lo = s.cur[ci];
+                            hi = s.hi[ci];
+                        } else {
+                            const int32_t k = colA[e];
+                            hi = rpB[k + 1];
+                            lo = rgw_seek(colB, rpB[k], hi, lo_col);
+                        }
                     }
+                    int64_t next = lo;  // where the segment ends (the next sub-window's start)
                     int sl[RGW_PF];
                     V bv[RGW_PF];
                     bool more = false, any = false;
@@ -424,7 +454,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
 #pragma unroll
                         for (int t = 0; t < RGW_PF; t++) {
                             tc[t] = (jj[t] >> 4) - c0;
-                            const bool v = lo + t < hi && jj[t] < hi_col && (unsigned)tc[t] < (unsigned)RGW_COLS;
+                            const bool v = lo + t < hi && jj[t] < hi_col && (unsigned)tc[t] < (unsigned)RGN_COLS;
                             ok |= (uint32_t)v << t;
                             tc[t] = v ? tc[t] : 0;
                         }
@@ -437,8 +467,8 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                         }
 #pragma unroll
                         for (int t = 0; t < RGW_PF; t++) {
-                            tw[t] = s.tw[lt[t] * 16 + r];
-                            tbo[t] = s.tb[lt[t]];
+                            tw[t] = tws[lt[t] * 16 + r];
+                            tbo[t] = tbs[lt[t]];
                         }
 #pragma unroll
                         for (int t = 0; t < RGW_PF; t++) {
@@ -449,6 +479,8 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                         }
                         any = jj[0] < hi_col;
                         more = lo + RGW_PF < hi && jj[RGW_PF - 1] < hi_col;
+#pragma unroll
+                        for (int t = 0; t < RGW_PF; t++) next += (lo + t < hi && jj[t] < hi_col) ? 1 : 0;
                     }
                     uint64_t mk = __ballot(any);
                     const uint64_t mm = __ballot(more);
@@ -458,23 +490,33 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                         if (l == sgi) {  // the owner adds its staged products (all reads, then all writes)
                             V old[RGW_PF];
 #pragma unroll
-                            for (int t = 0; t < RGW_PF; t++) old[t] = s.val[sl[t] >= 0 ? sl[t] : 0];
+                            for (int t = 0; t < RGW_PF; t++) old[t] = vals[sl[t] >= 0 ? sl[t] : 0];
 #pragma unroll
                             for (int t = 0; t < RGW_PF; t++)
-                                if (sl[t] >= 0) s.val[sl[t]] = rgw_fma<V>(a, bv[t], old[t]);
+                                if (sl[t] >= 0) vals[sl[t]] = rgw_fma<V>(a, bv[t], old[t]);
                         }
                         if ((mm >> sgi) & 1) {  // the rest of a long segment: the whole wave, 64 at a time
                             const V sa = __shfl(a, sgi, WAVE);
                             const int64_t slo = __shfl(lo, sgi, WAVE) + RGW_PF, shi = __shfl(hi, sgi, WAVE);
-                            for (int64_t f0 = slo; f0 < shi; f0 += WAVE) {  // wave-uniform trip
+                            int64_t f0 = slo;
+                            for (; f0 < shi; f0 += WAVE) {  // wave-uniform trip
                                 const int64_t f = f0 + l;
                                 const int32_t j = f < shi ? colB[f] : INT_MAX;
                                 const V b = f < shi ? valB[f] : V(0);
-                                const int off = j < hi_col ? rgw_slot(s, j, c0, nt, nv, r) : -1;
-                                if (off >= 0) s.val[off] = rgw_fma<V>(sa, b, s.val[off]);
-                                if (__ballot(j >= hi_col)) break;  // the segment left the sub-window
+                                const int off = j < hi_col ? rgw_slot(s.map, tws, tbs, j, c0, nt, nv, r) : -1;
+                                if (off >= 0) vals[off] = rgw_fma<V>(sa, b, vals[off]);
+                                const uint64_t past = __ballot(j >= hi_col);
+                                if (past) {  // the segment left the sub-window
+                                    f0 += __ffsll((unsigned long long)past) - 1;
+                                    break;
+                                }
                             }
+                            if (l == sgi) next = min(f0, shi);
                         }
+                    }
+                    if (e < eb && cached) {
+                        s.cur[ci] = next;
+                        s.hi[ci] = hi;
                     }
                 }
             }
@@ -482,14 +524,14 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
             // values and col bytes of the sub-window (16 threads per tile)
             for (int q = tid; q < nt * 16; q += RGW_THREADS) {
                 const int lt = q >> 4, rr = q & 15;
-                const uint32_t tw = s.tw[q];
+                const uint32_t tw = tws[q];
                 uint32_t mrow = tw & 0xFFFFu;
-                int o = s.tb[lt] + (int)(tw >> 16);
+                int o = tbs[lt] + (int)(tw >> 16);
                 while (mrow) {
                     const int c = __clz(mrow) - 16;
                     mrow ^= 0x8000u >> c;
                     if (o < nv) {
-                        valC[vbase + o] = s.val[o];
+                        valC[vbase + o] = vals[o];
                         colC[vbase + o] = (uint8_t)((rr << 4) | c);
                     }
                     o++;
