@@ -344,7 +344,8 @@ def main():
         runs["mfma"] = measure("C-tile + matrix cores", variant=2)
     if args.scheduler == "learned" or rmat:
         train_learned(g, ctx, log, int(args.workload[4:]) if rmat else 0)
-        _, st = ctx.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
+        C0, st = ctx.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
+        C0.free()
         log(f"learned: fallback {st['fallback_used']} slots {st['pred_numtileC']} (tiles {st['numtileC']}) "
             f"values {st['pred_nnzC']} (nnz {st['nnzC']})")
         learned_fallback = st["fallback_used"]

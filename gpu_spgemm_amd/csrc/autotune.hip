@@ -108,7 +108,7 @@ int tsg_autotune(tsg_context *c, int32_t count, const tsg_tile_matrix *A, const 
     bool dups = false;
     for (int32_t i = 0; i < count; i++) dups = dups || ((A[i].flags | B[i].flags) & 1);
     std::vector<int> masks = staged ? std::vector<int>{7, 5, 1, 4, 0} : std::vector<int>{0};
-    std::vector<int> dens = rg ? std::vector<int>{0, 2, 8, 32} : std::vector<int>{0};
+    std::vector<int> dens = rg ? std::vector<int>{0, 8, 64, 1024} : std::vector<int>{0};
     std::vector<TuneCfg> cands;
     for (int mk : masks)
         for (int d : dens) {

@@ -57,6 +57,7 @@ constexpr int RGN_TILE_BYTES = 68;     // 16 packed row words + the tile's value
 constexpr int RGW_LONG = 64;           // B segments longer than this are walked by the workgroup
 constexpr int RGW_ILP = 4;             // A entries per thread per symbolic pass (independent loads)
 constexpr int RGW_PF = 16;             // B entries loaded at once by one lane
+constexpr int RGW_PF0 = 4;             // of which the first stage
 
 __device__ __forceinline__ uint32_t rgw_below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
 
@@ -216,10 +217,18 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
                     if (qi < RGW_THREADS) {
                         s_q[qi] = RgwSeg{lo, b1[u], r};
                     } else {
-                        for (int64_t f0 = lo; f0 < b1[u]; f0 += RGW_PF) {  // RGW_PF loads issued together
+                        for (int64_t f0 = lo; f0 < b1[u]; f0 += RGW_PF) {  // up to RGW_PF loads issued together
                             int32_t jj[RGW_PF];
 #pragma unroll
-                            for (int t = 0; t < RGW_PF; t++) jj[t] = colB[min<int64_t>(f0 + t, b1[u] - 1)];
+                            for (int t = 0; t < RGW_PF; t++) jj[t] = INT_MAX;
+#pragma unroll
+                            for (int t = 0; t < RGW_PF0; t++)
+                                if (f0 + t < b1[u]) jj[t] = colB[f0 + t];
+                            if (f0 + RGW_PF0 < b1[u] && jj[RGW_PF0 - 1] < hi_col) {
+#pragma unroll
+                                for (int t = RGW_PF0; t < RGW_PF; t++)
+                                    if (f0 + t < b1[u]) jj[t] = colB[f0 + t];
+                            }
                             bool past = false;
 #pragma unroll
                             for (int t = 0; t < RGW_PF; t++) {
@@ -444,11 +453,26 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                     if (lo < hi) {
                         int32_t jj[RGW_PF], tc[RGW_PF], lt[RGW_PF], tbo[RGW_PF];
                         uint32_t tw[RGW_PF];
+                        // two stages: the first RGW_PF0 entries, the rest only while the
+                        // segment stays in the sub-window (no load for lanes past the row)
 #pragma unroll
                         for (int t = 0; t < RGW_PF; t++) {
-                            const int64_t f = min<int64_t>(lo + t, hi - 1);  // clamped: in B row k
-                            jj[t] = colB[f];
-                            bv[t] = valB[f];
+                            jj[t] = INT_MAX;
+                            bv[t] = V(0);
+                        }
+#pragma unroll
+                        for (int t = 0; t < RGW_PF0; t++)
+                            if (lo + t < hi) {
+                                jj[t] = colB[lo + t];
+                                bv[t] = valB[lo + t];
+                            }
+                        if (lo + RGW_PF0 < hi && jj[RGW_PF0 - 1] < hi_col) {
+#pragma unroll
+                            for (int t = RGW_PF0; t < RGW_PF; t++)
+                                if (lo + t < hi) {
+                                    jj[t] = colB[lo + t];
+                                    bv[t] = valB[lo + t];
+                                }
                         }
                         uint32_t ok = 0;
 #pragma unroll

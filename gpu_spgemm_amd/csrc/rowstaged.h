@@ -22,7 +22,7 @@ struct BinTable {
     int32_t rg;          // 1: the row-by-row bin exists
     int32_t rg_density;  // A entries per A tile at or below which a heavy row goes row-by-row
 };
-constexpr int RG_DENSITY_DEFAULT = 8;
+constexpr int RG_DENSITY_DEFAULT = 64;
 
 // Per-call working set: A->B links, per-row bins, bin-ordered row lists.
 struct Plan {
