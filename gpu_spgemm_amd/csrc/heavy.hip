@@ -56,8 +56,9 @@ constexpr int RGN_CUR = 512;           // A entries of a tile-row whose B positi
 constexpr int RGN_TILE_BYTES = 68;     // 16 packed row words + the tile's value offset
 constexpr int RGW_LONG = 64;           // B segments longer than this are walked by the workgroup
 constexpr int RGW_ILP = 4;             // A entries per thread per symbolic pass (independent loads)
-constexpr int RGW_PF = 16;             // B entries loaded at once by one lane
+constexpr int RGW_PF = 16;             // B entries loaded at once by one symbolic thread
 constexpr int RGW_PF0 = 4;             // of which the first stage
+constexpr int RGN_PF = 8;              // B entries a numeric lane stages for its A entry (VGPRs)
 
 __device__ __forceinline__ uint32_t rgw_below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
 
@@ -421,7 +422,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                 const int64_t ea = s_rp[r], eb = s_rp[r + 1];
                 for (int64_t e0 = ea; e0 < eb; e0 += WAVE) {  // 64 A entries of row r at a time
                     // lane l owns entry e0 + l: its B segment in the sub-window,
-                    // the first RGW_PF products' B values and output slots (each
+                    // the first RGN_PF products' B values and output slots (each
                     // stage's loads issued together, all lanes at once)
                     // Within an item the sub-windows follow in column order and no B
                     // entry falls between two of them (it would have a C tile), so
@@ -445,18 +446,18 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                         }
                     }
                     int64_t next = lo;  // where the segment ends (the next sub-window's start)
-                    int sl[RGW_PF];
-                    V bv[RGW_PF];
+                    int sl[RGN_PF];
+                    V bv[RGN_PF];
                     bool more = false, any = false;
 #pragma unroll
-                    for (int t = 0; t < RGW_PF; t++) sl[t] = -1;
+                    for (int t = 0; t < RGN_PF; t++) sl[t] = -1;
                     if (lo < hi) {
-                        int32_t jj[RGW_PF], tc[RGW_PF], lt[RGW_PF], tbo[RGW_PF];
-                        uint32_t tw[RGW_PF];
+                        int32_t jj[RGN_PF], tc[RGN_PF], lt[RGN_PF], tbo[RGN_PF];
+                        uint32_t tw[RGN_PF];
                         // two stages: the first RGW_PF0 entries, the rest only while the
                         // segment stays in the sub-window (no load for lanes past the row)
 #pragma unroll
-                        for (int t = 0; t < RGW_PF; t++) {
+                        for (int t = 0; t < RGN_PF; t++) {
                             jj[t] = INT_MAX;
                             bv[t] = V(0);
                         }
@@ -468,7 +469,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                             }
                         if (lo + RGW_PF0 < hi && jj[RGW_PF0 - 1] < hi_col) {
 #pragma unroll
-                            for (int t = RGW_PF0; t < RGW_PF; t++)
+                            for (int t = RGW_PF0; t < RGN_PF; t++)
                                 if (lo + t < hi) {
                                     jj[t] = colB[lo + t];
                                     bv[t] = valB[lo + t];
@@ -476,35 +477,35 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                         }
                         uint32_t ok = 0;
 #pragma unroll
-                        for (int t = 0; t < RGW_PF; t++) {
+                        for (int t = 0; t < RGN_PF; t++) {
                             tc[t] = (jj[t] >> 4) - c0;
                             const bool v = lo + t < hi && jj[t] < hi_col && (unsigned)tc[t] < (unsigned)RGN_COLS;
                             ok |= (uint32_t)v << t;
                             tc[t] = v ? tc[t] : 0;
                         }
 #pragma unroll
-                        for (int t = 0; t < RGW_PF; t++) lt[t] = s.map[tc[t]];
+                        for (int t = 0; t < RGN_PF; t++) lt[t] = s.map[tc[t]];
 #pragma unroll
-                        for (int t = 0; t < RGW_PF; t++) {
+                        for (int t = 0; t < RGN_PF; t++) {
                             if (lt[t] >= nt) ok &= ~(1u << t);
                             lt[t] = lt[t] < nt ? lt[t] : 0;
                         }
 #pragma unroll
-                        for (int t = 0; t < RGW_PF; t++) {
+                        for (int t = 0; t < RGN_PF; t++) {
                             tw[t] = tws[lt[t] * 16 + r];
                             tbo[t] = tbs[lt[t]];
                         }
 #pragma unroll
-                        for (int t = 0; t < RGW_PF; t++) {
+                        for (int t = 0; t < RGN_PF; t++) {
                             const int c = jj[t] & 15;
                             const uint32_t mrow = tw[t] & 0xFFFFu;
                             const int off = tbo[t] + (int)(tw[t] >> 16) + __popc(mrow & rgw_below(c));
                             sl[t] = ((ok >> t) & 1u) && (mrow & (0x8000u >> c)) && off < nv ? off : -1;
                         }
                         any = jj[0] < hi_col;
-                        more = lo + RGW_PF < hi && jj[RGW_PF - 1] < hi_col;
+                        more = lo + RGN_PF < hi && jj[RGN_PF - 1] < hi_col;
 #pragma unroll
-                        for (int t = 0; t < RGW_PF; t++) next += (lo + t < hi && jj[t] < hi_col) ? 1 : 0;
+                        for (int t = 0; t < RGN_PF; t++) next += (lo + t < hi && jj[t] < hi_col) ? 1 : 0;
                     }
                     uint64_t mk = __ballot(any);
                     const uint64_t mm = __ballot(more);
@@ -512,16 +513,16 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                         const int sgi = __ffsll((unsigned long long)mk) - 1;
                         mk &= mk - 1;
                         if (l == sgi) {  // the owner adds its staged products (all reads, then all writes)
-                            V old[RGW_PF];
+                            V old[RGN_PF];
 #pragma unroll
-                            for (int t = 0; t < RGW_PF; t++) old[t] = vals[sl[t] >= 0 ? sl[t] : 0];
+                            for (int t = 0; t < RGN_PF; t++) old[t] = vals[sl[t] >= 0 ? sl[t] : 0];
 #pragma unroll
-                            for (int t = 0; t < RGW_PF; t++)
+                            for (int t = 0; t < RGN_PF; t++)
                                 if (sl[t] >= 0) vals[sl[t]] = rgw_fma<V>(a, bv[t], old[t]);
                         }
                         if ((mm >> sgi) & 1) {  // the rest of a long segment: the whole wave, 64 at a time
                             const V sa = __shfl(a, sgi, WAVE);
-                            const int64_t slo = __shfl(lo, sgi, WAVE) + RGW_PF, shi = __shfl(hi, sgi, WAVE);
+                            const int64_t slo = __shfl(lo, sgi, WAVE) + RGN_PF, shi = __shfl(hi, sgi, WAVE);
                             int64_t f0 = slo;
                             for (; f0 < shi; f0 += WAVE) {  // wave-uniform trip
                                 const int64_t f = f0 + l;
