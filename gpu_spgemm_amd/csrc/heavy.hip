@@ -52,7 +52,8 @@ constexpr int RGW_ITEM_TILES = 4096;   // C tiles per work item (aim)
 constexpr int RGW_GMAX = 64;           // windows per work item (cap)
 constexpr int RGN_COLS = 4096;         // tile-column span of a numeric sub-window (LDS map)
 constexpr int RGN_BYTES = 36864;       // LDS bytes of a numeric sub-window: RGN_TILE_BYTES a tile + its values
-constexpr int RGN_CUR = 512;           // A entries of a tile-row whose B positions carry over sub-windows
+constexpr int RGN_CUR = 512;           // A entries of a tile-row whose B positions carry over sub-windows in LDS
+constexpr int64_t RGN_GCUR = 32768;    // and up to this many more, in a global area per workgroup
 constexpr int RGN_TILE_BYTES = 68;     // 16 packed row words + the tile's value offset
 constexpr int RGW_LONG = 64;           // B segments longer than this are walked by the workgroup
 constexpr int RGW_ILP = 4;             // A entries per thread per symbolic pass (independent loads)
@@ -334,8 +335,10 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
               const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC,
               const uint16_t *__restrict__ maskC, const uint8_t *__restrict__ ptrRowC,
               const int64_t *__restrict__ tile_nnzC, V *__restrict__ valC, uint8_t *__restrict__ colC,
-              int32_t *__restrict__ overflow, int64_t val_cap) {
+              int32_t *__restrict__ overflow, int64_t val_cap, int64_t *__restrict__ gcur, int64_t gcap) {
     constexpr int64_t H = rgw_item_values<V>();
+    // this workgroup's global cursor area: positions [0, gcap), row ends [gcap, 2 gcap)
+    int64_t *const gb = gcur + (int64_t)blockIdx.x * gcap * 2;
     __shared__ RgwNumLDS s;
     __shared__ int64_t s_rp[17];
     __shared__ RgwNumItem s_it;
@@ -431,14 +434,15 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                     // LDS and search only in the item's first sub-window.
                     const int64_t e = e0 + l;
                     const int64_t ci = e - s_rp[0];
-                    const bool cached = ci < RGN_CUR;
+                    const bool in_lds = ci < RGN_CUR;
+                    const bool cached = in_lds || ci - RGN_CUR < gcap;
                     int64_t lo = 0, hi = 0;
                     V a = V(0);
                     if (e < eb) {
                         a = valA[e];
                         if (cached && !first) {
-                            lo = s.cur[ci];
-                            hi = s.hi[ci];
+                            lo = in_lds ? s.cur[ci] : gb[ci - RGN_CUR];
+                            hi = in_lds ? s.hi[ci] : gb[gcap + ci - RGN_CUR];
                         } else {
                             const int32_t k = colA[e];
                             hi = rpB[k + 1];
@@ -540,8 +544,13 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                         }
                     }
                     if (e < eb && cached) {
-                        s.cur[ci] = next;
-                        s.hi[ci] = hi;
+                        if (in_lds) {
+                            s.cur[ci] = next;
+                            s.hi[ci] = hi;
+                        } else {
+                            gb[ci - RGN_CUR] = next;
+                            gb[gcap + ci - RGN_CUR] = hi;
+                        }
                     }
                 }
             }
@@ -588,9 +597,13 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
     return TSG_OK;
 }
 
-int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
-                       int64_t *woffn, const SlotLayout &sl) {
+static int64_t rgn_gcap(const tsg_tile_matrix *A) { return std::min<int64_t>(RGN_GCUR, A->nnz); }
+
+int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, const int32_t *rows,
+                       int64_t begin, int64_t count, int64_t *woffn, int64_t **gcur, const SlotLayout &sl) {
     if (count <= 0) return TSG_OK;
+    if (!*gcur && rgn_gcap(A) > 0)
+        TSG_CHECK(palloc(ctx, gcur, (size_t)rgw_grid(ctx, 3) * (size_t)rgn_gcap(A) * 2));
     const int64_t h = C->dtype == TSG_F32 ? rgw_item_values<float>() : rgw_item_values<double>();
     k_rgw_count_num<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, C->tile_ptr,
                                                                              sl.row_nc, C->tile_nnz, h, woffn);
@@ -600,20 +613,23 @@ int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *C, const int32_t *ro
 
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                       const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const int64_t *woffn,
-                      const SlotLayout &sl) {
+                      int64_t *gcur, const SlotLayout &sl) {
     if (count <= 0) return TSG_OK;
+    const int64_t gcap = gcur ? rgn_gcap(A) : 0;
     unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
     k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);
     if (A->dtype == TSG_F32)
         k_rgw_numeric<float><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
             rows, begin, count, woffn, ctr, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr,
             B->row_col, (const float *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
-            C->tile_csr_Ptr, C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
+            C->tile_csr_Ptr, C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap,
+            gcur, gcap);
     else
         k_rgw_numeric<double><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
             rows, begin, count, woffn, ctr, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr,
             B->row_col, (const double *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
-            C->tile_csr_Ptr, C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap);
+            C->tile_csr_Ptr, C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap,
+            gcur, gcap);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }

@@ -515,7 +515,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     sl.val_cap = vcap;
     if (!rc) rc = run_symbolic(ctx, A, B, &Cp, p, sl);
     if (!rc) rc = scan_inclusive_i64(ctx, pnnz, slots + 1);
-    if (!rc) rc = prepare_numeric(ctx, &Cp, p, sl);
+    if (!rc) rc = prepare_numeric(ctx, A, &Cp, p, sl);
     if (!rc && tilem > 0) {
         k_rownc_ptr<<<nblk(tilem, 256), 256, 0, s>>>(tilem, row_nc, C->tile_ptr);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_rownc_ptr launch");

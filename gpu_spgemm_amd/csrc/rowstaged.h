@@ -35,6 +35,7 @@ struct Plan {
     int64_t *lpay = nullptr;     // per A tile: B tile-row payload start
     int64_t *rgw = nullptr;      // row-by-row bin: symbolic work-item offsets + counters (tilem + 3)
     int64_t *rgn = nullptr;      // row-by-row bin: numeric work-item offsets (tilem + 1)
+    int64_t *rgc = nullptr;      // row-by-row bin: numeric cursor area (allocated with rgn's offsets)
     int64_t bin_off[MAX_BINS + 1] = {};
     int nbins = 0;
     int nvar = 0;     // staged bins; bin nvar = C-tile kernels
@@ -74,7 +75,7 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
 int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                 const Plan &p, const SlotLayout &sl);
 // Set-up of run_numeric that needs C's scanned value offsets (call once after the scan).
-int prepare_numeric(Context *ctx, const tsg_tile_matrix *C, const Plan &p, const SlotLayout &sl);
+int prepare_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, Plan &p, const SlotLayout &sl);
 void free_c(Context *ctx, tsg_tile_matrix *C);
 // keep != nullptr: the plan (row bins, links) is handed over instead of freed,
 // for re-running the numeric step.

@@ -714,7 +714,7 @@ BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask) {
 }
 
 void free_plan(Context *ctx, Plan &p) {
-    void *ps[] = {p.rows, p.bin, p.counts, p.hdr, p.overflow, p.link, p.lpay, p.rgw, p.rgn};
+    void *ps[] = {p.rows, p.bin, p.counts, p.hdr, p.overflow, p.link, p.lpay, p.rgw, p.rgn, p.rgc};
     for (void *q : ps) pool_free(ctx, q);
     p = Plan{};
 }
@@ -866,17 +866,18 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
     TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, p.bin_off[fb], p.bin_off[fb + 1] - p.bin_off[fb], sl));
     if (p.rg)
         TSG_CHECK(launch_rg_numeric(ctx, A, B, C, p.rows, p.bin_off[fb + 1], p.bin_off[fb + 2] - p.bin_off[fb + 1],
-                                    p.rgw, p.rgn, sl));
+                                    p.rgw, p.rgn, p.rgc, sl));
     return TSG_OK;
 }
 
 // Numeric set-up that needs C's scanned value offsets and must stay out of
 // the numeric launches (which a plan captures in a graph): the row-by-row
 // bin's work items.
-int prepare_numeric(Context *ctx, const tsg_tile_matrix *C, const Plan &p, const SlotLayout &sl) {
+int prepare_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, Plan &p, const SlotLayout &sl) {
     if (!p.rg) return TSG_OK;
     const int fb = p.nvar;
-    return prepare_rg_numeric(ctx, C, p.rows, p.bin_off[fb + 1], p.bin_off[fb + 2] - p.bin_off[fb + 1], p.rgn, sl);
+    return prepare_rg_numeric(ctx, A, C, p.rows, p.bin_off[fb + 1], p.bin_off[fb + 2] - p.bin_off[fb + 1], p.rgn,
+                              &p.rgc, sl);
 }
 
 void free_c(Context *ctx, tsg_tile_matrix *C) {
@@ -918,7 +919,7 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     exact.overflow = p.overflow;
     if (!rc) rc = run_symbolic(ctx, A, B, C, p, exact);
     if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1);
-    if (!rc) rc = prepare_numeric(ctx, C, p, exact);
+    if (!rc) rc = prepare_numeric(ctx, A, C, p, exact);
     if (!rc) {
         // nnzC and the capacity flag in one read
         if (hipMemcpyAsync(ctx->pinned, C->tile_nnz + C->numtile, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -135,11 +135,12 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
                        const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const SlotLayout &sl);
 // After C's value offsets are scanned: the rows' numeric work-item offsets
 // (count + 1 int64 in woffn).
-int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
-                       int64_t *woffn, const SlotLayout &sl);
+// *gcur: the numeric kernel's per-workgroup cursor area, allocated on first use.
+int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, const int32_t *rows,
+                       int64_t begin, int64_t count, int64_t *woffn, int64_t **gcur, const SlotLayout &sl);
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                       const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const int64_t *woffn,
-                      const SlotLayout &sl);
+                      int64_t *gcur, const SlotLayout &sl);
 int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B);  // api.hip
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                    tsg_tile_matrix *C, const tsg_options *opts);
