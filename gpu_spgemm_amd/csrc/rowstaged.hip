@@ -649,29 +649,47 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         }
         wave_sync();
         const int64_t obase = s.coff[ct0];
-        for (int o = lane; o < nout; o += WAVE) {
-            const int d = s.desc[o];
-            const int ct = d >> 8, r = (d >> 4) & 15, c = d & 15;
-            V acc = V(0);
-            const int pe = s.pstart[ct + 1];
-            for (int p = s.pstart[ct]; p < pe; p++) {
-                const int e = s.plist[p];
-                const int b = e & 0xFF, t = e >> 8;
-                const uint32_t ma = s.amask[t * 16 + r];
-                const uint32_t cm = s.bcmask[b * 16 + c];
-                uint32_t m = ma & cm;  // k of the products, ascending from the top bit
-                if (!m) continue;
-                const int ea = s.aoff[t] + s.aptr[t * 16 + r];
-                const V *bv = cvalB + (s.bgoff[b] + s.bcptr[b * 16 + c]);
-                do {
-                    const int kk = __clz(m) - 16;
-                    m ^= 0x8000u >> kk;
-                    const uint32_t up = below(kk);
-                    acc = fma_v<V>(s.aval[ea + __popc(ma & up)], bv[__popc(cm & up)], acc);
-                } while (m);
+        // results stay in registers until the chunk is done: the stores issue
+        // together after the last output (no store in flight while computing)
+        V res[DESC_CAP / WAVE];
+        uint8_t cb[DESC_CAP / WAVE];
+#pragma unroll
+        for (int it = 0; it < DESC_CAP / WAVE; it++) {
+            const int o = lane + it * WAVE;
+            res[it] = V(0);
+            cb[it] = 0;
+            if (it * WAVE < nout && o < nout) {
+                const int d = s.desc[o];
+                const int ct = d >> 8, r = (d >> 4) & 15, c = d & 15;
+                V acc = V(0);
+                const int pe = s.pstart[ct + 1];
+                for (int p = s.pstart[ct]; p < pe; p++) {
+                    const int e = s.plist[p];
+                    const int b = e & 0xFF, t = e >> 8;
+                    const uint32_t ma = s.amask[t * 16 + r];
+                    const uint32_t cm = s.bcmask[b * 16 + c];
+                    uint32_t m = ma & cm;  // k of the products, ascending from the top bit
+                    if (!m) continue;
+                    const int ea = s.aoff[t] + s.aptr[t * 16 + r];
+                    const V *bv = cvalB + (s.bgoff[b] + s.bcptr[b * 16 + c]);
+                    do {
+                        const int kk = __clz(m) - 16;
+                        m ^= 0x8000u >> kk;
+                        const uint32_t up = below(kk);
+                        acc = fma_v<V>(s.aval[ea + __popc(ma & up)], bv[__popc(cm & up)], acc);
+                    } while (m);
+                }
+                res[it] = acc;
+                cb[it] = (uint8_t)((r << 4) | c);
             }
-            valC[obase + o] = acc;
-            colC[obase + o] = (uint8_t)((r << 4) | c);
+        }
+#pragma unroll
+        for (int it = 0; it < DESC_CAP / WAVE; it++) {
+            const int o = lane + it * WAVE;
+            if (o < nout) {
+                valC[obase + o] = res[it];
+                colC[obase + o] = cb[it];
+            }
         }
         wave_sync();
         ct0 = ct1;

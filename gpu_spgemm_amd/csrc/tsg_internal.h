@@ -228,7 +228,8 @@ __device__ __forceinline__ int dpp_incl_max(int v) {
 // Value of lane 63 (a wave-uniform scalar).
 __device__ __forceinline__ int last_lane(int v) { return __builtin_amdgcn_readlane(v, WAVE - 1); }
 
-// 16-lane group inclusive sum (groups are aligned lane ranges of 16).
+// 16-lane group inclusive sum (groups are aligned lane ranges of 16, all
+// lanes of a group active).
 template <typename T>
 __device__ __forceinline__ T group16_incl_sum(T v) {
     int l = lane_id() & 15;
@@ -237,6 +238,16 @@ __device__ __forceinline__ T group16_incl_sum(T v) {
         T o = __shfl_up(v, d, 16);
         if (l >= d) v += o;
     }
+    return v;
+}
+// int: a 16-lane group is one DPP row, so row_shr 1, 2, 4, 8 (zero shifted in)
+// does it without LDS traffic.
+template <>
+__device__ __forceinline__ int group16_incl_sum<int>(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);
     return v;
 }
 
