@@ -41,7 +41,7 @@ using VarS = Variant<8, 16, 32, 128, 1 << 20, 4>;
 using VarW = Variant<32, 16, 32, 512, 1 << 20, 2>;
 using VarM = Variant<32, 48, 64, 512, 1 << 20, 2>;
 constexpr int NUM_STAGED = 3;
-constexpr int DESC_CAP = 512;  // output descriptors of one chunk of (C tile, row) items
+constexpr int DESC_CAP = 256;  // output descriptors of one chunk of (C tile, row) items
 
 template <typename Var>
 static StageCaps caps_of() {
@@ -630,8 +630,10 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
             int blk = (int)(s.coff[ct1 + span] - s.coff[ct1]);
             if (nout + blk > DESC_CAP) {
                 if (nout > 0) break;
-                span = 2;  // 2 tiles hold at most 512 = DESC_CAP outputs
-                blk = (int)(s.coff[ct1 + span] - s.coff[ct1]);
+                while (span > 1 && blk > DESC_CAP) {  // one tile holds at most 256 = DESC_CAP outputs
+                    span >>= 1;
+                    blk = (int)(s.coff[ct1 + span] - s.coff[ct1]);
+                }
             }
             const int item = ct1 * 16 + lane;
             const uint32_t mc = (lane < span * 16) ? s.cmask[item] : 0u;
