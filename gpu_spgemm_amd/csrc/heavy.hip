@@ -25,8 +25,8 @@
 //     workgroup.
 //   k_rgw_numeric: a sub-window of C tiles (as many as fit RGN_BYTES of LDS
 //     with their values, within RGN_COLS tile columns): the values, a map tile
-//     column -> sub-window tile, and each tile's packed (row offset, row mask)
-//     words.  The waves
+//     column -> sub-window tile, and each tile's row masks, row offsets and
+//     value offset (copied from C's arrays).  The waves
 //     take the tile-row's 16 scalar rows in turn; for its row a wave loads 64
 //     A entries at a time (a lane per entry: the entry's B segment in the
 //     sub-window, its first products' B values and output slots), then adds
@@ -54,12 +54,13 @@ constexpr int RGN_COLS = 4096;         // tile-column span of a numeric sub-wind
 constexpr int RGN_BYTES = 36864;       // LDS bytes of a numeric sub-window: RGN_TILE_BYTES a tile + its values
 constexpr int RGN_CUR = 512;           // A entries of a tile-row whose B positions carry over sub-windows in LDS
 constexpr int64_t RGN_GCUR = 32768;    // and up to this many more, in a global area per workgroup
-constexpr int RGN_TILE_BYTES = 68;     // 16 packed row words + the tile's value offset
+constexpr int RGN_TILE_BYTES = 52;     // 16 row masks, 16 row offsets and the tile's value offset
 constexpr int RGW_LONG = 64;           // B segments longer than this are walked by the workgroup
 constexpr int RGW_ILP = 4;             // A entries per thread per symbolic pass (independent loads)
 constexpr int RGW_PF = 16;             // B entries loaded at once by one symbolic thread
 constexpr int RGW_PF0 = 4;             // of which the first stage
 constexpr int RGN_PF = 8;              // B entries a numeric lane stages for its A entry (VGPRs)
+constexpr int RGN_FILL = (RGN_BYTES / RGN_TILE_BYTES + RGW_THREADS - 1) / RGW_THREADS;  // sub-window tiles per thread
 
 __device__ __forceinline__ uint32_t rgw_below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
 
@@ -276,9 +277,9 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
     }
 }
 
-// A sub-window's LDS: nt tiles' packed row words tw[nt * 16] ((row offset <<
-// 16) | row mask), their value offsets tb[nt], then the values, in buf; and
-// the map tile column - c0 -> sub-window tile (0xFFFF: none).
+// A sub-window's LDS: nt tiles' row masks msk[nt * 16], row offsets
+// prw[nt * 16] and value offsets tbs[nt], then the values, in buf; and the map
+// tile column - c0 -> sub-window tile (0xFFFF: none).
 struct RgwNumLDS {
     alignas(16) unsigned char buf[RGN_BYTES];
     int64_t cur[RGN_CUR];  // per A entry (first RGN_CUR of the tile-row): next B position
@@ -312,17 +313,16 @@ struct RgwNumItem {
 };
 
 // Output slot of product column j for scalar row r in the staged sub-window, or -1.
-__device__ __forceinline__ int rgw_slot(const uint16_t *map, const uint32_t *tws, const int32_t *tbs, int32_t j,
-                                        int32_t c0, int nt, int nv, int r) {
+__device__ __forceinline__ int rgw_slot(const uint16_t *map, const uint16_t *msk, const uint8_t *prw,
+                                        const int32_t *tbs, int32_t j, int32_t c0, int nt, int nv, int r) {
     const int tc = (j >> 4) - c0;
     if ((unsigned)tc >= (unsigned)RGN_COLS) return -1;
     const int lt = map[tc];
     if (lt >= nt) return -1;
-    const uint32_t tw = tws[lt * 16 + r];
+    const uint32_t mrow = msk[lt * 16 + r];
     const int c = j & 15;
-    const uint32_t mrow = tw & 0xFFFFu;
     if (!(mrow & (0x8000u >> c))) return -1;
-    const int off = tbs[lt] + (int)(tw >> 16) + __popc(mrow & rgw_below(c));
+    const int off = tbs[lt] + (int)prw[lt * 16 + r] + __popc(mrow & rgw_below(c));
     return off < nv ? off : -1;
 }
 
@@ -381,35 +381,67 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
         const int64_t tb = s_it.tb;
         for (int64_t sp0 = s_it.ta; sp0 < tb;) {  // uniform sub-window loop
             const bool first = sp0 == s_it.ta;
-            if (tid == 0) {  // largest end whose tiles and values fit RGN_BYTES, within RGN_COLS columns
-                const int64_t v0 = tile_nnzC[sp0];
-                int64_t lo = sp0 + 1, hi = min<int64_t>(tb, sp0 + RGN_BYTES / RGN_TILE_BYTES);
-                while (lo < hi) {
-                    const int64_t mid = (lo + hi + 1) >> 1;
-                    if (rgn_val_offset((int)(mid - sp0)) + (tile_nnzC[mid] - v0) * (int64_t)sizeof(V) <= RGN_BYTES)
-                        lo = mid;
-                    else
-                        hi = mid - 1;
-                }
-                s_sp1 = lower_bound<int64_t, int32_t>(colidxC, sp0 + 1, lo, colidxC[sp0] + RGN_COLS);
+            if (tid == 0) {
+                s_sp1 = sp0 + 1;
                 s_row = 0;
             }
             for (int q = tid; q < RGN_COLS; q += RGW_THREADS) s.map[q] = 0xFFFF;
             __syncthreads();
+            {  // the sub-window ends at the largest end whose tiles [sp0, end) and their values fit
+               // RGN_BYTES and span fewer than RGN_COLS tile columns (both tests are monotone in
+               // end): every thread tests a few candidate ends at once, one coalesced load each
+                const int64_t v0 = tile_nnzC[sp0];
+                const int32_t cl = colidxC[sp0] + RGN_COLS;
+                const int64_t hmax = min<int64_t>(tb, sp0 + RGN_BYTES / RGN_TILE_BYTES);
+                int64_t best = sp0 + 1;
+                for (int64_t end = sp0 + 2 + tid; end <= hmax; end += RGW_THREADS) {
+                    if (rgn_val_offset((int)(end - sp0)) + (tile_nnzC[end] - v0) * (int64_t)sizeof(V) > RGN_BYTES ||
+                        colidxC[end - 1] >= cl)
+                        break;
+                    best = end;
+                }
+                if (best > sp0 + 1)
+                    atomicMax(reinterpret_cast<unsigned long long *>(&s_sp1), (unsigned long long)best);
+            }
+            __syncthreads();
             const int64_t sp1 = s_sp1;
             const int nt = (int)(sp1 - sp0);
-            uint32_t *const tws = reinterpret_cast<uint32_t *>(s.buf);
-            int32_t *const tbs = reinterpret_cast<int32_t *>(s.buf + nt * 64);
+            uint16_t *const msk = reinterpret_cast<uint16_t *>(s.buf);  // row masks of tile q: [16 q, 16 q + 16)
+            uint8_t *const prw = s.buf + nt * 32;                          // row offsets within the tile
+            int32_t *const tbs = reinterpret_cast<int32_t *>(s.buf + nt * 48);  // tile value offsets
             V *const vals = reinterpret_cast<V *>(s.buf + rgn_val_offset(nt));
             const int64_t vbase = tile_nnzC[sp0];
             const int nv = (int)min<int64_t>(tile_nnzC[sp1] - vbase, (RGN_BYTES - rgn_val_offset(nt)) / (int)sizeof(V));
             const int32_t c0 = colidxC[sp0];
-            for (int q = tid; q < nt; q += RGW_THREADS) {
-                s.map[colidxC[sp0 + q] - c0] = (uint16_t)q;
-                tbs[q] = (int32_t)(tile_nnzC[sp0 + q] - vbase);
+            {  // a thread per tile, all of its loads issued before the first store: the tile's
+               // masks and row offsets copied as they are, its value offset and map entry
+                uint4 m0[RGN_FILL], m1[RGN_FILL], pr[RGN_FILL];
+                int32_t cc[RGN_FILL];
+                int64_t tn[RGN_FILL];
+#pragma unroll
+                for (int f = 0; f < RGN_FILL; f++) {
+                    const int q = tid + f * RGW_THREADS;
+                    if (q < nt) {
+                        const int64_t t = sp0 + q;
+                        m0[f] = reinterpret_cast<const uint4 *>(maskC)[t * 2];
+                        m1[f] = reinterpret_cast<const uint4 *>(maskC)[t * 2 + 1];
+                        pr[f] = reinterpret_cast<const uint4 *>(ptrRowC)[t];
+                        cc[f] = colidxC[t];
+                        tn[f] = tile_nnzC[t];
+                    }
+                }
+#pragma unroll
+                for (int f = 0; f < RGN_FILL; f++) {
+                    const int q = tid + f * RGW_THREADS;
+                    if (q < nt) {
+                        reinterpret_cast<uint4 *>(msk)[q * 2] = m0[f];
+                        reinterpret_cast<uint4 *>(msk)[q * 2 + 1] = m1[f];
+                        reinterpret_cast<uint4 *>(prw)[q] = pr[f];
+                        tbs[q] = (int32_t)(tn[f] - vbase);
+                        s.map[cc[f] - c0] = (uint16_t)q;
+                    }
+                }
             }
-            for (int q = tid; q < nt * 16; q += RGW_THREADS)
-                tws[q] = ((uint32_t)ptrRowC[sp0 * 16 + q] << 16) | maskC[sp0 * 16 + q];
             for (int q = tid; q < nv; q += RGW_THREADS) vals[q] = V(0);
             __syncthreads();
             const int32_t lo_col = c0 * 16;
@@ -457,7 +489,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                     for (int t = 0; t < RGN_PF; t++) sl[t] = -1;
                     if (lo < hi) {
                         int32_t jj[RGN_PF], tc[RGN_PF], lt[RGN_PF], tbo[RGN_PF];
-                        uint32_t tw[RGN_PF];
+                        uint32_t mr[RGN_PF], po[RGN_PF];
                         // two stages: the first RGW_PF0 entries, the rest only while the
                         // segment stays in the sub-window (no load for lanes past the row)
 #pragma unroll
@@ -496,14 +528,15 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                         }
 #pragma unroll
                         for (int t = 0; t < RGN_PF; t++) {
-                            tw[t] = tws[lt[t] * 16 + r];
+                            mr[t] = msk[lt[t] * 16 + r];
+                            po[t] = prw[lt[t] * 16 + r];
                             tbo[t] = tbs[lt[t]];
                         }
 #pragma unroll
                         for (int t = 0; t < RGN_PF; t++) {
                             const int c = jj[t] & 15;
-                            const uint32_t mrow = tw[t] & 0xFFFFu;
-                            const int off = tbo[t] + (int)(tw[t] >> 16) + __popc(mrow & rgw_below(c));
+                            const uint32_t mrow = mr[t];
+                            const int off = tbo[t] + (int)po[t] + __popc(mrow & rgw_below(c));
                             sl[t] = ((ok >> t) & 1u) && (mrow & (0x8000u >> c)) && off < nv ? off : -1;
                         }
                         any = jj[0] < hi_col;
@@ -532,7 +565,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                                 const int64_t f = f0 + l;
                                 const int32_t j = f < shi ? colB[f] : INT_MAX;
                                 const V b = f < shi ? valB[f] : V(0);
-                                const int off = j < hi_col ? rgw_slot(s.map, tws, tbs, j, c0, nt, nv, r) : -1;
+                                const int off = j < hi_col ? rgw_slot(s.map, msk, prw, tbs, j, c0, nt, nv, r) : -1;
                                 if (off >= 0) vals[off] = rgw_fma<V>(sa, b, vals[off]);
                                 const uint64_t past = __ballot(j >= hi_col);
                                 if (past) {  // the segment left the sub-window
@@ -555,19 +588,16 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                 }
             }
             __syncthreads();
-            // values and col bytes of the sub-window (16 threads per tile)
+            // values of the sub-window (contiguous), then its col bytes (16 threads per tile)
+            for (int q = tid; q < nv; q += RGW_THREADS) valC[vbase + q] = vals[q];
             for (int q = tid; q < nt * 16; q += RGW_THREADS) {
                 const int lt = q >> 4, rr = q & 15;
-                const uint32_t tw = tws[q];
-                uint32_t mrow = tw & 0xFFFFu;
-                int o = tbs[lt] + (int)(tw >> 16);
+                uint32_t mrow = msk[q];
+                int o = tbs[lt] + (int)prw[q];
                 while (mrow) {
                     const int c = __clz(mrow) - 16;
                     mrow ^= 0x8000u >> c;
-                    if (o < nv) {
-                        valC[vbase + o] = vals[o];
-                        colC[vbase + o] = (uint8_t)((rr << 4) | c);
-                    }
+                    if (o < nv) colC[vbase + o] = (uint8_t)((rr << 4) | c);
                     o++;
                 }
             }
