@@ -62,6 +62,8 @@ constexpr int RGW_PF0 = 4;             // of which the first stage
 constexpr int RGN_PF = 8;              // B entries a numeric lane stages for its A entry (VGPRs)
 constexpr int RGN_FILL = (RGN_BYTES / RGN_TILE_BYTES + RGW_THREADS - 1) / RGW_THREADS;  // sub-window tiles per thread
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // 16-byte piece
+
 __device__ __forceinline__ uint32_t rgw_below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
 
 template <typename V>
@@ -413,30 +415,29 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
             const int64_t vbase = tile_nnzC[sp0];
             const int nv = (int)min<int64_t>(tile_nnzC[sp1] - vbase, (RGN_BYTES - rgn_val_offset(nt)) / (int)sizeof(V));
             const int32_t c0 = colidxC[sp0];
-            {  // a thread per tile, all of its loads issued before the first store: the tile's
-               // masks and row offsets copied as they are, its value offset and map entry
-                uint4 m0[RGN_FILL], m1[RGN_FILL], pr[RGN_FILL];
+            {  // the tiles' masks and row offsets copied as they are, 16 bytes a lane, by
+               // loads that write LDS directly: lane i of a wave writes the wave's base
+               // + 16 i, the base being piece q - l, the same for the whole wave
+                const u32x4 *const gm = reinterpret_cast<const u32x4 *>(maskC) + sp0 * 2;
+                const u32x4 *const gp = reinterpret_cast<const u32x4 *>(ptrRowC) + sp0;
+                for (int q = tid; q < nt * 2; q += RGW_THREADS)
+                    __builtin_amdgcn_global_load_lds(gm + q, reinterpret_cast<u32x4 *>(msk) + (q - l), 16, 0, 0);
+                for (int q = tid; q < nt; q += RGW_THREADS)
+                    __builtin_amdgcn_global_load_lds(gp + q, reinterpret_cast<u32x4 *>(prw) + (q - l), 16, 0, 0);
                 int32_t cc[RGN_FILL];
                 int64_t tn[RGN_FILL];
 #pragma unroll
                 for (int f = 0; f < RGN_FILL; f++) {
                     const int q = tid + f * RGW_THREADS;
                     if (q < nt) {
-                        const int64_t t = sp0 + q;
-                        m0[f] = reinterpret_cast<const uint4 *>(maskC)[t * 2];
-                        m1[f] = reinterpret_cast<const uint4 *>(maskC)[t * 2 + 1];
-                        pr[f] = reinterpret_cast<const uint4 *>(ptrRowC)[t];
-                        cc[f] = colidxC[t];
-                        tn[f] = tile_nnzC[t];
+                        cc[f] = colidxC[sp0 + q];
+                        tn[f] = tile_nnzC[sp0 + q];
                     }
                 }
 #pragma unroll
                 for (int f = 0; f < RGN_FILL; f++) {
                     const int q = tid + f * RGW_THREADS;
                     if (q < nt) {
-                        reinterpret_cast<uint4 *>(msk)[q * 2] = m0[f];
-                        reinterpret_cast<uint4 *>(msk)[q * 2 + 1] = m1[f];
-                        reinterpret_cast<uint4 *>(prw)[q] = pr[f];
                         tbs[q] = (int32_t)(tn[f] - vbase);
                         s.map[cc[f] - c0] = (uint16_t)q;
                     }
