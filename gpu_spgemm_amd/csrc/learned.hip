@@ -19,11 +19,6 @@
 //   4. one host read at the end returns (overflow, numtileC, nnzC); on
 //      overflow the exact binned pipeline runs instead, so the result never
 //      depends on the model.
-// When the prediction puts every row in one staged variant (stencil-like
-// products), steps 2-3 are one kernel instead (k_fused_staged): each
-// workgroup of rows computes structure and values, publishes its tile and
-// value counts, finds its place in C by a decoupled look-back over the
-// workgroups before it and writes C in the compact layout directly.
 // The network is trained on the GPU by the kernels below (forward, backward
 // with per-block partial gradients reduced in a fixed order, Adam), on
 // (features -> exact tile count, exact nnz) pairs that the exact pipeline
@@ -58,7 +53,7 @@ struct Learned {
         int64_t annz = -1, bnnz = -1, atiles = -1, btiles = -1, tilem = -1, tilen = -1;
         int64_t slots = 0, valcap = 0;
         int64_t bin_off[MAX_BINS + 1] = {};
-        int nbins = 0, fused = -1;
+        int nbins = 0;
         bool valid = false;
     } cache;
 };
@@ -234,8 +229,7 @@ k_learned_plan(int32_t tilem, const int64_t *__restrict__ tile_ptrA, const int64
                const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, const float *__restrict__ wg,
                BinTable bt, float off_nc, float off_nnz, float scale, int extra, float *__restrict__ Xout,
                float *__restrict__ Yout,
-               int64_t *__restrict__ slots, int64_t *__restrict__ valcap, uint8_t *__restrict__ bin,
-               FusedTable ft, int64_t *__restrict__ fused) {
+               int64_t *__restrict__ slots, int64_t *__restrict__ valcap, uint8_t *__restrict__ bin) {
     __shared__ float w[NP];
     if (wg) {
         for (int i = threadIdx.x; i < NP; i += 256) w[i] = wg[i];
@@ -296,16 +290,6 @@ k_learned_plan(int32_t tilem, const int64_t *__restrict__ tile_ptrA, const int64
     }
     if (b == bt.nvar && bt.rg && pa <= (int64_t)bt.rg_density * na) b = bt.nvar + 1;
     bin[bi] = (uint8_t)b;
-    if (fused) {  // the smallest one-pass variant that holds the row by the plain prediction (+1/8)
-        const float rnc = fmaxf(expm1f(fminf(y[0], 40.0f)), 0.0f) * scale * 1.125f;
-        int f = ft.n;
-        for (int v = 0; v < ft.n; v++) {
-            const StageCaps c = ft.v[v];
-            if (na <= c.na && rnc <= (float)c.nc && nb <= c.nb && pa <= c.pa && pb <= c.pb) { f = v; break; }
-        }
-        if (bt.nvar == 0) f = ft.n;  // staged kernels not usable for these operands
-        atomicMax(reinterpret_cast<unsigned long long *>(fused), (unsigned long long)f);
-    }
 }
 
 // Training targets from an exact C: tiles and nonzeros of every tile-row.
@@ -383,19 +367,6 @@ __global__ void k_learned_result(const int32_t *__restrict__ overflow, const int
     hdr[2] = pnnz[slotp[tilem]];
 }
 
-constexpr int32_t FUSED_MAX_TILEM = 1 << 17;
-
-// hdr = (flags, numtileC, nnzC) of a one-pass product (sizes only when no flag is set).
-__global__ void k_fused_result(const int32_t *__restrict__ flags, const int64_t *__restrict__ tile_ptr, int32_t tilem,
-                               const int64_t *__restrict__ tile_nnz, int64_t tcap, int64_t *__restrict__ hdr) {
-    if (threadIdx.x != 0) return;
-    const int64_t f = flags[0] | flags[1];
-    const int64_t nt = f ? 0 : tile_ptr[tilem];
-    hdr[0] = f;
-    hdr[1] = nt;
-    hdr[2] = (f || nt > tcap) ? 0 : tile_nnz[nt];
-}
-
 static unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
 // ---------------------------------------------------------------- the learned product
@@ -437,12 +408,10 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     if (!rc) rc = palloc(ctx, &valc, (size_t)tilem + 1);
     if (!rc) rc = palloc(ctx, &expect, MAX_BINS + 3);
     if (!rc) rc = palloc(ctx, &res, 4);
-    if (!rc && hipMemsetAsync(p.hdr + HDR_FUSED, 0, sizeof(int64_t), s) != hipSuccess) rc = fail(TSG_ERR_HIP, "memset");
     if (!rc && tilem > 0) {
         k_learned_plan<<<nblk(tilem, 256), 256, 0, s>>>(tilem, A->tile_ptr, A->tile_nnz, p.link, B->tile_columnidx,
                                                          L->w, bt, L->off_nc, L->off_nnz, cscale, cextra, nullptr,
-                                                         nullptr, slotp, valc, p.bin, fused_table(),
-                                                         p.hdr + HDR_FUSED);
+                                                         nullptr, slotp, valc, p.bin);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_learned_plan launch");
         if (!rc) rc = scan_inclusive_i64(ctx, slotp, (int64_t)tilem + 1);
         if (!rc) rc = scan_inclusive_i64(ctx, valc, (int64_t)tilem + 1);
@@ -463,7 +432,6 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
             slots = cc.slots;
             vcap = cc.valcap;
             for (int b = 0; b <= p.nbins; b++) p.bin_off[b] = cc.bin_off[b];
-            p.fused = cc.fused;
         } else {
             rc = plan_read(ctx, p, &slots, &vcap);
             syncs++;
@@ -472,7 +440,6 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
                 cc.tilem = A->tilem; cc.tilen = B->tilen;
                 cc.atiles = A->numtile; cc.btiles = B->numtile; cc.slots = slots; cc.valcap = vcap;
                 cc.nbins = p.nbins;
-                cc.fused = p.fused;
                 for (int b = 0; b <= p.nbins; b++) cc.bin_off[b] = p.bin_off[b];
                 cc.valid = true;
             }
@@ -515,89 +482,6 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
                 for (auto &e : ev) (void)hipEventDestroy(e);
             return TSG_OK;
         }
-    }
-    // One pass: when every row fits one staged variant by the plain
-    // prediction, a single kernel writes C's structure and values in the
-    // compact layout (each row placed by a look-back over the rows before it)
-    // into C arrays of the calibrated totals: slots tiles, vcap values.  It
-    // runs at 6 waves per SIMD against the separate kernels' 8, so it is used
-    // up to FUSED_MAX_TILEM tile-rows (MI355X, 5-point Laplacian: 0.35 ms
-    // against 0.40 ms for the separate steps at 65,536 tile-rows; 1.30 ms
-    // against 0.88 ms at 262,144).
-    if (!rc && p.fused >= 0 && p.fused < fused_table().n && tilem > 0 && tilem <= FUSED_MAX_TILEM) {
-        uint64_t *lb = nullptr;
-        unsigned *tk = nullptr;
-        int32_t *fl = nullptr;
-        rc = palloc(ctx, &lb, (size_t)fused_status_words(tilem));
-        if (!rc) rc = palloc(ctx, &tk, 1);
-        if (!rc) rc = palloc(ctx, &fl, 2);
-        if (!rc) rc = palloc(ctx, &C->tile_ptr, (size_t)tilem + 1);
-        if (!rc) rc = palloc(ctx, &C->tile_rowidx, (size_t)slots);
-        if (!rc) rc = palloc(ctx, &C->tile_columnidx, (size_t)slots);
-        if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)slots + 1);
-        if (!rc) rc = palloc(ctx, &C->mask, (size_t)slots * 16);
-        if (!rc) rc = palloc(ctx, &C->tile_csr_Ptr, (size_t)slots * 16);
-        if (!rc) rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)vcap * vsize(C->dtype));
-        if (!rc) rc = palloc(ctx, &C->tile_csr_Col, (size_t)vcap);
-        if (!rc && (hipMemsetAsync(tk, 0, sizeof(unsigned), s) != hipSuccess ||
-                    hipMemsetAsync(fl, 0, 2 * sizeof(int32_t), s) != hipSuccess))
-            rc = fail(TSG_ERR_HIP, "memset");
-        if (timing && !rc) (void)hipEventRecord(ev[1], s);
-        if (timing && !rc) (void)hipEventRecord(ev[2], s);
-        if (!rc) rc = launch_fused(ctx, p.fused, A, B, C, p, slots, vcap, lb, tk, fl);
-        if (timing && !rc) (void)hipEventRecord(ev[3], s);
-        bool redo = false;
-        if (!rc) {
-            k_fused_result<<<1, 64, 0, s>>>(fl, C->tile_ptr, tilem, C->tile_nnz, slots, res);
-            if (hipGetLastError() != hipSuccess ||
-                hipMemcpyAsync(ctx->pinned, res, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess)
-                rc = fail(TSG_ERR_HIP, "reading the one-pass result failed");
-            syncs++;
-            if (!rc) {
-                redo = ctx->pinned[0] != 0;
-                if (redo) cc.valid = false;
-                C->numtile = ctx->pinned[1];
-                C->nnz = ctx->pinned[2];
-            }
-        }
-        if (timing) {
-            if (!rc && !redo && st) {
-                float a = 0, c3 = 0;
-                (void)hipEventElapsedTime(&a, ev[0], ev[1]);
-                (void)hipEventElapsedTime(&c3, ev[2], ev[3]);
-                st->ms_step1 = a; st->ms_step2 = 0.0f; st->ms_step3 = c3; st->ms_total = a + c3;
-            }
-            for (auto &e : ev) (void)hipEventDestroy(e);
-        }
-        pool_free(ctx, lb);
-        pool_free(ctx, tk);
-        pool_free(ctx, fl);
-        cleanup();
-        if (rc || redo) free_c(ctx, C);
-        if (rc) return rc;
-        if (redo) {  // a row outgrew its variant or C's predicted totals: exact pipeline, same result
-            tsg_options o = opts ? *opts : tsg_options{};
-            o.scheduler = TSG_SCHED_PLAIN;
-            TSG_CHECK(spgemm_binned(ctx, A, B, C, &o, st));
-            if (st) {
-                st->fallback_used = 1;
-                st->host_syncs += syncs;
-                st->pred_numtileC = slots;
-                st->pred_nnzC = vcap;
-            }
-            return TSG_OK;
-        }
-        if (st) {
-            st->numtileC = C->numtile;
-            st->nnzC = C->nnz;
-            st->host_syncs = syncs;
-            st->fallback_used = 0;
-            st->pred_numtileC = slots;
-            st->pred_nnzC = vcap;
-            st->one_pass = 1;
-        }
-        return TSG_OK;
     }
     // slot-layout structure and the compact C
     if (!rc) rc = palloc(ctx, &row_nc, (size_t)tilem);
@@ -808,7 +692,7 @@ static int features_or_predict(tsg_context *c, const tsg_tile_matrix *A, const t
     if (!rc && tilem > 0) {
         k_learned_plan<<<nblk(tilem, 256), 256, 0, ctx->stream>>>(
             tilem, A->tile_ptr, A->tile_nnz, p.link, B->tile_columnidx, pred ? L->w : nullptr,
-            bt, L->off_nc, L->off_nnz, 1.0f, 1, X, Y, slotp, valc, p.bin, FusedTable{}, nullptr);
+            bt, L->off_nc, L->off_nnz, 1.0f, 1, X, Y, slotp, valc, p.bin);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_learned_plan launch");
     }
     if (!rc && feats && tilem > 0 &&
@@ -853,8 +737,7 @@ int tsg_learned_collect(tsg_context *c, const tsg_tile_matrix *A, const tsg_tile
     if (!rc) {
         k_learned_plan<<<nblk(tilem, 256), 256, 0, ctx->stream>>>(tilem, A->tile_ptr, A->tile_nnz, p.link,
                                                                    B->tile_columnidx, nullptr, BinTable{}, 0.0f, 0.0f,
-                                                                   1.0f, 0, X, nullptr, slotp, valc, p.bin,
-                                                                   FusedTable{}, nullptr);
+                                                                   1.0f, 0, X, nullptr, slotp, valc, p.bin);
         k_targets<<<nblk(tilem, 256), 256, 0, ctx->stream>>>(tilem, C.tile_ptr, C.tile_nnz, T);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "feature/target launch");
     }

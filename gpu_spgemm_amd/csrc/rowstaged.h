@@ -23,14 +23,13 @@ struct BinTable {
     int32_t rg_density;  // A entries per A tile at or below which a heavy row goes row-by-row
 };
 constexpr int RG_DENSITY_DEFAULT = 64;
-constexpr int HDR_FUSED = MAX_BINS + 3;  // plan header slot of the one-pass variant (hdr holds HDR_FUSED + 1)
 
 // Per-call working set: A->B links, per-row bins, bin-ordered row lists.
 struct Plan {
     int32_t *rows = nullptr;     // row ids grouped by bin, ascending in each bin
     uint8_t *bin = nullptr;      // bin of each tile-row
     int64_t *counts = nullptr;   // (bin, block) counts, scanned
-    int64_t *hdr = nullptr;      // [total0, total1, bin offsets...] for one host read; [HDR_FUSED]: fused variant
+    int64_t *hdr = nullptr;      // [total0, total1, bin offsets...] for one host read
     int32_t *overflow = nullptr;
     int4 *link = nullptr;        // per A tile: B tile-row start, tiles, payload count
     int64_t *lpay = nullptr;     // per A tile: B tile-row payload start
@@ -39,7 +38,6 @@ struct Plan {
     int64_t *rgc = nullptr;      // row-by-row bin: numeric cursor area (allocated with rgn's offsets)
     int64_t bin_off[MAX_BINS + 1] = {};
     int nbins = 0;
-    int fused = -1;   // one-pass variant that holds every row (learned sizes); fused_table().n = none
     int nvar = 0;     // staged bins; bin nvar = C-tile kernels
     bool rg = false;  // bin nvar + 1 = row-by-row kernels
 };
@@ -79,20 +77,6 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
 // Set-up of run_numeric that needs C's scanned value offsets (call once after the scan).
 int prepare_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, Plan &p, const SlotLayout &sl);
 void free_c(Context *ctx, tsg_tile_matrix *C);
-// One-pass product of a plan whose rows all fit staged variant `var`: C's
-// arrays hold tcap tiles and vcap values; st holds fused_status_words(tilem)
-// look-back words (set by the launch), *ticket and flags[0..1] start at 0;
-// flags[0] / flags[1] report a row that did not fit or a look-back that gave
-// up (C is then to be discarded).
-int64_t fused_status_words(int32_t tilem);
-// Capacities of the one-pass variants (`var` of launch_fused indexes v).
-struct FusedTable {
-    StageCaps v[2];
-    int32_t n;
-};
-FusedTable fused_table();
-int launch_fused(Context *ctx, int var, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                 const Plan &p, int64_t tcap, int64_t vcap, uint64_t *st, unsigned *ticket, int32_t *flags);
 // keep != nullptr: the plan (row bins, links) is handed over instead of freed,
 // for re-running the numeric step.
 int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,

@@ -41,10 +41,6 @@ using VarS = Variant<8, 16, 32, 128, 1 << 20, 4>;
 using VarW = Variant<32, 16, 32, 512, 1 << 20, 2>;
 using VarM = Variant<32, 48, 64, 512, 1 << 20, 2>;
 constexpr int NUM_STAGED = 3;
-// One-pass variants (learned sizes): S holds twice VarS's C tiles, since the
-// variant is picked from a predicted tile count; M is VarM.
-using FusS = Variant<8, 32, 32, 128, 1 << 20, 4>;
-using FusM = VarM;
 constexpr int DESC_CAP = 256;  // output descriptors of one chunk of (C tile, row) items
 
 template <typename Var>
@@ -702,457 +698,6 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     }
 }
 
-// ---------------------------------------------------------------- one-pass staged product
-// Used when predicted sizes let C be allocated before its structure is known:
-// one wave per tile-row does the symbolic and the numeric step together, and
-// the row's first C tile and first value come from a decoupled look-back over
-// the rows before it.  Status of a row (one word for its C tiles, one for its
-// values): bits 63-62 = 1: the row's own count, 2: the inclusive prefix over
-// rows 0..row, 0: not known yet; bits 61-0 the number.
-constexpr uint64_t LB_AGG = 1ull << 62, LB_PRE = 2ull << 62, LB_VAL = (1ull << 62) - 1;
-constexpr int LB_SPIN = 1 << 16;  // status reads before a wait gives up (the result is then discarded)
-
-__device__ __forceinline__ uint64_t lb_load(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void lb_store(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Exclusive prefix of `row` from the statuses st[0, row): back to the
-// nearest inclusive prefix, adding the rows' own counts on the way.  Lane l
-// reads LB_LANE consecutive rows (the window's end - 1 - l LB_LANE downwards),
-// so one step covers WAVE LB_LANE rows: with thousands of rows in flight the
-// nearest prefix is a step or two away.  st[-LB_PAD, 0) hold a prefix of 0, so
-// a step that reaches before row 0 reads no further.  Rows are taken in ticket
-// order, so every row waited for belongs to a wave that is already running; a
-// wait longer than LB_SPIN steps sets *err and counts unknown rows as 0.
-constexpr int LB_LANE = 16;
-constexpr int LB_PAD = WAVE * LB_LANE;
-
-__device__ int64_t lookback(const uint64_t *st, int64_t row, int32_t *err) {
-    const int lane = lane_id();
-    int64_t acc = 0, base = row;
-    int spins = 0;
-    for (;;) {  // wave-uniform
-        const uint64_t *p = st + (base - 1 - (int64_t)lane * LB_LANE);  // this lane's nearest row
-        uint64_t w[LB_LANE];
-#pragma unroll
-        for (int i = 0; i < LB_LANE; i++) w[i] = lb_load(p - i);
-        bool pre = false, miss = false;
-        int64_t sum = 0;
-#pragma unroll
-        for (int i = 0; i < LB_LANE; i++) {  // up to and including the nearest prefix
-            if (!pre) {
-                const uint64_t f = w[i] >> 62;
-                miss = miss || f == 0;
-                sum += (int64_t)(w[i] & LB_VAL);
-                pre = f == 2;
-            }
-        }
-        const uint64_t pm = __ballot(pre);
-        const int fl = pm ? __ffsll((unsigned long long)pm) - 1 : WAVE;  // nearest lane with a prefix
-        const uint64_t need = fl < WAVE ? ((2ull << fl) - 1ull) : ~0ull;
-        if (__ballot(miss) & need) {
-            if (++spins < LB_SPIN) {  // back off: 1, 2, 4, .. 64 sleeps between reads
-                for (int z = 0; z < (1 << min(spins - 1, 6)); z++) __builtin_amdgcn_s_sleep(2);
-                continue;
-            }
-            if (lane == 0) atomicOr(err, 1);
-        }
-        acc += wave_sum<int64_t>(((need >> lane) & 1ull) ? sum : 0);
-        if (pm) return acc;
-        base -= LB_PAD;
-    }
-}
-
-// Two status arrays of n rows each, laid out [LB_PAD][n][LB_PAD][n]: the
-// padding holds a prefix of 0, the rows 0 (unknown).
-__global__ void __launch_bounds__(256) k_lb_init(uint64_t *__restrict__ st, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= 2 * (LB_PAD + n)) return;
-    const int64_t r = i < LB_PAD + n ? i : i - (LB_PAD + n);
-    st[i] = r < LB_PAD ? LB_PRE : 0ull;
-}
-
-template <typename V, typename Var>
-struct FusedLDS {
-    alignas(16) uint16_t amask[Var::NA * 16];
-    uint32_t cmask[Var::NC * 16];  // C row masks (OR of B rows), 32-bit for LDS atomics
-    int32_t abs_[Var::NA];
-    int32_t aseg[Var::NA + 1];
-    int32_t cj[Var::NC];        // C tile columns, ascending
-    int32_t coff[Var::NC + 1];  // first value of each C tile within the row
-    uint8_t bown[Var::NB];
-    uint8_t bct[Var::NB];  // C tile of each B tile
-    union {
-        struct {  // symbolic step
-            alignas(16) uint16_t bmask[Var::NB * 16];  // B row masks
-            uint32_t bm[WAVE];
-            int32_t wpre[WAVE];
-        } y;
-        struct {  // numeric step
-            V aval[Var::PA];
-            int64_t bgoff[Var::NB];
-            int32_t aoff[Var::NA + 1];
-            uint32_t tmask[Var::NC];
-            int16_t pstart[Var::NC + 1];
-            uint16_t desc[DESC_CAP];
-            alignas(16) uint16_t bcmask[Var::NB * 16];  // B column masks
-            alignas(16) uint8_t aptr[Var::NA * 16];
-            alignas(16) uint8_t bcptr[Var::NB * 16];
-            uint16_t plist[Var::NB];
-        } n;
-    } u;
-};
-
-// Output chunk of a staged row: the descriptors of C tiles [ct0, ct1) (at
-// most DESC_CAP outputs, lane = output) and their values and col bytes in
-// registers (as k_num_staged).
-template <typename V, typename Var, typename S>
-__device__ __forceinline__ void fused_chunk(S &s, int nc, int ct0, int &ct1, int &nout, const V *__restrict__ cvalB,
-                                            V *res, uint8_t *cb) {
-    const int lane = lane_id();
-    ct1 = ct0;
-    nout = 0;
-    while (ct1 < nc) {
-        int span = min(4, nc - ct1);
-        int blk = s.coff[ct1 + span] - s.coff[ct1];
-        if (nout + blk > DESC_CAP) {
-            if (nout > 0) break;
-            while (span > 1 && blk > DESC_CAP) {  // one tile holds at most 256 = DESC_CAP outputs
-                span >>= 1;
-                blk = s.coff[ct1 + span] - s.coff[ct1];
-            }
-        }
-        const int item = ct1 * 16 + lane;
-        const uint32_t mc = (lane < span * 16) ? s.cmask[item] : 0u;
-        const int pc = __popc(mc);
-        const int oincl = dpp_incl_sum(pc);
-        int o = nout + oincl - pc;
-        uint32_t m = mc;
-        while (m) {
-            const int c = __clz(m) - 16;
-            m ^= 0x8000u >> c;
-            s.u.n.desc[o++] = (uint16_t)((item << 4) | c);
-        }
-        nout += blk;
-        ct1 += span;
-    }
-    wave_sync();
-#pragma unroll
-    for (int it = 0; it < DESC_CAP / WAVE; it++) {
-        const int o = lane + it * WAVE;
-        res[it] = V(0);
-        cb[it] = 0;
-        if (it * WAVE < nout && o < nout) {
-            const int d = s.u.n.desc[o];
-            const int ct = d >> 8, r = (d >> 4) & 15, c = d & 15;
-            V acc = V(0);
-            const int pe = s.u.n.pstart[ct + 1];
-            for (int p = s.u.n.pstart[ct]; p < pe; p++) {
-                const int e = s.u.n.plist[p];
-                const int b = e & 0xFF, t = e >> 8;
-                const uint32_t ma = s.amask[t * 16 + r];
-                const uint32_t cm = s.u.n.bcmask[b * 16 + c];
-                uint32_t m = ma & cm;  // k of the products, ascending from the top bit
-                if (!m) continue;
-                const int ea = s.u.n.aoff[t] + s.u.n.aptr[t * 16 + r];
-                const V *bv = cvalB + (s.u.n.bgoff[b] + s.u.n.bcptr[b * 16 + c]);
-                do {
-                    const int kk = __clz(m) - 16;
-                    m ^= 0x8000u >> kk;
-                    const uint32_t up = below(kk);
-                    acc = fma_v<V>(s.u.n.aval[ea + __popc(ma & up)], bv[__popc(cm & up)], acc);
-                } while (m);
-            }
-            res[it] = acc;
-            cb[it] = (uint8_t)((r << 4) | c);
-        }
-    }
-}
-
-// Rows (waves) per workgroup of the one-pass kernel: the look-back runs per
-// workgroup, so one status word stands for this many rows; with the small
-// variant (about 6.5 KB of LDS a wave) two workgroups fit a CU.
-template <typename Var>
-constexpr int fused_rows() { return Var::NA <= 8 ? 12 : 8; }
-
-// The symbolic part follows k_sym_staged, the numeric part k_num_staged; C is
-// written in its final (compact) layout.  A workgroup takes fused_rows()
-// consecutive tile-rows (one per wave, ticket order); its tile and value
-// counts are published at once, and after every wave has computed its first
-// output chunk, wave 0 finds the workgroup's place in C by a look-back over
-// the workgroups before it.  flags[0]: a row outgrew the variant or C's
-// capacity (tcap tiles, vcap values); flags[1]: a look-back wait gave up;
-// either way the caller discards C.  Every workgroup publishes both statuses,
-// also when a row does not fit, so the ones after it never wait for ever.
-template <typename V, typename Var>
-__global__ void __launch_bounds__(fused_rows<Var>() *WAVE)
-k_fused_staged(int32_t tilem, unsigned *__restrict__ ticket, const int64_t *__restrict__ tile_ptrA,
-               const int64_t *__restrict__ tile_nnzA, const uint8_t *__restrict__ ptrA,
-               const uint16_t *__restrict__ maskA, const V *__restrict__ valA, const int4 *__restrict__ link,
-               const int32_t *__restrict__ colidxB, const uint16_t *__restrict__ csr_maskB,
-               const int64_t *__restrict__ csr_nnzB, const uint8_t *__restrict__ cptrB,
-               const uint16_t *__restrict__ cmaskB, const V *__restrict__ cvalB, int64_t *__restrict__ tile_ptrC,
-               int32_t *__restrict__ rowidxC, int32_t *__restrict__ colidxC, int64_t *__restrict__ tile_nnzC,
-               uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC, V *__restrict__ valC,
-               uint8_t *__restrict__ colC, int64_t tcap, int64_t vcap, uint64_t *__restrict__ st_t,
-               uint64_t *__restrict__ st_v, int32_t *__restrict__ flags) {
-    static_assert(Var::NC < WAVE && Var::NA <= 32 && Var::NB <= WAVE, "caps exceed the kernel's index widths");
-    constexpr int FW = fused_rows<Var>();
-    constexpr int ITA = (Var::PA + WAVE - 1) / WAVE;
-    __shared__ FusedLDS<V, Var> lds[FW];
-    __shared__ int32_t s_nc[FW], s_nv[FW];
-    __shared__ int64_t s_t0, s_v0;
-    __shared__ unsigned s_tk;
-    if (threadIdx.x == 0) s_tk = atomicAdd(ticket, 1u);  // workgroups take rows in the order they start
-    __syncthreads();
-    const int w = threadIdx.x / WAVE, lane = lane_id();
-    const unsigned tk = s_tk;
-    const int64_t row = (int64_t)tk * FW + w;
-    const bool valid = row < tilem;
-    const int32_t bi = valid ? (int32_t)row : 0;
-    FusedLDS<V, Var> &s = lds[w];
-    int64_t as = 0, ae = 0;
-    if (valid) {
-        as = tile_ptrA[bi];
-        ae = tile_ptrA[bi + 1];
-    }
-    bool bad = ae - as > Var::NA;
-    const int na = bad ? 0 : (int)(ae - as);
-    int nB = 0, nc = 0, nv = 0, pA = 0;
-    int64_t abase = 0;
-    // A headers and links (all loads issued, then stored)
-    int4 lk = make_int4(0, 0, 0, 0);
-    uint4 ap, am0, am1;
-    int32_t anz = 0;
-    if (valid && !bad) {
-        abase = tile_nnzA[as];
-        const int64_t pa64 = tile_nnzA[ae] - abase;
-        bad = pa64 > Var::PA;
-        pA = bad ? 0 : (int)pa64;
-    }
-    if (valid && !bad && lane < na) {
-        anz = (int32_t)(tile_nnzA[as + lane] - abase);
-        lk = link[as + lane];
-        ap = load_ptr16(ptrA, as + lane);
-        load_mask16(maskA, as + lane, am0, am1);
-    }
-    V av[ITA];
-#pragma unroll
-    for (int it = 0; it < ITA; it++) {
-        const int i = lane + it * WAVE;
-        if (i < pA) av[it] = valA[abase + i];
-    }
-    const int incl = dpp_incl_sum(lk.y);
-    nB = last_lane(incl);
-    bad = bad || nB > Var::NB;
-    // B tiles, one lane per B tile: symbolic (column, row masks) and numeric
-    // (column offsets and masks, value offset) headers
-    int32_t j = 0;
-    int bt = 0;
-    int64_t bg = 0;
-    uint4 bp, bm0, bm1, bc0, bc1;
-    if (valid && !bad) {
-        if (lane < na) {
-            s.aseg[lane + 1] = incl;
-            s.abs_[lane] = lk.x;
-            store_mask16(&s.amask[lane * 16], am0, am1);
-        }
-        if (lane == 0) s.aseg[0] = 0;
-        for (int i = lane; i < Var::NC * 16; i += WAVE) s.cmask[i] = 0;
-        wave_sync();
-        if (lane < nB) {
-            bt = owner_of(s.aseg, na, lane);
-            const int64_t q = (int64_t)s.abs_[bt] + (lane - s.aseg[bt]);
-            j = colidxB[q];
-            load_mask16(csr_maskB, q, bm0, bm1);
-            bg = csr_nnzB[q];
-            bp = load_ptr16(cptrB, q);
-            load_mask16(cmaskB, q, bc0, bc1);
-            s.bown[lane] = (uint8_t)bt;
-            store_mask16(&s.u.y.bmask[lane * 16], bm0, bm1);
-        }
-        // distinct tile columns and their ranks (as k_sym_staged)
-        int32_t jmin = 0, jmax = -1;
-        if (nB > 0) {
-            jmin = -last_lane(dpp_incl_max(lane < nB ? -j : -INT_MAX));
-            jmax = last_lane(dpp_incl_max(lane < nB ? j : -1));
-        }
-        int rank = 0, sb = lane;
-        bool first = false;
-        if (nB > 0 && jmax - jmin < WAVE * 32) {
-            s.u.y.bm[lane] = 0u;
-            wave_sync();
-            const int off = j - jmin;
-            if (lane < nB) atomicOr(&s.u.y.bm[off >> 5], 1u << (off & 31));
-            wave_sync();
-            const uint32_t word = s.u.y.bm[lane];
-            const int pc = __popc(word);
-            const int winc = dpp_incl_sum(pc);
-            nc = last_lane(winc);
-            s.u.y.wpre[lane] = winc - pc;
-            wave_sync();
-            if (lane < nB) rank = s.u.y.wpre[off >> 5] + __popc(s.u.y.bm[off >> 5] & ((1u << (off & 31)) - 1u));
-            first = lane < nB;
-        } else {
-            const uint64_t key = wave_sort64(lane < nB ? ((uint64_t)(uint32_t)j << 6) | (uint64_t)lane : ~0ull);
-            const int32_t sj = (int32_t)(key >> 6);
-            sb = (int)(key & 63);
-            const int32_t prev = __shfl_up(sj, 1, WAVE);
-            first = lane < nB && (lane == 0 || prev != sj);
-            const uint64_t fb = __ballot(first);
-            nc = __popcll(fb);
-            rank = mbcnt(fb) + (first ? 1 : 0) - 1;
-            j = sj;
-        }
-        bad = nc > Var::NC;
-        if (!bad) {
-            if (first) s.cj[rank] = j;
-            if (lane < nB) s.bct[sb] = (uint8_t)rank;
-            wave_sync();
-            // masks: item (B tile b, row r) ORs B's rows selected by A's row mask
-            for (int it = lane; it < nB * 16; it += WAVE) {
-                const int b = it >> 4, r = it & 15;
-                uint32_t ma = s.amask[s.bown[b] * 16 + r];
-                uint32_t rm = 0;
-                while (ma) {
-                    const int kk = __clz(ma) - 16;
-                    ma ^= 0x8000u >> kk;
-                    rm |= s.u.y.bmask[b * 16 + kk];
-                }
-                if (rm) atomicOr(&s.cmask[s.bct[b] * 16 + r], rm);
-            }
-            wave_sync();
-            // values of each C tile, then the row's tile offsets
-            for (int it = lane; it < nc * 16; it += WAVE) {
-                const int tot = group16_incl_sum(__popc(s.cmask[it]));
-                if ((it & 15) == 15) s.coff[(it >> 4) + 1] = tot;
-            }
-            wave_sync();
-            const int tc = lane < nc ? s.coff[lane + 1] : 0;
-            const int tinc = dpp_incl_sum(tc);
-            nv = last_lane(tinc);
-            wave_sync();
-            if (lane < nc) s.coff[lane + 1] = tinc;
-            if (lane == 0) s.coff[0] = 0;
-        }
-    }
-    if (bad || !valid) nc = nv = 0;
-    if (lane == 0) {
-        s_nc[w] = nc;
-        s_nv[w] = nv;
-    }
-    if (valid && !bad) {  // numeric headers into LDS (the symbolic arrays are no longer read)
-        wave_sync();
-        if (lane < na) {
-            s.u.n.aoff[lane] = anz;
-            reinterpret_cast<uint4 *>(s.u.n.aptr)[lane] = ap;
-        }
-        if (lane == 0) s.u.n.aoff[na] = pA;
-        if (lane < nc) s.u.n.tmask[lane] = 0;
-#pragma unroll
-        for (int it = 0; it < ITA; it++) {
-            const int i = lane + it * WAVE;
-            if (i < pA) s.u.n.aval[i] = av[it];
-        }
-        wave_sync();
-        int ct_b = 0;
-        if (lane < nB) {
-            ct_b = s.bct[lane];
-            s.u.n.bgoff[lane] = bg;
-            reinterpret_cast<uint4 *>(s.u.n.bcptr)[lane] = bp;
-            store_mask16(&s.u.n.bcmask[lane * 16], bc0, bc1);
-            atomicOr(&s.u.n.tmask[ct_b], 1u << bt);
-        }
-        wave_sync();
-        const int pinc = dpp_incl_sum(lane < nc ? __popc(s.u.n.tmask[lane]) : 0);
-        if (lane < nc) s.u.n.pstart[lane + 1] = (int16_t)pinc;
-        if (lane == 0) s.u.n.pstart[0] = 0;
-        wave_sync();
-        if (lane < nB)
-            s.u.n.plist[s.u.n.pstart[ct_b] + __popc(s.u.n.tmask[ct_b] & ((1u << bt) - 1u))] =
-                (uint16_t)(lane | (bt << 8));
-        wave_sync();
-    }
-    __syncthreads();
-    // the workgroup's counts become visible before its values are computed
-    int gnc = 0, gnv = 0;
-    if (w == 0) {
-        gnc = lane < FW ? s_nc[lane] : 0;
-        gnv = lane < FW ? s_nv[lane] : 0;
-        gnc = wave_sum(gnc);
-        gnv = wave_sum(gnv);
-        if (lane == 0) {
-            lb_store(st_t + tk, LB_AGG | (uint64_t)gnc);
-            lb_store(st_v + tk, LB_AGG | (uint64_t)gnv);
-        }
-    }
-    // first output chunk (in registers)
-    V res[DESC_CAP / WAVE];
-    uint8_t cb[DESC_CAP / WAVE];
-    int ct1 = 0, nout = 0;
-    if (nc > 0) fused_chunk<V, Var>(s, nc, 0, ct1, nout, cvalB, res, cb);
-    __syncthreads();
-    if (w == 0) {  // the workgroup's place in C
-        const int64_t t0 = lookback(st_t, tk, flags + 1);
-        const int64_t v0 = lookback(st_v, tk, flags + 1);
-        if (lane == 0) {
-            lb_store(st_t + tk, LB_PRE | (uint64_t)(t0 + gnc));
-            lb_store(st_v + tk, LB_PRE | (uint64_t)(v0 + gnv));
-            s_t0 = t0;
-            s_v0 = v0;
-        }
-    }
-    __syncthreads();
-    if (!valid) return;
-    int64_t t0 = s_t0, v0 = s_v0;
-    for (int q = 0; q < w; q++) {
-        t0 += s_nc[q];
-        v0 += s_nv[q];
-    }
-    if (bad || t0 + nc > tcap || v0 + nv > vcap) {
-        if (lane == 0) atomicOr(flags, 1);
-        return;
-    }
-    // C headers
-    if (lane == 0) {
-        tile_ptrC[bi] = t0;
-        if (bi == tilem - 1) {
-            tile_ptrC[tilem] = t0 + nc;
-            tile_nnzC[t0 + nc] = v0 + nv;
-        }
-    }
-    if (lane < nc) {
-        colidxC[t0 + lane] = s.cj[lane];
-        rowidxC[t0 + lane] = bi;
-        tile_nnzC[t0 + lane] = v0 + s.coff[lane];
-    }
-    for (int it = lane; it < nc * 16; it += WAVE) {
-        const uint32_t rm = s.cmask[it];
-        const int pc = __popc(rm);
-        const int inc = group16_incl_sum(pc);
-        maskC[t0 * 16 + it] = (uint16_t)rm;
-        ptrRowC[t0 * 16 + it] = (uint8_t)(inc - pc);
-    }
-    // values: the first chunk, then the rest
-    for (int ct0 = 0; ct0 < nc;) {
-        if (ct0 > 0) fused_chunk<V, Var>(s, nc, ct0, ct1, nout, cvalB, res, cb);
-        const int64_t obase = v0 + s.coff[ct0];
-#pragma unroll
-        for (int it = 0; it < DESC_CAP / WAVE; it++) {
-            const int o = lane + it * WAVE;
-            if (o < nout) {
-                valC[obase + o] = res[it];
-                colC[obase + o] = cb[it];
-            }
-        }
-        wave_sync();
-        ct0 = ct1;
-    }
-}
-
 // ---------------------------------------------------------------- host side
 bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
     if (opts && opts->numeric_variant != 0) return false;  // C-tile kernels everywhere
@@ -1206,7 +751,7 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
     TSG_CHECK(palloc(ctx, &p.rows, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.bin, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.counts, (size_t)(p.nbins * nb + 1)));
-    TSG_CHECK(palloc(ctx, &p.hdr, HDR_FUSED + 1));
+    TSG_CHECK(palloc(ctx, &p.hdr, MAX_BINS + 3));
     TSG_CHECK(palloc(ctx, &p.overflow, 1));
     TSG_CHECK(palloc(ctx, &p.link, (size_t)A->numtile));
     TSG_CHECK(palloc(ctx, &p.lpay, (size_t)A->numtile));
@@ -1240,12 +785,11 @@ int plan_bin(Context *ctx, Plan &p, int32_t tilem, const int64_t *total0, const 
 }
 
 int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1) {
-    TSG_HIP(hipMemcpyAsync(ctx->pinned, p.hdr, sizeof(int64_t) * (HDR_FUSED + 1), hipMemcpyDeviceToHost, ctx->stream));
+    TSG_HIP(hipMemcpyAsync(ctx->pinned, p.hdr, sizeof(int64_t) * (p.nbins + 3), hipMemcpyDeviceToHost, ctx->stream));
     TSG_HIP(hipStreamSynchronize(ctx->stream));
     if (total0) *total0 = ctx->pinned[0];
     if (total1) *total1 = ctx->pinned[1];
     for (int b = 0; b <= p.nbins; b++) p.bin_off[b] = ctx->pinned[2 + b];
-    p.fused = (int)ctx->pinned[HDR_FUSED];
     return TSG_OK;
 }
 
@@ -1447,51 +991,6 @@ int replay_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     tsg_tile_matrix Cm = *C;
     if (r.path == 2) return launch_numeric_mfma(ctx, A, B, &Cm, &r.o);
     return launch_numeric(ctx, A, B, &Cm, &r.o);
-}
-
-template <typename V, typename Var>
-static void launch_fused_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                           const Plan &p, int64_t tcap, int64_t vcap, uint64_t *st, unsigned *ticket,
-                           int32_t *flags) {
-    const int32_t tilem = A->tilem;
-    constexpr int FW = fused_rows<Var>();  // status words: one per workgroup (tilem are set up)
-    k_fused_staged<V, Var><<<nblk(tilem, FW), FW * WAVE, 0, ctx->stream>>>(
-        tilem, ticket, A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val, p.link,
-        B->tile_columnidx, B->csr_mask, B->csr_tile_nnz, B->csr_col_Ptr, B->csr_col_mask, (const V *)B->csr_col_Val,
-        C->tile_ptr, C->tile_rowidx, C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr, (V *)C->tile_csr_Val,
-        C->tile_csr_Col, tcap, vcap, st + LB_PAD, st + 2 * LB_PAD + tilem, flags);
-}
-
-int64_t fused_status_words(int32_t tilem) { return 2 * ((int64_t)LB_PAD + tilem); }
-
-FusedTable fused_table() {
-    FusedTable t{};
-    t.v[0] = caps_of<FusS>();
-    t.v[1] = caps_of<FusM>();
-    t.n = 2;
-    return t;
-}
-
-int launch_fused(Context *ctx, int var, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                 const Plan &p, int64_t tcap, int64_t vcap, uint64_t *st, unsigned *ticket, int32_t *flags) {
-    if (A->tilem <= 0) return TSG_OK;
-    k_lb_init<<<nblk(fused_status_words(A->tilem), 256), 256, 0, ctx->stream>>>(st, A->tilem);
-    TSG_HIP(hipGetLastError());
-    const bool f32 = A->dtype == TSG_F32;
-    switch (var) {
-        case 0:
-            if (f32) launch_fused_t<float, FusS>(ctx, A, B, C, p, tcap, vcap, st, ticket, flags);
-            else launch_fused_t<double, FusS>(ctx, A, B, C, p, tcap, vcap, st, ticket, flags);
-            break;
-        case 1:
-            if (f32) launch_fused_t<float, FusM>(ctx, A, B, C, p, tcap, vcap, st, ticket, flags);
-            else launch_fused_t<double, FusM>(ctx, A, B, C, p, tcap, vcap, st, ticket, flags);
-            break;
-        default:
-            return fail(TSG_ERR_ARG, "no staged variant for the one-pass product");
-    }
-    TSG_HIP(hipGetLastError());
-    return TSG_OK;
 }
 
 }  // namespace tsg

@@ -137,9 +137,6 @@ typedef struct tsg_stats {
                                predicted layout exceeded int32 tile ids or free device memory */
     int32_t host_syncs;     /* blocking device-to-host reads made inside the call */
     int64_t pred_numtileC, pred_nnzC;  /* learned: allocated capacities */
-    int32_t one_pass;       /* learned: 1 when one kernel wrote C's structure and values (every row fit
-                               one staged variant), 0 otherwise */
-    int32_t reserved;
 } tsg_stats;
 
 /* ---- context ---- */

@@ -201,52 +201,6 @@ def test_learned_gain_over_untrained_and_upper_bound(trained):
 
 
 @pytest.mark.parametrize("f32", [False, True])
-def test_one_pass_equals_exact(trained, f32):
-    """Products whose rows all fit one staged variant run as one kernel
-    (structure and values; each row's first tile and value from a look-back
-    over the rows before it): bit-identical to the exact pipeline over
-    thousands of workgroups, on the first call and with cached sizes."""
-    A = g.gen_laplace2d(512, 384)
-    tA, tB = _pair(trained, A, f32)
-    print(f"[case] one-pass lap2d 512x384 f32={f32}", flush=True)
-    Ce, se = trained.spgemm(tA, tB)
-    e = Ce.download()
-    for call in range(2):
-        Cl, sl = trained.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
-        print(f"    call {call}: one_pass {sl['one_pass']} fallback {sl['fallback_used']} syncs {sl['host_syncs']} "
-              f"slots/tiles {sl['pred_numtileC'] / se['numtileC']:.3f} values/nnz {sl['pred_nnzC'] / se['nnzC']:.3f}",
-              flush=True)
-        assert sl["one_pass"] == 1 and sl["fallback_used"] == 0
-        assert sl["host_syncs"] == (2 if call == 0 else 1)
-        assert_tiles_equal(Cl.download(), e, keys=CKEYS, label=f"call {call} ")
-        Cl.free()
-
-
-def test_one_pass_other_stencils_equal_exact(trained):
-    """3-D stencil products (whichever path the plan picks) equal the exact pipeline."""
-    for name, A in [("lap3d_40", g.gen_laplace3d(40, 40, 40)), ("st27_24", g.gen_stencil27(24, 24, 24))]:
-        tA, tB = _pair(trained, A)
-        print(f"[case] learned {name}", flush=True)
-        Ce, _ = trained.spgemm(tA, tB)
-        Cl, sl = trained.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
-        print(f"    one_pass {sl['one_pass']} fallback {sl['fallback_used']}", flush=True)
-        assert_tiles_equal(Cl.download(), Ce.download(), keys=CKEYS, label=f"{name} ")
-        Ce.free(); Cl.free(); tA.free(); tB.free()
-
-
-@pytest.mark.parametrize("f32", [False, True])
-def test_one_pass_overflow_falls_back(trained, f32):
-    """C's predicted totals shrunk 100x: the one-pass kernel finds rows past
-    the capacity, flags them, and the exact pipeline runs instead."""
-    A = g.gen_laplace2d(256, 200)
-    tA, tB = _pair(trained, A, f32)
-    Ce, _ = trained.spgemm(tA, tB)
-    Cl, sl = trained.spgemm(tA, tB, scheduler=g.SCHED_LEARNED, pred_scale=0.01)
-    assert sl["fallback_used"] == 1 and sl["one_pass"] == 0
-    assert_tiles_equal(Cl.download(), Ce.download(), keys=CKEYS, label="C ")
-
-
-@pytest.mark.parametrize("f32", [False, True])
 def test_forced_fallback_is_exact(trained, f32):
     A = g.gen_er(4000, 10, seed=31)
     tA, tB = _pair(trained, A, f32)
