@@ -35,6 +35,9 @@ def make_workload(g, name):
     if name == "lap2d_2048":
         A = g.gen_laplace2d(2048, 2048)
         return A, A, {"workload": "laplace2d_5pt_2048x2048_AxA", "rows": A.m, "nnzA": A.nnz, "dtype": "f64"}
+    if name == "lap2d_1024":  # 65,536 tile-rows
+        A = g.gen_laplace2d(1024, 1024)
+        return A, A, {"workload": "laplace2d_5pt_1024x1024_AxA", "rows": A.m, "nnzA": A.nnz, "dtype": "f64"}
     if name == "er4096":
         A = g.gen_er(4096, 8, seed=0)
         return A, A, {"workload": "erdos_renyi_4096_8perrow_AxA", "rows": A.m, "nnzA": A.nnz, "dtype": "f64"}
