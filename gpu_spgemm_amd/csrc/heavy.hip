@@ -169,6 +169,7 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
     __shared__ int64_t s_item;
     __shared__ int s_nq;
     const int64_t total = woff[nr];
+    if ((int64_t)blockIdx.x >= total) return;  // more workgroups than items: no counter traffic
     const int tid = threadIdx.x;
     for (;;) {
         if (tid == 0) {
@@ -347,6 +348,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
     __shared__ int64_t s_item, s_sp1;
     __shared__ int s_row;
     const int64_t total = woffn[nr];
+    if ((int64_t)blockIdx.x >= total) return;  // more workgroups than items: no counter traffic
     const int tid = threadIdx.x, l = lane_id();
     for (;;) {
         if (tid == 0) s_item = (int64_t)atomicAdd(&ctr[1], 1ull);

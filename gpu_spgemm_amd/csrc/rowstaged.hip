@@ -759,7 +759,7 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
         TSG_CHECK(palloc(ctx, &p.rgw, (size_t)tilem + 3));
         TSG_CHECK(palloc(ctx, &p.rgn, (size_t)tilem + 1));
     }
-    TSG_HIP(hipMemsetAsync(p.overflow, 0, sizeof(int32_t), ctx->stream));
+    TSG_CHECK(zero_small(ctx, p.overflow, 1, nullptr, 0));
     if (A->numtile > 0) {
         k_ab_link<<<nblk(A->numtile, 256), 256, 0, ctx->stream>>>(A->numtile, A->tile_columnidx, B->tile_ptr,
                                                                    ok ? B->row_nnz_ptr : nullptr, p.link, p.lpay);
@@ -806,7 +806,7 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
         int32_t *hlist = nullptr, *hc = nullptr;  // heavy rows (counted by a workgroup each)
         TSG_CHECK(palloc(ctx, &hlist, (size_t)tilem));
         int rc = palloc(ctx, &hc, 4);
-        if (!rc && hipMemsetAsync(hc, 0, 4 * sizeof(int32_t), ctx->stream) != hipSuccess) rc = fail(TSG_ERR_HIP, "memset");
+        if (!rc) rc = zero_small(ctx, hc, 4, nullptr, 0);
         if (!rc) {
             k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
                 A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
@@ -934,7 +934,7 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     if (!rc) rc = palloc(ctx, &C->tile_csr_Ptr, (size_t)C->numtile * 16);
     if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)C->numtile + 1);
     if (timing && !rc) (void)hipEventRecord(ev[1], s);
-    if (!rc && hipMemsetAsync(C->tile_nnz, 0, sizeof(int64_t), s) != hipSuccess) rc = fail(TSG_ERR_HIP, "memset");
+    if (!rc) rc = zero_small(ctx, nullptr, 0, C->tile_nnz, 1);
     SlotLayout exact;
     exact.overflow = p.overflow;
     if (!rc) rc = run_symbolic(ctx, A, B, C, p, exact);

@@ -74,6 +74,22 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict
     }
 }
 
+// Zeroes a few small buffers in one launch (a kernel: capturable in a graph,
+// and no runtime fill between the pipeline's own kernels).
+__global__ void k_zero_small(int32_t *__restrict__ a, int na, int64_t *__restrict__ b, int nb) {
+    const int t = threadIdx.x;
+    if (t < na) a[t] = 0;
+    if (t < nb) b[t] = 0;
+}
+
+int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb) {
+    if (na > 64 || nb > 64) return fail(TSG_ERR_ARG, "zero_small: at most 64 words each");
+    if ((!a || na <= 0) && (!b || nb <= 0)) return TSG_OK;
+    k_zero_small<<<1, 64, 0, ctx->stream>>>(a, a ? na : 0, b, b ? nb : 0);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
 int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n) {
     if (n <= 0) return TSG_OK;
     int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;

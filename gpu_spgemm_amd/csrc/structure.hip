@@ -198,6 +198,7 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
     __shared__ int32_t s_min, s_max;
     const int tid = threadIdx.x;
     const int nh = *hcount;
+    if ((int64_t)blockIdx.x >= nh) return;  // more workgroups than heavy rows: no counter traffic
     for (;;) {
         if (tid == 0) {
             s_h = (int)atomicAdd(ctr, 1ull);

@@ -94,6 +94,8 @@ constexpr int64_t ROW_CHUNK = int64_t(1) << 23;
 // data[1..n-1], this leaves exclusive offsets (the reference's CUB
 // ExclusiveSum, src/spgemm_raw_v2.cu:131-138, 202-208, widened to int64).
 int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n);
+// Zeroes a[0..na) and b[0..nb) (at most 64 each) with one small kernel on the context's stream.
+int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb);
 // Reads one int64 from device memory (blocking; counts as a host sync).
 int read_i64(Context *ctx, const int64_t *dptr, int64_t *out);
 int read_i32(Context *ctx, const int32_t *dptr, int32_t *out);
