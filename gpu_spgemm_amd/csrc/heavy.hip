@@ -315,6 +315,45 @@ struct RgwNumItem {
     int64_t ce, ta, tb;  // row's end tile; the item's tiles [ta, tb)
 };
 
+// Head of a batch of 64 A entries (lane l: entry eh + l, if below eb): its B
+// segment in the sub-window -- the carried position, or a search in the item's
+// first sub-window -- and its first RGW_PF0 products' columns and values.
+template <typename V>
+__device__ __forceinline__ void rgn_head(int64_t eh, int64_t eb, int64_t rp0, bool first, const int64_t *cur,
+                                         const int64_t *chi, const int64_t *gb, int64_t gcap,
+                                         const V *__restrict__ valA, const int32_t *__restrict__ colA,
+                                         const int64_t *__restrict__ rpB, const int32_t *__restrict__ colB,
+                                         const V *__restrict__ valB, int32_t lo_col, int64_t &lo, int64_t &hi, V &a,
+                                         int32_t *jj, V *bv) {
+    const int64_t e = eh + lane_id();
+    const int64_t ci = e - rp0;
+    const bool in_lds = ci < RGN_CUR;
+    const bool cached = in_lds || ci - RGN_CUR < gcap;
+    lo = 0;
+    hi = 0;
+    a = V(0);
+    if (e < eb) {
+        a = valA[e];
+        if (cached && !first) {
+            lo = in_lds ? cur[ci] : gb[ci - RGN_CUR];
+            hi = in_lds ? chi[ci] : gb[gcap + ci - RGN_CUR];
+        } else {
+            const int32_t k = colA[e];
+            hi = rpB[k + 1];
+            lo = rgw_seek(colB, rpB[k], hi, lo_col);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < RGW_PF0; t++) {
+        jj[t] = INT_MAX;
+        bv[t] = V(0);
+        if (lo + t < hi) {
+            jj[t] = colB[lo + t];
+            bv[t] = valB[lo + t];
+        }
+    }
+}
+
 // Output slot of product column j for scalar row r in the staged sub-window, or -1.
 __device__ __forceinline__ int rgw_slot(const uint16_t *map, const uint16_t *msk, const uint8_t *prw,
                                         const int32_t *tbs, int32_t j, int32_t c0, int nt, int nv, int r) {
@@ -458,6 +497,13 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                 const int32_t i = s_it.bi * 16 + r;
                 if (i >= m) continue;
                 const int64_t ea = s_rp[r], eb = s_rp[r + 1];
+                // the next batch's head is loaded while this batch's products are added
+                int64_t plo, phi;
+                V pa;
+                int32_t pjj[RGW_PF0];
+                V pbv[RGW_PF0];
+                rgn_head<V>(ea, eb, s_rp[0], first, s.cur, s.hi, gb, gcap, valA, colA, rpB, colB, valB, lo_col, plo,
+                            phi, pa, pjj, pbv);
                 for (int64_t e0 = ea; e0 < eb; e0 += WAVE) {  // 64 A entries of row r at a time
                     // lane l owns entry e0 + l: its B segment in the sub-window,
                     // the first RGN_PF products' B values and output slots (each
@@ -471,41 +517,24 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                     const int64_t ci = e - s_rp[0];
                     const bool in_lds = ci < RGN_CUR;
                     const bool cached = in_lds || ci - RGN_CUR < gcap;
-                    int64_t lo = 0, hi = 0;
-                    V a = V(0);
-                    if (e < eb) {
-                        a = valA[e];
-                        if (cached && !first) {
-                            lo = in_lds ? s.cur[ci] : gb[ci - RGN_CUR];
-                            hi = in_lds ? s.hi[ci] : gb[gcap + ci - RGN_CUR];
-                        } else {
-                            const int32_t k = colA[e];
-                            hi = rpB[k + 1];
-                            lo = rgw_seek(colB, rpB[k], hi, lo_col);
-                        }
-                    }
+                    const int64_t lo = plo, hi = phi;
+                    const V a = pa;
                     int64_t next = lo;  // where the segment ends (the next sub-window's start)
                     int sl[RGN_PF];
                     V bv[RGN_PF];
                     bool more = false, any = false;
 #pragma unroll
                     for (int t = 0; t < RGN_PF; t++) sl[t] = -1;
-                    if (lo < hi) {
+                    {
                         int32_t jj[RGN_PF], tc[RGN_PF], lt[RGN_PF], tbo[RGN_PF];
                         uint32_t mr[RGN_PF], po[RGN_PF];
-                        // two stages: the first RGW_PF0 entries, the rest only while the
-                        // segment stays in the sub-window (no load for lanes past the row)
+                        // two stages: the first RGW_PF0 entries (loaded with the batch
+                        // head), the rest only while the segment stays in the sub-window
 #pragma unroll
                         for (int t = 0; t < RGN_PF; t++) {
-                            jj[t] = INT_MAX;
-                            bv[t] = V(0);
+                            jj[t] = t < RGW_PF0 ? pjj[t < RGW_PF0 ? t : 0] : INT_MAX;
+                            bv[t] = t < RGW_PF0 ? pbv[t < RGW_PF0 ? t : 0] : V(0);
                         }
-#pragma unroll
-                        for (int t = 0; t < RGW_PF0; t++)
-                            if (lo + t < hi) {
-                                jj[t] = colB[lo + t];
-                                bv[t] = valB[lo + t];
-                            }
                         if (lo + RGW_PF0 < hi && jj[RGW_PF0 - 1] < hi_col) {
 #pragma unroll
                             for (int t = RGW_PF0; t < RGN_PF; t++)
@@ -547,6 +576,9 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
 #pragma unroll
                         for (int t = 0; t < RGN_PF; t++) next += (lo + t < hi && jj[t] < hi_col) ? 1 : 0;
                     }
+                    if (e0 + WAVE < eb)  // wave-uniform
+                        rgn_head<V>(e0 + WAVE, eb, s_rp[0], first, s.cur, s.hi, gb, gcap, valA, colA, rpB, colB, valB,
+                                    lo_col, plo, phi, pa, pjj, pbv);
                     uint64_t mk = __ballot(any);
                     const uint64_t mm = __ballot(more);
                     while (mk) {  // entries in ascending k; one entry's outputs are distinct
