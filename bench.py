@@ -109,13 +109,14 @@ def copy_probe(mib):
 
 def profiled_traffic(workload, kernel):
     """HBM bytes per launch of `kernel` on `workload` from the committed
-    rocprofv3 --pmc summaries (lines 'TRAFFIC <workload> <kernel> <bytes>'), or None."""
+    rocprofv3 --pmc summaries (lines 'TRAFFIC <workload> <kernel> <bytes>',
+    the kernel name may hold spaces), or None."""
     import glob
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.txt"))):
         for line in open(f):
             p = line.split()
-            if len(p) == 4 and p[0] == "TRAFFIC" and p[1] == workload and p[2] == kernel:
-                return int(p[3])
+            if len(p) >= 4 and p[0] == "TRAFFIC" and p[1] == workload and " ".join(p[2:-1]) == kernel:
+                return int(p[-1])
     return None
 
 
