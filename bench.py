@@ -236,10 +236,11 @@ def bench_galerkin(g, args):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(A, B, seconds):
-    """The oracle's Gustavson loop (the reference's CPU check), 1 thread: the
-    whole product when it finishes within `seconds`, else row blocks at 8
-    evenly spaced starts holding about `seconds` worth of products in total."""
+def cpu_baseline(A, B, seconds, nthreads=1):
+    """The oracle's Gustavson loop (the reference's CPU check) on `nthreads`
+    threads (row blocks): the whole product when it finishes within
+    `seconds`, else row blocks at 8 evenly spaced starts holding about
+    `seconds` worth of products in total."""
     import numpy as np
     from oracle import oracle as O
     m = A.m
@@ -251,32 +252,32 @@ def cpu_baseline(A, B, seconds):
         return start, max(start + 1, min(stop, end))
 
     r0, r1 = block(m // 2, min(total, 20_000_000), m)
-    p0, g0, _ = O.cpu_baseline(A, B, r0, r1, nthreads=1, convert=False)
+    p0, g0, _ = O.cpu_baseline(A, B, r0, r1, nthreads=nthreads, convert=False)
     rate = p0 / max(g0, 1e-6)
     if total / rate <= seconds:
         times = []
         prod = 0
         t_end = time.time() + seconds
         while time.time() < t_end or len(times) < 3:
-            prod, gus, _ = O.cpu_baseline(A, B, 0, m, nthreads=1, convert=False)
+            prod, gus, _ = O.cpu_baseline(A, B, 0, m, nthreads=nthreads, convert=False)
             times.append(gus)
             if len(times) >= 7:
                 break
         med = statistics.median(times)
-        return {"value": round(2.0 * prod / med / 1e9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+        return {"value": round(2.0 * prod / med / 1e9, 4), "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
                 "sample": f"whole product, Gustavson FP64 loop of the reference's CPU check "
-                          f"(src/spgemm_raw_v2.cu:297-308), 1 thread, median of {len(times)} runs "
+                          f"(src/spgemm_raw_v2.cu:297-308), {nthreads} thread(s), median of {len(times)} runs "
                           f"({med:.3f} s each, {prod} products)"}
     want = rate * seconds / 8
     prod, gus, rows = 0, 0.0, 0
     for i in range(8):
         a, b = block(i * m // 8, want, (i + 1) * m // 8)
-        p, t, _ = O.cpu_baseline(A, B, a, b, nthreads=1, convert=False)
+        p, t, _ = O.cpu_baseline(A, B, a, b, nthreads=nthreads, convert=False)
         prod, gus, rows = prod + p, gus + t, rows + (b - a)
-    return {"value": round(2.0 * prod / gus / 1e9, 4), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+    return {"value": round(2.0 * prod / gus / 1e9, 4), "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
             "sample": f"{rows} of {m} rows in 8 blocks at evenly spaced starts, {prod} of {total} products "
                       f"({gus:.2f} s), Gustavson FP64 loop of the reference's CPU check "
-                      f"(src/spgemm_raw_v2.cu:297-308), 1 thread"}
+                      f"(src/spgemm_raw_v2.cu:297-308), {nthreads} thread(s)"}
 
 
 def main():
@@ -399,6 +400,9 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds)
+        # the same loop over row blocks on the host cores this run may use (at most 16)
+        nt = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
+        out["cpu_baseline_parallel"] = cpu_baseline(A, B, args.cpu_seconds, nthreads=nt)
     ctx.close()
     print(json.dumps(out), flush=True)
 
