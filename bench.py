@@ -146,7 +146,8 @@ def bench_batch(g, args):
     best, rep = ctx.autotune(tA, tB, reps=1)
     log(f"autotune ({len(rep)} configurations, {time.time() - t1:.1f} s): best var_mask {best['var_mask']} "
         f"rg_density {best['rg_density']} {best['ms']:.1f} ms; " +
-        " ".join(f"{r['var_mask']}/{r['rg_density']}:{r['ms']:.0f}" for r in rep))
+        " ".join(f"{r['var_mask']}/{r['rg_density']}:{r['ms']:.0f}{'*' if r['sampled'] else ''}" for r in rep) +
+        " (* = timed on every 7th problem only)")
     res = {}
     for label, fn in (("loop", loop), ("batched", batched)):
         for _ in range(args.warmup):

@@ -57,7 +57,7 @@ class Options(C.Structure):
 
 class TuneEntry(C.Structure):
     _fields_ = [("var_mask", C.c_int32), ("rg_density", C.c_int32), ("mfma_min_products", C.c_int32),
-                ("reserved", C.c_int32), ("ms", C.c_double)]
+                ("sampled", C.c_int32), ("ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -35,6 +35,10 @@ void tuner_release(Context *ctx) {
     ctx->tuner = nullptr;
 }
 
+constexpr int32_t TUNE_SAMPLE_MIN = 16;  // batches this large are sampled first
+constexpr int32_t TUNE_STRIDE = 7;
+constexpr int TUNE_FINALISTS = 3;
+
 static uint64_t lb(double x) { return x <= 1.0 ? 0 : (uint64_t)std::min(63.0, std::floor(std::log2(x))); }
 
 uint64_t tune_signature(int64_t tilemA, int64_t numtileA, int64_t nnzA, int64_t tilemB, int64_t numtileB,
@@ -128,25 +132,57 @@ int tsg_autotune(tsg_context *c, int32_t count, const tsg_tile_matrix *A, const 
     Tuner *t = tuner(ctx);
     std::vector<tsg_tune_entry> rep;
     int rc = TSG_OK;
-    {
-        for (const TuneCfg &cand : cands) {
-            t->forcing = true;
-            t->forced = cand;
-            double w = 0;
-            rc = run_once(ctx, count, A, B, &w);  // warm-up (pool sizes, caches)
-            std::vector<double> ms;
-            for (int r = 0; r < reps && !rc; r++) {
-                rc = run_once(ctx, count, A, B, &w);
-                ms.push_back(w);
-            }
-            t->forcing = false;
-            if (rc) return rc;
-            std::sort(ms.begin(), ms.end());
-            rep.push_back(tsg_tune_entry{cand.var_mask, cand.rg_density, cand.mfma, 0, ms[ms.size() / 2]});
+    // A batch of TUNE_SAMPLE_MIN or more products is first timed on a sample
+    // (every TUNE_STRIDE-th product: an odd stride, so alternating problem
+    // kinds both appear; one warm-up and one timed run); only the
+    // TUNE_FINALISTS fastest there are timed on the whole batch.
+    std::vector<tsg_tile_matrix> SA, SB;
+    if (count >= TUNE_SAMPLE_MIN)
+        for (int32_t i = 0; i < count; i += TUNE_STRIDE) {
+            SA.push_back(A[i]);
+            SB.push_back(B[i]);
         }
+    std::vector<char> full(cands.size(), 1);
+    std::vector<double> sample_ms(cands.size(), 0.0);
+    if (!SA.empty()) {
+        for (size_t c = 0; c < cands.size() && !rc; c++) {
+            t->forcing = true;
+            t->forced = cands[c];
+            double w = 0;
+            rc = run_once(ctx, (int32_t)SA.size(), SA.data(), SB.data(), &w);
+            if (!rc) rc = run_once(ctx, (int32_t)SA.size(), SA.data(), SB.data(), &w);
+            t->forcing = false;
+            sample_ms[c] = w;
+        }
+        if (rc) return rc;
+        std::vector<size_t> order(cands.size());
+        for (size_t c = 0; c < order.size(); c++) order[c] = c;
+        std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return sample_ms[x] < sample_ms[y]; });
+        for (size_t r = 0; r < order.size(); r++) full[order[r]] = r < (size_t)TUNE_FINALISTS;
     }
-    const auto it = std::min_element(rep.begin(), rep.end(),
-                                     [](const tsg_tune_entry &x, const tsg_tune_entry &y) { return x.ms < y.ms; });
+    for (size_t c = 0; c < cands.size(); c++) {
+        const TuneCfg &cand = cands[c];
+        if (!full[c]) {
+            rep.push_back(tsg_tune_entry{cand.var_mask, cand.rg_density, cand.mfma, 1, sample_ms[c]});
+            continue;
+        }
+        t->forcing = true;
+        t->forced = cand;
+        double w = 0;
+        rc = run_once(ctx, count, A, B, &w);  // warm-up (pool sizes, caches)
+        std::vector<double> ms;
+        for (int r = 0; r < reps && !rc; r++) {
+            rc = run_once(ctx, count, A, B, &w);
+            ms.push_back(w);
+        }
+        t->forcing = false;
+        if (rc) return rc;
+        std::sort(ms.begin(), ms.end());
+        rep.push_back(tsg_tune_entry{cand.var_mask, cand.rg_density, cand.mfma, 0, ms[ms.size() / 2]});
+    }
+    const auto it = std::min_element(rep.begin(), rep.end(), [](const tsg_tune_entry &x, const tsg_tune_entry &y) {
+        return x.sampled != y.sampled ? x.sampled < y.sampled : x.ms < y.ms;  // fully timed first
+    });
     TuneCfg cfg;
     cfg.var_mask = it->var_mask;
     cfg.rg_density = it->rg_density;

@@ -267,7 +267,8 @@ typedef struct tsg_tune_entry {
     int32_t var_mask;
     int32_t rg_density;
     int32_t mfma_min_products;  /* > 0: matrix-core C-tile kernels with this gate */
-    int32_t reserved;
+    int32_t sampled;         /* report only: 1 = timed on a sample of the batch (every 7th product) and
+                                not among the fastest there, so not timed on the whole batch */
     double ms;               /* median wall time of one product (or batch), host syncs included */
 } tsg_tune_entry;
 int tsg_autotune(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,

@@ -85,3 +85,26 @@ def test_autotune_batch(ctx):
         C1, _ = ctx.spgemm(tA[i], tB[i], variant=1)
         assert_tiles_equal(Cs[i].download(), C1.download(), keys=CKEYS, label=f"problem {i} ")
     ctx.autotune_clear()
+
+
+def test_autotune_large_batch_samples_then_times_finalists(ctx):
+    """A batch of 16 or more products: every configuration is timed on every
+    7th product, the 3 fastest there on the whole batch; the stored winner is
+    the fastest of those, and the batch under it equals the C-tile method."""
+    ps = [g.gen_er(300 + 37 * i, 3 + (i % 5), seed=200 + i) if i % 2 else g.gen_laplace2d(20 + i, 17)
+          for i in range(20)]
+    tA = [ctx.tiles_from_csr(a) for a in ps]
+    tB = [ctx.tiles_from_csr(a, as_b=True) for a in ps]
+    best, rep = ctx.autotune(tA, tB, reps=1)
+    full = [r for r in rep if r["sampled"] == 0]
+    print("    finalists:", [(r["var_mask"], r["rg_density"], r["mfma_min_products"], round(r["ms"], 3)) for r in full],
+          flush=True)
+    assert len(rep) == 22 and len(full) == 3
+    assert best["sampled"] == 0 and best["ms"] == min(r["ms"] for r in full)
+    q = ctx.autotune_query(tA, tB)
+    assert q is not None and q["var_mask"] == best["var_mask"] and q["rg_density"] == best["rg_density"]
+    Cs, _ = ctx.spgemm_batched(tA, tB)
+    for i in range(len(ps)):
+        C1, _ = ctx.spgemm(tA[i], tB[i], variant=1)
+        assert_tiles_equal(Cs[i].download(), C1.download(), keys=CKEYS, label=f"problem {i} ")
+    ctx.autotune_clear()
