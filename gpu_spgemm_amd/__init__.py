@@ -46,7 +46,9 @@ class TileStruct(C.Structure):
                 ("csr_col_mask", C.POINTER(C.c_uint16)), ("csr_col_Ptr", C.POINTER(C.c_uint8)),
                 ("csr_col_Val", C.c_void_p), ("row_nnz_ptr", C.POINTER(C.c_int64)),
                 ("row_ptr", C.POINTER(C.c_int64)), ("row_col", C.POINTER(C.c_int32)), ("row_val", C.c_void_p),
-                ("val_perm", C.POINTER(C.c_int64)), ("val_perm_col", C.POINTER(C.c_int64))]
+                ("val_perm", C.POINTER(C.c_int64)), ("val_perm_col", C.POINTER(C.c_int64)),
+                ("tile_csc_Ptr", C.POINTER(C.c_uint8)), ("tile_csc_Row", C.POINTER(C.c_uint8)),
+                ("tile_csc_Val", C.c_void_p)]
 
 
 class Options(C.Structure):
@@ -527,6 +529,13 @@ class DeviceTiles:
         _check(lib().tsg_build_csc_tiles(self.ctx.handle, C.byref(self.s)))
         return self
 
+    def build_tile_csc(self):
+        """Within-tile CSC of every tile on the GPU, in place: the reference's
+        tile_csc_* of a B operand (src/tile_format.h:547-584) or tile_a_csc_* of
+        row tiles (build_a_tile_csc, src/tile_format.h:589-627)."""
+        _check(lib().tsg_build_tile_csc(self.ctx.handle, C.byref(self.s)))
+        return self
+
     def download(self):
         """Returns a dict of numpy arrays in the reference's field names."""
         h = TileStruct()
@@ -553,6 +562,10 @@ class DeviceTiles:
                 d["csr_col_Ptr"] = _np(h.csr_col_Ptr, nt * 16, np.uint8)
                 d["csr_col_Val"] = _np(h.csr_col_Val, nnz, vdt)
                 d["row_nnz_ptr"] = _np(h.row_nnz_ptr, h.tilem + 1, np.int64)
+        if h.tile_csc_Ptr:          # within-tile CSC (tsg_build_tile_csc)
+            d["tile_csc_Ptr"] = _np(h.tile_csc_Ptr, nt * 16, np.uint8)
+            d["tile_csc_Row"] = _np(h.tile_csc_Row, nnz, np.uint8)
+            d["tile_csc_Val"] = _np(h.tile_csc_Val, nnz, vdt)
         _check(lib().tsg_tile_free_host(C.byref(h)))
         return d
 
@@ -605,6 +618,12 @@ def upload_tiles(ctx, d):
     h.mask = ptr("mask", C.c_uint16)
     h.csc_tile_ptr = ptr("csc_tile_ptr", C.c_int64)
     h.csc_tile_rowidx = ptr("csc_tile_rowidx", C.c_int32)
+    if d.get("tile_csc_Ptr") is not None:
+        h.tile_csc_Ptr = ptr("tile_csc_Ptr", C.c_uint8)
+        h.tile_csc_Row = ptr("tile_csc_Row", C.c_uint8)
+        cv = np.ascontiguousarray(d["tile_csc_Val"])
+        keep["cv"] = cv
+        h.tile_csc_Val = cv.ctypes.data_as(C.c_void_p)
     t = DeviceTiles(ctx)
     _check(lib().tsg_tile_upload(ctx.handle, C.byref(h), C.byref(t.s)))
     return t

@@ -253,6 +253,12 @@ int tsg_tile_upload(tsg_context *c, const tsg_tile_matrix *h, tsg_tile_matrix *d
     t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
     t.row_ptr = nullptr; t.row_col = nullptr; t.row_val = nullptr;
     t.val_perm = nullptr; t.val_perm_col = nullptr;
+    t.tile_csc_Ptr = nullptr; t.tile_csc_Row = nullptr; t.tile_csc_Val = nullptr;
+    if (h->tile_csc_Ptr && h->tile_csc_Row && h->tile_csc_Val) {
+        TSG_CHECK(up(ctx, h->tile_csc_Ptr, nt * 16, &t.tile_csc_Ptr));
+        TSG_CHECK(up(ctx, h->tile_csc_Row, (size_t)h->nnz, &t.tile_csc_Row));
+        TSG_CHECK(up(ctx, (const uint8_t *)h->tile_csc_Val, (size_t)h->nnz * vsize(h->dtype), (uint8_t **)&t.tile_csc_Val));
+    }
     if (h->row_ptr && h->row_col && h->row_val) {
         TSG_CHECK(up(ctx, h->row_ptr, (size_t)h->m + 1, &t.row_ptr));
         TSG_CHECK(up(ctx, h->row_col, (size_t)h->nnz, &t.row_col));
@@ -297,6 +303,12 @@ int tsg_tile_download(tsg_context *c, const tsg_tile_matrix *d, tsg_tile_matrix 
     t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
     t.row_ptr = nullptr; t.row_col = nullptr; t.row_val = nullptr;
     t.val_perm = nullptr; t.val_perm_col = nullptr;
+    t.tile_csc_Ptr = nullptr; t.tile_csc_Row = nullptr; t.tile_csc_Val = nullptr;
+    if (d->tile_csc_Ptr && d->tile_csc_Row && d->tile_csc_Val) {
+        TSG_CHECK(down(ctx, d->tile_csc_Ptr, nt * 16, &t.tile_csc_Ptr));
+        TSG_CHECK(down(ctx, d->tile_csc_Row, (size_t)d->nnz, &t.tile_csc_Row));
+        TSG_CHECK(down(ctx, (const uint8_t *)d->tile_csc_Val, (size_t)d->nnz * vsize(d->dtype), (uint8_t **)&t.tile_csc_Val));
+    }
     if (d->row_ptr && d->row_col && d->row_val) {
         TSG_CHECK(down(ctx, d->row_ptr, (size_t)d->m + 1, &t.row_ptr));
         TSG_CHECK(down(ctx, d->row_col, (size_t)d->nnz, &t.row_col));
@@ -326,7 +338,8 @@ int tsg_tile_free(tsg_context *c, tsg_tile_matrix *d) {
     void *ps[] = {d->tile_ptr, d->tile_columnidx, d->tile_rowidx, d->tile_nnz, d->tile_csr_Val,
                   d->tile_csr_Col, d->tile_csr_Ptr, d->mask, d->csc_tile_ptr, d->csc_tile_rowidx,
                   d->csr_tile_nnz, d->csr_mask, d->csr_col_mask, d->csr_col_Ptr, d->csr_col_Val, d->row_nnz_ptr,
-                  d->row_ptr, d->row_col, d->row_val, d->val_perm, d->val_perm_col};
+                  d->row_ptr, d->row_col, d->row_val, d->val_perm, d->val_perm_col,
+                  d->tile_csc_Ptr, d->tile_csc_Row, d->tile_csc_Val};
     for (void *p : ps) pool_free(ctx, p);
     memset(d, 0, sizeof *d);
     return TSG_OK;
@@ -338,7 +351,8 @@ int tsg_tile_free_host(tsg_tile_matrix *h) {
     void *ps[] = {h->tile_ptr, h->tile_columnidx, h->tile_rowidx, h->tile_nnz, h->tile_csr_Val,
                   h->tile_csr_Col, h->tile_csr_Ptr, h->mask, h->csc_tile_ptr, h->csc_tile_rowidx,
                   h->csr_tile_nnz, h->csr_mask, h->csr_col_mask, h->csr_col_Ptr, h->csr_col_Val, h->row_nnz_ptr,
-                  h->row_ptr, h->row_col, h->row_val, h->val_perm, h->val_perm_col};
+                  h->row_ptr, h->row_col, h->row_val, h->val_perm, h->val_perm_col,
+                  h->tile_csc_Ptr, h->tile_csc_Row, h->tile_csc_Val};
     for (void *p : ps) free(p);
     memset(h, 0, sizeof *h);
     return TSG_OK;
@@ -441,6 +455,14 @@ namespace tsg {
 int spgemm_dispatch(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                     const tsg_options *opts, tsg_stats *st, NumericReplay *keep) {
     if (st) memset(st, 0, sizeof *st);
+    if (opts && (opts->numeric_variant < 0 || opts->numeric_variant > 3))
+        return fail(TSG_ERR_ARG, "numeric_variant %d is not 0-3", opts->numeric_variant);
+    if (opts && opts->numeric_variant == 3) {
+        if (opts->scheduler == TSG_SCHED_LEARNED)
+            return fail(TSG_ERR_ARG, "numeric_variant 3 runs under the plain scheduler");
+        if (!(A->tile_csc_Ptr && A->tile_csc_Row && A->tile_csc_Val))
+            return fail(TSG_ERR_ARG, "numeric_variant 3 needs A's within-tile CSC (tsg_build_tile_csc)");
+    }
     TuneCfg cfg;  // a tuned configuration may select the matrix-core kernels
     if ((!opts || (opts->numeric_variant == 0 && (opts->scheduler != TSG_SCHED_LEARNED || keep))) &&
         tuned_config(ctx, A, B, opts, &cfg) && cfg.mfma > 0 && !((A->flags | B->flags) & 1)) {
@@ -455,7 +477,7 @@ int spgemm_dispatch(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matri
     }
     // a kept plan needs the exact pipeline (the learned one re-plans every call)
     if (opts && opts->scheduler == TSG_SCHED_LEARNED && !keep) return spgemm_learned(ctx, A, B, C, opts, st);
-    if (opts && (opts->numeric_variant == 1 || opts->numeric_variant == 2)) {
+    if (opts && opts->numeric_variant >= 1 && opts->numeric_variant <= 3) {
         if (keep) {
             keep->path = opts->numeric_variant;
             keep->o = *opts;

@@ -109,7 +109,7 @@ void free_batch_keep(Context *ctx, BatchKeep &k) {
         void *ps[] = {M->tile_ptr, M->tile_columnidx, M->tile_rowidx, M->tile_nnz, M->tile_csr_Val, M->tile_csr_Col,
                       M->tile_csr_Ptr, M->mask, M->csc_tile_ptr, M->csc_tile_rowidx, M->csr_tile_nnz, M->csr_mask,
                       M->csr_col_mask, M->csr_col_Ptr, M->csr_col_Val, M->row_nnz_ptr, M->row_ptr, M->row_col,
-                      M->row_val, M->val_perm, M->val_perm_col};
+                      M->row_val, M->val_perm, M->val_perm_col, M->tile_csc_Ptr, M->tile_csc_Row, M->tile_csc_Val};
         for (void *p : ps) pool_free(ctx, p);
     }
     free_plan(ctx, k.nr.p);

@@ -609,6 +609,9 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
         }
     }
     pool_free(ctx, pos_of);
+    // a within-tile CSC built in row-tile order no longer matches the data arrays
+    pool_free(ctx, B->tile_csc_Ptr); pool_free(ctx, B->tile_csc_Row); pool_free(ctx, B->tile_csc_Val);
+    B->tile_csc_Ptr = nullptr; B->tile_csc_Row = nullptr; B->tile_csc_Val = nullptr;
     B->val_perm_col = vpc;
     pool_free(ctx, B->tile_csr_Col);
     pool_free(ctx, B->tile_csr_Ptr);
@@ -629,6 +632,111 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     B->tile_rowidx = rowidx;
     B->is_csc = 1;
     TSG_HIP(hipStreamSynchronize(s));
+    return TSG_OK;
+}
+
+// ---------------------------------------------------------------- within-tile CSC
+namespace tsg {
+
+// A 16-lane group per tile, lane = local column c.  Without repeated columns,
+// column c holds row r exactly when bit 15-c of mask[r] is set, and that entry
+// sits at tile_csr_Ptr[r] + (set bits of mask[r] left of c); rows ascending
+// then give the column's entries in the order the reference's counting
+// scatter writes them (src/tile_format.h:567-582, :609-625).  With repeated
+// columns (flags bit 0) the lane walks the tile's col bytes in storage order
+// and keeps those of column c, which is that same order.
+template <typename V, bool DUPS>
+__global__ void __launch_bounds__(256)
+k_tile_csc(int64_t nt, const int64_t *__restrict__ tile_nnz, const uint8_t *__restrict__ ptr,
+           const uint16_t *__restrict__ mask, const uint8_t *__restrict__ colb, const V *__restrict__ val,
+           uint8_t *__restrict__ cptr, uint8_t *__restrict__ crow, V *__restrict__ cval) {
+    const int c = lane_id() & 15;
+    const uint32_t bit = 0x8000u >> c, left = (0xFFFF0000u >> c) & 0xFFFFu;
+    for (int64_t t = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4); t < nt; t += (int64_t)gridDim.x * 16) {
+        const int64_t base = tile_nnz[t];
+        const int tn = (int)(tile_nnz[t + 1] - base);
+        uint32_t m[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) m[r] = mask[t * 16 + r];
+        int cnt = 0;
+        if (DUPS) {
+            for (int e = 0; e < tn; e++) cnt += (colb[base + e] & 15) == c;
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; r++) cnt += (m[r] & bit) ? 1 : 0;
+        }
+        int o = group16_incl_sum(cnt) - cnt;
+        cptr[t * 16 + c] = (uint8_t)o;
+        if (DUPS) {
+            int r = 0;
+            for (int e = 0; e < tn; e++) {
+                while (r < 15 && e >= (int)ptr[t * 16 + r + 1]) r++;  // the row holding entry e
+                if ((colb[base + e] & 15) == c) {
+                    crow[base + o] = (uint8_t)r;
+                    cval[base + o] = val[base + e];
+                    o++;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                if (m[r] & bit) {
+                    const int e = (int)ptr[t * 16 + r] + __popc(m[r] & left);
+                    crow[base + o] = (uint8_t)r;
+                    cval[base + o] = val[base + e];
+                    o++;
+                }
+            }
+        }
+    }
+}
+
+template <typename V>
+static int launch_tile_csc_t(Context *ctx, const tsg_tile_matrix *T, uint8_t *cptr, uint8_t *crow, void *cval) {
+    const unsigned grid = grid_capped(T->numtile, 16);
+    if (T->flags & 1)
+        k_tile_csc<V, true><<<grid, 256, 0, ctx->stream>>>(T->numtile, T->tile_nnz, T->tile_csr_Ptr, T->mask,
+                                                           T->tile_csr_Col, (const V *)T->tile_csr_Val, cptr, crow,
+                                                           (V *)cval);
+    else
+        k_tile_csc<V, false><<<grid, 256, 0, ctx->stream>>>(T->numtile, T->tile_nnz, T->tile_csr_Ptr, T->mask,
+                                                            T->tile_csr_Col, (const V *)T->tile_csr_Val, cptr, crow,
+                                                            (V *)cval);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
+// Fills the within-tile CSC arrays of T (allocated by the caller).
+static int launch_tile_csc(Context *ctx, const tsg_tile_matrix *T, uint8_t *cptr, uint8_t *crow, void *cval) {
+    if (T->numtile == 0) return TSG_OK;
+    if (T->dtype == TSG_F32) return launch_tile_csc_t<float>(ctx, T, cptr, crow, cval);
+    return launch_tile_csc_t<double>(ctx, T, cptr, crow, cval);
+}
+
+}  // namespace tsg
+
+extern "C" int tsg_build_tile_csc(tsg_context *c, tsg_tile_matrix *T) {
+    using namespace tsg;
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !T) return fail(TSG_ERR_ARG, "null argument");
+    if (T->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles must be on the device");
+    if (!T->mask || !T->tile_csr_Ptr || !T->tile_nnz) return fail(TSG_ERR_ARG, "tiles lack masks or row offsets");
+    (void)hipSetDevice(ctx->device);
+    uint8_t *cptr = nullptr, *crow = nullptr;
+    void *cval = nullptr;
+    int rc = palloc(ctx, &cptr, (size_t)T->numtile * 16);
+    if (!rc) rc = palloc(ctx, &crow, (size_t)T->nnz);
+    if (!rc) rc = pool_alloc(ctx, &cval, (size_t)T->nnz * vsize(T->dtype));
+    if (!rc) rc = launch_tile_csc(ctx, T, cptr, crow, cval);
+    if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(TSG_ERR_HIP, "within-tile CSC failed");
+    if (rc) {
+        pool_free(ctx, cptr); pool_free(ctx, crow); pool_free(ctx, cval);
+        return rc;
+    }
+    pool_free(ctx, T->tile_csc_Ptr); pool_free(ctx, T->tile_csc_Row); pool_free(ctx, T->tile_csc_Val);
+    T->tile_csc_Ptr = cptr;
+    T->tile_csc_Row = crow;
+    T->tile_csc_Val = cval;
     return TSG_OK;
 }
 
@@ -737,5 +845,6 @@ extern "C" int tsg_tile_set_values(tsg_context *c, tsg_tile_matrix *T, const voi
                                                              T->val_perm_col, (double *)T->tile_csr_Val,
                                                              (double *)T->csr_col_Val, (double *)T->row_val);
     TSG_HIP(hipGetLastError());
+    if (T->tile_csc_Val) TSG_CHECK(launch_tile_csc(ctx, T, T->tile_csc_Ptr, T->tile_csc_Row, T->tile_csc_Val));
     return TSG_OK;
 }

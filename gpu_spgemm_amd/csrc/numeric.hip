@@ -15,6 +15,9 @@
 // multiply-add per product, pairs in ascending k and A entries in storage
 // order: the summation order is fixed, so results are reproducible run to run.
 //
+// k_numeric_outer is the outer-product form of the same step over A's
+// within-tile CSC (numeric_variant 3), with identical results.
+//
 // This is the C-tile-centric method of the reference.  It runs every tile in
 // the reference-method variant and the heavy tile-rows (one workgroup per row)
 // in the binned pipeline; light rows go to the row-staged kernels.
@@ -110,6 +113,85 @@ __device__ __forceinline__ void num_tile(int64_t ti, const NumArgs<V> &p) {
     }
 }
 
+// Outer-product form of the same step (numeric_variant 3), over A's
+// within-tile CSC (tile_csc_Ptr/Row/Val, the reference's tile_a_csc_* of
+// src/tile_format.h:39-41 built by build_a_tile_csc :589-627): a 16-lane
+// group per C tile, lane = tile column c, 16 register accumulators indexed by
+// row.  For every pair (ascending k tile) and every local k, A's column k
+// (its rows, read from the CSC arrays) times B's row k adds one product to
+// each (r, c) it meets.  Output (r, c) therefore sees its products in the
+// order the row form gives them (pairs ascending, k ascending, repeated
+// entries in storage order), so the two forms produce identical bits.
+template <typename V, bool DUPS>
+__device__ __forceinline__ void num_tile_outer(int64_t ti, const NumArgs<V> &p, const uint8_t *__restrict__ acp,
+                                               const uint8_t *__restrict__ acr, const V *__restrict__ acv) {
+    const int c = lane_id() & 15;
+    const uint32_t bit = 0x8000u >> c, left = (0xFFFF0000u >> c) & 0xFFFFu;
+    const int64_t c0 = p.tile_nnzC[ti];
+    if (p.tile_nnzC[ti + 1] == c0) return;  // structurally empty tile
+    const int32_t bi = p.rowidxC[ti], bj = p.colidxC[ti];
+    V acc[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = V(0);
+    for_each_pair(p.tile_ptrA, p.colidxA, p.csc_ptrB, p.csc_rowB, bi, bj, [&](int64_t a, int64_t b) {
+        const int64_t an0 = p.tile_nnzA[a];
+        const int an = (int)(p.tile_nnzA[a + 1] - an0);
+        const int64_t bn0 = p.tile_nnzB[b];
+        const int bn = (int)(p.tile_nnzB[b + 1] - bn0);
+        for (int k = 0; k < 16; k++) {
+            const int e0 = acp[a * 16 + k];
+            const int e1 = k == 15 ? an : (int)acp[a * 16 + k + 1];
+            if (e0 == e1) continue;  // A column k empty
+            if (DUPS) {
+                const int q0 = p.ptrRowB[b * 16 + k], q1 = k == 15 ? bn : (int)p.ptrRowB[b * 16 + k + 1];
+                for (int e = e0; e < e1; e++) {
+                    const int r = acr[an0 + e];
+                    const V va = acv[an0 + e];
+                    for (int q = q0; q < q1; q++) {
+                        if ((p.colB[bn0 + q] & 15) != c) continue;
+                        const V vb = p.valB[bn0 + q];
+#pragma unroll
+                        for (int rr = 0; rr < 16; rr++)
+                            if (rr == r) acc[rr] = fmav<V>(va, vb, acc[rr]);
+                    }
+                }
+                continue;
+            }
+            const uint32_t mb = p.maskB[b * 16 + k];
+            if (!(mb & bit)) continue;  // B[k, c] absent
+            const V vb = p.valB[bn0 + p.ptrRowB[b * 16 + k] + __popc(mb & left)];
+            uint32_t ca = 0;  // rows of A's column k
+            for (int e = e0; e < e1; e++) ca |= 0x8000u >> acr[an0 + e];
+            int e = e0;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                if (ca & (0x8000u >> r)) {
+                    acc[r] = fmav<V>(acv[an0 + e], vb, acc[r]);
+                    e++;
+                }
+            }
+        }
+    });
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const uint32_t mc = p.maskC[ti * 16 + r];
+        if (mc & bit) {
+            const int64_t o = c0 + p.ptrRowC[ti * 16 + r] + __popc(mc & left);
+            p.valC[o] = acc[r];
+            p.colC[o] = (uint8_t)((r << 4) | c);
+        }
+    }
+}
+
+template <typename V, bool DUPS>
+__global__ void __launch_bounds__(NU_THREADS)
+k_numeric_outer(int64_t numtileC, NumArgs<V> p, const uint8_t *__restrict__ acp, const uint8_t *__restrict__ acr,
+                const V *__restrict__ acv) {
+    for (int64_t ti = (int64_t)blockIdx.x * NU_GROUPS + (threadIdx.x >> 4); ti < numtileC;
+         ti += (int64_t)gridDim.x * NU_GROUPS)  // grid-stride over tiles
+        num_tile_outer<V, DUPS>(ti, p, acp, acr, acv);
+}
+
 template <typename V, bool DUPS>
 __global__ void __launch_bounds__(NU_THREADS) k_numeric_tiles(int64_t numtileC, NumArgs<V> p) {
     for (int64_t ti = (int64_t)blockIdx.x * NU_GROUPS + (threadIdx.x >> 4); ti < numtileC;
@@ -148,7 +230,18 @@ static int launch_list_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
 
 template <typename V>
 static int launch_tiles_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                          unsigned grid) {
+                          unsigned grid, bool outer) {
+    if (outer) {
+        const V *acv = (const V *)A->tile_csc_Val;
+        if ((A->flags | B->flags) & 1)
+            k_numeric_outer<V, true><<<grid, NU_THREADS, 0, ctx->stream>>>(C->numtile, num_args<V>(A, B, C),
+                                                                           A->tile_csc_Ptr, A->tile_csc_Row, acv);
+        else
+            k_numeric_outer<V, false><<<grid, NU_THREADS, 0, ctx->stream>>>(C->numtile, num_args<V>(A, B, C),
+                                                                            A->tile_csc_Ptr, A->tile_csc_Row, acv);
+        TSG_HIP(hipGetLastError());
+        return TSG_OK;
+    }
     if ((A->flags | B->flags) & 1)
         k_numeric_tiles<V, true><<<grid, NU_THREADS, 0, ctx->stream>>>(C->numtile, num_args<V>(A, B, C));
     else
@@ -166,11 +259,13 @@ int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_m
 
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                    const tsg_options *opts) {
-    (void)opts;
+    const bool outer = opts && opts->numeric_variant == 3;
+    if (outer && !(A->tile_csc_Ptr && A->tile_csc_Row && A->tile_csc_Val))
+        return fail(TSG_ERR_ARG, "numeric_variant 3 needs A's within-tile CSC (tsg_build_tile_csc)");
     if (C->numtile == 0 || C->nnz == 0) return TSG_OK;
     const unsigned grid = grid_capped(C->numtile, NU_GROUPS);
-    if (A->dtype == TSG_F32) return launch_tiles_t<float>(ctx, A, B, C, grid);
-    return launch_tiles_t<double>(ctx, A, B, C, grid);
+    if (A->dtype == TSG_F32) return launch_tiles_t<float>(ctx, A, B, C, grid, outer);
+    return launch_tiles_t<double>(ctx, A, B, C, grid, outer);
 }
 
 }  // namespace tsg

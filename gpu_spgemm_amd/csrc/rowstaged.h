@@ -85,7 +85,7 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
 // What re-running a product's numeric step needs (plan.hip): the kernels the
 // product used and, for the binned pipeline, its plan.
 struct NumericReplay {
-    int path = 0;  // 0 binned pipeline, 1 C-tile kernels, 2 C-tile kernels with matrix cores
+    int path = 0;  // 0 binned pipeline, 1 C-tile kernels, 2 C-tile kernels with matrix cores, 3 outer-product C-tile
     Plan p;
     tsg_options o{};
 };

@@ -102,6 +102,15 @@ typedef struct tsg_tile_matrix {
      * entry's value lives, so tsg_tile_set_values can refresh every copy. */
     int64_t *val_perm;        /* [nnz] position in tile_csr_Val */
     int64_t *val_perm_col;    /* [nnz] B only: position in csr_col_Val */
+    /* Optional (tsg_build_tile_csc): every tile's entries column by column,
+     * rows ascending, in the tile order of the data arrays above.  For a B
+     * operand these are the reference's tile_csc_Ptr/Row/Val
+     * (src/tile_format.h:35-37, built at :547-584); for row tiles (A) its
+     * tile_a_csc_Ptr/Row/Val (:39-41, built by build_a_tile_csc :589-627).
+     * NULL when absent. */
+    uint8_t *tile_csc_Ptr;    /* [nt*16] exclusive column offsets inside each tile */
+    uint8_t *tile_csc_Row;    /* [nnz] local row of each entry */
+    void *tile_csc_Val;       /* [nnz] values in that order */
 } tsg_tile_matrix;
 
 typedef struct tsg_context tsg_context;
@@ -117,7 +126,8 @@ typedef struct tsg_options {
     int32_t numeric_variant;/* 0 = auto: tile-rows binned to the row-staged kernels, heavy rows to the
                                row-by-row or C-tile kernels; 1 = C-tile kernels for every tile (the
                                reference's method); 2 = C-tile kernels with matrix cores (MFMA) for dense
-                               tile pairs */
+                               tile pairs; 3 = C-tile kernels in outer-product form over A's
+                               within-tile CSC (needs tsg_build_tile_csc on A; plain scheduler) */
     int32_t mfma_min_products; /* variant 2: a pair goes to the matrix cores when nnz(A tile) * nnz(B tile)
                                   / 16 (its expected product count) >= this (0 = 256) */
     int32_t timing;         /* 1 = record per-step hipEvent times in tsg_stats */
@@ -201,6 +211,14 @@ int tsg_build_csc_tiles(tsg_context *ctx, tsg_tile_matrix *B_dev);
  * every value copy (tiles, column-major copy, scalar CSR) is rewritten.  No
  * reference counterpart (the reference rebuilds the tiles on the host). */
 int tsg_tile_set_values(tsg_context *ctx, tsg_tile_matrix *dev, const void *vals);
+/* Within-tile CSC of every tile (tile_csc_Ptr/Row/Val above), on the
+ * device; replaces the within-tile transpose of build_csc_tiles
+ * (src/tile_format.h:547-584) for a B operand and build_a_tile_csc
+ * (src/tile_format.h:589-627) for row tiles.  Entries of a column keep row
+ * order, and repeated columns of a row (flags bit 0) keep storage order, as
+ * the reference's counting scatter does.  Rebuilding replaces the arrays;
+ * tsg_tile_set_values refreshes tile_csc_Val. */
+int tsg_build_tile_csc(tsg_context *ctx, tsg_tile_matrix *dev);
 /* Row tiles -> CSR (the reference's verification unpacking, src/spgemm_raw_v2.cu:268-284). */
 int tsg_tile2csr(tsg_context *ctx, const tsg_tile_matrix *dev, tsg_csr *out_dev);
 

@@ -249,6 +249,33 @@ def load_mtx(path):
     return m.value, n.value, rp, col[:nnz.value], val[:nnz.value]
 
 
+def tile_csc(d):
+    """Within-tile CSC of every tile of a tile dict (the download() layout, data
+    arrays in their own tile order): the reference's tile_csc_Ptr/Row/Val of a
+    B operand (src/tile_format.h:547-584) or tile_a_csc_Ptr/Row/Val of row
+    tiles (build_a_tile_csc, src/tile_format.h:589-627).
+
+    Restated as a stable sort of the entries, in storage order, by (tile, local
+    column): the reference's counting scatter walks each tile's rows ascending
+    and entries in storage order (:567-582, :609-625), which is that order.
+    Ptr is the exclusive prefix of the column counts, kept as unsigned char."""
+    tnz = np.asarray(d["tile_nnz"], np.int64)
+    nt = len(tnz) - 1
+    nnz = int(tnz[-1])
+    tile_of = np.repeat(np.arange(nt, dtype=np.int64), np.diff(tnz))
+    local = np.arange(nnz, dtype=np.int64) - tnz[tile_of]
+    ptr = np.asarray(d["tile_csr_Ptr"], np.int64).reshape(nt, 16)
+    # row r holds entries [ptr[r], ptr[r+1]): the row is the count of later row
+    # starts (r' = 1..15) at or below the entry
+    row = (ptr[tile_of, 1:] <= local[:, None]).sum(axis=1).astype(np.uint8) if nnz else np.zeros(0, np.uint8)
+    col = np.asarray(d["tile_csr_Col"], np.uint8).astype(np.int64) & 15
+    order = np.lexsort((np.arange(nnz), col, tile_of))
+    cnt = np.bincount(tile_of * 16 + col, minlength=nt * 16).reshape(nt, 16)
+    cptr = (np.cumsum(cnt, axis=1) - cnt).astype(np.int64) & 0xFF
+    return dict(tile_csc_Ptr=cptr.astype(np.uint8).reshape(-1), tile_csc_Row=row[order],
+                tile_csc_Val=np.asarray(d["tile_csr_Val"])[order])
+
+
 # ---------------------------------------------------------------- reference-compiled checker
 def ref_csr2tile(csr, as_b=False):
     """The reference's own csr2tile (+ build_csc_tiles) compiled from its sources."""
@@ -257,6 +284,20 @@ def ref_csr2tile(csr, as_b=False):
     fn = ref().ref_csr2tile_csc if as_b else ref().ref_csr2tile
     fn(csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double), C.byref(t))
     return tile_to_dict(t, ref().ref_tile_free)
+
+
+def ref_tile_csc(csr, as_b=False):
+    """The reference's own within-tile CSC (build_csc_tiles :547-584 for a B
+    operand, build_a_tile_csc :589-627 for row tiles), compiled from its sources."""
+    d = ref_csr2tile(csr, as_b)
+    nt, nnz = len(d["tile_nnz"]) - 1, int(d["tile_nnz"][-1])
+    rp, col, val = _csr_args(csr)
+    ptr = np.zeros(max(nt * 16, 1), np.uint8)
+    row = np.zeros(max(nnz, 1), np.uint8)
+    v = np.zeros(max(nnz, 1), np.float64)
+    ref().ref_tile_csc(csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double), 1 if as_b else 0,
+                       _p(ptr, C.c_uint8), _p(row, C.c_uint8), _p(v, C.c_double))
+    return dict(tile_csc_Ptr=ptr[:nt * 16], tile_csc_Row=row[:nnz], tile_csc_Val=v[:nnz])
 
 
 def ref_load_mtx(path):

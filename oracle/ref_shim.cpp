@@ -97,6 +97,31 @@ int ref_build_bitmasks(int32_t mA, int32_t nA, const int64_t *rpA, const int32_t
     return 0;
 }
 
+// The reference's within-tile CSC: of a B operand (build_csc_tiles,
+// tile_format.h:547-584) or of row tiles (build_a_tile_csc, :589-627).  The
+// caller sizes the outputs from ref_csr2tile(_csc): ptr [nt*16], row/val [nnz].
+int ref_tile_csc(int32_t m, int32_t n, const int64_t *rp, const int32_t *col, const double *val, int32_t as_b,
+                 uint8_t *ptr, uint8_t *row, double *vout) {
+    HostCSR h = make_csr(m, n, rp, col, val);
+    TileMatrix T = csr2tile(h);
+    int nt;
+    const unsigned char *p, *r;
+    const TVAL *v;
+    if (as_b) {
+        build_csc_tiles(T, h);
+        nt = T.numtileB_csc; p = T.tile_csc_Ptr; r = T.tile_csc_Row; v = T.tile_csc_Val;
+    } else {
+        build_a_tile_csc(T);
+        nt = T.numtile; p = T.tile_a_csc_Ptr; r = T.tile_a_csc_Row; v = T.tile_a_csc_Val;
+    }
+    const int nnz = T.tile_nnz[nt];
+    memcpy(ptr, p, (size_t)nt * TILE_SIZE);
+    memcpy(row, r, (size_t)nnz);
+    memcpy(vout, v, sizeof(double) * (size_t)nnz);
+    free_tile_matrix(T);
+    return 0;
+}
+
 // Reference MTX reader (tile_format.h:51-90).  Two-phase: rp == NULL returns sizes.
 int ref_load_mtx(const char *path, int32_t *m, int32_t *n, int64_t *nnz, int64_t *rp, int32_t *col,
                  double *val) {

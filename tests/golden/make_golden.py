@@ -1,7 +1,7 @@
 """Writes the golden fixtures for the tile-format conversion and the MTX reader.
 
 Expected outputs come from the reference's own host code (src/tile_format.h:
-load_mtx, csr2tile, build_csc_tiles) compiled from the reference checkout into
+load_mtx, csr2tile, build_csc_tiles with its within-tile CSC, build_a_tile_csc) compiled from the reference checkout into
 oracle/_ref/libref.so by oracle/Makefile; inputs are small synthetic matrices
 and hand-written .mtx files.  Run in a container that has the reference
 checkout:  make -C oracle && python tests/golden/make_golden.py
@@ -92,6 +92,9 @@ def main():
                     data[f"{name}/{tag}/{k}"] = v
                 elif v is not None:
                     data[f"{name}/{tag}/{k}"] = np.array(v)
+            # within-tile CSC: build_csc_tiles (:547-584) for B, build_a_tile_csc (:589-627) for A
+            for k, v in O.ref_tile_csc(A, as_b=as_b).items():
+                data[f"{name}/{tag}csc/{k}"] = v
     for fname, text in MTX.items():
         p = os.path.join(HERE, fname)
         with open(p, "w") as f:

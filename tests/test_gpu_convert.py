@@ -1,4 +1,5 @@
-"""GPU tile-format conversion (tsg_csr2tile, tsg_build_csc_tiles, tsg_tile2csr)
+"""GPU tile-format conversion (tsg_csr2tile, tsg_build_csc_tiles, tsg_tile2csr,
+tsg_build_tile_csc)
 against the reference's own conversion output (golden fixtures written by
 src/tile_format.h) and against the oracle, bit for bit."""
 import os
@@ -53,6 +54,67 @@ def test_gpu_conversion_matches_oracle(ctx, name, f32):
     h = A.astype(g.F32) if f32 else A
     assert_tiles_equal(ctx.tiles_from_csr(h).download(), O.csr2tile(A, f32=f32), label="A ")
     assert_tiles_equal(ctx.tiles_from_csr(h, as_b=True).download(), O.csr2tile_b(A, f32=f32), label="B ")
+
+
+CSC_KEYS = ("tile_csc_Ptr", "tile_csc_Row", "tile_csc_Val")
+
+
+def assert_csc_equal(got, exp, label=""):
+    for k in CSC_KEYS:
+        a, b = np.asarray(got[k]), np.asarray(exp[k])
+        assert a.shape == b.shape, f"{label}{k}: shape {a.shape} != {b.shape}"
+        if a.dtype.kind == "f":
+            vt = np.uint64 if a.dtype == np.float64 else np.uint32
+            a, b = a.view(vt), b.astype(a.dtype).view(vt)
+        assert np.array_equal(a, b), f"{label}{k}: first mismatches at {np.nonzero(a != b)[0][:5]}"
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_gpu_tile_csc_matches_reference_golden(ctx, name):
+    """tsg_build_tile_csc against the reference's own within-tile CSC
+    (build_csc_tiles :547-584 for B, build_a_tile_csc :589-627 for A)."""
+    print(f"[case] tile csc golden {name}", flush=True)
+    A, _ = gold(name)
+    z = np.load(GOLD, allow_pickle=False)
+    for tag, as_b in (("A", False), ("B", True)):
+        exp = {k.split("/")[-1]: z[k] for k in z.files if k.startswith(f"{name}/{tag}csc/")}
+        assert_csc_equal(ctx.tiles_from_csr(A, as_b=as_b).build_tile_csc().download(), exp, label=f"{tag} ")
+
+
+@pytest.mark.parametrize("name", list(_matrices()))
+@pytest.mark.parametrize("f32", [False, True])
+def test_gpu_tile_csc_matches_oracle(ctx, name, f32):
+    print(f"[case] tile csc {name} f32={f32}", flush=True)
+    A = _matrices()[name]()
+    h = A.astype(g.F32) if f32 else A
+    assert_csc_equal(ctx.tiles_from_csr(h).build_tile_csc().download(), O.tile_csc(O.csr2tile(A, f32=f32)),
+                     label="A ")
+    assert_csc_equal(ctx.tiles_from_csr(h, as_b=True).build_tile_csc().download(),
+                     O.tile_csc(O.csr2tile_b(A, f32=f32)), label="B ")
+
+
+def test_gpu_tile_csc_duplicates_refresh_and_relayout(ctx):
+    """Repeated columns keep storage order; tsg_tile_set_values refreshes the
+    CSC values; re-laying row tiles as a B operand drops the row-order CSC."""
+    print("[case] tile csc duplicates / set_values / relayout", flush=True)
+    rp = np.array([0, 3, 5, 6, 8, 8, 8, 8, 9, 12], dtype=np.int64)
+    col = np.array([1, 1, 7, 0, 3, 2, 2, 3, 6, 0, 0, 0], dtype=np.int32)
+    A = g.HostCSR(9, 8, rp, col, np.arange(1.0, 13.0))
+    for as_b, conv in ((False, O.csr2tile), (True, O.csr2tile_b)):
+        t = ctx.tiles_from_csr(A, as_b=as_b)
+        assert t.s.flags & 1
+        assert_csc_equal(t.build_tile_csc().download(), O.tile_csc(conv(A)), label=f"dups b={as_b} ")
+    B = g.gen_er_var(500, 0, 30, seed=4)
+    t = ctx.tiles_from_csr(B).build_tile_csc()
+    B2 = g.HostCSR(B.m, B.n, B.rowptr, B.col, np.asarray(B.val) * -3.0 + 0.5)
+    ctx.set_values(t, B2)
+    assert_csc_equal(t.download(), O.tile_csc(O.csr2tile(B2)), label="refreshed ")
+    t.build_csc_tiles()
+    assert "tile_csc_Ptr" not in t.download()
+    assert_csc_equal(t.build_tile_csc().download(), O.tile_csc(O.csr2tile_b(B2)), label="relaid ")
+    Z = g.HostCSR(20, 20, np.zeros(21, np.int64), np.zeros(0, np.int32), np.zeros(0))
+    z = ctx.tiles_from_csr(Z).build_tile_csc().download()
+    assert len(z["tile_csc_Ptr"]) == 0 and len(z["tile_csc_Row"]) == 0
 
 
 def test_gpu_conversion_wide_rows_take_several_windows(ctx):

@@ -19,14 +19,17 @@ from tests.conftest import assert_tiles_equal
 pytestmark = pytest.mark.gpu
 TOL = {g.F64: 1e-12, g.F32: 1e-5}
 # 0: tile-rows binned to the row-staged kernels (heavy rows to the C-tile
-# kernels); 1: the reference's C-tile method for every tile
-VARIANTS = [0, 1]
+# kernels); 1: the reference's C-tile method for every tile; 3: the C-tile
+# method in outer-product form over A's within-tile CSC
+VARIANTS = [0, 1, 3]
 
 
 def run(ctx, A, B, f32=False, label="", **kw):
     print(f"[case] spgemm {label} A={A.m}x{A.n} nnz={A.nnz} B={B.m}x{B.n} f32={f32} {kw}", flush=True)
     ha, hb = (A.astype(g.F32), B.astype(g.F32)) if f32 else (A, B)
     tA = ctx.tiles_from_csr(ha)
+    if kw.get("variant") == 3:
+        tA.build_tile_csc()
     tB = ctx.tiles_from_csr(hb, as_b=True)
     Cm, st = ctx.spgemm(tA, tB, **kw)
     got = Cm.download()
@@ -100,7 +103,7 @@ def test_laplace2d_integer_exact(ctx, variant):
     assert chk["max_abs"] == 0.0
 
 
-@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("variant", [0, 1])
 def test_laplace2d_2048_full(ctx, variant):
     """The bench workload at full size: 5-point Laplacian on a 2048 x 2048 grid."""
     A = g.gen_laplace2d(2048, 2048)
@@ -128,11 +131,15 @@ def test_galerkin_chain_on_device(ctx, variant):
     Pt = P.transpose()
     tA = ctx.tiles_from_csr(A)
     tP = ctx.tiles_from_csr(P, as_b=True)
+    if variant == 3:
+        tA.build_tile_csc()
     C1, _ = ctx.spgemm(tA, tP, variant=variant)
     c1 = C1.download()
     check(c1, A, P)
     C1.build_csc_tiles()
     tPt = ctx.tiles_from_csr(Pt)
+    if variant == 3:
+        tPt.build_tile_csc()
     C2, _ = ctx.spgemm(tPt, C1, variant=variant)
     c2 = C2.download()
     AP = O.gustavson_csr(A, P)
@@ -179,6 +186,20 @@ def test_duplicate_columns(ctx, variant):
                                        "tile_csr_Val"])
 
 
+def test_outer_product_variant_needs_a_tile_csc(ctx):
+    A = g.gen_er(300, 6, seed=2)
+    tA, tB = ctx.tiles_from_csr(A), ctx.tiles_from_csr(A, as_b=True)
+    with pytest.raises(g.TsgError, match="within-tile CSC"):
+        ctx.spgemm(tA, tB, variant=3)
+    tA.build_tile_csc()
+    with pytest.raises(g.TsgError, match="plain scheduler"):
+        ctx.spgemm(tA, tB, variant=3, scheduler=g.SCHED_LEARNED)
+    with pytest.raises(g.TsgError, match="numeric_variant"):
+        ctx.spgemm(tA, tB, variant=4)
+    C, _ = ctx.spgemm(tA, tB, variant=3)
+    check(C.download(), A, A)
+
+
 def test_variants_agree_bitwise(ctx):
     """The binned pipeline and the C-tile method give identical bits on a
     matrix with light (staged) and heavy (C-tile) rows."""
@@ -186,6 +207,8 @@ def test_variants_agree_bitwise(ctx):
     a, _ = run(ctx, A, A, label="rmat11 binned", variant=0)
     b, _ = run(ctx, A, A, label="rmat11 ctile", variant=1)
     assert_tiles_equal(a, b)
+    c, _ = run(ctx, A, A, label="rmat11 outer", variant=3)
+    assert_tiles_equal(a, c)
 
 
 def test_deterministic_repeat(ctx):

@@ -1,6 +1,7 @@
 """Pins the oracle (CPU restatement of the reference path) before it is trusted.
 
-- conversion (csr2tile / build_csc_tiles / load_mtx) against golden fixtures
+- conversion (csr2tile / build_csc_tiles / load_mtx, the within-tile CSC of
+  build_csc_tiles and build_a_tile_csc) against golden fixtures
   written by the reference's own src/tile_format.h (tests/golden/), and live
   against oracle/_ref when it is built;
 - Step 1 against the closed form: the C tile set is the boolean product of
@@ -95,6 +96,104 @@ def test_csr2tile_matches_reference_live(seed):
     A = rand_csr(m, n, float(rng.uniform(0.005, 0.2)), seed)
     assert_tiles_equal(O.csr2tile(A), O.ref_csr2tile(A), label="A ")
     assert_tiles_equal(O.csr2tile_b(A), O.ref_csr2tile(A, as_b=True), label="B ")
+
+
+CSC_KEYS = ("tile_csc_Ptr", "tile_csc_Row", "tile_csc_Val")
+
+
+def assert_csc_equal(got, exp, label=""):
+    for k in CSC_KEYS:
+        a, b = np.asarray(got[k]), np.asarray(exp[k])
+        assert a.shape == b.shape, f"{label}{k}: shape {a.shape} != {b.shape}"
+        if a.dtype.kind == "f":
+            a, b = a.view(np.uint64), b.astype(np.float64).view(np.uint64)
+        assert np.array_equal(a, b), f"{label}{k}: first mismatches at {np.nonzero(a != b)[0][:5]}"
+
+
+def with_duplicates(A, seed):
+    """A copy of A in which some entries appear twice (adjacent, as load_mtx
+    keeps them: src/tile_format.h:78-89), with a different value."""
+    rng = np.random.default_rng(seed)
+    rows = np.repeat(np.arange(A.m), np.diff(A.rowptr))
+    dup = rng.random(len(A.col)) < 0.15
+    col = np.repeat(A.col, 1 + dup)
+    val = np.repeat(A.val, 1 + dup)
+    first = np.r_[0, np.cumsum(1 + dup)[:-1]]
+    val[first[dup] + 1] = rng.uniform(-1, 1, int(dup.sum()))
+    rp = np.zeros(A.m + 1, np.int64)
+    np.add.at(rp, rows + 1, 1 + dup)
+    return CSR(A.m, A.n, np.cumsum(rp), col, val)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_tile_csc_matches_reference_golden(name):
+    """Within-tile CSC against the reference's own build_csc_tiles (:547-584)
+    and build_a_tile_csc (:589-627) output."""
+    z = golden()
+    A, _ = gold_case(z, name)
+    for tag, d in (("A", O.csr2tile(A)), ("B", O.csr2tile_b(A))):
+        exp = {k.split("/")[-1]: z[k] for k in z.files if k.startswith(f"{name}/{tag}csc/")}
+        assert_csc_equal(O.tile_csc(d), exp, label=f"{name} {tag} ")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_tile_csc_matches_reference_live(seed):
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    rng = np.random.default_rng(100 + seed)
+    m, n = int(rng.integers(1, 300)), int(rng.integers(1, 300))
+    A = rand_csr(m, n, float(rng.uniform(0.005, 0.2)), seed)
+    if seed % 2:
+        A = with_duplicates(A, seed)  # repeated columns keep storage order
+    assert_csc_equal(O.tile_csc(O.csr2tile(A)), O.ref_tile_csc(A), label="A ")
+    assert_csc_equal(O.tile_csc(O.csr2tile_b(A)), O.ref_tile_csc(A, as_b=True), label="B ")
+
+
+def csc_vs_transpose(A, csc):
+    """Closed form: the within-tile CSC of A's tile (I, J) is the within-tile
+    CSR of the transpose's tile (J, I) with rows and columns swapped."""
+    a = O.csr2tile(A)
+    t = O.csr2tile(from_scipy(A.to_scipy().T))
+    where = {}
+    for J in range(t["tilem"]):
+        for q in range(t["tile_ptr"][J], t["tile_ptr"][J + 1]):
+            where[(J, int(t["tile_columnidx"][q]))] = q
+    for I in range(a["tilem"]):
+        for p in range(a["tile_ptr"][I], a["tile_ptr"][I + 1]):
+            q = where[(int(a["tile_columnidx"][p]), I)]
+            s0, s1 = a["tile_nnz"][p], a["tile_nnz"][p + 1]
+            t0, t1 = t["tile_nnz"][q], t["tile_nnz"][q + 1]
+            if not (np.array_equal(csc["tile_csc_Ptr"][p * 16:p * 16 + 16], t["tile_csr_Ptr"][q * 16:q * 16 + 16])
+                    and np.array_equal(csc["tile_csc_Row"][s0:s1], t["tile_csr_Col"][t0:t1] & 15)
+                    and np.array_equal(csc["tile_csc_Val"][s0:s1], t["tile_csr_Val"][t0:t1])):
+                return False
+    return True
+
+
+@pytest.mark.parametrize("shape", [(150, 170, 0.08), (40, 40, 1.0), (97, 33, 0.3)])
+def test_tile_csc_is_the_tile_transpose(shape):
+    m, n, d = shape
+    A = rand_csr(m, n, d, 7)
+    assert csc_vs_transpose(A, O.tile_csc(O.csr2tile(A)))
+
+
+def test_planted_csc_row_order_is_caught():
+    """Rows of a column written in descending order (a plausible scatter
+    mistake) must fail the transpose check."""
+    A = rand_csr(64, 64, 0.3, 3)
+    csc = O.tile_csc(O.csr2tile(A))
+    tnz = O.csr2tile(A)["tile_nnz"]
+    bad = dict(csc)
+    row, val = csc["tile_csc_Row"].copy(), csc["tile_csc_Val"].copy()
+    t = 0
+    ptr = csc["tile_csc_Ptr"][t * 16:t * 16 + 16].astype(np.int64)
+    end = np.r_[ptr[1:], tnz[t + 1] - tnz[t]]
+    c = int(np.argmax(end - ptr))  # the fullest column of tile 0
+    sl = slice(tnz[t] + ptr[c], tnz[t] + end[c])
+    row[sl], val[sl] = row[sl][::-1].copy(), val[sl][::-1].copy()
+    bad["tile_csc_Row"], bad["tile_csc_Val"] = row, val
+    assert end[c] - ptr[c] > 1
+    assert not csc_vs_transpose(A, bad)
 
 
 @pytest.mark.parametrize("name", ["sym", "pattern", "complex"])
