@@ -117,14 +117,32 @@ __device__ __forceinline__ void num_tile(int64_t ti, const NumArgs<V> &p) {
 // within-tile CSC (tile_csc_Ptr/Row/Val, the reference's tile_a_csc_* of
 // src/tile_format.h:39-41 built by build_a_tile_csc :589-627): a 16-lane
 // group per C tile, lane = tile column c, 16 register accumulators indexed by
-// row.  For every pair (ascending k tile) and every local k, A's column k
+// row.  For every pair (ascending k tile) and every local k where A's column
+// k and B's row k both hold entries (from the tiles' masks), A's column k
 // (its rows, read from the CSC arrays) times B's row k adds one product to
 // each (r, c) it meets.  Output (r, c) therefore sees its products in the
 // order the row form gives them (pairs ascending, k ascending, repeated
 // entries in storage order), so the two forms produce identical bits.
+// A tile's occupied columns (OR of its 16 row masks, bit 15-c) and its
+// non-empty rows (bit 15-r), from two 16-byte loads of the masks.
+__device__ __forceinline__ void tile_cols_rows(const uint16_t *__restrict__ m, int64_t t, uint32_t &cols,
+                                               uint32_t &rows) {
+    const uint4 lo = reinterpret_cast<const uint4 *>(m)[t * 2], hi = reinterpret_cast<const uint4 *>(m)[t * 2 + 1];
+    const uint32_t w[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};  // w[i] = rows 2i (low), 2i+1
+    cols = 0;
+    rows = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        cols |= (w[i] | (w[i] >> 16)) & 0xFFFFu;
+        rows |= ((w[i] & 0xFFFFu) ? 0x8000u : 0u) >> (2 * i);
+        rows |= ((w[i] >> 16) ? 0x8000u : 0u) >> (2 * i + 1);
+    }
+}
+
 template <typename V, bool DUPS>
-__device__ __forceinline__ void num_tile_outer(int64_t ti, const NumArgs<V> &p, const uint8_t *__restrict__ acp,
-                                               const uint8_t *__restrict__ acr, const V *__restrict__ acv) {
+__device__ __forceinline__ void num_tile_outer(int64_t ti, const NumArgs<V> &p, const uint16_t *__restrict__ amask,
+                                               const uint8_t *__restrict__ acp, const uint8_t *__restrict__ acr,
+                                               const V *__restrict__ acv) {
     const int c = lane_id() & 15;
     const uint32_t bit = 0x8000u >> c, left = (0xFFFF0000u >> c) & 0xFFFFu;
     const int64_t c0 = p.tile_nnzC[ti];
@@ -138,10 +156,15 @@ __device__ __forceinline__ void num_tile_outer(int64_t ti, const NumArgs<V> &p, 
         const int an = (int)(p.tile_nnzA[a + 1] - an0);
         const int64_t bn0 = p.tile_nnzB[b];
         const int bn = (int)(p.tile_nnzB[b + 1] - bn0);
-        for (int k = 0; k < 16; k++) {
+        uint32_t colsA, rowsA, colsB, rowsB;
+        tile_cols_rows(amask, a, colsA, rowsA);
+        tile_cols_rows(p.maskB, b, colsB, rowsB);
+        uint32_t ks = colsA & rowsB;  // k with entries in A's column k and B's row k
+        while (ks) {
+            const int k = __clz(ks) - 16;  // ascending k
+            ks ^= 0x8000u >> k;
             const int e0 = acp[a * 16 + k];
             const int e1 = k == 15 ? an : (int)acp[a * 16 + k + 1];
-            if (e0 == e1) continue;  // A column k empty
             if (DUPS) {
                 const int q0 = p.ptrRowB[b * 16 + k], q1 = k == 15 ? bn : (int)p.ptrRowB[b * 16 + k + 1];
                 for (int e = e0; e < e1; e++) {
@@ -185,11 +208,11 @@ __device__ __forceinline__ void num_tile_outer(int64_t ti, const NumArgs<V> &p, 
 
 template <typename V, bool DUPS>
 __global__ void __launch_bounds__(NU_THREADS)
-k_numeric_outer(int64_t numtileC, NumArgs<V> p, const uint8_t *__restrict__ acp, const uint8_t *__restrict__ acr,
-                const V *__restrict__ acv) {
+k_numeric_outer(int64_t numtileC, NumArgs<V> p, const uint16_t *__restrict__ amask, const uint8_t *__restrict__ acp,
+                const uint8_t *__restrict__ acr, const V *__restrict__ acv) {
     for (int64_t ti = (int64_t)blockIdx.x * NU_GROUPS + (threadIdx.x >> 4); ti < numtileC;
          ti += (int64_t)gridDim.x * NU_GROUPS)  // grid-stride over tiles
-        num_tile_outer<V, DUPS>(ti, p, acp, acr, acv);
+        num_tile_outer<V, DUPS>(ti, p, amask, acp, acr, acv);
 }
 
 template <typename V, bool DUPS>
@@ -234,11 +257,11 @@ static int launch_tiles_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
     if (outer) {
         const V *acv = (const V *)A->tile_csc_Val;
         if ((A->flags | B->flags) & 1)
-            k_numeric_outer<V, true><<<grid, NU_THREADS, 0, ctx->stream>>>(C->numtile, num_args<V>(A, B, C),
-                                                                           A->tile_csc_Ptr, A->tile_csc_Row, acv);
+            k_numeric_outer<V, true><<<grid, NU_THREADS, 0, ctx->stream>>>(
+                C->numtile, num_args<V>(A, B, C), A->mask, A->tile_csc_Ptr, A->tile_csc_Row, acv);
         else
-            k_numeric_outer<V, false><<<grid, NU_THREADS, 0, ctx->stream>>>(C->numtile, num_args<V>(A, B, C),
-                                                                            A->tile_csc_Ptr, A->tile_csc_Row, acv);
+            k_numeric_outer<V, false><<<grid, NU_THREADS, 0, ctx->stream>>>(
+                C->numtile, num_args<V>(A, B, C), A->mask, A->tile_csc_Ptr, A->tile_csc_Row, acv);
         TSG_HIP(hipGetLastError());
         return TSG_OK;
     }
