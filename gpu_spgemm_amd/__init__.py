@@ -48,7 +48,7 @@ class TileStruct(C.Structure):
                 ("row_ptr", C.POINTER(C.c_int64)), ("row_col", C.POINTER(C.c_int32)), ("row_val", C.c_void_p),
                 ("val_perm", C.POINTER(C.c_int64)), ("val_perm_col", C.POINTER(C.c_int64)),
                 ("tile_csc_Ptr", C.POINTER(C.c_uint8)), ("tile_csc_Row", C.POINTER(C.c_uint8)),
-                ("tile_csc_Val", C.c_void_p)]
+                ("tile_csc_Val", C.c_void_p), ("tile_size", C.c_int32), ("reserved_ts", C.c_int32)]
 
 
 class Options(C.Structure):
@@ -296,10 +296,16 @@ class Context:
         _check(lib().tsg_csr2tile(self._p, C.byref(dcsr.s), C.byref(t.s)))
         return t
 
-    def tiles_from_csr(self, h, as_b=False):
+    def tiles_from_csr(self, h, as_b=False, ts=16):
+        """Host CSR -> device tiles of ts x ts (csr2tile, or csr2tile_ts for
+        ts != 16; as_b: then build_csc_tiles / build_csc_tiles_ts)."""
         d = self.upload_csr(h)
         try:
-            t = self.csr2tile(d)
+            if ts == 16:
+                t = self.csr2tile(d)
+            else:
+                t = DeviceTiles(self)
+                _check(lib().tsg_csr2tile_ts(self._p, C.byref(d.s), int(ts), C.byref(t.s)))
         finally:
             d.free()
         if as_b:
@@ -541,17 +547,18 @@ class DeviceTiles:
         h = TileStruct()
         _check(lib().tsg_tile_download(self.ctx.handle, C.byref(self.s), C.byref(h)))
         nt, nnz = h.numtile, h.nnz
+        rows = h.tile_size or 16     # per-tile row entries of tile_csr_Ptr and mask
         vdt = np.float32 if h.dtype == F32 else np.float64
         d = dict(m=h.m, n=h.n, tilem=h.tilem, tilen=h.tilen, nnz=nnz, numtile=nt, is_csc=h.is_csc,
-                 dtype=h.dtype,
+                 dtype=h.dtype, tile_size=rows,
                  tile_ptr=_np(h.tile_ptr, h.tilem + 1, np.int64),
                  tile_columnidx=_np(h.tile_columnidx, nt, np.int32),
                  tile_rowidx=_np(h.tile_rowidx, nt, np.int32) if h.tile_rowidx else None,
                  tile_nnz=_np(h.tile_nnz, nt + 1, np.int64),
                  tile_csr_Val=_np(h.tile_csr_Val, nnz, vdt),
                  tile_csr_Col=_np(h.tile_csr_Col, nnz, np.uint8),
-                 tile_csr_Ptr=_np(h.tile_csr_Ptr, nt * 16, np.uint8),
-                 mask=_np(h.mask, nt * 16, np.uint16))
+                 tile_csr_Ptr=_np(h.tile_csr_Ptr, nt * rows, np.uint8),
+                 mask=_np(h.mask, nt * rows, np.uint16))
         if h.is_csc:
             d["csc_tile_ptr"] = _np(h.csc_tile_ptr, h.tilen + 1, np.int64)
             d["csc_tile_rowidx"] = _np(h.csc_tile_rowidx, nt, np.int32)
@@ -606,6 +613,7 @@ def upload_tiles(ctx, d):
     h = TileStruct()
     h.m, h.n, h.tilem, h.tilen = d["m"], d["n"], d["tilem"], d["tilen"]
     h.nnz, h.numtile, h.dtype, h.location, h.is_csc = d["nnz"], d["numtile"], d["dtype"], HOST, d["is_csc"]
+    h.tile_size = 0 if d.get("tile_size", 16) == 16 else int(d["tile_size"])
     h.tile_ptr = ptr("tile_ptr", C.c_int64)
     h.tile_columnidx = ptr("tile_columnidx", C.c_int32)
     h.tile_rowidx = ptr("tile_rowidx", C.c_int32)

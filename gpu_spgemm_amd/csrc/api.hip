@@ -247,8 +247,9 @@ int tsg_tile_upload(tsg_context *c, const tsg_tile_matrix *h, tsg_tile_matrix *d
     TSG_CHECK(up(ctx, h->tile_nnz, nt + 1, &t.tile_nnz));
     TSG_CHECK(up(ctx, (const uint8_t *)h->tile_csr_Val, (size_t)h->nnz * vsize(h->dtype), (uint8_t **)&t.tile_csr_Val));
     TSG_CHECK(up(ctx, h->tile_csr_Col, (size_t)h->nnz, &t.tile_csr_Col));
-    TSG_CHECK(up(ctx, h->tile_csr_Ptr, nt * 16, &t.tile_csr_Ptr));
-    TSG_CHECK(up(ctx, h->mask, nt * 16, &t.mask));
+    const size_t rows = h->tile_size ? (size_t)h->tile_size : 16;  // per-tile row entries
+    TSG_CHECK(up(ctx, h->tile_csr_Ptr, nt * rows, &t.tile_csr_Ptr));
+    TSG_CHECK(up(ctx, h->mask, nt * rows, &t.mask));
     t.csr_tile_nnz = nullptr; t.csr_mask = nullptr; t.csr_col_mask = nullptr; t.csr_col_Ptr = nullptr;
     t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
     t.row_ptr = nullptr; t.row_col = nullptr; t.row_val = nullptr;
@@ -297,8 +298,9 @@ int tsg_tile_download(tsg_context *c, const tsg_tile_matrix *d, tsg_tile_matrix 
     TSG_CHECK(down(ctx, d->tile_nnz, nt + 1, &t.tile_nnz));
     TSG_CHECK(down(ctx, (const uint8_t *)d->tile_csr_Val, (size_t)d->nnz * vsize(d->dtype), (uint8_t **)&t.tile_csr_Val));
     TSG_CHECK(down(ctx, d->tile_csr_Col, (size_t)d->nnz, &t.tile_csr_Col));
-    TSG_CHECK(down(ctx, d->tile_csr_Ptr, nt * 16, &t.tile_csr_Ptr));
-    TSG_CHECK(down(ctx, d->mask, nt * 16, &t.mask));
+    const size_t rows = d->tile_size ? (size_t)d->tile_size : 16;  // per-tile row entries
+    TSG_CHECK(down(ctx, d->tile_csr_Ptr, nt * rows, &t.tile_csr_Ptr));
+    TSG_CHECK(down(ctx, d->mask, nt * rows, &t.mask));
     t.csr_tile_nnz = nullptr; t.csr_mask = nullptr; t.csr_col_mask = nullptr; t.csr_col_Ptr = nullptr;
     t.csr_col_Val = nullptr; t.row_nnz_ptr = nullptr;
     t.row_ptr = nullptr; t.row_col = nullptr; t.row_val = nullptr;
@@ -370,6 +372,8 @@ int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B) {
     if (A->is_csc) return fail(TSG_ERR_ARG, "A must be in row-tile order (is_csc = 1)");
     if (A->n != B->m) return fail(TSG_ERR_ARG, "inner dimensions differ: A is %dx%d, B is %dx%d", A->m, A->n, B->m, B->n);
     if (A->dtype != B->dtype) return fail(TSG_ERR_ARG, "A and B value types differ");
+    if ((A->tile_size && A->tile_size != 16) || (B->tile_size && B->tile_size != 16))
+        return fail(TSG_ERR_UNSUPPORTED, "products need 16 x 16 tiles (tile size %d / %d)", A->tile_size, B->tile_size);
     if (!A->mask) return fail(TSG_ERR_ARG, "A lacks row masks");
     if (A->tilem > MAX_TILE_ROWS || B->tilen > MAX_TILE_ROWS)
         return fail(TSG_ERR_OVERFLOW, "more than %lld tile-rows or tile-columns", (long long)MAX_TILE_ROWS);

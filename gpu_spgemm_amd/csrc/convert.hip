@@ -33,25 +33,34 @@ constexpr int CV_WAVES = 4;
 constexpr int CV_WIN_WORDS = 1024;  // 32,768 tile columns per pass
 
 // ---------------------------------------------------------------- csr2tile
-template <int MODE>  // 0 = count, 1 = fill
+// Tile column of column c for tile size TS (16, a shift) or ts (TS = 0, the
+// parameterized tile sizes 1-15); negative (invalid) columns map below 0.
+template <int TS>
+__device__ __forceinline__ int32_t tile_of(int32_t c, int ts) {
+    if (TS == 16) return c >> 4;
+    return c >= 0 ? c / ts : -1;
+}
+
+template <int MODE, int TS>  // MODE 0 = count, 1 = fill; TS = 16, or 0 for a runtime tile size ts
 __global__ void __launch_bounds__(CV_WAVES *WAVE)
 k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m, int32_t n, int32_t tilem,
       int64_t *__restrict__ tile_ptr, int32_t *__restrict__ tile_colidx, uint8_t *__restrict__ rowcnt,
-      uint16_t *__restrict__ mask, int32_t *__restrict__ flags) {
+      uint16_t *__restrict__ mask, int32_t *__restrict__ flags, int ts_rt) {
+    const int ts = TS ? TS : ts_rt;
     __shared__ uint32_t s_bm[CV_WAVES][CV_WIN_WORDS];
     __shared__ int32_t s_wpre[CV_WAVES][CV_WIN_WORDS];
     __shared__ int64_t s_rp[CV_WAVES][17];
     const int w = threadIdx.x / WAVE, lane = lane_id();
     const int32_t bi = blockIdx.x * CV_WAVES + w;
     if (bi >= tilem) return;
-    const int32_t rs = bi * 16, re = min(rs + 16, m);
-    if (lane <= 16) s_rp[w][lane] = rp[min(rs + lane, re)];  // rows past m repeat the end
+    const int32_t rs = bi * ts, re = min(rs + ts, m);
+    if (lane <= ts) s_rp[w][lane] = rp[min(rs + lane, re)];  // rows past m repeat the end
     wave_sync();
     const int64_t js = s_rp[w][0], je = s_rp[w][re - rs];
     int32_t cmin = INT_MAX, cmax = -1;
     if (lane < re - rs) {
         int64_t a = s_rp[w][lane], b = s_rp[w][lane + 1];
-        if (b > a) { cmin = col[a] >> 4; cmax = col[b - 1] >> 4; }
+        if (b > a) { cmin = tile_of<TS>(col[a], ts); cmax = tile_of<TS>(col[b - 1], ts); }
     }
     cmin = wave_min(cmin);
     cmax = wave_max(cmax);
@@ -65,7 +74,7 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
         for (int i = lane; i < nwords; i += WAVE) bm[i] = 0;
         wave_sync();
         for (int64_t j = js + lane; j < je; j += WAVE) {
-            int64_t off = (int64_t)(col[j] >> 4) - w0;
+            int64_t off = (int64_t)tile_of<TS>(col[j], ts) - w0;
             if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
         }
         wave_sync();
@@ -94,15 +103,16 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
             }
             wave_sync();
             // segment heads: (tile,row) counts and masks
+            const uint32_t top = 1u << (ts - 1);  // mask bit of local column 0
             for (int64_t j = js + lane; j < je; j += WAVE) {
                 const int32_t cj = col[j];
-                const int32_t tc = cj >> 4;
+                const int32_t tc = tile_of<TS>(cj, ts);
                 int64_t off = (int64_t)tc - w0;
                 if (off < 0 || off >= wbits) continue;
                 int r = 0;
-                while (r < 15 && s_rp[w][r + 1] <= j) r++;
+                while (r < ts - 1 && s_rp[w][r + 1] <= j) r++;
                 const int64_t rowbeg = s_rp[w][r], rowend = s_rp[w][r + 1];
-                if (j > rowbeg && (col[j - 1] >> 4) == tc) continue;  // not a segment head
+                if (j > rowbeg && tile_of<TS>(col[j - 1], ts) == tc) continue;  // not a segment head
                 const int wi = (int)(off >> 5);
                 const int32_t t = s_wpre[w][wi] + __popc(bm[wi] & ((1u << (off & 31)) - 1u));
                 const int64_t T = tbase + t;
@@ -110,15 +120,15 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
                 int32_t n = 0, prev = -1;
                 for (int64_t q = j; q < rowend; q++) {
                     const int32_t cq = col[q];
-                    if ((cq >> 4) != tc) break;
+                    if (tile_of<TS>(cq, ts) != tc) break;
                     if (cq < prev) bad = true;
                     prev = cq;
-                    mk |= 0x8000u >> (cq & 15);
+                    mk |= top >> (cq - tc * ts);
                     n++;
                 }
                 if (n != __popc(mk)) dup = true;  // a column repeated inside the row
-                rowcnt[T * 16 + r] = (uint8_t)n;
-                mask[T * 16 + r] = (uint16_t)mk;
+                rowcnt[T * ts + r] = (uint8_t)n;
+                mask[T * ts + r] = (uint16_t)mk;
             }
             tbase += tot;
             wave_sync();
@@ -132,7 +142,7 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
             if (col[j] < 0 || col[j] >= n) range_bad = true;
             if (j == js) continue;
             int r = 0;
-            while (r < 15 && s_rp[w][r + 1] <= j) r++;
+            while (r < ts - 1 && s_rp[w][r + 1] <= j) r++;
             if (j > s_rp[w][r] && col[j - 1] > col[j]) bad = true;
         }
         if (lane == 0) {
@@ -145,45 +155,46 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
     if (__ballot(range_bad) && lane == 0) atomicOr(flags, 4);
 }
 
-// Per tile: exclusive scan of 16 row counts (in place) and the tile's nnz.
+// Per tile: exclusive scan of its ts (<= 16) row counts (in place) and the tile's nnz.
 __global__ void __launch_bounds__(256) k_tile_rowscan(int64_t numtile, uint8_t *__restrict__ ptr,
-                                                      int64_t *__restrict__ tile_nnz) {
+                                                      int64_t *__restrict__ tile_nnz, int ts) {
     const int l = threadIdx.x & 15;
     for (int64_t T = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4; T < numtile; T += (int64_t)gridDim.x * 16) {
-        const int c = ptr[T * 16 + l];
+        const int c = l < ts ? ptr[T * ts + l] : 0;
         const int incl = group16_incl_sum(c);
-        ptr[T * 16 + l] = (uint8_t)(incl - c);
+        if (l < ts) ptr[T * ts + l] = (uint8_t)(incl - c);
         if (l == 15) tile_nnz[T + 1] = incl;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) tile_nnz[0] = 0;
 }
 
-template <typename V>
+template <typename V, int TS>
 __global__ void __launch_bounds__(CV_WAVES *WAVE)
 k_c2t_scatter(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, const V *__restrict__ val,
               int32_t m, int32_t tilem, const int64_t *__restrict__ tile_ptr,
               const int32_t *__restrict__ tile_colidx, const int64_t *__restrict__ tile_nnz,
               const uint8_t *__restrict__ ptr, V *__restrict__ tval, uint8_t *__restrict__ tcol,
-              int64_t *__restrict__ perm) {
+              int64_t *__restrict__ perm, int ts_rt) {
+    const int ts = TS ? TS : ts_rt;
     __shared__ int64_t s_rp[CV_WAVES][17];
     const int w = threadIdx.x / WAVE, lane = lane_id();
     const int32_t bi = blockIdx.x * CV_WAVES + w;
     if (bi >= tilem) return;
-    const int32_t rs = bi * 16, re = min(rs + 16, m);
-    if (lane <= 16) s_rp[w][lane] = rp[min(rs + lane, re)];  // rows past m repeat the end
+    const int32_t rs = bi * ts, re = min(rs + ts, m);
+    if (lane <= ts) s_rp[w][lane] = rp[min(rs + lane, re)];  // rows past m repeat the end
     wave_sync();
     const int64_t js = s_rp[w][0], je = s_rp[w][re - rs];
     const int64_t tb = tile_ptr[bi], te = tile_ptr[bi + 1];
     for (int64_t j = js + lane; j < je; j += WAVE) {
         const int32_t cj = col[j];
-        const int32_t tc = cj >> 4;
+        const int32_t tc = tile_of<TS>(cj, ts);
         int r = 0;
-        while (r < 15 && s_rp[w][r + 1] <= j) r++;
+        while (r < ts - 1 && s_rp[w][r + 1] <= j) r++;
         const int64_t T = lower_bound<int64_t, int32_t>(tile_colidx, tb, te, tc);
-        const int64_t seg = lower_bound<int64_t, int32_t>(col, s_rp[w][r], j, tc << 4);
-        const int64_t pos = tile_nnz[T] + ptr[T * 16 + r] + (j - seg);
+        const int64_t seg = lower_bound<int64_t, int32_t>(col, s_rp[w][r], j, tc * ts);
+        const int64_t pos = tile_nnz[T] + ptr[T * ts + r] + (j - seg);
         tval[pos] = val[j];
-        tcol[pos] = (uint8_t)((r << 4) | (cj & 15));
+        tcol[pos] = (uint8_t)((r << 4) | (cj - tc * ts));  // (row<<4)|col for every tile size
         perm[j] = pos;
     }
 }
@@ -272,19 +283,22 @@ k_csc_perm(int64_t numtile, const int32_t *__restrict__ colidx, const int32_t *_
     if (t == 0) nnz_new[0] = 0;
 }
 
-// 16 lanes per CSC tile: row pointers, masks, then values and local cols.
+// 16 lanes per CSC tile: row pointers, masks (ts <= 16 rows), then values
+// and local cols (the low 4 bits of (row<<4)|col for every tile size).
 template <typename V>
 __global__ void __launch_bounds__(256)
 k_csc_gather(int64_t numtile, const int64_t *__restrict__ inv, const int64_t *__restrict__ nnz_old,
              const int64_t *__restrict__ nnz_new, const uint8_t *__restrict__ ptr_old,
              const uint16_t *__restrict__ mask_old, const uint8_t *__restrict__ col_old,
              const V *__restrict__ val_old, uint8_t *__restrict__ ptr_new, uint16_t *__restrict__ mask_new,
-             uint8_t *__restrict__ col_new, V *__restrict__ val_new) {
+             uint8_t *__restrict__ col_new, V *__restrict__ val_new, int ts) {
     const int l = threadIdx.x & 15;
     for (int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4; p < numtile; p += (int64_t)gridDim.x * 16) {
         const int64_t t = inv[p];
-        ptr_new[p * 16 + l] = ptr_old[t * 16 + l];
-        mask_new[p * 16 + l] = mask_old[t * 16 + l];
+        if (l < ts) {
+            ptr_new[p * ts + l] = ptr_old[t * ts + l];
+            mask_new[p * ts + l] = mask_old[t * ts + l];
+        }
         const int64_t so = nnz_old[t], sn = nnz_new[p], n = nnz_old[t + 1] - so;
         for (int64_t e = l; e < n; e += 16) {
             val_new[sn + e] = val_old[so + e];
@@ -381,16 +395,17 @@ static unsigned blocks_for(int64_t threads, int per) { return (unsigned)((thread
 
 using namespace tsg;
 
-extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T) {
-    Context *ctx = reinterpret_cast<Context *>(c);
-    if (!ctx || !A || !T) return fail(TSG_ERR_ARG, "null argument");
+// csr2tile (ts = 16) and csr2tile_ts (src/tile_format.h:225-324, ts 1-15):
+// the same kernels, the tile size a compile-time 16 or a runtime value.
+static int csr2tile_impl(Context *ctx, const tsg_csr *A, tsg_tile_matrix *T, int ts) {
     if (A->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "CSR must be on the device");
     if (A->m < 0 || A->n < 0 || A->nnz < 0) return fail(TSG_ERR_ARG, "negative sizes");
     (void)hipSetDevice(ctx->device);
     memset(T, 0, sizeof *T);
     T->m = A->m; T->n = A->n; T->nnz = A->nnz; T->dtype = A->dtype; T->location = TSG_DEVICE;
-    T->tilem = (A->m + 15) / 16; T->tilen = (A->n + 15) / 16;
-    if (T->tilem > MAX_TILE_ROWS || T->tilen > MAX_TILE_ROWS) {
+    T->tile_size = ts == 16 ? 0 : ts;
+    T->tilem = (int32_t)(((int64_t)A->m + ts - 1) / ts); T->tilen = (int32_t)(((int64_t)A->n + ts - 1) / ts);
+    if ((int64_t)A->m > (int64_t)ts * MAX_TILE_ROWS || (int64_t)A->n > (int64_t)ts * MAX_TILE_ROWS) {
         const int32_t m = A->m, n = A->n;
         memset(T, 0, sizeof *T);
         return fail(TSG_ERR_OVERFLOW, "%d x %d: more than %lld tile-rows or tile-columns", m, n,
@@ -413,8 +428,12 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
     if (T->tilem == 0) {
         (void)hipMemsetAsync(T->tile_ptr, 0, sizeof(int64_t), s);
     } else {
-        k_c2t<0><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
-            A->rowptr, A->col, A->m, A->n, T->tilem, T->tile_ptr, nullptr, nullptr, nullptr, flags);
+        if (ts == 16)
+            k_c2t<0, 16><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
+                A->rowptr, A->col, A->m, A->n, T->tilem, T->tile_ptr, nullptr, nullptr, nullptr, flags, ts);
+        else
+            k_c2t<0, 0><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
+                A->rowptr, A->col, A->m, A->n, T->tilem, T->tile_ptr, nullptr, nullptr, nullptr, flags, ts);
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "k_c2t count launch"));
     }
     if ((rc = scan_inclusive_i64(ctx, T->tile_ptr, (int64_t)T->tilem + 1))) return cleanup(rc);
@@ -427,36 +446,52 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
     const int64_t nt = T->numtile;
     if ((rc = palloc(ctx, &T->tile_columnidx, (size_t)nt))) return cleanup(rc);
     if ((rc = palloc(ctx, &T->tile_nnz, (size_t)nt + 1))) return cleanup(rc);
-    if ((rc = palloc(ctx, &T->tile_csr_Ptr, (size_t)nt * 16))) return cleanup(rc);
-    if ((rc = palloc(ctx, &T->mask, (size_t)nt * 16))) return cleanup(rc);
+    if ((rc = palloc(ctx, &T->tile_csr_Ptr, (size_t)nt * ts))) return cleanup(rc);
+    if ((rc = palloc(ctx, &T->mask, (size_t)nt * ts))) return cleanup(rc);
     if ((rc = pool_alloc(ctx, &T->tile_csr_Val, (size_t)T->nnz * vsize(T->dtype)))) return cleanup(rc);
     if ((rc = palloc(ctx, &T->tile_csr_Col, (size_t)T->nnz))) return cleanup(rc);
     if ((rc = palloc(ctx, &T->val_perm, (size_t)T->nnz))) return cleanup(rc);
-    if (hipMemsetAsync(T->tile_csr_Ptr, 0, (size_t)nt * 16, s) != hipSuccess ||
-        hipMemsetAsync(T->mask, 0, (size_t)nt * 32, s) != hipSuccess)
+    if (hipMemsetAsync(T->tile_csr_Ptr, 0, (size_t)nt * ts, s) != hipSuccess ||
+        hipMemsetAsync(T->mask, 0, (size_t)nt * ts * 2, s) != hipSuccess)
         return cleanup(fail(TSG_ERR_HIP, "memset"));
     if (T->tilem > 0) {
-        k_c2t<1><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
-            A->rowptr, A->col, A->m, A->n, T->tilem, T->tile_ptr, T->tile_columnidx, T->tile_csr_Ptr, T->mask, flags);
+        if (ts == 16)
+            k_c2t<1, 16><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
+                A->rowptr, A->col, A->m, A->n, T->tilem, T->tile_ptr, T->tile_columnidx, T->tile_csr_Ptr, T->mask,
+                flags, ts);
+        else
+            k_c2t<1, 0><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
+                A->rowptr, A->col, A->m, A->n, T->tilem, T->tile_ptr, T->tile_columnidx, T->tile_csr_Ptr, T->mask,
+                flags, ts);
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "k_c2t fill launch"));
     }
     if (nt > 0) {
-        k_tile_rowscan<<<grid_capped(nt, 16), 256, 0, s>>>(nt, T->tile_csr_Ptr, T->tile_nnz);
+        k_tile_rowscan<<<grid_capped(nt, 16), 256, 0, s>>>(nt, T->tile_csr_Ptr, T->tile_nnz, ts);
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "rowscan launch"));
     } else {
         (void)hipMemsetAsync(T->tile_nnz, 0, sizeof(int64_t), s);
     }
     if ((rc = scan_inclusive_i64(ctx, T->tile_nnz, nt + 1))) return cleanup(rc);
     if (T->tilem > 0 && T->nnz > 0) {
-        if (T->dtype == TSG_F32)
-            k_c2t_scatter<float><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
-                A->rowptr, A->col, (const float *)A->val, A->m, T->tilem, T->tile_ptr, T->tile_columnidx,
-                T->tile_nnz, T->tile_csr_Ptr, (float *)T->tile_csr_Val, T->tile_csr_Col, T->val_perm);
-        else
-            k_c2t_scatter<double><<<blocks_for(T->tilem, CV_WAVES), CV_WAVES * WAVE, 0, s>>>(
-                A->rowptr, A->col, (const double *)A->val, A->m, T->tilem, T->tile_ptr, T->tile_columnidx,
-                T->tile_nnz, T->tile_csr_Ptr, (double *)T->tile_csr_Val, T->tile_csr_Col, T->val_perm);
+        const unsigned g = blocks_for(T->tilem, CV_WAVES);
+        if (T->dtype == TSG_F32) {
+            auto k = ts == 16 ? k_c2t_scatter<float, 16> : k_c2t_scatter<float, 0>;
+            k<<<g, CV_WAVES * WAVE, 0, s>>>(A->rowptr, A->col, (const float *)A->val, A->m, T->tilem, T->tile_ptr,
+                                            T->tile_columnidx, T->tile_nnz, T->tile_csr_Ptr,
+                                            (float *)T->tile_csr_Val, T->tile_csr_Col, T->val_perm, ts);
+        } else {
+            auto k = ts == 16 ? k_c2t_scatter<double, 16> : k_c2t_scatter<double, 0>;
+            k<<<g, CV_WAVES * WAVE, 0, s>>>(A->rowptr, A->col, (const double *)A->val, A->m, T->tilem, T->tile_ptr,
+                                            T->tile_columnidx, T->tile_nnz, T->tile_csr_Ptr,
+                                            (double *)T->tile_csr_Val, T->tile_csr_Col, T->val_perm, ts);
+        }
         if (hipGetLastError() != hipSuccess) return cleanup(fail(TSG_ERR_HIP, "scatter launch"));
+    }
+    if (ts != 16) {  // no scalar CSR copy: no product runs at this tile size
+        if ((rc = read_i32(ctx, flags, &hf))) return cleanup(rc);
+        T->flags = (hf & 2) ? 1 : 0;
+        pool_free(ctx, flags);
+        return TSG_OK;
     }
     // keep the scalar CSR beside the tiles (row-by-row product of sparse heavy rows)
     const size_t vb = (size_t)T->nnz * vsize(T->dtype);
@@ -476,6 +511,20 @@ extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T
     return TSG_OK;
 }
 
+extern "C" int tsg_csr2tile(tsg_context *c, const tsg_csr *A, tsg_tile_matrix *T) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !A || !T) return fail(TSG_ERR_ARG, "null argument");
+    return csr2tile_impl(ctx, A, T, 16);
+}
+
+extern "C" int tsg_csr2tile_ts(tsg_context *c, const tsg_csr *A, int32_t ts, tsg_tile_matrix *T) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !A || !T) return fail(TSG_ERR_ARG, "null argument");
+    if (ts < 1 || ts > 16)
+        return fail(TSG_ERR_ARG, "tile size %d outside 1-16 (uint16 row masks and the (row<<4)|col byte)", ts);
+    return csr2tile_impl(ctx, A, T, ts);
+}
+
 extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     Context *ctx = reinterpret_cast<Context *>(c);
     if (!ctx || !B) return fail(TSG_ERR_ARG, "null argument");
@@ -484,6 +533,7 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     (void)hipSetDevice(ctx->device);
     hipStream_t s = ctx->stream;
     const int64_t nt = B->numtile;
+    const int ts = B->tile_size ? B->tile_size : 16;  // build_csc_tiles_ts (src/tile_format.h:327-417) when not 16
     int32_t *rowidx = B->tile_rowidx, *tmp = nullptr, *crow = nullptr;
     int64_t *cptr = nullptr, *cursor = nullptr, *inv = nullptr, *nnz_new = nullptr, *pos_of = nullptr;
     uint8_t *ptr_new = nullptr, *col_new = nullptr;
@@ -506,8 +556,8 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     if (!rc) rc = palloc(ctx, &inv, (size_t)nt);
     if (!rc) rc = palloc(ctx, &pos_of, (size_t)nt);
     if (!rc) rc = palloc(ctx, &nnz_new, (size_t)nt + 1);
-    if (!rc) rc = palloc(ctx, &ptr_new, (size_t)nt * 16);
-    if (!rc) rc = palloc(ctx, &mask_new, (size_t)nt * 16);
+    if (!rc) rc = palloc(ctx, &ptr_new, (size_t)nt * ts);
+    if (!rc) rc = palloc(ctx, &mask_new, (size_t)nt * ts);
     if (!rc) rc = palloc(ctx, &col_new, (size_t)B->nnz);
     if (!rc) rc = pool_alloc(ctx, &val_new, (size_t)B->nnz * vsize(B->dtype));
     if (!rc && hipMemsetAsync(cptr, 0, sizeof(int64_t) * ((size_t)B->tilen + 1), s) != hipSuccess)
@@ -533,11 +583,11 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
         if (B->dtype == TSG_F32)
             k_csc_gather<float><<<grid_capped(nt, 16), 256, 0, s>>>(
                 nt, inv, B->tile_nnz, nnz_new, B->tile_csr_Ptr, B->mask, B->tile_csr_Col,
-                (const float *)B->tile_csr_Val, ptr_new, mask_new, col_new, (float *)val_new);
+                (const float *)B->tile_csr_Val, ptr_new, mask_new, col_new, (float *)val_new, ts);
         else
             k_csc_gather<double><<<grid_capped(nt, 16), 256, 0, s>>>(
                 nt, inv, B->tile_nnz, nnz_new, B->tile_csr_Ptr, B->mask, B->tile_csr_Col,
-                (const double *)B->tile_csr_Val, ptr_new, mask_new, col_new, (double *)val_new);
+                (const double *)B->tile_csr_Val, ptr_new, mask_new, col_new, (double *)val_new, ts);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "csc gather launch");
     }
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(TSG_ERR_HIP, "build_csc_tiles: %s", hipGetErrorString(hipGetLastError()));
@@ -553,8 +603,8 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     // keep the CSR-tile-order data for the row-staged kernels; the packed col
     // bytes are implied by the masks there and are dropped
     int64_t *rnp = nullptr;
-    rc = palloc(ctx, &rnp, (size_t)B->tilem + 1);
-    if (!rc) {
+    if (ts == 16) rc = palloc(ctx, &rnp, (size_t)B->tilem + 1);
+    if (!rc && ts == 16) {
         k_row_nnz_ptr<<<blocks_for((int64_t)B->tilem + 1, 256), 256, 0, s>>>(B->tilem, B->tile_ptr, B->tile_nnz, rnp);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "row_nnz_ptr launch");
     }
@@ -569,7 +619,7 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     uint16_t *ccm = nullptr;
     uint8_t *ccp = nullptr;
     void *ccv = nullptr;
-    if (!(B->flags & 1)) {
+    if (!(B->flags & 1) && ts == 16) {
         rc = palloc(ctx, &ccm, (size_t)nt * 16);
         if (!rc) rc = palloc(ctx, &ccp, (size_t)nt * 16);
         if (!rc) rc = pool_alloc(ctx, &ccv, (size_t)B->nnz * vsize(B->dtype));
@@ -616,8 +666,13 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     pool_free(ctx, B->tile_csr_Col);
     pool_free(ctx, B->tile_csr_Ptr);
     pool_free(ctx, B->tile_csr_Val);
-    B->csr_tile_nnz = B->tile_nnz;
-    B->csr_mask = B->mask;
+    if (ts == 16) {
+        B->csr_tile_nnz = B->tile_nnz;
+        B->csr_mask = B->mask;
+    } else {  // the row-staged kernels (the only readers of the CSR-order copy) need 16-row tiles
+        pool_free(ctx, B->tile_nnz);
+        pool_free(ctx, B->mask);
+    }
     B->csr_col_mask = ccm;
     B->csr_col_Ptr = ccp;
     B->csr_col_Val = ccv;
@@ -721,6 +776,7 @@ extern "C" int tsg_build_tile_csc(tsg_context *c, tsg_tile_matrix *T) {
     if (!ctx || !T) return fail(TSG_ERR_ARG, "null argument");
     if (T->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles must be on the device");
     if (!T->mask || !T->tile_csr_Ptr || !T->tile_nnz) return fail(TSG_ERR_ARG, "tiles lack masks or row offsets");
+    if (T->tile_size && T->tile_size != 16) return fail(TSG_ERR_UNSUPPORTED, "within-tile CSC needs 16 x 16 tiles");
     (void)hipSetDevice(ctx->device);
     uint8_t *cptr = nullptr, *crow = nullptr;
     void *cval = nullptr;
@@ -812,6 +868,7 @@ extern "C" int tsg_tile2csr(tsg_context *c, const tsg_tile_matrix *T, tsg_csr *o
     if (!ctx || !T || !o) return fail(TSG_ERR_ARG, "null argument");
     if (T->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles must be on the device");
     if (T->is_csc) return fail(TSG_ERR_ARG, "tile2csr needs row-tile order (is_csc = 0)");
+    if (T->tile_size && T->tile_size != 16) return fail(TSG_ERR_UNSUPPORTED, "tile2csr needs 16 x 16 tiles");
     (void)hipSetDevice(ctx->device);
     memset(o, 0, sizeof *o);
     o->m = T->m; o->n = T->n; o->nnz = T->nnz; o->dtype = T->dtype; o->location = TSG_DEVICE;

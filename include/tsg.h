@@ -111,6 +111,13 @@ typedef struct tsg_tile_matrix {
     uint8_t *tile_csc_Ptr;    /* [nt*16] exclusive column offsets inside each tile */
     uint8_t *tile_csc_Row;    /* [nnz] local row of each entry */
     void *tile_csc_Val;       /* [nnz] values in that order */
+    /* Rows and columns per tile: 0 = 16 (TILE_SIZE, src/tile_format.h:11),
+     * else 1-15 for tiles made by tsg_csr2tile_ts.  Per-tile row arrays
+     * (tile_csr_Ptr, mask) then hold tile_size entries per tile, and only the
+     * conversion entry points (tsg_build_csc_tiles, transfers, free) accept
+     * such a matrix. */
+    int32_t tile_size;
+    int32_t reserved_ts;
 } tsg_tile_matrix;
 
 typedef struct tsg_context tsg_context;
@@ -211,6 +218,12 @@ int tsg_build_csc_tiles(tsg_context *ctx, tsg_tile_matrix *B_dev);
  * every value copy (tiles, column-major copy, scalar CSR) is rewritten.  No
  * reference counterpart (the reference rebuilds the tiles on the host). */
 int tsg_tile_set_values(tsg_context *ctx, tsg_tile_matrix *dev, const void *vals);
+/* CSR -> tiles of ts x ts (1 <= ts <= 16) on the device; replaces csr2tile_ts
+ * (src/tile_format.h:225-324): mask bit ts-1-c is local column c, col byte
+ * (row<<4)|col, ts row offsets and masks per tile.  ts = 16 is tsg_csr2tile.
+ * tsg_build_csc_tiles on the result replaces build_csc_tiles_ts (:327-417).
+ * No product runs on ts != 16 (neither does the reference's). */
+int tsg_csr2tile_ts(tsg_context *ctx, const tsg_csr *csr_dev, int32_t ts, tsg_tile_matrix *out_dev);
 /* Within-tile CSC of every tile (tile_csc_Ptr/Row/Val above), on the
  * device; replaces the within-tile transpose of build_csc_tiles
  * (src/tile_format.h:547-584) for a B operand and build_a_tile_csc

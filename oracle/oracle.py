@@ -83,7 +83,7 @@ def _arr(ptr, n, dt):
     return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(n,)).copy()
 
 
-def tile_to_dict(t, free_fn):
+def tile_to_dict(t, free_fn, rows=16):
     nt = t.numtile_csc if t.is_csc else t.numtile
     d = dict(m=t.m, n=t.n, tilem=t.tilem, tilen=t.tilen, nnz=t.nnz, numtile=t.numtile,
              is_csc=t.is_csc, dtype=1 if t.is_f32 else 0,
@@ -91,8 +91,8 @@ def tile_to_dict(t, free_fn):
              tile_columnidx=_arr(t.tile_colidx, t.numtile, np.int32),
              tile_rowidx=_arr(t.tile_rowidx, t.numtile, np.int32) if t.tile_rowidx else None,
              tile_nnz=_arr(t.tile_nnz, nt + 1, np.int64),
-             tile_csr_Ptr=_arr(t.tile_ptrRow, nt * 16, np.uint8),
-             mask=_arr(t.mask, nt * 16, np.uint16),
+             tile_csr_Ptr=_arr(t.tile_ptrRow, nt * rows, np.uint8),
+             mask=_arr(t.mask, nt * rows, np.uint16),
              tile_csr_Col=_arr(t.tile_col, t.nnz, np.uint8),
              tile_csr_Val=(_arr(t.val32, t.nnz, np.float32) if t.is_f32 else _arr(t.val64, t.nnz, np.float64)))
     if t.is_csc:
@@ -249,6 +249,59 @@ def load_mtx(path):
     return m.value, n.value, rp, col[:nnz.value], val[:nnz.value]
 
 
+def csr2tile_ts(csr, ts, as_b=False, f32=False):
+    """csr2tile_ts (src/tile_format.h:225-324) and, for as_b, build_csc_tiles_ts
+    (:327-417): tiles of ts x ts (1 <= ts <= 16), restated with numpy.
+
+    Tiles are the distinct (row // ts, col // ts) pairs, ascending tile column
+    within a tile-row; a tile's entries go row by row, each row in CSR storage
+    order (duplicates kept), col byte (row<<4)|col, mask bit ts-1-col, row
+    offsets the exclusive prefix of the row counts (unsigned char).  A B
+    operand keeps tile_ptr / tile_columnidx and re-orders the per-tile data by
+    tile column then tile row (csc_tile_ptr / csc_tile_rowidx) with the col
+    byte reduced to the local column."""
+    m, n = int(csr.m), int(csr.n)
+    rp = np.asarray(csr.rowptr, np.int64)
+    col = np.asarray(csr.col, np.int64)
+    val = np.asarray(csr.val, np.float32 if f32 else np.float64)
+    nnz = len(col)
+    tilem, tilen = (m + ts - 1) // ts, (n + ts - 1) // ts
+    row = np.repeat(np.arange(m, dtype=np.int64), np.diff(rp))
+    bi, lr, tc, lc = row // ts, row % ts, col // ts, col % ts
+    key = bi * max(tilen, 1) + tc
+    tiles = np.unique(key)
+    nt = len(tiles)
+    tid = np.searchsorted(tiles, key)
+    trow, tcol = tiles // max(tilen, 1), tiles % max(tilen, 1)
+    tile_ptr = np.zeros(tilem + 1, np.int64)
+    np.add.at(tile_ptr, trow + 1, 1)
+    tile_ptr = np.cumsum(tile_ptr)
+    cnt = np.bincount(tid * ts + lr, minlength=nt * ts).reshape(nt, ts)
+    ptr = ((np.cumsum(cnt, axis=1) - cnt) & 0xFF).astype(np.uint8)
+    tnz = np.zeros(nt + 1, np.int64)
+    tnz[1:] = np.cumsum(cnt.sum(axis=1))
+    order = np.lexsort((np.arange(nnz), lr, tid))
+    mask = np.zeros(nt * ts, np.int64)
+    np.bitwise_or.at(mask, tid * ts + lr, np.left_shift(1, ts - 1 - lc))
+    d = dict(m=m, n=n, tilem=tilem, tilen=tilen, nnz=nnz, numtile=nt, is_csc=0, dtype=1 if f32 else 0,
+             tile_ptr=tile_ptr, tile_columnidx=tcol.astype(np.int32), tile_rowidx=None, tile_nnz=tnz,
+             tile_csr_Ptr=ptr.reshape(-1), mask=mask.astype(np.uint16),
+             tile_csr_Col=((lr << 4) | lc)[order].astype(np.uint8), tile_csr_Val=val[order])
+    if not as_b:
+        return d
+    perm = np.lexsort((trow, tcol))               # CSC tile order: tile column, then tile row
+    cp = np.zeros(tilen + 1, np.int64)
+    np.add.at(cp, tcol + 1, 1)
+    starts, sizes = tnz[:-1][perm], np.diff(tnz)[perm]
+    gather = np.repeat(starts - np.r_[0, np.cumsum(sizes)[:-1]], sizes) + np.arange(nnz)
+    nz_new = np.zeros(nt + 1, np.int64)
+    nz_new[1:] = np.cumsum(sizes)
+    d.update(is_csc=1, csc_tile_ptr=np.cumsum(cp), csc_tile_rowidx=trow[perm].astype(np.int32), tile_nnz=nz_new,
+             tile_csr_Ptr=ptr[perm].reshape(-1), mask=d["mask"].reshape(nt, ts)[perm].reshape(-1),
+             tile_csr_Col=(d["tile_csr_Col"][gather] & 15).astype(np.uint8), tile_csr_Val=d["tile_csr_Val"][gather])
+    return d
+
+
 def tile_csc(d):
     """Within-tile CSC of every tile of a tile dict (the download() layout, data
     arrays in their own tile order): the reference's tile_csc_Ptr/Row/Val of a
@@ -284,6 +337,15 @@ def ref_csr2tile(csr, as_b=False):
     fn = ref().ref_csr2tile_csc if as_b else ref().ref_csr2tile
     fn(csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double), C.byref(t))
     return tile_to_dict(t, ref().ref_tile_free)
+
+
+def ref_csr2tile_ts(csr, ts, as_b=False):
+    """The reference's own csr2tile_ts (+ build_csc_tiles_ts) compiled from its sources."""
+    rp, col, val = _csr_args(csr)
+    t = OraTile()
+    ref().ref_csr2tile_ts(csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double), int(ts),
+                          1 if as_b else 0, C.byref(t))
+    return tile_to_dict(t, ref().ref_tile_free, rows=ts)
 
 
 def ref_tile_csc(csr, as_b=False):

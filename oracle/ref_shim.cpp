@@ -35,8 +35,9 @@ static HostCSR make_csr(int32_t m, int32_t n, const int64_t *rp, const int32_t *
     return h;
 }
 
-// Copy a reference TileMatrix into an ora_tile (values as f64).
-static void to_ora(const TileMatrix &T, bool is_B, ora_tile *o) {
+// Copy a reference TileMatrix into an ora_tile (values as f64); rows = the
+// tile size (per-tile entries of tile_csr_Ptr and mask).
+static void to_ora(const TileMatrix &T, bool is_B, ora_tile *o, int rows = TILE_SIZE) {
     memset(o, 0, sizeof *o);
     o->m = T.m; o->n = T.n; o->tilem = T.tilem; o->tilen = T.tilen;
     o->numtile = T.numtile;
@@ -45,8 +46,8 @@ static void to_ora(const TileMatrix &T, bool is_B, ora_tile *o) {
     o->tile_ptr = widen(T.tile_ptr, T.tilem + 1);
     o->tile_colidx = dup(T.tile_columnidx, T.numtile);
     o->tile_nnz = widen(T.tile_nnz, nt + 1);
-    o->tile_ptrRow = dup(T.tile_csr_Ptr, (size_t)nt * TILE_SIZE);
-    o->mask = dup(T.mask, (size_t)nt * TILE_SIZE);
+    o->tile_ptrRow = dup(T.tile_csr_Ptr, (size_t)nt * rows);
+    o->mask = dup(T.mask, (size_t)nt * rows);
     o->tile_col = dup(T.tile_csr_Col, (size_t)o->nnz);
     o->val64 = dup(T.tile_csr_Val, (size_t)o->nnz);
     if (is_B) {
@@ -74,6 +75,18 @@ int ref_csr2tile_csc(int32_t m, int32_t n, const int64_t *rp, const int32_t *col
     TileMatrix T = csr2tile(h);
     build_csc_tiles(T, h);
     to_ora(T, true, out);
+    free_tile_matrix(T);
+    return 0;
+}
+
+// The reference's csr2tile_ts (tile_format.h:225-324), and for as_b its
+// build_csc_tiles_ts (:327-417), at tile size ts.
+int ref_csr2tile_ts(int32_t m, int32_t n, const int64_t *rp, const int32_t *col, const double *val, int32_t ts,
+                    int32_t as_b, ora_tile *out) {
+    HostCSR h = make_csr(m, n, rp, col, val);
+    TileMatrix T = csr2tile_ts(h, ts);
+    if (as_b) build_csc_tiles_ts(T, h, ts);
+    to_ora(T, as_b != 0, out, ts);
     free_tile_matrix(T);
     return 0;
 }

@@ -1,7 +1,8 @@
 """Writes the golden fixtures for the tile-format conversion and the MTX reader.
 
 Expected outputs come from the reference's own host code (src/tile_format.h:
-load_mtx, csr2tile, build_csc_tiles with its within-tile CSC, build_a_tile_csc) compiled from the reference checkout into
+load_mtx, csr2tile, build_csc_tiles with its within-tile CSC, build_a_tile_csc,
+csr2tile_ts and build_csc_tiles_ts) compiled from the reference checkout into
 oracle/_ref/libref.so by oracle/Makefile; inputs are small synthetic matrices
 and hand-written .mtx files.  Run in a container that has the reference
 checkout:  make -C oracle && python tests/golden/make_golden.py
@@ -71,6 +72,8 @@ def cases():
     return out
 
 
+TS_SIZES = (1, 5, 8, 13)
+
 MTX = {
     "sym.mtx": "%%MatrixMarket matrix coordinate real symmetric\n% comment\n5 5 6\n1 1 2.5\n2 1 -1\n3 2 0.5\n4 4 7\n5 1 3\n5 5 1\n",
     "pattern.mtx": "%%MatrixMarket matrix coordinate pattern general\n4 6 5\n1 6\n2 2\n4 1\n3 3\n1 2\n",
@@ -95,6 +98,13 @@ def main():
             # within-tile CSC: build_csc_tiles (:547-584) for B, build_a_tile_csc (:589-627) for A
             for k, v in O.ref_tile_csc(A, as_b=as_b).items():
                 data[f"{name}/{tag}csc/{k}"] = v
+            # parameterized tile sizes: csr2tile_ts (:225-324), build_csc_tiles_ts (:327-417)
+            for ts in TS_SIZES:
+                for k, v in O.ref_csr2tile_ts(A, ts, as_b=as_b).items():
+                    if isinstance(v, np.ndarray):
+                        data[f"{name}/{tag}ts{ts}/{k}"] = v
+                    elif v is not None:
+                        data[f"{name}/{tag}ts{ts}/{k}"] = np.array(v)
     for fname, text in MTX.items():
         p = os.path.join(HERE, fname)
         with open(p, "w") as f:

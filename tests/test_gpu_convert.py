@@ -117,6 +117,68 @@ def test_gpu_tile_csc_duplicates_refresh_and_relayout(ctx):
     assert len(z["tile_csc_Ptr"]) == 0 and len(z["tile_csc_Row"]) == 0
 
 
+TS_SIZES = (1, 5, 8, 13)
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("ts", TS_SIZES)
+def test_gpu_csr2tile_ts_matches_reference_golden(ctx, name, ts):
+    """tsg_csr2tile_ts (+ tsg_build_csc_tiles on it) against the reference's
+    csr2tile_ts / build_csc_tiles_ts output."""
+    print(f"[case] csr2tile_ts golden {name} ts={ts}", flush=True)
+    A, _ = gold(name)
+    z = np.load(GOLD, allow_pickle=False)
+    for tag, as_b in (("A", False), ("B", True)):
+        exp = {k.split("/")[-1]: (z[k] if z[k].ndim else z[k].item())
+               for k in z.files if k.startswith(f"{name}/{tag}ts{ts}/")}
+        got = ctx.tiles_from_csr(A, as_b=as_b, ts=ts).download()
+        assert got["tile_size"] == ts
+        assert_tiles_equal(got, exp, label=f"{tag} ts{ts} ")
+
+
+@pytest.mark.parametrize("name", ["er4096", "lap2d_64x48", "rmat10", "er_var"])
+@pytest.mark.parametrize("ts", [2, 3, 7, 12, 15, 16])
+def test_gpu_csr2tile_ts_matches_oracle(ctx, name, ts):
+    print(f"[case] csr2tile_ts {name} ts={ts}", flush=True)
+    A = _matrices()[name]()
+    f32 = ts % 2 == 1
+    h = A.astype(g.F32) if f32 else A
+    assert_tiles_equal(ctx.tiles_from_csr(h, ts=ts).download(), O.csr2tile_ts(A, ts, f32=f32), label="A ")
+    assert_tiles_equal(ctx.tiles_from_csr(h, as_b=True, ts=ts).download(), O.csr2tile_ts(A, ts, as_b=True, f32=f32),
+                       label="B ")
+
+
+def test_gpu_tile_size_limits(ctx):
+    """Duplicates keep storage order at every tile size; products and the
+    16-row-only conversions reject other tile sizes; ts outside 1-16 is an error."""
+    print("[case] tile size limits", flush=True)
+    rp = np.array([0, 3, 5, 6, 8, 8, 8, 8, 9, 12], dtype=np.int64)
+    col = np.array([1, 1, 7, 0, 3, 2, 2, 3, 6, 0, 0, 0], dtype=np.int32)
+    D = g.HostCSR(9, 8, rp, col, np.arange(1.0, 13.0))
+    for ts in (3, 4, 9):
+        t = ctx.tiles_from_csr(D, ts=ts)
+        assert t.s.flags & 1
+        assert_tiles_equal(t.download(), O.csr2tile_ts(D, ts), label=f"dups ts{ts} A ")
+        assert_tiles_equal(ctx.tiles_from_csr(D, as_b=True, ts=ts).download(), O.csr2tile_ts(D, ts, as_b=True),
+                           label=f"dups ts{ts} B ")
+    A = g.gen_er(200, 5, seed=3)
+    t8, b8 = ctx.tiles_from_csr(A, ts=8), ctx.tiles_from_csr(A, as_b=True, ts=8)
+    with pytest.raises(g.TsgError, match="16 x 16"):
+        ctx.spgemm(t8, b8)
+    with pytest.raises(g.TsgError, match="16 x 16"):
+        t8.build_tile_csc()
+    with pytest.raises(g.TsgError, match="16 x 16"):
+        t8.to_csr()
+    for bad in (0, 17, -1):
+        with pytest.raises(g.TsgError, match="tile size"):
+            ctx.tiles_from_csr(A, ts=bad)
+    # host round trip keeps the tile size
+    d = t8.download()
+    back = g.upload_tiles(ctx, d).download()
+    assert back["tile_size"] == 8
+    assert_tiles_equal(back, d, label="round trip ")
+
+
 def test_gpu_conversion_wide_rows_take_several_windows(ctx):
     # tile columns span more than one LDS window (32,768 tile columns per pass)
     n = 16 * 70000 + 5

@@ -196,6 +196,64 @@ def test_planted_csc_row_order_is_caught():
     assert not csc_vs_transpose(A, bad)
 
 
+TS_SIZES = (1, 5, 8, 13)
+
+
+def gold_ts(z, name, tag, ts):
+    return {k.split("/")[-1]: (z[k] if z[k].ndim else z[k].item())
+            for k in z.files if k.startswith(f"{name}/{tag}ts{ts}/")}
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("ts", TS_SIZES)
+def test_csr2tile_ts_matches_reference_golden(name, ts):
+    """csr2tile_ts (:225-324) and build_csc_tiles_ts (:327-417) against the
+    reference's own output."""
+    z = golden()
+    A, _ = gold_case(z, name)
+    assert_tiles_equal(O.csr2tile_ts(A, ts), gold_ts(z, name, "A", ts), label=f"{name} ts{ts} A ")
+    assert_tiles_equal(O.csr2tile_ts(A, ts, as_b=True), gold_ts(z, name, "B", ts), label=f"{name} ts{ts} B ")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_csr2tile_ts_matches_reference_live(seed):
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built (reference checkout absent)")
+    rng = np.random.default_rng(200 + seed)
+    m, n = int(rng.integers(1, 250)), int(rng.integers(1, 250))
+    A = rand_csr(m, n, float(rng.uniform(0.01, 0.25)), seed)
+    if seed % 2:
+        A = with_duplicates(A, seed)
+    for ts in (2, 3, 7, 11, 15, 16):
+        assert_tiles_equal(O.csr2tile_ts(A, ts), O.ref_csr2tile_ts(A, ts), label=f"ts{ts} A ")
+        assert_tiles_equal(O.csr2tile_ts(A, ts, as_b=True), O.ref_csr2tile_ts(A, ts, as_b=True), label=f"ts{ts} B ")
+
+
+def test_csr2tile_ts_closed_forms():
+    """ts = 16 is csr2tile / build_csc_tiles; for every ts the tiles hold exactly
+    the CSR's entries: row = tile row * ts + (col byte >> 4), column = tile
+    column * ts + (col byte & 15), each row's entries in storage order."""
+    A = rand_csr(97, 83, 0.15, 11)
+    assert_tiles_equal(O.csr2tile_ts(A, 16), O.csr2tile(A))
+    assert_tiles_equal(O.csr2tile_ts(A, 16, as_b=True), O.csr2tile_b(A))
+    S = A.to_scipy()
+    for ts in range(1, 17):
+        d = O.csr2tile_ts(A, ts)
+        trow = np.repeat(np.arange(d["tilem"]), np.diff(d["tile_ptr"]))
+        per_tile = np.diff(d["tile_nnz"])
+        r = np.repeat(trow, per_tile) * ts + (d["tile_csr_Col"] >> 4)
+        c = np.repeat(d["tile_columnidx"], per_tile) * ts + (d["tile_csr_Col"] & 15)
+        order = np.lexsort((c, r))
+        R = sp.csr_matrix((d["tile_csr_Val"][order], (r[order], c[order])), shape=S.shape)
+        assert (R != S).nnz == 0, f"ts={ts}"
+        # masks: bit ts-1-c for every stored column; row offsets: exclusive prefix
+        ptr = d["tile_csr_Ptr"].reshape(-1, ts).astype(np.int64)
+        assert np.all(ptr[:, 0] == 0) and np.all(np.diff(ptr, axis=1) >= 0)
+        pop = np.array([bin(int(x)).count("1") for x in d["mask"]]).reshape(-1, ts)
+        assert np.array_equal(pop.sum(axis=1), per_tile)
+        assert int(d["mask"].max(initial=0)) < (1 << ts)
+
+
 @pytest.mark.parametrize("name", ["sym", "pattern", "complex"])
 def test_load_mtx_matches_reference_golden(name):
     z = golden()
