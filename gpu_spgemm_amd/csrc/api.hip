@@ -58,9 +58,37 @@ int pool_alloc(Context *ctx, void **p, size_t bytes) {
     return TSG_OK;
 }
 
+int pool_view(Context *ctx, void *block, size_t offset, void **out) {
+    *out = nullptr;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->pool.live.find(block);
+    if (it == ctx->pool.live.end()) return fail(TSG_ERR_ARG, "view of a pointer %p not owned by this context", block);
+    if (offset >= it->second) return fail(TSG_ERR_ARG, "view offset %zu beyond a block of %zu bytes", offset, it->second);
+    auto r = ctx->pool.refs.emplace(block, 1);  // the owner's reference
+    r.first->second++;
+    *out = static_cast<uint8_t *>(block) + offset;
+    ctx->pool.views.emplace(*out, block);
+    return TSG_OK;
+}
+
 int pool_free(Context *ctx, void *p) {
     if (!p) return TSG_OK;
     std::lock_guard<std::mutex> lk(ctx->mu);
+    // a view, or the owner of a block with views: drop one reference
+    void *blk = nullptr;
+    auto v = ctx->pool.views.find(p);
+    if (v != ctx->pool.views.end()) {
+        blk = v->second;
+        ctx->pool.views.erase(v);
+    } else if (ctx->pool.refs.count(p)) {
+        blk = p;
+    }
+    if (blk) {
+        auto r = ctx->pool.refs.find(blk);
+        if (--r->second > 0) return TSG_OK;
+        ctx->pool.refs.erase(r);
+        p = blk;
+    }
     auto it = ctx->pool.live.find(p);
     if (it == ctx->pool.live.end()) return fail(TSG_ERR_ARG, "pointer %p not owned by this context", p);
     size_t r = it->second;

@@ -8,9 +8,12 @@
 // a tile boundary, so partial edge tiles stay inside their own block), the
 // binned pipeline runs once over all tile-rows of all problems (one launch
 // per kernel variant for the whole batch, rows of every problem binned
-// together), and one split launch copies every C_i out with its offsets
-// removed.  Host syncs per batch: those of one product plus one read of the
-// per-problem boundaries, instead of those of n products.
+// together), and one split launch copies every C_i's tile offsets and
+// indices out with the batch offsets removed; its masks, row offsets, values
+// and col bytes stay where the product wrote them (pool views into C's
+// arrays, recycled when the last of them is freed).  Host syncs per batch:
+// those of one product plus one read of the per-problem boundaries, instead
+// of those of n products.
 //
 // Results are bit-identical to n separate calls: every kernel computes each
 // C tile from the same A and B tiles in the same order.
@@ -294,14 +297,27 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
         c.dtype = dtype; c.location = TSG_DEVICE;
         c.numtile = hb[2 * i + 2] - hb[2 * i];
         c.nnz = hb[2 * i + 3] - hb[2 * i + 1];
+        // offsets are rebased in copies; masks, row offsets, values and col
+        // bytes stay where the product wrote them (views into C's arrays)
+        const int64_t t0 = hb[2 * i], v0 = hb[2 * i + 1];
         rc = palloc(ctx, &c.tile_ptr, (size_t)c.tilem + 1);
         if (!rc) rc = palloc(ctx, &c.tile_rowidx, (size_t)c.numtile);
         if (!rc) rc = palloc(ctx, &c.tile_columnidx, (size_t)c.numtile);
         if (!rc) rc = palloc(ctx, &c.tile_nnz, (size_t)c.numtile + 1);
-        if (!rc) rc = palloc(ctx, &c.mask, (size_t)c.numtile * 16);
-        if (!rc) rc = palloc(ctx, &c.tile_csr_Ptr, (size_t)c.numtile * 16);
-        if (!rc) rc = pool_alloc(ctx, &c.tile_csr_Val, (size_t)c.nnz * vs);
-        if (!rc) rc = palloc(ctx, &c.tile_csr_Col, (size_t)c.nnz);
+        if (!rc && c.numtile > 0) {
+            rc = pool_view(ctx, BC.mask, (size_t)t0 * 32, reinterpret_cast<void **>(&c.mask));
+            if (!rc) rc = pool_view(ctx, BC.tile_csr_Ptr, (size_t)t0 * 16, reinterpret_cast<void **>(&c.tile_csr_Ptr));
+        } else if (!rc) {
+            rc = palloc(ctx, &c.mask, 16);
+            if (!rc) rc = palloc(ctx, &c.tile_csr_Ptr, 16);
+        }
+        if (!rc && c.nnz > 0) {
+            rc = pool_view(ctx, BC.tile_csr_Val, (size_t)v0 * vs, &c.tile_csr_Val);
+            if (!rc) rc = pool_view(ctx, BC.tile_csr_Col, (size_t)v0, reinterpret_cast<void **>(&c.tile_csr_Col));
+        } else if (!rc) {
+            rc = pool_alloc(ctx, &c.tile_csr_Val, vs);
+            if (!rc) rc = palloc(ctx, &c.tile_csr_Col, 1);
+        }
     }
     if (!rc) {
         JobList sj;
@@ -312,17 +328,11 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
             sj.add(BC.tile_rowidx + t0, c.tile_rowidx, c.numtile, 2, -R[i]);
             sj.add(BC.tile_columnidx + t0, c.tile_columnidx, c.numtile, 2, -Cc[i]);
             sj.add(BC.tile_nnz + t0, c.tile_nnz, c.numtile + 1, 3, -v0);
-            sj.add(BC.mask + t0 * 16, c.mask, c.numtile * 2, 1);
-            sj.add(BC.tile_csr_Ptr + t0 * 16, c.tile_csr_Ptr, c.numtile, 1);
-            sj.add((uint8_t *)BC.tile_csr_Val + v0 * vs, c.tile_csr_Val, c.nnz, vk);
-            sj.add(BC.tile_csr_Col + v0, c.tile_csr_Col, c.nnz, 0);
         }
         rc = run_jobs(ctx, sj);
     }
     if (!rc && keep) {  // keep the operands, the product and the value copies for re-runs
-        JobList so;
-        for (int32_t i = 0; i < n; i++)
-            so.add((uint8_t *)BC.tile_csr_Val + hb[2 * i + 1] * vs, C[i].tile_csr_Val, C[i].nnz, vk);
+        JobList so;  // empty: each C_i's values are a view of the product's, which re-runs rewrite
         rc = upload_jobs(ctx, jv, &keep->vin);
         if (!rc) rc = upload_jobs(ctx, so, &keep->vout);
         if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(TSG_ERR_HIP, "job table upload failed");

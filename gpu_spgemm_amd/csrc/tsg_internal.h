@@ -39,9 +39,14 @@ int fail(int code, const char *fmt, ...);
 // ---------------------------------------------------------------- device pool
 // Caching allocator: freed blocks are kept per exact rounded size and reused,
 // so repeated products of one shape make no hipMalloc after the first.
+// A view is an address inside a live block handed out as an array of its
+// own (a batch's outputs lie inside the one product that computed them);
+// the block is recycled when its owner and every view have been freed.
 struct Pool {
     std::multimap<size_t, void *> free_blocks;
     std::map<void *, size_t> live;
+    std::multimap<void *, void *> views;  // view address -> the block it lies in
+    std::map<void *, int64_t> refs;       // block with views -> holders (owner + views)
     int64_t in_use = 0, cached = 0;
     static size_t round(size_t b) {
         if (b < 256) return 256;
@@ -69,6 +74,9 @@ struct Context {
 
 int pool_alloc(Context *ctx, void **p, size_t bytes);
 int pool_free(Context *ctx, void *p);
+// *out = block + offset (offset < the block's size), freed with pool_free like
+// any array; the block stays allocated until its owner and all views are freed.
+int pool_view(Context *ctx, void *block, size_t offset, void **out);
 template <typename T>
 inline int palloc(Context *ctx, T **p, size_t count) {
     return pool_alloc(ctx, reinterpret_cast<void **>(p), count * sizeof(T));

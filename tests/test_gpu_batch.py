@@ -95,6 +95,43 @@ def test_batched_with_duplicates_is_exact(ctx):
                                                         "tile_csr_Col", "tile_csr_Val"], label=f"{i} ")
 
 
+@pytest.mark.parametrize("order", ["forward", "reverse", "interleaved"])
+def test_batched_outputs_share_the_product_memory(ctx, order):
+    """Each C_i's masks, row offsets, values and col bytes are views into the
+    batch product's arrays: freeing the C_i in any order (and re-laying one as
+    a B operand, which frees its views) returns every byte to the pool, and a
+    re-laid C_i still multiplies correctly."""
+    print(f"[case] batched views, free order {order}", flush=True)
+    ps = problems()[:10]
+    tA = [ctx.tiles_from_csr(a) for a, _ in ps]
+    tB = [ctx.tiles_from_csr(b, as_b=True) for _, b in ps]
+    ctx.synchronize()
+    base_in_use, _ = ctx.pool_stats()
+    Cs, _ = ctx.spgemm_batched(tA, tB)
+    exp = [C1.download() for C1 in (ctx.spgemm(tA[i], tB[i])[0] for i in range(len(ps)))]
+    ctx.synchronize()
+    # chained use of one output: re-lay C_0 (= A_0 B_0, square ER) as a B operand
+    a0, b0 = ps[0]
+    Cs[0].build_csc_tiles()
+    C2, _ = ctx.spgemm(tA[0], Cs[0])
+    AB = O.gustavson_csr(a0, b0)
+    chk = O.gustavson_check(a0, g.HostCSR(a0.m, b0.n, *AB), C2.download())
+    assert chk["missing"] == 0 and chk["extra"] == 0 and chk["max_scaled"] <= 1e-12
+    C2.free()
+    idx = list(range(len(Cs)))
+    if order == "reverse":
+        idx.reverse()
+    elif order == "interleaved":
+        idx = idx[::2] + idx[1::2]
+    for k, i in enumerate(idx):
+        if i != 0:
+            assert_tiles_equal(Cs[i].download(), exp[i], keys=CKEYS, label=f"problem {i} ")
+        Cs[i].free()
+    ctx.synchronize()
+    in_use, _ = ctx.pool_stats()
+    assert in_use == base_in_use, (in_use, base_in_use)
+
+
 def test_batched_errors_and_empty(ctx):
     Cs, sts = ctx.spgemm_batched([], [])
     assert Cs == [] and sts == []
