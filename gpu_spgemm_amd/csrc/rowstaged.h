@@ -36,6 +36,7 @@ struct Plan {
     int64_t *rgw = nullptr;      // row-by-row bin: symbolic work-item offsets + counters (tilem + 3)
     int64_t *rgn = nullptr;      // row-by-row bin: numeric work-item offsets (tilem + 1)
     int64_t *rgc = nullptr;      // row-by-row bin: numeric cursor area (allocated with rgn's offsets)
+    HeavyBitmaps hb;             // heavy rows' union bitmaps, Step-1 count -> fill (freed after the fill)
     int64_t bin_off[MAX_BINS + 1] = {};
     int nbins = 0;
     int nvar = 0;     // staged bins; bin nvar = C-tile kernels
@@ -64,6 +65,8 @@ uint64_t tune_signature(int64_t tilemA, int64_t numtileA, int64_t nnzA, int64_t 
 bool tuned_config(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts,
                   TuneCfg *cfg);
 void free_plan(Context *ctx, Plan &p);
+// Frees the heavy rows' union bitmaps (after the Step-1 fill).
+void free_heavy_bitmaps(Context *ctx, Plan &p);
 // Allocates the plan's arrays and computes the A->B links.
 int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const BinTable &bt, Plan &p);
 // Partitions rows by p.bin and gathers hdr = [*total0, *total1 (or 0), bin offsets].

@@ -733,7 +733,15 @@ BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask) {
     return t;
 }
 
+void free_heavy_bitmaps(Context *ctx, Plan &p) {
+    pool_free(ctx, p.hb.words);
+    pool_free(ctx, p.hb.top);
+    pool_free(ctx, p.hb.off);
+    p.hb = HeavyBitmaps{};
+}
+
 void free_plan(Context *ctx, Plan &p) {
+    free_heavy_bitmaps(ctx, p);
     void *ps[] = {p.rows, p.bin, p.counts, p.hdr, p.overflow, p.link, p.lpay, p.rgw, p.rgn, p.rgc};
     for (void *q : ps) pool_free(ctx, q);
     p = Plan{};
@@ -807,13 +815,20 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
         TSG_CHECK(palloc(ctx, &hlist, (size_t)tilem));
         int rc = palloc(ctx, &hc, 4);
         if (!rc) rc = zero_small(ctx, hc, 4, nullptr, 0);
+        if (!rc && B->numtile > HEAVY_NB) {  // a heavy row reaches more B tiles than that
+            p.hb.cap = std::min<int64_t>(HEAVY_BITMAP_WORDS, (int64_t)tilem * ((int64_t)B->tilen / 32 + 1));
+            rc = palloc(ctx, &p.hb.words, (size_t)p.hb.cap);
+            if (!rc) rc = palloc(ctx, &p.hb.off, (size_t)tilem);
+            if (!rc) rc = palloc(ctx, reinterpret_cast<int64_t **>(&p.hb.top), 1);
+            if (!rc) rc = zero_small(ctx, nullptr, 0, reinterpret_cast<int64_t *>(p.hb.top), 1);
+        }
         if (!rc) {
             k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
                 A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
                 hlist, hc);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_stats launch");
         }
-        if (!rc) rc = launch_heavy_count(ctx, A, B, C->tile_ptr, hlist, hc, stats_out);
+        if (!rc) rc = launch_heavy_count(ctx, A, B, C->tile_ptr, hlist, hc, stats_out, p.hb);
         pool_free(ctx, hlist);
         pool_free(ctx, hc);
         TSG_CHECK(rc);
@@ -859,11 +874,11 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
         TSG_CHECK(launch_sym_staged<VarM>(ctx, A, B, C, p, 2, sl));
     }
     const int64_t nf = p.bin_off[fb + 1] - p.bin_off[fb];
-    TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb], nf, sl));
+    TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb], nf, sl, p.hb));
     TSG_CHECK(launch_symbolic_list(ctx, A, B, p.rows, p.bin_off[fb], nf, C, sl));
     if (p.rg) {  // row-by-row kernels: Step 1 fill as the C-tile rows, then masks from the products
         const int64_t nr = p.bin_off[fb + 2] - p.bin_off[fb + 1];
-        TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, sl));
+        TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, sl, p.hb));
         TSG_CHECK(launch_rg_symbolic(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, p.rgw, sl));
     }
     return TSG_OK;
@@ -938,6 +953,7 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     SlotLayout exact;
     exact.overflow = p.overflow;
     if (!rc) rc = run_symbolic(ctx, A, B, C, p, exact);
+    free_heavy_bitmaps(ctx, p);  // the fill has used them
     if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1);
     if (!rc) rc = prepare_numeric(ctx, A, C, p, exact);
     if (!rc) {

@@ -189,8 +189,9 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
                   const int32_t *__restrict__ colidxA, const int64_t *__restrict__ tile_ptrB,
                   const int32_t *__restrict__ colidxB, int64_t *__restrict__ tile_ptrC,
                   int32_t *__restrict__ rowidxC, int32_t *__restrict__ colidxC, int32_t *__restrict__ row_nc,
-                  int32_t *__restrict__ overflow, int32_t *__restrict__ stats) {
+                  int32_t *__restrict__ overflow, int32_t *__restrict__ stats, HeavyBitmaps hb) {
     __shared__ uint32_t bm[HS_WIN_WORDS];
+    __shared__ int64_t s_off;
     __shared__ int64_t s_qbs[HS_THREADS];
     __shared__ int32_t s_qlen[HS_THREADS];
     __shared__ int s_w[HS_THREADS / WAVE];
@@ -230,43 +231,68 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
         }
         __syncthreads();
         const int32_t cmin = s_min, cmax = s_max;
+        if (tid == 0) {  // where this row's union bitmap is kept (count) or found (fill)
+            int64_t o = -1;
+            if (!FILL && hb.off) {
+                const int64_t words = cmax >= cmin ? (int64_t)(cmax - cmin) / 32 + 1 : 0;
+                if (words > 0) {
+                    o = (int64_t)atomicAdd(hb.top, (unsigned long long)words);
+                    if (o + words > hb.cap) o = -1;
+                }
+                hb.off[bi] = o;
+            } else if (FILL && hb.off) {
+                o = hb.off[bi];
+            }
+            s_off = o;
+        }
+        __syncthreads();
+        const int64_t boff = s_off;
         int64_t out = FILL ? tile_ptrC[bi] : 0;
         const int64_t slot_end = (FILL && row_nc) ? tile_ptrC[bi + 1] : INT64_MAX;
         int64_t count = 0;
         for (int64_t w0 = cmin; w0 <= cmax; w0 += (int64_t)HS_WIN_WORDS * 32) {  // uniform
             const int64_t wbits = min<int64_t>((int64_t)HS_WIN_WORDS * 32, cmax - w0 + 1);
-            for (int i = tid; i < HS_WIN_WORDS; i += HS_THREADS) bm[i] = 0u;
-            __syncthreads();
-            for (int64_t base = as; base < ae; base += HS_THREADS) {  // uniform trip count
-                const int64_t i = base + tid;
-                if (i < ae) {
-                    const int32_t k = colidxA[i];
-                    const int64_t bs = tile_ptrB[k];
-                    const int32_t len = (int32_t)(tile_ptrB[k + 1] - bs);
-                    if (len > HS_SHORT) {
-                        const int qi = atomicAdd(&s_nq, 1);
-                        s_qbs[qi] = bs;
-                        s_qlen[qi] = len;
-                    } else {
-                        for (int32_t q = 0; q < len; q++) {
+            const int nwords = (int)((wbits + 31) >> 5);
+            const int64_t wbase = boff + ((w0 - cmin) >> 5);  // this window's first kept word
+            if (FILL && boff >= 0) {  // the count left this row's bitmap
+                for (int i = tid; i < HS_WIN_WORDS; i += HS_THREADS) bm[i] = i < nwords ? hb.words[wbase + i] : 0u;
+                __syncthreads();
+            } else {
+                for (int i = tid; i < HS_WIN_WORDS; i += HS_THREADS) bm[i] = 0u;
+                __syncthreads();
+                for (int64_t base = as; base < ae; base += HS_THREADS) {  // uniform trip count
+                    const int64_t i = base + tid;
+                    if (i < ae) {
+                        const int32_t k = colidxA[i];
+                        const int64_t bs = tile_ptrB[k];
+                        const int32_t len = (int32_t)(tile_ptrB[k + 1] - bs);
+                        if (len > HS_SHORT) {
+                            const int qi = atomicAdd(&s_nq, 1);
+                            s_qbs[qi] = bs;
+                            s_qlen[qi] = len;
+                        } else {
+                            for (int32_t q = 0; q < len; q++) {
+                                const int64_t off = (int64_t)colidxB[bs + q] - w0;
+                                if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    const int nq = s_nq;
+                    for (int qi = 0; qi < nq; qi++) {  // long lists: the whole block, coalesced
+                        const int64_t bs = s_qbs[qi];
+                        const int32_t len = s_qlen[qi];
+                        for (int32_t q = tid; q < len; q += HS_THREADS) {
                             const int64_t off = (int64_t)colidxB[bs + q] - w0;
                             if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
                         }
                     }
+                    __syncthreads();
+                    if (tid == 0) s_nq = 0;
+                    __syncthreads();
                 }
-                __syncthreads();
-                const int nq = s_nq;
-                for (int qi = 0; qi < nq; qi++) {  // long lists: the whole block, coalesced
-                    const int64_t bs = s_qbs[qi];
-                    const int32_t len = s_qlen[qi];
-                    for (int32_t q = tid; q < len; q += HS_THREADS) {
-                        const int64_t off = (int64_t)colidxB[bs + q] - w0;
-                        if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
-                    }
-                }
-                __syncthreads();
-                if (tid == 0) s_nq = 0;
-                __syncthreads();
+                if (!FILL && boff >= 0)  // keep the window's words for the fill
+                    for (int i = tid; i < nwords; i += HS_THREADS) hb.words[wbase + i] = bm[i];
             }
             // each thread owns HS_WIN_WORDS / HS_THREADS consecutive words
             constexpr int WPT = HS_WIN_WORDS / HS_THREADS;
@@ -321,17 +347,17 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
 // Exact C tile counts of the heavy rows listed by k_row_stats (written into
 // tile_ptrC[bi + 1] before its scan).  hc = {row count, work counter}.
 int launch_heavy_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, int64_t *tile_ptrC,
-                       const int32_t *hlist, int32_t *hc, int32_t *stats) {
+                       const int32_t *hlist, int32_t *hc, int32_t *stats, const HeavyBitmaps &hb) {
     k_heavy_structure<false><<<(unsigned)std::max(1, ctx->prop.multiProcessorCount * 8), HS_THREADS, 0, ctx->stream>>>(
         hlist, hc, reinterpret_cast<unsigned long long *>(hc + 2), A->tile_ptr, A->tile_columnidx, B->tile_ptr,
-        B->tile_columnidx, tile_ptrC, nullptr, nullptr, nullptr, nullptr, stats);
+        B->tile_columnidx, tile_ptrC, nullptr, nullptr, nullptr, nullptr, stats, hb);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
 
 int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                                const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
-                               const SlotLayout &sl) {
+                               const SlotLayout &sl, const HeavyBitmaps &hb) {
     if (count <= 0) return TSG_OK;
     int32_t *hlist = nullptr, *hc = nullptr;  // heavy rows; hc = {count, pad, work counter (8 bytes)}
     TSG_CHECK(palloc(ctx, &hlist, (size_t)count));
@@ -348,7 +374,7 @@ int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg
         k_heavy_structure<true><<<(unsigned)std::max(1, ctx->prop.multiProcessorCount * 8), HS_THREADS, 0,
                                    ctx->stream>>>(
             hlist, hc, reinterpret_cast<unsigned long long *>(hc + 2), A->tile_ptr, A->tile_columnidx, B->tile_ptr,
-            B->tile_columnidx, C->tile_ptr, C->tile_rowidx, C->tile_columnidx, sl.row_nc, sl.overflow, nullptr);
+            B->tile_columnidx, C->tile_ptr, C->tile_rowidx, C->tile_columnidx, sl.row_nc, sl.overflow, nullptr, hb);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_heavy_structure launch");
     }
     pool_free(ctx, hlist);

@@ -125,13 +125,25 @@ struct SlotLayout {
 // Tile-rows reaching more than HEAVY_NB B tiles are counted and filled by a
 // workgroup each (structure.hip) instead of one wave.
 constexpr int64_t HEAVY_NB = 8192;
+// Words of union bitmaps kept from the heavy rows' count for their fill (64 MB).
+constexpr int64_t HEAVY_BITMAP_WORDS = 16ll << 20;
+// Union bitmaps the heavy rows' Step-1 count leaves for their fill: the
+// bitmap of tile-row bi (bit i = tile column cmin + i, cmin its first
+// reachable column) starts at words[off[bi]], or off[bi] = -1 when it did not
+// fit in cap words (the fill then rebuilds it).  Null members: none kept.
+struct HeavyBitmaps {
+    uint32_t *words = nullptr;
+    int64_t cap = 0;
+    unsigned long long *top = nullptr;  // words handed out (zeroed before the count)
+    int64_t *off = nullptr;             // [tilem]
+};
 // C tile counts of the heavy rows in hlist (hc = {row count, pad, 64-bit work
 // counter}, zeroed before the list is built), written to tile_ptrC[bi + 1].
 int launch_heavy_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, int64_t *tile_ptrC,
-                       const int32_t *hlist, int32_t *hc, int32_t *stats);
+                       const int32_t *hlist, int32_t *hc, int32_t *stats, const HeavyBitmaps &hb);
 int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                                const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
-                               const SlotLayout &sl);
+                               const SlotLayout &sl, const HeavyBitmaps &hb = HeavyBitmaps{});
 int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const int32_t *rows,
                          int64_t begin, int64_t count, const tsg_tile_matrix *C, const SlotLayout &sl);
 int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
