@@ -814,14 +814,13 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
         int32_t *hlist = nullptr, *hc = nullptr;  // heavy rows (counted by a workgroup each)
         TSG_CHECK(palloc(ctx, &hlist, (size_t)tilem));
         int rc = palloc(ctx, &hc, 4);
-        if (!rc) rc = zero_small(ctx, hc, 4, nullptr, 0);
         if (!rc && B->numtile > HEAVY_NB) {  // a heavy row reaches more B tiles than that
             p.hb.cap = std::min<int64_t>(HEAVY_BITMAP_WORDS, (int64_t)tilem * ((int64_t)B->tilen / 32 + 1));
             rc = palloc(ctx, &p.hb.words, (size_t)p.hb.cap);
             if (!rc) rc = palloc(ctx, &p.hb.off, (size_t)tilem);
             if (!rc) rc = palloc(ctx, reinterpret_cast<int64_t **>(&p.hb.top), 1);
-            if (!rc) rc = zero_small(ctx, nullptr, 0, reinterpret_cast<int64_t *>(p.hb.top), 1);
         }
+        if (!rc) rc = zero_small(ctx, hc, 4, reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
         if (!rc) {
             k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
                 A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
