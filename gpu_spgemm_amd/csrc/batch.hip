@@ -138,7 +138,7 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
                  B[i].csr_col_Val && B[i].row_nnz_ptr;
         brow = brow && B[i].tile_rowidx;
         scal = scal && A[i].row_ptr && A[i].row_col && A[i].row_val && B[i].row_ptr && B[i].row_col && B[i].row_val;
-        flags |= A[i].flags | B[i].flags;
+        flags |= ((A[i].flags | B[i].flags) & 1) | (B[i].flags & 2);  // repeats; long B tile-rows
         R[i + 1] = R[i] + A[i].tilem;
         K[i + 1] = K[i] + A[i].tilen;
         Cc[i + 1] = Cc[i] + B[i].tilen;
@@ -154,7 +154,7 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
     BA.m = (int32_t)(R[n] * 16); BA.n = (int32_t)(K[n] * 16); BA.tilem = (int32_t)R[n]; BA.tilen = (int32_t)K[n];
     BA.numtile = at[n]; BA.nnz = an[n]; BA.dtype = dtype; BA.location = TSG_DEVICE; BA.flags = flags & 1;
     BB.m = BA.n; BB.n = (int32_t)(Cc[n] * 16); BB.tilem = BA.tilen; BB.tilen = (int32_t)Cc[n];
-    BB.numtile = bt[n]; BB.nnz = bn[n]; BB.dtype = dtype; BB.location = TSG_DEVICE; BB.is_csc = 1; BB.flags = flags & 1;
+    BB.numtile = bt[n]; BB.nnz = bn[n]; BB.dtype = dtype; BB.location = TSG_DEVICE; BB.is_csc = 1; BB.flags = flags;
     auto free_m = [&](tsg_tile_matrix &M) {
         void *ps[] = {M.tile_ptr, M.tile_columnidx, M.tile_rowidx, M.tile_nnz, M.tile_csr_Val, M.tile_csr_Col,
                       M.tile_csr_Ptr, M.mask, M.csc_tile_ptr, M.csc_tile_rowidx, M.csr_tile_nnz, M.csr_mask,

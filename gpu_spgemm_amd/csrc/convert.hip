@@ -148,6 +148,7 @@ k_c2t(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int32_t m
         if (lane == 0) {
             tile_ptr[bi + 1] = count;
             if (bi == 0) tile_ptr[0] = 0;
+            if (count > HEAVY_LIST) atomicOr(flags, 8);  // a long tile-row
         }
     }
     if (__ballot(bad) && lane == 0) atomicOr(flags, 1);
@@ -489,7 +490,7 @@ static int csr2tile_impl(Context *ctx, const tsg_csr *A, tsg_tile_matrix *T, int
     }
     if (ts != 16) {  // no scalar CSR copy: no product runs at this tile size
         if ((rc = read_i32(ctx, flags, &hf))) return cleanup(rc);
-        T->flags = (hf & 2) ? 1 : 0;
+        T->flags = ((hf & 2) ? 1 : 0) | ((hf & 8) ? 2 : 0);
         pool_free(ctx, flags);
         return TSG_OK;
     }
@@ -506,7 +507,7 @@ static int csr2tile_impl(Context *ctx, const tsg_csr *A, tsg_tile_matrix *T, int
           hipMemcpyAsync(T->row_val, A->val, vb, hipMemcpyDeviceToDevice, s) != hipSuccess)))
         return cleanup(fail(TSG_ERR_HIP, "copy of the scalar CSR failed"));
     if ((rc = read_i32(ctx, flags, &hf))) return cleanup(rc);
-    T->flags = (hf & 2) ? 1 : 0;
+    T->flags = ((hf & 2) ? 1 : 0) | ((hf & 8) ? 2 : 0);
     pool_free(ctx, flags);
     return TSG_OK;
 }

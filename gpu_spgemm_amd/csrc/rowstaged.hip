@@ -734,6 +734,8 @@ BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask) {
 }
 
 void free_heavy_bitmaps(Context *ctx, Plan &p) {
+    pool_free(ctx, p.hb.bwords);
+    pool_free(ctx, p.hb.boff);
     pool_free(ctx, p.hb.words);
     pool_free(ctx, p.hb.top);
     pool_free(ctx, p.hb.off);
@@ -819,6 +821,7 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
             rc = palloc(ctx, &p.hb.words, (size_t)p.hb.cap);
             if (!rc) rc = palloc(ctx, &p.hb.off, (size_t)tilem);
             if (!rc) rc = palloc(ctx, reinterpret_cast<int64_t **>(&p.hb.top), 1);
+            if (!rc && (B->flags & 2)) rc = build_brow_bitmaps(ctx, B, p.hb);  // long B tile-rows exist
         }
         if (!rc) rc = zero_small(ctx, hc, 4, reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
         if (!rc) {

@@ -170,7 +170,7 @@ int launch_structure_fill(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
 // device list; a persistent grid takes them from a counter.
 constexpr int HS_THREADS = 256;
 constexpr int HS_WIN_WORDS = 2048;  // 65,536 tile columns per pass
-constexpr int HS_SHORT = 64;        // B tile-row lists walked by one thread
+constexpr int HS_SHORT = HEAVY_LIST;  // B tile-row lists walked by one thread
 
 __device__ __forceinline__ int hs_block_sum(int v, int *s_w) {
     v = wave_sum(v);
@@ -192,6 +192,10 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
                   int32_t *__restrict__ overflow, int32_t *__restrict__ stats, HeavyBitmaps hb) {
     __shared__ uint32_t bm[HS_WIN_WORDS];
     __shared__ int64_t s_off;
+    __shared__ int64_t s_bo[HS_THREADS];  // queued B tile-row bitmaps: first word, word count, first column word
+    __shared__ int32_t s_bw[HS_THREADS];
+    __shared__ int32_t s_bl[HS_THREADS];
+    __shared__ int s_nb;
     __shared__ int64_t s_qbs[HS_THREADS];
     __shared__ int32_t s_qlen[HS_THREADS];
     __shared__ int s_w[HS_THREADS / WAVE];
@@ -204,6 +208,7 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
         if (tid == 0) {
             s_h = (int)atomicAdd(ctr, 1ull);
             s_nq = 0;
+            s_nb = 0;
             s_min = INT_MAX;
             s_max = -1;
         }
@@ -230,7 +235,8 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
             }
         }
         __syncthreads();
-        const int32_t cmin = s_min, cmax = s_max;
+        // windows start on a word boundary (B tile-row bitmaps are word-aligned)
+        const int32_t cmin = s_min & ~31, cmax = s_max;
         if (tid == 0) {  // where this row's union bitmap is kept (count) or found (fill)
             int64_t o = -1;
             if (!FILL && hb.off) {
@@ -266,7 +272,17 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
                         const int32_t k = colidxA[i];
                         const int64_t bs = tile_ptrB[k];
                         const int32_t len = (int32_t)(tile_ptrB[k + 1] - bs);
-                        if (len > HS_SHORT) {
+                        int64_t bo = 0, bw = 0;
+                        if (hb.boff) {
+                            bo = hb.boff[k];
+                            bw = hb.boff[k + 1] - bo;
+                        }
+                        if (bw > 0) {  // OR the row's bitmap words
+                            const int qi = atomicAdd(&s_nb, 1);
+                            s_bo[qi] = bo;
+                            s_bw[qi] = (int32_t)bw;
+                            s_bl[qi] = colidxB[bs] >> 5;
+                        } else if (len > HS_SHORT) {
                             const int qi = atomicAdd(&s_nq, 1);
                             s_qbs[qi] = bs;
                             s_qlen[qi] = len;
@@ -287,8 +303,25 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
                             if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
                         }
                     }
+                    const int nbq = s_nb;
+                    const int64_t wlo = w0 >> 5;
+                    for (int qi = 0; qi < nbq; qi++) {  // B tile-row bitmaps: one LDS atomic per word
+                        const int64_t bo = s_bo[qi];
+                        const int32_t bw = s_bw[qi];
+                        const int64_t bl = s_bl[qi];
+                        for (int32_t q = tid; q < bw; q += HS_THREADS) {
+                            const int64_t wi = bl + q - wlo;
+                            if (wi >= 0 && wi < nwords) {
+                                const uint32_t v = hb.bwords[bo + q];
+                                if (v) atomicOr(&bm[wi], v);
+                            }
+                        }
+                    }
                     __syncthreads();
-                    if (tid == 0) s_nq = 0;
+                    if (tid == 0) {
+                        s_nq = 0;
+                        s_nb = 0;
+                    }
                     __syncthreads();
                 }
                 if (!FILL && boff >= 0)  // keep the window's words for the fill
@@ -342,6 +375,61 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
         }
         __syncthreads();
     }
+}
+
+// Words of B tile-row k's column bitmap when it pays (more than HS_SHORT
+// tiles, at least two per word, at most one LDS window), else 0, into
+// boff[k + 1]; boff[0] = 0 (a scan then gives each row's first word).
+__global__ void __launch_bounds__(256) k_brow_words(int32_t tilemB, const int64_t *__restrict__ tile_ptrB,
+                                                    const int32_t *__restrict__ colidxB, int64_t *__restrict__ boff) {
+    const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (k == 0) boff[0] = 0;
+    if (k >= tilemB) return;
+    const int64_t bs = tile_ptrB[k], len = tile_ptrB[k + 1] - bs;
+    int64_t words = 0;
+    if (len > HS_SHORT) {
+        const int64_t w = (int64_t)(colidxB[bs + len - 1] >> 5) - (colidxB[bs] >> 5) + 1;
+        if (2 * w <= len && w <= HS_WIN_WORDS) words = w;
+    }
+    boff[k + 1] = words;
+}
+
+// A workgroup per B tile-row with a bitmap: set its bits in LDS, store the words.
+__global__ void __launch_bounds__(HS_THREADS)
+k_brow_fill(int32_t tilemB, const int64_t *__restrict__ tile_ptrB, const int32_t *__restrict__ colidxB,
+            const int64_t *__restrict__ boff, uint32_t *__restrict__ bwords) {
+    __shared__ uint32_t bm[HS_WIN_WORDS];
+    const int tid = threadIdx.x;
+    for (int64_t k = blockIdx.x; k < tilemB; k += gridDim.x) {  // uniform
+        const int64_t o = boff[k];
+        const int32_t w = (int32_t)(boff[k + 1] - o);
+        if (w == 0) continue;
+        const int64_t bs = tile_ptrB[k], be = tile_ptrB[k + 1];
+        const int32_t lo = colidxB[bs] >> 5;
+        for (int i = tid; i < w; i += HS_THREADS) bm[i] = 0u;
+        __syncthreads();
+        for (int64_t q = bs + tid; q < be; q += HS_THREADS) {
+            const int32_t c = colidxB[q];
+            atomicOr(&bm[(c >> 5) - lo], 1u << (c & 31));
+        }
+        __syncthreads();
+        for (int i = tid; i < w; i += HS_THREADS) bwords[o + i] = bm[i];
+        __syncthreads();
+    }
+}
+
+int build_brow_bitmaps(Context *ctx, const tsg_tile_matrix *B, HeavyBitmaps &hb) {
+    const int32_t tilemB = B->tilem;
+    TSG_CHECK(palloc(ctx, &hb.boff, (size_t)tilemB + 1));
+    TSG_CHECK(palloc(ctx, &hb.bwords, (size_t)(B->numtile / 2 + 1)));
+    k_brow_words<<<(unsigned)(((int64_t)tilemB + 256) / 256), 256, 0, ctx->stream>>>(tilemB, B->tile_ptr,
+                                                                                     B->tile_columnidx, hb.boff);
+    TSG_HIP(hipGetLastError());
+    TSG_CHECK(scan_inclusive_i64(ctx, hb.boff, (int64_t)tilemB + 1));
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(tilemB, ctx->prop.multiProcessorCount * 8));
+    k_brow_fill<<<grid, HS_THREADS, 0, ctx->stream>>>(tilemB, B->tile_ptr, B->tile_columnidx, hb.boff, hb.bwords);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
 }
 
 // Exact C tile counts of the heavy rows listed by k_row_stats (written into

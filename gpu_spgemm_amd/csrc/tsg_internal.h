@@ -125,6 +125,9 @@ struct SlotLayout {
 // Tile-rows reaching more than HEAVY_NB B tiles are counted and filled by a
 // workgroup each (structure.hip) instead of one wave.
 constexpr int64_t HEAVY_NB = 8192;
+// B tile-row lists longer than this are walked by a whole workgroup (or
+// OR-ed as bitmaps); tsg_tile_matrix.flags bit 1 marks a matrix with such rows.
+constexpr int HEAVY_LIST = 64;
 // Words of union bitmaps kept from the heavy rows' count for their fill (64 MB).
 constexpr int64_t HEAVY_BITMAP_WORDS = 16ll << 20;
 // Union bitmaps the heavy rows' Step-1 count leaves for their fill: the
@@ -136,7 +139,15 @@ struct HeavyBitmaps {
     int64_t cap = 0;
     unsigned long long *top = nullptr;  // words handed out (zeroed before the count)
     int64_t *off = nullptr;             // [tilem]
+    // Column bitmaps of B's long, dense tile-rows (more than HEAVY_LIST tiles,
+    // at least two per word): row k's words are bwords[boff[k] .. boff[k+1]),
+    // bit c & 31 of word (c >> 5) - (first column >> 5); boff[k+1] == boff[k]
+    // for the other rows.  Null: walk every list.
+    uint32_t *bwords = nullptr;
+    int64_t *boff = nullptr;  // [tilemB+1]
 };
+// Build the B tile-row bitmaps of hb (bwords sized numtileB / 2 + 1 words).
+int build_brow_bitmaps(Context *ctx, const tsg_tile_matrix *B, HeavyBitmaps &hb);
 // C tile counts of the heavy rows in hlist (hc = {row count, pad, 64-bit work
 // counter}, zeroed before the list is built), written to tile_ptrC[bi + 1].
 int launch_heavy_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, int64_t *tile_ptrC,

@@ -71,7 +71,9 @@ typedef struct tsg_tile_matrix {
     int32_t dtype;            /* tsg_dtype */
     int32_t location;         /* tsg_location */
     int32_t is_csc;           /* 1 for a B operand laid out by tsg_build_csc_tiles */
-    int32_t flags;            /* bit 0: some row holds a column twice (duplicates kept) */
+    int32_t flags;            /* bit 0: some row holds a column twice (duplicates kept);
+                                 bit 1: some tile-row holds more than 64 tiles (set by
+                                 tsg_csr2tile; products then keep bitmaps of such B tile-rows) */
     int64_t *tile_ptr;        /* [tilem+1] */
     int32_t *tile_columnidx;  /* [numtile] */
     int32_t *tile_rowidx;     /* [numtile] row of each tile (C output), may be NULL for A/B */

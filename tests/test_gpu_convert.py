@@ -179,6 +179,18 @@ def test_gpu_tile_size_limits(ctx):
     assert_tiles_equal(back, d, label="round trip ")
 
 
+def test_gpu_long_tile_row_flag(ctx):
+    """flags bit 1: some tile-row holds more than 64 tiles (products then keep
+    column bitmaps of such B tile-rows for the heavy rows' Step 1)."""
+    print("[case] long tile-row flag", flush=True)
+    E = g.gen_er(4096, 40, seed=2)                 # 640 entries a tile-row over 256 tile columns
+    d = O.csr2tile(E)
+    assert int(np.diff(d["tile_ptr"]).max()) > 64
+    assert ctx.tiles_from_csr(E).s.flags & 2 and ctx.tiles_from_csr(E, as_b=True).s.flags & 2
+    L = g.gen_laplace2d(64, 48)                    # at most 3 tiles a tile-row
+    assert not ctx.tiles_from_csr(L).s.flags & 2
+
+
 def test_gpu_conversion_wide_rows_take_several_windows(ctx):
     # tile columns span more than one LDS window (32,768 tile columns per pass)
     n = 16 * 70000 + 5
