@@ -334,7 +334,10 @@ int tsg_plan_destroy(tsg_plan *plan);
 
 /* Many independent products in one pipeline run (no reference counterpart):
  * the batch is one product of block-diagonal operands; C is an array of count
- * outputs, stats (may be NULL) an array of count entries. */
+ * outputs, stats (may be NULL) an array of count entries.  Each C_i is freed
+ * with tsg_tile_free as usual; its mask, tile_csr_Ptr, tile_csr_Val and
+ * tile_csr_Col lie inside allocations shared by the batch's outputs, which
+ * return to the context's pool when the last C_i holding them is freed. */
 int tsg_spgemm_batched(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A,
                        const tsg_tile_matrix *B, tsg_tile_matrix *C, const tsg_options *opts,
                        tsg_stats *stats);
