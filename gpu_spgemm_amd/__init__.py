@@ -352,6 +352,18 @@ class Context:
             self.synchronize()
             d.free()
 
+    def build_bitmasks(self, A, B):
+        """build_bitmasks (src/tile_format.h:628-645) on the GPU: returns
+        (bmA [tilemA, bm_len], bmB [tilenB, bm_len]) as uint32 numpy arrays."""
+        bl = C.c_int32()
+        _check(lib().tsg_build_bitmasks(self._p, C.byref(A.s), C.byref(B.s), None, None, C.byref(bl)))
+        bmA = np.zeros((A.s.tilem, bl.value), np.uint32)
+        bmB = np.zeros((B.s.tilen, bl.value), np.uint32)
+        _check(lib().tsg_build_bitmasks(self._p, C.byref(A.s), C.byref(B.s),
+                                        bmA.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                        bmB.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(bl)))
+        return bmA, bmB
+
     def plan(self, As, Bs, variant=0, mfma_min_products=0, tuning=0):
         """Product (or batch, for lists) plus a plan that recomputes the values
         from the operands' current values as one captured hipGraph."""

@@ -471,3 +471,54 @@ int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg
 }
 
 }  // namespace tsg
+
+// ---------------------------------------------------------------- tile bitmaps
+namespace tsg {
+
+// One wave per list (a tile-row of A, or a tile column of B in CSC-tile
+// order): bit 31 - k % 32 of word k / 32 for every tile index k it holds.
+__global__ void __launch_bounds__(256) k_tile_bitmaps(int32_t nlists, const int64_t *__restrict__ ptr,
+                                                      const int32_t *__restrict__ idx, int32_t bm_len,
+                                                      uint32_t *__restrict__ bm) {
+    const int64_t l = ((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE;
+    if (l >= nlists) return;
+    for (int64_t j = ptr[l] + lane_id(); j < ptr[l + 1]; j += WAVE) {
+        const int32_t k = idx[j];
+        atomicOr(&bm[l * bm_len + (k >> 5)], 1u << (31 - (k & 31)));
+    }
+}
+
+// Bitmaps of n lists into the caller's host array out[n * bm_len].
+static int tile_bitmaps_to_host(Context *ctx, int32_t n, const int64_t *ptr, const int32_t *idx, int32_t bm_len,
+                                uint32_t *out) {
+    const size_t words = (size_t)n * (size_t)bm_len;
+    if (words == 0) return TSG_OK;
+    uint32_t *d = nullptr;
+    TSG_CHECK(palloc(ctx, &d, words));
+    int rc = TSG_OK;
+    if (hipMemsetAsync(d, 0, words * sizeof(uint32_t), ctx->stream) != hipSuccess) rc = fail(TSG_ERR_HIP, "memset");
+    if (!rc) {
+        k_tile_bitmaps<<<(unsigned)(((int64_t)n * WAVE + 255) / 256), 256, 0, ctx->stream>>>(n, ptr, idx, bm_len, d);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_tile_bitmaps launch");
+    }
+    if (!rc && (hipMemcpyAsync(out, d, words * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess))
+        rc = fail(TSG_ERR_HIP, "bitmap download failed");
+    pool_free(ctx, d);
+    return rc;
+}
+
+}  // namespace tsg
+
+extern "C" int tsg_build_bitmasks(tsg_context *c, const tsg_tile_matrix *A, const tsg_tile_matrix *B, uint32_t *bmA,
+                                  uint32_t *bmB, int32_t *bm_len) {
+    using namespace tsg;
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !bm_len) return fail(TSG_ERR_ARG, "null argument");
+    TSG_CHECK(check_operands(A, B));
+    (void)hipSetDevice(ctx->device);
+    *bm_len = (A->tilen + 31) / 32;
+    if (bmA) TSG_CHECK(tile_bitmaps_to_host(ctx, A->tilem, A->tile_ptr, A->tile_columnidx, *bm_len, bmA));
+    if (bmB) TSG_CHECK(tile_bitmaps_to_host(ctx, B->tilen, B->csc_tile_ptr, B->csc_tile_rowidx, *bm_len, bmB));
+    return TSG_OK;
+}

@@ -234,6 +234,16 @@ int tsg_csr2tile_ts(tsg_context *ctx, const tsg_csr *csr_dev, int32_t ts, tsg_ti
  * the reference's counting scatter does.  Rebuilding replaces the arrays;
  * tsg_tile_set_values refreshes tile_csc_Val. */
 int tsg_build_tile_csc(tsg_context *ctx, tsg_tile_matrix *dev);
+/* Per A tile-row and per B tile column, a bitmap over the k tile index:
+ * replaces build_bitmasks (src/tile_format.h:628-645), which the reference's
+ * driver does not call.  *bm_len = ceil(A.tilen / 32) words per row; bit
+ * 31 - k % 32 of word k / 32 is set when A's tile-row holds a tile in tile
+ * column k (bmA, tilemA * bm_len words) or B's tile column holds a tile in
+ * tile row k (bmB, tilenB * bm_len words); AND-ing them finds the matched k.
+ * A: row tiles, B: a B operand (tsg_build_csc_tiles); bmA / bmB are HOST
+ * arrays the caller sizes (NULL: skipped), as the reference's are. */
+int tsg_build_bitmasks(tsg_context *ctx, const tsg_tile_matrix *A_dev, const tsg_tile_matrix *B_dev,
+                       uint32_t *bmA, uint32_t *bmB, int32_t *bm_len);
 /* Row tiles -> CSR (the reference's verification unpacking, src/spgemm_raw_v2.cu:268-284). */
 int tsg_tile2csr(tsg_context *ctx, const tsg_tile_matrix *dev, tsg_csr *out_dev);
 

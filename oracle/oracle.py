@@ -302,6 +302,24 @@ def csr2tile_ts(csr, ts, as_b=False, f32=False):
     return d
 
 
+def build_bitmasks(dA, dB):
+    """build_bitmasks (src/tile_format.h:628-645) restated with numpy from tile
+    dicts: A's tile-rows and B's tile columns (CSC-tile order) as bitmaps over
+    the k tile index, bm_len = ceil(A.tilen / 32), bit 31 - k % 32 of word k / 32."""
+    bl = (int(dA["tilen"]) + 31) // 32
+
+    def bits(n, ptr, idx):
+        bm = np.zeros((n, bl), np.uint32)
+        ptr = np.asarray(ptr, np.int64)
+        lst = np.repeat(np.arange(n), np.diff(ptr))
+        k = np.asarray(idx, np.int64)
+        np.bitwise_or.at(bm, (lst, k // 32), (np.uint32(1) << (31 - k % 32).astype(np.uint32)))
+        return bm
+
+    return (bits(int(dA["tilem"]), dA["tile_ptr"], dA["tile_columnidx"]),
+            bits(int(dB["tilen"]), dB["csc_tile_ptr"], dB["csc_tile_rowidx"]))
+
+
 def tile_csc(d):
     """Within-tile CSC of every tile of a tile dict (the download() layout, data
     arrays in their own tile order): the reference's tile_csc_Ptr/Row/Val of a
@@ -360,6 +378,23 @@ def ref_tile_csc(csr, as_b=False):
     ref().ref_tile_csc(csr.m, csr.n, _p(rp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double), 1 if as_b else 0,
                        _p(ptr, C.c_uint8), _p(row, C.c_uint8), _p(v, C.c_double))
     return dict(tile_csc_Ptr=ptr[:nt * 16], tile_csc_Row=row[:nnz], tile_csc_Val=v[:nnz])
+
+
+def ref_build_bitmasks(A, B):
+    """The reference's own build_bitmasks (tile_format.h:628-645) on csr2tile(A)
+    and build_csc_tiles(B), compiled from its sources."""
+    ra, ca, va = _csr_args(A)
+    rb, cb, vb = _csr_args(B)
+    bl = C.c_int32()
+    ref().ref_build_bitmasks(A.m, A.n, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(va, C.c_double), B.m, B.n,
+                             _p(rb, C.c_int64), _p(cb, C.c_int32), _p(vb, C.c_double), None, None, C.byref(bl))
+    tma, tnb = (A.m + 15) // 16, (B.n + 15) // 16
+    bmA = np.zeros(max(tma * bl.value, 1), np.uint32)
+    bmB = np.zeros(max(tnb * bl.value, 1), np.uint32)
+    ref().ref_build_bitmasks(A.m, A.n, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(va, C.c_double), B.m, B.n,
+                             _p(rb, C.c_int64), _p(cb, C.c_int32), _p(vb, C.c_double), _p(bmA, C.c_uint32),
+                             _p(bmB, C.c_uint32), C.byref(bl))
+    return bmA[:tma * bl.value].reshape(tma, bl.value), bmB[:tnb * bl.value].reshape(tnb, bl.value)
 
 
 def ref_load_mtx(path):

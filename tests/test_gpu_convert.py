@@ -179,6 +179,17 @@ def test_gpu_tile_size_limits(ctx):
     assert_tiles_equal(back, d, label="round trip ")
 
 
+@pytest.mark.parametrize("name", ["er4096", "rmat10", "aggregation"])
+def test_gpu_build_bitmasks_matches_oracle(ctx, name):
+    """tsg_build_bitmasks (build_bitmasks, src/tile_format.h:628-645)."""
+    print(f"[case] bitmasks {name}", flush=True)
+    A = _matrices()[name]()
+    B = A if A.m == A.n else A.transpose()
+    bmA, bmB = ctx.build_bitmasks(ctx.tiles_from_csr(A), ctx.tiles_from_csr(B, as_b=True))
+    eA, eB = O.build_bitmasks(O.csr2tile(A), O.csr2tile_b(B))
+    assert np.array_equal(bmA, eA) and np.array_equal(bmB, eB)
+
+
 def test_gpu_long_tile_row_flag(ctx):
     """flags bit 1: some tile-row holds more than 64 tiles (products then keep
     column bitmaps of such B tile-rows for the heavy rows' Step 1)."""

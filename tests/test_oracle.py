@@ -196,6 +196,32 @@ def test_planted_csc_row_order_is_caught():
     assert not csc_vs_transpose(A, bad)
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_build_bitmasks_matches_reference_and_tile_pattern_product(seed):
+    """build_bitmasks (:628-645): against the reference's own build, and the
+    closed form popcount(bmA[I] & bmB[J]) = number of k with A tile (I, k)
+    and B tile (k, J), i.e. the (I, J) entry of the tile-pattern product."""
+    rng = np.random.default_rng(300 + seed)
+    m, k, n = (int(x) for x in rng.integers(20, 400, 3))
+    A = rand_csr(m, k, float(rng.uniform(0.005, 0.1)), seed)
+    B = rand_csr(k, n, float(rng.uniform(0.005, 0.1)), seed + 10)
+    bmA, bmB = O.build_bitmasks(O.csr2tile(A), O.csr2tile_b(B))
+    if O.ref_available():
+        rA, rB = O.ref_build_bitmasks(A, B)
+        assert np.array_equal(bmA, rA) and np.array_equal(bmB, rB)
+    def tile_pattern(S, tm, tn):
+        S = S.tocoo()
+        P = sp.csr_matrix((np.ones(S.nnz), (S.row // 16, S.col // 16)), shape=(tm, tn))
+        P.data[:] = 1.0
+        return P
+    PA = tile_pattern(A.to_scipy(), (m + 15) // 16, (k + 15) // 16)
+    PB = tile_pattern(B.to_scipy(), (k + 15) // 16, (n + 15) // 16)
+    pairs = (PA @ PB).toarray()
+    inter = np.bitwise_and(bmA[:, None, :], bmB[None, :, :])
+    pop = np.vectorize(lambda x: bin(int(x)).count("1"))(inter).sum(axis=2)
+    assert np.array_equal(pop, pairs.astype(np.int64))
+
+
 TS_SIZES = (1, 5, 8, 13)
 
 
