@@ -816,7 +816,10 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
         int32_t *hlist = nullptr, *hc = nullptr;  // heavy rows (counted by a workgroup each)
         TSG_CHECK(palloc(ctx, &hlist, (size_t)tilem));
         int rc = palloc(ctx, &hc, 4);
-        if (!rc && B->numtile > HEAVY_NB) {  // a heavy row reaches more B tiles than that
+        // a heavy row reaches more than HEAVY_NB B tiles, so one operand has a
+        // tile-row of more than HEAVY_LIST tiles (flags bit 1 marks those
+        // tsg_csr2tile saw; without it the fill just rebuilds every bitmap)
+        if (!rc && B->numtile > HEAVY_NB && ((A->flags | B->flags) & 2)) {
             p.hb.cap = std::min<int64_t>(HEAVY_BITMAP_WORDS, (int64_t)tilem * ((int64_t)B->tilen / 32 + 1));
             rc = palloc(ctx, &p.hb.words, (size_t)p.hb.cap);
             if (!rc) rc = palloc(ctx, &p.hb.off, (size_t)tilem);

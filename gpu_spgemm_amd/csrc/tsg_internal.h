@@ -128,8 +128,8 @@ constexpr int64_t HEAVY_NB = 8192;
 // B tile-row lists longer than this are walked by a whole workgroup (or
 // OR-ed as bitmaps); tsg_tile_matrix.flags bit 1 marks a matrix with such rows.
 constexpr int HEAVY_LIST = 64;
-// Words of union bitmaps kept from the heavy rows' count for their fill (64 MB).
-constexpr int64_t HEAVY_BITMAP_WORDS = 16ll << 20;
+// Words of union bitmaps kept from the heavy rows' count for their fill (256 MB).
+constexpr int64_t HEAVY_BITMAP_WORDS = 64ll << 20;
 // Union bitmaps the heavy rows' Step-1 count leaves for their fill: the
 // bitmap of tile-row bi (bit i = tile column cmin + i, cmin its first
 // reachable column) starts at words[off[bi]], or off[bi] = -1 when it did not
