@@ -196,6 +196,7 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
     __shared__ int32_t s_bw[HS_THREADS];
     __shared__ int32_t s_bl[HS_THREADS];
     __shared__ int s_nb;
+    __shared__ int32_t s_tpre[HS_THREADS];  // fill: first window entry of each thread's words
     __shared__ int64_t s_qbs[HS_THREADS];
     __shared__ int32_t s_qlen[HS_THREADS];
     __shared__ int s_w[HS_THREADS / WAVE];
@@ -345,18 +346,43 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
                     if (w < tid / WAVE) before += s_w[w];
                     tot += s_w[w];
                 }
-                int64_t pos = out + before + incl - cnt;
+                // entry e of the window (ascending column) is written by thread
+                // e % HS_THREADS, so the stores are coalesced: the owner of its
+                // words is found by a binary search of the threads' starts, the
+                // word by the owner's word counts, the bit by halving selects
+                s_tpre[tid] = before + incl - cnt;
+                __syncthreads();
+                for (int e = tid; e < tot; e += HS_THREADS) {
+                    int lo = 0, hi = HS_THREADS - 1;  // last thread starting at or before e
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_tpre[mid] <= e) lo = mid; else hi = mid - 1;
+                    }
+                    int r = e - s_tpre[lo], wd = lo * WPT;
+                    uint32_t x = bm[wd];
+                    for (int q = 1; q < WPT; q++) {  // stops at the word holding entry r
+                        const int pc = __popc(x);
+                        if (r < pc) break;
+                        r -= pc;
+                        x = bm[++wd];
+                    }
+                    int bit = 0;  // the r-th lowest set bit of x
 #pragma unroll
-                for (int q = 0; q < WPT; q++) {
-                    uint32_t b = bm[tid * WPT + q];
-                    while (b) {
-                        const int bit = __ffs(b) - 1;
-                        b &= b - 1;
-                        if (pos < slot_end) {
-                            rowidxC[pos] = bi;
-                            colidxC[pos] = (int32_t)(w0 + (tid * WPT + q) * 32 + bit);
+                    for (int half = 16; half > 0; half >>= 1) {
+                        const uint32_t low = x & ((1u << half) - 1u);
+                        const int c = __popc(low);
+                        if (r >= c) {
+                            r -= c;
+                            x >>= half;
+                            bit += half;
+                        } else {
+                            x = low;
                         }
-                        pos++;
+                    }
+                    const int64_t pos = out + e;
+                    if (pos < slot_end) {
+                        rowidxC[pos] = bi;
+                        colidxC[pos] = (int32_t)(w0 + (int64_t)wd * 32 + bit);
                     }
                 }
                 out += tot;
