@@ -675,10 +675,11 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
                     const int ea = s.aoff[t] + s.aptr[t * 16 + r];
                     const V *bv = cvalB + (s.bgoff[b] + s.bcptr[b * 16 + c]);
                     do {
-                        const int kk = __clz(m) - 16;
-                        m ^= 0x8000u >> kk;
-                        const uint32_t up = below(kk);
-                        acc = fma_v<V>(s.aval[ea + __popc(ma & up)], bv[__popc(cm & up)], acc);
+                        // top set bit hb (m != 0): k = 15 - hb; the entries before it
+                        // in A's row and B's column are the set bits above hb
+                        const int hb = 31 - __builtin_clz(m);
+                        m ^= 1u << hb;
+                        acc = fma_v<V>(s.aval[ea + __popc(ma >> (hb + 1))], bv[__popc(cm >> (hb + 1))], acc);
                     } while (m);
                 }
                 res[it] = acc;
