@@ -299,7 +299,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported here "
-              f"(the product is a single-GPU kernel path; sharding is described in DESIGN.md)",
+              f"(the product is a single-GPU kernel path: shard A's tile-rows over GPUs and run it on each)",
               file=sys.stderr)
         sys.exit(2)
 
