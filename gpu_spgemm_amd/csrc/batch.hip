@@ -289,10 +289,11 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
     }
     pool_free(ctx, drow);
     pool_free(ctx, dbounds);
-    // split C into the per-problem outputs
+    // split C into the per-problem outputs (all cleared first: a failure part
+    // way frees only what was made)
+    for (int32_t i = 0; i < n; i++) std::memset(&C[i], 0, sizeof C[i]);
     for (int32_t i = 0; i < n && !rc; i++) {
         tsg_tile_matrix &c = C[i];
-        std::memset(&c, 0, sizeof c);
         c.m = A[i].m; c.n = B[i].n; c.tilem = A[i].tilem; c.tilen = B[i].tilen;
         c.dtype = dtype; c.location = TSG_DEVICE;
         c.numtile = hb[2 * i + 2] - hb[2 * i];
