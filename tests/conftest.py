@@ -45,3 +45,18 @@ def assert_tiles_equal(got, exp, keys=None, label=""):
         if not ok:
             bad = np.nonzero(a != b)[0][:5]
             raise AssertionError(f"{label}{k}: first mismatches at {bad}: {a[bad]} vs {b[bad]}")
+
+
+CSC_KEYS = ("tile_csc_Ptr", "tile_csc_Row", "tile_csc_Val")
+
+
+def assert_csc_equal(got, exp, label=""):
+    """Bit-exact comparison of within-tile CSC arrays (tile_csc_Ptr/Row/Val)."""
+    for k in CSC_KEYS:
+        a, b = np.asarray(got[k]), np.asarray(exp[k])
+        assert a.shape == b.shape, f"{label}{k}: shape {a.shape} != {b.shape}"
+        if a.dtype.kind == "f":
+            vt = np.uint64 if a.dtype == np.float64 else np.uint32
+            a, b = a.view(vt), b.astype(a.dtype).view(vt)
+        assert np.array_equal(a, b), f"{label}{k}: first mismatches at {np.nonzero(a != b)[0][:5]}"
+

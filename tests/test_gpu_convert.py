@@ -9,7 +9,7 @@ import pytest
 
 import gpu_spgemm_amd as g
 from oracle import oracle as O
-from tests.conftest import assert_tiles_equal
+from tests.conftest import assert_csc_equal, assert_tiles_equal
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "tile_format_golden.npz")
@@ -54,19 +54,6 @@ def test_gpu_conversion_matches_oracle(ctx, name, f32):
     h = A.astype(g.F32) if f32 else A
     assert_tiles_equal(ctx.tiles_from_csr(h).download(), O.csr2tile(A, f32=f32), label="A ")
     assert_tiles_equal(ctx.tiles_from_csr(h, as_b=True).download(), O.csr2tile_b(A, f32=f32), label="B ")
-
-
-CSC_KEYS = ("tile_csc_Ptr", "tile_csc_Row", "tile_csc_Val")
-
-
-def assert_csc_equal(got, exp, label=""):
-    for k in CSC_KEYS:
-        a, b = np.asarray(got[k]), np.asarray(exp[k])
-        assert a.shape == b.shape, f"{label}{k}: shape {a.shape} != {b.shape}"
-        if a.dtype.kind == "f":
-            vt = np.uint64 if a.dtype == np.float64 else np.uint32
-            a, b = a.view(vt), b.astype(a.dtype).view(vt)
-        assert np.array_equal(a, b), f"{label}{k}: first mismatches at {np.nonzero(a != b)[0][:5]}"
 
 
 @pytest.mark.parametrize("name", CASES)

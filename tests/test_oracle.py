@@ -20,7 +20,7 @@ import pytest
 import scipy.sparse as sp
 
 from oracle import oracle as O
-from tests.conftest import assert_tiles_equal
+from tests.conftest import assert_csc_equal, assert_tiles_equal
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "tile_format_golden.npz")
 CASES = ["er200", "lap20x13", "rect77x300", "dense40", "sparse50"]
@@ -96,18 +96,6 @@ def test_csr2tile_matches_reference_live(seed):
     A = rand_csr(m, n, float(rng.uniform(0.005, 0.2)), seed)
     assert_tiles_equal(O.csr2tile(A), O.ref_csr2tile(A), label="A ")
     assert_tiles_equal(O.csr2tile_b(A), O.ref_csr2tile(A, as_b=True), label="B ")
-
-
-CSC_KEYS = ("tile_csc_Ptr", "tile_csc_Row", "tile_csc_Val")
-
-
-def assert_csc_equal(got, exp, label=""):
-    for k in CSC_KEYS:
-        a, b = np.asarray(got[k]), np.asarray(exp[k])
-        assert a.shape == b.shape, f"{label}{k}: shape {a.shape} != {b.shape}"
-        if a.dtype.kind == "f":
-            a, b = a.view(np.uint64), b.astype(np.float64).view(np.uint64)
-        assert np.array_equal(a, b), f"{label}{k}: first mismatches at {np.nonzero(a != b)[0][:5]}"
 
 
 def with_duplicates(A, seed):
