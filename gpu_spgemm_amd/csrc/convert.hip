@@ -490,7 +490,7 @@ static int csr2tile_impl(Context *ctx, const tsg_csr *A, tsg_tile_matrix *T, int
     }
     if (ts != 16) {  // no scalar CSR copy: no product runs at this tile size
         if ((rc = read_i32(ctx, flags, &hf))) return cleanup(rc);
-        T->flags = ((hf & 2) ? 1 : 0) | ((hf & 8) ? 2 : 0);
+        T->flags = ((hf & 2) ? 1 : 0) | ((hf & 8) ? 2 : 0) | 4;  // bit 2: tile-row lengths checked
         pool_free(ctx, flags);
         return TSG_OK;
     }
@@ -507,7 +507,7 @@ static int csr2tile_impl(Context *ctx, const tsg_csr *A, tsg_tile_matrix *T, int
           hipMemcpyAsync(T->row_val, A->val, vb, hipMemcpyDeviceToDevice, s) != hipSuccess)))
         return cleanup(fail(TSG_ERR_HIP, "copy of the scalar CSR failed"));
     if ((rc = read_i32(ctx, flags, &hf))) return cleanup(rc);
-    T->flags = ((hf & 2) ? 1 : 0) | ((hf & 8) ? 2 : 0);
+    T->flags = ((hf & 2) ? 1 : 0) | ((hf & 8) ? 2 : 0) | 4;  // bit 2: tile-row lengths checked
     pool_free(ctx, flags);
     return TSG_OK;
 }

@@ -68,7 +68,10 @@ void free_plan(Context *ctx, Plan &p);
 // Frees the heavy rows' union bitmaps (after the Step-1 fill).
 void free_heavy_bitmaps(Context *ctx, Plan &p);
 // Allocates the plan's arrays and computes the A->B links.
-int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const BinTable &bt, Plan &p);
+// zero_flags: clear the capacity flag here (the exact path clears it with its
+// heavy-row counters in one launch).
+int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const BinTable &bt, Plan &p,
+               bool zero_flags = true);
 // Partitions rows by p.bin and gathers hdr = [*total0, *total1 (or 0), bin offsets].
 int plan_bin(Context *ctx, Plan &p, int32_t tilem, const int64_t *total0, const int64_t *total1);
 // Reads hdr to the host: totals and bin offsets (one blocking read).

@@ -752,7 +752,8 @@ void free_plan(Context *ctx, Plan &p) {
 
 static unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const BinTable &bt, Plan &p) {
+int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const BinTable &bt, Plan &p,
+               bool zero_flags) {
     const int64_t tilem = A->tilem;
     const int64_t nb = (tilem + BIN_TILE - 1) / BIN_TILE;
     const bool ok = bt.nvar > 0;
@@ -763,14 +764,14 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
     TSG_CHECK(palloc(ctx, &p.bin, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.counts, (size_t)(p.nbins * nb + 1)));
     TSG_CHECK(palloc(ctx, &p.hdr, MAX_BINS + 3));
-    TSG_CHECK(palloc(ctx, &p.overflow, 1));
+    TSG_CHECK(palloc(ctx, &p.overflow, 8));  // [0] the capacity flag; [4..7] the exact path's heavy-row counters
     TSG_CHECK(palloc(ctx, &p.link, (size_t)A->numtile));
     TSG_CHECK(palloc(ctx, &p.lpay, (size_t)A->numtile));
     if (p.rg) {
         TSG_CHECK(palloc(ctx, &p.rgw, (size_t)tilem + 3));
         TSG_CHECK(palloc(ctx, &p.rgn, (size_t)tilem + 1));
     }
-    TSG_CHECK(zero_small(ctx, p.overflow, 1, nullptr, 0));
+    if (zero_flags) TSG_CHECK(zero_small(ctx, p.overflow, 1, nullptr, 0));
     if (A->numtile > 0) {
         k_ab_link<<<nblk(A->numtile, 256), 256, 0, ctx->stream>>>(A->numtile, A->tile_columnidx, B->tile_ptr,
                                                                    ok ? B->row_nnz_ptr : nullptr, p.link, p.lpay);
@@ -809,17 +810,20 @@ int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1) {
 static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                      const tsg_options *opts, Plan &p, int32_t *stats_out) {
     const BinTable bt = bin_table_for(ctx, A, B, opts);
-    TSG_CHECK(plan_setup(ctx, A, B, bt, p));
     const int32_t tilem = A->tilem;
+    TSG_CHECK(plan_setup(ctx, A, B, bt, p, tilem == 0));
     if (tilem == 0) {
         TSG_HIP(hipMemsetAsync(C->tile_ptr, 0, sizeof(int64_t), ctx->stream));
     } else {
-        int32_t *hlist = nullptr, *hc = nullptr;  // heavy rows (counted by a workgroup each)
-        TSG_CHECK(palloc(ctx, &hlist, (size_t)tilem));
-        int rc = palloc(ctx, &hc, 4);
+        // heavy rows (counted by a workgroup each); hc = {count, pad, 64-bit work counter}
+        int32_t *hlist = nullptr, *hc = p.overflow + 4;
         // a heavy row reaches more than HEAVY_NB B tiles, so one operand has a
-        // tile-row of more than HEAVY_LIST tiles (flags bit 1 marks those
-        // tsg_csr2tile saw; without it the fill just rebuilds every bitmap)
+        // tile-row of more than HEAVY_LIST tiles: none when tsg_csr2tile
+        // checked both (flags bit 2) and found none (bit 1)
+        const bool no_heavy = (A->flags & 4) && (B->flags & 4) && !((A->flags | B->flags) & 2);
+        int rc = no_heavy ? TSG_OK : palloc(ctx, &hlist, (size_t)tilem);
+        // keep union bitmaps only where long tile-rows are known (without them
+        // the fill just rebuilds every bitmap)
         if (!rc && B->numtile > HEAVY_NB && ((A->flags | B->flags) & 2)) {
             p.hb.cap = std::min<int64_t>(HEAVY_BITMAP_WORDS, (int64_t)tilem * ((int64_t)B->tilen / 32 + 1));
             rc = palloc(ctx, &p.hb.words, (size_t)p.hb.cap);
@@ -827,16 +831,15 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
             if (!rc) rc = palloc(ctx, reinterpret_cast<int64_t **>(&p.hb.top), 1);
             if (!rc && (B->flags & 2)) rc = build_brow_bitmaps(ctx, B, p.hb);  // long B tile-rows exist
         }
-        if (!rc) rc = zero_small(ctx, hc, 4, reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
+        if (!rc) rc = zero_small(ctx, p.overflow, 8, reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
         if (!rc) {
             k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
                 A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
                 hlist, hc);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_stats launch");
         }
-        if (!rc) rc = launch_heavy_count(ctx, A, B, C->tile_ptr, hlist, hc, stats_out, p.hb);
+        if (!rc && hlist) rc = launch_heavy_count(ctx, A, B, C->tile_ptr, hlist, hc, stats_out, p.hb);
         pool_free(ctx, hlist);
-        pool_free(ctx, hc);
         TSG_CHECK(rc);
         TSG_CHECK(scan_inclusive_i64(ctx, C->tile_ptr, (int64_t)tilem + 1));
     }

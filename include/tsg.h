@@ -73,7 +73,8 @@ typedef struct tsg_tile_matrix {
     int32_t is_csc;           /* 1 for a B operand laid out by tsg_build_csc_tiles */
     int32_t flags;            /* bit 0: some row holds a column twice (duplicates kept);
                                  bit 1: some tile-row holds more than 64 tiles (set by
-                                 tsg_csr2tile; products then keep bitmaps of such B tile-rows) */
+                                 tsg_csr2tile; products then keep bitmaps of such B tile-rows);
+                                 bit 2: bit 1 is known (tsg_csr2tile checked every tile-row) */
     int64_t *tile_ptr;        /* [tilem+1] */
     int32_t *tile_columnidx;  /* [numtile] */
     int32_t *tile_rowidx;     /* [numtile] row of each tile (C output), may be NULL for A/B */
