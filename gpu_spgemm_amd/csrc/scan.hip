@@ -1,4 +1,5 @@
-// scan.hip -- device-wide int64 inclusive scan (reduce, scan partials, apply).
+// scan.hip -- device-wide int64 inclusive scan (reduce, apply; partials scanned
+// in between only for more than 2048 blocks).
 //
 // Replaces the reference's cub::DeviceScan::ExclusiveSum calls
 // (src/spgemm_raw_v2.cu:131-138 and :202-208).  Callers store 0 in data[0] and
@@ -34,13 +35,23 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const int64_t *__r
     }
 }
 
-// Inclusive scan of one SCAN_TILE chunk per block, plus the carried-in offset
-// (inclusive partial of the previous block).
+// Inclusive scan of one SCAN_TILE chunk per block, plus the carried-in offset:
+// the inclusive partial of the previous block (partial_incl), or the sum of
+// the earlier blocks' raw partials added up by this block (partial_raw, few
+// blocks: saves scanning the partials in a launch of their own).
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict__ data, int64_t n,
-                                                              const int64_t *__restrict__ partial_incl) {
+                                                              const int64_t *__restrict__ partial_incl,
+                                                              const int64_t *__restrict__ partial_raw) {
     __shared__ int64_t s_d[SCAN_TILE + SCAN_TILE / 32];
     __shared__ int64_t s_w[SCAN_THREADS / WAVE];
+    __shared__ int64_t s_c[SCAN_THREADS / WAVE];
     int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    int64_t carry = 0;
+    if (partial_raw) {
+        for (int64_t j = threadIdx.x; j < blockIdx.x; j += SCAN_THREADS) carry += partial_raw[j];
+        carry = wave_sum(carry);
+        if (lane_id() == 0) s_c[threadIdx.x / WAVE] = carry;
+    }
     auto pad = [](int i) { return i + (i >> 5); };
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) {
@@ -61,6 +72,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict
     if (lane_id() == WAVE - 1) s_w[w] = incl;
     __syncthreads();
     int64_t off = (blockIdx.x > 0 && partial_incl) ? partial_incl[blockIdx.x - 1] : 0;
+    if (partial_raw)
+        for (int q = 0; q < SCAN_THREADS / WAVE; q++) off += s_c[q];
     for (int q = 0; q < w; q++) off += s_w[q];
     off += incl - run;
 #pragma unroll
@@ -95,7 +108,7 @@ int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n) {
     int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "scan of %lld items exceeds grid limit", (long long)n);
     if (nb == 1) {
-        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, nullptr);
+        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, nullptr, nullptr);
         TSG_HIP(hipGetLastError());
         return TSG_OK;
     }
@@ -104,9 +117,13 @@ int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n) {
     k_scan_reduce<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, partial);
     int rc = TSG_OK;
     if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_reduce launch failed");
-    if (rc == TSG_OK) rc = scan_inclusive_i64(ctx, partial, nb);
+    // up to SCAN_TILE blocks each add up the raw partials before them (at most
+    // 16 KB of reads, cached); more blocks scan the partials first
+    const bool raw = nb <= SCAN_TILE;
+    if (rc == TSG_OK && !raw) rc = scan_inclusive_i64(ctx, partial, nb);
     if (rc == TSG_OK) {
-        k_scan_apply<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, partial);
+        k_scan_apply<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, raw ? nullptr : partial,
+                                                                     raw ? partial : nullptr);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_apply launch failed");
     }
     pool_free(ctx, partial);
