@@ -958,12 +958,11 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     if (!rc) rc = palloc(ctx, &C->tile_csr_Ptr, (size_t)C->numtile * 16);
     if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)C->numtile + 1);
     if (timing && !rc) (void)hipEventRecord(ev[1], s);
-    if (!rc) rc = zero_small(ctx, nullptr, 0, C->tile_nnz, 1);
     SlotLayout exact;
     exact.overflow = p.overflow;
     if (!rc) rc = run_symbolic(ctx, A, B, C, p, exact);
     free_heavy_bitmaps(ctx, p);  // the fill has used them
-    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1);
+    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1, true);  // tile_nnz[0] = 0 by the scan
     if (!rc) rc = prepare_numeric(ctx, A, C, p, exact);
     if (!rc) {
         // nnzC and the capacity flag in one read

@@ -16,14 +16,15 @@ constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const int64_t *__restrict__ data,
-                                                               int64_t n, int64_t *__restrict__ partial) {
+                                                               int64_t n, int64_t *__restrict__ partial,
+                                                               bool zero_first) {
     __shared__ int64_t s_w[SCAN_THREADS / WAVE];
     int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
     int64_t s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; i++) {
         int64_t idx = base + i * SCAN_THREADS + threadIdx.x;
-        if (idx < n) s += data[idx];
+        if (idx < n && !(zero_first && idx == 0)) s += data[idx];
     }
     s = wave_sum(s);
     if (lane_id() == 0) s_w[threadIdx.x / WAVE] = s;
@@ -41,7 +42,8 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const int64_t *__r
 // blocks: saves scanning the partials in a launch of their own).
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict__ data, int64_t n,
                                                               const int64_t *__restrict__ partial_incl,
-                                                              const int64_t *__restrict__ partial_raw) {
+                                                              const int64_t *__restrict__ partial_raw,
+                                                              bool zero_first) {
     __shared__ int64_t s_d[SCAN_TILE + SCAN_TILE / 32];
     __shared__ int64_t s_w[SCAN_THREADS / WAVE];
     __shared__ int64_t s_c[SCAN_THREADS / WAVE];
@@ -57,7 +59,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict
     for (int i = 0; i < SCAN_ITEMS; i++) {
         int li = i * SCAN_THREADS + threadIdx.x;
         int64_t idx = base + li;
-        s_d[pad(li)] = idx < n ? data[idx] : 0;
+        s_d[pad(li)] = (idx < n && !(zero_first && idx == 0)) ? data[idx] : 0;
     }
     __syncthreads();
     int64_t v[SCAN_ITEMS];
@@ -103,18 +105,18 @@ int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb) {
     return TSG_OK;
 }
 
-int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n) {
+int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first) {
     if (n <= 0) return TSG_OK;
     int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "scan of %lld items exceeds grid limit", (long long)n);
     if (nb == 1) {
-        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, nullptr, nullptr);
+        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, nullptr, nullptr, zero_first);
         TSG_HIP(hipGetLastError());
         return TSG_OK;
     }
     int64_t *partial = nullptr;
     TSG_CHECK(palloc(ctx, &partial, (size_t)nb));
-    k_scan_reduce<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, partial);
+    k_scan_reduce<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, partial, zero_first);
     int rc = TSG_OK;
     if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_reduce launch failed");
     // up to SCAN_TILE blocks each add up the raw partials before them (at most
@@ -123,7 +125,7 @@ int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n) {
     if (rc == TSG_OK && !raw) rc = scan_inclusive_i64(ctx, partial, nb);
     if (rc == TSG_OK) {
         k_scan_apply<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, raw ? nullptr : partial,
-                                                                     raw ? partial : nullptr);
+                                                                     raw ? partial : nullptr, zero_first);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_apply launch failed");
     }
     pool_free(ctx, partial);
