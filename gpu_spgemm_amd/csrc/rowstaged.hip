@@ -260,10 +260,17 @@ k_bin_count(const uint8_t *__restrict__ bin, int64_t n, int nbins, int64_t nbloc
 // [0, (bin, block) counts in bin-major order].
 __global__ void __launch_bounds__(BIN_THREADS)
 k_bin_scatter(const uint8_t *__restrict__ bin, int64_t n, int nbins, int64_t nblocks,
-              const int64_t *__restrict__ counts, int32_t *__restrict__ rows) {
+              const int64_t *__restrict__ counts, int32_t *__restrict__ rows, const int64_t *__restrict__ total0,
+              const int64_t *__restrict__ total1, int64_t *__restrict__ hdr) {
     __shared__ int s_wc[BIN_THREADS / WAVE][MAX_BINS];
     __shared__ int64_t s_base[MAX_BINS];
     const int w = threadIdx.x / WAVE, lane = lane_id();
+    if (blockIdx.x == 0) {  // the header of the host read (as k_bin_header)
+        const int t = threadIdx.x;
+        if (t == 0) hdr[0] = total0 ? *total0 : 0;
+        if (t == 1) hdr[1] = total1 ? *total1 : 0;
+        if (t <= nbins) hdr[2 + t] = counts[(int64_t)t * nblocks];
+    }
     if (threadIdx.x < nbins) s_base[threadIdx.x] = counts[threadIdx.x * nblocks + blockIdx.x];
     __syncthreads();
     for (int round = 0; round < BIN_ITEMS; round++) {
@@ -787,11 +794,12 @@ int plan_bin(Context *ctx, Plan &p, int32_t tilem, const int64_t *total0, const 
         k_bin_count<<<(unsigned)nb, BIN_THREADS, 0, s>>>(p.bin, tilem, p.nbins, nb, p.counts);
         TSG_HIP(hipGetLastError());
         TSG_CHECK(scan_inclusive_i64(ctx, p.counts, p.nbins * nb + 1));
-        k_bin_scatter<<<(unsigned)nb, BIN_THREADS, 0, s>>>(p.bin, tilem, p.nbins, nb, p.counts, p.rows);
+        k_bin_scatter<<<(unsigned)nb, BIN_THREADS, 0, s>>>(p.bin, tilem, p.nbins, nb, p.counts, p.rows, total0,
+                                                            total1, p.hdr);
     } else {
         TSG_HIP(hipMemsetAsync(p.counts, 0, sizeof(int64_t), s));
+        k_bin_header<<<1, 64, 0, s>>>(p.counts, p.nbins, nb, total0, total1, p.hdr);
     }
-    k_bin_header<<<1, 64, 0, s>>>(p.counts, p.nbins, nb, total0, total1, p.hdr);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
