@@ -970,17 +970,19 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     exact.overflow = p.overflow;
     if (!rc) rc = run_symbolic(ctx, A, B, C, p, exact);
     free_heavy_bitmaps(ctx, p);  // the fill has used them
-    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1, true);  // tile_nnz[0] = 0 by the scan
+    // tile_nnz[0] = 0 by the scan; nnzC also lands next to the capacity flag
+    int64_t *nnz_slot = reinterpret_cast<int64_t *>(p.overflow + 2);
+    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1, true, nnz_slot);
     if (!rc) rc = prepare_numeric(ctx, A, C, p, exact);
     if (!rc) {
-        // nnzC and the capacity flag in one read
-        if (hipMemcpyAsync(ctx->pinned, C->tile_nnz + C->numtile, sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipMemcpyAsync(ctx->pinned + 1, p.overflow, sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        // the capacity flag and nnzC in one read (p.overflow[0], p.overflow[2..3])
+        if (hipMemcpyAsync(ctx->pinned, p.overflow, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
             rc = fail(TSG_ERR_HIP, "reading nnzC failed");
         syncs++;
-        C->nnz = ctx->pinned[0];
-        if (!rc && (int32_t)ctx->pinned[1]) rc = fail(TSG_ERR_OVERFLOW, "a staged row exceeded its variant's capacity");
+        C->nnz = ctx->pinned[1];
+        if (!rc && *reinterpret_cast<int32_t *>(ctx->pinned))
+            rc = fail(TSG_ERR_OVERFLOW, "a staged row exceeded its variant's capacity");
     }
     if (timing && !rc) (void)hipEventRecord(ev[2], s);
     if (!rc) rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)C->nnz * vsize(C->dtype));

@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const int64_t *__r
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict__ data, int64_t n,
                                                               const int64_t *__restrict__ partial_incl,
                                                               const int64_t *__restrict__ partial_raw,
-                                                              bool zero_first) {
+                                                              bool zero_first, int64_t *__restrict__ total_out) {
     __shared__ int64_t s_d[SCAN_TILE + SCAN_TILE / 32];
     __shared__ int64_t s_w[SCAN_THREADS / WAVE];
     __shared__ int64_t s_c[SCAN_THREADS / WAVE];
@@ -85,7 +85,10 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict
     for (int i = 0; i < SCAN_ITEMS; i++) {
         int li = i * SCAN_THREADS + threadIdx.x;
         int64_t idx = base + li;
-        if (idx < n) data[idx] = s_d[pad(li)];
+        if (idx < n) {
+            data[idx] = s_d[pad(li)];
+            if (total_out && idx == n - 1) *total_out = s_d[pad(li)];
+        }
     }
 }
 
@@ -105,12 +108,12 @@ int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb) {
     return TSG_OK;
 }
 
-int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first) {
+int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, int64_t *total_out) {
     if (n <= 0) return TSG_OK;
     int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "scan of %lld items exceeds grid limit", (long long)n);
     if (nb == 1) {
-        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, nullptr, nullptr, zero_first);
+        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, nullptr, nullptr, zero_first, total_out);
         TSG_HIP(hipGetLastError());
         return TSG_OK;
     }
@@ -125,7 +128,7 @@ int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first) 
     if (rc == TSG_OK && !raw) rc = scan_inclusive_i64(ctx, partial, nb);
     if (rc == TSG_OK) {
         k_scan_apply<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, raw ? nullptr : partial,
-                                                                     raw ? partial : nullptr, zero_first);
+                                                                     raw ? partial : nullptr, zero_first, total_out);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_apply launch failed");
     }
     pool_free(ctx, partial);

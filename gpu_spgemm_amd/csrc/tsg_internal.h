@@ -101,8 +101,10 @@ constexpr int64_t ROW_CHUNK = int64_t(1) << 23;
 // In-place inclusive scan of n int64 values.  With data[0] = 0 and counts in
 // data[1..n-1], this leaves exclusive offsets (the reference's CUB
 // ExclusiveSum, src/spgemm_raw_v2.cu:131-138, 202-208, widened to int64).
-// zero_first: data[0] is taken as 0 whatever it holds (saves clearing it).
-int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first = false);
+// zero_first: data[0] is taken as 0 whatever it holds (saves clearing it);
+// total_out (device, may be null): also receives data[n-1] after the scan.
+int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first = false,
+                       int64_t *total_out = nullptr);
 // Zeroes a[0..na) and b[0..nb) (at most 64 each) with one small kernel on the context's stream.
 int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb);
 // Reads one int64 from device memory (blocking; counts as a host sync).
