@@ -281,6 +281,22 @@ class Context:
         _check(lib().tsg_device_name(self._p, buf, C.c_size_t(256)))
         return buf.value.decode()
 
+    def stream(self):
+        """The hipStream_t every call of this context is queued on (an int)."""
+        return lib().tsg_stream(self._p) or 0
+
+    def set_stream(self, stream):
+        """Queues every later call on the caller's hipStream_t (an int handle,
+        e.g. torch.cuda.current_stream().cuda_stream); 0 or None: back to the
+        context's own stream."""
+        _check(lib().tsg_set_stream(self._p, C.c_void_p(stream or None)))
+
+    def scan_i64(self, dev_ptr, n, zero_first=False, total_ptr=0):
+        """In-place inclusive scan of n int64 at a device address (asynchronous,
+        on the context's stream); total_ptr (device, optional) gets data[n-1]."""
+        _check(lib().tsg_scan_i64(self._p, C.c_void_p(dev_ptr), C.c_int64(n), C.c_int32(1 if zero_first else 0),
+                                  C.c_void_p(total_ptr or None)))
+
     def pool_stats(self):
         a, b = C.c_int64(), C.c_int64()
         _check(lib().tsg_pool_stats(self._p, C.byref(a), C.byref(b)))

@@ -155,10 +155,17 @@ int tsg_create(int device, tsg_context **out) {
     if (device < 0 || device >= cnt) return fail(TSG_ERR_ARG, "device %d out of range (%d devices)", device, cnt);
     Context *ctx = new Context();
     ctx->device = device;
-    TSG_HIP(hipSetDevice(device));
-    TSG_HIP(hipGetDeviceProperties(&ctx->prop, device));
-    TSG_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    TSG_HIP(hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault));
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&ctx->prop, device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+        if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return fail(TSG_ERR_HIP, "context set-up on device %d failed: %s", device, hipGetErrorString(e));
+    }
+    ctx->stream = ctx->own_stream;
     *out = reinterpret_cast<tsg_context *>(ctx);
     return TSG_OK;
 }
@@ -173,7 +180,7 @@ int tsg_destroy(tsg_context *c) {
     for (auto &kv : ctx->pool.free_blocks) (void)hipFree(kv.second);
     for (auto &kv : ctx->pool.live) (void)hipFree(kv.first);
     (void)hipHostFree(ctx->pinned);
-    (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
     return TSG_OK;
 }
@@ -196,6 +203,17 @@ int tsg_device_name(tsg_context *c, char *buf, size_t len) {
 void *tsg_stream(tsg_context *c) {
     Context *ctx = reinterpret_cast<Context *>(c);
     return ctx ? (void *)ctx->stream : nullptr;
+}
+
+int tsg_set_stream(tsg_context *c, void *stream) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx) return fail(TSG_ERR_ARG, "null context");
+    DeviceGuard g(ctx->device);
+    // work already queued on the previous stream finishes before the switch,
+    // so nothing of this context runs on two streams at once
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->stream = stream ? static_cast<hipStream_t>(stream) : ctx->own_stream;
+    return TSG_OK;
 }
 
 int tsg_pool_stats(tsg_context *c, int64_t *in_use, int64_t *cached) {
@@ -405,6 +423,10 @@ int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B) {
     if (!A->mask) return fail(TSG_ERR_ARG, "A lacks row masks");
     if (A->tilem > MAX_TILE_ROWS || B->tilen > MAX_TILE_ROWS)
         return fail(TSG_ERR_OVERFLOW, "more than %lld tile-rows or tile-columns", (long long)MAX_TILE_ROWS);
+    // the kernels address operand tiles (and B tile-row starts) with int32 ids
+    if (A->numtile > INT32_MAX || B->numtile > INT32_MAX)
+        return fail(TSG_ERR_OVERFLOW, "operand tiles (%lld / %lld) exceed int32 tile ids", (long long)A->numtile,
+                    (long long)B->numtile);
     return TSG_OK;
 }
 

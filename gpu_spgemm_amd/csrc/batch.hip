@@ -149,6 +149,10 @@ int spgemm_batched(Context *ctx, int32_t n, const tsg_tile_matrix *A, const tsg_
     }
     if (R[n] > MAX_TILE_ROWS || K[n] > MAX_TILE_ROWS || Cc[n] > MAX_TILE_ROWS)
         return fail(TSG_ERR_OVERFLOW, "batch dimensions exceed %lld tile-rows", (long long)MAX_TILE_ROWS);
+    // the block-diagonal operands are addressed with int32 tile ids like any other
+    if (at[n] > INT32_MAX || bt[n] > INT32_MAX)
+        return fail(TSG_ERR_OVERFLOW, "the batch's operand tiles (%lld / %lld) exceed int32 tile ids", (long long)at[n],
+                    (long long)bt[n]);
     const size_t vs = vsize(dtype);
     tsg_tile_matrix BA{}, BB{}, BC{};
     BA.m = (int32_t)(R[n] * 16); BA.n = (int32_t)(K[n] * 16); BA.tilem = (int32_t)R[n]; BA.tilen = (int32_t)K[n];

@@ -455,7 +455,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
             hipSuccess)
             rc = fail(TSG_ERR_HIP, "upload of the plan sizes failed");
         if (!rc) {
-            k_plan_guard<<<1, 256, 0, s>>>(p.hdr, expect, p.nbins + 3, slotp, tilem, p.overflow);
+            k_plan_guard<<<1, 256, 0, s>>>(p.hdr, expect, p.nbins + 3, slotp, tilem, p.overflow());
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_plan_guard launch");
         }
     }
@@ -511,7 +511,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     Cp.numtile = slots;
     SlotLayout sl;
     sl.row_nc = row_nc;
-    sl.overflow = p.overflow;
+    sl.overflow = p.overflow();
     sl.val_cap = vcap;
     if (!rc) rc = run_symbolic(ctx, A, B, &Cp, p, sl);
     if (!rc) rc = scan_inclusive_i64(ctx, pnnz, slots + 1);
@@ -536,7 +536,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     if (timing && !rc) (void)hipEventRecord(ev[3], s);
     bool redo = false;
     if (!rc) {
-        k_learned_result<<<1, 64, 0, s>>>(p.overflow, C->tile_ptr, tilem, pnnz, slotp, res);
+        k_learned_result<<<1, 64, 0, s>>>(p.overflow(), C->tile_ptr, tilem, pnnz, slotp, res);
         if (hipGetLastError() != hipSuccess ||
             hipMemcpyAsync(ctx->pinned, res, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)

@@ -1,6 +1,8 @@
 // rowstaged.h -- host-side interface of the binned pipeline (rowstaged.hip),
 // shared with the learned scheduler (learned.hip).
 #pragma once
+#include <cstddef>
+
 #include "tsg_internal.h"
 
 namespace tsg {
@@ -24,13 +26,28 @@ struct BinTable {
 };
 constexpr int RG_DENSITY_DEFAULT = 64;
 
+// Device-side flags and counters of one binned product, cleared together by
+// one small kernel and read back together with nnz(C).
+struct PlanFlags {
+    int32_t cap;                    // a staged row exceeded its variant's capacity (or C's value capacity)
+    int32_t pad0;
+    int64_t nnz;                    // nnz(C), written by the tile_nnz scan
+    int32_t heavy_rows;             // length of the heavy-row list built by k_row_stats
+    int32_t pad1;
+    unsigned long long heavy_work;  // work counter of the heavy rows' Step-1 count
+};
+static_assert(offsetof(PlanFlags, nnz) == 8 && offsetof(PlanFlags, heavy_rows) == 16 &&
+                  offsetof(PlanFlags, heavy_work) == 24 && sizeof(PlanFlags) == 32,
+              "PlanFlags layout: the heavy-row kernels take heavy_rows and heavy_work as {int32, pad, uint64}");
+constexpr int PLAN_FLAG_WORDS = (int)(sizeof(PlanFlags) / sizeof(int32_t));
+
 // Per-call working set: A->B links, per-row bins, bin-ordered row lists.
 struct Plan {
     int32_t *rows = nullptr;     // row ids grouped by bin, ascending in each bin
     uint8_t *bin = nullptr;      // bin of each tile-row
     int64_t *counts = nullptr;   // (bin, block) counts, scanned
     int64_t *hdr = nullptr;      // [total0, total1, bin offsets...] for one host read
-    int32_t *overflow = nullptr;
+    PlanFlags *flags = nullptr;  // device
     int4 *link = nullptr;        // per A tile: B tile-row start, tiles, payload count
     int64_t *lpay = nullptr;     // per A tile: B tile-row payload start
     int64_t *rgw = nullptr;      // row-by-row bin: symbolic work-item offsets + counters (tilem + 3)
@@ -41,6 +58,8 @@ struct Plan {
     int nbins = 0;
     int nvar = 0;     // staged bins; bin nvar = C-tile kernels
     bool rg = false;  // bin nvar + 1 = row-by-row kernels
+    // device address of the capacity flag (kernels take it as int32_t *)
+    int32_t *overflow() const { return flags ? &flags->cap : nullptr; }
 };
 
 bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);

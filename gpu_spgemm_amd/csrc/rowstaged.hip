@@ -752,7 +752,7 @@ void free_heavy_bitmaps(Context *ctx, Plan &p) {
 
 void free_plan(Context *ctx, Plan &p) {
     free_heavy_bitmaps(ctx, p);
-    void *ps[] = {p.rows, p.bin, p.counts, p.hdr, p.overflow, p.link, p.lpay, p.rgw, p.rgn, p.rgc};
+    void *ps[] = {p.rows, p.bin, p.counts, p.hdr, p.flags, p.link, p.lpay, p.rgw, p.rgn, p.rgc};
     for (void *q : ps) pool_free(ctx, q);
     p = Plan{};
 }
@@ -771,14 +771,14 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
     TSG_CHECK(palloc(ctx, &p.bin, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.counts, (size_t)(p.nbins * nb + 1)));
     TSG_CHECK(palloc(ctx, &p.hdr, MAX_BINS + 3));
-    TSG_CHECK(palloc(ctx, &p.overflow, 8));  // [0] the capacity flag; [4..7] the exact path's heavy-row counters
+    TSG_CHECK(pool_alloc(ctx, reinterpret_cast<void **>(&p.flags), sizeof(PlanFlags)));
     TSG_CHECK(palloc(ctx, &p.link, (size_t)A->numtile));
     TSG_CHECK(palloc(ctx, &p.lpay, (size_t)A->numtile));
     if (p.rg) {
         TSG_CHECK(palloc(ctx, &p.rgw, (size_t)tilem + 3));
         TSG_CHECK(palloc(ctx, &p.rgn, (size_t)tilem + 1));
     }
-    if (zero_flags) TSG_CHECK(zero_small(ctx, p.overflow, 1, nullptr, 0));
+    if (zero_flags) TSG_CHECK(zero_small(ctx, p.overflow(), 1, nullptr, 0));
     if (A->numtile > 0) {
         k_ab_link<<<nblk(A->numtile, 256), 256, 0, ctx->stream>>>(A->numtile, A->tile_columnidx, B->tile_ptr,
                                                                    ok ? B->row_nnz_ptr : nullptr, p.link, p.lpay);
@@ -824,7 +824,7 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
         TSG_HIP(hipMemsetAsync(C->tile_ptr, 0, sizeof(int64_t), ctx->stream));
     } else {
         // heavy rows (counted by a workgroup each); hc = {count, pad, 64-bit work counter}
-        int32_t *hlist = nullptr, *hc = p.overflow + 4;
+        int32_t *hlist = nullptr, *hc = &p.flags->heavy_rows;
         // a heavy row reaches more than HEAVY_NB B tiles, so one operand has a
         // tile-row of more than HEAVY_LIST tiles: none when tsg_csr2tile
         // checked both (flags bit 2) and found none (bit 1)
@@ -839,7 +839,7 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
             if (!rc) rc = palloc(ctx, reinterpret_cast<int64_t **>(&p.hb.top), 1);
             if (!rc && (B->flags & 2)) rc = build_brow_bitmaps(ctx, B, p.hb);  // long B tile-rows exist
         }
-        if (!rc) rc = zero_small(ctx, p.overflow, 8, reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
+        if (!rc) rc = zero_small(ctx, p.overflow(), PLAN_FLAG_WORDS, reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
         if (!rc) {
             k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
                 A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
@@ -862,7 +862,7 @@ static int launch_sym_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
     if (n <= 0) return TSG_OK;
     k_sym_staged<Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
         p.rows, p.bin_off[bin], n, A->tile_ptr, A->mask, p.link, B->tile_columnidx, B->csr_mask, C->tile_ptr,
-        C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, p.overflow, sl.row_nc);
+        C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, p.overflow(), sl.row_nc);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -876,7 +876,7 @@ static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
         p.rows, p.bin_off[bin], n, A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
         p.link, B->tile_columnidx, B->csr_tile_nnz, B->csr_col_Ptr, B->csr_col_mask,
         (const V *)B->csr_col_Val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, (V *)C->tile_csr_Val,
-        C->tile_csr_Col, p.overflow, sl.row_nc, sl.val_cap);
+        C->tile_csr_Col, p.overflow(), sl.row_nc, sl.val_cap);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -967,16 +967,15 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)C->numtile + 1);
     if (timing && !rc) (void)hipEventRecord(ev[1], s);
     SlotLayout exact;
-    exact.overflow = p.overflow;
+    exact.overflow = p.overflow();
     if (!rc) rc = run_symbolic(ctx, A, B, C, p, exact);
     free_heavy_bitmaps(ctx, p);  // the fill has used them
     // tile_nnz[0] = 0 by the scan; nnzC also lands next to the capacity flag
-    int64_t *nnz_slot = reinterpret_cast<int64_t *>(p.overflow + 2);
-    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1, true, nnz_slot);
+    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1, true, &p.flags->nnz);
     if (!rc) rc = prepare_numeric(ctx, A, C, p, exact);
     if (!rc) {
-        // the capacity flag and nnzC in one read (p.overflow[0], p.overflow[2..3])
-        if (hipMemcpyAsync(ctx->pinned, p.overflow, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        // the capacity flag and nnzC in one read
+        if (hipMemcpyAsync(ctx->pinned, p.flags, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
             rc = fail(TSG_ERR_HIP, "reading nnzC failed");
         syncs++;
@@ -1019,7 +1018,7 @@ int replay_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     if (C->nnz == 0) return TSG_OK;
     if (r.path == 0) {
         SlotLayout exact;
-        exact.overflow = r.p.overflow;
+        exact.overflow = r.p.overflow();
         return run_numeric(ctx, A, B, C, r.p, exact);
     }
     tsg_tile_matrix Cm = *C;

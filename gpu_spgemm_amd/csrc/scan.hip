@@ -150,3 +150,15 @@ int read_i32(Context *ctx, const int32_t *dptr, int32_t *out) {
 }
 
 }  // namespace tsg
+
+extern "C" int tsg_scan_i64(tsg_context *c, int64_t *data, int64_t n, int32_t zero_first, int64_t *total) {
+    using namespace tsg;
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || (n > 0 && !data) || n < 0) return fail(TSG_ERR_ARG, "bad arguments");
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    if (prev != ctx->device) TSG_HIP(hipSetDevice(ctx->device));
+    const int rc = scan_inclusive_i64(ctx, data, n, zero_first != 0, total);
+    if (prev >= 0 && prev != ctx->device) (void)hipSetDevice(prev);
+    return rc;
+}

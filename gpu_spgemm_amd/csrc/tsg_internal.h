@@ -60,7 +60,8 @@ struct Pool {
 
 struct Context {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // where every call's work is queued (own_stream or the caller's)
+    hipStream_t own_stream = nullptr;  // created with the context
     hipDeviceProp_t prop{};
     Pool pool;
     std::mutex mu;

@@ -169,9 +169,25 @@ int tsg_synchronize(tsg_context *ctx);
 int tsg_device_name(tsg_context *ctx, char *buf, size_t len);
 /* The HIP stream every call of this context runs on (hipStream_t). */
 void *tsg_stream(tsg_context *ctx);
+/* Runs every later call of this context on the caller's hipStream_t (NULL:
+ * back to the context's own stream), so library work orders with the caller's
+ * own kernels and copies.  The reference queues everything on the legacy
+ * default stream (src/spgemm_raw_v2.cu:120-129, :204-240).  Work already queued
+ * on the previous stream is waited for first; the caller keeps the stream
+ * alive while the context uses it.  Calls that return host values (sizes,
+ * downloads) still synchronize that stream. */
+int tsg_set_stream(tsg_context *ctx, void *stream);
 /* Bytes held by the context's device pool (in use, cached). */
 int tsg_pool_stats(tsg_context *ctx, int64_t *in_use, int64_t *cached);
 int tsg_pool_trim(tsg_context *ctx);
+
+/* In-place inclusive scan of n int64 values in device memory, on the
+ * context's stream (asynchronous): the library's replacement for CUB
+ * DeviceScan::ExclusiveSum (src/spgemm_raw_v2.cu:131-138, :202-208).  With 0 in
+ * data[0] (or zero_first = 1, which takes data[0] as 0 whatever it holds) and
+ * counts in data[1..n-1] it leaves exclusive offsets plus the total in
+ * data[n-1]; total (device, may be NULL) also receives data[n-1]. */
+int tsg_scan_i64(tsg_context *ctx, int64_t *data, int64_t n, int32_t zero_first, int64_t *total);
 
 /* ---- host matrices (generators, MTX I/O) ---- */
 /* MTX reader, replaces load_mtx (src/tile_format.h:51-90): symmetric mirrored,
