@@ -297,6 +297,28 @@ class Context:
         _check(lib().tsg_scan_i64(self._p, C.c_void_p(dev_ptr), C.c_int64(n), C.c_int32(1 if zero_first else 0),
                                   C.c_void_p(total_ptr or None)))
 
+    def malloc(self, nbytes):
+        """A raw device array of nbytes from the context's pool (an int address)."""
+        p = C.c_void_p()
+        _check(lib().tsg_device_malloc(self._p, C.c_size_t(nbytes), C.byref(p)))
+        return p.value
+
+    def free(self, ptr):
+        _check(lib().tsg_device_free(self._p, C.c_void_p(ptr)))
+
+    def to_device(self, arr):
+        """Copies a numpy array into a new device array; returns its address."""
+        a = np.ascontiguousarray(arr)
+        p = self.malloc(a.nbytes)
+        _check(lib().tsg_memcpy(self._p, C.c_void_p(p), a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes)))
+        return p
+
+    def to_host(self, ptr, count, dtype):
+        """Copies count elements of dtype from a device address into numpy."""
+        out = np.empty(count, dtype=dtype)
+        _check(lib().tsg_memcpy(self._p, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), C.c_size_t(out.nbytes)))
+        return out
+
     def pool_stats(self):
         a, b = C.c_int64(), C.c_int64()
         _check(lib().tsg_pool_stats(self._p, C.byref(a), C.byref(b)))

@@ -225,6 +225,30 @@ int tsg_pool_stats(tsg_context *c, int64_t *in_use, int64_t *cached) {
     return TSG_OK;
 }
 
+int tsg_device_malloc(tsg_context *c, size_t bytes, void **ptr) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !ptr) return fail(TSG_ERR_ARG, "null argument");
+    *ptr = nullptr;
+    DeviceGuard g(ctx->device);
+    return pool_alloc(ctx, ptr, bytes ? bytes : 1);
+}
+
+int tsg_device_free(tsg_context *c, void *ptr) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx) return fail(TSG_ERR_ARG, "null context");
+    return pool_free(ctx, ptr);
+}
+
+int tsg_memcpy(tsg_context *c, void *dst, const void *src, size_t bytes) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || (bytes && (!dst || !src))) return fail(TSG_ERR_ARG, "null argument");
+    if (!bytes) return TSG_OK;
+    DeviceGuard g(ctx->device);
+    TSG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, ctx->stream));
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    return TSG_OK;
+}
+
 int tsg_pool_trim(tsg_context *c) {
     Context *ctx = reinterpret_cast<Context *>(c);
     if (!ctx) return fail(TSG_ERR_ARG, "null context");

@@ -180,6 +180,13 @@ int tsg_set_stream(tsg_context *ctx, void *stream);
 /* Bytes held by the context's device pool (in use, cached). */
 int tsg_pool_stats(tsg_context *ctx, int64_t *in_use, int64_t *cached);
 int tsg_pool_trim(tsg_context *ctx);
+/* Raw device arrays from the context's pool (for caller-owned outputs, e.g. of
+ * tsg_spgemm_noalloc), and a blocking copy between any host and device
+ * addresses on the context's stream (hipMemcpyDefault).  The reference uses
+ * cudaMalloc / cudaMemcpy directly (src/spgemm_raw_v2.cu:25-48). */
+int tsg_device_malloc(tsg_context *ctx, size_t bytes, void **ptr);
+int tsg_device_free(tsg_context *ctx, void *ptr);
+int tsg_memcpy(tsg_context *ctx, void *dst, const void *src, size_t bytes);
 
 /* In-place inclusive scan of n int64 values in device memory, on the
  * context's stream (asynchronous): the library's replacement for CUB

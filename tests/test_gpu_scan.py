@@ -3,9 +3,11 @@ reference's CUB ExclusiveSum, src/spgemm_raw_v2.cu:131-138) against numpy's
 cumsum, at the sizes where its strategy changes: one block (n <= 2048), blocks
 that add up the raw partials before them (up to 2048 blocks), and the
 recursive scan of the partials beyond that; with data[0] taken as given or as
-zero (zero_first), and the total written to a separate device word.  The
-context runs on torch's current stream (tsg_set_stream), so torch's copies
-and the library's kernels are ordered on one stream."""
+zero (zero_first), and the total written to a separate device word.  Also:
+a context moved onto a caller's stream (tsg_set_stream) computes the same
+product as on its own stream."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -18,7 +20,6 @@ T = 2048  # items per scan block
 @pytest.mark.parametrize("n", [1, 2, T, T + 1, T * T, T * T + 1, (T + 1) * T + 7])
 @pytest.mark.parametrize("zero_first", [False, True])
 def test_scan_matches_cumsum(ctx, n, zero_first):
-    import torch
     print(f"[case] scan n={n} zero_first={zero_first}", flush=True)
     rng = np.random.default_rng(n + (7 if zero_first else 0))
     x = rng.integers(-(1 << 20), 1 << 40, n, dtype=np.int64)
@@ -26,35 +27,38 @@ def test_scan_matches_cumsum(ctx, n, zero_first):
     if zero_first:
         exp[0] = 0
     exp = np.cumsum(exp)
-    s = torch.cuda.current_stream()
-    ctx.set_stream(s.cuda_stream)
+    d = ctx.to_device(x)
+    tot = ctx.to_device(np.full(2, -1, np.int64))
     try:
-        d = torch.from_numpy(x).to("cuda")
-        tot = torch.full((2,), -1, dtype=torch.int64, device="cuda")
-        ctx.scan_i64(d.data_ptr(), n, zero_first, tot.data_ptr())
-        got = d.cpu().numpy()
-        t = tot.cpu().numpy()
+        ctx.scan_i64(d, n, zero_first, tot)
+        got = ctx.to_host(d, n, np.int64)
+        t = ctx.to_host(tot, 2, np.int64)
     finally:
-        ctx.synchronize()
-        ctx.set_stream(None)
+        ctx.free(d)
+        ctx.free(tot)
     assert np.array_equal(got, exp), f"first mismatch at {np.nonzero(got != exp)[0][:5]}"
     assert t[0] == exp[-1] and t[1] == -1
 
 
 def test_set_stream_round_trip(ctx):
-    import torch
+    hip = C.CDLL("libamdhip64.so.7")   # the runtime libtsg is linked against
+    s = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(s)) == 0
     own = ctx.stream()
-    s = torch.cuda.Stream()
-    ctx.set_stream(s.cuda_stream)
-    assert ctx.stream() == s.cuda_stream
-    A = g.gen_er(1500, 5, seed=3)
-    tA, tB = ctx.tiles_from_csr(A), ctx.tiles_from_csr(A, as_b=True)
-    C1, _ = ctx.spgemm(tA, tB)
-    d1 = C1.download()
-    ctx.set_stream(None)
-    assert ctx.stream() == own
-    C2, _ = ctx.spgemm(tA, tB)
-    d2 = C2.download()
+    try:
+        ctx.set_stream(s.value)
+        assert ctx.stream() == s.value
+        A = g.gen_er(1500, 5, seed=3)
+        tA, tB = ctx.tiles_from_csr(A), ctx.tiles_from_csr(A, as_b=True)
+        C1, _ = ctx.spgemm(tA, tB)
+        d1 = C1.download()
+        ctx.set_stream(None)
+        assert ctx.stream() == own
+        C2, _ = ctx.spgemm(tA, tB)
+        d2 = C2.download()
+    finally:
+        ctx.set_stream(None)
+        hip.hipStreamDestroy(s)
     for k in ("tile_ptr", "tile_columnidx", "tile_nnz", "mask", "tile_csr_Col"):
         assert np.array_equal(d1[k], d2[k])
     assert np.array_equal(d1["tile_csr_Val"].view(np.uint64), d2["tile_csr_Val"].view(np.uint64))
