@@ -360,6 +360,28 @@ class Context:
                                 C.byref(st)))
         return Cm, st.as_dict()
 
+    def spgemm_noalloc(self, A, B, out, cap_tiles, cap_nnz):
+        """C = A*B with no device allocation (tsg_spgemm_noalloc): C is written
+        into the caller's device arrays, `out` a dict of device addresses (ints)
+        named as TileStruct's fields (tile_ptr, tile_rowidx, tile_columnidx,
+        tile_nnz, mask, tile_csr_Ptr, tile_csr_Val, tile_csr_Col).  Returns
+        (TileStruct describing C in those arrays, stats dict); raises TsgError
+        (ERR_OVERFLOW, stats in .stats) when a capacity is too small."""
+        o = Options(SCHED_PLAIN, 0, 0, 0, 1.0, 0)
+        st = Stats()
+        c = TileStruct()
+        ftype = dict(TileStruct._fields_)
+        for k, v in out.items():
+            t = ftype[k]
+            setattr(c, k, v if t is C.c_void_p else C.cast(C.c_void_p(v), t))
+        rc = lib().tsg_spgemm_noalloc(self._p, C.byref(A.s), C.byref(B.s), C.byref(c), C.c_int64(cap_tiles),
+                                      C.c_int64(cap_nnz), C.byref(o), C.byref(st))
+        if rc != 0:
+            err = TsgError(rc, lib().tsg_last_error_string().decode(errors="replace"))
+            err.stats = st.as_dict()
+            raise err
+        return c, st.as_dict()
+
     def spgemm_batched(self, As, Bs, scheduler=SCHED_PLAIN, timing=False, variant=0, pred_scale=1.0,
                        mfma_min_products=0, tuning=0):
         """C_i = A_i*B_i for every pair in one pipeline run; returns ([C_i], [stats_i])."""

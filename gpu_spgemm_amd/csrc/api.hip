@@ -30,6 +30,18 @@ int fail(int code, const char *fmt, ...) {
 }
 
 int pool_alloc(Context *ctx, void **p, size_t bytes) {
+    if (ctx->arena) {  // no-alloc call: bump-allocate from the scratch region
+        Arena &a = *ctx->arena;
+        const size_t r = (bytes + 255) & ~size_t(255);
+        if (r > a.size - a.top) {
+            *p = nullptr;
+            return fail(TSG_ERR_OVERFLOW, "the no-alloc scratch (%zu bytes) cannot hold this product's work arrays "
+                        "(%zu more bytes needed)", a.size, r - (a.size - a.top));
+        }
+        *p = a.base + a.top;
+        a.top += r;
+        return TSG_OK;
+    }
     size_t r = Pool::round(bytes);
     std::lock_guard<std::mutex> lk(ctx->mu);
     auto it = ctx->pool.free_blocks.find(r);
@@ -72,7 +84,7 @@ int pool_view(Context *ctx, void *block, size_t offset, void **out) {
 }
 
 int pool_free(Context *ctx, void *p) {
-    if (!p) return TSG_OK;
+    if (!p || ctx->arena) return TSG_OK;  // arena arrays (and the caller's C) are not the pool's
     std::lock_guard<std::mutex> lk(ctx->mu);
     // a view, or the owner of a block with views: drop one reference
     void *blk = nullptr;
@@ -166,6 +178,13 @@ int tsg_create(int device, tsg_context **out) {
         return fail(TSG_ERR_HIP, "context set-up on device %d failed: %s", device, hipGetErrorString(e));
     }
     ctx->stream = ctx->own_stream;
+    const int rc = noalloc_setup(ctx);  // the no-alloc entry point's scratch on this device
+    if (rc) {
+        (void)hipHostFree(ctx->pinned);
+        (void)hipStreamDestroy(ctx->own_stream);
+        delete ctx;
+        return rc;
+    }
     *out = reinterpret_cast<tsg_context *>(ctx);
     return TSG_OK;
 }

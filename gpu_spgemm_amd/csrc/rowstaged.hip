@@ -722,7 +722,10 @@ bool rg_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options
 BinTable bin_table_for(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
     TuneCfg cfg;
     tuned_config(ctx, A, B, opts, &cfg);
-    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts), cfg.rg_density, cfg.var_mask);
+    // the row-by-row bin needs a per-workgroup cursor area far larger than the
+    // no-alloc scratch: inside such a call heavy rows take the C-tile kernels
+    // (every path computes the same bits)
+    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts) && !ctx->arena, cfg.rg_density, cfg.var_mask);
 }
 
 BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask) {
@@ -832,7 +835,7 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
         int rc = no_heavy ? TSG_OK : palloc(ctx, &hlist, (size_t)tilem);
         // keep union bitmaps only where long tile-rows are known (without them
         // the fill just rebuilds every bitmap)
-        if (!rc && B->numtile > HEAVY_NB && ((A->flags | B->flags) & 2)) {
+        if (!rc && !ctx->arena && B->numtile > HEAVY_NB && ((A->flags | B->flags) & 2)) {
             p.hb.cap = std::min<int64_t>(HEAVY_BITMAP_WORDS, (int64_t)tilem * ((int64_t)B->tilen / 32 + 1));
             rc = palloc(ctx, &p.hb.words, (size_t)p.hb.cap);
             if (!rc) rc = palloc(ctx, &p.hb.off, (size_t)tilem);
@@ -957,14 +960,29 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     int rc = TSG_OK;
     int syncs = 0;
     if (timing) (void)hipEventRecord(ev[0], s);
-    rc = palloc(ctx, &C->tile_ptr, (size_t)C->tilem + 1);
+    const FixedOut *fo = ctx->fixed_out;  // caller-owned C arrays (no-alloc entry point)
+    if (fo) C->tile_ptr = fo->c.tile_ptr;
+    else rc = palloc(ctx, &C->tile_ptr, (size_t)C->tilem + 1);
     if (!rc) { rc = plan_rows(ctx, A, B, C, opts, p, nullptr); syncs++; }
     if (!rc && C->numtile > INT32_MAX) rc = fail(TSG_ERR_OVERFLOW, "numtileC exceeds int32 tile ids");
-    if (!rc) rc = palloc(ctx, &C->tile_rowidx, (size_t)C->numtile);
-    if (!rc) rc = palloc(ctx, &C->tile_columnidx, (size_t)C->numtile);
-    if (!rc) rc = palloc(ctx, &C->mask, (size_t)C->numtile * 16);
-    if (!rc) rc = palloc(ctx, &C->tile_csr_Ptr, (size_t)C->numtile * 16);
-    if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)C->numtile + 1);
+    if (!rc && fo) {
+        if (C->numtile > fo->cap_tiles) {
+            rc = fail(TSG_ERR_OVERFLOW, "C has %lld tiles; the caller's tile arrays hold %lld", (long long)C->numtile,
+                      (long long)fo->cap_tiles);
+        } else {
+            C->tile_rowidx = fo->c.tile_rowidx;
+            C->tile_columnidx = fo->c.tile_columnidx;
+            C->mask = fo->c.mask;
+            C->tile_csr_Ptr = fo->c.tile_csr_Ptr;
+            C->tile_nnz = fo->c.tile_nnz;
+        }
+    } else {
+        if (!rc) rc = palloc(ctx, &C->tile_rowidx, (size_t)C->numtile);
+        if (!rc) rc = palloc(ctx, &C->tile_columnidx, (size_t)C->numtile);
+        if (!rc) rc = palloc(ctx, &C->mask, (size_t)C->numtile * 16);
+        if (!rc) rc = palloc(ctx, &C->tile_csr_Ptr, (size_t)C->numtile * 16);
+        if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)C->numtile + 1);
+    }
     if (timing && !rc) (void)hipEventRecord(ev[1], s);
     SlotLayout exact;
     exact.overflow = p.overflow();
@@ -984,8 +1002,18 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
             rc = fail(TSG_ERR_OVERFLOW, "a staged row exceeded its variant's capacity");
     }
     if (timing && !rc) (void)hipEventRecord(ev[2], s);
-    if (!rc) rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)C->nnz * vsize(C->dtype));
-    if (!rc) rc = palloc(ctx, &C->tile_csr_Col, (size_t)C->nnz);
+    if (!rc && fo) {
+        if (C->nnz > fo->cap_nnz) {
+            rc = fail(TSG_ERR_OVERFLOW, "C has %lld nonzeros; the caller's value arrays hold %lld", (long long)C->nnz,
+                      (long long)fo->cap_nnz);
+        } else {
+            C->tile_csr_Val = fo->c.tile_csr_Val;
+            C->tile_csr_Col = fo->c.tile_csr_Col;
+        }
+    } else {
+        if (!rc) rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)C->nnz * vsize(C->dtype));
+        if (!rc) rc = palloc(ctx, &C->tile_csr_Col, (size_t)C->nnz);
+    }
     if (!rc && C->nnz > 0) rc = run_numeric(ctx, A, B, C, p, exact);
     if (timing && !rc) (void)hipEventRecord(ev[3], s);
     if (st) {

@@ -58,6 +58,20 @@ struct Pool {
     }
 };
 
+// A bump allocator over one fixed device region (the no-alloc entry point's
+// static scratch): while a context holds one, pool_alloc takes its arrays from
+// it and pool_free gives nothing back; the region is reset per call.
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t size = 0, top = 0;
+};
+// Caller-owned output arrays of C (the no-alloc entry point): the pipeline
+// writes C into them instead of allocating, after checking the capacities.
+struct FixedOut {
+    tsg_tile_matrix c{};
+    int64_t cap_tiles = 0, cap_nnz = 0;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;      // where every call's work is queued (own_stream or the caller's)
@@ -71,6 +85,9 @@ struct Context {
     void *learned = nullptr;
     // autotuner state lives in autotune.hip
     void *tuner = nullptr;
+    // set only inside tsg_spgemm_noalloc
+    Arena *arena = nullptr;
+    const FixedOut *fixed_out = nullptr;
 };
 
 int pool_alloc(Context *ctx, void **p, size_t bytes);
@@ -179,6 +196,8 @@ int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_mat
                       const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const int64_t *woffn,
                       int64_t *gcur, const SlotLayout &sl);
 int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B);  // api.hip
+// Looks up the no-alloc scratch's device address on ctx's device (current).
+int noalloc_setup(Context *ctx);  // noalloc.hip
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                    tsg_tile_matrix *C, const tsg_options *opts);
 // Matrix-core variant of launch_numeric (mfma.hip).

@@ -279,6 +279,26 @@ int tsg_tile2csr(tsg_context *ctx, const tsg_tile_matrix *dev, tsg_csr *out_dev)
 int tsg_spgemm(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                tsg_tile_matrix *C, const tsg_options *opts, tsg_stats *stats);
 
+/* C = A*B with no device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The exact binned
+ * pipeline of tsg_spgemm (plain scheduler, numeric_variant 0; timing ignored)
+ * with its work arrays in a static device scratch array of 64 MiB per GPU,
+ * held by the library until the process ends, and C written into the
+ * caller's device arrays, which C carries in: tile_ptr [A->tilem + 1],
+ * tile_rowidx and tile_columnidx [cap_tiles], tile_nnz [cap_tiles + 1], mask and
+ * tile_csr_Ptr [cap_tiles * 16], tile_csr_Val and tile_csr_Col [cap_nnz] (values
+ * of A's dtype).  On success C's sizes are set and its arrays stay the
+ * caller's (do not pass C to tsg_tile_free).  TSG_ERR_OVERFLOW when C exceeds a
+ * capacity (stats->numtileC, and once the tiles fit stats->nnzC, give the sizes
+ * needed) or when A's work arrays exceed the scratch (about 24 bytes per A
+ * tile plus 10 per tile-row; e.g. the 2048^2 5-point Laplacian needs 33 MB).
+ * Heavy tile-rows take the C-tile kernels here, never the row-by-row ones
+ * (whose cursor area does not fit): results are the same bits.  Every call on
+ * one GPU shares its scratch: calls that overlap on one GPU race. */
+int tsg_spgemm_noalloc(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                       int64_t cap_tiles, int64_t cap_nnz, const tsg_options *opts, tsg_stats *stats);
+
 /* ---- Learned scheduler (no reference counterpart: the reference sizes C with
  * exact count kernels and host reads, src/spgemm_raw_v2.cu:119-141, :191-211) ----
  * An MLP (8 features -> 32 -> 32 -> 2, ReLU, FP32; tsg_learned_num_params()
