@@ -590,6 +590,39 @@ class DeviceCSR:
             pass
 
 
+def _host_tiles_to_dict(h):
+    """A library-allocated host TileStruct -> dict of numpy arrays (frees h)."""
+    nt, nnz = h.numtile, h.nnz
+    rows = h.tile_size or 16     # per-tile row entries of tile_csr_Ptr and mask
+    vdt = np.float32 if h.dtype == F32 else np.float64
+    d = dict(m=h.m, n=h.n, tilem=h.tilem, tilen=h.tilen, nnz=nnz, numtile=nt, is_csc=h.is_csc,
+             dtype=h.dtype, tile_size=rows,
+             tile_ptr=_np(h.tile_ptr, h.tilem + 1, np.int64),
+             tile_columnidx=_np(h.tile_columnidx, nt, np.int32),
+             tile_rowidx=_np(h.tile_rowidx, nt, np.int32) if h.tile_rowidx else None,
+             tile_nnz=_np(h.tile_nnz, nt + 1, np.int64),
+             tile_csr_Val=_np(h.tile_csr_Val, nnz, vdt),
+             tile_csr_Col=_np(h.tile_csr_Col, nnz, np.uint8),
+             tile_csr_Ptr=_np(h.tile_csr_Ptr, nt * rows, np.uint8),
+             mask=_np(h.mask, nt * rows, np.uint16))
+    if h.is_csc:
+        d["csc_tile_ptr"] = _np(h.csc_tile_ptr, h.tilen + 1, np.int64)
+        d["csc_tile_rowidx"] = _np(h.csc_tile_rowidx, nt, np.int32)
+        if h.csr_tile_nnz:      # the CSR-order, column-major copy for the staged kernels
+            d["csr_tile_nnz"] = _np(h.csr_tile_nnz, nt + 1, np.int64)
+            d["csr_mask"] = _np(h.csr_mask, nt * 16, np.uint16)
+            d["csr_col_mask"] = _np(h.csr_col_mask, nt * 16, np.uint16)
+            d["csr_col_Ptr"] = _np(h.csr_col_Ptr, nt * 16, np.uint8)
+            d["csr_col_Val"] = _np(h.csr_col_Val, nnz, vdt)
+            d["row_nnz_ptr"] = _np(h.row_nnz_ptr, h.tilem + 1, np.int64)
+    if h.tile_csc_Ptr:          # within-tile CSC (tsg_build_tile_csc)
+        d["tile_csc_Ptr"] = _np(h.tile_csc_Ptr, nt * 16, np.uint8)
+        d["tile_csc_Row"] = _np(h.tile_csc_Row, nnz, np.uint8)
+        d["tile_csc_Val"] = _np(h.tile_csc_Val, nnz, vdt)
+    _check(lib().tsg_tile_free_host(C.byref(h)))
+    return d
+
+
 class DeviceTiles:
     """A tile matrix in device memory (owned by a Context's pool)."""
 
@@ -618,35 +651,15 @@ class DeviceTiles:
         """Returns a dict of numpy arrays in the reference's field names."""
         h = TileStruct()
         _check(lib().tsg_tile_download(self.ctx.handle, C.byref(self.s), C.byref(h)))
-        nt, nnz = h.numtile, h.nnz
-        rows = h.tile_size or 16     # per-tile row entries of tile_csr_Ptr and mask
-        vdt = np.float32 if h.dtype == F32 else np.float64
-        d = dict(m=h.m, n=h.n, tilem=h.tilem, tilen=h.tilen, nnz=nnz, numtile=nt, is_csc=h.is_csc,
-                 dtype=h.dtype, tile_size=rows,
-                 tile_ptr=_np(h.tile_ptr, h.tilem + 1, np.int64),
-                 tile_columnidx=_np(h.tile_columnidx, nt, np.int32),
-                 tile_rowidx=_np(h.tile_rowidx, nt, np.int32) if h.tile_rowidx else None,
-                 tile_nnz=_np(h.tile_nnz, nt + 1, np.int64),
-                 tile_csr_Val=_np(h.tile_csr_Val, nnz, vdt),
-                 tile_csr_Col=_np(h.tile_csr_Col, nnz, np.uint8),
-                 tile_csr_Ptr=_np(h.tile_csr_Ptr, nt * rows, np.uint8),
-                 mask=_np(h.mask, nt * rows, np.uint16))
-        if h.is_csc:
-            d["csc_tile_ptr"] = _np(h.csc_tile_ptr, h.tilen + 1, np.int64)
-            d["csc_tile_rowidx"] = _np(h.csc_tile_rowidx, nt, np.int32)
-            if h.csr_tile_nnz:      # the CSR-order, column-major copy for the staged kernels
-                d["csr_tile_nnz"] = _np(h.csr_tile_nnz, nt + 1, np.int64)
-                d["csr_mask"] = _np(h.csr_mask, nt * 16, np.uint16)
-                d["csr_col_mask"] = _np(h.csr_col_mask, nt * 16, np.uint16)
-                d["csr_col_Ptr"] = _np(h.csr_col_Ptr, nt * 16, np.uint8)
-                d["csr_col_Val"] = _np(h.csr_col_Val, nnz, vdt)
-                d["row_nnz_ptr"] = _np(h.row_nnz_ptr, h.tilem + 1, np.int64)
-        if h.tile_csc_Ptr:          # within-tile CSC (tsg_build_tile_csc)
-            d["tile_csc_Ptr"] = _np(h.tile_csc_Ptr, nt * 16, np.uint8)
-            d["tile_csc_Row"] = _np(h.tile_csc_Row, nnz, np.uint8)
-            d["tile_csc_Val"] = _np(h.tile_csc_Val, nnz, vdt)
-        _check(lib().tsg_tile_free_host(C.byref(h)))
-        return d
+        return _host_tiles_to_dict(h)
+
+    def download_rows(self, tr_lo, tr_hi):
+        """Tile-rows [tr_lo, tr_hi) as a dict (download() layout) of a matrix of
+        tr_hi - tr_lo tile-rows: tile_ptr, tile_nnz, tile_rowidx rebased."""
+        h = TileStruct()
+        _check(lib().tsg_tile_download_rows(self.ctx.handle, C.byref(self.s), C.c_int32(tr_lo), C.c_int32(tr_hi),
+                                            C.byref(h)))
+        return _host_tiles_to_dict(h)
 
     def to_csr(self):
         """Row tiles -> host CSR through the GPU tile2csr kernels."""

@@ -1,5 +1,6 @@
 // api.hip -- C ABI: context, device pool, transfers and the plain SpGEMM
 // pipeline (Steps 1-3 of src/spgemm_raw_v2.cu:111-240).
+#include <algorithm>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -418,6 +419,57 @@ int tsg_tile_download(tsg_context *c, const tsg_tile_matrix *d, tsg_tile_matrix 
         }
     }
     TSG_HIP(hipStreamSynchronize(ctx->stream));
+    *h = t;
+    return TSG_OK;
+}
+
+int tsg_tile_download_rows(tsg_context *c, const tsg_tile_matrix *d, int32_t tr_lo, int32_t tr_hi,
+                           tsg_tile_matrix *h) {
+    Context *ctx = reinterpret_cast<Context *>(c);
+    if (!ctx || !h || !d) return fail(TSG_ERR_ARG, "null argument");
+    if (d->location != TSG_DEVICE) return fail(TSG_ERR_ARG, "tiles are not on the device");
+    if (d->is_csc) return fail(TSG_ERR_ARG, "a tile-row range of a B operand (CSC-tile data order) is not contiguous");
+    if (d->tile_size && d->tile_size != 16) return fail(TSG_ERR_UNSUPPORTED, "16 x 16 tiles only");
+    if (tr_lo < 0 || tr_hi < tr_lo || tr_hi > d->tilem)
+        return fail(TSG_ERR_ARG, "tile-row range [%d, %d) outside [0, %d)", tr_lo, tr_hi, d->tilem);
+    DeviceGuard g(ctx->device);
+    // tile range [t0, t1) and value range [v0, v1) of the rows: two small reads
+    int64_t t0 = 0, t1 = 0, v0 = 0, v1 = 0;
+    TSG_CHECK(read_i64(ctx, d->tile_ptr + tr_lo, &t0));
+    TSG_CHECK(read_i64(ctx, d->tile_ptr + tr_hi, &t1));
+    TSG_CHECK(read_i64(ctx, d->tile_nnz + t0, &v0));
+    TSG_CHECK(read_i64(ctx, d->tile_nnz + t1, &v1));
+    tsg_tile_matrix t{};
+    t.tilem = tr_hi - tr_lo;
+    t.tilen = d->tilen;
+    t.m = (int32_t)std::min<int64_t>((int64_t)t.tilem * 16, std::max<int64_t>(0, (int64_t)d->m - (int64_t)tr_lo * 16));
+    t.n = d->n;
+    t.numtile = t1 - t0;
+    t.nnz = v1 - v0;
+    t.dtype = d->dtype;
+    t.location = TSG_HOST;
+    t.flags = d->flags;
+    const size_t nt = (size_t)t.numtile, vs = vsize(d->dtype);
+    int rc = down(ctx, d->tile_ptr + tr_lo, (size_t)t.tilem + 1, &t.tile_ptr);
+    if (!rc) rc = down(ctx, d->tile_columnidx + t0, nt, &t.tile_columnidx);
+    if (!rc && d->tile_rowidx) rc = down(ctx, d->tile_rowidx + t0, nt, &t.tile_rowidx);
+    if (!rc) rc = down(ctx, d->tile_nnz + t0, nt + 1, &t.tile_nnz);
+    if (!rc) rc = down(ctx, (const uint8_t *)d->tile_csr_Val + v0 * vs, (size_t)t.nnz * vs, (uint8_t **)&t.tile_csr_Val);
+    if (!rc) rc = down(ctx, d->tile_csr_Col + v0, (size_t)t.nnz, &t.tile_csr_Col);
+    if (!rc) rc = down(ctx, d->tile_csr_Ptr + t0 * 16, nt * 16, &t.tile_csr_Ptr);
+    if (!rc) rc = down(ctx, d->mask + t0 * 16, nt * 16, &t.mask);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(TSG_ERR_HIP, "%s", hipGetErrorString(e));
+    }
+    if (rc) {
+        tsg_tile_free_host(&t);
+        return rc;
+    }
+    for (int32_t i = 0; i <= t.tilem; i++) t.tile_ptr[i] -= t0;  // rebased to the range
+    for (size_t i = 0; i <= nt; i++) t.tile_nnz[i] -= v0;
+    if (t.tile_rowidx)
+        for (size_t i = 0; i < nt; i++) t.tile_rowidx[i] -= tr_lo;
     *h = t;
     return TSG_OK;
 }

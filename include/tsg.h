@@ -227,6 +227,14 @@ int tsg_csr_free(tsg_context *ctx, tsg_csr *dev);
  * (src/spgemm_raw_v2.cu:25-48).  Download allocates host arrays with malloc. */
 int tsg_tile_upload(tsg_context *ctx, const tsg_tile_matrix *host, tsg_tile_matrix *dev);
 int tsg_tile_download(tsg_context *ctx, const tsg_tile_matrix *dev, tsg_tile_matrix *host);
+/* Tile-rows [tr_lo, tr_hi) of a row-tiled matrix (A or C) to the host, as a
+ * matrix of tr_hi - tr_lo tile-rows: tile_ptr, tile_nnz and tile_rowidx
+ * rebased to the range, everything else as in tsg_tile_download (malloc'ed,
+ * free with tsg_tile_free_host).  No reference counterpart: the reference
+ * copies all of C back to check it (src/spgemm_raw_v2.cu:253-266), which at
+ * R-MAT scale 20 (162 GB of C) does not fit; checks then run on row blocks. */
+int tsg_tile_download_rows(tsg_context *ctx, const tsg_tile_matrix *dev, int32_t tr_lo, int32_t tr_hi,
+                           tsg_tile_matrix *host);
 /* Frees device arrays (free_dev, src/spgemm_raw_v2.cu:50-59). */
 int tsg_tile_free(tsg_context *ctx, tsg_tile_matrix *dev);
 /* Frees host arrays (free_tile_matrix, src/tile_format.h:647-659). */

@@ -636,6 +636,112 @@ int64_t ora_cpu_baseline(int32_t m, int32_t n, const int64_t *rpA, const int32_t
     return prod;
 }
 
+/* ---- whole-product counts: nnz(C) of the Gustavson product with its
+ * structural union (spgemm_raw_v2.cu:297-308), and numtileC of Step 1, the
+ * boolean product of the tile patterns (spgemm_raw_kernels.cuh:23-57: for
+ * tile-row I the distinct tile columns of B's tile-rows k reached by A's
+ * tiles (I, k)), both straight from the CSRs, over row blocks in threads. */
+typedef struct {
+    int32_t n, k, tlo, thi, m;
+    const int64_t *rpA, *rpB, *btp;   /* btp/btc: B's tile-row patterns */
+    const int32_t *colA, *colB, *btc;
+    int64_t nnz, tiles;
+} cnt_job;
+
+static void *cnt_worker(void *p) {
+    cnt_job *J = p;
+    const int32_t tn = (J->n + 15) / 16, tk = (J->k + 15) / 16;
+    char *used = calloc((size_t)J->n + 1, 1);
+    int32_t *cols = malloc(sizeof(int32_t) * ((size_t)J->n + 1));
+    char *tused = calloc((size_t)tn + 1, 1), *kused = calloc((size_t)tk + 1, 1);
+    int32_t *tc = malloc(sizeof(int32_t) * ((size_t)tn + 1)), *kc = malloc(sizeof(int32_t) * ((size_t)tk + 1));
+    int64_t nnz = 0, tiles = 0;
+    for (int32_t ti = J->tlo; ti < J->thi; ti++) {
+        int32_t nk = 0, nt = 0;
+        const int32_t r1 = (ti + 1) * 16 < J->m ? (ti + 1) * 16 : J->m;
+        for (int32_t i = ti * 16; i < r1; i++) {
+            int32_t nc = 0;
+            for (int64_t ja = J->rpA[i]; ja < J->rpA[i + 1]; ja++) {
+                const int32_t k = J->colA[ja];
+                if (!kused[k >> 4]) { kused[k >> 4] = 1; kc[nk++] = k >> 4; }
+                for (int64_t jb = J->rpB[k]; jb < J->rpB[k + 1]; jb++) {
+                    const int32_t c = J->colB[jb];
+                    if (!used[c]) { used[c] = 1; cols[nc++] = c; }
+                }
+            }
+            nnz += nc;
+            for (int32_t q = 0; q < nc; q++) used[cols[q]] = 0;
+        }
+        for (int32_t q = 0; q < nk; q++) {      /* union of the reached B tile-rows' patterns */
+            const int32_t kt = kc[q];
+            kused[kt] = 0;
+            for (int64_t e = J->btp[kt]; e < J->btp[kt + 1]; e++) {
+                const int32_t c = J->btc[e];
+                if (!tused[c]) { tused[c] = 1; tc[nt++] = c; }
+            }
+        }
+        tiles += nt;
+        for (int32_t q = 0; q < nt; q++) tused[tc[q]] = 0;
+    }
+    J->nnz = nnz;
+    J->tiles = tiles;
+    free(used); free(cols); free(tused); free(kused); free(tc); free(kc);
+    return NULL;
+}
+
+int ora_count_product(int32_t m, int32_t k, int32_t n, const int64_t *rpA, const int32_t *colA,
+                      const int64_t *rpB, const int32_t *colB, int32_t nthreads, int64_t *nnz_out,
+                      int64_t *tiles_out) {
+    const int32_t tmA = (m + 15) / 16, tmB = (k + 15) / 16, tn = (n + 15) / 16;
+    /* B's tile pattern per tile-row: distinct tile columns, any order */
+    int64_t *btp = calloc((size_t)tmB + 1, sizeof(int64_t));
+    char *f = calloc((size_t)tn + 1, 1);
+    int32_t *tmp = malloc(sizeof(int32_t) * ((size_t)tn + 1));
+    int32_t *btc = NULL;
+    for (int pass = 0; pass < 2; pass++) {
+        for (int32_t t = 0; t < tmB; t++) {
+            int32_t c = 0;
+            const int32_t r1 = (t + 1) * 16 < k ? (t + 1) * 16 : k;
+            for (int32_t i = t * 16; i < r1; i++)
+                for (int64_t j = rpB[i]; j < rpB[i + 1]; j++)
+                    if (!f[colB[j] >> 4]) { f[colB[j] >> 4] = 1; tmp[c++] = colB[j] >> 4; }
+            if (pass == 0) btp[t + 1] = c;
+            else memcpy(btc + btp[t], tmp, sizeof(int32_t) * (size_t)c);
+            for (int32_t q = 0; q < c; q++) f[tmp[q]] = 0;
+        }
+        if (pass == 0) {
+            for (int32_t t = 0; t < tmB; t++) btp[t + 1] += btp[t];
+            btc = malloc(sizeof(int32_t) * ((size_t)btp[tmB] + 1));
+        }
+    }
+    free(f); free(tmp);
+    if (nthreads < 1) nthreads = 1;
+    cnt_job *jobs = calloc((size_t)nthreads, sizeof(cnt_job));
+    pthread_t *th = calloc((size_t)nthreads, sizeof(pthread_t));
+    /* interleaved blocks of 64 tile-rows would balance better; contiguous
+     * ranges split by A's nonzeros are enough here */
+    int32_t t = 0;
+    for (int q = 0; q < nthreads; q++) {
+        cnt_job *J = &jobs[q];
+        J->n = n; J->k = k; J->m = m; J->rpA = rpA; J->colA = colA; J->rpB = rpB; J->colB = colB; J->btp = btp; J->btc = btc;
+        J->tlo = t;
+        const int64_t target = rpA[m] * (q + 1) / nthreads;
+        while (t < tmA && (rpA[(t + 1) * 16 < m ? (t + 1) * 16 : m] <= target || q == nthreads - 1)) t++;
+        J->thi = t;
+    }
+    for (int q = 0; q < nthreads; q++) pthread_create(&th[q], NULL, cnt_worker, &jobs[q]);
+    int64_t nz = 0, tl = 0;
+    for (int q = 0; q < nthreads; q++) {
+        pthread_join(th[q], NULL);
+        nz += jobs[q].nnz;
+        tl += jobs[q].tiles;
+    }
+    *nnz_out = nz;
+    *tiles_out = tl;
+    free(jobs); free(th); free(btp); free(btc);
+    return 0;
+}
+
 /* tile_format.h:51-90 (load_mtx): banner flags from the first line, '%'
  * comments skipped, 1-based -> 0-based, symmetric off-diagonals mirrored,
  * pattern -> 1.0, complex -> real part, entries sorted by (row,col) and

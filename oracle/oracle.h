@@ -56,6 +56,12 @@ typedef struct ora_check {
 /* --- CSR helpers (values are always carried as double on input) --- */
 /* flops = sum_{(i,k) in A} nnz(B[k,:])           spgemm_raw_v2.cu:95-99 */
 uint64_t ora_flops(int32_t m, const int64_t *rpA, const int32_t *colA, const int64_t *rpB);
+/* nnz(C) of the Gustavson product (structural union) and numtileC of Step 1
+ * (tile-pattern product, empty tiles included) of A (m x k) * B (k x n), from
+ * the CSRs, over row blocks in nthreads threads. */
+int ora_count_product(int32_t m, int32_t k, int32_t n, const int64_t *rpA, const int32_t *colA,
+                      const int64_t *rpB, const int32_t *colB, int32_t nthreads, int64_t *nnz_out,
+                      int64_t *tiles_out);
 /* B = A^T by counting sort                        spgemm_raw_v2.cu:78-89 */
 void ora_transpose(int32_t m, int32_t n, const int64_t *rp, const int32_t *col, const double *val,
                    int64_t *rpT, int32_t *colT, double *valT);

@@ -175,6 +175,62 @@ def spgemm_tiled(A, B, f32=False, steps=3):
     return d
 
 
+class _Rows:
+    """Rows [r0, r1) of a CSR as a CSR of r1 - r0 rows (same columns)."""
+
+    def __init__(self, A, r0, r1):
+        a, b = int(A.rowptr[r0]), int(A.rowptr[r1])
+        self.m, self.n = r1 - r0, A.n
+        self.rowptr = np.asarray(A.rowptr[r0:r1 + 1], dtype=np.int64) - a
+        self.col = A.col[a:b]
+        self.val = A.val[a:b]
+
+
+def row_block(A, r0, r1):
+    """Rows [r0, r1) of CSR A as a CSR with r1 - r0 rows."""
+    return _Rows(A, r0, r1)
+
+
+def spgemm_tiled_rows(A, B, blocks, f32=False):
+    """Steps 1-3 of the tiled method for A's tile-row ranges [lo, hi) in
+    `blocks` times all of B (B converted once): one C dict per block, a matrix
+    of hi - lo tile-rows (C's tile-rows lo..hi-1 of the whole product, tile_ptr
+    and tile_nnz from 0, tile_rowidx relative to lo)."""
+    keep = []
+    tB = csr2tile_raw(B, f32, True, keep)
+    out = []
+    try:
+        for lo, hi in blocks:
+            Ab = _Rows(A, lo * 16, min(hi * 16, A.m))
+            tA = csr2tile_raw(Ab, f32, False, keep)
+            tC = OraTile()
+            lib().ora_step1(C.byref(tA), C.byref(tB), C.byref(tC))
+            lib().ora_step2(C.byref(tA), C.byref(tB), C.byref(tC))
+            lib().ora_step3(C.byref(tA), C.byref(tB), C.byref(tC))
+            out.append(tile_to_dict(tC, lib().ora_tile_free))
+            lib().ora_tile_free(C.byref(tA))
+    finally:
+        lib().ora_tile_free(C.byref(tB))
+    return out
+
+
+def embed_rows(Cd, lo, tilem):
+    """A C dict of tile-rows [lo, lo + Cd['tilem']) (download_rows layout) as a
+    dict of `tilem` tile-rows whose other tile-rows are empty, for
+    gustavson_check over rows of that range."""
+    d = dict(Cd)
+    tp = np.zeros(tilem + 1, np.int64)
+    n = Cd["tilem"]
+    tp[lo + 1:lo + n + 1] = Cd["tile_ptr"][1:]
+    tp[lo + n + 1:] = Cd["tile_ptr"][-1]
+    d["tile_ptr"] = tp
+    d["tilem"] = tilem
+    d["m"] = tilem * 16
+    if Cd.get("tile_rowidx") is not None:
+        d["tile_rowidx"] = Cd["tile_rowidx"] + lo
+    return d
+
+
 def csr2tile_raw(csr, f32, as_b, keep):
     rp, col, val = _csr_args(csr)
     keep += [rp, col, val]
@@ -217,6 +273,18 @@ def gustavson_csr(A, B):
                             _p(rb, C.c_int64), _p(cb, C.c_int32), _p(vb, C.c_double),
                             _p(rpc, C.c_int64), _p(colc, C.c_int32), _p(valc, C.c_double))
     return rpc, colc[:nnz], valc[:nnz]
+
+
+def count_product(A, B, nthreads=8):
+    """(nnz(C) with the structural union, numtileC of Step 1 with empty tiles)
+    of A*B straight from the CSRs, threaded: a whole-product check at sizes
+    where C cannot come back to the host."""
+    ra, ca, _ = _csr_args(A)
+    rb, cb, _ = _csr_args(B)
+    nz, nt = C.c_int64(), C.c_int64()
+    lib().ora_count_product(A.m, A.n, B.n, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(rb, C.c_int64),
+                            _p(cb, C.c_int32), C.c_int32(nthreads), C.byref(nz), C.byref(nt))
+    return nz.value, nt.value
 
 
 def flops(A, B):

@@ -411,3 +411,38 @@ def test_planted_mask_bit_is_caught():
     bad["mask"][np.nonzero(bad["mask"])[0][0]] ^= 1
     with pytest.raises(AssertionError):
         check_structure(bad, A, A)
+
+
+@pytest.mark.parametrize("case", ["er", "rect", "rmat", "dup", "empty_rows"])
+def test_count_product_matches_tiled_steps_and_gustavson(case):
+    """ora_count_product (threaded nnz(C) and numtileC straight from the CSRs,
+    used where C cannot come back to the host) against Step 1's tile set and
+    Step 2's nnz of the tiled restatement and the Gustavson structural union,
+    on square and rectangular products (a swapped dimension fails the latter)."""
+    import gpu_spgemm_amd as g
+    rng = np.random.default_rng(3)
+    if case == "er":
+        A = g.gen_er(3000, 7, seed=1); B = A
+    elif case == "rect":   # 200 x 530 times 530 x 77
+        A = sp.random(200, 530, density=0.02, random_state=4, format="csr")
+        B = sp.random(530, 77, density=0.05, random_state=5, format="csr")
+        A = g.HostCSR(200, 530, A.indptr, A.indices, A.data)
+        B = g.HostCSR(530, 77, B.indptr, B.indices, B.data)
+    elif case == "rmat":
+        A = g.gen_rmat(12, 16, seed=2, dtype=g.F64); B = A
+    elif case == "dup":
+        rp = np.array([0, 3, 5, 6, 8, 8, 8, 8, 9], dtype=np.int64)
+        col = np.array([1, 1, 7, 0, 3, 2, 2, 3, 6], dtype=np.int32)
+        A = g.HostCSR(8, 8, rp, col, np.arange(1.0, 10.0)); B = A
+    else:
+        rp = np.concatenate([[0], np.sort(rng.integers(0, 120, 99)), [120]]).astype(np.int64)
+        col = rng.integers(0, 100, 120).astype(np.int32)
+        for i in range(100):
+            col[rp[i]:rp[i + 1]] = np.sort(col[rp[i]:rp[i + 1]])
+        A = g.HostCSR(100, 100, rp, col, np.ones(120)); B = A
+    for nt in (1, 5):
+        nnz, tiles = O.count_product(A, B, nthreads=nt)
+        Cd = O.spgemm_tiled(A, B)
+        rp, _, _ = O.gustavson_csr(A, B)
+        assert tiles == Cd["numtile"]
+        assert nnz == int(rp[-1]) == Cd["nnz"]
