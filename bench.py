@@ -77,13 +77,13 @@ def numeric_bytes(A, B, C, s):
 def train_learned(g, ctx, log, rmat_scale=0):
     """Trains the learned scheduler on a family of smaller matrices (never the
     benchmarked product itself), outside any timed region; for an R-MAT
-    workload the family adds R-MATs 4-16x smaller with other seeds."""
+    workload the family adds R-MATs 2-8x smaller with other seeds."""
     fam = [g.gen_laplace2d(256, 256), g.gen_laplace2d(512, 384), g.gen_laplace2d(300, 700),
            g.gen_laplace3d(40, 40, 40), g.gen_stencil27(32, 32, 32), g.gen_er(20000, 6, seed=11),
            g.gen_er(20000, 12, seed=12), g.gen_er_var(20000, 0, 24, seed=14),
            g.gen_rmat(13, 8, seed=15, dtype=g.F64)]
-    if rmat_scale:
-        fam += [g.gen_rmat(sc, 16, seed=100 + sc, dtype=g.F32) for sc in range(rmat_scale - 4, rmat_scale - 1)]
+    if rmat_scale:   # R-MATs 2-8x smaller than the benchmarked one, other seeds
+        fam += [g.gen_rmat(sc, 16, seed=100 + sc, dtype=g.F32) for sc in range(rmat_scale - 3, rmat_scale)]
     ctx.learned_clear()
     t0 = time.time()
     for M in fam:
@@ -352,8 +352,8 @@ def main():
         train_learned(g, ctx, log, int(args.workload[4:]) if rmat else 0)
         C0, st = ctx.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
         C0.free()
-        log(f"learned: fallback {st['fallback_used']} slots {st['pred_numtileC']} (tiles {st['numtileC']}) "
-            f"values {st['pred_nnzC']} (nnz {st['nnzC']})")
+        log(f"learned: fallback {st['fallback_used']} predicted tiles {st['pred_numtileC']} (tiles {st['numtileC']}) "
+            f"value capacity {st['pred_nnzC']} (nnz {st['nnzC']})")
         learned_fallback = st["fallback_used"]
         runs["learned"] = measure("learned", scheduler=g.SCHED_LEARNED)
     best = min(runs, key=lambda k: runs[k][0])      # the faster exact-result scheduler is the headline

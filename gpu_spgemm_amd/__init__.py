@@ -319,6 +319,10 @@ class Context:
         _check(lib().tsg_memcpy(self._p, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), C.c_size_t(out.nbytes)))
         return out
 
+    def pool_trim(self):
+        """Returns the pool's cached (freed) blocks to the device."""
+        _check(lib().tsg_pool_trim(self._p))
+
     def pool_stats(self):
         a, b = C.c_int64(), C.c_int64()
         _check(lib().tsg_pool_stats(self._p, C.byref(a), C.byref(b)))

@@ -377,7 +377,8 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
               const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC,
               const uint16_t *__restrict__ maskC, const uint8_t *__restrict__ ptrRowC,
               const int64_t *__restrict__ tile_nnzC, V *__restrict__ valC, uint8_t *__restrict__ colC,
-              int32_t *__restrict__ overflow, int64_t val_cap, int64_t *__restrict__ gcur, int64_t gcap) {
+              int32_t *__restrict__ overflow, int64_t val_cap, int64_t val_done, int64_t *__restrict__ gcur,
+              int64_t gcap) {
     constexpr int64_t H = rgw_item_values<V>();
     // this workgroup's global cursor area: positions [0, gcap), row ends [gcap, 2 gcap)
     int64_t *const gb = gcur + (int64_t)blockIdx.x * gcap * 2;
@@ -416,8 +417,12 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
             s_rp[tid] = rpA[ri < m ? ri : (int64_t)m];
         }
         __syncthreads();
-        if (tile_nnzC[s_it.ce] > val_cap) {  // slot layout: C's values outgrew their capacity (uniform)
+        if (tile_nnzC[s_it.ce] > val_cap) {  // C's values outgrew their capacity (uniform)
             if (tid == 0) atomicOr(overflow, 1);
+            __syncthreads();
+            continue;
+        }
+        if (tile_nnzC[s_it.ce] <= val_done) {  // a re-run: this tile-row's values are already written
             __syncthreads();
             continue;
         }
@@ -687,13 +692,13 @@ int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_mat
         k_rgw_numeric<float><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
             rows, begin, count, woffn, ctr, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr,
             B->row_col, (const float *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
-            C->tile_csr_Ptr, C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap,
+            C->tile_csr_Ptr, C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done,
             gcur, gcap);
     else
         k_rgw_numeric<double><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
             rows, begin, count, woffn, ctr, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr,
             B->row_col, (const double *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
-            C->tile_csr_Ptr, C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap,
+            C->tile_csr_Ptr, C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done,
             gcur, gcap);
     TSG_HIP(hipGetLastError());
     return TSG_OK;

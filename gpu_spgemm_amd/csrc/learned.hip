@@ -1,28 +1,26 @@
 // learned.hip -- learned scheduler: a small MLP predicts, per C tile-row, its
-// tile count and nonzero count from cheap features, and the product runs
-// without the Step-1 count pass.
+// tile count and nonzero count, and the predicted nonzeros size C's value
+// arrays so the product needs no host read between its symbolic and numeric
+// steps.
 //
-// No reference counterpart: the reference sizes C with an exact count kernel
-// and host reads (src/spgemm_raw_v2.cu:119-141, :191-211).  Here:
-//   1. one thread per tile-row computes features from A's tile-row and the
-//      B tile-rows it reaches (tiles, payloads, column spans), evaluates the
-//      MLP (8 -> 32 -> 32 -> 2, ReLU, FP32, weights in LDS) and turns the
-//      predicted tile count into a slot capacity (prediction + margin, never
-//      above the B tiles reached, an exact upper bound) and a bin;
-//   2. C's structure is written into per-row slots (slot layout), each row's
-//      actual tile count kept in row_nc; a row that needs more slots than
-//      predicted, or values beyond the predicted total, raises an overflow
-//      flag instead of writing out of bounds;
-//   3. value offsets come from scanning the slot counts (unused slots count
-//      0), the exact tile_ptr from scanning row_nc, and a compaction kernel
-//      moves the structure to the compact layout;
-//   4. one host read at the end returns (overflow, numtileC, nnzC); on
-//      overflow the exact binned pipeline runs instead, so the result never
-//      depends on the model.
+// No reference counterpart: the reference sizes C with exact counts and two
+// blocking reads, numtileC after Step 1 and nnzC after Step 2
+// (src/spgemm_raw_v2.cu:119-141, :191-211).  Here:
+//   1. the exact Step-1 count (k_row_stats) also emits each tile-row's
+//      features; k_learned_cap evaluates the MLP (8 -> 32 -> 32 -> 2, ReLU,
+//      FP32, weights in LDS) per tile-row and sums the predicted nonzeros,
+//      each clamped to the row's exact bound (256 a C tile);
+//   2. one host read returns numtileC, the bins and the predicted sum; C's
+//      tile arrays are allocated exactly, its value arrays at the calibrated
+//      prediction;
+//   3. symbolic and numeric steps run back to back; the numeric kernels write
+//      no value past the capacity and raise the overflow flag instead;
+//   4. one read at the end returns nnz(C) and the flag; when the prediction
+//      was short, only the numeric step runs again, into exactly sized arrays,
+//      so the result never depends on the model.
 // The network is trained on the GPU by the kernels below (forward, backward
 // with per-block partial gradients reduced in a fixed order, Adam), on
-// (features -> exact tile count, exact nnz) pairs that the exact pipeline
-// emits.
+// (features -> exact tile count, exact nonzeros) pairs from the exact pipeline.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -38,24 +36,19 @@ constexpr int P_W1 = 0, P_B1 = P_W1 + NF * NH, P_W2 = P_B1 + NH, P_B2 = P_W2 + N
               P_B3 = P_W3 + NH * NO, NP = P_B3 + NO;
 // targets: log1p(C tiles of the tile-row), log1p(C nonzeros of the tile-row)
 constexpr int GRAD_BLOCK = 128;
+// capacity = calibrated predicted nonzeros + this many
+constexpr int64_t CAP_EXTRA = 1024;
 
 struct Learned {
     float *w = nullptr, *m = nullptr, *v = nullptr;
     int64_t step = 0;
     bool init = false;
-    // log-space offsets added to the predictions before sizing (calibrated on
-    // the training set after each training run)
-    float off_nc = 0.0487902f, off_nnz = 0.0487902f;  // log(1.05)
+    // log-space offsets applied to the summed predictions (tiles, nonzeros):
+    // calibrated after training so every training product's totals are covered
+    float off_nc = 0.0198026f, off_nnz = 0.0198026f;  // log(1.02)
     float *X = nullptr, *T = nullptr;  // training set (device)
     int64_t n = 0, cap = 0;
-    struct Cache {
-        const void *a = nullptr, *b = nullptr;
-        int64_t annz = -1, bnnz = -1, atiles = -1, btiles = -1, tilem = -1, tilen = -1;
-        int64_t slots = 0, valcap = 0;
-        int64_t bin_off[MAX_BINS + 1] = {};
-        int nbins = 0;
-        bool valid = false;
-    } cache;
+    std::vector<int64_t> mat_end;      // end row of each collected product in the training set
 };
 
 static Learned *state(Context *ctx) {
@@ -221,75 +214,76 @@ __global__ void __launch_bounds__(256) k_mlp_adam(float *__restrict__ w, float *
     w[p] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
 }
 
-// ---------------------------------------------------------------- features, plan, targets
-// One thread per tile-row: features; with weights, the MLP's prediction,
-// slot capacity, value capacity and bin.
+// ---------------------------------------------------------------- features and predictions
+// Features of tile-row bi from the Step-1 statistics: log1p of (A tiles, B
+// tiles reached, A nonzeros, B nonzeros reached, span of the reached B tile
+// columns, longest reached B tile-row, summed spans of the reached B
+// tile-rows, reached B tile-rows holding tiles).
+__device__ __forceinline__ void row_features(const int32_t *__restrict__ st, float *x) {
+    x[0] = log1pf((float)st[RS_NA]);
+    x[1] = log1pf((float)st[RS_NB]);
+    x[2] = log1pf((float)st[RS_PA]);
+    x[3] = log1pf((float)st[RS_PB]);
+    x[4] = log1pf((float)st[RS_SPAN]);
+    x[5] = log1pf((float)st[RS_MAXLEN]);
+    x[6] = log1pf((float)st[RS_SUMSPAN]);
+    x[7] = log1pf((float)st[RS_NONEMPTY]);
+}
+
+// One thread per tile-row: features (Xout), and with weights the network's
+// outputs (Yout: expm1, i.e. predicted tiles and nonzeros) and the sums of the
+// per-row predictions, each rounded up and clamped to the row's exact bounds
+// (tiles: B tiles reached and their column span; nonzeros: 256 per C tile).
 __global__ void __launch_bounds__(256)
-k_learned_plan(int32_t tilem, const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA,
-               const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, const float *__restrict__ wg,
-               BinTable bt, float off_nc, float off_nnz, float scale, int extra, float *__restrict__ Xout,
-               float *__restrict__ Yout,
-               int64_t *__restrict__ slots, int64_t *__restrict__ valcap, uint8_t *__restrict__ bin) {
+k_learned_cap(int32_t tilem, const int32_t *__restrict__ stats, const float *__restrict__ wg,
+              float *__restrict__ Xout, float *__restrict__ Yout, unsigned long long *__restrict__ sum_tiles,
+              unsigned long long *__restrict__ sum_nnz) {
     __shared__ float w[NP];
+    __shared__ unsigned long long s_t[256 / WAVE], s_n[256 / WAVE];
     if (wg) {
         for (int i = threadIdx.x; i < NP; i += 256) w[i] = wg[i];
         __syncthreads();
     }
     const int32_t bi = blockIdx.x * 256 + threadIdx.x;
-    if (bi >= tilem) return;
-    const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
-    const int64_t na = ae - as, pa = tile_nnzA[ae] - tile_nnzA[as];
-    int64_t nb = 0, pb = 0, sumspan = 0, nonempty = 0, maxlen = 0;
-    int32_t cmin = INT_MAX, cmax = -1;
-    for (int64_t t = as; t < ae; t++) {
-        const int4 l = link[t];
-        nb += l.y;
-        pb += l.z;
-        if (l.y > 0) {
-            const int32_t f = colidxB[l.x], e = colidxB[l.x + l.y - 1];
-            cmin = min(cmin, f);
-            cmax = max(cmax, e);
-            sumspan += e - f + 1;
-            nonempty++;
-            maxlen = max<int64_t>(maxlen, l.y);
+    unsigned long long pt = 0, pn = 0;
+    if (bi < tilem) {
+        const int32_t *st = stats + (int64_t)bi * RS_NSTAT;
+        float x[NF];
+        row_features(st, x);
+        if (Xout)
+#pragma unroll
+            for (int i = 0; i < NF; i++) Xout[(int64_t)bi * NF + i] = x[i];
+        if (wg) {
+            float h1[NH], h2[NH], y[NO];
+            mlp_eval(w, x, h1, h2, y);
+            const float et = expm1f(fminf(y[0], 40.0f)), en = expm1f(fminf(y[1], 40.0f));
+            if (Yout) {
+                Yout[(int64_t)bi * NO + 0] = et;
+                Yout[(int64_t)bi * NO + 1] = en;
+            }
+            const double tb = (double)min(st[RS_NB], st[RS_SPAN]);
+            const double nbnd = 256.0 * (double)st[RS_COUNT];
+            pt = (unsigned long long)fmin(ceil(fmax((double)et, 0.0)), tb);
+            pn = (unsigned long long)fmin(ceil(fmax((double)en, 0.0)), nbnd);
         }
     }
-    const int64_t span = cmax >= cmin ? (int64_t)cmax - cmin + 1 : 0;
-    float x[NF] = {log1pf((float)na), log1pf((float)nb), log1pf((float)pa), log1pf((float)pb),
-                   log1pf((float)span), log1pf((float)maxlen), log1pf((float)sumspan), log1pf((float)nonempty)};
-    if (Xout)
-#pragma unroll
-        for (int i = 0; i < NF; i++) Xout[(int64_t)bi * NF + i] = x[i];
     if (!wg) return;
-    float h1[NH], h2[NH], y[NO];
-    mlp_eval(w, x, h1, h2, y);
-    if (Yout) {
-        Yout[(int64_t)bi * NO + 0] = expm1f(y[0]);
-        Yout[(int64_t)bi * NO + 1] = expm1f(y[1]);
+    pt = wave_sum(pt);
+    pn = wave_sum(pn);
+    if (lane_id() == 0) {
+        s_t[threadIdx.x / WAVE] = pt;
+        s_n[threadIdx.x / WAVE] = pn;
     }
-    // capacities: expm1(prediction + calibrated offset) * scale (+ a small
-    // constant), never above the exact upper bounds (distinct B tile columns
-    // reached; 256 values a tile)
-    const float pnc = fmaxf(expm1f(fminf(y[0] + off_nc, 40.0f)), 0.0f) * scale;
-    const float pnz = fmaxf(expm1f(fminf(y[1] + off_nnz, 40.0f)), 0.0f) * scale;
-    const int64_t bound = min<int64_t>(nb, span);
-    int64_t cap = pnc >= (float)bound ? bound : (int64_t)ceilf(pnc) + extra;
-    cap = min<int64_t>(cap, bound);
-    int64_t vcap = pnz >= (float)(cap * 256) ? cap * 256 : (int64_t)ceilf(pnz) + 8 * extra;
-    vcap = min<int64_t>(vcap, cap * 256);
-    slots[bi + 1] = cap;
-    valcap[bi + 1] = vcap;
-    if (bi == 0) {
-        slots[0] = 0;
-        valcap[0] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, b = 0;
+        for (int q = 0; q < 256 / WAVE; q++) {
+            a += s_t[q];
+            b += s_n[q];
+        }
+        atomicAdd(sum_tiles, a);
+        atomicAdd(sum_nnz, b);
     }
-    int b = bt.nvar;
-    for (int v = 0; v < bt.nvar; v++) {
-        const StageCaps c = bt.v[v];
-        if (na <= c.na && cap <= c.nc && nb <= c.nb && pa <= c.pa && pb <= c.pb) { b = v; break; }
-    }
-    if (b == bt.nvar && bt.rg && pa <= (int64_t)bt.rg_density * na) b = bt.nvar + 1;
-    bin[bi] = (uint8_t)b;
 }
 
 // Training targets from an exact C: tiles and nonzeros of every tile-row.
@@ -302,292 +296,139 @@ __global__ void __launch_bounds__(256) k_targets(int32_t tilem, const int64_t *_
     T[(int64_t)bi * NO + 1] = log1pf((float)(tile_nnzC[e] - tile_nnzC[s]));
 }
 
-// ---------------------------------------------------------------- slot layout -> compact layout
-__global__ void __launch_bounds__(256) k_rownc_ptr(int32_t tilem, const int32_t *__restrict__ row_nc,
-                                                   int64_t *__restrict__ tile_ptr) {
-    const int32_t bi = blockIdx.x * 256 + threadIdx.x;
-    if (bi >= tilem) return;
-    tile_ptr[bi + 1] = row_nc[bi];
-    if (bi == 0) tile_ptr[0] = 0;
-}
-
-// One wave per tile-row copies its tiles' headers from the slots to the
-// compact arrays; the value offsets are already exact.
-__global__ void __launch_bounds__(256)
-k_compact(int32_t tilem, const int64_t *__restrict__ slot_ptr, const int32_t *__restrict__ row_nc,
-          const int64_t *__restrict__ tile_ptr, const int32_t *__restrict__ pcol, const int64_t *__restrict__ pnnz,
-          const uint16_t *__restrict__ pmask, const uint8_t *__restrict__ pptr, int32_t *__restrict__ rowidx,
-          int32_t *__restrict__ colidx, int64_t *__restrict__ tile_nnz, uint16_t *__restrict__ mask,
-          uint8_t *__restrict__ ptr) {
-    const int32_t bi = (int32_t)(((int64_t)blockIdx.x * 256 + threadIdx.x) / WAVE);
-    const int lane = lane_id();
-    if (bi >= tilem) return;
-    const int64_t s0 = slot_ptr[bi], d0 = tile_ptr[bi];
-    const int n = row_nc[bi];
-    for (int t = lane; t < n; t += WAVE) {
-        const int64_t src = s0 + t, dst = d0 + t;
-        rowidx[dst] = bi;
-        colidx[dst] = pcol[src];
-        tile_nnz[dst] = pnnz[src];
-        reinterpret_cast<uint4 *>(ptr)[dst] = reinterpret_cast<const uint4 *>(pptr)[src];
-        reinterpret_cast<uint4 *>(mask)[dst * 2] = reinterpret_cast<const uint4 *>(pmask)[src * 2];
-        reinterpret_cast<uint4 *>(mask)[dst * 2 + 1] = reinterpret_cast<const uint4 *>(pmask)[src * 2 + 1];
-    }
-    if (bi == tilem - 1 && lane == 0) tile_nnz[tile_ptr[tilem]] = pnnz[slot_ptr[tilem]];
-}
-
-// Launch sizes taken from the capacity cache are checked on the device before
-// any kernel uses them: when this call's plan (total slots, value capacity,
-// bin offsets) differs from the cached one, every slot capacity becomes 0, so
-// no kernel writes C (each row overflows or is empty), and the overflow flag
-// sends the call to the exact pipeline.
-__global__ void __launch_bounds__(256) k_plan_guard(const int64_t *__restrict__ plan_hdr,
-                                                    const int64_t *__restrict__ expect, int nexp,
-                                                    int64_t *__restrict__ slotp, int32_t tilem,
-                                                    int32_t *__restrict__ overflow) {
-    __shared__ int bad;
-    if (threadIdx.x == 0) {
-        int m = 0;
-        for (int i = 0; i < nexp; i++) m |= plan_hdr[i] != expect[i];
-        bad = m;
-        if (m) *overflow = 1;
-    }
-    __syncthreads();
-    if (!bad) return;
-    for (int64_t i = threadIdx.x; i <= tilem; i += 256) slotp[i] = 0;
-}
-
-// hdr[0] = overflow, hdr[1] = numtileC, hdr[2] = nnzC
-__global__ void k_learned_result(const int32_t *__restrict__ overflow, const int64_t *__restrict__ tile_ptr,
-                                 int32_t tilem, const int64_t *__restrict__ pnnz, const int64_t *__restrict__ slotp,
-                                 int64_t *__restrict__ hdr) {
-    if (threadIdx.x != 0) return;
-    hdr[0] = *overflow;
-    hdr[1] = tile_ptr[tilem];
-    hdr[2] = pnnz[slotp[tilem]];
-}
-
 static unsigned nblk(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
 // ---------------------------------------------------------------- the learned product
+// Step-1 count with the per-row statistics, then the network's predicted
+// nonzeros summed on the device (into p.flags).
+static int learned_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                         const tsg_options *opts, Plan &p, int32_t *stats, const float *w, float *X, float *Y) {
+    TSG_CHECK(plan_count(ctx, A, B, C, opts, p, stats));
+    const int32_t tilem = A->tilem;
+    if (tilem > 0) {
+        k_learned_cap<<<nblk(tilem, 256), 256, 0, ctx->stream>>>(tilem, stats, w, X, Y, &p.flags->pred_tiles,
+                                                                  &p.flags->pred_nnz);
+        TSG_HIP(hipGetLastError());
+    }
+    return TSG_OK;
+}
+
 int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                    const tsg_options *opts, tsg_stats *st) {
     Learned *L = state(ctx);
     if (!L->init) return fail(TSG_ERR_ARG, "learned scheduler has no model: train it or set its weights first");
     const bool timing = opts && opts->timing;
+    // pred_scale < 1 shrinks the capacity (without the constant): a test hook
+    // that forces the numeric re-run
     const double pscale = (opts && opts->pred_scale > 0) ? opts->pred_scale : 1.0;
-    // capacity = calibrated prediction + 1 slot (+ 8 values); pred_scale < 1
-    // shrinks it without the constant (a test hook that forces the fallback)
-    const float cscale = (float)pscale;
-    const int cextra = pscale >= 1.0 ? 1 : 0;
     hipStream_t s = ctx->stream;
-    hipEvent_t ev[4] = {};
-    if (timing)
-        for (auto &e : ev) TSG_HIP(hipEventCreate(&e));
-    if (timing) (void)hipEventRecord(ev[0], s);
     memset(C, 0, sizeof *C);
     C->m = A->m; C->n = B->n; C->tilem = A->tilem; C->tilen = B->tilen;
     C->dtype = A->dtype; C->location = TSG_DEVICE;
     const int32_t tilem = A->tilem;
-    const BinTable bt = bin_table_for(ctx, A, B, opts);
+    hipEvent_t ev[4] = {};
+    int rc = TSG_OK;
+    if (timing)
+        for (auto &e : ev)
+            if (!rc && hipEventCreate(&e) != hipSuccess) rc = fail(TSG_ERR_HIP, "hipEventCreate failed");
     Plan p;
-    int64_t *slotp = nullptr, *valc = nullptr, *expect = nullptr, *res = nullptr;
-    int32_t *row_nc = nullptr, *pcol = nullptr, *prow = nullptr;
-    int64_t *pnnz = nullptr;
-    uint16_t *pmask = nullptr;
-    uint8_t *pptr = nullptr;
-    int syncs = 0;
-    int64_t slots = 0, vcap = 0;
-    auto cleanup = [&]() {
-        void *ps[] = {slotp, valc, expect, res, row_nc, pcol, prow, pnnz, pmask, pptr};
-        for (void *q : ps) pool_free(ctx, q);
-        free_plan(ctx, p);
-    };
-    int rc = plan_setup(ctx, A, B, bt, p);
-    if (!rc) rc = palloc(ctx, &slotp, (size_t)tilem + 1);
-    if (!rc) rc = palloc(ctx, &valc, (size_t)tilem + 1);
-    if (!rc) rc = palloc(ctx, &expect, MAX_BINS + 3);
-    if (!rc) rc = palloc(ctx, &res, 4);
-    if (!rc && tilem > 0) {
-        k_learned_plan<<<nblk(tilem, 256), 256, 0, s>>>(tilem, A->tile_ptr, A->tile_nnz, p.link, B->tile_columnidx,
-                                                         L->w, bt, L->off_nc, L->off_nnz, cscale, cextra, nullptr,
-                                                         nullptr, slotp, valc, p.bin);
-        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_learned_plan launch");
-        if (!rc) rc = scan_inclusive_i64(ctx, slotp, (int64_t)tilem + 1);
-        if (!rc) rc = scan_inclusive_i64(ctx, valc, (int64_t)tilem + 1);
-    } else if (!rc) {
-        if (hipMemsetAsync(slotp, 0, 8, s) != hipSuccess || hipMemsetAsync(valc, 0, 8, s) != hipSuccess)
-            rc = fail(TSG_ERR_HIP, "memset");
-    }
-    if (!rc) rc = plan_bin(ctx, p, tilem, slotp + tilem, valc + tilem);
-    // sizes: from the cache when this product was planned before (the final
-    // check compares the cached plan with this one), else one host read
-    Learned::Cache &cc = L->cache;
-    const bool hit = cc.valid && cc.a == A->tile_ptr && cc.b == B->tile_ptr && cc.annz == A->nnz &&
-                     cc.tilem == A->tilem && cc.tilen == B->tilen &&
-                     cc.bnnz == B->nnz && cc.atiles == A->numtile && cc.btiles == B->numtile && cc.nbins == p.nbins &&
-                     pscale == 1.0;
-    if (!rc) {
-        if (hit) {
-            slots = cc.slots;
-            vcap = cc.valcap;
-            for (int b = 0; b <= p.nbins; b++) p.bin_off[b] = cc.bin_off[b];
-        } else {
-            rc = plan_read(ctx, p, &slots, &vcap);
-            syncs++;
-            if (!rc && pscale == 1.0) {
-                cc.a = A->tile_ptr; cc.b = B->tile_ptr; cc.annz = A->nnz; cc.bnnz = B->nnz;
-                cc.tilem = A->tilem; cc.tilen = B->tilen;
-                cc.atiles = A->numtile; cc.btiles = B->numtile; cc.slots = slots; cc.valcap = vcap;
-                cc.nbins = p.nbins;
-                for (int b = 0; b <= p.nbins; b++) cc.bin_off[b] = p.bin_off[b];
-                cc.valid = true;
-            }
-        }
-    }
-    if (!rc && hit) {  // check the cached sizes against this call's plan before any launch uses them
-        int64_t h[MAX_BINS + 3];
-        h[0] = slots;
-        h[1] = vcap;
-        for (int b = 0; b <= p.nbins; b++) h[2 + b] = p.bin_off[b];
-        memcpy(ctx->pinned + 8, h, sizeof(int64_t) * (p.nbins + 3));
-        if (hipMemcpyAsync(expect, ctx->pinned + 8, sizeof(int64_t) * (p.nbins + 3), hipMemcpyHostToDevice, s) !=
-            hipSuccess)
-            rc = fail(TSG_ERR_HIP, "upload of the plan sizes failed");
-        if (!rc) {
-            k_plan_guard<<<1, 256, 0, s>>>(p.hdr, expect, p.nbins + 3, slotp, tilem, p.overflow());
-            if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_plan_guard launch");
-        }
-    }
-    // The slot layout and the compact C both hold `slots` tiles (120 bytes a
-    // tile) next to the values: when that exceeds int32 tile ids or the free
-    // device memory (with the pool's cached blocks), the exact pipeline runs
-    // instead (same result).
-    if (!rc) {
-        size_t freeb = 0, totalb = 0;
-        const double need = (double)slots * 120.0 + (double)vcap * (double)(vsize(A->dtype) + 1);
-        if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) rc = fail(TSG_ERR_HIP, "hipMemGetInfo failed");
-        if (!rc && (slots > INT32_MAX || need > 0.9 * ((double)freeb + (double)ctx->pool.cached))) {
-            cleanup();
-            tsg_options o = opts ? *opts : tsg_options{};
-            o.scheduler = TSG_SCHED_PLAIN;
-            TSG_CHECK(spgemm_binned(ctx, A, B, C, &o, st));
-            if (st) {
-                st->fallback_used = 2;
-                st->host_syncs += syncs;
-                st->pred_numtileC = slots;
-                st->pred_nnzC = vcap;
-            }
-            if (timing)
-                for (auto &e : ev) (void)hipEventDestroy(e);
-            return TSG_OK;
-        }
-    }
-    // slot-layout structure and the compact C
-    if (!rc) rc = palloc(ctx, &row_nc, (size_t)tilem);
-    if (!rc) rc = palloc(ctx, &prow, (size_t)slots);
-    if (!rc) rc = palloc(ctx, &pcol, (size_t)slots);
-    if (!rc) rc = palloc(ctx, &pnnz, (size_t)slots + 1);
-    if (!rc) rc = palloc(ctx, &pmask, (size_t)slots * 16);
-    if (!rc) rc = palloc(ctx, &pptr, (size_t)slots * 16);
+    int32_t *stats = nullptr;
+    int syncs = 0, fallback = 0;
+    int64_t cap = 0, pred_nnz = 0;
+    PlanFlags fl{};
+    if (timing && !rc) (void)hipEventRecord(ev[0], s);
     if (!rc) rc = palloc(ctx, &C->tile_ptr, (size_t)tilem + 1);
-    if (!rc) rc = palloc(ctx, &C->tile_rowidx, (size_t)slots);
-    if (!rc) rc = palloc(ctx, &C->tile_columnidx, (size_t)slots);
-    if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)slots + 1);
-    if (!rc) rc = palloc(ctx, &C->mask, (size_t)slots * 16);
-    if (!rc) rc = palloc(ctx, &C->tile_csr_Ptr, (size_t)slots * 16);
-    if (!rc) rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)vcap * vsize(C->dtype));
-    if (!rc) rc = palloc(ctx, &C->tile_csr_Col, (size_t)vcap);
-    if (!rc && hipMemsetAsync(pnnz, 0, sizeof(int64_t) * ((size_t)slots + 1), s) != hipSuccess)
-        rc = fail(TSG_ERR_HIP, "memset");
-    if (timing && !rc) (void)hipEventRecord(ev[1], s);
-    tsg_tile_matrix Cp = *C;  // slot-layout view: kernels index tiles by slot
-    Cp.tile_ptr = slotp;
-    Cp.tile_rowidx = prow;
-    Cp.tile_columnidx = pcol;
-    Cp.tile_nnz = pnnz;
-    Cp.mask = pmask;
-    Cp.tile_csr_Ptr = pptr;
-    Cp.numtile = slots;
-    SlotLayout sl;
-    sl.row_nc = row_nc;
-    sl.overflow = p.overflow();
-    sl.val_cap = vcap;
-    if (!rc) rc = run_symbolic(ctx, A, B, &Cp, p, sl);
-    if (!rc) rc = scan_inclusive_i64(ctx, pnnz, slots + 1);
-    if (!rc) rc = prepare_numeric(ctx, A, &Cp, p, sl);
-    if (!rc && tilem > 0) {
-        k_rownc_ptr<<<nblk(tilem, 256), 256, 0, s>>>(tilem, row_nc, C->tile_ptr);
-        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_rownc_ptr launch");
-    } else if (!rc && hipMemsetAsync(C->tile_ptr, 0, 8, s) != hipSuccess) {
-        rc = fail(TSG_ERR_HIP, "memset");
+    if (!rc) rc = palloc(ctx, &stats, (size_t)std::max(tilem, 1) * RS_NSTAT);
+    if (!rc) rc = learned_count(ctx, A, B, C, opts, p, stats, L->w, nullptr, nullptr);
+    if (!rc) rc = plan_bin(ctx, p, tilem, C->tile_ptr + tilem, reinterpret_cast<const int64_t *>(&p.flags->pred_nnz));
+    if (!rc) { rc = plan_read(ctx, p, &C->numtile, &pred_nnz); syncs++; }
+    if (!rc && C->numtile > INT32_MAX) rc = fail(TSG_ERR_OVERFLOW, "numtileC exceeds int32 tile ids");
+    if (!rc) rc = palloc(ctx, &C->tile_rowidx, (size_t)C->numtile);
+    if (!rc) rc = palloc(ctx, &C->tile_columnidx, (size_t)C->numtile);
+    if (!rc) rc = palloc(ctx, &C->mask, (size_t)C->numtile * 16);
+    if (!rc) rc = palloc(ctx, &C->tile_csr_Ptr, (size_t)C->numtile * 16);
+    if (!rc) rc = palloc(ctx, &C->tile_nnz, (size_t)C->numtile + 1);
+    if (!rc) {
+        // the calibrated prediction, never above the exact bound of 256 a tile
+        const double want = std::ceil((double)pred_nnz * std::exp((double)L->off_nnz) * pscale) +
+                            (pscale >= 1.0 ? (double)CAP_EXTRA : 0.0);
+        cap = (int64_t)std::min(want, 256.0 * (double)C->numtile);
+        if (pool_alloc(ctx, &C->tile_csr_Val, (size_t)cap * vsize(C->dtype)) != TSG_OK ||
+            palloc(ctx, &C->tile_csr_Col, (size_t)cap) != TSG_OK) {
+            // the prediction does not fit next to C's structure: values are
+            // allocated once nnz(C) is known (the exact pipeline's order)
+            pool_free(ctx, C->tile_csr_Val);
+            C->tile_csr_Val = nullptr;
+            cap = 0;
+            fallback = 2;
+        }
     }
-    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_ptr, (int64_t)tilem + 1);
+    if (timing && !rc) (void)hipEventRecord(ev[1], s);
+    SlotLayout sl;
+    sl.overflow = p.overflow();
+    sl.val_cap = cap;
+    if (!rc) rc = run_symbolic(ctx, A, B, C, p, sl);
+    free_heavy_bitmaps(ctx, p);  // the fill has used them
+    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1, true, &p.flags->nnz);
+    if (!rc) rc = prepare_numeric(ctx, A, C, p, sl);
     if (timing && !rc) (void)hipEventRecord(ev[2], s);
-    if (!rc) rc = run_numeric(ctx, A, B, &Cp, p, sl);
-    if (!rc && tilem > 0) {
-        k_compact<<<nblk((int64_t)tilem * WAVE, 256), 256, 0, s>>>(tilem, slotp, row_nc, C->tile_ptr, pcol, pnnz,
-                                                                     pmask, pptr, C->tile_rowidx, C->tile_columnidx,
-                                                                     C->tile_nnz, C->mask, C->tile_csr_Ptr);
-        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_compact launch");
-    } else if (!rc && hipMemsetAsync(C->tile_nnz, 0, 8, s) != hipSuccess) {
-        rc = fail(TSG_ERR_HIP, "memset");
+    if (!rc && cap > 0) rc = run_numeric(ctx, A, B, C, p, sl);
+    // nnz(C), the capacity flag and the predicted sums in one read
+    if (!rc) {
+        if (hipMemcpyAsync(ctx->pinned, p.flags, sizeof(PlanFlags), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = fail(TSG_ERR_HIP, "reading the learned product's result failed");
+        syncs++;
+        memcpy(&fl, ctx->pinned, sizeof fl);
+        C->nnz = fl.nnz;
+    }
+    if (!rc && fl.nnz <= cap && fl.cap) rc = fail(TSG_ERR_OVERFLOW, "a staged row exceeded its variant's capacity");
+    if (!rc && fl.nnz > cap) {
+        // the prediction was short: exact arrays, the values of every tile-row
+        // (tile) that fit below the capacity copied over, the rest computed
+        if (!fallback) fallback = 1;
+        void *old_val = C->tile_csr_Val, *old_col = C->tile_csr_Col;
+        C->tile_csr_Val = nullptr;
+        C->tile_csr_Col = nullptr;
+        rc = pool_alloc(ctx, &C->tile_csr_Val, (size_t)C->nnz * vsize(C->dtype));
+        if (!rc) rc = palloc(ctx, &C->tile_csr_Col, (size_t)C->nnz);
+        if (!rc && cap > 0 &&
+            (hipMemcpyAsync(C->tile_csr_Val, old_val, (size_t)cap * vsize(C->dtype), hipMemcpyDeviceToDevice, s) !=
+                 hipSuccess ||
+             hipMemcpyAsync(C->tile_csr_Col, old_col, (size_t)cap, hipMemcpyDeviceToDevice, s) != hipSuccess))
+            rc = fail(TSG_ERR_HIP, "copying the computed values failed");
+        pool_free(ctx, old_val);  // stream-ordered reuse: the copies are queued first
+        pool_free(ctx, old_col);
+        sl.val_cap = INT64_MAX;
+        sl.val_done = cap;
+        if (!rc && C->nnz > 0) rc = run_numeric(ctx, A, B, C, p, sl);
     }
     if (timing && !rc) (void)hipEventRecord(ev[3], s);
-    bool redo = false;
-    if (!rc) {
-        k_learned_result<<<1, 64, 0, s>>>(p.overflow(), C->tile_ptr, tilem, pnnz, slotp, res);
-        if (hipGetLastError() != hipSuccess ||
-            hipMemcpyAsync(ctx->pinned, res, 3 * sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            rc = fail(TSG_ERR_HIP, "reading the learned result failed");
-        syncs++;
-        if (!rc) {
-            redo = ctx->pinned[0] != 0;
-            if (redo) cc.valid = false;
-            C->numtile = ctx->pinned[1];
-            C->nnz = ctx->pinned[2];
-        }
+    if (st) {
+        st->numtileC = C->numtile;
+        st->nnzC = C->nnz;
+        st->host_syncs = syncs;
+        st->fallback_used = fallback;
+        st->pred_numtileC = (int64_t)std::ceil((double)fl.pred_tiles * std::exp((double)L->off_nc));
+        st->pred_nnzC = cap;
     }
     if (timing) {
-        if (!rc && !redo && st) {
+        if (!rc && st) {
             float a = 0, b = 0, c3 = 0;
+            (void)hipEventSynchronize(ev[3]);
             (void)hipEventElapsedTime(&a, ev[0], ev[1]);
             (void)hipEventElapsedTime(&b, ev[1], ev[2]);
             (void)hipEventElapsedTime(&c3, ev[2], ev[3]);
             st->ms_step1 = a; st->ms_step2 = b; st->ms_step3 = c3; st->ms_total = a + b + c3;
         }
-        for (auto &e : ev) (void)hipEventDestroy(e);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
     }
-    cleanup();
-    if (rc) {
-        free_c(ctx, C);
-        return rc;
-    }
-    if (redo) {  // the model under-predicted somewhere: exact pipeline, same result
-        free_c(ctx, C);
-        tsg_options o = opts ? *opts : tsg_options{};
-        o.scheduler = TSG_SCHED_PLAIN;
-        TSG_CHECK(spgemm_binned(ctx, A, B, C, &o, st));
-        if (st) {
-            st->fallback_used = 1;
-            st->host_syncs += syncs;
-            st->pred_numtileC = slots;
-            st->pred_nnzC = vcap;
-        }
-        return TSG_OK;
-    }
-    if (st) {
-        st->numtileC = C->numtile;
-        st->nnzC = C->nnz;
-        st->host_syncs = syncs;
-        st->fallback_used = 0;
-        st->pred_numtileC = slots;
-        st->pred_nnzC = vcap;
-    }
-    return TSG_OK;
+    pool_free(ctx, stats);
+    free_plan(ctx, p);
+    if (rc) free_c(ctx, C);
+    return rc;
 }
-
 // ---------------------------------------------------------------- training
 static int ensure_weights(Context *ctx, uint64_t seed) {
     Learned *L = state(ctx);
@@ -605,36 +446,58 @@ static int ensure_weights(Context *ctx, uint64_t seed) {
     init(P_W1, NF, NH);
     init(P_W2, NH, NH);
     init(P_W3, NH, NO);
-    TSG_HIP(hipMemcpy(L->w, h.data(), NP * sizeof(float), hipMemcpyHostToDevice));
-    TSG_HIP(hipMemset(L->m, 0, NP * sizeof(float)));
-    TSG_HIP(hipMemset(L->v, 0, NP * sizeof(float)));
+    TSG_HIP(hipMemcpyAsync(L->w, h.data(), NP * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    TSG_HIP(hipMemsetAsync(L->m, 0, NP * sizeof(float), ctx->stream));
+    TSG_HIP(hipMemsetAsync(L->v, 0, NP * sizeof(float), ctx->stream));
+    TSG_HIP(hipStreamSynchronize(ctx->stream));
     L->step = 0;
     return TSG_OK;
 }
 
-// Log-space offsets that make every training row's capacity cover its exact
-// size: off = max over rows of (target - prediction), clamped to
-// [log 1.05, log 16].
+// Log-space offsets that make the summed prediction of every collected
+// product cover that product's exact totals (C tiles, C nonzeros): off = max
+// over products of log(actual / predicted) + log 1.02, clamped to
+// [log 1.02, log 16].  Each row's prediction is rounded up and clamped to its
+// exact bounds as k_learned_cap does, so the offsets apply to the device sums.
 static int calibrate(Context *ctx) {
     Learned *L = state(ctx);
     const int64_t n = L->n;
     float *Y = nullptr;
     TSG_CHECK(palloc(ctx, &Y, (size_t)n * NO));
-    std::vector<float> hy((size_t)n * NO), ht((size_t)n * NO);
+    std::vector<float> hy((size_t)n * NO), ht((size_t)n * NO), hx((size_t)n * NF);
     k_mlp_forward<<<nblk(n, 256), 256, 0, ctx->stream>>>(L->w, L->X, n, Y);
     int rc = TSG_OK;
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess ||
-        hipMemcpy(hy.data(), Y, sizeof(float) * n * NO, hipMemcpyDeviceToHost) != hipSuccess ||
-        hipMemcpy(ht.data(), L->T, sizeof(float) * n * NO, hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(hy.data(), Y, sizeof(float) * n * NO, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(ht.data(), L->T, sizeof(float) * n * NO, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipMemcpyAsync(hx.data(), L->X, sizeof(float) * n * NF, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess)
         rc = fail(TSG_ERR_HIP, "calibration failed");
     pool_free(ctx, Y);
     if (rc) return rc;
-    float off[NO] = {0.0f, 0.0f};
-    for (int64_t i = 0; i < n; i++)
-        for (int o = 0; o < NO; o++) off[o] = std::max(off[o], ht[i * NO + o] - hy[i * NO + o]);
-    const float lo = std::log(1.05f), hi = std::log(16.0f);
-    L->off_nc = std::min(std::max(off[0], lo), hi);
-    L->off_nnz = std::min(std::max(off[1], lo), hi);
+    double off[NO] = {-1e30, -1e30};
+    int64_t r0 = 0;
+    for (const int64_t r1 : L->mat_end) {
+        double pred[NO] = {0, 0}, act[NO] = {0, 0};
+        for (int64_t i = r0; i < r1; i++) {
+            const double cnt = std::rint(std::expm1((double)ht[i * NO + 0]));
+            const double nnz = std::rint(std::expm1((double)ht[i * NO + 1]));
+            const double nb = std::rint(std::expm1((double)hx[i * NF + 1]));
+            const double span = std::rint(std::expm1((double)hx[i * NF + 4]));
+            const double et = std::expm1((double)std::min(hy[i * NO + 0], 40.0f));
+            const double en = std::expm1((double)std::min(hy[i * NO + 1], 40.0f));
+            pred[0] += std::min(std::ceil(std::max(et, 0.0)), std::min(nb, span));
+            pred[1] += std::min(std::ceil(std::max(en, 0.0)), 256.0 * cnt);
+            act[0] += cnt;
+            act[1] += nnz;
+        }
+        for (int o = 0; o < NO; o++)
+            if (act[o] > 0) off[o] = std::max(off[o], std::log(act[o] / std::max(pred[o], 1.0)));
+        r0 = r1;
+    }
+    const double margin = std::log(1.02), hi = std::log(16.0);
+    L->off_nc = (float)std::min(std::max(off[0] + margin, margin), hi);
+    L->off_nnz = (float)std::min(std::max(off[1] + margin, margin), hi);
     return TSG_OK;
 }
 
@@ -681,20 +544,15 @@ static int features_or_predict(tsg_context *c, const tsg_tile_matrix *A, const t
     if (pred && !L->init) return fail(TSG_ERR_ARG, "learned scheduler has no model");
     const int32_t tilem = A->tilem;
     Plan p;
+    tsg_tile_matrix Ct{};
     float *X = nullptr, *Y = nullptr;
-    int64_t *slotp = nullptr, *valc = nullptr;
-    const BinTable bt = bin_table_for(ctx, A, B, nullptr);
-    int rc = plan_setup(ctx, A, B, bt, p);
-    if (!rc) rc = palloc(ctx, &X, (size_t)std::max(tilem, 1) * NF);
-    if (!rc) rc = palloc(ctx, &Y, (size_t)std::max(tilem, 1) * NO);
-    if (!rc) rc = palloc(ctx, &slotp, (size_t)tilem + 1);
-    if (!rc) rc = palloc(ctx, &valc, (size_t)tilem + 1);
-    if (!rc && tilem > 0) {
-        k_learned_plan<<<nblk(tilem, 256), 256, 0, ctx->stream>>>(
-            tilem, A->tile_ptr, A->tile_nnz, p.link, B->tile_columnidx, pred ? L->w : nullptr,
-            bt, L->off_nc, L->off_nnz, 1.0f, 1, X, Y, slotp, valc, p.bin);
-        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_learned_plan launch");
-    }
+    int32_t *stats = nullptr;
+    const size_t rows = (size_t)std::max(tilem, 1);
+    int rc = palloc(ctx, &Ct.tile_ptr, (size_t)tilem + 1);
+    if (!rc) rc = palloc(ctx, &stats, rows * RS_NSTAT);
+    if (!rc) rc = palloc(ctx, &X, rows * NF);
+    if (!rc) rc = palloc(ctx, &Y, rows * NO);
+    if (!rc) rc = learned_count(ctx, A, B, &Ct, nullptr, p, stats, pred ? L->w : nullptr, X, pred ? Y : nullptr);
     if (!rc && feats && tilem > 0 &&
         hipMemcpyAsync(feats, X, sizeof(float) * tilem * NF, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
         rc = fail(TSG_ERR_HIP, "copy");
@@ -702,7 +560,7 @@ static int features_or_predict(tsg_context *c, const tsg_tile_matrix *A, const t
         hipMemcpyAsync(pred, Y, sizeof(float) * tilem * NO, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess)
         rc = fail(TSG_ERR_HIP, "copy");
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(TSG_ERR_HIP, "sync");
-    void *ps[] = {X, Y, slotp, valc};
+    void *ps[] = {Ct.tile_ptr, stats, X, Y};
     for (void *q : ps) pool_free(ctx, q);
     free_plan(ctx, p);
     return rc;
@@ -727,23 +585,23 @@ int tsg_learned_collect(tsg_context *c, const tsg_tile_matrix *A, const tsg_tile
     tsg_options o{};
     TSG_CHECK(spgemm_binned(ctx, A, B, &C, &o, nullptr));
     Plan p;
+    tsg_tile_matrix Ct{};
     float *X = nullptr, *T = nullptr;
-    int64_t *slotp = nullptr, *valc = nullptr;
-    int rc = plan_setup(ctx, A, B, bin_table_for(ctx, A, B, nullptr), p);
+    int32_t *stats = nullptr;
+    int rc = palloc(ctx, &Ct.tile_ptr, (size_t)tilem + 1);
+    if (!rc) rc = palloc(ctx, &stats, (size_t)tilem * RS_NSTAT);
     if (!rc) rc = palloc(ctx, &X, (size_t)tilem * NF);
     if (!rc) rc = palloc(ctx, &T, (size_t)tilem * NO);
-    if (!rc) rc = palloc(ctx, &slotp, (size_t)tilem + 1);
-    if (!rc) rc = palloc(ctx, &valc, (size_t)tilem + 1);
+    if (!rc) rc = learned_count(ctx, A, B, &Ct, &o, p, stats, nullptr, X, nullptr);
     if (!rc) {
-        k_learned_plan<<<nblk(tilem, 256), 256, 0, ctx->stream>>>(tilem, A->tile_ptr, A->tile_nnz, p.link,
-                                                                   B->tile_columnidx, nullptr, BinTable{}, 0.0f, 0.0f,
-                                                                   1.0f, 0, X, nullptr, slotp, valc, p.bin);
         k_targets<<<nblk(tilem, 256), 256, 0, ctx->stream>>>(tilem, C.tile_ptr, C.tile_nnz, T);
-        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "feature/target launch");
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_targets launch");
     }
+    Learned *L = state(ctx);
     if (!rc) rc = append_rows(ctx, X, T, tilem);
+    if (!rc) L->mat_end.push_back(L->n);
     if (!rc && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(TSG_ERR_HIP, "sync");
-    void *ps[] = {X, T, slotp, valc};
+    void *ps[] = {Ct.tile_ptr, stats, X, T};
     for (void *q : ps) pool_free(ctx, q);
     free_plan(ctx, p);
     free_c(ctx, &C);
@@ -755,7 +613,7 @@ int tsg_learned_clear(tsg_context *c) {
     if (!ctx) return fail(TSG_ERR_ARG, "null context");
     Learned *L = state(ctx);
     L->n = 0;
-    L->cache.valid = false;
+    L->mat_end.clear();
     return TSG_OK;
 }
 
@@ -781,7 +639,10 @@ int tsg_learned_train(tsg_context *c, int32_t epochs, int32_t batch, float lr, u
     for (int e = 0; e < epochs && !rc; e++) {
         for (int64_t i = 0; i < n; i++) order[i] = (int32_t)i;
         std::shuffle(order.begin(), order.end(), rng);
-        if (hipMemcpy(perm, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        // stream-ordered after the previous epoch's kernels; synchronous so
+        // `order` can be reshuffled next epoch
+        if (hipMemcpyAsync(perm, order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess) {
             rc = fail(TSG_ERR_HIP, "perm upload");
             break;
         }
@@ -795,7 +656,9 @@ int tsg_learned_train(tsg_context *c, int32_t epochs, int32_t batch, float lr, u
             k_mlp_adam<<<nblk(NP, 256), 256, 0, ctx->stream>>>(L->w, L->m, L->v, partial, nblocks, lr, bc1, bc2);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "training launch");
             if (!rc && e == epochs - 1) {
-                if (hipMemcpy(hl.data(), lossp, sizeof(float) * nblocks, hipMemcpyDeviceToHost) != hipSuccess) {
+                if (hipMemcpyAsync(hl.data(), lossp, sizeof(float) * nblocks, hipMemcpyDeviceToHost, ctx->stream) !=
+                        hipSuccess ||
+                    hipStreamSynchronize(ctx->stream) != hipSuccess) {
                     rc = fail(TSG_ERR_HIP, "loss read");
                     break;
                 }
@@ -810,7 +673,6 @@ int tsg_learned_train(tsg_context *c, int32_t epochs, int32_t batch, float lr, u
     if (rc) return rc;
     TSG_CHECK(calibrate(ctx));
     L->init = true;
-    L->cache.valid = false;
     if (final_loss) *final_loss = (float)(epoch_loss / (double)n);
     return TSG_OK;
 }
@@ -824,8 +686,7 @@ int tsg_learned_calibration(tsg_context *c, float *get, const float *set) {
             return fail(TSG_ERR_ARG, "calibration offsets must lie in [0, 20]");
         L->off_nc = set[0];
         L->off_nnz = set[1];
-        L->cache.valid = false;
-    }
+        }
     if (get) {
         get[0] = L->off_nc;
         get[1] = L->off_nnz;
@@ -839,8 +700,8 @@ int tsg_learned_get_weights(tsg_context *c, float *w, int32_t n) {
     Learned *L = state(ctx);
     if (!L->w) return fail(TSG_ERR_ARG, "no weights yet");
     (void)hipSetDevice(ctx->device);
+    TSG_HIP(hipMemcpyAsync(w, L->w, sizeof(float) * NP, hipMemcpyDeviceToHost, ctx->stream));
     TSG_HIP(hipStreamSynchronize(ctx->stream));
-    TSG_HIP(hipMemcpy(w, L->w, sizeof(float) * NP, hipMemcpyDeviceToHost));
     return TSG_OK;
 }
 
@@ -850,13 +711,12 @@ int tsg_learned_set_weights(tsg_context *c, const float *w, int32_t n) {
     (void)hipSetDevice(ctx->device);
     TSG_CHECK(ensure_weights(ctx, 0));
     Learned *L = state(ctx);
+    TSG_HIP(hipMemcpyAsync(L->w, w, sizeof(float) * NP, hipMemcpyHostToDevice, ctx->stream));
+    TSG_HIP(hipMemsetAsync(L->m, 0, NP * sizeof(float), ctx->stream));
+    TSG_HIP(hipMemsetAsync(L->v, 0, NP * sizeof(float), ctx->stream));
     TSG_HIP(hipStreamSynchronize(ctx->stream));
-    TSG_HIP(hipMemcpy(L->w, w, sizeof(float) * NP, hipMemcpyHostToDevice));
-    TSG_HIP(hipMemset(L->m, 0, NP * sizeof(float)));
-    TSG_HIP(hipMemset(L->v, 0, NP * sizeof(float)));
     L->step = 0;
     L->init = true;
-    L->cache.valid = false;
     return TSG_OK;
 }
 
@@ -873,11 +733,13 @@ int tsg_learned_forward(tsg_context *c, const float *X, int64_t n, float *Y) {
     TSG_CHECK(palloc(ctx, &dy, (size_t)std::max<int64_t>(n, 1) * NO));
     int rc = TSG_OK;
     if (n > 0) {
-        if (hipMemcpy(dx, X, sizeof(float) * n * NF, hipMemcpyHostToDevice) != hipSuccess) rc = fail(TSG_ERR_HIP, "copy");
+        if (hipMemcpyAsync(dx, X, sizeof(float) * n * NF, hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            rc = fail(TSG_ERR_HIP, "copy");
         if (!rc) {
             k_mlp_forward<<<nblk(n, 256), 256, 0, ctx->stream>>>(L->w, dx, n, dy);
-            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess ||
-                hipMemcpy(Y, dy, sizeof(float) * n * NO, hipMemcpyDeviceToHost) != hipSuccess)
+            if (hipGetLastError() != hipSuccess ||
+                hipMemcpyAsync(Y, dy, sizeof(float) * n * NO, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+                hipStreamSynchronize(ctx->stream) != hipSuccess)
                 rc = fail(TSG_ERR_HIP, "forward failed");
         }
     }
@@ -899,16 +761,18 @@ int tsg_learned_gradient(tsg_context *c, const float *X, const float *T, int64_t
     if (!rc) rc = palloc(ctx, &partial, (size_t)nblocks * NP);
     if (!rc) rc = palloc(ctx, &lossp, (size_t)nblocks);
     if (!rc) rc = palloc(ctx, &g, NP);
-    if (!rc && (hipMemcpy(dx, X, sizeof(float) * n * NF, hipMemcpyHostToDevice) != hipSuccess ||
-                hipMemcpy(dt, T, sizeof(float) * n * NO, hipMemcpyHostToDevice) != hipSuccess))
+    if (!rc && (hipMemcpyAsync(dx, X, sizeof(float) * n * NF, hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+                hipMemcpyAsync(dt, T, sizeof(float) * n * NO, hipMemcpyHostToDevice, ctx->stream) != hipSuccess))
         rc = fail(TSG_ERR_HIP, "copy");
     if (!rc) {
         k_mlp_grad<<<nblocks, GRAD_BLOCK, 0, ctx->stream>>>(L->w, dx, dt, nullptr, 0, n, partial, lossp);
         k_grad_reduce<<<nblk(NP, 256), 256, 0, ctx->stream>>>(partial, nblocks, g);
         std::vector<float> hl((size_t)nblocks);
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess ||
-            hipMemcpy(grad, g, sizeof(float) * NP, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(hl.data(), lossp, sizeof(float) * nblocks, hipMemcpyDeviceToHost) != hipSuccess)
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(grad, g, sizeof(float) * NP, hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+            hipMemcpyAsync(hl.data(), lossp, sizeof(float) * nblocks, hipMemcpyDeviceToHost, ctx->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(ctx->stream) != hipSuccess)
             rc = fail(TSG_ERR_HIP, "gradient failed");
         if (!rc && loss) {
             double l = 0;

@@ -43,7 +43,8 @@ struct NumArgs {
     const uint16_t *maskB; const uint8_t *colB; const V *valB;
     const int64_t *tile_ptrC; const int32_t *rowidxC; const int32_t *colidxC; const int64_t *tile_nnzC;
     const uint16_t *maskC; const uint8_t *ptrRowC; V *valC; uint8_t *colC;
-    const int32_t *row_nc; int32_t *overflow; int64_t val_cap;  // slot layout (learned scheduler)
+    const int32_t *row_nc; int32_t *overflow; int64_t val_cap;  // slot layout / value capacity (learned scheduler)
+    int64_t val_done;  // a numeric re-run: values below this are already written
 };
 
 template <typename V>
@@ -52,7 +53,7 @@ static NumArgs<V> num_args(const tsg_tile_matrix *A, const tsg_tile_matrix *B, c
                       (const V *)A->tile_csr_Val, B->csc_tile_ptr, B->csc_tile_rowidx, B->tile_nnz,
                       B->tile_csr_Ptr, B->mask, B->tile_csr_Col, (const V *)B->tile_csr_Val, C->tile_ptr, C->tile_rowidx,
                       C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr, (V *)C->tile_csr_Val,
-                      C->tile_csr_Col, nullptr, nullptr, INT64_MAX};
+                      C->tile_csr_Col, nullptr, nullptr, INT64_MAX, -1};
 }
 
 // Numeric step of one C tile by a 16-lane group (lane = tile row).  DUPS:
@@ -63,10 +64,11 @@ __device__ __forceinline__ void num_tile(int64_t ti, const NumArgs<V> &p) {
     const int l = lane_id() & 15;
     const int64_t c0 = p.tile_nnzC[ti];
     if (p.tile_nnzC[ti + 1] == c0) return;  // structurally empty tile
-    if (p.tile_nnzC[ti + 1] > p.val_cap) {   // slot layout: C's values outgrew their capacity
+    if (p.tile_nnzC[ti + 1] > p.val_cap) {   // C's values outgrew their capacity
         if ((lane_id() & 15) == 0) atomicOr(p.overflow, 1);
         return;
     }
+    if (p.tile_nnzC[ti + 1] <= p.val_done) return;  // a re-run: already written
     const int32_t bi = p.rowidxC[ti], bj = p.colidxC[ti];
     V acc[16];
 #pragma unroll
@@ -240,6 +242,7 @@ static int launch_list_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
     a.row_nc = sl.row_nc;
     a.overflow = sl.overflow;
     a.val_cap = sl.val_cap;
+    a.val_done = sl.val_done;
     for (int64_t c0 = 0; c0 < count; c0 += ROW_CHUNK) {  // work-items per launch stay below 2^32
         const unsigned n = (unsigned)(count - c0 < ROW_CHUNK ? count - c0 : ROW_CHUNK);
         if ((A->flags | B->flags) & 1)

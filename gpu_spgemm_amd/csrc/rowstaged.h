@@ -35,9 +35,12 @@ struct PlanFlags {
     int32_t heavy_rows;             // length of the heavy-row list built by k_row_stats
     int32_t pad1;
     unsigned long long heavy_work;  // work counter of the heavy rows' Step-1 count
+    unsigned long long pred_tiles;  // learned scheduler: summed per-row predictions (C tiles, C nonzeros)
+    unsigned long long pred_nnz;
 };
 static_assert(offsetof(PlanFlags, nnz) == 8 && offsetof(PlanFlags, heavy_rows) == 16 &&
-                  offsetof(PlanFlags, heavy_work) == 24 && sizeof(PlanFlags) == 32,
+                  offsetof(PlanFlags, heavy_work) == 24 && offsetof(PlanFlags, pred_nnz) == 40 &&
+                  sizeof(PlanFlags) == 48,
               "PlanFlags layout: the heavy-row kernels take heavy_rows and heavy_work as {int32, pad, uint64}");
 constexpr int PLAN_FLAG_WORDS = (int)(sizeof(PlanFlags) / sizeof(int32_t));
 
@@ -95,6 +98,10 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
 int plan_bin(Context *ctx, Plan &p, int32_t tilem, const int64_t *total0, const int64_t *total1);
 // Reads hdr to the host: totals and bin offsets (one blocking read).
 int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1);
+// A->B links, Step-1 count (with the per-row statistics when stats_out, RS_NSTAT
+// int32 a tile-row) and the scan of C->tile_ptr; clears p.flags.
+int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+               const tsg_options *opts, Plan &p, int32_t *stats_out);
 int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                  const Plan &p, const SlotLayout &sl);
 int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,

@@ -119,12 +119,20 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     const int64_t na = ae - as;
     int32_t cmin = INT_MAX, cmax = -1;
     int64_t nb = 0, pb = 0, count = 0;
+    int64_t f_maxlen = 0, f_sumspan = 0, f_nonempty = 0;  // learned-scheduler features (stats only)
     bool done = false;
     if (na <= WAVE) {
         const int4 l = lane < na ? link[as + lane] : make_int4(0, 0, 0, 0);
         const int incl = dpp_incl_sum(l.y);
         nb = last_lane(incl);
         pb = last_lane(dpp_incl_sum(l.z));
+        if (stats && nb <= WAVE) {
+            const bool ne = lane < na && l.y > 0;
+            const int64_t sp = ne ? (int64_t)colidxB[l.x + l.y - 1] - colidxB[l.x] + 1 : 0;
+            f_sumspan = wave_sum(sp);
+            f_maxlen = wave_max(l.y);
+            f_nonempty = __popcll(__ballot(ne));
+        }
         if (nb <= WAVE) {
             if (lane < na) {
                 s_pre[w][lane + 1] = incl;
@@ -164,8 +172,12 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
         for (int64_t i = as + lane; i < ae; i += WAVE) {
             const int4 l = link[i];
             if (l.y > 0) {
-                cmin = min(cmin, colidxB[l.x]);
-                cmax = max(cmax, colidxB[l.x + l.y - 1]);
+                const int32_t f = colidxB[l.x], e = colidxB[l.x + l.y - 1];
+                cmin = min(cmin, f);
+                cmax = max(cmax, e);
+                f_sumspan += e - f + 1;
+                f_maxlen = max<int64_t>(f_maxlen, l.y);
+                f_nonempty++;
             }
             nb += l.y;
             pb += l.z;
@@ -174,6 +186,11 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
         cmax = wave_max(cmax);
         nb = wave_sum(nb);
         pb = wave_sum(pb);
+        if (stats) {
+            f_sumspan = wave_sum(f_sumspan);
+            f_maxlen = wave_max(f_maxlen);
+            f_nonempty = wave_sum(f_nonempty);
+        }
         const bool heavy = hlist && nb > HEAVY_NB;  // counted by a workgroup afterwards
         if (heavy && lane == 0) hlist[atomicAdd(hcount, 1)] = bi;
         for (int64_t w0 = cmin; !heavy && w0 <= cmax; w0 += (int64_t)RS_WIN_WORDS * 32) {
@@ -223,14 +240,17 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
         }
         if (b == bt.nvar && bt.rg && pa <= (int64_t)bt.rg_density * na) b = bt.nvar + 1;
         bin[bi] = (uint8_t)b;
-        if (stats) {  // per-row features for the learned scheduler
-            int32_t *st = stats + (int64_t)bi * 8;
-            st[0] = (int32_t)min<int64_t>(na, INT_MAX);
-            st[1] = (int32_t)min<int64_t>(pa, INT_MAX);
-            st[2] = (int32_t)min<int64_t>(nb, INT_MAX);
-            st[3] = (int32_t)min<int64_t>(pb, INT_MAX);
-            st[4] = (int32_t)min<int64_t>(count, INT_MAX);
-            st[5] = cmax >= cmin ? cmax - cmin + 1 : 0;
+        if (stats) {  // per-row features for the learned scheduler (heavy rows' count: k_heavy_structure)
+            int32_t *st = stats + (int64_t)bi * RS_NSTAT;
+            st[RS_NA] = (int32_t)min<int64_t>(na, INT_MAX);
+            st[RS_PA] = (int32_t)min<int64_t>(pa, INT_MAX);
+            st[RS_NB] = (int32_t)min<int64_t>(nb, INT_MAX);
+            st[RS_PB] = (int32_t)min<int64_t>(pb, INT_MAX);
+            st[RS_COUNT] = (int32_t)min<int64_t>(count, INT_MAX);
+            st[RS_SPAN] = cmax >= cmin ? cmax - cmin + 1 : 0;
+            st[RS_MAXLEN] = (int32_t)min<int64_t>(f_maxlen, INT_MAX);
+            st[RS_SUMSPAN] = (int32_t)min<int64_t>(f_sumspan, INT_MAX);
+            st[RS_NONEMPTY] = (int32_t)min<int64_t>(f_nonempty, INT_MAX);
         }
     }
 }
@@ -522,7 +542,7 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
              const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ colidxC,
              const int64_t *__restrict__ tile_nnzC, const uint16_t *__restrict__ maskC, V *__restrict__ valC,
              uint8_t *__restrict__ colC, int32_t *__restrict__ overflow, const int32_t *__restrict__ row_nc,
-             int64_t val_cap) {
+             int64_t val_cap, int64_t val_done) {
     static_assert(Var::NC < WAVE && Var::NA <= 32 && Var::NB <= WAVE, "caps exceed the kernel's index widths");
     constexpr int ITA = (Var::PA + WAVE - 1) / WAVE;
     __shared__ NumLDS<V, Var> lds[Var::WAVES];
@@ -566,6 +586,7 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         if (lane == 0) atomicOr(overflow, 1);
         return;
     }
+    if (cend <= val_done) return;  // a re-run: this tile-row's values are already written
     // A values (one contiguous range)
     V av[ITA];
 #pragma unroll
@@ -781,7 +802,7 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
         TSG_CHECK(palloc(ctx, &p.rgw, (size_t)tilem + 3));
         TSG_CHECK(palloc(ctx, &p.rgn, (size_t)tilem + 1));
     }
-    if (zero_flags) TSG_CHECK(zero_small(ctx, p.overflow(), 1, nullptr, 0));
+    if (zero_flags) TSG_CHECK(zero_small(ctx, p.overflow(), PLAN_FLAG_WORDS, nullptr, 0));
     if (A->numtile > 0) {
         k_ab_link<<<nblk(A->numtile, 256), 256, 0, ctx->stream>>>(A->numtile, A->tile_columnidx, B->tile_ptr,
                                                                    ok ? B->row_nnz_ptr : nullptr, p.link, p.lpay);
@@ -816,10 +837,9 @@ int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1) {
     return TSG_OK;
 }
 
-// A->B links, Step 1 count, statistics, exclusive scan of tile_ptrC, binning
-// and one host read of (numtileC, bin offsets).
-static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                     const tsg_options *opts, Plan &p, int32_t *stats_out) {
+// A->B links, Step 1 count, statistics, exclusive scan of tile_ptrC.
+int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+               const tsg_options *opts, Plan &p, int32_t *stats_out) {
     const BinTable bt = bin_table_for(ctx, A, B, opts);
     const int32_t tilem = A->tilem;
     TSG_CHECK(plan_setup(ctx, A, B, bt, p, tilem == 0));
@@ -854,7 +874,14 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
         TSG_CHECK(rc);
         TSG_CHECK(scan_inclusive_i64(ctx, C->tile_ptr, (int64_t)tilem + 1));
     }
-    TSG_CHECK(plan_bin(ctx, p, tilem, C->tile_ptr + tilem, nullptr));
+    return TSG_OK;
+}
+
+// plan_count, binning and one host read of (numtileC, bin offsets).
+static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                     const tsg_options *opts, Plan &p) {
+    TSG_CHECK(plan_count(ctx, A, B, C, opts, p, nullptr));
+    TSG_CHECK(plan_bin(ctx, p, A->tilem, C->tile_ptr + A->tilem, nullptr));
     return plan_read(ctx, p, &C->numtile, nullptr);
 }
 
@@ -879,7 +906,7 @@ static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
         p.rows, p.bin_off[bin], n, A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
         p.link, B->tile_columnidx, B->csr_tile_nnz, B->csr_col_Ptr, B->csr_col_mask,
         (const V *)B->csr_col_Val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, (V *)C->tile_csr_Val,
-        C->tile_csr_Col, p.overflow(), sl.row_nc, sl.val_cap);
+        C->tile_csr_Col, p.overflow(), sl.row_nc, sl.val_cap, sl.val_done);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -963,7 +990,7 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     const FixedOut *fo = ctx->fixed_out;  // caller-owned C arrays (no-alloc entry point)
     if (fo) C->tile_ptr = fo->c.tile_ptr;
     else rc = palloc(ctx, &C->tile_ptr, (size_t)C->tilem + 1);
-    if (!rc) { rc = plan_rows(ctx, A, B, C, opts, p, nullptr); syncs++; }
+    if (!rc) { rc = plan_rows(ctx, A, B, C, opts, p); syncs++; }
     if (!rc && C->numtile > INT32_MAX) rc = fail(TSG_ERR_OVERFLOW, "numtileC exceeds int32 tile ids");
     if (!rc && fo) {
         if (C->numtile > fo->cap_tiles) {

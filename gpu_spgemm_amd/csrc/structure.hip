@@ -392,7 +392,7 @@ k_heavy_structure(const int32_t *__restrict__ hlist, const int32_t *__restrict__
         if (tid == 0) {
             if (!FILL) {
                 tile_ptrC[bi + 1] = count;
-                if (stats) stats[(int64_t)bi * 8 + 4] = (int32_t)min<int64_t>(count, INT_MAX);
+                if (stats) stats[(int64_t)bi * RS_NSTAT + RS_COUNT] = (int32_t)min<int64_t>(count, INT_MAX);
             } else if (row_nc) {
                 const int64_t n = min<int64_t>(out, slot_end) - tile_ptrC[bi];
                 row_nc[bi] = (int32_t)min<int64_t>(n, INT32_MAX);

@@ -142,7 +142,15 @@ struct SlotLayout {
     int32_t *row_nc = nullptr;     // actual tiles of each tile-row
     int32_t *overflow = nullptr;   // set when a row or C's values outgrow their capacity
     int64_t val_cap = INT64_MAX;   // capacity of C's value arrays
+    // values below this already hold their final results (a numeric re-run
+    // after a short capacity computes only the rows or tiles ending past it)
+    int64_t val_done = -1;
 };
+// Per-tile-row statistics the Step-1 count can emit (int32 each, clamped):
+// A tiles, A nonzeros, B tiles reached, B nonzeros reached, C tiles, span of
+// the reached B tile columns, longest reached B tile-row, summed spans of the
+// reached B tile-rows, reached B tile-rows that hold tiles.
+enum RowStat { RS_NA = 0, RS_PA, RS_NB, RS_PB, RS_COUNT, RS_SPAN, RS_MAXLEN, RS_SUMSPAN, RS_NONEMPTY, RS_NSTAT = 10 };
 // Tile-rows reaching more than HEAVY_NB B tiles are counted and filled by a
 // workgroup each (structure.hip) instead of one wave.
 constexpr int64_t HEAVY_NB = 8192;

@@ -308,17 +308,22 @@ int tsg_spgemm_noalloc(tsg_context *ctx, const tsg_tile_matrix *A, const tsg_til
                        int64_t cap_tiles, int64_t cap_nnz, const tsg_options *opts, tsg_stats *stats);
 
 /* ---- Learned scheduler (no reference counterpart: the reference sizes C with
- * exact count kernels and host reads, src/spgemm_raw_v2.cu:119-141, :191-211) ----
+ * exact counts and two blocking reads, numtileC after Step 1 and nnzC after
+ * Step 2, src/spgemm_raw_v2.cu:119-141, :191-211) ----
  * An MLP (8 features -> 32 -> 32 -> 2, ReLU, FP32; tsg_learned_num_params()
  * weights: W1[8][32], b1[32], W2[32][32], b2[32], W3[32][2], b3[2], row-major,
  * x @ W) predicts log1p of each C tile-row's tile count and nonzero count from
- * per-tile-row features of A*B: log1p of (A tiles, B tiles reached, A nonzeros,
- * B nonzeros reached, span of B tile columns, longest B tile-row reached, summed
- * B tile-row spans, nonempty B tile-rows reached).  With scheduler
- * TSG_SCHED_LEARNED, tsg_spgemm sizes C from the prediction and falls back to the
- * exact pipeline when any row outgrows it (stats->fallback_used = 1) or when the
- * predicted layout does not fit int32 tile ids or the free device memory
- * (stats->fallback_used = 2): the result is always the exact product. */
+ * per-tile-row features of A*B that the Step-1 count emits: log1p of (A tiles,
+ * B tiles reached, A nonzeros, B nonzeros reached, span of B tile columns,
+ * longest B tile-row reached, summed B tile-row spans, nonempty B tile-rows
+ * reached).  With scheduler TSG_SCHED_LEARNED, tsg_spgemm keeps the exact
+ * Step-1 count (C's tile arrays are exact) and sizes C's value arrays from the
+ * calibrated sum of the predicted nonzeros, so no host read separates the
+ * symbolic and numeric steps (stats->pred_nnzC: the capacity; pred_numtileC:
+ * the predicted tiles).  The numeric kernels write nothing past the capacity;
+ * when nnz(C) exceeds it, the numeric step runs again into exact arrays
+ * (stats->fallback_used = 1; 2 when the predicted arrays did not fit in
+ * memory): the result is always the exact product. */
 int tsg_learned_num_params(void);
 /* Appends (features, exact targets) of every tile-row of A*B to the context's
  * training set (runs the exact product once). */
@@ -330,9 +335,10 @@ int tsg_learned_clear(tsg_context *ctx);
 int tsg_learned_train(tsg_context *ctx, int32_t epochs, int32_t batch, float lr, uint64_t seed,
                       float *final_loss);
 int tsg_learned_get_weights(tsg_context *ctx, float *w, int32_t n);
-/* Log-space offsets (tiles, nonzeros) added to the predictions before C is
- * sized; training sets them to max(target - prediction) over the training set,
- * clamped to [log 1.05, log 16].  get and/or set: 2 floats each (may be NULL). */
+/* Log-space offsets (tiles, nonzeros) applied to the summed predictions before
+ * C is sized; training sets each to log 1.02 + the largest log(actual /
+ * predicted total) over the collected products, clamped to [log 1.02, log 16].
+ * get and/or set: 2 floats each (may be NULL). */
 int tsg_learned_calibration(tsg_context *ctx, float *get, const float *set);
 int tsg_learned_set_weights(tsg_context *ctx, const float *w, int32_t n);
 /* Host arrays: feats tilem x 8, pred tilem x 2 (expm1 of the network's outputs:

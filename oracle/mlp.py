@@ -16,7 +16,7 @@ checked in tests/test_oracle_mlp.py:
   * features   -- brute-force set loops over the nonzeros on small matrices,
                   and closed forms (identity, dense);
   * targets    -- tile and nonzero counts of the tiled oracle's C;
-  * calibrate  -- its defining property: every training row's capacity
+  * calibrate  -- its defining property: every training product's capacity
                   expm1(prediction + offset) covers the exact count.
 
 Parameter layout (row-major, x @ W): W1[8][32], b1[32], W2[32][32], b2[32],
@@ -33,7 +33,8 @@ P_B2 = P_W2 + NH * NH
 P_W3 = P_B2 + NH
 P_B3 = P_W3 + NH * NO
 NP = P_B3 + NO
-OFF_LO, OFF_HI = np.log(1.05), np.log(16.0)
+OFF_LO, OFF_HI = np.log(1.02), np.log(16.0)
+CAP_EXTRA = 1024
 
 
 def unpack(w):
@@ -148,21 +149,35 @@ def targets(Cd):
     return np.log1p(np.stack([nc, nz], 1).astype(np.float64)).astype(np.float32)
 
 
-def calibrate(Y, T):
-    """Log-space offsets (tiles, nonzeros): max over rows of (target - output),
-    clamped to [log 1.05, log 16]."""
-    d = np.max(np.asarray(T, np.float64) - np.asarray(Y, np.float64), axis=0) if len(Y) else np.zeros(2)
-    return np.clip(d, OFF_LO, OFF_HI)
+def row_predictions(F, Y, count):
+    """Per-row predictions as the device sums them (k_learned_cap): raw
+    outputs Y -> ceil(max(expm1(y), 0)), tiles clamped to min(B tiles reached,
+    tile-column span), nonzeros to 256 per exact C tile (count)."""
+    et = np.expm1(np.minimum(np.asarray(Y, np.float64), 40.0))
+    pt = np.minimum(np.ceil(np.maximum(et[:, 0], 0)), np.minimum(F[:, 1], F[:, 4]))
+    pn = np.minimum(np.ceil(np.maximum(et[:, 1], 0)), 256.0 * np.asarray(count, np.float64))
+    return pt, pn
 
 
-def capacities(F, Y, off, scale=1.0):
-    """Slot and value capacity of every tile-row from integer features F, raw
-    outputs Y and offsets: min(ceil(expm1(y+off)*scale)+1, exact bound), the
-    bound being min(B tiles reached, tile-column span) (and 256 values a tile)."""
-    bound = np.minimum(F[:, 1], F[:, 4])
-    extra = 1 if scale >= 1.0 else 0
-    pnc = np.maximum(np.expm1(np.minimum(Y[:, 0] + off[0], 40.0)), 0) * scale
-    pnz = np.maximum(np.expm1(np.minimum(Y[:, 1] + off[1], 40.0)), 0) * scale
-    cap = np.minimum(np.where(pnc >= bound, bound, np.ceil(pnc) + extra), bound).astype(np.int64)
-    vcap = np.where(pnz >= cap * 256, cap * 256, np.ceil(pnz) + 8 * extra).astype(np.int64)
-    return cap, vcap
+def calibrate(Y, T, F, mat_end):
+    """Log-space offsets (tiles, nonzeros): log 1.02 + the largest log(actual /
+    predicted total) over the products whose rows end at mat_end, clamped to
+    [log 1.02, log 16]; totals from row_predictions and the targets T."""
+    cnt = np.rint(np.expm1(np.asarray(T, np.float64)))
+    pt, pn = row_predictions(F, Y, cnt[:, 0])
+    off = np.full(2, -np.inf)
+    r0 = 0
+    for r1 in mat_end:
+        for o, pred in ((0, pt), (1, pn)):
+            act = cnt[r0:r1, o].sum()
+            if act > 0:
+                off[o] = max(off[o], np.log(act / max(pred[r0:r1].sum(), 1.0)))
+        r0 = r1
+    return np.clip(off + OFF_LO, OFF_LO, OFF_HI)
+
+
+def value_capacity(pred_nnz_sum, off_nnz, numtileC, scale=1.0):
+    """C's value capacity from the summed row predictions: the calibrated sum
+    (+ CAP_EXTRA unless scaled down), never above 256 values a C tile."""
+    want = np.ceil(pred_nnz_sum * np.exp(off_nnz) * scale) + (CAP_EXTRA if scale >= 1.0 else 0)
+    return int(min(want, 256 * numtileC))

@@ -45,7 +45,7 @@ def rmat20():
     print(f"[rmat20] nnzA {A.nnz}", flush=True)
     nnz, tiles = O.count_product(A, A, nthreads=NTHREADS)
     print(f"[rmat20] oracle count: nnzC {nnz} numtileC {tiles}", flush=True)
-    return A, nnz, tiles
+    return {"A": A, "nnz": nnz, "tiles": tiles}
 
 
 def check_rmat20_blocks(ctx, C, A, tiled_idx=(2, 4, 6), label=""):
@@ -65,23 +65,60 @@ def check_rmat20_blocks(ctx, C, A, tiled_idx=(2, 4, 6), label=""):
               f"extra {chk['extra']} max_scaled {chk['max_scaled']:.2e}", flush=True)
         assert chk["missing"] == 0 and chk["extra"] == 0 and chk["total_gpu"] == d["nnz"]
         assert chk["max_scaled"] <= 1e-5
-    exp = O.spgemm_tiled_rows(A, A, [blocks[i] for i in tiled_idx], f32=True)
+    exp = O.spgemm_tiled_rows(A, A, [blocks[i] for i in tiled_idx], f32=True) if tiled_idx else []
     for i, e in zip(tiled_idx, exp):
         assert_tiles_equal(got[i], e, keys=CKEYS, label=f"{label}block {blocks[i]} ")
     return got
 
 
 def test_rmat20_exact(ctx, rmat20):
-    A, nnz, tiles = rmat20
+    A, nnz, tiles = rmat20["A"], rmat20["nnz"], rmat20["tiles"]
     print("[case] R-MAT 20 exact pipeline", flush=True)
     tA, tB = ctx.tiles_from_csr(A), ctx.tiles_from_csr(A, as_b=True)
     C, st = ctx.spgemm(tA, tB)
     print(f"    numtileC {st['numtileC']} nnzC {st['nnzC']}", flush=True)
     assert st["numtileC"] == tiles and st["nnzC"] == nnz
-    check_rmat20_blocks(ctx, C, A)
+    rmat20["exact_blocks"] = check_rmat20_blocks(ctx, C, A)
     C.free()
     tA.free()
     tB.free()
+    ctx.pool_trim()   # 162 GB of freed C back to the device for the next context
+
+
+def test_rmat20_learned(rmat20):
+    """The learned scheduler on its own configuration: trained on R-MATs of
+    scales 17-19 with other seeds and a mixed family (never this product), it
+    sizes C's 9.7e9 values without a re-run, and C equals the exact
+    pipeline's."""
+    A = rmat20["A"]
+    print("[case] R-MAT 20 learned scheduler", flush=True)
+    c = g.Context(0)
+    try:
+        fam = [g.gen_rmat(sc, 16, seed=100 + sc, dtype=g.F32) for sc in (17, 18, 19)]
+        fam += [g.gen_laplace2d(256, 256), g.gen_er(20000, 6, seed=11), g.gen_er_var(20000, 0, 24, seed=14)]
+        for M in fam:
+            tA, tB = c.tiles_from_csr(M), c.tiles_from_csr(M, as_b=True)
+            c.learned_collect(tA, tB)
+            tA.free()
+            tB.free()
+        loss = c.learned_train(epochs=300, batch=4096, lr=3e-3, seed=0)
+        print(f"    trained: loss {loss:.4f} calibration {c.learned_calibration()}", flush=True)
+        tA, tB = c.tiles_from_csr(A), c.tiles_from_csr(A, as_b=True)
+        C, st = c.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
+        print(f"    fallback {st['fallback_used']} capacity {st['pred_nnzC']} nnz {st['nnzC']} "
+              f"predicted tiles {st['pred_numtileC']} tiles {st['numtileC']}", flush=True)
+        assert st["fallback_used"] == 0
+        assert st["numtileC"] == rmat20["tiles"] and st["nnzC"] == rmat20["nnz"]
+        assert st["pred_nnzC"] <= 1.5 * st["nnzC"]
+        got = check_rmat20_blocks(c, C, A, tiled_idx=(), label="learned ")
+        if "exact_blocks" in rmat20:
+            for i, (e, l) in enumerate(zip(rmat20["exact_blocks"], got)):
+                assert_tiles_equal(l, e, keys=CKEYS, label=f"learned block {i} ")
+        C.free()
+        tA.free()
+        tB.free()
+    finally:
+        c.close()
 
 
 def test_batch512_uneven(ctx):

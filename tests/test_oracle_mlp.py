@@ -154,19 +154,34 @@ def test_targets_from_tiled_oracle():
     assert np.allclose(T[:, 0], tiles, rtol=1e-6)
 
 
-def test_calibrated_capacities_cover_every_row():
+def test_calibrated_capacity_covers_every_training_product():
+    """calibrate's defining property: the value capacity of every product it
+    saw covers that product's nnz; capacities never exceed 256 a C tile; the
+    forced-rerun scale makes one fall short; a perfect model gets the margin."""
     r = np.random.default_rng(11)
-    F = r.integers(1, 400, (500, 8))
-    exact = np.minimum(r.integers(1, 300, 500), np.minimum(F[:, 1], F[:, 4]))
-    nnz = exact * r.integers(1, 200, 500)
-    T = np.log1p(np.stack([exact, nnz], 1).astype(np.float64))
-    Y = T + r.normal(0, 0.2, T.shape)         # a model with 20 % log error
-    off = M.calibrate(Y, T)
-    cap, vcap = M.capacities(F, Y, off)
-    assert np.all(cap >= exact) and vcap.sum() >= nnz.sum()
-    assert np.all(cap <= np.minimum(F[:, 1], F[:, 4]))
-    # shrinking (the forced-fallback hook) makes some row too small
-    cap2, _ = M.capacities(F, Y, off, scale=0.01)
-    assert np.any(cap2 < exact)
-    # a perfect model gets the minimum offset
-    assert np.allclose(M.calibrate(T, T), M.OFF_LO)
+    nrow = 600
+    F = r.integers(1, 400, (nrow, 8))
+    count = np.minimum(r.integers(1, 300, nrow), np.minimum(F[:, 1], F[:, 4]))
+    nnz = count * r.integers(1, 200, nrow)
+    T = np.log1p(np.stack([count, nnz], 1).astype(np.float64))
+    Y = T + r.normal(-0.1, 0.3, T.shape)        # a biased model with 30 % log error
+    mat_end = [100, 250, 400, 600]              # four products
+    off = M.calibrate(Y, T, F, mat_end)
+    _, pn = M.row_predictions(F, Y, count)
+    r0 = 0
+    for r1 in mat_end:
+        tiles = int(count[r0:r1].sum())
+        cap = M.value_capacity(pn[r0:r1].sum(), off[1], tiles)
+        assert cap >= nnz[r0:r1].sum() and cap <= 256 * tiles
+        r0 = r1
+    # shrinking (the forced re-run hook) makes some product too small
+    short = [M.value_capacity(pn[a:b].sum(), off[1], int(count[a:b].sum()), scale=0.01) < nnz[a:b].sum()
+             for a, b in zip([0] + mat_end[:-1], mat_end)]
+    assert any(short)
+    # a perfect model gets the minimum offset (its rounded-up rows cover the totals)
+    assert np.allclose(M.calibrate(T, T, F, mat_end), M.OFF_LO)
+    # a planted mistake -- offsets from the best instead of the worst product -- leaves one short
+    cnt = np.rint(np.expm1(T))
+    best = min(np.log(cnt[a:b, 1].sum() / pn[a:b].sum()) for a, b in zip([0] + mat_end[:-1], mat_end))
+    assert any(M.value_capacity(pn[a:b].sum(), best, int(count[a:b].sum())) - M.CAP_EXTRA < nnz[a:b].sum()
+               for a, b in zip([0] + mat_end[:-1], mat_end))
