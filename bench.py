@@ -79,6 +79,7 @@ def train_learned(g, ctx, log, rmat_scale=0):
     benchmarked product itself), outside any timed region; for an R-MAT
     workload the family adds R-MATs 2-8x smaller with other seeds."""
     fam = [g.gen_laplace2d(256, 256), g.gen_laplace2d(512, 384), g.gen_laplace2d(300, 700),
+           g.gen_laplace2d(1024, 96), g.gen_laplace2d(4096, 24),   # wider grids, few rows
            g.gen_laplace3d(40, 40, 40), g.gen_stencil27(32, 32, 32), g.gen_er(20000, 6, seed=11),
            g.gen_er(20000, 12, seed=12), g.gen_er_var(20000, 0, 24, seed=14),
            g.gen_rmat(13, 8, seed=15, dtype=g.F64)]

@@ -49,6 +49,17 @@ struct Learned {
     float *X = nullptr, *T = nullptr;  // training set (device)
     int64_t n = 0, cap = 0;
     std::vector<int64_t> mat_end;      // end row of each collected product in the training set
+    // The last product's summed predictions, keyed by its operands: a repeat
+    // of the same operands reuses them instead of evaluating the network
+    // again.  Only C's value capacity depends on it and the kernels check
+    // that capacity, so a stale entry can cost a numeric re-run, never a
+    // wrong result.
+    struct Reuse {
+        const void *a = nullptr, *b = nullptr;
+        int64_t key[8] = {};
+        int64_t pred_tiles = 0, pred_nnz = 0;
+        bool valid = false;
+    } reuse;
 };
 
 static Learned *state(Context *ctx) {
@@ -66,26 +77,37 @@ void learned_release(Context *ctx) {
 }
 
 // ---------------------------------------------------------------- MLP
+// Every unit sums bias + inputs in ascending input order, one FMA each.  The
+// loops run input-major so each input's weight row (32 consecutive floats in
+// LDS) is read as vectors and every unit's partial sum stays in a register.
 __device__ __forceinline__ void mlp_eval(const float *w, const float *x, float *h1, float *h2, float *y) {
-#pragma unroll 4
-    for (int j = 0; j < NH; j++) {
-        float s = w[P_B1 + j];
 #pragma unroll
-        for (int i = 0; i < NF; i++) s = __builtin_fmaf(x[i], w[P_W1 + i * NH + j], s);
-        h1[j] = s > 0.0f ? s : 0.0f;
-    }
-#pragma unroll 4
-    for (int j = 0; j < NH; j++) {
-        float s = w[P_B2 + j];
-        for (int i = 0; i < NH; i++) s = __builtin_fmaf(h1[i], w[P_W2 + i * NH + j], s);
-        h2[j] = s > 0.0f ? s : 0.0f;
+    for (int j = 0; j < NH; j++) h1[j] = w[P_B1 + j];
+#pragma unroll
+    for (int i = 0; i < NF; i++) {
+        const float xi = x[i];
+#pragma unroll
+        for (int j = 0; j < NH; j++) h1[j] = __builtin_fmaf(xi, w[P_W1 + i * NH + j], h1[j]);
     }
 #pragma unroll
-    for (int o = 0; o < NO; o++) {
-        float s = w[P_B3 + o];
-        for (int i = 0; i < NH; i++) s = __builtin_fmaf(h2[i], w[P_W3 + i * NO + o], s);
-        y[o] = s;
+    for (int j = 0; j < NH; j++) {
+        h1[j] = h1[j] > 0.0f ? h1[j] : 0.0f;
+        h2[j] = w[P_B2 + j];
     }
+#pragma unroll 4
+    for (int i = 0; i < NH; i++) {
+        const float hi = h1[i];
+#pragma unroll
+        for (int j = 0; j < NH; j++) h2[j] = __builtin_fmaf(hi, w[P_W2 + i * NH + j], h2[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NH; j++) h2[j] = h2[j] > 0.0f ? h2[j] : 0.0f;
+#pragma unroll
+    for (int o = 0; o < NO; o++) y[o] = w[P_B3 + o];
+#pragma unroll
+    for (int i = 0; i < NH; i++)
+#pragma unroll
+        for (int o = 0; o < NO; o++) y[o] = __builtin_fmaf(h2[i], w[P_W3 + i * NO + o], y[o]);
 }
 
 __global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ wg, const float *__restrict__ X,
@@ -336,12 +358,23 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     int syncs = 0, fallback = 0;
     int64_t cap = 0, pred_nnz = 0;
     PlanFlags fl{};
+    Learned::Reuse &ru = L->reuse;
+    const int64_t key[8] = {A->nnz, A->numtile, B->nnz, B->numtile, A->tilem, B->tilen, A->dtype, pscale == 1.0};
+    const bool reuse = ru.valid && ru.a == A->tile_ptr && ru.b == B->tile_ptr && !memcmp(ru.key, key, sizeof key);
     if (timing && !rc) (void)hipEventRecord(ev[0], s);
     if (!rc) rc = palloc(ctx, &C->tile_ptr, (size_t)tilem + 1);
-    if (!rc) rc = palloc(ctx, &stats, (size_t)std::max(tilem, 1) * RS_NSTAT);
-    if (!rc) rc = learned_count(ctx, A, B, C, opts, p, stats, L->w, nullptr, nullptr);
-    if (!rc) rc = plan_bin(ctx, p, tilem, C->tile_ptr + tilem, reinterpret_cast<const int64_t *>(&p.flags->pred_nnz));
-    if (!rc) { rc = plan_read(ctx, p, &C->numtile, &pred_nnz); syncs++; }
+    if (reuse) {  // the plain Step-1 count; the predictions are this operand pair's last ones
+        if (!rc) rc = plan_count(ctx, A, B, C, opts, p, nullptr);
+        if (!rc) rc = plan_bin(ctx, p, tilem, C->tile_ptr + tilem, nullptr);
+        if (!rc) { rc = plan_read(ctx, p, &C->numtile, nullptr); syncs++; }
+        pred_nnz = ru.pred_nnz;
+    } else {
+        if (!rc) rc = palloc(ctx, &stats, (size_t)std::max(tilem, 1) * RS_NSTAT);
+        if (!rc) rc = learned_count(ctx, A, B, C, opts, p, stats, L->w, nullptr, nullptr);
+        if (!rc)
+            rc = plan_bin(ctx, p, tilem, C->tile_ptr + tilem, reinterpret_cast<const int64_t *>(&p.flags->pred_nnz));
+        if (!rc) { rc = plan_read(ctx, p, &C->numtile, &pred_nnz); syncs++; }
+    }
     if (!rc && C->numtile > INT32_MAX) rc = fail(TSG_ERR_OVERFLOW, "numtileC exceeds int32 tile ids");
     if (!rc) rc = palloc(ctx, &C->tile_rowidx, (size_t)C->numtile);
     if (!rc) rc = palloc(ctx, &C->tile_columnidx, (size_t)C->numtile);
@@ -381,6 +414,16 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
         syncs++;
         memcpy(&fl, ctx->pinned, sizeof fl);
         C->nnz = fl.nnz;
+        if (reuse) {
+            fl.pred_tiles = (unsigned long long)ru.pred_tiles;
+        } else if (!rc) {
+            ru.a = A->tile_ptr;
+            ru.b = B->tile_ptr;
+            memcpy(ru.key, key, sizeof key);
+            ru.pred_tiles = (int64_t)fl.pred_tiles;
+            ru.pred_nnz = pred_nnz;
+            ru.valid = true;
+        }
     }
     if (!rc && fl.nnz <= cap && fl.cap) rc = fail(TSG_ERR_OVERFLOW, "a staged row exceeded its variant's capacity");
     if (!rc && fl.nnz > cap) {
@@ -411,6 +454,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
         st->fallback_used = fallback;
         st->pred_numtileC = (int64_t)std::ceil((double)fl.pred_tiles * std::exp((double)L->off_nc));
         st->pred_nnzC = cap;
+        st->pred_reused = reuse ? 1 : 0;
     }
     if (timing) {
         if (!rc && st) {
@@ -614,6 +658,7 @@ int tsg_learned_clear(tsg_context *c) {
     Learned *L = state(ctx);
     L->n = 0;
     L->mat_end.clear();
+    L->reuse.valid = false;
     return TSG_OK;
 }
 
@@ -673,6 +718,7 @@ int tsg_learned_train(tsg_context *c, int32_t epochs, int32_t batch, float lr, u
     if (rc) return rc;
     TSG_CHECK(calibrate(ctx));
     L->init = true;
+    L->reuse.valid = false;
     if (final_loss) *final_loss = (float)(epoch_loss / (double)n);
     return TSG_OK;
 }
@@ -686,6 +732,7 @@ int tsg_learned_calibration(tsg_context *c, float *get, const float *set) {
             return fail(TSG_ERR_ARG, "calibration offsets must lie in [0, 20]");
         L->off_nc = set[0];
         L->off_nnz = set[1];
+        L->reuse.valid = false;
         }
     if (get) {
         get[0] = L->off_nc;
@@ -717,6 +764,7 @@ int tsg_learned_set_weights(tsg_context *c, const float *w, int32_t n) {
     TSG_HIP(hipStreamSynchronize(ctx->stream));
     L->step = 0;
     L->init = true;
+    L->reuse.valid = false;
     return TSG_OK;
 }
 

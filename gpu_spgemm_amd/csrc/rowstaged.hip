@@ -126,13 +126,6 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
         const int incl = dpp_incl_sum(l.y);
         nb = last_lane(incl);
         pb = last_lane(dpp_incl_sum(l.z));
-        if (stats && nb <= WAVE) {
-            const bool ne = lane < na && l.y > 0;
-            const int64_t sp = ne ? (int64_t)colidxB[l.x + l.y - 1] - colidxB[l.x] + 1 : 0;
-            f_sumspan = wave_sum(sp);
-            f_maxlen = wave_max(l.y);
-            f_nonempty = __popcll(__ballot(ne));
-        }
         if (nb <= WAVE) {
             if (lane < na) {
                 s_pre[w][lane + 1] = incl;
@@ -141,9 +134,19 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
             if (lane == 0) s_pre[w][0] = 0;
             wave_sync();
             int32_t j = 0;
+            bool seg_first = false, seg_last = false;  // first / last B tile of its A tile's B tile-row
             if (lane < nb) {
                 const int t = owner_of(s_pre[w], (int)na, lane);
-                j = colidxB[s_bs[w][t] + (lane - s_pre[w][t])];
+                const int pos = lane - s_pre[w][t];
+                j = colidxB[s_bs[w][t] + pos];
+                seg_first = pos == 0;
+                seg_last = lane == s_pre[w][t + 1] - 1;
+            }
+            if (stats) {  // learned-scheduler features from the columns already loaded
+                // each A tile's span = last - first + 1 of its B tile-row's columns
+                f_sumspan = wave_sum((int64_t)(seg_last ? j : 0) - (seg_first ? (int64_t)j - 1 : 0));
+                f_maxlen = last_lane(dpp_incl_max(lane < na ? l.y : 0));
+                f_nonempty = __popcll(__ballot(lane < na && l.y > 0));
             }
             if (nb > 0) {
                 cmin = -last_lane(dpp_incl_max(lane < nb ? -j : -INT_MAX));
@@ -555,6 +558,7 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
     int64_t cs, ce;
     c_row_range(tile_ptrC, row_nc, bi, cs, ce);
+    if (val_done >= 0 && tile_nnzC[ce] <= val_done) return;  // a re-run: this tile-row's values are written
     const int na = (int)(ae - as), nc = (int)(ce - cs);
     if (na > Var::NA || nc > Var::NC) {
         if (lane == 0) atomicOr(overflow, 1);
@@ -586,7 +590,6 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         if (lane == 0) atomicOr(overflow, 1);
         return;
     }
-    if (cend <= val_done) return;  // a re-run: this tile-row's values are already written
     // A values (one contiguous range)
     V av[ITA];
 #pragma unroll

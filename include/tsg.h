@@ -156,7 +156,9 @@ typedef struct tsg_stats {
     int32_t fallback_used;  /* learned: 1 when the exact path re-ran after an overflow, 2 when the
                                predicted layout exceeded int32 tile ids or free device memory */
     int32_t host_syncs;     /* blocking device-to-host reads made inside the call */
-    int64_t pred_numtileC, pred_nnzC;  /* learned: allocated capacities */
+    int64_t pred_numtileC, pred_nnzC;  /* learned: predicted C tiles; C's value capacity */
+    int32_t pred_reused;    /* learned: 1 when the predictions were this operand pair's previous ones */
+    int32_t reserved;
 } tsg_stats;
 
 /* ---- context ---- */
