@@ -121,6 +121,30 @@ def profiled_traffic(workload, kernel):
     return None
 
 
+def tile_sizes(T):
+    return {"tilem": T.s.tilem, "tilen": T.s.tilen, "numtile": T.s.numtile, "nnz": T.s.nnz}
+
+
+def cpu_baseline_many(pairs, seconds, nthreads=1):
+    """cpu_baseline's CPU path over a list of products (A_i, B_i): whole
+    products in order until `seconds` have passed (at least one), the rate
+    scaled to all of them."""
+    from oracle import oracle as O
+    total = sum(O.flops(A, B) for A, B in pairs)
+    done, t = 0, 0.0
+    n = 0
+    for A, B in pairs:
+        prod, gus, conv = O.cpu_baseline(A, B, 0, A.m, nthreads=nthreads, convert=True)
+        done, t, n = done + prod, t + gus + conv, n + 1
+        if t >= seconds:
+            break
+    est = t * total / max(done, 1)
+    return {"value": round(2.0 * total / est / 1e9, 4), "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
+            "sample": f"the first {n} of {len(pairs)} products ({done} of {total} products, {t:.2f} s): host "
+                      f"csr2tile + build_csc_tiles on 1 thread and the Gustavson FP64 loop of the reference's check "
+                      f"on {nthreads} thread(s), scaled to all products ({est:.1f} s)"}
+
+
 def bench_batch(g, args):
     """512 uneven independent products A_i*A_i (FP64): one batched call vs a
     loop of single calls over the same resident operands."""
@@ -171,6 +195,10 @@ def bench_batch(g, args):
     res["graph_replay_numeric"] = (time.perf_counter() - t1) * 1e3 / args.steps
     log(f"graph replay (values only): {res['graph_replay_numeric']:.3f} ms per batch")
     plan.destroy()
+    Cs, sts = ctx.spgemm_batched(tA, tB)
+    pipe = sum(alg_bytes(tile_sizes(a), tile_sizes(b), tile_sizes(c), 8) for a, b, c in zip(tA, tB, Cs))
+    for C in Cs:
+        C.free()
     ms = res["batched"]
     out = {
         "metric": "SpGEMM GFLOP/s + HBM GB/s, learned/batched vs plain tiled, 1 MI355X",
@@ -184,16 +212,49 @@ def bench_batch(g, args):
         "batched_speedup_vs_loop": round(res["loop"] / res["batched"], 3),
         "autotuned": {"var_mask": best["var_mask"], "rg_density": best["rg_density"],
                       "candidates": len(rep)},
-        "roofline": None, "cpu_baseline": None,
+        # the batch runs dozens of kernels: the bound is the whole call's algorithmic bytes
+        "roofline": {"bound": "hbm", "kernel": "batched pipeline (all kernels)",
+                     "achieved": round(pipe / (ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(pipe / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": pipe, "launch_ms": round(ms, 4)},
+        "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline_many([(A, A) for _, A in probs], args.cpu_seconds)
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def train_galerkin(g, ctx, log):
+    """Trains the learned scheduler on smaller Galerkin chains (A P and
+    P^T (A P) of 27-point stencils on 24^3-48^3 grids, other than the
+    benchmarked 128^3) and a mixed family, outside any timed region."""
+    ctx.learned_clear()
+    for n in (24, 32, 40, 48):
+        A, P = g.gen_stencil27(n, n, n), g.gen_aggregation(n, n, n)
+        tA, tP, tPt = ctx.tiles_from_csr(A), ctx.tiles_from_csr(P, as_b=True), ctx.tiles_from_csr(P.transpose())
+        ctx.learned_collect(tA, tP)
+        C1, _ = ctx.spgemm(tA, tP)
+        C1.build_csc_tiles()
+        ctx.learned_collect(tPt, C1)
+        for T in (C1, tA, tP, tPt):
+            T.free()
+    for M in (g.gen_laplace3d(40, 40, 40), g.gen_er(20000, 6, seed=11)):
+        tA, tB = ctx.tiles_from_csr(M), ctx.tiles_from_csr(M, as_b=True)
+        ctx.learned_collect(tA, tB)
+        tA.free()
+        tB.free()
+    loss = ctx.learned_train(epochs=300, batch=4096, lr=3e-3, seed=0)
+    log(f"learned scheduler trained on Galerkin chains 24^3-48^3, loss {loss:.4f}, "
+        f"calibration {ctx.learned_calibration().tolist()}")
 
 
 def bench_galerkin(g, args):
     """Galerkin triple product P^T (A P), FP64: A = 27-point stencil on a 128^3
     grid, P = 2x2x2 piecewise-constant aggregation.  A step = C1 = A P, C1
-    re-laid as a B operand on the GPU, C2 = P^T C1 (two chained products)."""
+    re-laid as a B operand on the GPU, C2 = P^T C1 (two chained products),
+    under the plain and the learned scheduler."""
     n = 128
     A = g.gen_stencil27(n, n, n)
     P = g.gen_aggregation(n, n, n)
@@ -204,26 +265,39 @@ def bench_galerkin(g, args):
     C1, _ = ctx.spgemm(tA, tP)
     c1 = C1.to_csr()          # products of the second product, counted once before timing
     p2 = Pt.products(c1)
+    C1.build_csc_tiles()
+    C2, _ = ctx.spgemm(tPt, C1)
+    pipe = (alg_bytes(tile_sizes(tA), tile_sizes(tP), tile_sizes(C1), 8) +
+            alg_bytes(tile_sizes(tPt), tile_sizes(C1), tile_sizes(C2), 8))
     C1.free()
+    C2.free()
     log(f"galerkin 128^3: A nnz {A.nnz}, products A*P {p1}, P^T*(AP) {p2}")
 
-    def step():
-        C1, s1 = ctx.spgemm(tA, tP)
+    def step(**kw):
+        C1, s1 = ctx.spgemm(tA, tP, **kw)
         C1.build_csc_tiles()
-        C2, s2 = ctx.spgemm(tPt, C1)
+        C2, s2 = ctx.spgemm(tPt, C1, **kw)
         C1.free()
         C2.free()
         return s1, s2
 
-    for _ in range(args.warmup):
-        step()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    ctx.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    log(f"chain: {ms:.3f} ms per step")
+    def timed(**kw):
+        for _ in range(args.warmup):
+            step(**kw)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step(**kw)
+        ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps
+
+    ms = timed()
+    log(f"chain (plain): {ms:.3f} ms per step")
+    train_galerkin(g, ctx, log)
+    s1, s2 = step(scheduler=g.SCHED_LEARNED)
+    ms_l = timed(scheduler=g.SCHED_LEARNED)
+    log(f"chain (learned): {ms_l:.3f} ms per step, re-runs {s1['fallback_used']}/{s2['fallback_used']}, "
+        f"value capacity / nnz {s1['pred_nnzC'] / s1['nnzC']:.3f} / {s2['pred_nnzC'] / s2['nnzC']:.3f}")
     out = {
         "metric": "SpGEMM GFLOP/s + HBM GB/s, learned/batched vs plain tiled, 1 MI355X",
         "value": round(2.0 * (p1 + p2) / (ms * 1e6), 3), "unit": "GFLOP/s", "n_gpus": 1,
@@ -232,17 +306,29 @@ def bench_galerkin(g, args):
         "data": "synthetic (generated on the host, converted to tiles on the GPU before timing)",
         "config": {"workload": "galerkin_PtAP_stencil27_128cube", "products": p1 + p2,
                    "scheduler": "plain", "parallelism": "single GPU"},
-        "roofline": None, "cpu_baseline": None,
+        "learned": {"ms_per_step": round(ms_l, 4), "vs_plain": round(ms / ms_l, 3),
+                    "fallback": [s1["fallback_used"], s2["fallback_used"]]},
+        "roofline": {"bound": "hbm", "kernel": "chain (both products, all kernels)",
+                     "achieved": round(pipe / (ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(pipe / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "alg_bytes_per_launch": pipe, "launch_ms": round(ms, 4)},
+        "cpu_baseline": None,
     }
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline_many([(A, P), (Pt, c1)], args.cpu_seconds)
     ctx.close()
     print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(A, B, seconds, nthreads=1):
-    """The oracle's Gustavson loop (the reference's CPU check) on `nthreads`
-    threads (row blocks): the whole product when it finishes within
-    `seconds`, else row blocks at 8 evenly spaced starts holding about
-    `seconds` worth of products in total."""
+    """The reference's CPU path restated by the oracle: host csr2tile of A and B
+    plus build_csc_tiles of B (src/tile_format.h:93-222, :424-546; always one
+    thread, as the reference) and the Gustavson FP64 loop of its check
+    (src/spgemm_raw_v2.cu:297-308) on `nthreads` threads over row blocks.  The
+    whole product when it finishes within `seconds`, else the conversion once
+    and the Gustavson loop on row blocks at 8 evenly spaced starts holding
+    about `seconds` worth of products, scaled to the whole product."""
     import numpy as np
     from oracle import oracle as O
     m = A.m
@@ -253,33 +339,42 @@ def cpu_baseline(A, B, seconds, nthreads=1):
         end = int(np.searchsorted(cum, cum[start] + want, side="left"))
         return start, max(start + 1, min(stop, end))
 
+    _, _, conv = O.cpu_baseline(A, B, 0, 0, nthreads=1, convert=True)      # conversion alone
     r0, r1 = block(m // 2, min(total, 20_000_000), m)
     p0, g0, _ = O.cpu_baseline(A, B, r0, r1, nthreads=nthreads, convert=False)
     rate = p0 / max(g0, 1e-6)
-    if total / rate <= seconds:
+    what = (f"host csr2tile(A, B) + build_csc_tiles(B) on 1 thread (src/tile_format.h:93-222, :424-546) and the "
+            f"Gustavson FP64 loop of the reference's check (src/spgemm_raw_v2.cu:297-308) on {nthreads} thread(s)")
+    if conv + total / rate <= seconds:
         times = []
         prod = 0
         t_end = time.time() + seconds
         while time.time() < t_end or len(times) < 3:
-            prod, gus, _ = O.cpu_baseline(A, B, 0, m, nthreads=nthreads, convert=False)
-            times.append(gus)
+            prod, gus, conv = O.cpu_baseline(A, B, 0, m, nthreads=nthreads, convert=True)
+            times.append(gus + conv)
             if len(times) >= 7:
                 break
         med = statistics.median(times)
         return {"value": round(2.0 * prod / med / 1e9, 4), "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
-                "sample": f"whole product, Gustavson FP64 loop of the reference's CPU check "
-                          f"(src/spgemm_raw_v2.cu:297-308), {nthreads} thread(s), median of {len(times)} runs "
-                          f"({med:.3f} s each, {prod} products)"}
-    want = rate * seconds / 8
+                "sample": f"whole product: {what}; median of {len(times)} runs ({med:.3f} s each, of which "
+                          f"conversion {conv:.3f} s; {prod} products)"}
+    want = rate * max(seconds - conv, 1.0) / 8
     prod, gus, rows = 0, 0.0, 0
     for i in range(8):
         a, b = block(i * m // 8, want, (i + 1) * m // 8)
         p, t, _ = O.cpu_baseline(A, B, a, b, nthreads=nthreads, convert=False)
         prod, gus, rows = prod + p, gus + t, rows + (b - a)
-    return {"value": round(2.0 * prod / gus / 1e9, 4), "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
-            "sample": f"{rows} of {m} rows in 8 blocks at evenly spaced starts, {prod} of {total} products "
-                      f"({gus:.2f} s), Gustavson FP64 loop of the reference's CPU check "
-                      f"(src/spgemm_raw_v2.cu:297-308), {nthreads} thread(s)"}
+    est = conv + gus * total / prod
+    return {"value": round(2.0 * total / est / 1e9, 4), "unit": "GFLOP/s", "cores": nthreads, "kind": "port",
+            "sample": f"{what}: conversion of the whole operands ({conv:.2f} s) plus the Gustavson loop on {rows} of "
+                      f"{m} rows in 8 blocks at evenly spaced starts ({prod} of {total} products, {gus:.2f} s), "
+                      f"scaled to the whole product ({est:.1f} s)"}
+
+
+def host_threads():
+    """Host cores this run may use (at most 16)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
 def main():
@@ -288,9 +383,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="lap2d_2048")
-    ap.add_argument("--scheduler", default="plain", choices=["plain", "learned"])
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-scheduler leg")
+    ap.add_argument("--no-batch-leg", action="store_true", help="skip the batched-vs-loop leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB 5 times (the known-size kernel "
                          "that scales the FETCH_SIZE / WRITE_SIZE counters) and exit")
@@ -349,16 +445,24 @@ def main():
     runs["plain"] = measure("plain (binned, exact sizes)", variant=0)
     if args.workload == "band":     # full tiles: matrix cores for the dense pairs
         runs["mfma"] = measure("C-tile + matrix cores", variant=2)
-    if args.scheduler == "learned" or rmat:
+    learned = None
+    if not args.no_learned:
         train_learned(g, ctx, log, int(args.workload[4:]) if rmat else 0)
-        C0, st = ctx.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
+        t0 = time.perf_counter()
+        C0, st = ctx.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)   # the network evaluated on this product
+        ctx.synchronize()
+        first_ms = (time.perf_counter() - t0) * 1e3
         C0.free()
-        log(f"learned: fallback {st['fallback_used']} predicted tiles {st['pred_numtileC']} (tiles {st['numtileC']}) "
-            f"value capacity {st['pred_nnzC']} (nnz {st['nnzC']})")
-        learned_fallback = st["fallback_used"]
-        runs["learned"] = measure("learned", scheduler=g.SCHED_LEARNED)
-    best = min(runs, key=lambda k: runs[k][0])      # the faster exact-result scheduler is the headline
-    ms, ev, Cd = runs[best]
+        log(f"learned: first call {first_ms:.3f} ms, fallback {st['fallback_used']}, predicted tiles "
+            f"{st['pred_numtileC']} (tiles {st['numtileC']}), value capacity {st['pred_nnzC']} (nnz {st['nnzC']})")
+        runs["learned"] = measure("learned (predictions reused)", scheduler=g.SCHED_LEARNED)
+        learned = {"ms_per_step": round(runs["learned"][0], 5),
+                   "vs_plain": round(runs["plain"][0] / runs["learned"][0], 3),
+                   "fallback": st["fallback_used"], "first_call_ms": round(first_ms, 4),
+                   "value_capacity_over_nnz": round(st["pred_nnzC"] / max(1, st["nnzC"]), 4),
+                   "predicted_tiles_over_tiles": round(st["pred_numtileC"] / max(1, st["numtileC"]), 4)}
+    # the headline is the exact plain pipeline; the other schedulers are reported beside it
+    ms, ev, Cd = runs["plain"]
     step1, step2, step3 = ev["ms_step1"], ev["ms_step2"], ev["ms_step3"]
     gflops = 2.0 * products / (ms * 1e6)
     pipe_bytes = alg_bytes(Ad, Bd, Cd, s)
@@ -366,8 +470,7 @@ def main():
     achieved = nbytes / (step3 * 1e-3) / 1e9
     # the numeric step of this workload is one launch of the staged numeric
     # kernel (every tile-row fits the small variant); its HIP-event time is step3
-    kname = {"plain": "k_num_staged<double,VarS>", "mfma": "k_numeric_mfma<double>",
-             "ctile": "k_numeric_tiles<double>"}.get(best, "numeric step (learned)")
+    kname = "k_num_staged<double,VarS>"
     if rmat:   # heavy rows: several numeric launches (row-by-row and C-tile kernels)
         kname = "numeric step (all bins)"
     out = {
@@ -383,7 +486,7 @@ def main():
         "vs_baseline": None,
         "dtype": cfg["dtype"],
         "data": "synthetic (generated on the host, converted to tiles on the GPU before timing)",
-        "config": {**cfg, "scheduler": best, "products": products,
+        "config": {**cfg, "scheduler": "plain", "products": products,
                    "numtileC": Cd["numtile"], "nnzC": Cd["nnz"], "parallelism": "single GPU"},
         "pipeline_alg_GBps": round(pipe_bytes / (ms * 1e-3) / 1e9, 2),
         "pipeline_alg_bytes": pipe_bytes,
@@ -396,17 +499,57 @@ def main():
                      "launch_ms": round(step3, 5)},
         "cpu_baseline": None,
     }
-    if "learned" in runs:
-        out["learned_vs_plain"] = round(runs["plain"][0] / runs["learned"][0], 3)
-        out["learned_fallback"] = learned_fallback
+    if learned:
+        out["learned"] = learned
+        out["learned_vs_plain"] = learned["vs_plain"]
+        out["learned_fallback"] = learned["fallback"]
+    if not args.no_batch_leg and not rmat:
+        out["batched"] = batch_leg(g, ctx, args)
+        out["batched_vs_loop"] = out["batched"]["vs_loop"]
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(A, B, args.cpu_seconds)
-        # the same loop over row blocks on the host cores this run may use (at most 16)
-        nt = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
-        out["cpu_baseline_parallel"] = cpu_baseline(A, B, args.cpu_seconds, nthreads=nt)
+        out["cpu_baseline_parallel"] = cpu_baseline(A, B, args.cpu_seconds, nthreads=host_threads())
     ctx.close()
     print(json.dumps(out), flush=True)
+
+
+def batch_leg(g, ctx, args, count=64, hi=16384):
+    """Batched vs a loop of single calls on a fixed small uneven batch (the
+    batch generator's first `count` problems at seed 4 with n_i up to `hi`),
+    same warmup and steps as the headline."""
+    probs = g.gen_batch_uneven(count, seed=4, hi=hi)
+    products = sum(A.products(A) for _, A in probs)
+    tA = [ctx.tiles_from_csr(A) for _, A in probs]
+    tB = [ctx.tiles_from_csr(A, as_b=True) for _, A in probs]
+
+    def loop():
+        for a, b in zip(tA, tB):
+            C, _ = ctx.spgemm(a, b)
+            C.free()
+
+    def batched():
+        Cs, _ = ctx.spgemm_batched(tA, tB)
+        for C in Cs:
+            C.free()
+
+    res = {}
+    for label, fn in (("loop", loop), ("batched", batched)):
+        for _ in range(args.warmup):
+            fn()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        ctx.synchronize()
+        res[label] = (time.perf_counter() - t1) * 1e3 / args.steps
+    log(f"batch of {count} (n_i <= {hi}): loop {res['loop']:.3f} ms, batched {res['batched']:.3f} ms")
+    for t in tA + tB:
+        t.free()
+    return {"problems": count, "max_rows": hi, "seed": 4, "products": products,
+            "ms_loop": round(res["loop"], 4), "ms_batched": round(res["batched"], 4),
+            "vs_loop": round(res["loop"] / res["batched"], 3),
+            "batched_gflops": round(2.0 * products / (res["batched"] * 1e6), 3)}
 
 
 if __name__ == "__main__":

@@ -49,17 +49,24 @@ struct Learned {
     float *X = nullptr, *T = nullptr;  // training set (device)
     int64_t n = 0, cap = 0;
     std::vector<int64_t> mat_end;      // end row of each collected product in the training set
-    // The last product's summed predictions, keyed by its operands: a repeat
-    // of the same operands reuses them instead of evaluating the network
-    // again.  Only C's value capacity depends on it and the kernels check
-    // that capacity, so a stale entry can cost a numeric re-run, never a
-    // wrong result.
+    // The summed predictions of the last few products, keyed by their
+    // operands: a repeat of the same operands reuses them instead of
+    // evaluating the network again (a chain of products repeats several).
+    // Only C's value capacity depends on them and the kernels check that
+    // capacity, so a stale entry can cost a numeric re-run, never a wrong
+    // result.
     struct Reuse {
         const void *a = nullptr, *b = nullptr;
         int64_t key[8] = {};
         int64_t pred_tiles = 0, pred_nnz = 0;
-        bool valid = false;
-    } reuse;
+        uint64_t used = 0;  // 0 = empty
+    };
+    static constexpr int NREUSE = 4;
+    Reuse reuse[NREUSE];
+    uint64_t tick = 0;
+    void forget() {
+        for (auto &r : reuse) r.used = 0;
+    }
 };
 
 static Learned *state(Context *ctx) {
@@ -358,9 +365,15 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     int syncs = 0, fallback = 0;
     int64_t cap = 0, pred_nnz = 0;
     PlanFlags fl{};
-    Learned::Reuse &ru = L->reuse;
     const int64_t key[8] = {A->nnz, A->numtile, B->nnz, B->numtile, A->tilem, B->tilen, A->dtype, pscale == 1.0};
-    const bool reuse = ru.valid && ru.a == A->tile_ptr && ru.b == B->tile_ptr && !memcmp(ru.key, key, sizeof key);
+    Learned::Reuse *hit = nullptr, *slot = &L->reuse[0];
+    for (auto &r : L->reuse) {
+        if (r.used && r.a == A->tile_ptr && r.b == B->tile_ptr && !memcmp(r.key, key, sizeof key)) hit = &r;
+        if (r.used < slot->used) slot = &r;  // empty or least recently used
+    }
+    const bool reuse = hit != nullptr;
+    Learned::Reuse &ru = reuse ? *hit : *slot;
+    ru.used = ++L->tick;
     if (timing && !rc) (void)hipEventRecord(ev[0], s);
     if (!rc) rc = palloc(ctx, &C->tile_ptr, (size_t)tilem + 1);
     if (reuse) {  // the plain Step-1 count; the predictions are this operand pair's last ones
@@ -422,7 +435,6 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
             memcpy(ru.key, key, sizeof key);
             ru.pred_tiles = (int64_t)fl.pred_tiles;
             ru.pred_nnz = pred_nnz;
-            ru.valid = true;
         }
     }
     if (!rc && fl.nnz <= cap && fl.cap) rc = fail(TSG_ERR_OVERFLOW, "a staged row exceeded its variant's capacity");
@@ -658,7 +670,7 @@ int tsg_learned_clear(tsg_context *c) {
     Learned *L = state(ctx);
     L->n = 0;
     L->mat_end.clear();
-    L->reuse.valid = false;
+    L->forget();
     return TSG_OK;
 }
 
@@ -718,7 +730,7 @@ int tsg_learned_train(tsg_context *c, int32_t epochs, int32_t batch, float lr, u
     if (rc) return rc;
     TSG_CHECK(calibrate(ctx));
     L->init = true;
-    L->reuse.valid = false;
+    L->forget();
     if (final_loss) *final_loss = (float)(epoch_loss / (double)n);
     return TSG_OK;
 }
@@ -732,7 +744,7 @@ int tsg_learned_calibration(tsg_context *c, float *get, const float *set) {
             return fail(TSG_ERR_ARG, "calibration offsets must lie in [0, 20]");
         L->off_nc = set[0];
         L->off_nnz = set[1];
-        L->reuse.valid = false;
+        L->forget();
         }
     if (get) {
         get[0] = L->off_nc;
@@ -764,7 +776,7 @@ int tsg_learned_set_weights(tsg_context *c, const float *w, int32_t n) {
     TSG_HIP(hipStreamSynchronize(ctx->stream));
     L->step = 0;
     L->init = true;
-    L->reuse.valid = false;
+    L->forget();
     return TSG_OK;
 }
 

@@ -589,17 +589,17 @@ static void *gus_worker(void *p) {
     return NULL;
 }
 
-int64_t ora_cpu_baseline(int32_t m, int32_t n, const int64_t *rpA, const int32_t *colA,
+int64_t ora_cpu_baseline(int32_t m, int32_t k, int32_t n, const int64_t *rpA, const int32_t *colA,
                          const double *valA, const int64_t *rpB, const int32_t *colB,
                          const double *valB, int32_t row_lo, int32_t row_hi, int32_t nthreads,
                          int32_t do_convert, double *conv_s, double *gus_s) {
     *conv_s = 0;
-    if (do_convert) {
+    if (do_convert) {  /* A is m x k, B is k x n */
         double t0 = now_s();
         ora_tile tA, tB;
-        ora_csr2tile(m, n, rpA, colA, valA, 0, &tA);
-        ora_csr2tile(n, n, rpB, colB, valB, 0, &tB);
-        ora_build_csc_tiles(&tB, n, n, rpB, colB, valB);
+        ora_csr2tile(m, k, rpA, colA, valA, 0, &tA);
+        ora_csr2tile(k, n, rpB, colB, valB, 0, &tB);
+        ora_build_csc_tiles(&tB, k, n, rpB, colB, valB);
         *conv_s = now_s() - t0;
         ora_tile_free(&tA); ora_tile_free(&tB);
     }

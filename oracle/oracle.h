@@ -102,7 +102,7 @@ int ora_gustavson_csr(int32_t m, int32_t n, const int64_t *rpA, const int32_t *c
  * product) for rows [row_lo,row_hi) of C = A*B with nthreads workers.
  * Returns elapsed seconds of the Gustavson part in *gus_s and of the two
  * conversions in *conv_s.  Products processed are returned. */
-int64_t ora_cpu_baseline(int32_t m, int32_t n, const int64_t *rpA, const int32_t *colA,
+int64_t ora_cpu_baseline(int32_t m, int32_t k, int32_t n, const int64_t *rpA, const int32_t *colA,
                          const double *valA, const int64_t *rpB, const int32_t *colB,
                          const double *valB, int32_t row_lo, int32_t row_hi, int32_t nthreads,
                          int32_t do_convert, double *conv_s, double *gus_s);

@@ -299,7 +299,7 @@ def cpu_baseline(A, B, row_lo, row_hi, nthreads=1, convert=False):
     ra, ca, va = _csr_args(A)
     rb, cb, vb = _csr_args(B)
     conv, gus = C.c_double(), C.c_double()
-    prod = lib().ora_cpu_baseline(A.m, B.n, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(va, C.c_double),
+    prod = lib().ora_cpu_baseline(A.m, A.n, B.n, _p(ra, C.c_int64), _p(ca, C.c_int32), _p(va, C.c_double),
                                   _p(rb, C.c_int64), _p(cb, C.c_int32), _p(vb, C.c_double),
                                   C.c_int32(row_lo), C.c_int32(row_hi), C.c_int32(nthreads),
                                   C.c_int32(1 if convert else 0), C.byref(conv), C.byref(gus))

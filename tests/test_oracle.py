@@ -446,3 +446,19 @@ def test_count_product_matches_tiled_steps_and_gustavson(case):
         rp, _, _ = O.gustavson_csr(A, B)
         assert tiles == Cd["numtile"]
         assert nnz == int(rp[-1]) == Cd["nnz"]
+
+
+def test_cpu_baseline_on_rectangular_products():
+    """The CPU baseline (conversion + Gustavson) on an m x k times k x n product
+    with m, k, n all different, as in the Galerkin chain's A P and P^T (A P):
+    its product count equals the flop count, with and without conversion and
+    on several threads."""
+    import gpu_spgemm_amd as g
+    A = sp.random(700, 1900, density=0.01, random_state=7, format="csr")
+    B = sp.random(1900, 260, density=0.02, random_state=8, format="csr")
+    A = g.HostCSR(700, 1900, A.indptr, A.indices, A.data)
+    B = g.HostCSR(1900, 260, B.indptr, B.indices, B.data)
+    f = O.flops(A, B)
+    for nt, conv in ((1, True), (3, True), (2, False)):
+        prod, gus, ct = O.cpu_baseline(A, B, 0, A.m, nthreads=nt, convert=conv)
+        assert prod == f and gus >= 0 and (ct > 0) == conv
