@@ -17,6 +17,9 @@ constexpr int MAX_BINS = MAX_VAR + 2;    // + C-tile kernels + row-by-row kernel
 // nvar+1 row-by-row.
 struct StageCaps {
     int32_t na, nc, nb, pa, pb;
+    // > 0: the products of a scalar row, estimated as (A entries per scalar
+    // row) x (B entries per reached B row) = (pa / 16) (pb / (16 na)), at most this
+    int32_t prow = 0;
 };
 struct BinTable {
     StageCaps v[MAX_VAR];
@@ -68,14 +71,18 @@ struct Plan {
 bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
 // The row-by-row kernels need the scalar CSR of A and B and no duplicate columns.
 bool rg_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
-// var_mask bit v admits staged variant v; rg_density 0 disables the row-by-row bin.
-BinTable bin_table(bool ok, bool rg = false, int rg_density = RG_DENSITY_DEFAULT, int var_mask = 7);
+// The row-lane numeric kernel (rownum.hip): needs both operands' scalar CSR and no repeated columns.
+bool num_rows_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B);
+// var_mask bits 0-2 admit staged variants S, W, M and bit 3 variant R (a lane
+// per scalar row; only when row_lanes); rg_density 0 disables the row-by-row bin.
+BinTable bin_table(bool ok, bool rg = false, int rg_density = RG_DENSITY_DEFAULT, int var_mask = 15,
+                   bool row_lanes = false);
 // The bin table of a product under its options and the context's tuned configuration.
 BinTable bin_table_for(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts);
 
 // Autotuned configuration (autotune.hip).
 struct TuneCfg {
-    int32_t var_mask = 7;
+    int32_t var_mask = 15;
     int32_t rg_density = RG_DENSITY_DEFAULT;
     int32_t mfma = 0;  // > 0: matrix-core C-tile kernels with this density gate
 };
@@ -109,6 +116,8 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
 // Set-up of run_numeric that needs C's scanned value offsets (call once after the scan).
 int prepare_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, Plan &p, const SlotLayout &sl);
 void free_c(Context *ctx, tsg_tile_matrix *C);
+int launch_num_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                    const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
 // keep != nullptr: the plan (row bins, links) is handed over instead of freed,
 // for re-running the numeric step.
 int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,

@@ -40,7 +40,13 @@ struct Variant {
 using VarS = Variant<8, 16, 32, 128, 1 << 20, 4>;
 using VarW = Variant<32, 16, 32, 512, 1 << 20, 2>;
 using VarM = Variant<32, 48, 64, 512, 1 << 20, 2>;
-constexpr int NUM_STAGED = 3;
+// R: rows of VarS whose scalar rows make about 32 products or fewer; the
+// symbolic step is VarS's, the numeric step a lane per scalar row (rownum.hip),
+// which wins on short rows (2-D 5-point stencils: 25) and loses on long ones
+// (3-D 7-point: 49).
+constexpr StageCaps CAPS_R{8, 16, 32, 128, 1 << 20, 32};
+constexpr int NUM_STAGED = 4;  // bins 0..3: R, S, W, M
+constexpr int64_t ROW_LANES_MIN = 32768;  // tile-rows below which R's rows take VarS's numeric kernel
 constexpr int DESC_CAP = 256;  // output descriptors of one chunk of (C tile, row) items
 
 template <typename Var>
@@ -239,7 +245,11 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
         int b = bt.nvar;
         for (int v = 0; v < bt.nvar; v++) {
             const StageCaps c = bt.v[v];
-            if (na <= c.na && count <= c.nc && nb <= c.nb && pa <= c.pa && pb <= c.pb) { b = v; break; }
+            if (na <= c.na && count <= c.nc && nb <= c.nb && pa <= c.pa && pb <= c.pb &&
+                (c.prow <= 0 || pa * pb <= (int64_t)c.prow * 256 * na)) {
+                b = v;
+                break;
+            }
         }
         if (b == bt.nvar && bt.rg && pa <= (int64_t)bt.rg_density * na) b = bt.nvar + 1;
         bin[bi] = (uint8_t)b;
@@ -731,14 +741,16 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
 }
 
 // ---------------------------------------------------------------- host side
+static bool binned_variant(const tsg_options *opts) { return !opts || opts->numeric_variant == 0; }
+
 bool staged_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
-    if (opts && opts->numeric_variant != 0) return false;  // C-tile kernels everywhere
+    if (!binned_variant(opts)) return false;  // C-tile kernels everywhere
     return B->csr_tile_nnz && B->csr_mask && B->csr_col_mask && B->csr_col_Ptr && B->csr_col_Val && B->row_nnz_ptr &&
            !(A->flags & 1) && !(B->flags & 1);
 }
 
 bool rg_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options *opts) {
-    if (opts && opts->numeric_variant != 0) return false;  // C-tile kernels everywhere
+    if (!binned_variant(opts)) return false;  // C-tile kernels everywhere
     return A->row_ptr && A->row_col && A->row_val && B->row_ptr && B->row_col && B->row_val && !(A->flags & 1) &&
            !(B->flags & 1);
 }
@@ -749,20 +761,22 @@ BinTable bin_table_for(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
     // the row-by-row bin needs a per-workgroup cursor area far larger than the
     // no-alloc scratch: inside such a call heavy rows take the C-tile kernels
     // (every path computes the same bits)
-    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts) && !ctx->arena, cfg.rg_density, cfg.var_mask);
+    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts) && !ctx->arena, cfg.rg_density, cfg.var_mask,
+                     num_rows_ok(A, B));
 }
 
-BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask) {
+BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask, bool row_lanes) {
     BinTable t{};
     t.nvar = 0;
     t.rg = (rg && rg_density > 0) ? 1 : 0;
     t.rg_density = rg_density;
     if (ok) {
-        t.v[0] = caps_of<VarS>();
-        t.v[1] = caps_of<VarW>();
-        t.v[2] = caps_of<VarM>();
-        for (int v = 0; v < NUM_STAGED; v++)  // a variant left out takes no row
-            if (!(var_mask & (1 << v))) t.v[v] = StageCaps{-1, -1, -1, -1, -1};
+        // bins in first-fit order R, S, W, M; var_mask bits 0-2 = S, W, M, bit 3 = R
+        const StageCaps none{-1, -1, -1, -1, -1};
+        t.v[0] = (row_lanes && (var_mask & 8)) ? CAPS_R : none;
+        t.v[1] = (var_mask & 1) ? caps_of<VarS>() : none;
+        t.v[2] = (var_mask & 2) ? caps_of<VarW>() : none;
+        t.v[3] = (var_mask & 4) ? caps_of<VarM>() : none;
         t.nvar = NUM_STAGED;
     }
     return t;
@@ -888,10 +902,12 @@ static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matr
     return plan_read(ctx, p, &C->numtile, nullptr);
 }
 
+// Staged kernels over the rows of bins [bin, bin_end) (adjacent in the row list).
 template <typename Var>
 static int launch_sym_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
-                             const tsg_tile_matrix *C, const Plan &p, int bin, const SlotLayout &sl) {
-    const int64_t n = p.bin_off[bin + 1] - p.bin_off[bin];
+                             const tsg_tile_matrix *C, const Plan &p, int bin, const SlotLayout &sl,
+                             int bin_end = -1) {
+    const int64_t n = p.bin_off[bin_end < 0 ? bin + 1 : bin_end] - p.bin_off[bin];
     if (n <= 0) return TSG_OK;
     k_sym_staged<Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
         p.rows, p.bin_off[bin], n, A->tile_ptr, A->mask, p.link, B->tile_columnidx, B->csr_mask, C->tile_ptr,
@@ -902,8 +918,9 @@ static int launch_sym_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
 
 template <typename V, typename Var>
 static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
-                             const tsg_tile_matrix *C, const Plan &p, int bin, const SlotLayout &sl) {
-    const int64_t n = p.bin_off[bin + 1] - p.bin_off[bin];
+                             const tsg_tile_matrix *C, const Plan &p, int bin, const SlotLayout &sl,
+                             int bin_end = -1) {
+    const int64_t n = p.bin_off[bin_end < 0 ? bin + 1 : bin_end] - p.bin_off[bin];
     if (n <= 0) return TSG_OK;
     k_num_staged<V, Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
         p.rows, p.bin_off[bin], n, A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
@@ -919,9 +936,9 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
                  const Plan &p, const SlotLayout &sl) {
     const int fb = p.nvar;  // C-tile kernels
     if (p.nvar > 0) {
-        TSG_CHECK(launch_sym_staged<VarS>(ctx, A, B, C, p, 0, sl));
-        TSG_CHECK(launch_sym_staged<VarW>(ctx, A, B, C, p, 1, sl));
-        TSG_CHECK(launch_sym_staged<VarM>(ctx, A, B, C, p, 2, sl));
+        TSG_CHECK(launch_sym_staged<VarS>(ctx, A, B, C, p, 0, sl, 2));  // R's rows fit VarS's symbolic caps
+        TSG_CHECK(launch_sym_staged<VarW>(ctx, A, B, C, p, 2, sl));
+        TSG_CHECK(launch_sym_staged<VarM>(ctx, A, B, C, p, 3, sl));
     }
     const int64_t nf = p.bin_off[fb + 1] - p.bin_off[fb];
     TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb], nf, sl, p.hb));
@@ -938,14 +955,19 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
                 const Plan &p, const SlotLayout &sl) {
     const int fb = p.nvar;
     if (p.nvar > 0) {
+        // R's numeric kernel pays for its own launch only on many rows; fewer
+        // go to VarS's with S's rows (they fit its caps)
+        const bool rn = p.bin_off[1] - p.bin_off[0] >= ROW_LANES_MIN;
+        if (rn) TSG_CHECK(launch_num_rows(ctx, A, B, C, p.rows, p.bin_off[0], p.bin_off[1] - p.bin_off[0], sl));
+        const int s0 = rn ? 1 : 0;
         if (A->dtype == TSG_F32) {
-            TSG_CHECK((launch_num_staged<float, VarS>(ctx, A, B, C, p, 0, sl)));
-            TSG_CHECK((launch_num_staged<float, VarW>(ctx, A, B, C, p, 1, sl)));
-            TSG_CHECK((launch_num_staged<float, VarM>(ctx, A, B, C, p, 2, sl)));
+            TSG_CHECK((launch_num_staged<float, VarS>(ctx, A, B, C, p, s0, sl, 2)));
+            TSG_CHECK((launch_num_staged<float, VarW>(ctx, A, B, C, p, 2, sl)));
+            TSG_CHECK((launch_num_staged<float, VarM>(ctx, A, B, C, p, 3, sl)));
         } else {
-            TSG_CHECK((launch_num_staged<double, VarS>(ctx, A, B, C, p, 0, sl)));
-            TSG_CHECK((launch_num_staged<double, VarW>(ctx, A, B, C, p, 1, sl)));
-            TSG_CHECK((launch_num_staged<double, VarM>(ctx, A, B, C, p, 2, sl)));
+            TSG_CHECK((launch_num_staged<double, VarS>(ctx, A, B, C, p, s0, sl, 2)));
+            TSG_CHECK((launch_num_staged<double, VarW>(ctx, A, B, C, p, 2, sl)));
+            TSG_CHECK((launch_num_staged<double, VarM>(ctx, A, B, C, p, 3, sl)));
         }
     }
     TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, p.bin_off[fb], p.bin_off[fb + 1] - p.bin_off[fb], sl));

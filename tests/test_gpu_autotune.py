@@ -17,7 +17,7 @@ def mixed():
     return g.gen_rmat(11, 16, seed=4, dtype=g.F64)
 
 
-@pytest.mark.parametrize("var_mask", [7, 5, 1, 4, 2, 0])
+@pytest.mark.parametrize("var_mask", [15, 8, 9, 7, 5, 1, 4, 2, 0])
 @pytest.mark.parametrize("rg_density", [0, 2, 8, 1000])
 def test_every_configuration_gives_identical_bits(ctx, var_mask, rg_density):
     A = mixed()
@@ -38,7 +38,7 @@ def test_autotune_picks_the_fastest_and_stores_it(ctx):
     tA, tB = ctx.tiles_from_csr(A), ctx.tiles_from_csr(A, as_b=True)
     best, rep = ctx.autotune(tA, tB, reps=2)
     print("    report:", [(r["var_mask"], r["rg_density"], round(r["ms"], 3)) for r in rep], flush=True)
-    assert len(rep) == 22
+    assert len(rep) == 30   # 7 staged-variant masks x 4 row-by-row densities + 2 matrix-core gates
     assert best["ms"] == min(r["ms"] for r in rep)
     q = ctx.autotune_query(tA, tB)
     assert q["var_mask"] == best["var_mask"] and q["rg_density"] == best["rg_density"]
@@ -79,7 +79,7 @@ def test_autotune_batch(ctx):
     tA = [ctx.tiles_from_csr(a) for a in ps]
     tB = [ctx.tiles_from_csr(a, as_b=True) for a in ps]
     best, rep = ctx.autotune(tA, tB, reps=1)
-    assert len(rep) == 22 and ctx.autotune_query(tA, tB) is not None
+    assert len(rep) == 30 and ctx.autotune_query(tA, tB) is not None
     Cs, _ = ctx.spgemm_batched(tA, tB)
     for i in range(len(ps)):
         C1, _ = ctx.spgemm(tA[i], tB[i], variant=1)
@@ -99,7 +99,7 @@ def test_autotune_large_batch_samples_then_times_finalists(ctx):
     full = [r for r in rep if r["sampled"] == 0]
     print("    finalists:", [(r["var_mask"], r["rg_density"], r["mfma_min_products"], round(r["ms"], 3)) for r in full],
           flush=True)
-    assert len(rep) == 22 and len(full) == 3
+    assert len(rep) == 30 and len(full) == 3
     assert best["sampled"] == 0 and best["ms"] == min(r["ms"] for r in full)
     q = ctx.autotune_query(tA, tB)
     assert q is not None and q["var_mask"] == best["var_mask"] and q["rg_density"] == best["rg_density"]

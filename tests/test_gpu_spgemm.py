@@ -22,6 +22,8 @@ TOL = {g.F64: 1e-12, g.F32: 1e-5}
 # kernels); 1: the reference's C-tile method for every tile; 3: the C-tile
 # method in outer-product form over A's within-tile CSC
 VARIANTS = [0, 1, 3]
+CKEYS = ["tile_ptr", "tile_columnidx", "tile_rowidx", "tile_nnz", "tile_csr_Ptr", "mask", "tile_csr_Col",
+         "tile_csr_Val"]
 
 
 def run(ctx, A, B, f32=False, label="", **kw):
@@ -279,3 +281,43 @@ def test_heavy_rows_paths_agree(ctx, name, f32):
         assert chk["max_scaled"] <= TOL[g.F32 if f32 else g.F64]
     else:
         check(got0, A, B, f32=f32)
+
+
+def _short_rows_wide_b(n=4096, seed=3):
+    """A tridiagonal (3 A entries a scalar row: the row-lane variant's rows)
+    times a B whose rows hold 40 columns within +-60: C tile-rows of about 1,600
+    values, so the row-lane kernel runs each in several LDS windows."""
+    rng = np.random.default_rng(seed)
+    i = np.arange(n)
+    ca = np.stack([i - 1, i, i + 1], 1)
+    keep = (ca >= 0) & (ca < n)
+    rpa = np.concatenate([[0], np.cumsum(keep.sum(1))]).astype(np.int64)
+    A = g.HostCSR(n, n, rpa, ca[keep].astype(np.int32), rng.uniform(-1, 1, int(rpa[-1])))
+    cols = []
+    for k in range(n):
+        lo, hi = max(0, k - 60), min(n, k + 61)
+        cols.append(np.sort(rng.choice(np.arange(lo, hi), min(40, hi - lo), replace=False)))
+    rpb = np.concatenate([[0], np.cumsum([len(c) for c in cols])]).astype(np.int64)
+    B = g.HostCSR(n, n, rpb, np.concatenate(cols).astype(np.int32), rng.uniform(-1, 1, int(rpb[-1])))
+    return A, B
+
+
+@pytest.mark.parametrize("var_mask", [None, 8])
+@pytest.mark.parametrize("f32", [False, True])
+def test_row_lane_variant_with_windows(ctx, var_mask, f32):
+    """Staged variant R (a lane per scalar row, C values staged in LDS windows)
+    on tile-rows needing several windows, bit for bit against the tiled
+    restatement; var_mask 8 leaves R as the only staged variant."""
+    A, B = _short_rows_wide_b()
+    print(f"[case] row-lane windows var_mask={var_mask} f32={f32}", flush=True)
+    H, HB = (A.astype(g.F32), B.astype(g.F32)) if f32 else (A, B)
+    tA, tB = ctx.tiles_from_csr(H), ctx.tiles_from_csr(HB, as_b=True)
+    if var_mask is not None:
+        ctx.autotune_set(tA, tB, var_mask, 64)
+    try:
+        C, st = ctx.spgemm(tA, tB)
+        got = C.download()
+    finally:
+        ctx.autotune_clear()
+    assert st["nnzC"] / max(1, tA.s.tilem) > 1000      # several 256-value windows a tile-row
+    assert_tiles_equal(got, O.spgemm_tiled(A, B, f32=f32), keys=CKEYS, label="C ")
