@@ -173,8 +173,16 @@ def bench_batch(g, args):
         f"rg_density {best['rg_density']} {best['ms']:.1f} ms; " +
         " ".join(f"{r['var_mask']}/{r['rg_density']}:{r['ms']:.0f}{'*' if r['sampled'] else ''}" for r in rep) +
         " (* = timed on every 7th problem only)")
+    G = ctx.batch_graph(tA, tB, slack=1.25)
+    fallbacks = []
+
+    def graph():  # one graph launch, then the wait for its result row
+        G.launch()
+        _, sts = G.result()
+        fallbacks.append(sts[0]["fallback_used"])
+
     res = {}
-    for label, fn in (("loop", loop), ("batched", batched)):
+    for label, fn in (("loop", loop), ("batched", batched), ("graph", graph)):
         for _ in range(args.warmup):
             fn()
         ctx.synchronize()
@@ -195,6 +203,7 @@ def bench_batch(g, args):
     res["graph_replay_numeric"] = (time.perf_counter() - t1) * 1e3 / args.steps
     log(f"graph replay (values only): {res['graph_replay_numeric']:.3f} ms per batch")
     plan.destroy()
+    G.destroy()
     Cs, sts = ctx.spgemm_batched(tA, tB)
     pipe = sum(alg_bytes(tile_sizes(a), tile_sizes(b), tile_sizes(c), 8) for a, b, c in zip(tA, tB, Cs))
     for C in Cs:
@@ -210,6 +219,8 @@ def bench_batch(g, args):
                    "seed": 4, "scheduler": "batched", "parallelism": "single GPU"},
         "variants_ms_per_step": {k: round(v, 4) for k, v in res.items()},
         "batched_speedup_vs_loop": round(res["loop"] / res["batched"], 3),
+        "graph_speedup_vs_loop": round(res["loop"] / res["graph"], 3),
+        "graph_fallbacks": int(sum(fallbacks)),
         "autotuned": {"var_mask": best["var_mask"], "rg_density": best["rg_density"],
                       "candidates": len(rep)},
         # the batch runs dozens of kernels: the bound is the whole call's algorithmic bytes
@@ -533,8 +544,16 @@ def batch_leg(g, ctx, args, count=64, hi=16384):
         for C in Cs:
             C.free()
 
+    G = ctx.batch_graph(tA, tB, slack=1.25)
+    fallbacks = []
+
+    def graph():  # one graph launch, then the wait for its result row
+        G.launch()
+        _, sts = G.result()
+        fallbacks.append(sts[0]["fallback_used"])
+
     res = {}
-    for label, fn in (("loop", loop), ("batched", batched)):
+    for label, fn in (("loop", loop), ("batched", batched), ("graph", graph)):
         for _ in range(args.warmup):
             fn()
         ctx.synchronize()
@@ -543,13 +562,20 @@ def batch_leg(g, ctx, args, count=64, hi=16384):
             fn()
         ctx.synchronize()
         res[label] = (time.perf_counter() - t1) * 1e3 / args.steps
-    log(f"batch of {count} (n_i <= {hi}): loop {res['loop']:.3f} ms, batched {res['batched']:.3f} ms")
+    log(f"batch of {count} (n_i <= {hi}): loop {res['loop']:.3f} ms, batched {res['batched']:.3f} ms, "
+        f"graph {res['graph']:.3f} ms")
+    G.destroy()
     for t in tA + tB:
         t.free()
     return {"problems": count, "max_rows": hi, "seed": 4, "products": products,
             "ms_loop": round(res["loop"], 4), "ms_batched": round(res["batched"], 4),
+            "ms_graph": round(res["graph"], 4),
             "vs_loop": round(res["loop"] / res["batched"], 3),
-            "batched_gflops": round(2.0 * products / (res["batched"] * 1e6), 3)}
+            "graph_vs_loop": round(res["loop"] / res["graph"], 3),
+            "graph_vs_batched": round(res["batched"] / res["graph"], 3),
+            "graph_fallbacks": int(sum(fallbacks)),
+            "batched_gflops": round(2.0 * products / (res["batched"] * 1e6), 3),
+            "graph_gflops": round(2.0 * products / (res["graph"] * 1e6), 3)}
 
 
 if __name__ == "__main__":

@@ -320,6 +320,10 @@ class Context:
         _check(lib().tsg_memcpy(self._p, out.ctypes.data_as(C.c_void_p), C.c_void_p(ptr), C.c_size_t(out.nbytes)))
         return out
 
+    def copy(self, dst, src, nbytes):
+        """Copies nbytes between two addresses (device or host; blocking)."""
+        _check(lib().tsg_memcpy(self._p, C.c_void_p(dst), C.c_void_p(src), C.c_size_t(nbytes)))
+
     def pool_trim(self):
         """Returns the pool's cached (freed) blocks to the device."""
         _check(lib().tsg_pool_trim(self._p))
@@ -428,6 +432,10 @@ class Context:
                                         bmA.ctypes.data_as(C.POINTER(C.c_uint32)),
                                         bmB.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(bl)))
         return bmA, bmB
+
+    def batch_graph(self, As, Bs, slack=1.25, tuning=0):
+        """One captured hipGraph for the whole batch product (tsg_batch_graph_*)."""
+        return BatchGraph(self, As, Bs, slack, Options(SCHED_PLAIN, 0, 0, 0, 1.0, tuning))
 
     def plan(self, As, Bs, variant=0, mfma_min_products=0, tuning=0):
         """Product (or batch, for lists) plus a plan that recomputes the values
@@ -575,6 +583,54 @@ class Plan:
             pass
 
 
+class BatchGraph:
+    """tsg_batch_graph: launch() queues one graph launch of the whole batch
+    product; result() waits for it and returns ([C_i views], [stats_i]).  The
+    views are valid until the next launch or destroy()."""
+
+    def __init__(self, ctx, As, Bs, slack, opts):
+        self.ctx = ctx
+        n = len(As)
+        assert len(Bs) == n
+        self.n = n
+        a = (TileStruct * n)(*[x.s for x in As])
+        b = (TileStruct * n)(*[x.s for x in Bs])
+        self._p = C.c_void_p()
+        _check(lib().tsg_batch_graph_create(ctx.handle, C.c_int32(n), a, b, C.byref(opts), C.c_double(slack),
+                                            C.byref(self._p)))
+        self._keep = (As, Bs)
+
+    def launch(self):
+        _check(lib().tsg_batch_graph_launch(self._p))
+
+    def result(self):
+        c = (TileStruct * self.n)()
+        st = (Stats * self.n)()
+        _check(lib().tsg_batch_graph_result(self._p, c, st))
+        out = []
+        for i in range(self.n):
+            t = TileView(self.ctx)
+            t.s = TileStruct.from_buffer_copy(c[i])
+            out.append(t)
+        return out, [st[i].as_dict() for i in range(self.n)]
+
+    def capacity(self):
+        t, z = C.c_int64(), C.c_int64()
+        _check(lib().tsg_batch_graph_capacity(self._p, C.byref(t), C.byref(z)))
+        return t.value, z.value
+
+    def destroy(self):
+        if self._p:
+            lib().tsg_batch_graph_destroy(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class DeviceCSR:
     def __init__(self, ctx, h):
         self.ctx = ctx
@@ -686,6 +742,13 @@ class DeviceTiles:
                 self.free()
         except Exception:
             pass
+
+
+class TileView(DeviceTiles):
+    """A tile matrix whose arrays another object owns (a batch graph's outputs)."""
+
+    def free(self):
+        pass
 
 
 def upload_tiles(ctx, d):

@@ -87,6 +87,10 @@ int pool_view(Context *ctx, void *block, size_t offset, void **out) {
 int pool_free(Context *ctx, void *p) {
     if (!p || ctx->arena) return TSG_OK;  // arena arrays (and the caller's C) are not the pool's
     std::lock_guard<std::mutex> lk(ctx->mu);
+    if (ctx->deferred) {
+        ctx->deferred->push_back(p);
+        return TSG_OK;
+    }
     // a view, or the owner of a block with views: drop one reference
     void *blk = nullptr;
     auto v = ctx->pool.views.find(p);

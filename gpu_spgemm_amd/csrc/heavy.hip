@@ -652,8 +652,10 @@ static unsigned rgw_grid(Context *ctx, int per_cu) {
 }
 
 int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                       const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const SlotLayout &sl) {
+                       const int32_t *rows, const BinRange &br, int64_t *woff, const SlotLayout &sl) {
+    const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
+    if (br.lo) return fail(TSG_ERR_ARG, "the row-by-row kernels need host-known bin ranges");
     unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
     k_rgw_count<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, C->tile_ptr,
                                                                          sl.row_nc, C->tile_columnidx, woff);
@@ -670,8 +672,10 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
 static int64_t rgn_gcap(const tsg_tile_matrix *A) { return std::min<int64_t>(RGN_GCUR, A->nnz); }
 
 int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, const int32_t *rows,
-                       int64_t begin, int64_t count, int64_t *woffn, int64_t **gcur, const SlotLayout &sl) {
+                       const BinRange &br, int64_t *woffn, int64_t **gcur, const SlotLayout &sl) {
+    const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
+    if (br.lo) return fail(TSG_ERR_ARG, "the row-by-row kernels need host-known bin ranges");
     if (!*gcur && rgn_gcap(A) > 0)
         TSG_CHECK(palloc(ctx, gcur, (size_t)rgw_grid(ctx, 3) * (size_t)rgn_gcap(A) * 2));
     const int64_t h = C->dtype == TSG_F32 ? rgw_item_values<float>() : rgw_item_values<double>();
@@ -682,9 +686,11 @@ int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
 }
 
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                      const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const int64_t *woffn,
+                      const int32_t *rows, const BinRange &br, int64_t *woff, const int64_t *woffn,
                       int64_t *gcur, const SlotLayout &sl) {
+    const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
+    if (br.lo) return fail(TSG_ERR_ARG, "the row-by-row kernels need host-known bin ranges");
     const int64_t gcap = gcur ? rgn_gcap(A) : 0;
     unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
     k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);

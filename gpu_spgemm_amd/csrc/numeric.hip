@@ -228,8 +228,11 @@ __global__ void __launch_bounds__(NU_THREADS) k_numeric_tiles(int64_t numtileC, 
 // row's C tiles.
 template <typename V, bool DUPS>
 __global__ void __launch_bounds__(NU_THREADS) k_numeric_list(const int32_t *__restrict__ rows, int64_t row_begin,
+                                                             int64_t nrows, const int64_t *__restrict__ dlo,
+                                                             const int64_t *__restrict__ dhi, int64_t c0,
                                                              NumArgs<V> p) {
-    const int32_t bi = rows[row_begin + blockIdx.x];
+    const int32_t bi = bin_row(rows, row_begin, nrows, dlo, dhi, c0 + blockIdx.x);
+    if (bi < 0) return;
     int64_t cs, ce;
     c_row_range(p.tile_ptrC, p.row_nc, bi, cs, ce);
     for (int64_t ti = cs + (threadIdx.x >> 4); ti < ce; ti += NU_GROUPS) num_tile<V, DUPS>(ti, p);
@@ -237,7 +240,8 @@ __global__ void __launch_bounds__(NU_THREADS) k_numeric_list(const int32_t *__re
 
 template <typename V>
 static int launch_list_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                         const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
+                         const int32_t *rows, const BinRange &br, const SlotLayout &sl) {
+    const int64_t count = br.count;
     NumArgs<V> a = num_args<V>(A, B, C);
     a.row_nc = sl.row_nc;
     a.overflow = sl.overflow;
@@ -246,9 +250,9 @@ static int launch_list_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
     for (int64_t c0 = 0; c0 < count; c0 += ROW_CHUNK) {  // work-items per launch stay below 2^32
         const unsigned n = (unsigned)(count - c0 < ROW_CHUNK ? count - c0 : ROW_CHUNK);
         if ((A->flags | B->flags) & 1)
-            k_numeric_list<V, true><<<n, NU_THREADS, 0, ctx->stream>>>(rows, begin + c0, a);
+            k_numeric_list<V, true><<<n, NU_THREADS, 0, ctx->stream>>>(rows, br.begin, count, br.lo, br.hi, c0, a);
         else
-            k_numeric_list<V, false><<<n, NU_THREADS, 0, ctx->stream>>>(rows, begin + c0, a);
+            k_numeric_list<V, false><<<n, NU_THREADS, 0, ctx->stream>>>(rows, br.begin, count, br.lo, br.hi, c0, a);
         TSG_HIP(hipGetLastError());
     }
     return TSG_OK;
@@ -277,10 +281,10 @@ static int launch_tiles_t(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
 }
 
 int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                        const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
-    if (count <= 0) return TSG_OK;
-    if (A->dtype == TSG_F32) return launch_list_t<float>(ctx, A, B, C, rows, begin, count, sl);
-    return launch_list_t<double>(ctx, A, B, C, rows, begin, count, sl);
+                        const int32_t *rows, const BinRange &br, const SlotLayout &sl) {
+    if (br.count <= 0) return TSG_OK;
+    if (A->dtype == TSG_F32) return launch_list_t<float>(ctx, A, B, C, rows, br, sl);
+    return launch_list_t<double>(ctx, A, B, C, rows, br, sl);
 }
 
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,

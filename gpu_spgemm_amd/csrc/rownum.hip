@@ -46,7 +46,8 @@ __device__ __forceinline__ float fma_rn<float>(float a, float b, float c) { retu
 
 template <typename V>
 __global__ void __launch_bounds__(WAVE)
-k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, int32_t m,
+k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, const int64_t *__restrict__ dlo,
+           const int64_t *__restrict__ dhi, int32_t m,
            const int64_t *__restrict__ rpA, const int32_t *__restrict__ colA, const V *__restrict__ valA,
            const int64_t *__restrict__ rpB, const int32_t *__restrict__ colB, const V *__restrict__ valB,
            const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ colidxC,
@@ -58,10 +59,9 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, i
     const int g = lane >> 4, r = lane & 15;
     const int64_t idx = (int64_t)blockIdx.x * RN_GROUPS + g;
     // this group's tile-row and C tile range (uniform in the group)
-    int32_t bi = -1;
+    const int32_t bi = bin_row(rows, row_begin, nrows, dlo, dhi, idx);
     int64_t cs = 0, ce = 0;
-    if (idx < nrows) {
-        bi = rows[row_begin + idx];
+    if (bi >= 0) {
         cs = tile_ptrC[bi];
         ce = tile_ptrC[bi + 1];
     }
@@ -200,18 +200,19 @@ bool num_rows_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B) {
 }
 
 int launch_num_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                    const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl) {
+                    const int32_t *rows, const BinRange &br, const SlotLayout &sl) {
+    const int64_t count = br.count;
     if (count <= 0) return TSG_OK;
     if (sl.row_nc) return fail(TSG_ERR_ARG, "the row-lane numeric kernel runs on the exact C layout only");
     const unsigned grid = (unsigned)((count + RN_GROUPS - 1) / RN_GROUPS);
     if (A->dtype == TSG_F32)
         k_num_rows<float><<<grid, WAVE, 0, ctx->stream>>>(
-            rows, begin, count, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr, B->row_col,
+            rows, br.begin, count, br.lo, br.hi, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr, B->row_col,
             (const float *)B->row_val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr,
             (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done);
     else
         k_num_rows<double><<<grid, WAVE, 0, ctx->stream>>>(
-            rows, begin, count, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr, B->row_col,
+            rows, br.begin, count, br.lo, br.hi, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr, B->row_col,
             (const double *)B->row_val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr,
             (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done);
     TSG_HIP(hipGetLastError());

@@ -64,6 +64,11 @@ struct Plan {
     int nbins = 0;
     int nvar = 0;     // staged bins; bin nvar = C-tile kernels
     bool rg = false;  // bin nvar + 1 = row-by-row kernels
+    // A captured graph: the bin offsets (bin_off) live only on the device, at
+    // d_bins[0..nbins]; the launches for bin b cover bin_cap[b] rows and take
+    // the bin's range from there.  Null: host-known bins.
+    const int64_t *d_bins = nullptr;
+    int64_t bin_cap[MAX_BINS] = {};
     // device address of the capacity flag (kernels take it as int32_t *)
     int32_t *overflow() const { return flags ? &flags->cap : nullptr; }
 };
@@ -117,7 +122,23 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
 int prepare_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, Plan &p, const SlotLayout &sl);
 void free_c(Context *ctx, tsg_tile_matrix *C);
 int launch_num_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                    const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
+                    const int32_t *rows, const BinRange &br, const SlotLayout &sl);
+BinRange bin_range(const Plan &p, int b, int e);
+
+// Capacities of a device-sized product (a captured graph): C's tile and value
+// arrays and the rows each bin's launches cover.
+struct DeviceCaps {
+    int64_t tiles = 0, nnz = 0;
+    int64_t bins[MAX_BINS] = {};
+};
+// The binned pipeline with no host reads (Context::device_sizes must be set):
+// sizes stay on the device, launches cover `caps`, C's arrays are allocated
+// at the capacities (C->numtile, C->nnz hold them), and a product past any
+// capacity sets p.overflow() and leaves C undefined.  numtileC ends in
+// p.hdr[0] and nnzC in p.flags->nnz.  The plan and C stay allocated.
+int spgemm_binned_device(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                         const tsg_options *opts, const DeviceCaps &caps, Plan &p);
+
 // keep != nullptr: the plan (row bins, links) is handed over instead of freed,
 // for re-running the numeric step.
 int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,

@@ -376,8 +376,8 @@ struct SymLDS {
 // tile columns come from one wave-wide bitonic sort of the B tiles' columns.
 template <typename Var>
 __global__ void __launch_bounds__(Var::WAVES *WAVE)
-k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
-             const int64_t *__restrict__ tile_ptrA, const uint16_t *__restrict__ maskA,
+k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, const int64_t *__restrict__ dlo,
+             const int64_t *__restrict__ dhi, const int64_t *__restrict__ tile_ptrA, const uint16_t *__restrict__ maskA,
              const int4 *__restrict__ link, const int32_t *__restrict__ colidxB,
              const uint16_t *__restrict__ csr_maskB, const int64_t *__restrict__ tile_ptrC,
              int32_t *__restrict__ rowidxC, int32_t *__restrict__ colidxC, uint16_t *__restrict__ maskC,
@@ -387,6 +387,7 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     __shared__ SymLDS<Var> lds[Var::WAVES];
     const int w = threadIdx.x / WAVE, lane = lane_id();
     const int64_t idx = (int64_t)blockIdx.x * Var::WAVES + w;
+    bin_rows(dlo, dhi, row_begin, nrows);
     if (idx >= nrows) return;
     SymLDS<Var> &s = lds[w];
     const int32_t bi = rows[row_begin + idx];
@@ -546,8 +547,9 @@ __device__ __forceinline__ float fma_v<float>(float a, float b, float c) { retur
 // (A row mask r) & (B column mask c): the loop runs once per product.
 template <typename V, typename Var>
 __global__ void __launch_bounds__(Var::WAVES *WAVE)
-k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
-             const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA,
+k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, const int64_t *__restrict__ dlo,
+             const int64_t *__restrict__ dhi, const int64_t *__restrict__ tile_ptrA,
+             const int64_t *__restrict__ tile_nnzA,
              const uint8_t *__restrict__ ptrA, const uint16_t *__restrict__ maskA, const V *__restrict__ valA,
              const int4 *__restrict__ link, const int32_t *__restrict__ colidxB,
              const int64_t *__restrict__ csr_nnzB, const uint8_t *__restrict__ cptrB,
@@ -561,6 +563,7 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     __shared__ NumLDS<V, Var> lds[Var::WAVES];
     const int w = threadIdx.x / WAVE, lane = lane_id();
     const int64_t idx = (int64_t)blockIdx.x * Var::WAVES + w;
+    bin_rows(dlo, dhi, row_begin, nrows);
     if (idx >= nrows) return;
     NumLDS<V, Var> &s = lds[w];
     const int32_t bi = rows[row_begin + idx];
@@ -761,7 +764,7 @@ BinTable bin_table_for(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
     // the row-by-row bin needs a per-workgroup cursor area far larger than the
     // no-alloc scratch: inside such a call heavy rows take the C-tile kernels
     // (every path computes the same bits)
-    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts) && !ctx->arena, cfg.rg_density, cfg.var_mask,
+    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts) && !ctx->arena && !ctx->device_sizes, cfg.rg_density, cfg.var_mask,
                      num_rows_ok(A, B));
 }
 
@@ -872,7 +875,7 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
         int rc = no_heavy ? TSG_OK : palloc(ctx, &hlist, (size_t)tilem);
         // keep union bitmaps only where long tile-rows are known (without them
         // the fill just rebuilds every bitmap)
-        if (!rc && !ctx->arena && B->numtile > HEAVY_NB && ((A->flags | B->flags) & 2)) {
+        if (!rc && !ctx->arena && !ctx->device_sizes && B->numtile > HEAVY_NB && ((A->flags | B->flags) & 2)) {
             p.hb.cap = std::min<int64_t>(HEAVY_BITMAP_WORDS, (int64_t)tilem * ((int64_t)B->tilen / 32 + 1));
             rc = palloc(ctx, &p.hb.words, (size_t)p.hb.cap);
             if (!rc) rc = palloc(ctx, &p.hb.off, (size_t)tilem);
@@ -907,10 +910,11 @@ template <typename Var>
 static int launch_sym_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                              const tsg_tile_matrix *C, const Plan &p, int bin, const SlotLayout &sl,
                              int bin_end = -1) {
-    const int64_t n = p.bin_off[bin_end < 0 ? bin + 1 : bin_end] - p.bin_off[bin];
-    if (n <= 0) return TSG_OK;
-    k_sym_staged<Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
-        p.rows, p.bin_off[bin], n, A->tile_ptr, A->mask, p.link, B->tile_columnidx, B->csr_mask, C->tile_ptr,
+    const BinRange br = bin_range(p, bin, bin_end < 0 ? bin + 1 : bin_end);
+    if (br.count <= 0) return TSG_OK;
+    k_sym_staged<Var><<<nblk(br.count, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
+        p.rows, br.begin, br.count, br.lo, br.hi,
+        A->tile_ptr, A->mask, p.link, B->tile_columnidx, B->csr_mask, C->tile_ptr,
         C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, p.overflow(), sl.row_nc);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
@@ -920,10 +924,11 @@ template <typename V, typename Var>
 static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                              const tsg_tile_matrix *C, const Plan &p, int bin, const SlotLayout &sl,
                              int bin_end = -1) {
-    const int64_t n = p.bin_off[bin_end < 0 ? bin + 1 : bin_end] - p.bin_off[bin];
-    if (n <= 0) return TSG_OK;
-    k_num_staged<V, Var><<<nblk(n, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
-        p.rows, p.bin_off[bin], n, A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
+    const BinRange br = bin_range(p, bin, bin_end < 0 ? bin + 1 : bin_end);
+    if (br.count <= 0) return TSG_OK;
+    k_num_staged<V, Var><<<nblk(br.count, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
+        p.rows, br.begin, br.count, br.lo, br.hi,
+        A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
         p.link, B->tile_columnidx, B->csr_tile_nnz, B->csr_col_Ptr, B->csr_col_mask,
         (const V *)B->csr_col_Val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, (V *)C->tile_csr_Val,
         C->tile_csr_Col, p.overflow(), sl.row_nc, sl.val_cap, sl.val_done);
@@ -931,7 +936,23 @@ static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
     return TSG_OK;
 }
 
+// The rows of bins [b, e): host offsets, or in a captured graph the device
+// offsets with the bins' capacities as the count the launches cover.
+BinRange bin_range(const Plan &p, int b, int e) {
+    BinRange r;
+    if (p.d_bins) {
+        for (int q = b; q < e; q++) r.count += p.bin_cap[q];
+        r.lo = p.d_bins + b;
+        r.hi = p.d_bins + e;
+    } else {
+        r.begin = p.bin_off[b];
+        r.count = p.bin_off[e] - p.bin_off[b];
+    }
+    return r;
+}
+
 // Structure fill + symbolic for every bin.
+
 int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                  const Plan &p, const SlotLayout &sl) {
     const int fb = p.nvar;  // C-tile kernels
@@ -940,13 +961,13 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
         TSG_CHECK(launch_sym_staged<VarW>(ctx, A, B, C, p, 2, sl));
         TSG_CHECK(launch_sym_staged<VarM>(ctx, A, B, C, p, 3, sl));
     }
-    const int64_t nf = p.bin_off[fb + 1] - p.bin_off[fb];
-    TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb], nf, sl, p.hb));
-    TSG_CHECK(launch_symbolic_list(ctx, A, B, p.rows, p.bin_off[fb], nf, C, sl));
+    const BinRange f = bin_range(p, fb, fb + 1);
+    TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, f, sl, p.hb));
+    TSG_CHECK(launch_symbolic_list(ctx, A, B, p.rows, f, C, sl));
     if (p.rg) {  // row-by-row kernels: Step 1 fill as the C-tile rows, then masks from the products
-        const int64_t nr = p.bin_off[fb + 2] - p.bin_off[fb + 1];
-        TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, sl, p.hb));
-        TSG_CHECK(launch_rg_symbolic(ctx, A, B, C, p.rows, p.bin_off[fb + 1], nr, p.rgw, sl));
+        const BinRange r = bin_range(p, fb + 1, fb + 2);
+        TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, r, sl, p.hb));
+        TSG_CHECK(launch_rg_symbolic(ctx, A, B, C, p.rows, r, p.rgw, sl));
     }
     return TSG_OK;
 }
@@ -957,8 +978,9 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
     if (p.nvar > 0) {
         // R's numeric kernel pays for its own launch only on many rows; fewer
         // go to VarS's with S's rows (they fit its caps)
-        const bool rn = p.bin_off[1] - p.bin_off[0] >= ROW_LANES_MIN;
-        if (rn) TSG_CHECK(launch_num_rows(ctx, A, B, C, p.rows, p.bin_off[0], p.bin_off[1] - p.bin_off[0], sl));
+        // (a captured graph does not know the bin's size: there always its own kernel)
+        const bool rn = p.d_bins || p.bin_off[1] - p.bin_off[0] >= ROW_LANES_MIN;
+        if (rn) TSG_CHECK(launch_num_rows(ctx, A, B, C, p.rows, bin_range(p, 0, 1), sl));
         const int s0 = rn ? 1 : 0;
         if (A->dtype == TSG_F32) {
             TSG_CHECK((launch_num_staged<float, VarS>(ctx, A, B, C, p, s0, sl, 2)));
@@ -970,10 +992,8 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
             TSG_CHECK((launch_num_staged<double, VarM>(ctx, A, B, C, p, 3, sl)));
         }
     }
-    TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, p.bin_off[fb], p.bin_off[fb + 1] - p.bin_off[fb], sl));
-    if (p.rg)
-        TSG_CHECK(launch_rg_numeric(ctx, A, B, C, p.rows, p.bin_off[fb + 1], p.bin_off[fb + 2] - p.bin_off[fb + 1],
-                                    p.rgw, p.rgn, p.rgc, sl));
+    TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, bin_range(p, fb, fb + 1), sl));
+    if (p.rg) TSG_CHECK(launch_rg_numeric(ctx, A, B, C, p.rows, bin_range(p, fb + 1, fb + 2), p.rgw, p.rgn, p.rgc, sl));
     return TSG_OK;
 }
 
@@ -983,8 +1003,7 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
 int prepare_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, Plan &p, const SlotLayout &sl) {
     if (!p.rg) return TSG_OK;
     const int fb = p.nvar;
-    return prepare_rg_numeric(ctx, A, C, p.rows, p.bin_off[fb + 1], p.bin_off[fb + 2] - p.bin_off[fb + 1], p.rgn,
-                              &p.rgc, sl);
+    return prepare_rg_numeric(ctx, A, C, p.rows, bin_range(p, fb + 1, fb + 2), p.rgn, &p.rgc, sl);
 }
 
 void free_c(Context *ctx, tsg_tile_matrix *C) {
@@ -1091,6 +1110,65 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     free_plan(ctx, p);
     if (rc) free_c(ctx, C);
     return rc;
+}
+
+// Capacity check of the device-sized pipeline, after binning: numtileC
+// (hdr[0]) against C's tile arrays and every bin's row count against the rows
+// its launches cover.  Past any of them the flag is set and every bin emptied,
+// so no later kernel writes C (the caller then computes the batch exactly).
+struct BinCaps {
+    int64_t v[MAX_BINS];
+};
+__global__ void k_device_guard(int64_t *__restrict__ hdr, int nbins, int64_t cap_tiles, BinCaps caps,
+                               int32_t *__restrict__ overflow) {
+    if (threadIdx.x != 0) return;
+    bool over = hdr[0] > cap_tiles;
+    for (int b = 0; b < nbins; b++) over = over || hdr[3 + b] - hdr[2 + b] > caps.v[b];
+    if (!over) return;
+    atomicOr(overflow, 1);
+    for (int b = 0; b <= nbins; b++) hdr[2 + b] = 0;
+}
+
+int spgemm_binned_device(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                         const tsg_options *opts, const DeviceCaps &caps, Plan &p) {
+    if (!ctx->device_sizes) return fail(TSG_ERR_ARG, "device-sized pipeline outside a graph build");
+    if (A->tilem <= 0) return fail(TSG_ERR_ARG, "a device-sized product needs tile-rows");
+    if (!binned_variant(opts) || !staged_ok(A, B, opts))
+        return fail(TSG_ERR_ARG, "a device-sized product runs the binned kernels: numeric_variant 0 and B's "
+                                 "column-major tile copies");
+    memset(C, 0, sizeof *C);
+    C->m = A->m; C->n = B->n; C->tilem = A->tilem; C->tilen = B->tilen;
+    C->dtype = A->dtype; C->location = TSG_DEVICE;
+    p = Plan{};
+    const int32_t tilem = A->tilem;
+    TSG_CHECK(palloc(ctx, &C->tile_ptr, (size_t)tilem + 1));
+    TSG_CHECK(plan_count(ctx, A, B, C, opts, p, nullptr));
+    TSG_CHECK(plan_bin(ctx, p, tilem, C->tile_ptr + tilem, nullptr));
+    if (p.nbins > MAX_BINS) return fail(TSG_ERR_ARG, "bin table");
+    BinCaps bc{};
+    for (int b = 0; b < p.nbins; b++) {
+        bc.v[b] = std::min<int64_t>(caps.bins[b], tilem);
+        p.bin_cap[b] = bc.v[b];
+    }
+    k_device_guard<<<1, 64, 0, ctx->stream>>>(p.hdr, p.nbins, caps.tiles, bc, p.overflow());
+    TSG_HIP(hipGetLastError());
+    p.d_bins = p.hdr + 2;
+    C->numtile = caps.tiles;
+    TSG_CHECK(palloc(ctx, &C->tile_rowidx, (size_t)caps.tiles));
+    TSG_CHECK(palloc(ctx, &C->tile_columnidx, (size_t)caps.tiles));
+    TSG_CHECK(palloc(ctx, &C->mask, (size_t)caps.tiles * 16));
+    TSG_CHECK(palloc(ctx, &C->tile_csr_Ptr, (size_t)caps.tiles * 16));
+    TSG_CHECK(palloc(ctx, &C->tile_nnz, (size_t)caps.tiles + 1));
+    SlotLayout sl;
+    sl.overflow = p.overflow();
+    sl.val_cap = caps.nnz;
+    TSG_CHECK(run_symbolic(ctx, A, B, C, p, sl));
+    // C's value offsets over its numtileC tiles (hdr[0] = tile_ptrC[tilem])
+    TSG_CHECK(scan_inclusive_i64(ctx, C->tile_nnz, caps.tiles + 1, true, &p.flags->nnz, C->tile_ptr + tilem));
+    C->nnz = caps.nnz;
+    TSG_CHECK(pool_alloc(ctx, &C->tile_csr_Val, (size_t)std::max<int64_t>(caps.nnz, 1) * vsize(C->dtype)));
+    TSG_CHECK(palloc(ctx, &C->tile_csr_Col, (size_t)std::max<int64_t>(caps.nnz, 1)));
+    return run_numeric(ctx, A, B, C, p, sl);
 }
 
 int replay_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,

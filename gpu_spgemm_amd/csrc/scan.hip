@@ -15,10 +15,17 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
+// The items a scan covers: n, or with dlast (a captured graph, where the
+// length lives on the device) the items up to index *dlast, at most n.
+__device__ __forceinline__ int64_t scan_len(int64_t n, const int64_t *dlast) {
+    return dlast ? min(n, *dlast + 1) : n;
+}
+
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const int64_t *__restrict__ data,
-                                                               int64_t n, int64_t *__restrict__ partial,
-                                                               bool zero_first) {
+                                                               int64_t n, const int64_t *__restrict__ dlast,
+                                                               int64_t *__restrict__ partial, bool zero_first) {
     __shared__ int64_t s_w[SCAN_THREADS / WAVE];
+    n = scan_len(n, dlast);
     int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
     int64_t s = 0;
 #pragma unroll
@@ -41,12 +48,14 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const int64_t *__r
 // the earlier blocks' raw partials added up by this block (partial_raw, few
 // blocks: saves scanning the partials in a launch of their own).
 __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict__ data, int64_t n,
+                                                              const int64_t *__restrict__ dlast,
                                                               const int64_t *__restrict__ partial_incl,
                                                               const int64_t *__restrict__ partial_raw,
                                                               bool zero_first, int64_t *__restrict__ total_out) {
     __shared__ int64_t s_d[SCAN_TILE + SCAN_TILE / 32];
     __shared__ int64_t s_w[SCAN_THREADS / WAVE];
     __shared__ int64_t s_c[SCAN_THREADS / WAVE];
+    n = scan_len(n, dlast);
     int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
     int64_t carry = 0;
     if (partial_raw) {
@@ -108,18 +117,19 @@ int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb) {
     return TSG_OK;
 }
 
-int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, int64_t *total_out) {
+int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, int64_t *total_out,
+                       const int64_t *dlast) {
     if (n <= 0) return TSG_OK;
     int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "scan of %lld items exceeds grid limit", (long long)n);
     if (nb == 1) {
-        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, nullptr, nullptr, zero_first, total_out);
+        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, dlast, nullptr, nullptr, zero_first, total_out);
         TSG_HIP(hipGetLastError());
         return TSG_OK;
     }
     int64_t *partial = nullptr;
     TSG_CHECK(palloc(ctx, &partial, (size_t)nb));
-    k_scan_reduce<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, partial, zero_first);
+    k_scan_reduce<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, dlast, partial, zero_first);
     int rc = TSG_OK;
     if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_reduce launch failed");
     // up to SCAN_TILE blocks each add up the raw partials before them (at most
@@ -127,7 +137,7 @@ int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, 
     const bool raw = nb <= SCAN_TILE;
     if (rc == TSG_OK && !raw) rc = scan_inclusive_i64(ctx, partial, nb);
     if (rc == TSG_OK) {
-        k_scan_apply<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, raw ? nullptr : partial,
+        k_scan_apply<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, dlast, raw ? nullptr : partial,
                                                                      raw ? partial : nullptr, zero_first, total_out);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_apply launch failed");
     }

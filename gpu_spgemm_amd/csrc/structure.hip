@@ -31,15 +31,16 @@ k_structure(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ c
             const int64_t *__restrict__ tile_ptrB, const int32_t *__restrict__ colidxB,
             int64_t *__restrict__ tile_ptrC, int32_t *__restrict__ rowidxC,
             int32_t *__restrict__ colidxC, const int32_t *__restrict__ rows, int64_t row_begin,
-            int64_t nrows, int32_t *__restrict__ row_nc, int32_t *__restrict__ overflow,
+            int64_t nrows, const int64_t *__restrict__ dlo, const int64_t *__restrict__ dhi,
+            int32_t *__restrict__ row_nc, int32_t *__restrict__ overflow,
             int32_t *__restrict__ hlist, int32_t *__restrict__ hcount) {
     __shared__ uint32_t s_bm[ST_WAVES][ST_WIN_WORDS];
     __shared__ int64_t s_bs[ST_WAVES][WAVE];
     __shared__ int32_t s_pre[ST_WAVES][WAVE];
     const int w = threadIdx.x / WAVE, lane = lane_id();
     const int64_t idx = (int64_t)blockIdx.x * ST_WAVES + w;
-    if (idx >= (rows ? nrows : (int64_t)tilemA)) return;  // whole wave leaves; no block barrier below
-    const int32_t bi = rows ? rows[row_begin + idx] : (int32_t)idx;
+    const int32_t bi = rows ? bin_row(rows, row_begin, nrows, dlo, dhi, idx) : (idx < tilemA ? (int32_t)idx : -1);
+    if (bi < 0) return;  // whole wave leaves; no block barrier below
     uint32_t *bm = s_bm[w];
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
 
@@ -146,7 +147,7 @@ int launch_structure_count(Context *ctx, const tsg_tile_matrix *A, const tsg_til
     unsigned grid = (unsigned)((A->tilem + ST_WAVES - 1) / ST_WAVES);
     k_structure<false><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx, tile_ptrC, nullptr,
-        nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr);
+        nullptr, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -157,7 +158,8 @@ int launch_structure_fill(Context *ctx, const tsg_tile_matrix *A, const tsg_tile
     unsigned grid = (unsigned)((A->tilem + ST_WAVES - 1) / ST_WAVES);
     k_structure<true><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
         A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx,
-        const_cast<int64_t *>(tile_ptrC), rowidxC, colidxC, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr);
+        const_cast<int64_t *>(tile_ptrC), rowidxC, colidxC, nullptr, 0, 0, nullptr, nullptr, nullptr, nullptr,
+        nullptr, nullptr);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
@@ -470,18 +472,19 @@ int launch_heavy_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
 }
 
 int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
-                               const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
+                               const tsg_tile_matrix *C, const int32_t *rows, const BinRange &br,
                                const SlotLayout &sl, const HeavyBitmaps &hb) {
+    const int64_t count = br.count;
     if (count <= 0) return TSG_OK;
     int32_t *hlist = nullptr, *hc = nullptr;  // heavy rows; hc = {count, pad, work counter (8 bytes)}
     TSG_CHECK(palloc(ctx, &hlist, (size_t)count));
     int rc = palloc(ctx, &hc, 4);
-    if (!rc && hipMemsetAsync(hc, 0, 4 * sizeof(int32_t), ctx->stream) != hipSuccess) rc = fail(TSG_ERR_HIP, "memset");
+    if (!rc) rc = zero_small(ctx, hc, 4, nullptr, 0);  // a kernel: capturable
     if (!rc) {
         unsigned grid = (unsigned)((count + ST_WAVES - 1) / ST_WAVES);
         k_structure<true><<<grid, ST_WAVES * WAVE, 0, ctx->stream>>>(
             A->tile_ptr, A->tile_columnidx, A->tilem, B->tile_ptr, B->tile_columnidx, C->tile_ptr, C->tile_rowidx,
-            C->tile_columnidx, rows, begin, count, sl.row_nc, sl.overflow, hlist, hc);
+            C->tile_columnidx, rows, br.begin, count, br.lo, br.hi, sl.row_nc, sl.overflow, hlist, hc);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_structure launch");
     }
     if (!rc) {

@@ -70,7 +70,8 @@ k_symbolic(const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ co
 // Heavy tile-rows: one workgroup per listed row, its 16 groups stride over the
 // row's C tiles.
 __global__ void __launch_bounds__(SY_THREADS)
-k_symbolic_list(const int32_t *__restrict__ rows, int64_t row_begin, const int64_t *__restrict__ tile_ptrC,
+k_symbolic_list(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, const int64_t *__restrict__ dlo,
+                const int64_t *__restrict__ dhi, int64_t c0, const int64_t *__restrict__ tile_ptrC,
                 const int64_t *__restrict__ tile_ptrA, const int32_t *__restrict__ colidxA,
                 const uint16_t *__restrict__ maskA, const int64_t *__restrict__ csc_ptrB,
                 const int32_t *__restrict__ csc_rowB, const uint16_t *__restrict__ maskB,
@@ -79,7 +80,8 @@ k_symbolic_list(const int32_t *__restrict__ rows, int64_t row_begin, const int64
                 const int32_t *__restrict__ row_nc) {
     __shared__ uint16_t s_mb[SY_GROUPS][16];
     const int g = threadIdx.x >> 4;
-    const int32_t bi = rows[row_begin + blockIdx.x];
+    const int32_t bi = bin_row(rows, row_begin, nrows, dlo, dhi, c0 + blockIdx.x);
+    if (bi < 0) return;
     int64_t cs, ce;
     c_row_range(tile_ptrC, row_nc, bi, cs, ce);
     for (int64_t ti = cs + g; ti < ce; ti += SY_GROUPS)
@@ -88,11 +90,12 @@ k_symbolic_list(const int32_t *__restrict__ rows, int64_t row_begin, const int64
 }
 
 int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const int32_t *rows,
-                         int64_t begin, int64_t count, const tsg_tile_matrix *C, const SlotLayout &sl) {
+                         const BinRange &br, const tsg_tile_matrix *C, const SlotLayout &sl) {
+    const int64_t count = br.count;
     for (int64_t c0 = 0; c0 < count; c0 += ROW_CHUNK) {  // work-items per launch stay below 2^32
         const unsigned n = (unsigned)(count - c0 < ROW_CHUNK ? count - c0 : ROW_CHUNK);
         k_symbolic_list<<<n, SY_THREADS, 0, ctx->stream>>>(
-            rows, begin + c0, C->tile_ptr, A->tile_ptr, A->tile_columnidx, A->mask, B->csc_tile_ptr,
+            rows, br.begin, count, br.lo, br.hi, c0, C->tile_ptr, A->tile_ptr, A->tile_columnidx, A->mask, B->csc_tile_ptr,
             B->csc_tile_rowidx, B->mask, C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz,
             sl.row_nc);
         TSG_HIP(hipGetLastError());

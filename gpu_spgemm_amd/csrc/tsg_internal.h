@@ -88,6 +88,11 @@ struct Context {
     // set only inside tsg_spgemm_noalloc
     Arena *arena = nullptr;
     const FixedOut *fixed_out = nullptr;
+    // set while a batch graph is built: plans leave out the kernels that need
+    // host-known sizes (row-by-row bin, union bitmaps), and pool_free defers
+    // into `deferred` (a captured graph goes on using the arrays it ran with)
+    bool device_sizes = false;
+    std::vector<void *> *deferred = nullptr;
 };
 
 int pool_alloc(Context *ctx, void **p, size_t bytes);
@@ -121,8 +126,11 @@ constexpr int64_t ROW_CHUNK = int64_t(1) << 23;
 // ExclusiveSum, src/spgemm_raw_v2.cu:131-138, 202-208, widened to int64).
 // zero_first: data[0] is taken as 0 whatever it holds (saves clearing it);
 // total_out (device, may be null): also receives data[n-1] after the scan.
+// dlast (device, may be null): the scan covers items [0, min(n, *dlast + 1))
+// only -- a length known on the device alone (a captured graph); the partial
+// sums of the blocks past it are zero.
 int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first = false,
-                       int64_t *total_out = nullptr);
+                       int64_t *total_out = nullptr, const int64_t *dlast = nullptr);
 // Zeroes a[0..na) and b[0..nb) (at most 64 each) with one small kernel on the context's stream.
 int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb);
 // Reads one int64 from device memory (blocking; counts as a host sync).
@@ -181,27 +189,39 @@ int build_brow_bitmaps(Context *ctx, const tsg_tile_matrix *B, HeavyBitmaps &hb)
 // counter}, zeroed before the list is built), written to tile_ptrC[bi + 1].
 int launch_heavy_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, int64_t *tile_ptrC,
                        const int32_t *hlist, int32_t *hc, int32_t *stats, const HeavyBitmaps &hb);
+// The rows [begin, begin + count) of a bin's row list; in a captured graph the
+// range is read on the device from lo/hi and count is an upper bound.
+struct BinRange {
+    int64_t begin = 0, count = 0;
+    const int64_t *lo = nullptr, *hi = nullptr;
+};
+inline BinRange host_range(int64_t begin, int64_t count) {
+    BinRange r;
+    r.begin = begin;
+    r.count = count;
+    return r;
+}
 int launch_structure_fill_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
-                               const tsg_tile_matrix *C, const int32_t *rows, int64_t begin, int64_t count,
+                               const tsg_tile_matrix *C, const int32_t *rows, const BinRange &br,
                                const SlotLayout &sl, const HeavyBitmaps &hb = HeavyBitmaps{});
 int launch_symbolic_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const int32_t *rows,
-                         int64_t begin, int64_t count, const tsg_tile_matrix *C, const SlotLayout &sl);
+                         const BinRange &br, const tsg_tile_matrix *C, const SlotLayout &sl);
 int launch_numeric_list(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                        const int32_t *rows, int64_t begin, int64_t count, const SlotLayout &sl);
+                        const int32_t *rows, const BinRange &br, const SlotLayout &sl);
 void learned_release(Context *ctx);  // learned.hip
 void tuner_release(Context *ctx);    // autotune.hip
 // Row-by-row kernels for sparse heavy tile-rows (heavy.hip).  woff holds
 // count + 3 int64: the rows' work-item offsets (written by the symbolic
 // launch, read by the numeric one) and two work counters.
 int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                       const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const SlotLayout &sl);
+                       const int32_t *rows, const BinRange &br, int64_t *woff, const SlotLayout &sl);
 // After C's value offsets are scanned: the rows' numeric work-item offsets
 // (count + 1 int64 in woffn).
 // *gcur: the numeric kernel's per-workgroup cursor area, allocated on first use.
 int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, const int32_t *rows,
-                       int64_t begin, int64_t count, int64_t *woffn, int64_t **gcur, const SlotLayout &sl);
+                       const BinRange &br, int64_t *woffn, int64_t **gcur, const SlotLayout &sl);
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                      const int32_t *rows, int64_t begin, int64_t count, int64_t *woff, const int64_t *woffn,
+                      const int32_t *rows, const BinRange &br, int64_t *woff, const int64_t *woffn,
                       int64_t *gcur, const SlotLayout &sl);
 int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B);  // api.hip
 // Looks up the no-alloc scratch's device address on ctx's device (current).
@@ -320,6 +340,24 @@ __device__ __forceinline__ T group16_sum(T v) {
 #pragma unroll
     for (int d = 8; d > 0; d >>= 1) v += __shfl_xor(v, d, 16);
     return v;
+}
+
+// Rows [row_begin, row_begin + nrows) of a bin's row list; in a captured graph
+// the bin's offsets live on the device only: then [*dlo, *dhi), cut to the
+// host's nrows (the rows the grid covers; a longer bin is flagged by the
+// pipeline's capacity check).
+__device__ __forceinline__ void bin_rows(const int64_t *dlo, const int64_t *dhi, int64_t &row_begin, int64_t &nrows) {
+    if (dlo) {
+        row_begin = *dlo;
+        nrows = min(nrows, *dhi - row_begin);
+    }
+}
+// Item idx of a bin's row list (host range, or the device range dlo/dhi): its
+// tile-row, or -1 past the list's end.
+__device__ __forceinline__ int32_t bin_row(const int32_t *rows, int64_t row_begin, int64_t nrows,
+                                           const int64_t *dlo, const int64_t *dhi, int64_t idx) {
+    bin_rows(dlo, dhi, row_begin, nrows);
+    return idx < nrows ? rows[row_begin + idx] : -1;
 }
 
 // Tile range [s,e) of C tile-row bi.  Exact layout: ptr is tile_ptrC.  Slot

@@ -414,6 +414,35 @@ int tsg_spgemm_batched(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A
                        const tsg_tile_matrix *B, tsg_tile_matrix *C, const tsg_options *opts,
                        tsg_stats *stats);
 
+/* ---- Batch graphs: a whole batched product in one launch (no reference
+ * counterpart; the reference computes one product per process with host
+ * syncs between its steps, src/spgemm_raw_v2.cu:111-240) ----
+ * tsg_batch_graph_create captures C_i = A_i * B_i for the count problems into
+ * one hipGraph: the copies into the block-diagonal operands, Steps 1-3 with
+ * every size kept on the device, the split into per-problem C_i and one copy
+ * of a result row into pinned host memory.  C's arrays and the rows each bin's
+ * kernels cover are capacities: an exact first product's sizes times `slack`
+ * (>= 1; 1.25 leaves room for replays whose structure grows).
+ * tsg_batch_graph_launch queues one hipGraphLaunch on the context's stream and
+ * returns (no host sync).  A launch reads the operands' arrays again: their
+ * values and structure may change between launches; each problem's
+ * dimensions, tile count and nonzero count, and the arrays' addresses, may
+ * not.  tsg_batch_graph_result waits for the last launch and fills C[0..count)
+ * with views of the graph's outputs, valid until the next launch or destroy
+ * (never tsg_tile_free them).  A product past a capacity is flagged on the
+ * device; result then computes the batch with tsg_spgemm_batched's exact
+ * pipeline (stats[i].fallback_used = 1) and returns those outputs, owned by the
+ * graph as well.  Results are bit-identical to tsg_spgemm_batched.  Runs the
+ * binned kernels only: opts->numeric_variant 0 and B's column-major copies. */
+typedef struct tsg_batch_graph tsg_batch_graph;
+int tsg_batch_graph_create(tsg_context *ctx, int32_t count, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
+                           const tsg_options *opts, double slack, tsg_batch_graph **graph);
+int tsg_batch_graph_launch(tsg_batch_graph *graph);
+int tsg_batch_graph_result(tsg_batch_graph *graph, tsg_tile_matrix *C, tsg_stats *stats);
+/* C's tile and nonzero capacities over the whole batch. */
+int tsg_batch_graph_capacity(const tsg_batch_graph *graph, int64_t *cap_tiles, int64_t *cap_nnz);
+void tsg_batch_graph_destroy(tsg_batch_graph *graph);
+
 #ifdef __cplusplus
 }
 #endif
