@@ -85,11 +85,17 @@ __device__ __forceinline__ void rgw_split(const int32_t *__restrict__ colidxC, i
 // Work items of each listed row: woff[q + 1] = item count of row q, woff[0] = 0
 // (scanned afterwards into item offsets).
 __global__ void __launch_bounds__(256)
-k_rgw_count(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ tile_ptrC,
+k_rgw_count(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ dlo,
+            const int64_t *__restrict__ dhi, const int64_t *__restrict__ tile_ptrC,
             const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC, int64_t *__restrict__ woff) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t cover = nr;
+    bin_rows(dlo, dhi, row_begin, nr);
     if (q == 0) woff[0] = 0;
-    if (q >= nr) return;
+    if (q >= nr) {
+        if (q < cover) woff[q + 1] = 0;  // past a device-sized bin's rows: no items
+        return;
+    }
     int64_t cs, ce, nwin, per;
     c_row_range(tile_ptrC, row_nc, rows[row_begin + q], cs, ce);
     rgw_split(colidxC, cs, ce, nwin, per);
@@ -155,7 +161,8 @@ struct RgwSeg {
 };
 
 __global__ void __launch_bounds__(RGW_THREADS)
-k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ woff,
+k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ dlo,
+               const int64_t *__restrict__ dhi, const int64_t *__restrict__ woff,
                unsigned long long *__restrict__ ctr, int32_t m, const int64_t *__restrict__ rpA,
                const int32_t *__restrict__ colA, const int64_t *__restrict__ rpB, const int32_t *__restrict__ colB,
                const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ row_nc,
@@ -168,6 +175,7 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
     __shared__ RgwItem s_it;
     __shared__ int64_t s_item;
     __shared__ int s_nq;
+    bin_rows(dlo, dhi, row_begin, nr);
     const int64_t total = woff[nr];
     if ((int64_t)blockIdx.x >= total) return;  // more workgroups than items: no counter traffic
     const int tid = threadIdx.x;
@@ -298,12 +306,18 @@ constexpr int rgw_item_values() { return 2 * RGN_BYTES / (int)sizeof(V); }
 
 // Numeric work items of each listed row: woffn[q + 1] = ceil(row values / item values).
 __global__ void __launch_bounds__(256)
-k_rgw_count_num(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr,
-                const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ row_nc,
-                const int64_t *__restrict__ tile_nnzC, int64_t item_values, int64_t *__restrict__ woffn) {
+k_rgw_count_num(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ dlo,
+                const int64_t *__restrict__ dhi, const int64_t *__restrict__ tile_ptrC,
+                const int32_t *__restrict__ row_nc, const int64_t *__restrict__ tile_nnzC, int64_t item_values,
+                int64_t *__restrict__ woffn) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t cover = nr;
+    bin_rows(dlo, dhi, row_begin, nr);
     if (q == 0) woffn[0] = 0;
-    if (q >= nr) return;
+    if (q >= nr) {
+        if (q < cover) woffn[q + 1] = 0;  // past a device-sized bin's rows: no items
+        return;
+    }
     int64_t cs, ce;
     c_row_range(tile_ptrC, row_nc, rows[row_begin + q], cs, ce);
     const int64_t nv = ce > cs ? tile_nnzC[ce] - tile_nnzC[cs] : 0;
@@ -370,7 +384,8 @@ __device__ __forceinline__ int rgw_slot(const uint16_t *map, const uint16_t *msk
 
 template <typename V>
 __global__ void __launch_bounds__(RGW_THREADS)
-k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ woffn,
+k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ dlo,
+              const int64_t *__restrict__ dhi, const int64_t *__restrict__ woffn,
               unsigned long long *__restrict__ ctr, int32_t m, const int64_t *__restrict__ rpA,
               const int32_t *__restrict__ colA, const V *__restrict__ valA, const int64_t *__restrict__ rpB,
               const int32_t *__restrict__ colB, const V *__restrict__ valB, const int64_t *__restrict__ tile_ptrC,
@@ -387,6 +402,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
     __shared__ RgwNumItem s_it;
     __shared__ int64_t s_item, s_sp1;
     __shared__ int s_row;
+    bin_rows(dlo, dhi, row_begin, nr);
     const int64_t total = woffn[nr];
     if ((int64_t)blockIdx.x >= total) return;  // more workgroups than items: no counter traffic
     const int tid = threadIdx.x, l = lane_id();
@@ -655,15 +671,15 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
                        const int32_t *rows, const BinRange &br, int64_t *woff, const SlotLayout &sl) {
     const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
-    if (br.lo) return fail(TSG_ERR_ARG, "the row-by-row kernels need host-known bin ranges");
     unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
-    k_rgw_count<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, C->tile_ptr,
+    k_rgw_count<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, br.lo, br.hi,
+                                                                         C->tile_ptr,
                                                                          sl.row_nc, C->tile_columnidx, woff);
     TSG_HIP(hipGetLastError());
     TSG_CHECK(scan_inclusive_i64(ctx, woff, count + 1));
     k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);
     k_rgw_symbolic<<<rgw_grid(ctx, 4), RGW_THREADS, 0, ctx->stream>>>(
-        rows, begin, count, woff, ctr, A->m, A->row_ptr, A->row_col, B->row_ptr, B->row_col, C->tile_ptr,
+        rows, begin, count, br.lo, br.hi, woff, ctr, A->m, A->row_ptr, A->row_col, B->row_ptr, B->row_col, C->tile_ptr,
         sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
@@ -675,11 +691,11 @@ int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
                        const BinRange &br, int64_t *woffn, int64_t **gcur, const SlotLayout &sl) {
     const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
-    if (br.lo) return fail(TSG_ERR_ARG, "the row-by-row kernels need host-known bin ranges");
     if (!*gcur && rgn_gcap(A) > 0)
         TSG_CHECK(palloc(ctx, gcur, (size_t)rgw_grid(ctx, 3) * (size_t)rgn_gcap(A) * 2));
     const int64_t h = C->dtype == TSG_F32 ? rgw_item_values<float>() : rgw_item_values<double>();
-    k_rgw_count_num<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, C->tile_ptr,
+    k_rgw_count_num<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, br.lo, br.hi,
+                                                                             C->tile_ptr,
                                                                              sl.row_nc, C->tile_nnz, h, woffn);
     TSG_HIP(hipGetLastError());
     return scan_inclusive_i64(ctx, woffn, count + 1);
@@ -690,19 +706,18 @@ int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_mat
                       int64_t *gcur, const SlotLayout &sl) {
     const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
-    if (br.lo) return fail(TSG_ERR_ARG, "the row-by-row kernels need host-known bin ranges");
     const int64_t gcap = gcur ? rgn_gcap(A) : 0;
     unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
     k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);
     if (A->dtype == TSG_F32)
         k_rgw_numeric<float><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
-            rows, begin, count, woffn, ctr, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr,
+            rows, begin, count, br.lo, br.hi, woffn, ctr, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr,
             B->row_col, (const float *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
             C->tile_csr_Ptr, C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done,
             gcur, gcap);
     else
         k_rgw_numeric<double><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
-            rows, begin, count, woffn, ctr, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr,
+            rows, begin, count, br.lo, br.hi, woffn, ctr, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr,
             B->row_col, (const double *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
             C->tile_csr_Ptr, C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done,
             gcur, gcap);

@@ -764,7 +764,7 @@ BinTable bin_table_for(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
     // the row-by-row bin needs a per-workgroup cursor area far larger than the
     // no-alloc scratch: inside such a call heavy rows take the C-tile kernels
     // (every path computes the same bits)
-    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts) && !ctx->arena && !ctx->device_sizes, cfg.rg_density, cfg.var_mask,
+    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts) && !ctx->arena, cfg.rg_density, cfg.var_mask,
                      num_rows_ok(A, B));
 }
 
@@ -875,7 +875,7 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
         int rc = no_heavy ? TSG_OK : palloc(ctx, &hlist, (size_t)tilem);
         // keep union bitmaps only where long tile-rows are known (without them
         // the fill just rebuilds every bitmap)
-        if (!rc && !ctx->arena && !ctx->device_sizes && B->numtile > HEAVY_NB && ((A->flags | B->flags) & 2)) {
+        if (!rc && !ctx->arena && B->numtile > HEAVY_NB && ((A->flags | B->flags) & 2)) {
             p.hb.cap = std::min<int64_t>(HEAVY_BITMAP_WORDS, (int64_t)tilem * ((int64_t)B->tilen / 32 + 1));
             rc = palloc(ctx, &p.hb.words, (size_t)p.hb.cap);
             if (!rc) rc = palloc(ctx, &p.hb.off, (size_t)tilem);
@@ -1165,6 +1165,7 @@ int spgemm_binned_device(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
     TSG_CHECK(run_symbolic(ctx, A, B, C, p, sl));
     // C's value offsets over its numtileC tiles (hdr[0] = tile_ptrC[tilem])
     TSG_CHECK(scan_inclusive_i64(ctx, C->tile_nnz, caps.tiles + 1, true, &p.flags->nnz, C->tile_ptr + tilem));
+    TSG_CHECK(prepare_numeric(ctx, A, C, p, sl));
     C->nnz = caps.nnz;
     TSG_CHECK(pool_alloc(ctx, &C->tile_csr_Val, (size_t)std::max<int64_t>(caps.nnz, 1) * vsize(C->dtype)));
     TSG_CHECK(palloc(ctx, &C->tile_csr_Col, (size_t)std::max<int64_t>(caps.nnz, 1)));

@@ -88,9 +88,9 @@ struct Context {
     // set only inside tsg_spgemm_noalloc
     Arena *arena = nullptr;
     const FixedOut *fixed_out = nullptr;
-    // set while a batch graph is built: plans leave out the kernels that need
-    // host-known sizes (row-by-row bin, union bitmaps), and pool_free defers
-    // into `deferred` (a captured graph goes on using the arrays it ran with)
+    // set while a batch graph is built (the device-sized pipeline may run);
+    // with `deferred` set, pool_free defers into it (a captured graph goes on
+    // using the arrays it ran with)
     bool device_sizes = false;
     std::vector<void *> *deferred = nullptr;
 };
