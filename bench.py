@@ -74,6 +74,22 @@ def numeric_bytes(A, B, C, s):
     return a + b + c
 
 
+def rows_numeric_bytes(A, B, C, s):
+    """Algorithmic bytes of one launch of the row-lane numeric kernel: A's and
+    B's scalar CSR read once, C's structure (tile offsets, column indices,
+    value offsets, masks, row offsets) read once, C's col bytes and values
+    written once."""
+    a = 8 * (A["m"] + 1) + (4 + s) * A["nnz"]
+    b = 8 * (B["m"] + 1) + (4 + s) * B["nnz"]
+    c = 8 * (C["tilem"] + 1) + (4 + 8 + 32 + 16) * C["numtile"] + (1 + s) * C["nnz"]
+    return a + b + c
+
+
+# numeric kernels by their bit in tsg_stats.numeric_kernels
+NUMERIC_KERNELS = {1: "k_num_rows", 2: "k_num_staged<VarS>", 4: "k_num_staged<VarW>", 8: "k_num_staged<VarM>",
+                   16: "k_numeric_list", 32: "k_rgw_numeric"}
+
+
 def train_learned(g, ctx, log, rmat_scale=0):
     """Trains the learned scheduler on a family of smaller matrices (never the
     benchmarked product itself), outside any timed region; for an R-MAT
@@ -441,15 +457,16 @@ def main():
         for _ in range(max(5, min(args.steps, 20))):   # per-step HIP-event breakdown on the library stream
             C, st = ctx.spgemm(tA, tB, timing=True, **kw)
             br.append(st)
-            Cd = {"tilem": C.s.tilem, "tilen": C.s.tilen, "numtile": C.s.numtile, "nnz": C.s.nnz}
+            Cd = {"tilem": C.s.tilem, "tilen": C.s.tilen, "numtile": C.s.numtile, "nnz": C.s.nnz,
+                  "numeric_kernels": st["numeric_kernels"]}
             C.free()
         ev = {k: statistics.mean(x[k] for x in br) for k in ("ms_step1", "ms_step2", "ms_step3")}
         log(f"{label}: {ms:.4f} ms/step  events: structure {ev['ms_step1']:.4f}  symbolic {ev['ms_step2']:.4f}"
             f"  numeric {ev['ms_step3']:.4f} ms")
         return ms, ev, Cd
 
-    Ad = {"tilem": tA.s.tilem, "tilen": tA.s.tilen, "numtile": tA.s.numtile, "nnz": tA.s.nnz}
-    Bd = {"tilem": tB.s.tilem, "tilen": tB.s.tilen, "numtile": tB.s.numtile, "nnz": tB.s.nnz}
+    Ad = {"m": tA.s.m, "tilem": tA.s.tilem, "tilen": tA.s.tilen, "numtile": tA.s.numtile, "nnz": tA.s.nnz}
+    Bd = {"m": tB.s.m, "tilem": tB.s.tilem, "tilen": tB.s.tilen, "numtile": tB.s.numtile, "nnz": tB.s.nnz}
     runs = {}
     if not rmat:   # R-MAT compares the learned scheduler with the exact binned pipeline
         runs["ctile"] = measure("reference C-tile method", variant=1)
@@ -477,13 +494,16 @@ def main():
     step1, step2, step3 = ev["ms_step1"], ev["ms_step2"], ev["ms_step3"]
     gflops = 2.0 * products / (ms * 1e6)
     pipe_bytes = alg_bytes(Ad, Bd, Cd, s)
-    nbytes = numeric_bytes(Ad, Bd, Cd, s)
+    # the numeric kernels the library launched (tsg_stats.numeric_kernels): when
+    # one kernel is the whole numeric step, its HIP-event time is step3
+    names = [NUMERIC_KERNELS[b] for b in sorted(NUMERIC_KERNELS) if Cd["numeric_kernels"] & b]
+    vt = "double" if s == 8 else "float"
+    if len(names) == 1:
+        kname = names[0].replace("<", f"<{vt},") if "<" in names[0] else f"{names[0]}<{vt}>"
+    else:
+        kname = "numeric step (" + ", ".join(names) + ")"
+    nbytes = rows_numeric_bytes(Ad, Bd, Cd, s) if names == ["k_num_rows"] else numeric_bytes(Ad, Bd, Cd, s)
     achieved = nbytes / (step3 * 1e-3) / 1e9
-    # the numeric step of this workload is one launch of the staged numeric
-    # kernel (every tile-row fits the small variant); its HIP-event time is step3
-    kname = "k_num_staged<double,VarS>"
-    if rmat:   # heavy rows: several numeric launches (row-by-row and C-tile kernels)
-        kname = "numeric step (all bins)"
     out = {
         "metric": "SpGEMM GFLOP/s + HBM GB/s, learned/batched vs plain tiled, 1 MI355X",
         "value": round(gflops, 3),

@@ -70,7 +70,7 @@ class Stats(C.Structure):
                 ("ms_step1", C.c_double), ("ms_step2", C.c_double), ("ms_step3", C.c_double),
                 ("ms_total", C.c_double), ("fallback_used", C.c_int32), ("host_syncs", C.c_int32),
                 ("pred_numtileC", C.c_int64), ("pred_nnzC", C.c_int64), ("pred_reused", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("numeric_kernels", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -467,6 +467,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
         st->pred_numtileC = (int64_t)std::ceil((double)fl.pred_tiles * std::exp((double)L->off_nc));
         st->pred_nnzC = cap;
         st->pred_reused = reuse ? 1 : 0;
+        st->numeric_kernels = C->nnz > 0 ? numeric_kernel_mask(p) : 0;
     }
     if (timing) {
         if (!rc && st) {

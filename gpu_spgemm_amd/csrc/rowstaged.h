@@ -116,6 +116,8 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
                const tsg_options *opts, Plan &p, int32_t *stats_out);
 int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                  const Plan &p, const SlotLayout &sl);
+// Bits of tsg_stats.numeric_kernels for the plan's host-known bins.
+int numeric_kernel_mask(const Plan &p);
 int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                 const Plan &p, const SlotLayout &sl);
 // Set-up of run_numeric that needs C's scanned value offsets (call once after the scan).

@@ -972,6 +972,24 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
     return TSG_OK;
 }
 
+// The numeric kernels run_numeric launches for host-known bins (tsg_stats.numeric_kernels).
+int numeric_kernel_mask(const Plan &p) {
+    if (p.d_bins) return 0;
+    auto rows = [&](int b, int e) { return p.bin_off[e] - p.bin_off[b]; };
+    int m = 0;
+    const int fb = p.nvar;
+    if (p.nvar > 0) {
+        const bool rn = rows(0, 1) >= ROW_LANES_MIN;
+        if (rn) m |= 1;
+        if (rows(rn ? 1 : 0, 2) > 0) m |= 2;
+        if (rows(2, 3) > 0) m |= 4;
+        if (rows(3, 4) > 0) m |= 8;
+    }
+    if (rows(fb, fb + 1) > 0) m |= 16;
+    if (p.rg && rows(fb + 1, fb + 2) > 0) m |= 32;
+    return m;
+}
+
 int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                 const Plan &p, const SlotLayout &sl) {
     const int fb = p.nvar;
@@ -1091,6 +1109,7 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
         st->numtileC = C->numtile;
         st->nnzC = C->nnz;
         st->host_syncs = syncs;
+        st->numeric_kernels = C->nnz > 0 ? numeric_kernel_mask(p) : 0;
     }
     if (timing) {
         if (!rc && st) {

@@ -158,7 +158,10 @@ typedef struct tsg_stats {
     int32_t host_syncs;     /* blocking device-to-host reads made inside the call */
     int64_t pred_numtileC, pred_nnzC;  /* learned: predicted C tiles; C's value capacity */
     int32_t pred_reused;    /* learned: 1 when the predictions were this operand pair's previous ones */
-    int32_t reserved;
+    int32_t numeric_kernels; /* binned pipeline: numeric kernels launched, bit 0 the row-lane kernel
+                                (k_num_rows), 1-3 the staged kernels S, W, M (k_num_staged), 4 the
+                                C-tile list kernel (k_numeric_list), 5 the row-by-row kernel
+                                (k_rgw_numeric); 0 elsewhere */
 } tsg_stats;
 
 /* ---- context ---- */
