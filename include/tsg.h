@@ -128,7 +128,8 @@ typedef struct tsg_context tsg_context;
 /* Scheduler choice for tsg_spgemm. */
 typedef enum tsg_scheduler {
     TSG_SCHED_PLAIN = 0,    /* the reference's 3-step method: exact counts, host reads of numtileC / nnzC */
-    TSG_SCHED_LEARNED = 1   /* MLP-predicted sizes and bins, no host sync; exact fallback on overflow */
+    TSG_SCHED_LEARNED = 1   /* exact structure, MLP-predicted value capacity: no read between the symbolic
+                               and numeric steps; exact numeric re-run past the capacity */
 } tsg_scheduler;
 
 typedef struct tsg_options {
