@@ -27,7 +27,7 @@ constexpr int RN_BB = 8;       // B row entries whose loads issue together
 
 template <typename V>
 struct RowNumLDS {
-    V val[RN_GROUPS][RN_VCAP];
+    V val[RN_GROUPS][RN_VCAP + 16];  // + a slot per lane that takes the products outside the window
     alignas(16) uint16_t cmask[RN_GROUPS][RN_NC * 16];  // row masks of the tile-row's C tiles
     alignas(16) uint8_t cptr[RN_GROUPS][RN_NC * 16];    // row offsets inside each C tile
     int32_t coff[RN_GROUPS][RN_NC + 1];                 // each C tile's first value, from the tile-row's first
@@ -128,16 +128,22 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
                 int32_t kk[RN_EB];
                 V aa[RN_EB];
                 int64_t j0[RN_EB], j1[RN_EB];
+                // loads past the row's end repeat its last entry (in bounds,
+                // no branch) and are then given an empty B row
 #pragma unroll
                 for (int u = 0; u < RN_EB; u++) {
-                    kk[u] = u < ne ? colA[eb + u] : 0;
-                    aa[u] = u < ne ? valA[eb + u] : V(0);
+                    const int64_t e = eb + min(u, ne - 1);
+                    kk[u] = colA[e];
+                    aa[u] = valA[e];
                 }
 #pragma unroll
                 for (int u = 0; u < RN_EB; u++) {
-                    j0[u] = u < ne ? rpB[kk[u]] : 0;
-                    j1[u] = u < ne ? rpB[kk[u] + 1] : 0;
+                    j0[u] = rpB[kk[u]];
+                    j1[u] = rpB[kk[u] + 1];
                 }
+#pragma unroll
+                for (int u = 0; u < RN_EB; u++)
+                    if (u >= ne) j1[u] = j0[u];
 #pragma unroll
                 for (int u = 0; u < RN_EB; u++) {
                     // the current C tile's column, first value of this row in it and row
@@ -150,24 +156,26 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
                         int32_t cc[RN_BB];
                         V bv[RN_BB];
 #pragma unroll
-                        for (int q = 0; q < RN_BB; q++) {
-                            cc[q] = q < nb ? colB[jb + q] : INT_MAX;
-                            bv[q] = q < nb ? valB[jb + q] : V(0);
+                        for (int q = 0; q < RN_BB; q++) {  // past the row's end: its last entry again, column INT_MAX
+                            const int64_t j = jb + min(q, nb - 1);
+                            cc[q] = colB[j];
+                            bv[q] = valB[j];
+                            if (q >= nb) cc[q] = INT_MAX;
                         }
 #pragma unroll
                         for (int q = 0; q < RN_BB; q++) {
                             const int32_t J = cc[q] >> 4;
-                            if (J >= jlo && J <= jhi) {   // inside the window (past a row's end: INT_MAX)
-                                if (J != jc) {
-                                    while (s.cj[g][ct] < J) ct++;  // J is one of the tile-row's C tile columns
-                                    const int t = ct * 16 + r;
-                                    jc = J;
-                                    base = s.coff[g][ct] - vb + s.cptr[g][t];
-                                    mk = s.cmask[g][t];
-                                }
-                                const int pos = base + __popc(mk & mask_below(cc[q] & 15));
-                                s.val[g][pos] = fma_rn<V>(aa[u], bv[q], s.val[g][pos]);
+                            const bool in = J >= jlo && J <= jhi;  // inside the window
+                            if (in && J != jc) {
+                                while (s.cj[g][ct] < J) ct++;  // J is one of the tile-row's C tile columns
+                                const int t = ct * 16 + r;
+                                jc = J;
+                                base = s.coff[g][ct] - vb + s.cptr[g][t];
+                                mk = s.cmask[g][t];
                             }
+                            // outside the window: the lane's own spare slot, never read
+                            const int pos = in ? base + __popc(mk & mask_below(cc[q] & 15)) : RN_VCAP + r;
+                            s.val[g][pos] = fma_rn<V>(aa[u], bv[q], s.val[g][pos]);
                         }
                     }
                 }
