@@ -211,8 +211,8 @@ int tsg_autotune_set(tsg_context *c, int32_t count, const tsg_tile_matrix *A, co
                      const tsg_tune_entry *cfg) {
     Context *ctx = reinterpret_cast<Context *>(c);
     if (!ctx || count < 1 || !A || !B || !cfg) return fail(TSG_ERR_ARG, "bad arguments");
-    if (cfg->var_mask < 0 || cfg->var_mask > 15 || cfg->rg_density < 0 || cfg->mfma_min_products < 0)
-        return fail(TSG_ERR_ARG, "var_mask must lie in [0, 15], rg_density and mfma_min_products be >= 0");
+    if (cfg->var_mask < 0 || cfg->var_mask > 31 || cfg->rg_density < 0 || cfg->mfma_min_products < 0)
+        return fail(TSG_ERR_ARG, "var_mask must lie in [0, 31], rg_density and mfma_min_products be >= 0");
     TuneCfg t;
     t.var_mask = cfg->var_mask;
     t.rg_density = cfg->rg_density;

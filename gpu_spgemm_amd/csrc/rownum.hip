@@ -202,6 +202,246 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
     }
 }
 
+// ---------------------------------------------------------------- symbolic step of bin R
+// The staged symbolic kernel runs a wave per tile-row; a tile-row that fits
+// R's caps (8 A tiles, 32 B tiles, 16 C tiles) leaves most of its lanes idle
+// there.  k_sym_rows gives such a tile-row 16 lanes (4 tile-rows a wave):
+// two B tiles a lane for the C tile columns (a 1024-column LDS bitmap, ranks
+// by popcount prefix), then lane r builds row r of every C tile -- the OR of
+// B rows k selected by A's row mask r, as step2_symbolic_kernel
+// (src/spgemm_raw_kernels.cuh:105-163) -- with no atomics, and the per-tile
+// row offsets and counts by a 16-lane scan (step2_build_ptrRowC_kernel,
+// :166-215).
+constexpr int SR_NA = 8, SR_NB = 32, SR_NC = 16;  // R's caps
+constexpr int SR_BPL = SR_NB / 16;                 // B tiles a lane
+constexpr int SR_WORDS = 32;  // bitmap: 1024 tile columns (a 2-D 5-point row spans 513)
+
+// A group's arrays; the bitmap and the fallback's arrays share their space
+// (a group takes one of the two ways).
+struct SymRowsGroup {
+    alignas(16) uint16_t bmask[SR_NB * 16];
+    uint16_t cmask[SR_NC * 16];
+    union {
+        uint32_t bm[SR_WORDS];  // bitmap way
+        int32_t bj[SR_NB];      // fallback: the B tiles' columns
+    };
+    int32_t abs_[SR_NA];
+    int32_t aseg[SR_NA + 1];
+    union {
+        uint8_t wpre[SR_WORDS];  // bitmap way: distinct columns before each word (at most 32)
+        uint8_t first[SR_NB];    // fallback: first occurrence of a column
+    };
+    uint8_t bct[SR_NB];
+};
+struct SymRowsLDS {
+    SymRowsGroup q[RN_GROUPS];
+};
+
+// Maximum over a 16-lane group (one DPP row), returned to every lane of it.
+__device__ __forceinline__ int group16_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xF, 0xF, false));
+    return __shfl(v, 15, 16);
+}
+
+// Group sizes are read and combined with every lane active; `ok` only gates
+// memory traffic, so the DPP row operations always see their whole row.
+__global__ void __launch_bounds__(WAVE)
+k_sym_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, const int64_t *__restrict__ dlo,
+           const int64_t *__restrict__ dhi, const int64_t *__restrict__ tile_ptrA,
+           const uint16_t *__restrict__ maskA, const int4 *__restrict__ link, const int32_t *__restrict__ colidxB,
+           const uint16_t *__restrict__ csr_maskB, const int64_t *__restrict__ tile_ptrC,
+           int32_t *__restrict__ rowidxC, int32_t *__restrict__ colidxC, uint16_t *__restrict__ maskC,
+           uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC, int32_t *__restrict__ overflow) {
+    __shared__ SymRowsLDS s;
+    const int lane = lane_id();
+    const int g = lane >> 4, l = lane & 15;
+    SymRowsGroup &G = s.q[g];
+    const int64_t idx = (int64_t)blockIdx.x * RN_GROUPS + g;
+    const int32_t bi = bin_row(rows, row_begin, nrows, dlo, dhi, idx);
+    int64_t as = 0, ae = 0, cs = 0, ce = 0;
+    if (bi >= 0) {
+        as = tile_ptrA[bi];
+        ae = tile_ptrA[bi + 1];
+        cs = tile_ptrC[bi];
+        ce = tile_ptrC[bi + 1];
+    }
+    const int na = (int)(ae - as), cap = (int)(ce - cs);
+    bool ok = bi >= 0 && na <= SR_NA && cap <= SR_NC;
+    if (bi >= 0 && !ok && l == 0) atomicOr(overflow, 1);
+    // A tiles: lane l loads A tile l's link, and row l of every A tile's
+    // mask (the only row it needs later)
+    int4 lk = make_int4(0, 0, 0, 0);
+    uint32_t am[SR_NA];
+#pragma unroll
+    for (int a = 0; a < SR_NA; a++) am[a] = (ok && a < na) ? maskA[(as + a) * 16 + l] : 0u;
+    if (ok && l < na) {
+        lk = link[as + l];
+        G.abs_[l] = lk.x;
+    }
+    const int incl = group16_incl_sum(lk.y);
+    const int nB = __shfl(incl, 15, 16);
+    if (ok && nB > SR_NB) {
+        if (l == 0) atomicOr(overflow, 1);
+        ok = false;
+    }
+    if (ok && l < na) G.aseg[l + 1] = incl;
+    if (ok && l == 0) G.aseg[0] = 0;
+    wave_sync();
+    // B tiles t = l + 16 h: tile column and row masks
+    int32_t jv[SR_BPL] = {};
+    int jlo = -INT_MAX, jhi = -1;  // jlo holds the negated minimum
+#pragma unroll
+    for (int h = 0; h < SR_BPL; h++) {
+        const int t = l + 16 * h;
+        if (ok && t < nB) {
+            const int a = owner_of(G.aseg, na, t);
+            const int64_t q = (int64_t)G.abs_[a] + (t - G.aseg[a]);
+            jv[h] = colidxB[q];
+            uint4 m0, m1;
+            load_mask16(csr_maskB, q, m0, m1);
+            store_mask16(&G.bmask[t * 16], m0, m1);
+            jlo = max(jlo, -jv[h]);
+            jhi = max(jhi, jv[h]);
+        }
+    }
+    jlo = -group16_max(jlo);
+    jhi = group16_max(jhi);
+    // distinct tile columns and their ranks
+    const bool bitmap = ok && jhi - jlo < SR_WORDS * 32;  // uniform in the group
+    int rk[SR_BPL] = {};
+    int nc = 0;
+    if (bitmap)
+        for (int i = l; i < SR_WORDS; i += 16) G.bm[i] = 0u;
+    wave_sync();
+    if (bitmap) {
+#pragma unroll
+        for (int h = 0; h < SR_BPL; h++) {
+            const int t = l + 16 * h;
+            if (t < nB) atomicOr(&G.bm[(jv[h] - jlo) >> 5], 1u << ((jv[h] - jlo) & 31));
+        }
+    }
+    wave_sync();
+    uint32_t wd[SR_WORDS / 16];
+    int pc = 0;
+#pragma unroll
+    for (int i = 0; i < SR_WORDS / 16; i++) {
+        wd[i] = bitmap ? G.bm[l * (SR_WORDS / 16) + i] : 0u;
+        pc += __popc(wd[i]);
+    }
+    const int winc = group16_incl_sum(pc);
+    if (bitmap) {
+        nc = __shfl(winc, 15, 16);
+        int pre = winc - pc;
+#pragma unroll
+        for (int i = 0; i < SR_WORDS / 16; i++) {
+            G.wpre[l * (SR_WORDS / 16) + i] = (uint8_t)pre;
+            pre += __popc(wd[i]);
+        }
+    } else if (ok) {  // columns span more than the bitmap: the columns in LDS
+#pragma unroll
+        for (int h = 0; h < SR_BPL; h++)
+            if (l + 16 * h < nB) G.bj[l + 16 * h] = jv[h];
+    }
+    wave_sync();
+    if (ok && !bitmap) {  // first occurrences, then ranks by counting
+#pragma unroll
+        for (int h = 0; h < SR_BPL; h++) {
+            const int t = l + 16 * h;
+            if (t < nB) {
+                bool f = true;
+                for (int u = 0; u < t; u++) f = f && G.bj[u] != jv[h];
+                G.first[t] = f ? 1 : 0;
+            }
+        }
+    }
+    wave_sync();
+    if (ok) {
+#pragma unroll
+        for (int h = 0; h < SR_BPL; h++) {
+            const int t = l + 16 * h;
+            if (t >= nB) continue;
+            if (bitmap) {
+                const int off = jv[h] - jlo;
+                rk[h] = G.wpre[off >> 5] + __popc(G.bm[off >> 5] & ((1u << (off & 31)) - 1u));
+            } else {
+                int r = 0;
+                for (int u = 0; u < nB; u++) r += (G.first[u] && G.bj[u] < jv[h]) ? 1 : 0;
+                rk[h] = r;
+            }
+        }
+        if (!bitmap) {
+            int n = 0;
+            for (int u = 0; u < nB; u++) n += G.first[u];
+            nc = n;
+        }
+    }
+    if (ok && nc > cap) {  // cannot happen on the exact layout: the count said cap
+        if (l == 0) atomicOr(overflow, 1);
+        ok = false;
+    }
+    if (ok) {
+#pragma unroll
+        for (int h = 0; h < SR_BPL; h++) {
+            const int t = l + 16 * h;
+            if (t < nB) {  // duplicates write the same column at the same rank
+                colidxC[cs + rk[h]] = jv[h];
+                rowidxC[cs + rk[h]] = bi;
+                G.bct[t] = (uint8_t)rk[h];
+            }
+        }
+        for (int ct = 0; ct < nc; ct++) G.cmask[ct * 16 + l] = 0;
+    }
+    wave_sync();
+    // row l of every C tile: OR of B rows k over A's row mask l (no other
+    // lane writes it).  A tile by A tile: its row mask l is held in a
+    // register and its B tiles are the contiguous range aseg[a] .. aseg[a+1].
+    if (ok) {
+#pragma unroll
+        for (int a = 0; a < SR_NA; a++) {
+            if (a >= na) break;
+            const uint32_t ma0 = am[a];
+            if (!ma0) continue;  // row l of this A tile is empty
+            const int b1 = G.aseg[a + 1];
+            for (int b = G.aseg[a]; b < b1; b++) {
+                const int ct = G.bct[b];
+                uint32_t ma = ma0, rm = 0;
+                while (ma) {
+                    const int kk = __clz(ma) - 16;
+                    ma ^= 0x8000u >> kk;
+                    rm |= G.bmask[b * 16 + kk];
+                }
+                if (rm) G.cmask[ct * 16 + l] |= (uint16_t)rm;
+            }
+        }
+    }
+    // masks, row offsets (a 16-lane scan per C tile) and tile counts
+    if (!ok) nc = 0;
+    for (int ct = 0; ct < nc; ct++) {  // uniform in the group
+        const uint32_t rm = G.cmask[ct * 16 + l];
+        maskC[(cs + ct) * 16 + l] = (uint16_t)rm;
+        const int p = __popc(rm);
+        const int inc = group16_incl_sum(p);
+        ptrRowC[(cs + ct) * 16 + l] = (uint8_t)(inc - p);
+        if (l == 15) tile_nnzC[cs + ct + 1] = inc;
+    }
+}
+
+int launch_sym_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                    const int32_t *rows, const BinRange &br, const int4 *link, const SlotLayout &sl) {
+    const int64_t count = br.count;
+    if (count <= 0) return TSG_OK;
+    if (sl.row_nc) return fail(TSG_ERR_ARG, "the row-lane symbolic kernel runs on the exact C layout only");
+    const unsigned grid = (unsigned)((count + RN_GROUPS - 1) / RN_GROUPS);
+    k_sym_rows<<<grid, WAVE, 0, ctx->stream>>>(rows, br.begin, count, br.lo, br.hi, A->tile_ptr, A->mask, link,
+                                               B->tile_columnidx, B->csr_mask, C->tile_ptr, C->tile_rowidx,
+                                               C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, sl.overflow);
+    TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
 bool num_rows_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B) {
     return A->row_ptr && A->row_col && A->row_val && B->row_ptr && B->row_col && B->row_val &&
            !((A->flags | B->flags) & 1);

@@ -7,6 +7,29 @@
 
 namespace tsg {
 
+// Index of the segment holding i: largest q < n with seg[q] <= i (seg ascending).
+__device__ __forceinline__ int owner_of(const int32_t *seg, int n, int32_t i) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (seg[mid] <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// 16-byte loads of a tile's 16 row pointers (uint8) and 16 row masks (uint16).
+__device__ __forceinline__ uint4 load_ptr16(const uint8_t *p, int64_t tile) {
+    return reinterpret_cast<const uint4 *>(p)[tile];
+}
+__device__ __forceinline__ void load_mask16(const uint16_t *m, int64_t tile, uint4 &lo, uint4 &hi) {
+    lo = reinterpret_cast<const uint4 *>(m)[tile * 2];
+    hi = reinterpret_cast<const uint4 *>(m)[tile * 2 + 1];
+}
+__device__ __forceinline__ void store_mask16(uint16_t *dst, const uint4 &lo, const uint4 &hi) {
+    reinterpret_cast<uint4 *>(dst)[0] = lo;
+    reinterpret_cast<uint4 *>(dst)[1] = hi;
+}
+
 constexpr int MAX_VAR = 4;               // staged variants
 constexpr int MAX_BINS = MAX_VAR + 2;    // + C-tile kernels + row-by-row kernels
 
@@ -26,6 +49,7 @@ struct BinTable {
     int32_t nvar;
     int32_t rg;          // 1: the row-by-row bin exists
     int32_t rg_density;  // A entries per A tile at or below which a heavy row goes row-by-row
+    int32_t rl_any;      // 1: bin R runs its own kernels whatever its size (var_mask bit 4)
 };
 constexpr int RG_DENSITY_DEFAULT = 64;
 
@@ -69,6 +93,7 @@ struct Plan {
     // the bin's range from there.  Null: host-known bins.
     const int64_t *d_bins = nullptr;
     int64_t bin_cap[MAX_BINS] = {};
+    bool rl_any = false;  // BinTable::rl_any
     // device address of the capacity flag (kernels take it as int32_t *)
     int32_t *overflow() const { return flags ? &flags->cap : nullptr; }
 };
@@ -79,7 +104,8 @@ bool rg_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_options
 // The row-lane numeric kernel (rownum.hip): needs both operands' scalar CSR and no repeated columns.
 bool num_rows_ok(const tsg_tile_matrix *A, const tsg_tile_matrix *B);
 // var_mask bits 0-2 admit staged variants S, W, M and bit 3 variant R (a lane
-// per scalar row; only when row_lanes); rg_density 0 disables the row-by-row bin.
+// per scalar row; only when row_lanes); bit 4 gives R's rows their own kernels
+// however few they are; rg_density 0 disables the row-by-row bin.
 BinTable bin_table(bool ok, bool rg = false, int rg_density = RG_DENSITY_DEFAULT, int var_mask = 15,
                    bool row_lanes = false);
 // The bin table of a product under its options and the context's tuned configuration.
@@ -125,6 +151,9 @@ int prepare_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matri
 void free_c(Context *ctx, tsg_tile_matrix *C);
 int launch_num_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                     const int32_t *rows, const BinRange &br, const SlotLayout &sl);
+// Symbolic step of the row-lane bin (rownum.hip): 16 lanes per tile-row.
+int launch_sym_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
+                    const int32_t *rows, const BinRange &br, const int4 *link, const SlotLayout &sl);
 BinRange bin_range(const Plan &p, int b, int e);
 
 // Capacities of a device-sized product (a captured graph): C's tile and value

@@ -40,10 +40,11 @@ struct Variant {
 using VarS = Variant<8, 16, 32, 128, 1 << 20, 4>;
 using VarW = Variant<32, 16, 32, 512, 1 << 20, 2>;
 using VarM = Variant<32, 48, 64, 512, 1 << 20, 2>;
-// R: rows of VarS whose scalar rows make about 32 products or fewer; the
-// symbolic step is VarS's, the numeric step a lane per scalar row (rownum.hip),
-// which wins on short rows (2-D 5-point stencils: 25) and loses on long ones
-// (3-D 7-point: 49).
+// R: rows of VarS whose scalar rows make about 32 products or fewer; both
+// steps run 16 lanes per tile-row (rownum.hip): the symbolic step k_sym_rows,
+// the numeric step a lane per scalar row, which wins on short rows (2-D
+// 5-point stencils: 25) and loses on long ones (3-D 7-point: 49).  (Fewer
+// than ROW_LANES_MIN such rows take VarS's kernels.)
 constexpr StageCaps CAPS_R{8, 16, 32, 128, 1 << 20, 32};
 constexpr int NUM_STAGED = 4;  // bins 0..3: R, S, W, M
 constexpr int64_t ROW_LANES_MIN = 32768;  // tile-rows below which R's rows take VarS's numeric kernel
@@ -55,14 +56,6 @@ static StageCaps caps_of() {
 }
 
 // Largest t in [0,n) with seg[t] <= i (seg ascending, seg[0] = 0).
-__device__ __forceinline__ int owner_of(const int32_t *seg, int n, int32_t i) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (seg[mid] <= i) lo = mid; else hi = mid - 1;
-    }
-    return lo;
-}
 
 // Bits of a 16-bit row mask that stand for columns below c (bit 15-c is column c).
 __device__ __forceinline__ uint32_t below(int c) { return (0xFFFF0000u >> c) & 0xFFFFu; }
@@ -343,18 +336,6 @@ __global__ void k_bin_header(const int64_t *__restrict__ counts, int nbins, int6
 }
 
 // ---------------------------------------------------------------- wave helpers
-// 16-byte loads of a tile's 16 row pointers (uint8) and 16 row masks (uint16).
-__device__ __forceinline__ uint4 load_ptr16(const uint8_t *p, int64_t tile) {
-    return reinterpret_cast<const uint4 *>(p)[tile];
-}
-__device__ __forceinline__ void load_mask16(const uint16_t *m, int64_t tile, uint4 &lo, uint4 &hi) {
-    lo = reinterpret_cast<const uint4 *>(m)[tile * 2];
-    hi = reinterpret_cast<const uint4 *>(m)[tile * 2 + 1];
-}
-__device__ __forceinline__ void store_mask16(uint16_t *dst, const uint4 &lo, const uint4 &hi) {
-    reinterpret_cast<uint4 *>(dst)[0] = lo;
-    reinterpret_cast<uint4 *>(dst)[1] = hi;
-}
 
 // ---------------------------------------------------------------- staged symbolic (fill + masks)
 template <typename Var>
@@ -773,6 +754,7 @@ BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask, bool row_lane
     t.nvar = 0;
     t.rg = (rg && rg_density > 0) ? 1 : 0;
     t.rg_density = rg_density;
+    t.rl_any = (var_mask & 16) ? 1 : 0;
     if (ok) {
         // bins in first-fit order R, S, W, M; var_mask bits 0-2 = S, W, M, bit 3 = R
         const StageCaps none{-1, -1, -1, -1, -1};
@@ -810,6 +792,7 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
     const bool ok = bt.nvar > 0;
     p.nvar = bt.nvar;
     p.rg = bt.rg != 0;
+    p.rl_any = bt.rl_any != 0;
     p.nbins = bt.nvar + 1 + (p.rg ? 1 : 0);
     TSG_CHECK(palloc(ctx, &p.rows, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.bin, (size_t)tilem));
@@ -936,6 +919,13 @@ static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
     return TSG_OK;
 }
 
+// Whether bin R's rows run the row-lane kernels: many of them (a launch of
+// their own pays), var_mask bit 4, or a captured graph (sizes unknown); else
+// they join S's launches (they fit VarS's caps).
+static bool rows_own(const Plan &p) {
+    return p.d_bins || p.rl_any || p.bin_off[1] - p.bin_off[0] >= ROW_LANES_MIN;
+}
+
 // The rows of bins [b, e): host offsets, or in a captured graph the device
 // offsets with the bins' capacities as the count the launches cover.
 BinRange bin_range(const Plan &p, int b, int e) {
@@ -957,7 +947,9 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
                  const Plan &p, const SlotLayout &sl) {
     const int fb = p.nvar;  // C-tile kernels
     if (p.nvar > 0) {
-        TSG_CHECK(launch_sym_staged<VarS>(ctx, A, B, C, p, 0, sl, 2));  // R's rows fit VarS's symbolic caps
+        const bool rn = rows_own(p);
+        if (rn) TSG_CHECK(launch_sym_rows(ctx, A, B, C, p.rows, bin_range(p, 0, 1), p.link, sl));
+        TSG_CHECK(launch_sym_staged<VarS>(ctx, A, B, C, p, rn ? 1 : 0, sl, 2));
         TSG_CHECK(launch_sym_staged<VarW>(ctx, A, B, C, p, 2, sl));
         TSG_CHECK(launch_sym_staged<VarM>(ctx, A, B, C, p, 3, sl));
     }
@@ -979,7 +971,7 @@ int numeric_kernel_mask(const Plan &p) {
     int m = 0;
     const int fb = p.nvar;
     if (p.nvar > 0) {
-        const bool rn = rows(0, 1) >= ROW_LANES_MIN;
+        const bool rn = rows_own(p);
         if (rn) m |= 1;
         if (rows(rn ? 1 : 0, 2) > 0) m |= 2;
         if (rows(2, 3) > 0) m |= 4;
@@ -994,10 +986,7 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
                 const Plan &p, const SlotLayout &sl) {
     const int fb = p.nvar;
     if (p.nvar > 0) {
-        // R's numeric kernel pays for its own launch only on many rows; fewer
-        // go to VarS's with S's rows (they fit its caps)
-        // (a captured graph does not know the bin's size: there always its own kernel)
-        const bool rn = p.d_bins || p.bin_off[1] - p.bin_off[0] >= ROW_LANES_MIN;
+        const bool rn = rows_own(p);
         if (rn) TSG_CHECK(launch_num_rows(ctx, A, B, C, p.rows, bin_range(p, 0, 1), sl));
         const int s0 = rn ? 1 : 0;
         if (A->dtype == TSG_F32) {

@@ -360,8 +360,9 @@ int tsg_learned_gradient(tsg_context *ctx, const float *X, const float *T, int64
 /* ---- Autotuner (no reference counterpart) ----
  * A configuration chooses which row-staged kernel variants may take rows
  * (var_mask bit v: variant v, v = 0 small, 1 wide, 2 many-tiles, 3 small
- * tile-rows of short scalar rows whose numeric step runs a lane per scalar
- * row; the default admits all four) and the A
+ * tile-rows of short scalar rows whose steps run 16 lanes per tile-row and a
+ * lane per scalar row -- with fewer than 32,768 such tile-rows they run
+ * variant 0's kernels unless bit 4 is set; the default admits all four) and the A
  * tile density (entries per A tile) at or below which a heavy tile-row runs
  * the row-by-row kernels (rg_density; 0 = never), or, with
  * mfma_min_products > 0, the C-tile kernels with matrix cores for every tile

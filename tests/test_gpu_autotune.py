@@ -17,7 +17,7 @@ def mixed():
     return g.gen_rmat(11, 16, seed=4, dtype=g.F64)
 
 
-@pytest.mark.parametrize("var_mask", [15, 8, 9, 7, 5, 1, 4, 2, 0])
+@pytest.mark.parametrize("var_mask", [15, 31, 24, 8, 9, 7, 5, 1, 4, 2, 0])
 @pytest.mark.parametrize("rg_density", [0, 2, 8, 1000])
 def test_every_configuration_gives_identical_bits(ctx, var_mask, rg_density):
     A = mixed()

@@ -302,12 +302,15 @@ def _short_rows_wide_b(n=4096, seed=3):
     return A, B
 
 
-@pytest.mark.parametrize("var_mask", [None, 8])
+@pytest.mark.parametrize("var_mask", [None, 24, 31])
 @pytest.mark.parametrize("f32", [False, True])
 def test_row_lane_variant_with_windows(ctx, var_mask, f32):
-    """Staged variant R (a lane per scalar row, C values staged in LDS windows)
-    on tile-rows needing several windows, bit for bit against the tiled
-    restatement; var_mask 8 leaves R as the only staged variant."""
+    """Staged variant R (16 lanes a tile-row for the symbolic step, a lane per
+    scalar row with C values staged in LDS windows for the numeric step) on
+    tile-rows needing several windows, bit for bit against the tiled
+    restatement.  var_mask bit 4 gives R's rows their own kernels at this size
+    (24: R the only staged variant; 31: every variant); None: R's few rows run
+    VarS's kernels."""
     A, B = _short_rows_wide_b()
     print(f"[case] row-lane windows var_mask={var_mask} f32={f32}", flush=True)
     H, HB = (A.astype(g.F32), B.astype(g.F32)) if f32 else (A, B)
@@ -320,4 +323,28 @@ def test_row_lane_variant_with_windows(ctx, var_mask, f32):
     finally:
         ctx.autotune_clear()
     assert st["nnzC"] / max(1, tA.s.tilem) > 1000      # several 256-value windows a tile-row
+    if var_mask is not None:
+        assert st["numeric_kernels"] & 1, st["numeric_kernels"]   # k_num_rows ran
+    else:
+        assert not st["numeric_kernels"] & 1, st["numeric_kernels"]
     assert_tiles_equal(got, O.spgemm_tiled(A, B, f32=f32), keys=CKEYS, label="C ")
+
+
+@pytest.mark.parametrize("f32", [False, True])
+def test_row_lane_kernels_on_wide_columns(ctx, f32):
+    """A 2-D Laplacian 8192 points wide: every tile-row fits variant R but its
+    C tile columns span more than 1024, so the row-lane symbolic kernel ranks
+    them by counting instead of its bitmap; bit for bit against the tiled
+    restatement."""
+    A = g.gen_laplace2d(8192, 8)
+    print(f"[case] row-lane wide columns f32={f32}", flush=True)
+    H = A.astype(g.F32) if f32 else A
+    tA, tB = ctx.tiles_from_csr(H), ctx.tiles_from_csr(H, as_b=True)
+    ctx.autotune_set(tA, tB, 24, 64)
+    try:
+        C, st = ctx.spgemm(tA, tB)
+        got = C.download()
+    finally:
+        ctx.autotune_clear()
+    assert st["numeric_kernels"] == 1, st["numeric_kernels"]   # k_num_rows only: every row in R
+    assert_tiles_equal(got, O.spgemm_tiled(A, A, f32=f32), keys=CKEYS, label="C ")
