@@ -237,15 +237,6 @@ struct SymRowsLDS {
     SymRowsGroup q[RN_GROUPS];
 };
 
-// Maximum over a 16-lane group (one DPP row), returned to every lane of it.
-__device__ __forceinline__ int group16_max(int v) {
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xF, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xF, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xF, 0xF, false));
-    return __shfl(v, 15, 16);
-}
-
 // Group sizes are read and combined with every lane active; `ok` only gates
 // memory traffic, so the DPP row operations always see their whole row.
 __global__ void __launch_bounds__(WAVE)

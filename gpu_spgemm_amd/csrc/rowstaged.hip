@@ -96,6 +96,45 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t key) {
     return key;
 }
 
+// One tile-row's Step-1 results: its C tile count into tile_ptrC (scanned
+// later), its bin (the first staged variant whose capacities hold it, else
+// the row-by-row or the C-tile kernels) and, for the learned scheduler, its
+// statistics (heavy rows' count comes from k_heavy_structure).
+__device__ __forceinline__ void row_result(int32_t bi, int64_t na, int64_t nb, int64_t pb, int64_t count, int32_t cmin,
+                                           int32_t cmax, int64_t f_maxlen, int64_t f_sumspan, int64_t f_nonempty,
+                                           const int64_t *__restrict__ tile_ptrA,
+                                           const int64_t *__restrict__ tile_nnzA, const BinTable &bt,
+                                           int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin,
+                                           int32_t *__restrict__ stats) {
+    tile_ptrC[bi + 1] = count;
+    if (bi == 0) tile_ptrC[0] = 0;
+    const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
+    const int64_t pa = tile_nnzA[ae] - tile_nnzA[as];
+    int b = bt.nvar;
+    for (int v = 0; v < bt.nvar; v++) {
+        const StageCaps c = bt.v[v];
+        if (na <= c.na && count <= c.nc && nb <= c.nb && pa <= c.pa && pb <= c.pb &&
+            (c.prow <= 0 || pa * pb <= (int64_t)c.prow * 256 * na)) {
+            b = v;
+            break;
+        }
+    }
+    if (b == bt.nvar && bt.rg && pa <= (int64_t)bt.rg_density * na) b = bt.nvar + 1;
+    bin[bi] = (uint8_t)b;
+    if (stats) {
+        int32_t *st = stats + (int64_t)bi * RS_NSTAT;
+        st[RS_NA] = (int32_t)min<int64_t>(na, INT_MAX);
+        st[RS_PA] = (int32_t)min<int64_t>(pa, INT_MAX);
+        st[RS_NB] = (int32_t)min<int64_t>(nb, INT_MAX);
+        st[RS_PB] = (int32_t)min<int64_t>(pb, INT_MAX);
+        st[RS_COUNT] = (int32_t)min<int64_t>(count, INT_MAX);
+        st[RS_SPAN] = cmax >= cmin ? cmax - cmin + 1 : 0;
+        st[RS_MAXLEN] = (int32_t)min<int64_t>(f_maxlen, INT_MAX);
+        st[RS_SUMSPAN] = (int32_t)min<int64_t>(f_sumspan, INT_MAX);
+        st[RS_NONEMPTY] = (int32_t)min<int64_t>(f_nonempty, INT_MAX);
+    }
+}
+
 // Exact C tile count of each tile-row (as step1_count_kernel,
 // src/spgemm_raw_kernels.cuh:23-57) plus the sizes that decide its bin:
 // A tiles, A payload, B tiles and B payload reached, C tiles.  Rows reaching
@@ -106,13 +145,16 @@ __global__ void __launch_bounds__(RS_WAVES *WAVE)
 k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
             int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats,
-            int32_t *__restrict__ hlist, int32_t *__restrict__ hcount) {
+            int32_t *__restrict__ hlist, int32_t *__restrict__ hcount, const int32_t *__restrict__ list,
+            const int32_t *__restrict__ list_count) {
     __shared__ uint32_t s_bm[RS_WAVES][RS_WIN_WORDS];
     __shared__ int32_t s_bs[RS_WAVES][WAVE];
     __shared__ int32_t s_pre[RS_WAVES][WAVE + 1];
     const int w = threadIdx.x / WAVE, lane = lane_id();
-    const int32_t bi = blockIdx.x * RS_WAVES + w;
-    if (bi >= tilemA) return;
+    const int32_t wi = blockIdx.x * RS_WAVES + w;
+    // every tile-row, or the ones k_row_light listed
+    if (wi >= (list ? *list_count : tilemA)) return;
+    const int32_t bi = list ? list[wi] : wi;
     uint32_t *bm = s_bm[w];
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
     const int64_t na = ae - as;
@@ -231,33 +273,95 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
             wave_sync();
         }
     }
-    if (lane == 0) {
-        tile_ptrC[bi + 1] = count;
-        if (bi == 0) tile_ptrC[0] = 0;
-        const int64_t pa = tile_nnzA[ae] - tile_nnzA[as];
-        int b = bt.nvar;
-        for (int v = 0; v < bt.nvar; v++) {
-            const StageCaps c = bt.v[v];
-            if (na <= c.na && count <= c.nc && nb <= c.nb && pa <= c.pa && pb <= c.pb &&
-                (c.prow <= 0 || pa * pb <= (int64_t)c.prow * 256 * na)) {
-                b = v;
-                break;
-            }
+    if (lane == 0)
+        row_result(bi, na, nb, pb, count, cmin, cmax, f_maxlen, f_sumspan, f_nonempty, tile_ptrA, tile_nnzA, bt,
+                   tile_ptrC, bin, stats);
+}
+
+// Step-1 count of light tile-rows (at most 16 A tiles, 32 B tiles reached,
+// C tile columns spanning less than 1024) at 16 lanes each, 4 a wave: the
+// wave-per-row kernel above leaves most of its lanes idle on them.  Each other
+// row goes to `list` for that kernel.  Results and statistics are the ones the
+// wave-per-row kernel computes.
+constexpr int RL_NA = 16, RL_NB = 32, RL_WORDS = 32;
+__global__ void __launch_bounds__(WAVE)
+k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
+            const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
+            int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats,
+            int32_t *__restrict__ list, int32_t *__restrict__ list_count) {
+    __shared__ uint32_t s_bm[4][RL_WORDS];
+    __shared__ int32_t s_bs[4][RL_NA];
+    __shared__ int32_t s_pre[4][RL_NA + 1];
+    const int lane = lane_id(), g = lane >> 4, l = lane & 15;
+    const int32_t bi = blockIdx.x * 4 + g;
+    const bool row = bi < tilemA;
+    int64_t as = 0, ae = 0;
+    if (row) {
+        as = tile_ptrA[bi];
+        ae = tile_ptrA[bi + 1];
+    }
+    const int64_t na = ae - as;
+    bool light = row && na <= RL_NA;
+    // every lane of the group goes through the group operations; `light`
+    // (uniform in the group) gates memory traffic
+    int4 lk = make_int4(0, 0, 0, 0);
+    if (light && l < na) lk = link[as + l];
+    const int incl = group16_incl_sum(lk.y);
+    const int nb = __shfl(incl, 15, 16);
+    const int64_t pbi = group16_incl_sum<int64_t>((int64_t)lk.z);
+    const int64_t pb = __shfl(pbi, 15, 16);
+    light = light && nb <= RL_NB;
+    if (light && l < na) {
+        s_pre[g][l + 1] = incl;
+        s_bs[g][l] = lk.x;
+    }
+    if (light && l == 0) s_pre[g][0] = 0;
+    wave_sync();
+    int32_t j[2] = {0, 0};
+    int64_t sumspan = 0;
+    int32_t cmn = -INT_MAX, cmx = -1;  // cmn: the negated minimum
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int t = l + 16 * h;
+        if (light && t < nb) {
+            const int a = owner_of(s_pre[g], (int)na, t);
+            const int pos = t - s_pre[g][a];
+            j[h] = colidxB[s_bs[g][a] + pos];
+            // each A tile's span: last - first + 1 of its B tile-row's columns
+            if (t == s_pre[g][a + 1] - 1) sumspan += j[h];
+            if (pos == 0) sumspan -= (int64_t)j[h] - 1;
+            cmn = max(cmn, -j[h]);
+            cmx = max(cmx, j[h]);
         }
-        if (b == bt.nvar && bt.rg && pa <= (int64_t)bt.rg_density * na) b = bt.nvar + 1;
-        bin[bi] = (uint8_t)b;
-        if (stats) {  // per-row features for the learned scheduler (heavy rows' count: k_heavy_structure)
-            int32_t *st = stats + (int64_t)bi * RS_NSTAT;
-            st[RS_NA] = (int32_t)min<int64_t>(na, INT_MAX);
-            st[RS_PA] = (int32_t)min<int64_t>(pa, INT_MAX);
-            st[RS_NB] = (int32_t)min<int64_t>(nb, INT_MAX);
-            st[RS_PB] = (int32_t)min<int64_t>(pb, INT_MAX);
-            st[RS_COUNT] = (int32_t)min<int64_t>(count, INT_MAX);
-            st[RS_SPAN] = cmax >= cmin ? cmax - cmin + 1 : 0;
-            st[RS_MAXLEN] = (int32_t)min<int64_t>(f_maxlen, INT_MAX);
-            st[RS_SUMSPAN] = (int32_t)min<int64_t>(f_sumspan, INT_MAX);
-            st[RS_NONEMPTY] = (int32_t)min<int64_t>(f_nonempty, INT_MAX);
-        }
+    }
+    const int32_t cmin = -group16_max(cmn), cmax = group16_max(cmx);
+    light = light && (nb == 0 || cmax - cmin < RL_WORDS * 32);
+    if (light)
+        for (int i = l; i < RL_WORDS; i += 16) s_bm[g][i] = 0u;
+    wave_sync();
+    if (light)
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            if (l + 16 * h < nb) atomicOr(&s_bm[g][(j[h] - cmin) >> 5], 1u << ((j[h] - cmin) & 31));
+    wave_sync();
+    int pc = 0;
+    if (light)
+#pragma unroll
+        for (int i = 0; i < RL_WORDS / 16; i++) pc += __popc(s_bm[g][l * (RL_WORDS / 16) + i]);
+    const int cnt = __shfl(group16_incl_sum(pc), 15, 16);
+    int64_t f_sumspan = 0, f_maxlen = 0, f_nonempty = 0;
+    if (stats) {
+        const int64_t ss = group16_incl_sum<int64_t>(sumspan);
+        f_sumspan = __shfl(ss, 15, 16);
+        f_maxlen = group16_max(l < na ? lk.y : 0);
+        f_nonempty = __shfl(group16_incl_sum((light && l < na && lk.y > 0) ? 1 : 0), 15, 16);
+    }
+    if (l == 0 && row) {
+        if (light)
+            row_result(bi, na, nb, pb, cnt, nb > 0 ? cmin : INT_MAX, nb > 0 ? cmax : -1, f_maxlen, f_sumspan,
+                       f_nonempty, tile_ptrA, tile_nnzA, bt, tile_ptrC, bin, stats);
+        else
+            list[atomicAdd(list_count, 1)] = bi;
     }
 }
 
@@ -866,12 +970,22 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
             if (!rc && (B->flags & 2)) rc = build_brow_bitmaps(ctx, B, p.hb);  // long B tile-rows exist
         }
         if (!rc) rc = zero_small(ctx, p.overflow(), PLAN_FLAG_WORDS, reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
+        // light tile-rows at 16 lanes each; the rest listed for a wave each
+        int32_t *rest = nullptr;
+        if (!rc) rc = palloc(ctx, &rest, (size_t)tilem);
+        if (!rc) {
+            k_row_light<<<(unsigned)((tilem + 3) / 4), WAVE, 0, ctx->stream>>>(
+                A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
+                rest, &p.flags->rest_rows);
+            if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_light launch");
+        }
         if (!rc) {
             k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
                 A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
-                hlist, hc);
+                hlist, hc, rest, &p.flags->rest_rows);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_stats launch");
         }
+        pool_free(ctx, rest);
         if (!rc && hlist) rc = launch_heavy_count(ctx, A, B, C->tile_ptr, hlist, hc, stats_out, p.hb);
         pool_free(ctx, hlist);
         TSG_CHECK(rc);

@@ -335,6 +335,15 @@ __device__ __forceinline__ int group16_incl_sum<int>(int v) {
     return v;
 }
 
+// Maximum over a 16-lane group (one DPP row), returned to every lane of it.
+__device__ __forceinline__ int group16_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xF, 0xF, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xF, 0xF, false));
+    return __shfl(v, 15, 16);
+}
+
 template <typename T>
 __device__ __forceinline__ T group16_sum(T v) {
 #pragma unroll
