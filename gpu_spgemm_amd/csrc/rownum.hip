@@ -22,8 +22,8 @@ namespace tsg {
 constexpr int RN_GROUPS = 4;   // tile-rows per wave; lane = (tile-row, scalar row)
 constexpr int RN_NC = 16;      // C tiles a tile-row may hold (the small staged variants' cap)
 constexpr int RN_VCAP = 256;   // C values of one tile-row held in LDS at a time
-constexpr int RN_EB = 8;       // A entries whose loads issue together
-constexpr int RN_BB = 8;       // B row entries whose loads issue together
+constexpr int RN_EB = 6;       // A entries whose loads issue together (a 2-D 5-point row: 5)
+constexpr int RN_BB = 6;       // B row entries whose loads issue together (a 2-D 5-point row: 5)
 
 template <typename V>
 struct RowNumLDS {
@@ -127,7 +127,8 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
                 const int ne = (int)min<int64_t>(RN_EB, e1 - eb);
                 int32_t kk[RN_EB];
                 V aa[RN_EB];
-                int64_t j0[RN_EB], j1[RN_EB];
+                int64_t j0[RN_EB];
+                int32_t jn[RN_EB];  // B row lengths, 0 past the A row's end
                 // loads past the row's end repeat its last entry (in bounds,
                 // no branch) and are then given an empty B row
 #pragma unroll
@@ -138,44 +139,66 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
                 }
 #pragma unroll
                 for (int u = 0; u < RN_EB; u++) {
-                    j0[u] = rpB[kk[u]];
-                    j1[u] = rpB[kk[u] + 1];
+                    const int64_t b0 = rpB[kk[u]], b1 = rpB[kk[u] + 1];
+                    j0[u] = b0;
+                    jn[u] = u < ne ? (int32_t)(b1 - b0) : 0;
                 }
+                // RN_BB entries of B row u from its entry o (o < jn[u]); past
+                // the row's end its last entry again, column INT_MAX
+                auto load_b = [&](int u, int o, int32_t(&c)[RN_BB], V(&b)[RN_BB]) {
+                    const int n = jn[u] - o;
 #pragma unroll
-                for (int u = 0; u < RN_EB; u++)
-                    if (u >= ne) j1[u] = j0[u];
+                    for (int q = 0; q < RN_BB; q++) {
+                        const int64_t j = j0[u] + o + min(q, n - 1);
+                        c[q] = colB[j];
+                        b[q] = valB[j];
+                        if (q >= n) c[q] = INT_MAX;
+                    }
+                };
+                // a batch's products into the window, in order; the current C
+                // tile's column, this row's first value in it and its row mask
+                // stay in registers while the B row stays in that tile
+                auto apply = [&](V a, const int32_t(&c)[RN_BB], const V(&b)[RN_BB], int &ct, int32_t &jc,
+                                 int32_t &base, uint32_t &mk) {
+#pragma unroll
+                    for (int q = 0; q < RN_BB; q++) {
+                        const int32_t J = c[q] >> 4;
+                        const bool in = J >= jlo && J <= jhi;  // inside the window
+                        if (in && J != jc) {
+                            while (s.cj[g][ct] < J) ct++;  // J is one of the tile-row's C tile columns
+                            const int t = ct * 16 + r;
+                            jc = J;
+                            base = s.coff[g][ct] - vb + s.cptr[g][t];
+                            mk = s.cmask[g][t];
+                        }
+                        // outside the window: the lane's own spare slot, never read
+                        const int pos = in ? base + __popc(mk & mask_below(c[q] & 15)) : RN_VCAP + r;
+                        s.val[g][pos] = fma_rn<V>(a, b[q], s.val[g][pos]);
+                    }
+                };
+                // the first batch of B row u + 1 is loaded before row u's
+                // products are applied (its loads overlap that work)
+                int32_t cc[RN_BB], cn[RN_BB];
+                V bv[RN_BB], bn[RN_BB];
+                if (jn[0] > 0) load_b(0, 0, cc, bv);
 #pragma unroll
                 for (int u = 0; u < RN_EB; u++) {
-                    // the current C tile's column, first value of this row in it and row
-                    // mask, kept in registers while the B row stays in that tile
-                    int ct = w0;
-                    int32_t jc = -1, base = 0;
-                    uint32_t mk = 0;
-                    for (int64_t jb = j0[u]; jb < j1[u]; jb += RN_BB) {
-                        const int nb = (int)min<int64_t>(RN_BB, j1[u] - jb);
-                        int32_t cc[RN_BB];
-                        V bv[RN_BB];
-#pragma unroll
-                        for (int q = 0; q < RN_BB; q++) {  // past the row's end: its last entry again, column INT_MAX
-                            const int64_t j = jb + min(q, nb - 1);
-                            cc[q] = colB[j];
-                            bv[q] = valB[j];
-                            if (q >= nb) cc[q] = INT_MAX;
+                    if (u + 1 < RN_EB && jn[u + 1] > 0) load_b(u + 1, 0, cn, bn);
+                    if (jn[u] > 0) {
+                        int ct = w0;
+                        int32_t jc = -1, base = 0;
+                        uint32_t mk = 0;
+                        apply(aa[u], cc, bv, ct, jc, base, mk);
+                        for (int o = RN_BB; o < jn[u]; o += RN_BB) {  // a B row longer than one batch
+                            load_b(u, o, cc, bv);
+                            apply(aa[u], cc, bv, ct, jc, base, mk);
                         }
+                    }
+                    if (u + 1 < RN_EB) {
 #pragma unroll
                         for (int q = 0; q < RN_BB; q++) {
-                            const int32_t J = cc[q] >> 4;
-                            const bool in = J >= jlo && J <= jhi;  // inside the window
-                            if (in && J != jc) {
-                                while (s.cj[g][ct] < J) ct++;  // J is one of the tile-row's C tile columns
-                                const int t = ct * 16 + r;
-                                jc = J;
-                                base = s.coff[g][ct] - vb + s.cptr[g][t];
-                                mk = s.cmask[g][t];
-                            }
-                            // outside the window: the lane's own spare slot, never read
-                            const int pos = in ? base + __popc(mk & mask_below(cc[q] & 15)) : RN_VCAP + r;
-                            s.val[g][pos] = fma_rn<V>(aa[u], bv[q], s.val[g][pos]);
+                            cc[q] = cn[q];
+                            bv[q] = bn[q];
                         }
                     }
                 }
