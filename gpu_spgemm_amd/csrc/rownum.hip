@@ -205,7 +205,8 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
             }
         }
         wave_sync();
-        // the window's values and col bytes: lane r writes its row's run in each tile
+        // the window's col bytes: lane r writes its row's run in each tile;
+        // its values: the group copies them 16 consecutive a step (coalesced)
         if (act) {
             for (int ct = w0; ct < w1; ct++) {
                 const int q = ct * 16 + r;
@@ -214,11 +215,11 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
                 while (mr) {
                     const int c = __clz(mr) - 16;
                     mr ^= 0x8000u >> c;
-                    valC[vs + o] = s.val[g][o - vb];
                     colC[vs + o] = (uint8_t)((r << 4) | c);
                     o++;
                 }
             }
+            for (int i = r; i < vn; i += 16) valC[vs + vb + i] = s.val[g][i];
         }
         wave_sync();
         w0 = w1;
