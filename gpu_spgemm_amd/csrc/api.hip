@@ -176,7 +176,13 @@ int tsg_create(int device, tsg_context **out) {
     if (e == hipSuccess) e = hipGetDeviceProperties(&ctx->prop, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void **)&ctx->pinned, 64 * sizeof(int64_t), hipHostMallocDefault);
+    if (e == hipSuccess)  // PUB_WORDS words + the 64-bit sequence number
+        e = hipHostMalloc((void **)&ctx->pub, sizeof(int32_t) * PUB_WORDS + sizeof(int64_t),
+                          hipHostMallocCoherent | hipHostMallocMapped);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void **)&ctx->pub_dev, ctx->pub, 0);
+    if (e == hipSuccess) memset(ctx->pub, 0, sizeof(int32_t) * PUB_WORDS + sizeof(int64_t));
     if (e != hipSuccess) {
+        if (ctx->pub) (void)hipHostFree(ctx->pub);
         if (ctx->pinned) (void)hipHostFree(ctx->pinned);
         if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
         delete ctx;
@@ -185,6 +191,7 @@ int tsg_create(int device, tsg_context **out) {
     ctx->stream = ctx->own_stream;
     const int rc = noalloc_setup(ctx);  // the no-alloc entry point's scratch on this device
     if (rc) {
+        (void)hipHostFree(ctx->pub);
         (void)hipHostFree(ctx->pinned);
         (void)hipStreamDestroy(ctx->own_stream);
         delete ctx;
@@ -203,6 +210,7 @@ int tsg_destroy(tsg_context *c) {
     tuner_release(ctx);
     for (auto &kv : ctx->pool.free_blocks) (void)hipFree(kv.second);
     for (auto &kv : ctx->pool.live) (void)hipFree(kv.first);
+    (void)hipHostFree(ctx->pub);
     (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;

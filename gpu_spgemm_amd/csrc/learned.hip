@@ -421,9 +421,7 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     if (!rc && cap > 0) rc = run_numeric(ctx, A, B, C, p, sl);
     // nnz(C), the capacity flag and the predicted sums in one read
     if (!rc) {
-        if (hipMemcpyAsync(ctx->pinned, p.flags, sizeof(PlanFlags), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            rc = fail(TSG_ERR_HIP, "reading the learned product's result failed");
+        rc = read_words(ctx, p.flags, (int)(sizeof(PlanFlags) / sizeof(int32_t)), ctx->pinned);
         syncs++;
         memcpy(&fl, ctx->pinned, sizeof fl);
         C->nnz = fl.nnz;

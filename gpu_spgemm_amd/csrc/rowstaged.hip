@@ -936,8 +936,7 @@ int plan_bin(Context *ctx, Plan &p, int32_t tilem, const int64_t *total0, const 
 }
 
 int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1) {
-    TSG_HIP(hipMemcpyAsync(ctx->pinned, p.hdr, sizeof(int64_t) * (p.nbins + 3), hipMemcpyDeviceToHost, ctx->stream));
-    TSG_HIP(hipStreamSynchronize(ctx->stream));
+    TSG_CHECK(read_words(ctx, p.hdr, 2 * (p.nbins + 3), ctx->pinned));
     if (total0) *total0 = ctx->pinned[0];
     if (total1) *total1 = ctx->pinned[1];
     for (int b = 0; b <= p.nbins; b++) p.bin_off[b] = ctx->pinned[2 + b];
@@ -1185,9 +1184,7 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     if (!rc) rc = prepare_numeric(ctx, A, C, p, exact);
     if (!rc) {
         // the capacity flag and nnzC in one read
-        if (hipMemcpyAsync(ctx->pinned, p.flags, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            rc = fail(TSG_ERR_HIP, "reading nnzC failed");
+        rc = read_words(ctx, p.flags, 4, ctx->pinned);
         syncs++;
         C->nnz = ctx->pinned[1];
         if (!rc && *reinterpret_cast<int32_t *>(ctx->pinned))

@@ -7,6 +7,8 @@
 // exclusive offsets plus the total in data[n-1].  64-bit throughout, and the
 // temporary partial sums come from the context pool (no hipMalloc/hipFree per
 // call as in the reference).
+#include <cstring>
+
 #include "tsg_internal.h"
 
 namespace tsg {
@@ -145,19 +147,42 @@ int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, 
     return rc;
 }
 
-int read_i64(Context *ctx, const int64_t *dptr, int64_t *out) {
-    TSG_HIP(hipMemcpyAsync(ctx->pinned, dptr, sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream));
-    TSG_HIP(hipStreamSynchronize(ctx->stream));
-    *out = ctx->pinned[0];
+// The words, then (after a system-scope fence by every writer and a barrier)
+// the sequence number: a host that sees the number sees the words.
+__global__ void __launch_bounds__(PUB_WORDS) k_publish(const int32_t *__restrict__ src, int n, int32_t *dst,
+                                                        int64_t seq) {
+    const int t = threadIdx.x;
+    if (t < n) dst[t] = src[t];
+    __threadfence_system();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(reinterpret_cast<int64_t *>(dst + PUB_WORDS), seq, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+int read_words(Context *ctx, const void *dsrc, int n, void *out) {
+    if (n < 0 || n > PUB_WORDS) return fail(TSG_ERR_ARG, "read_words: at most %d words", PUB_WORDS);
+    const int64_t seq = ++ctx->pub_seq;
+    k_publish<<<1, PUB_WORDS, 0, ctx->stream>>>(static_cast<const int32_t *>(dsrc), n, ctx->pub_dev, seq);
+    TSG_HIP(hipGetLastError());
+    const int64_t *flag = reinterpret_cast<const int64_t *>(ctx->pub + PUB_WORDS);
+    for (uint64_t it = 1;; it++) {
+        if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) break;
+        if ((it & 16383) == 0) {  // bounded: a stream that finished without the number is an error
+            const hipError_t q = hipStreamQuery(ctx->stream);
+            if (q == hipSuccess) {
+                if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) break;
+                return fail(TSG_ERR_HIP, "a device-to-host read did not arrive");
+            }
+            if (q != hipErrorNotReady) return fail(TSG_ERR_HIP, "device-to-host read: %s", hipGetErrorString(q));
+        }
+    }
+    memcpy(out, ctx->pub, sizeof(int32_t) * (size_t)n);
     return TSG_OK;
 }
 
-int read_i32(Context *ctx, const int32_t *dptr, int32_t *out) {
-    TSG_HIP(hipMemcpyAsync(ctx->pinned, dptr, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-    TSG_HIP(hipStreamSynchronize(ctx->stream));
-    *out = ((int32_t *)ctx->pinned)[0];
-    return TSG_OK;
-}
+int read_i64(Context *ctx, const int64_t *dptr, int64_t *out) { return read_words(ctx, dptr, 2, out); }
+
+int read_i32(Context *ctx, const int32_t *dptr, int32_t *out) { return read_words(ctx, dptr, 1, out); }
 
 }  // namespace tsg
 

@@ -81,6 +81,11 @@ struct Context {
     std::mutex mu;
     // small pinned host staging for scalar reads
     int64_t *pinned = nullptr;
+    // host reads of a few device words (read_words): coherent pinned slots the
+    // device writes with a sequence number the host waits for
+    int32_t *pub = nullptr;
+    int32_t *pub_dev = nullptr;
+    int64_t pub_seq = 0;
     // learned scheduler state lives in learned.hip
     void *learned = nullptr;
     // autotuner state lives in autotune.hip
@@ -133,6 +138,12 @@ int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first =
                        int64_t *total_out = nullptr, const int64_t *dlast = nullptr);
 // Zeroes a[0..na) and b[0..nb) (at most 64 each) with one small kernel on the context's stream.
 int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb);
+// Reads n 32-bit words (n <= PUB_WORDS) of device memory into out after the
+// work queued before it: one small kernel writes them to coherent pinned
+// memory with a sequence number, the host spins on that number (blocking;
+// counts as a host sync).  Replaces a device-to-host copy and a stream sync.
+constexpr int PUB_WORDS = 64;
+int read_words(Context *ctx, const void *dsrc, int n, void *out);
 // Reads one int64 from device memory (blocking; counts as a host sync).
 int read_i64(Context *ctx, const int64_t *dptr, int64_t *out);
 int read_i32(Context *ctx, const int32_t *dptr, int32_t *out);
