@@ -100,16 +100,13 @@ __device__ __forceinline__ uint64_t wave_sort64(uint64_t key) {
 // later), its bin (the first staged variant whose capacities hold it, else
 // the row-by-row or the C-tile kernels) and, for the learned scheduler, its
 // statistics (heavy rows' count comes from k_heavy_structure).
-__device__ __forceinline__ void row_result(int32_t bi, int64_t na, int64_t nb, int64_t pb, int64_t count, int32_t cmin,
-                                           int32_t cmax, int64_t f_maxlen, int64_t f_sumspan, int64_t f_nonempty,
-                                           const int64_t *__restrict__ tile_ptrA,
-                                           const int64_t *__restrict__ tile_nnzA, const BinTable &bt,
-                                           int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin,
-                                           int32_t *__restrict__ stats) {
+// pa: the tile-row's A entries.
+__device__ __forceinline__ void row_result(int32_t bi, int64_t na, int64_t pa, int64_t nb, int64_t pb, int64_t count,
+                                           int32_t cmin, int32_t cmax, int64_t f_maxlen, int64_t f_sumspan,
+                                           int64_t f_nonempty, const BinTable &bt, int64_t *__restrict__ tile_ptrC,
+                                           uint8_t *__restrict__ bin, int32_t *__restrict__ stats) {
     tile_ptrC[bi + 1] = count;
     if (bi == 0) tile_ptrC[0] = 0;
-    const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
-    const int64_t pa = tile_nnzA[ae] - tile_nnzA[as];
     int b = bt.nvar;
     for (int v = 0; v < bt.nvar; v++) {
         const StageCaps c = bt.v[v];
@@ -274,8 +271,8 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
         }
     }
     if (lane == 0)
-        row_result(bi, na, nb, pb, count, cmin, cmax, f_maxlen, f_sumspan, f_nonempty, tile_ptrA, tile_nnzA, bt,
-                   tile_ptrC, bin, stats);
+        row_result(bi, na, tile_nnzA[ae] - tile_nnzA[as], nb, pb, count, cmin, cmax, f_maxlen, f_sumspan,
+                   f_nonempty, bt, tile_ptrC, bin, stats);
 }
 
 // Step-1 count of light tile-rows (at most 16 A tiles, 32 B tiles reached,
@@ -306,6 +303,10 @@ k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     // (uniform in the group) gates memory traffic
     int4 lk = make_int4(0, 0, 0, 0);
     if (light && l < na) lk = link[as + l];
+    // the row's A entries (its bin needs them), loaded with the links
+    int64_t nz = 0;
+    if (light && l < 2) nz = tile_nnzA[l == 0 ? as : ae];
+    const int64_t pa = __shfl(nz, 1, 16) - __shfl(nz, 0, 16);
     const int incl = group16_incl_sum(lk.y);
     const int nb = __shfl(incl, 15, 16);
     const int64_t pbi = group16_incl_sum<int64_t>((int64_t)lk.z);
@@ -358,8 +359,8 @@ k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     }
     if (l == 0 && row) {
         if (light)
-            row_result(bi, na, nb, pb, cnt, nb > 0 ? cmin : INT_MAX, nb > 0 ? cmax : -1, f_maxlen, f_sumspan,
-                       f_nonempty, tile_ptrA, tile_nnzA, bt, tile_ptrC, bin, stats);
+            row_result(bi, na, pa, nb, pb, cnt, nb > 0 ? cmin : INT_MAX, nb > 0 ? cmax : -1, f_maxlen, f_sumspan,
+                       f_nonempty, bt, tile_ptrC, bin, stats);
         else
             list[atomicAdd(list_count, 1)] = bi;
     }
