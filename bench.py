@@ -500,6 +500,8 @@ def main():
     vt = "double" if s == 8 else "float"
     if len(names) == 1:
         kname = names[0].replace("<", f"<{vt},") if "<" in names[0] else f"{names[0]}<{vt}>"
+    elif not names:  # the C-tile kernels (dense tiles take the matrix-core ones)
+        kname = "numeric step (C-tile kernels)"
     else:
         kname = "numeric step (" + ", ".join(names) + ")"
     nbytes = rows_numeric_bytes(Ad, Bd, Cd, s) if names == ["k_num_rows"] else numeric_bytes(Ad, Bd, Cd, s)

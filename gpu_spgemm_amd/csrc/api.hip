@@ -624,12 +624,20 @@ int spgemm_dispatch(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matri
         if (!(A->tile_csc_Ptr && A->tile_csc_Row && A->tile_csc_Val))
             return fail(TSG_ERR_ARG, "numeric_variant 3 needs A's within-tile CSC (tsg_build_tile_csc)");
     }
-    TuneCfg cfg;  // a tuned configuration may select the matrix-core kernels
-    if ((!opts || (opts->numeric_variant == 0 && (opts->scheduler != TSG_SCHED_LEARNED || keep))) &&
-        tuned_config(ctx, A, B, opts, &cfg) && cfg.mfma > 0 && !((A->flags | B->flags) & 1)) {
+    // the matrix-core C-tile kernels: when a tuned configuration selects
+    // them, or -- without one -- when both operands' tiles hold 64 entries or
+    // more on average (dense pairs, where they win: a band of half-width 48
+    // runs 4x faster than on the binned kernels)
+    TuneCfg cfg;
+    const bool eligible = (!opts || (opts->numeric_variant == 0 && (opts->scheduler != TSG_SCHED_LEARNED || keep))) &&
+                          !((A->flags | B->flags) & 1);
+    const bool tuned = eligible && tuned_config(ctx, A, B, opts, &cfg);
+    const bool dense = A->numtile > 0 && B->numtile > 0 && A->nnz >= 64 * A->numtile && B->nnz >= 64 * B->numtile;
+    const int32_t gate = !eligible ? 0 : tuned ? cfg.mfma : dense ? 256 : 0;
+    if (gate > 0) {
         tsg_options o = opts ? *opts : tsg_options{};
         o.numeric_variant = 2;
-        o.mfma_min_products = cfg.mfma;
+        o.mfma_min_products = gate;
         if (keep) {
             keep->path = 2;
             keep->o = o;

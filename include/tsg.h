@@ -135,7 +135,9 @@ typedef enum tsg_scheduler {
 typedef struct tsg_options {
     int32_t scheduler;      /* tsg_scheduler */
     int32_t numeric_variant;/* 0 = auto: tile-rows binned to the row-staged kernels, heavy rows to the
-                               row-by-row or C-tile kernels; 1 = C-tile kernels for every tile (the
+                               row-by-row or C-tile kernels -- or, when both operands' tiles hold 64
+                               entries or more on average and no tuned configuration says otherwise,
+                               variant 2 with its default gate; 1 = C-tile kernels for every tile (the
                                reference's method); 2 = C-tile kernels with matrix cores (MFMA) for dense
                                tile pairs; 3 = C-tile kernels in outer-product form over A's
                                within-tile CSC (needs tsg_build_tile_csc on A; plain scheduler) */
