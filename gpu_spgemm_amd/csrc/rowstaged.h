@@ -57,7 +57,7 @@ constexpr int RG_DENSITY_DEFAULT = 64;
 // one small kernel and read back together with nnz(C).
 struct PlanFlags {
     int32_t cap;                    // a staged row exceeded its variant's capacity (or C's value capacity)
-    int32_t rest_rows;              // tile-rows the light Step-1 kernel left to the wave-per-row one
+    int32_t pad0;
     int64_t nnz;                    // nnz(C), written by the tile_nnz scan
     int32_t heavy_rows;             // length of the heavy-row list built by k_row_stats
     int32_t pad1;

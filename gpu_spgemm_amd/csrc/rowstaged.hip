@@ -78,6 +78,7 @@ k_ab_link(int64_t numtileA, const int32_t *__restrict__ colidxA, const int64_t *
 
 // ---------------------------------------------------------------- Step 1 count + statistics
 constexpr int RS_WAVES = 4;
+constexpr uint8_t BIN_REST = 0xFF;  // k_row_light's mark: counted by the wave-per-row kernel (bins are < MAX_BINS)
 constexpr int RS_WIN_WORDS = 512;  // 16,384 tile columns per pass
 
 // Ascending bitonic sort of one 64-bit key per lane across the wave.
@@ -142,16 +143,14 @@ __global__ void __launch_bounds__(RS_WAVES *WAVE)
 k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
             int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats,
-            int32_t *__restrict__ hlist, int32_t *__restrict__ hcount, const int32_t *__restrict__ list,
-            const int32_t *__restrict__ list_count) {
+            int32_t *__restrict__ hlist, int32_t *__restrict__ hcount) {
     __shared__ uint32_t s_bm[RS_WAVES][RS_WIN_WORDS];
     __shared__ int32_t s_bs[RS_WAVES][WAVE];
     __shared__ int32_t s_pre[RS_WAVES][WAVE + 1];
     const int w = threadIdx.x / WAVE, lane = lane_id();
-    const int32_t wi = blockIdx.x * RS_WAVES + w;
-    // every tile-row, or the ones k_row_light listed
-    if (wi >= (list ? *list_count : tilemA)) return;
-    const int32_t bi = list ? list[wi] : wi;
+    const int32_t bi = blockIdx.x * RS_WAVES + w;
+    // the tile-rows k_row_light left (marked in bin)
+    if (bi >= tilemA || bin[bi] != BIN_REST) return;
     uint32_t *bm = s_bm[w];
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
     const int64_t na = ae - as;
@@ -278,14 +277,14 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
 // Step-1 count of light tile-rows (at most 16 A tiles, 32 B tiles reached,
 // C tile columns spanning less than 1024) at 16 lanes each, 4 a wave: the
 // wave-per-row kernel above leaves most of its lanes idle on them.  Each other
-// row goes to `list` for that kernel.  Results and statistics are the ones the
-// wave-per-row kernel computes.
+// row's bin is set to BIN_REST for that kernel (a mark, not a list: appending
+// to a list through one counter serialised on rows that are all heavy).
+// Results and statistics are the ones the wave-per-row kernel computes.
 constexpr int RL_NA = 16, RL_NB = 32, RL_WORDS = 32;
 __global__ void __launch_bounds__(WAVE)
 k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
-            int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats,
-            int32_t *__restrict__ list, int32_t *__restrict__ list_count) {
+            int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats) {
     __shared__ uint32_t s_bm[4][RL_WORDS];
     __shared__ int32_t s_bs[4][RL_NA];
     __shared__ int32_t s_pre[4][RL_NA + 1];
@@ -362,7 +361,7 @@ k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
             row_result(bi, na, pa, nb, pb, cnt, nb > 0 ? cmin : INT_MAX, nb > 0 ? cmax : -1, f_maxlen, f_sumspan,
                        f_nonempty, bt, tile_ptrC, bin, stats);
         else
-            list[atomicAdd(list_count, 1)] = bi;
+            bin[bi] = BIN_REST;
     }
 }
 
@@ -970,22 +969,18 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
             if (!rc && (B->flags & 2)) rc = build_brow_bitmaps(ctx, B, p.hb);  // long B tile-rows exist
         }
         if (!rc) rc = zero_small(ctx, p.overflow(), PLAN_FLAG_WORDS, reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
-        // light tile-rows at 16 lanes each; the rest listed for a wave each
-        int32_t *rest = nullptr;
-        if (!rc) rc = palloc(ctx, &rest, (size_t)tilem);
+        // light tile-rows at 16 lanes each; the rest (marked) a wave each
         if (!rc) {
             k_row_light<<<(unsigned)((tilem + 3) / 4), WAVE, 0, ctx->stream>>>(
-                A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
-                rest, &p.flags->rest_rows);
+                A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_light launch");
         }
         if (!rc) {
             k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
                 A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
-                hlist, hc, rest, &p.flags->rest_rows);
+                hlist, hc);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_stats launch");
         }
-        pool_free(ctx, rest);
         if (!rc && hlist) rc = launch_heavy_count(ctx, A, B, C->tile_ptr, hlist, hc, stats_out, p.hb);
         pool_free(ctx, hlist);
         TSG_CHECK(rc);
