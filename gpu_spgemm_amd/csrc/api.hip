@@ -528,6 +528,10 @@ int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B) {
     if ((A->tile_size && A->tile_size != 16) || (B->tile_size && B->tile_size != 16))
         return fail(TSG_ERR_UNSUPPORTED, "products need 16 x 16 tiles (tile size %d / %d)", A->tile_size, B->tile_size);
     if (!A->mask) return fail(TSG_ERR_ARG, "A lacks row masks");
+    // tile masks and row offsets are read as 16-byte words
+    const void *words[] = {A->mask, A->tile_csr_Ptr, B->mask, B->tile_csr_Ptr, B->csr_mask, B->csr_col_mask, B->csr_col_Ptr};
+    for (const void *w : words)
+        if (!aligned16(w)) return fail(TSG_ERR_ARG, "tile masks and row offsets must start on 16-byte boundaries");
     if (A->tilem > MAX_TILE_ROWS || B->tilen > MAX_TILE_ROWS)
         return fail(TSG_ERR_OVERFLOW, "more than %lld tile-rows or tile-columns", (long long)MAX_TILE_ROWS);
     // the kernels address operand tiles (and B tile-row starts) with int32 ids

@@ -268,7 +268,8 @@ k_csc_sort(int32_t tilen, const int64_t *__restrict__ cptr, const int32_t *__res
     }
 }
 
-// inv[p] = CSR tile id of CSC position p; per-tile header copied.
+// pos_of[t] = CSC position p of CSR tile t, inv[p] = t (when inv is given),
+// and the tile's entry count at nnz_new[p + 1].
 __global__ void __launch_bounds__(256)
 k_csc_perm(int64_t numtile, const int32_t *__restrict__ colidx, const int32_t *__restrict__ rowidx,
            const int64_t *__restrict__ cptr, const int32_t *__restrict__ crow,
@@ -278,7 +279,7 @@ k_csc_perm(int64_t numtile, const int32_t *__restrict__ colidx, const int32_t *_
     if (t >= numtile) return;
     const int32_t tc = colidx[t];
     const int64_t p = lower_bound<int64_t, int32_t>(crow, cptr[tc], cptr[tc + 1], rowidx[t]);
-    inv[p] = t;
+    if (inv) inv[p] = t;
     pos_of[t] = p;
     nnz_new[p + 1] = nnz_old[t + 1] - nnz_old[t];
     if (t == 0) nnz_new[0] = 0;
@@ -315,40 +316,72 @@ __global__ void __launch_bounds__(256) k_row_nnz_ptr(int32_t tilem, const int64_
     if (k <= tilem) rnp[k] = tile_nnz[tile_ptr[k]];
 }
 
-// Column-major copy of a row-tiled matrix's tiles (16 lanes per tile, lane =
-// column c): column masks (bit 15-r = row r), exclusive column offsets, and
-// the values of each column in ascending row order.  Rows hold each column at
-// most once (no duplicates).
+// The re-layout of 16 x 16 tiles without repeated columns in one pass over
+// the CSR-order tiles, 16 lanes per tile t (lane l = row l and column l):
+//  - CSC order: tile t's row offsets, row masks, values and local columns
+//    (the low 4 bits of the col byte) move to CSC position p = pos_of[t];
+//  - column-major copy at t (read by the staged numeric kernels): column
+//    masks (bit 15-r = row r), exclusive column offsets, and each column's
+//    values in ascending row order.
+// Entry (r, c) of the tile goes to column c's first value plus the rows
+// above r that hold column c.  Each entry is read once; the group reads the
+// tile's 16 row masks and offsets as three 16-byte words.
 template <typename V>
 __global__ void __launch_bounds__(256)
-k_colmajor(int64_t numtile, const int64_t *__restrict__ tile_nnz, const uint8_t *__restrict__ ptr,
-           const uint16_t *__restrict__ mask, const V *__restrict__ val, uint16_t *__restrict__ cmask,
-           uint8_t *__restrict__ cptr, V *__restrict__ cval) {
-    const int c = threadIdx.x & 15;
+k_csc_relayout(int64_t numtile, const int64_t *__restrict__ pos_of, const int64_t *__restrict__ nnz_old,
+               const int64_t *__restrict__ nnz_new, const uint8_t *__restrict__ ptr_old,
+               const uint16_t *__restrict__ mask_old, const uint8_t *__restrict__ col_old,
+               const V *__restrict__ val_old, uint8_t *__restrict__ ptr_new, uint16_t *__restrict__ mask_new,
+               uint8_t *__restrict__ col_new, V *__restrict__ val_new, uint16_t *__restrict__ cmask,
+               uint8_t *__restrict__ cptr, V *__restrict__ cval) {
+    const int l = threadIdx.x & 15;
+    const uint32_t cbit = 0x8000u >> l;
     // grid-stride over tiles; whole 16-lane groups move together
-    for (int64_t p = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4; p < numtile; p += (int64_t)gridDim.x * 16) {
-    const uint32_t rm = mask[p * 16 + c];  // lane c holds row c's mask and offset
-    const int ro = ptr[p * 16 + c];
-    const uint32_t cbit = 0x8000u >> c;
-    uint32_t cm = 0;
+    for (int64_t t = ((int64_t)blockIdx.x * 256 + threadIdx.x) >> 4; t < numtile; t += (int64_t)gridDim.x * 16) {
+        const int64_t p = pos_of[t];
+        const int64_t so = nnz_old[t], n = nnz_old[t + 1] - so;
+        const uint4 mlo = reinterpret_cast<const uint4 *>(mask_old)[t * 2];
+        const uint4 mhi = reinterpret_cast<const uint4 *>(mask_old)[t * 2 + 1];
+        const uint4 po = reinterpret_cast<const uint4 *>(ptr_old)[t];
+        const int64_t sn = nnz_new[p];
+        const uint32_t mw[8] = {mlo.x, mlo.y, mlo.z, mlo.w, mhi.x, mhi.y, mhi.z, mhi.w};
+        const uint32_t pw[4] = {po.x, po.y, po.z, po.w};
+        uint32_t cm = 0, rm = 0, ro = 0;
 #pragma unroll
-    for (int r = 0; r < 16; r++)
-        if (__shfl(rm, r, 16) & cbit) cm |= 0x8000u >> r;
-    const int pc = __popc(cm);
-    const int co = group16_incl_sum(pc) - pc;
-    cmask[p * 16 + c] = (uint16_t)cm;
-    cptr[p * 16 + c] = (uint8_t)co;
-    const int64_t base = tile_nnz[p];
-    int j = 0;
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const uint32_t m = __shfl(rm, r, 16);
-        const int o = __shfl(ro, r, 16);
-        if (m & cbit) {
-            cval[base + co + j] = val[base + o + __popc(m & ~((cbit << 1) - 1u))];
-            j++;
+        for (int r = 0; r < 16; r++) {
+            const uint32_t m = (mw[r >> 1] >> ((r & 1) * 16)) & 0xFFFFu;
+            if (m & cbit) cm |= 0x8000u >> r;
+            if (r == l) {
+                rm = m;
+                ro = (pw[r >> 2] >> ((r & 3) * 8)) & 0xFFu;
+            }
         }
-    }
+        const int pc = __popc(cm);
+        const int co = group16_incl_sum(pc) - pc;
+        ptr_new[p * 16 + l] = (uint8_t)ro;
+        mask_new[p * 16 + l] = (uint16_t)rm;
+        cmask[t * 16 + l] = (uint16_t)cm;
+        cptr[t * 16 + l] = (uint8_t)co;
+        // entries 16 at a time; n is uniform in the group, so every lane
+        // takes part in the exchanges of column masks and offsets
+        for (int64_t e0 = 0; e0 < n; e0 += 16) {
+            const int64_t e = e0 + l;
+            const bool in = e < n;
+            V v = V(0);
+            int cb = 0;
+            if (in) {
+                v = val_old[so + e];
+                cb = col_old[so + e];
+            }
+            const int r = cb >> 4, c = cb & 15;
+            const uint32_t cmc = __shfl(cm, c, 16);
+            const int coc = __shfl(co, c, 16);
+            if (in) {
+                val_new[sn + e] = v;
+                col_new[sn + e] = (uint8_t)c;
+                cval[so + coc + __popc(cmc >> (16 - r))] = v;  // rows 0..r-1 of column c
+            }
+        }
     }
 }
 
@@ -535,6 +568,8 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     hipStream_t s = ctx->stream;
     const int64_t nt = B->numtile;
     const int ts = B->tile_size ? B->tile_size : 16;  // build_csc_tiles_ts (src/tile_format.h:327-417) when not 16
+    if (ts == 16 && (!aligned16(B->mask) || !aligned16(B->tile_csr_Ptr)))
+        return fail(TSG_ERR_ARG, "row masks and row offsets must start on 16-byte boundaries (tiles are read as 16-byte words)");
     int32_t *rowidx = B->tile_rowidx, *tmp = nullptr, *crow = nullptr;
     int64_t *cptr = nullptr, *cursor = nullptr, *inv = nullptr, *nnz_new = nullptr, *pos_of = nullptr;
     uint8_t *ptr_new = nullptr, *col_new = nullptr;
@@ -550,17 +585,30 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "rowidx launch");
         }
     }
+    // the column-major copy of the CSR-order tiles for the staged numeric
+    // kernels (16 x 16 tiles without repeated columns: the staged kernels are
+    // not used otherwise), and the tile-row starts of the payload
+    const bool cm_copy = !(B->flags & 1) && ts == 16;
+    uint16_t *ccm = nullptr;
+    uint8_t *ccp = nullptr;
+    void *ccv = nullptr;
+    int64_t *rnp = nullptr, *vpc = nullptr;
     if (!rc) rc = palloc(ctx, &cptr, (size_t)B->tilen + 1);
     if (!rc) rc = palloc(ctx, &cursor, (size_t)B->tilen + 1);
     if (!rc) rc = palloc(ctx, &tmp, (size_t)nt);
     if (!rc) rc = palloc(ctx, &crow, (size_t)nt);
-    if (!rc) rc = palloc(ctx, &inv, (size_t)nt);
+    if (!rc && !cm_copy) rc = palloc(ctx, &inv, (size_t)nt);
     if (!rc) rc = palloc(ctx, &pos_of, (size_t)nt);
     if (!rc) rc = palloc(ctx, &nnz_new, (size_t)nt + 1);
     if (!rc) rc = palloc(ctx, &ptr_new, (size_t)nt * ts);
     if (!rc) rc = palloc(ctx, &mask_new, (size_t)nt * ts);
     if (!rc) rc = palloc(ctx, &col_new, (size_t)B->nnz);
     if (!rc) rc = pool_alloc(ctx, &val_new, (size_t)B->nnz * vsize(B->dtype));
+    if (!rc && cm_copy) rc = palloc(ctx, &ccm, (size_t)nt * 16);
+    if (!rc && cm_copy) rc = palloc(ctx, &ccp, (size_t)nt * 16);
+    if (!rc && cm_copy) rc = pool_alloc(ctx, &ccv, (size_t)B->nnz * vsize(B->dtype));
+    if (!rc && ts == 16) rc = palloc(ctx, &rnp, (size_t)B->tilem + 1);
+    if (!rc && cm_copy && B->val_perm && B->nnz > 0) rc = palloc(ctx, &vpc, (size_t)B->nnz);
     if (!rc && hipMemsetAsync(cptr, 0, sizeof(int64_t) * ((size_t)B->tilen + 1), s) != hipSuccess)
         rc = fail(TSG_ERR_HIP, "memset");
     if (!rc && nt > 0) {
@@ -581,7 +629,17 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     }
     if (!rc) rc = scan_inclusive_i64(ctx, nnz_new, nt + 1);
     if (!rc && nt > 0) {
-        if (B->dtype == TSG_F32)
+        const bool f32 = B->dtype == TSG_F32;
+        if (cm_copy && f32)
+            k_csc_relayout<float><<<grid_capped(nt, 16), 256, 0, s>>>(
+                nt, pos_of, B->tile_nnz, nnz_new, B->tile_csr_Ptr, B->mask, B->tile_csr_Col,
+                (const float *)B->tile_csr_Val, ptr_new, mask_new, col_new, (float *)val_new, ccm, ccp, (float *)ccv);
+        else if (cm_copy)
+            k_csc_relayout<double><<<grid_capped(nt, 16), 256, 0, s>>>(
+                nt, pos_of, B->tile_nnz, nnz_new, B->tile_csr_Ptr, B->mask, B->tile_csr_Col,
+                (const double *)B->tile_csr_Val, ptr_new, mask_new, col_new, (double *)val_new, ccm, ccp,
+                (double *)ccv);
+        else if (f32)
             k_csc_gather<float><<<grid_capped(nt, 16), 256, 0, s>>>(
                 nt, inv, B->tile_nnz, nnz_new, B->tile_csr_Ptr, B->mask, B->tile_csr_Col,
                 (const float *)B->tile_csr_Val, ptr_new, mask_new, col_new, (float *)val_new, ts);
@@ -591,73 +649,26 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
                 (const double *)B->tile_csr_Val, ptr_new, mask_new, col_new, (double *)val_new, ts);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "csc gather launch");
     }
-    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(TSG_ERR_HIP, "build_csc_tiles: %s", hipGetErrorString(hipGetLastError()));
+    if (!rc && rnp) {
+        k_row_nnz_ptr<<<blocks_for((int64_t)B->tilem + 1, 256), 256, 0, s>>>(B->tilem, B->tile_ptr, B->tile_nnz, rnp);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "row_nnz_ptr launch");
+    }
+    // value positions of the CSR entries in the CSC-order tiles and in the column-major copy
+    if (!rc && B->val_perm && B->nnz > 0) {
+        k_val_perm_b<<<grid_capped(B->nnz, 256), 256, 0, s>>>(B->nnz, nt, B->tile_nnz, pos_of, nnz_new,
+                                                              B->tile_csr_Col, ccm, ccp, B->val_perm, vpc);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "value positions launch");
+    }
+    // the work arrays go back to the pool in stream order: later work on this
+    // stream that reuses them runs after the kernels above
     pool_free(ctx, cursor);
     pool_free(ctx, tmp);
     pool_free(ctx, inv);
     if (rc) {
-        void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, pos_of};
-        for (void *p : ps) pool_free(ctx, p);
+        void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp, ccm, ccp, ccv, vpc, pos_of};
+        for (void *q : ps) pool_free(ctx, q);
         if (own_rowidx) pool_free(ctx, rowidx);
         return rc;
-    }
-    // keep the CSR-tile-order data for the row-staged kernels; the packed col
-    // bytes are implied by the masks there and are dropped
-    int64_t *rnp = nullptr;
-    if (ts == 16) rc = palloc(ctx, &rnp, (size_t)B->tilem + 1);
-    if (!rc && ts == 16) {
-        k_row_nnz_ptr<<<blocks_for((int64_t)B->tilem + 1, 256), 256, 0, s>>>(B->tilem, B->tile_ptr, B->tile_nnz, rnp);
-        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "row_nnz_ptr launch");
-    }
-    if (rc) {
-        void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp, pos_of};
-        for (void *p : ps) pool_free(ctx, p);
-        if (own_rowidx) pool_free(ctx, rowidx);
-        return rc;
-    }
-    // column-major values of the CSR-order tiles for the staged numeric kernel
-    // (only without duplicate columns: the staged kernels are not used otherwise)
-    uint16_t *ccm = nullptr;
-    uint8_t *ccp = nullptr;
-    void *ccv = nullptr;
-    if (!(B->flags & 1) && ts == 16) {
-        rc = palloc(ctx, &ccm, (size_t)nt * 16);
-        if (!rc) rc = palloc(ctx, &ccp, (size_t)nt * 16);
-        if (!rc) rc = pool_alloc(ctx, &ccv, (size_t)B->nnz * vsize(B->dtype));
-        if (!rc && nt > 0) {
-            if (B->dtype == TSG_F32)
-                k_colmajor<float><<<grid_capped(nt, 16), 256, 0, s>>>(
-                    nt, B->tile_nnz, B->tile_csr_Ptr, B->mask, (const float *)B->tile_csr_Val, ccm, ccp, (float *)ccv);
-            else
-                k_colmajor<double><<<grid_capped(nt, 16), 256, 0, s>>>(
-                    nt, B->tile_nnz, B->tile_csr_Ptr, B->mask, (const double *)B->tile_csr_Val, ccm, ccp,
-                    (double *)ccv);
-            if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "colmajor launch");
-        }
-        if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = fail(TSG_ERR_HIP, "colmajor failed");
-        if (rc) {
-            void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp, ccm, ccp, ccv, pos_of};
-            for (void *p : ps) pool_free(ctx, p);
-            if (own_rowidx) pool_free(ctx, rowidx);
-            return rc;
-        }
-    }
-    // value positions of the CSR entries in the CSC-order tiles and in the column-major copy
-    int64_t *vpc = nullptr;
-    if (B->val_perm && B->nnz > 0) {
-        if (ccm) rc = palloc(ctx, &vpc, (size_t)B->nnz);
-        if (!rc) {
-            k_val_perm_b<<<grid_capped(B->nnz, 256), 256, 0, s>>>(B->nnz, nt, B->tile_nnz, pos_of, nnz_new,
-                                                                  B->tile_csr_Col, ccm, ccp, B->val_perm, vpc);
-            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
-                rc = fail(TSG_ERR_HIP, "value positions failed");
-        }
-        if (rc) {
-            void *ps[] = {cptr, crow, nnz_new, ptr_new, mask_new, col_new, val_new, rnp, ccm, ccp, ccv, vpc, pos_of};
-            for (void *p : ps) pool_free(ctx, p);
-            if (own_rowidx) pool_free(ctx, rowidx);
-            return rc;
-        }
     }
     pool_free(ctx, pos_of);
     // a within-tile CSC built in row-tile order no longer matches the data arrays
@@ -687,7 +698,6 @@ extern "C" int tsg_build_csc_tiles(tsg_context *c, tsg_tile_matrix *B) {
     B->csc_tile_rowidx = crow;
     B->tile_rowidx = rowidx;
     B->is_csc = 1;
-    TSG_HIP(hipStreamSynchronize(s));
     return TSG_OK;
 }
 

@@ -235,6 +235,7 @@ int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_mat
                       const int32_t *rows, const BinRange &br, int64_t *woff, const int64_t *woffn,
                       int64_t *gcur, const SlotLayout &sl);
 int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B);  // api.hip
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 // Looks up the no-alloc scratch's device address on ctx's device (current).
 int noalloc_setup(Context *ctx);  // noalloc.hip
 int launch_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,

@@ -253,7 +253,10 @@ int tsg_tile_free_host(tsg_tile_matrix *host);
 int tsg_csr2tile(tsg_context *ctx, const tsg_csr *dev_csr, tsg_tile_matrix *out_dev);
 /* Re-lays a row-tiled matrix's data arrays in CSC-tile order, making it a B
  * operand; replaces build_csc_tiles (src/tile_format.h:424-546).  Works on any
- * row-tiled matrix, including a C output (so products chain without host work). */
+ * row-tiled matrix, including a C output (so products chain without host work).
+ * Asynchronous on the context's stream (no host value is returned).  The row
+ * masks and row offsets of 16 x 16 tiles must start on 16-byte boundaries, as
+ * every array this library allocates does. */
 int tsg_build_csc_tiles(tsg_context *ctx, tsg_tile_matrix *B_dev);
 /* New values for a matrix made by tsg_csr2tile (and tsg_build_csc_tiles), same
  * structure: vals = nnz device values in the CSR order of the original input;
