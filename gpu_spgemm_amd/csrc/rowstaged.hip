@@ -49,6 +49,7 @@ constexpr StageCaps CAPS_R{8, 16, 32, 128, 1 << 20, 32};
 constexpr int NUM_STAGED = 4;  // bins 0..3: R, S, W, M
 constexpr int64_t ROW_LANES_MIN = 32768;  // tile-rows below which R's rows take VarS's numeric kernel
 constexpr int DESC_CAP = 256;  // output descriptors of one chunk of (C tile, row) items
+constexpr int NS_K = 8;        // products of one output whose B loads issue together
 
 template <typename Var>
 static StageCaps caps_of() {
@@ -793,24 +794,48 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
                 const int d = s.desc[o];
                 const int ct = d >> 8, r = (d >> 4) & 15, c = d & 15;
                 V acc = V(0);
+                int p = s.pstart[ct];
                 const int pe = s.pstart[ct + 1];
-                for (int p = s.pstart[ct]; p < pe; p++) {
-                    const int e = s.plist[p];
-                    const int b = e & 0xFF, t = e >> 8;
-                    const uint32_t ma = s.amask[t * 16 + r];
-                    const uint32_t cm = s.bcmask[b * 16 + c];
-                    uint32_t m = ma & cm;  // k of the products, ascending from the top bit
-                    if (!m) continue;
-                    const int ea = s.aoff[t] + s.aptr[t * 16 + r];
-                    const V *bv = cvalB + (s.bgoff[b] + s.bcptr[b * 16 + c]);
-                    do {
-                        // top set bit hb (m != 0): k = 15 - hb; the entries before it
-                        // in A's row and B's column are the set bits above hb
-                        const int hb = 31 - __builtin_clz(m);
-                        m ^= 1u << hb;
-                        acc = fma_v<V>(s.aval[ea + __popc(ma >> (hb + 1))], bv[__popc(cm >> (hb + 1))], acc);
-                    } while (m);
-                }
+                uint32_t ma = 0, cm = 0, m = 0;  // the current pair's masks; m: its k not yet taken
+                int ea = 0;
+                const V *bv = cvalB;
+                // the output's products NS_K at a time: their B values are
+                // loaded together, then added in order (pairs ascending, k
+                // ascending within a pair)
+                do {
+                    int ai[NS_K];
+                    const V *bp[NS_K];
+                    int nk = 0;
+#pragma unroll
+                    for (int k = 0; k < NS_K; k++) {
+                        while (!m && p < pe) {
+                            const int e = s.plist[p++];
+                            const int b = e & 0xFF, t = e >> 8;
+                            ma = s.amask[t * 16 + r];
+                            cm = s.bcmask[b * 16 + c];
+                            m = ma & cm;  // k of the products, ascending from the top bit
+                            ea = s.aoff[t] + s.aptr[t * 16 + r];
+                            bv = cvalB + (s.bgoff[b] + s.bcptr[b * 16 + c]);
+                        }
+                        ai[k] = 0;
+                        bp[k] = bv;
+                        if (m) {
+                            // top set bit hb: k = 15 - hb; the entries before it
+                            // in A's row and B's column are the set bits above hb
+                            const int hb = 31 - __builtin_clz(m);
+                            m ^= 1u << hb;
+                            ai[k] = ea + __popc(ma >> (hb + 1));
+                            bp[k] = bv + __popc(cm >> (hb + 1));
+                            nk = k + 1;
+                        }
+                    }
+                    V bq[NS_K];
+#pragma unroll
+                    for (int k = 0; k < NS_K; k++) bq[k] = k < nk ? *bp[k] : V(0);
+#pragma unroll
+                    for (int k = 0; k < NS_K; k++)
+                        if (k < nk) acc = fma_v<V>(s.aval[ai[k]], bq[k], acc);
+                } while (m || p < pe);
                 res[it] = acc;
                 cb[it] = (uint8_t)((r << 4) | c);
             }
