@@ -60,11 +60,49 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
     const int64_t idx = (int64_t)blockIdx.x * RN_GROUPS + g;
     // this group's tile-row and C tile range (uniform in the group)
     const int32_t bi = bin_row(rows, row_begin, nrows, dlo, dhi, idx);
+    // this lane's scalar row of A, its first RN_EB entries and their B row
+    // ranges: issued before the C headers, so the two load chains overlap
+    const int64_t gr = (int64_t)(bi < 0 ? 0 : bi) * 16 + r;
+    int64_t e0 = 0, e1 = 0;
+    if (bi >= 0 && gr < m) {
+        e0 = rpA[gr];
+        e1 = rpA[gr + 1];
+    }
     int64_t cs = 0, ce = 0;
     if (bi >= 0) {
         cs = tile_ptrC[bi];
         ce = tile_ptrC[bi + 1];
     }
+    int32_t kk[RN_EB];
+    V aa[RN_EB];
+    int64_t j0[RN_EB];
+    int32_t jn[RN_EB];  // B row lengths, 0 past the A row's end
+    // A entries [eb, eb + RN_EB) and their B rows' ranges; loads past the
+    // row's end repeat its last entry (in bounds, no branch) and are then
+    // given an empty B row (eb < e1)
+    auto load_a = [&](int64_t eb) {
+        const int ne = (int)min<int64_t>(RN_EB, e1 - eb);
+#pragma unroll
+        for (int u = 0; u < RN_EB; u++) {
+            const int64_t e = eb + min(u, ne - 1);
+            kk[u] = colA[e];
+            aa[u] = valA[e];
+        }
+#pragma unroll
+        for (int u = 0; u < RN_EB; u++) {
+            const int64_t b0 = rpB[kk[u]], b1 = rpB[kk[u] + 1];
+            j0[u] = b0;
+            jn[u] = u < ne ? (int32_t)(b1 - b0) : 0;
+        }
+    };
+#pragma unroll
+    for (int u = 0; u < RN_EB; u++) {
+        kk[u] = 0;
+        aa[u] = V(0);
+        j0[u] = 0;
+        jn[u] = 0;
+    }
+    if (e1 > e0) load_a(e0);
     const int nc = (int)(ce - cs);
     bool ok = bi >= 0 && nc <= RN_NC;
     if (bi >= 0 && nc > RN_NC && r == 0) atomicOr(overflow, 1);
@@ -97,14 +135,8 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
         reinterpret_cast<uint4 *>(&s.cptr[g][r * 16])[0] = pp;
     }
     if (ok && r == 0) s.coff[g][nc] = (int32_t)(vend - vs);
-    // this lane's scalar row of A
-    const int64_t gr = (int64_t)(bi < 0 ? 0 : bi) * 16 + r;
-    int64_t e0 = 0, e1 = 0;
-    if (ok && gr < m) {
-        e0 = rpA[gr];
-        e1 = rpA[gr + 1];
-    }
     wave_sync();
+    bool pre = e1 > e0;  // the batch at e0 is loaded
     // windows of whole C tiles holding at most RN_VCAP values (a tile holds <= 256)
     int w0 = 0;
     while (true) {
@@ -124,25 +156,8 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
             // A entries RN_EB at a time, their B rows RN_BB entries at a time:
             // each batch's loads are issued together (one latency a batch)
             for (int64_t eb = e0; eb < e1; eb += RN_EB) {
-                const int ne = (int)min<int64_t>(RN_EB, e1 - eb);
-                int32_t kk[RN_EB];
-                V aa[RN_EB];
-                int64_t j0[RN_EB];
-                int32_t jn[RN_EB];  // B row lengths, 0 past the A row's end
-                // loads past the row's end repeat its last entry (in bounds,
-                // no branch) and are then given an empty B row
-#pragma unroll
-                for (int u = 0; u < RN_EB; u++) {
-                    const int64_t e = eb + min(u, ne - 1);
-                    kk[u] = colA[e];
-                    aa[u] = valA[e];
-                }
-#pragma unroll
-                for (int u = 0; u < RN_EB; u++) {
-                    const int64_t b0 = rpB[kk[u]], b1 = rpB[kk[u] + 1];
-                    j0[u] = b0;
-                    jn[u] = u < ne ? (int32_t)(b1 - b0) : 0;
-                }
+                if (!pre) load_a(eb);
+                pre = false;
                 // RN_BB entries of B row u from its entry o (o < jn[u]); past
                 // the row's end its last entry again, column INT_MAX
                 auto load_b = [&](int u, int o, int32_t(&c)[RN_BB], V(&b)[RN_BB]) {
@@ -444,13 +459,19 @@ k_sym_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
     }
 }
 
+// A host-sized bin that holds every tile-row lists them in order (the bin
+// scatter is stable): the kernels then skip the list.
+static const int32_t *row_list(const int32_t *rows, const BinRange &br, int32_t tilem) {
+    return (!br.lo && br.begin == 0 && br.count == tilem) ? nullptr : rows;
+}
+
 int launch_sym_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                     const int32_t *rows, const BinRange &br, const int4 *link, const SlotLayout &sl) {
     const int64_t count = br.count;
     if (count <= 0) return TSG_OK;
     if (sl.row_nc) return fail(TSG_ERR_ARG, "the row-lane symbolic kernel runs on the exact C layout only");
     const unsigned grid = (unsigned)((count + RN_GROUPS - 1) / RN_GROUPS);
-    k_sym_rows<<<grid, WAVE, 0, ctx->stream>>>(rows, br.begin, count, br.lo, br.hi, A->tile_ptr, A->mask, link,
+    k_sym_rows<<<grid, WAVE, 0, ctx->stream>>>(row_list(rows, br, C->tilem), br.begin, count, br.lo, br.hi, A->tile_ptr, A->mask, link,
                                                B->tile_columnidx, B->csr_mask, C->tile_ptr, C->tile_rowidx,
                                                C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, sl.overflow);
     TSG_HIP(hipGetLastError());
@@ -470,12 +491,12 @@ int launch_num_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matri
     const unsigned grid = (unsigned)((count + RN_GROUPS - 1) / RN_GROUPS);
     if (A->dtype == TSG_F32)
         k_num_rows<float><<<grid, WAVE, 0, ctx->stream>>>(
-            rows, br.begin, count, br.lo, br.hi, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr, B->row_col,
+            row_list(rows, br, C->tilem), br.begin, count, br.lo, br.hi, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr, B->row_col,
             (const float *)B->row_val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr,
             (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done);
     else
         k_num_rows<double><<<grid, WAVE, 0, ctx->stream>>>(
-            rows, br.begin, count, br.lo, br.hi, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr, B->row_col,
+            row_list(rows, br, C->tilem), br.begin, count, br.lo, br.hi, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr, B->row_col,
             (const double *)B->row_val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr,
             (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done);
     TSG_HIP(hipGetLastError());

@@ -374,11 +374,13 @@ __device__ __forceinline__ void bin_rows(const int64_t *dlo, const int64_t *dhi,
     }
 }
 // Item idx of a bin's row list (host range, or the device range dlo/dhi): its
-// tile-row, or -1 past the list's end.
+// tile-row, or -1 past the list's end.  rows == nullptr: the list is every
+// tile-row in order (item idx is tile-row row_begin + idx).
 __device__ __forceinline__ int32_t bin_row(const int32_t *rows, int64_t row_begin, int64_t nrows,
                                            const int64_t *dlo, const int64_t *dhi, int64_t idx) {
     bin_rows(dlo, dhi, row_begin, nrows);
-    return idx < nrows ? rows[row_begin + idx] : -1;
+    if (idx >= nrows) return -1;
+    return rows ? rows[row_begin + idx] : (int32_t)(row_begin + idx);
 }
 
 // Tile range [s,e) of C tile-row bi.  Exact layout: ptr is tile_ptrC.  Slot
