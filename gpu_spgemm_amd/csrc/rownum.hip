@@ -459,12 +459,6 @@ k_sym_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
     }
 }
 
-// A host-sized bin that holds every tile-row lists them in order (the bin
-// scatter is stable): the kernels then skip the list.
-static const int32_t *row_list(const int32_t *rows, const BinRange &br, int32_t tilem) {
-    return (!br.lo && br.begin == 0 && br.count == tilem) ? nullptr : rows;
-}
-
 int launch_sym_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                     const int32_t *rows, const BinRange &br, const int4 *link, const SlotLayout &sl) {
     const int64_t count = br.count;

@@ -18,6 +18,13 @@ __device__ __forceinline__ int owner_of(const int32_t *seg, int n, int32_t i) {
 }
 
 // 16-byte loads of a tile's 16 row pointers (uint8) and 16 row masks (uint16).
+// Host-sized bins [b, e) that hold every tile-row between them: the kernels
+// take item idx as tile-row idx (nullptr list), which covers the same rows
+// once each and skips a dependent load.
+inline const int32_t *row_list(const int32_t *rows, const BinRange &br, int32_t tilem) {
+    return (!br.lo && br.begin == 0 && br.count == tilem) ? nullptr : rows;
+}
+
 __device__ __forceinline__ uint4 load_ptr16(const uint8_t *p, int64_t tile) {
     return reinterpret_cast<const uint4 *>(p)[tile];
 }

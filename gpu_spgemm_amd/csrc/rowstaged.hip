@@ -49,7 +49,7 @@ constexpr StageCaps CAPS_R{8, 16, 32, 128, 1 << 20, 32};
 constexpr int NUM_STAGED = 4;  // bins 0..3: R, S, W, M
 constexpr int64_t ROW_LANES_MIN = 32768;  // tile-rows below which R's rows take VarS's numeric kernel
 constexpr int DESC_CAP = 256;  // output descriptors of one chunk of (C tile, row) items
-constexpr int NS_K = 8;        // products of one output whose B loads issue together
+constexpr int NS_K = 4;        // products of one output whose value loads issue together
 
 template <typename Var>
 static StageCaps caps_of() {
@@ -476,7 +476,7 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     bin_rows(dlo, dhi, row_begin, nrows);
     if (idx >= nrows) return;
     SymLDS<Var> &s = lds[w];
-    const int32_t bi = rows[row_begin + idx];
+    const int32_t bi = rows ? rows[row_begin + idx] : (int32_t)(row_begin + idx);
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
     // exact layout: [cs, ce) are the row's tiles; slot layout: its slots
     const int64_t cs = tile_ptrC[bi], ce = tile_ptrC[bi + 1];
@@ -625,7 +625,7 @@ template <>
 __device__ __forceinline__ float fma_v<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // A's and B's tile headers are staged in LDS; the values of both are read
-// from memory in the output loop, eight products of an output at a time
+// from memory in the output loop, four products of an output at a time
 // (values reused across outputs come from the caches), which keeps LDS per
 // wave small and occupancy high: with A's values staged too (4 KB a wave
 // for 512 FP64 values) a 27-point stencil times an aggregation ran 1.33x
@@ -653,7 +653,7 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
     bin_rows(dlo, dhi, row_begin, nrows);
     if (idx >= nrows) return;
     NumLDS<Var> &s = lds[w];
-    const int32_t bi = rows[row_begin + idx];
+    const int32_t bi = rows ? rows[row_begin + idx] : (int32_t)(row_begin + idx);
     // level 1: tile-row pointers (wave-uniform scalars)
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
     int64_t cs, ce;
@@ -788,9 +788,9 @@ k_num_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
                 uint32_t ma = 0, cm = 0, m = 0;  // the current pair's masks; m: its k not yet taken
                 int ea = 0;
                 const V *bv = cvalB;
-                // the output's products NS_K at a time: their B values are
-                // loaded together, then added in order (pairs ascending, k
-                // ascending within a pair)
+                // the output's products NS_K at a time: their A and B values
+                // are loaded together, then added in order (pairs ascending,
+                // k ascending within a pair)
                 do {
                     int ai[NS_K];
                     const V *bp[NS_K];
@@ -1022,7 +1022,7 @@ static int launch_sym_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
     const BinRange br = bin_range(p, bin, bin_end < 0 ? bin + 1 : bin_end);
     if (br.count <= 0) return TSG_OK;
     k_sym_staged<Var><<<nblk(br.count, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
-        p.rows, br.begin, br.count, br.lo, br.hi,
+        row_list(p.rows, br, C->tilem), br.begin, br.count, br.lo, br.hi,
         A->tile_ptr, A->mask, p.link, B->tile_columnidx, B->csr_mask, C->tile_ptr,
         C->tile_rowidx, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, p.overflow(), sl.row_nc);
     TSG_HIP(hipGetLastError());
@@ -1036,7 +1036,7 @@ static int launch_num_staged(Context *ctx, const tsg_tile_matrix *A, const tsg_t
     const BinRange br = bin_range(p, bin, bin_end < 0 ? bin + 1 : bin_end);
     if (br.count <= 0) return TSG_OK;
     k_num_staged<V, Var><<<nblk(br.count, Var::WAVES), Var::WAVES * WAVE, 0, ctx->stream>>>(
-        p.rows, br.begin, br.count, br.lo, br.hi,
+        row_list(p.rows, br, C->tilem), br.begin, br.count, br.lo, br.hi,
         A->tile_ptr, A->tile_nnz, A->tile_csr_Ptr, A->mask, (const V *)A->tile_csr_Val,
         p.link, B->tile_columnidx, B->csr_tile_nnz, B->csr_col_Ptr, B->csr_col_mask,
         (const V *)B->csr_col_Val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, (V *)C->tile_csr_Val,
