@@ -471,6 +471,8 @@ def main():
     if not rmat:   # R-MAT compares the learned scheduler with the exact binned pipeline
         runs["ctile"] = measure("reference C-tile method", variant=1)
     runs["plain"] = measure("plain (binned, exact sizes)", variant=0)
+    if rmat:   # heavy sparse rows: the row-by-row numeric step with products added in any order
+        runs["any_order"] = measure("plain, any-order sums (sum_order=1)", variant=0, sum_order=1)
     if args.workload == "band":     # full tiles: matrix cores for the dense pairs
         runs["mfma"] = measure("C-tile + matrix cores", variant=2)
     learned = None

@@ -54,7 +54,7 @@ class TileStruct(C.Structure):
 class Options(C.Structure):
     _fields_ = [("scheduler", C.c_int32), ("numeric_variant", C.c_int32),
                 ("mfma_min_products", C.c_int32), ("timing", C.c_int32), ("pred_scale", C.c_double),
-                ("tuning", C.c_int32), ("reserved", C.c_int32)]
+                ("tuning", C.c_int32), ("sum_order", C.c_int32)]
 
 
 class TuneEntry(C.Structure):
@@ -360,9 +360,10 @@ class Context:
         return t
 
     def spgemm(self, A, B, scheduler=SCHED_PLAIN, timing=False, variant=0, pred_scale=1.0,
-               mfma_min_products=0, tuning=0):
-        """C = A*B (Steps 1-3 of src/spgemm_raw_v2.cu:111-240); returns (C, stats dict)."""
-        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale, tuning)
+               mfma_min_products=0, tuning=0, sum_order=0):
+        """C = A*B (Steps 1-3 of src/spgemm_raw_v2.cu:111-240); returns (C, stats dict).
+        sum_order=1 lets the heavy-row numeric kernel add products in any order."""
+        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale, tuning, sum_order)
         st = Stats()
         Cm = DeviceTiles(self)
         _check(lib().tsg_spgemm(self._p, C.byref(A.s), C.byref(B.s), C.byref(Cm.s), C.byref(o),
@@ -392,11 +393,11 @@ class Context:
         return c, st.as_dict()
 
     def spgemm_batched(self, As, Bs, scheduler=SCHED_PLAIN, timing=False, variant=0, pred_scale=1.0,
-                       mfma_min_products=0, tuning=0):
+                       mfma_min_products=0, tuning=0, sum_order=0):
         """C_i = A_i*B_i for every pair in one pipeline run; returns ([C_i], [stats_i])."""
         n = len(As)
         assert len(Bs) == n
-        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale, tuning)
+        o = Options(scheduler, variant, mfma_min_products, 1 if timing else 0, pred_scale, tuning, sum_order)
         a = (TileStruct * max(n, 1))(*[x.s for x in As])
         b = (TileStruct * max(n, 1))(*[x.s for x in Bs])
         c = (TileStruct * max(n, 1))()
@@ -433,14 +434,14 @@ class Context:
                                         bmB.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(bl)))
         return bmA, bmB
 
-    def batch_graph(self, As, Bs, slack=1.25, tuning=0):
+    def batch_graph(self, As, Bs, slack=1.25, tuning=0, sum_order=0):
         """One captured hipGraph for the whole batch product (tsg_batch_graph_*)."""
-        return BatchGraph(self, As, Bs, slack, Options(SCHED_PLAIN, 0, 0, 0, 1.0, tuning))
+        return BatchGraph(self, As, Bs, slack, Options(SCHED_PLAIN, 0, 0, 0, 1.0, tuning, sum_order))
 
-    def plan(self, As, Bs, variant=0, mfma_min_products=0, tuning=0):
+    def plan(self, As, Bs, variant=0, mfma_min_products=0, tuning=0, sum_order=0):
         """Product (or batch, for lists) plus a plan that recomputes the values
         from the operands' current values as one captured hipGraph."""
-        return Plan(self, As, Bs, Options(SCHED_PLAIN, variant, mfma_min_products, 0, 1.0, tuning))
+        return Plan(self, As, Bs, Options(SCHED_PLAIN, variant, mfma_min_products, 0, 1.0, tuning, sum_order))
 
     # ---- autotuner
     def autotune(self, As, Bs, reps=3):

@@ -36,9 +36,14 @@
 //     in k order, so each output sums its products in ascending k with one
 //     FMA each -- the order of the tiled method (tile pairs ascending, entries
 //     ascending within a pair) -- and values are bit-identical to the other
-//     kernels.  Numeric work items are ranges of a tile-row's values (about
-//     one LDS load of values each), so a heavy tile-row spreads over many
-//     workgroups.
+//     kernels.  With sum_order 1 (template flag ANY) the staged products of a
+//     batch are instead added by all lanes at once with LDS atomics, in any
+//     order (values equal to rounding, not bit for bit).  Numeric work items
+//     are ranges of a tile-row's values (about eight sub-windows' worth), so a
+//     heavy tile-row spreads over many workgroups; an entry's B position
+//     carries from one sub-window to the next within an item (the first
+//     RGN_CUR entries of a tile-row in LDS, the next RGN_GCUR in a global area
+//     per workgroup), so only an item's first sub-window searches B.
 #include <algorithm>
 #include <climits>
 
@@ -53,7 +58,7 @@ constexpr int RGW_GMAX = 64;           // windows per work item (cap)
 constexpr int RGN_COLS = 4096;         // tile-column span of a numeric sub-window (LDS map)
 constexpr int RGN_BYTES = 36864;       // LDS bytes of a numeric sub-window: RGN_TILE_BYTES a tile + its values
 constexpr int RGN_CUR = 512;           // A entries of a tile-row whose B positions carry over sub-windows in LDS
-constexpr int64_t RGN_GCUR = 32768;    // and up to this many more, in a global area per workgroup
+constexpr int64_t RGN_GCUR = 262144;   // and up to this many more, in a global area per workgroup
 constexpr int RGN_TILE_BYTES = 52;     // 16 row masks, 16 row offsets and the tile's value offset
 constexpr int RGW_LONG = 64;           // B segments longer than this are walked by the workgroup
 constexpr int RGW_ILP = 4;             // A entries per thread per symbolic pass (independent loads)
@@ -300,9 +305,11 @@ struct RgwNumLDS {
 __host__ __device__ constexpr int rgn_val_offset(int nt) { return (nt * RGN_TILE_BYTES + 15) & ~15; }
 
 // Values of one numeric work item: the tiles of one tile-row whose value
-// offsets start in one block of this many values (a few sub-windows' worth).
+// offsets start in one block of this many values (about eight sub-windows'
+// worth: each item searches B once per A entry of its tile-row, so items much
+// smaller cost searches; much larger ones leave the grid's tail unbalanced).
 template <typename V>
-constexpr int rgw_item_values() { return 2 * RGN_BYTES / (int)sizeof(V); }
+constexpr int rgw_item_values() { return 8 * RGN_BYTES / (int)sizeof(V); }
 
 // Numeric work items of each listed row: woffn[q + 1] = ceil(row values / item values).
 __global__ void __launch_bounds__(256)
@@ -382,7 +389,7 @@ __device__ __forceinline__ int rgw_slot(const uint16_t *map, const uint16_t *msk
     return off < nv ? off : -1;
 }
 
-template <typename V>
+template <typename V, bool ANY>
 __global__ void __launch_bounds__(RGW_THREADS)
 k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ dlo,
               const int64_t *__restrict__ dhi, const int64_t *__restrict__ woffn,
@@ -600,12 +607,17 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                     if (e0 + WAVE < eb)  // wave-uniform
                         rgn_head<V>(e0 + WAVE, eb, s_rp[0], first, s.cur, s.hi, gb, gcap, valA, colA, rpB, colB, valB,
                                     lo_col, plo, phi, pa, pjj, pbv);
-                    uint64_t mk = __ballot(any);
+                    if (ANY) {  // every lane adds its staged products at once, in any order
+#pragma unroll
+                        for (int t = 0; t < RGN_PF; t++)
+                            if (sl[t] >= 0) atomicAdd(&vals[sl[t]], a * bv[t]);
+                    }
+                    uint64_t mk = ANY ? __ballot(more) : __ballot(any);
                     const uint64_t mm = __ballot(more);
                     while (mk) {  // entries in ascending k; one entry's outputs are distinct
                         const int sgi = __ffsll((unsigned long long)mk) - 1;
                         mk &= mk - 1;
-                        if (l == sgi) {  // the owner adds its staged products (all reads, then all writes)
+                        if (!ANY && l == sgi) {  // the owner adds its staged products (all reads, then all writes)
                             V old[RGN_PF];
 #pragma unroll
                             for (int t = 0; t < RGN_PF; t++) old[t] = vals[sl[t] >= 0 ? sl[t] : 0];
@@ -703,24 +715,26 @@ int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
 
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                       const int32_t *rows, const BinRange &br, int64_t *woff, const int64_t *woffn,
-                      int64_t *gcur, const SlotLayout &sl) {
+                      int64_t *gcur, const SlotLayout &sl, bool any_order) {
     const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
     const int64_t gcap = gcur ? rgn_gcap(A) : 0;
     unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
     k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);
-    if (A->dtype == TSG_F32)
-        k_rgw_numeric<float><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
-            rows, begin, count, br.lo, br.hi, woffn, ctr, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr,
-            B->row_col, (const float *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
-            C->tile_csr_Ptr, C->tile_nnz, (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done,
-            gcur, gcap);
-    else
-        k_rgw_numeric<double><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(
-            rows, begin, count, br.lo, br.hi, woffn, ctr, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr,
-            B->row_col, (const double *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,
-            C->tile_csr_Ptr, C->tile_nnz, (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done,
-            gcur, gcap);
+#define TSG_RG_NUMERIC(V, ANY)                                                                                       \
+    k_rgw_numeric<V, ANY><<<rgw_grid(ctx, 3), RGW_THREADS, 0, ctx->stream>>>(                                                    \
+        rows, begin, count, br.lo, br.hi, woffn, ctr, A->m, A->row_ptr, A->row_col, (const V *)A->row_val,           \
+        B->row_ptr, B->row_col, (const V *)B->row_val, C->tile_ptr, sl.row_nc, C->tile_columnidx, C->mask,           \
+        C->tile_csr_Ptr, C->tile_nnz, (V *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done,  \
+        gcur, gcap)
+    if (A->dtype == TSG_F32) {
+        if (any_order) TSG_RG_NUMERIC(float, true);
+        else TSG_RG_NUMERIC(float, false);
+    } else {
+        if (any_order) TSG_RG_NUMERIC(double, true);
+        else TSG_RG_NUMERIC(double, false);
+    }
+#undef TSG_RG_NUMERIC
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }

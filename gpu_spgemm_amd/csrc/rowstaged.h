@@ -57,6 +57,7 @@ struct BinTable {
     int32_t rg;          // 1: the row-by-row bin exists
     int32_t rg_density;  // A entries per A tile at or below which a heavy row goes row-by-row
     int32_t rl_any;      // 1: bin R runs its own kernels whatever its size (var_mask bit 4)
+    int32_t any_order;   // 1: the row-by-row numeric kernel may add products in any order (sum_order 1)
 };
 constexpr int RG_DENSITY_DEFAULT = 64;
 
@@ -101,6 +102,7 @@ struct Plan {
     const int64_t *d_bins = nullptr;
     int64_t bin_cap[MAX_BINS] = {};
     bool rl_any = false;  // BinTable::rl_any
+    bool any_order = false;  // BinTable::any_order
     // device address of the capacity flag (kernels take it as int32_t *)
     int32_t *overflow() const { return flags ? &flags->cap : nullptr; }
 };

@@ -866,8 +866,10 @@ BinTable bin_table_for(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
     // the row-by-row bin needs a per-workgroup cursor area far larger than the
     // no-alloc scratch: inside such a call heavy rows take the C-tile kernels
     // (every path computes the same bits)
-    return bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts) && !ctx->arena, cfg.rg_density, cfg.var_mask,
-                     num_rows_ok(A, B));
+    BinTable t = bin_table(staged_ok(A, B, opts), rg_ok(A, B, opts) && !ctx->arena, cfg.rg_density, cfg.var_mask,
+                           num_rows_ok(A, B));
+    t.any_order = (opts && opts->sum_order == 1) ? 1 : 0;
+    return t;
 }
 
 BinTable bin_table(bool ok, bool rg, int rg_density, int var_mask, bool row_lanes) {
@@ -914,6 +916,7 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
     p.nvar = bt.nvar;
     p.rg = bt.rg != 0;
     p.rl_any = bt.rl_any != 0;
+    p.any_order = bt.any_order != 0;
     p.nbins = bt.nvar + 1 + (p.rg ? 1 : 0);
     TSG_CHECK(palloc(ctx, &p.rows, (size_t)tilem));
     TSG_CHECK(palloc(ctx, &p.bin, (size_t)tilem));
@@ -1126,7 +1129,9 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
         }
     }
     TSG_CHECK(launch_numeric_list(ctx, A, B, C, p.rows, bin_range(p, fb, fb + 1), sl));
-    if (p.rg) TSG_CHECK(launch_rg_numeric(ctx, A, B, C, p.rows, bin_range(p, fb + 1, fb + 2), p.rgw, p.rgn, p.rgc, sl));
+    if (p.rg)
+        TSG_CHECK(launch_rg_numeric(ctx, A, B, C, p.rows, bin_range(p, fb + 1, fb + 2), p.rgw, p.rgn, p.rgc, sl,
+                                    p.any_order));
     return TSG_OK;
 }
 

@@ -231,9 +231,11 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
 // *gcur: the numeric kernel's per-workgroup cursor area, allocated on first use.
 int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, const int32_t *rows,
                        const BinRange &br, int64_t *woffn, int64_t **gcur, const SlotLayout &sl);
+// any_order: k_rgw_numeric adds the staged products of a batch of A entries with LDS
+// atomics, all lanes at once, instead of entry by entry in ascending k (sum_order 1).
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                       const int32_t *rows, const BinRange &br, int64_t *woff, const int64_t *woffn,
-                      int64_t *gcur, const SlotLayout &sl);
+                      int64_t *gcur, const SlotLayout &sl, bool any_order = false);
 int check_operands(const tsg_tile_matrix *A, const tsg_tile_matrix *B);  // api.hip
 inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 // Looks up the no-alloc scratch's device address on ctx's device (current).

@@ -148,7 +148,11 @@ typedef struct tsg_options {
                                margin, a test hook that forces the exact fallback) */
     int32_t tuning;         /* 0 = use the context's autotuned configuration for this operand class when
                                tsg_autotune stored one; 1 = built-in defaults */
-    int32_t reserved;
+    int32_t sum_order;      /* 0 = every output adds its products in ascending k, one FMA each (the
+                               reference's tiled order, src/spgemm_raw_kernels_v2.cuh:83-94): values are
+                               bit-identical across kernels and runs; 1 = the row-by-row numeric kernel
+                               (heavy sparse tile-rows) adds products in any order with LDS atomics:
+                               faster, values equal to rounding (not bit-reproducible) */
 } tsg_options;
 
 typedef struct tsg_stats {

@@ -85,6 +85,26 @@ def test_rmat20_exact(ctx, rmat20):
     ctx.pool_trim()   # 162 GB of freed C back to the device for the next context
 
 
+def test_rmat20_any_order(ctx, rmat20):
+    """sum_order=1 (heavy-row products added in any order): the same tile set,
+    masks and col bytes as the exact pipeline, values within the FP32
+    tolerance of the Gustavson recompute on the sampled blocks."""
+    A = rmat20["A"]
+    print("[case] R-MAT 20 exact pipeline, any-order sums", flush=True)
+    tA, tB = ctx.tiles_from_csr(A), ctx.tiles_from_csr(A, as_b=True)
+    C, st = ctx.spgemm(tA, tB, sum_order=1)
+    print(f"    numtileC {st['numtileC']} nnzC {st['nnzC']}", flush=True)
+    assert st["numtileC"] == rmat20["tiles"] and st["nnzC"] == rmat20["nnz"]
+    got = check_rmat20_blocks(ctx, C, A, tiled_idx=(), label="any-order ")
+    if "exact_blocks" in rmat20:
+        for i, (e, l) in enumerate(zip(rmat20["exact_blocks"], got)):
+            assert_tiles_equal(l, e, keys=CKEYS[:-1], label=f"any-order block {i} ")
+    C.free()
+    tA.free()
+    tB.free()
+    ctx.pool_trim()
+
+
 def test_rmat20_learned(rmat20):
     """The learned scheduler on its own configuration: trained on R-MATs of
     scales 17-19 with other seeds and a mixed family (never this product), it

@@ -283,6 +283,37 @@ def test_heavy_rows_paths_agree(ctx, name, f32):
         check(got0, A, B, f32=f32)
 
 
+@pytest.mark.parametrize("name", ["er20000x20", "er70000x32", "rmat12", "rmat15", "er_rect"])
+@pytest.mark.parametrize("f32", [False, True])
+def test_any_order_sums_heavy_rows(ctx, name, f32):
+    """sum_order=1: the row-by-row numeric kernel adds products with LDS atomics
+    in any order.  C's structure and col bytes stay bit-identical to the
+    ascending-k pipeline's; values agree with the Gustavson recompute within
+    the FP tolerance (max |c - c_ref| / sum |a b|).  rmat15 has B rows long
+    enough that segments go to a wave each."""
+    if name == "er20000x20":
+        A = g.gen_er(20000, 20, seed=5); B = A
+    elif name == "er70000x32":
+        A = g.gen_er(70000, 32, seed=8); B = A
+    elif name == "rmat12":
+        A = g.gen_rmat(12, 16, seed=3, dtype=g.F64); B = A
+    elif name == "rmat15":
+        A = g.gen_rmat(15, 16, seed=2, dtype=g.F64); B = A
+    else:
+        A = g.gen_er(3000, 24, seed=6)
+        S = g.gen_er(5000, 30, seed=7).to_scipy()[:3000].tocsr()
+        B = g.HostCSR(3000, 5000, S.indptr, S.indices, S.data)
+    got0, s0 = run(ctx, A, B, f32=f32, label=f"{name} ordered", variant=0)
+    got1, s1 = run(ctx, A, B, f32=f32, label=f"{name} any order", variant=0, sum_order=1)
+    assert s1["numeric_kernels"] & 32, "the row-by-row numeric kernel did not run"
+    assert_tiles_equal(got1, got0, keys=CKEYS[:-1], label="C structure ")
+    chk = O.gustavson_check(A, B, got1)
+    print(f"    gustavson: missing={chk['missing']} extra={chk['extra']} max_scaled={chk['max_scaled']:.3e}",
+          flush=True)
+    assert chk["missing"] == 0 and chk["extra"] == 0
+    assert chk["max_scaled"] <= TOL[g.F32 if f32 else g.F64]
+
+
 def _short_rows_wide_b(n=4096, seed=3):
     """A tridiagonal (3 A entries a scalar row: the row-lane variant's rows)
     times a B whose rows hold 40 columns within +-60: C tile-rows of about 1,600
