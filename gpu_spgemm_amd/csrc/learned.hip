@@ -377,9 +377,8 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     if (timing && !rc) (void)hipEventRecord(ev[0], s);
     if (!rc) rc = palloc(ctx, &C->tile_ptr, (size_t)tilem + 1);
     if (reuse) {  // the plain Step-1 count; the predictions are this operand pair's last ones
-        if (!rc) rc = plan_count(ctx, A, B, C, opts, p, nullptr);
-        if (!rc) rc = plan_bin(ctx, p, tilem, C->tile_ptr + tilem, nullptr);
-        if (!rc) { rc = plan_read(ctx, p, &C->numtile, nullptr); syncs++; }
+        if (!rc) rc = plan_count(ctx, A, B, C, opts, p, nullptr, !plan_tail_fused(tilem));
+        if (!rc) { rc = plan_tail(ctx, p, tilem, C->tile_ptr, nullptr, &C->numtile, nullptr); syncs++; }
         pred_nnz = ru.pred_nnz;
     } else {
         if (!rc) rc = palloc(ctx, &stats, (size_t)std::max(tilem, 1) * RS_NSTAT);

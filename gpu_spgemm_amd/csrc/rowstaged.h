@@ -148,7 +148,14 @@ int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1);
 // A->B links, Step-1 count (with the per-row statistics when stats_out, RS_NSTAT
 // int32 a tile-row) and the scan of C->tile_ptr; clears p.flags.
 int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-               const tsg_options *opts, Plan &p, int32_t *stats_out);
+               const tsg_options *opts, Plan &p, int32_t *stats_out, bool scan = true);
+// The Step-1 tail: scan of tile_ptrC, binning and the header read (numtileC,
+// *total1 when given, bin offsets into p.bin_off).  plan_tail_fused(tilem):
+// two launches and plan_count must have left tile_ptrC unscanned (scan =
+// false); else the separate scan, binning and read (plan_count scanned).
+bool plan_tail_fused(int32_t tilem);
+int plan_tail(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const int64_t *total1, int64_t *numtile,
+              int64_t *t1);
 int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
                  const Plan &p, const SlotLayout &sl);
 // Bits of tsg_stats.numeric_kernels for the plan's host-known bins.

@@ -66,8 +66,10 @@ __device__ __forceinline__ uint32_t below(int c) { return (0xFFFF0000u >> c) & 0
 // count} and lpay[t] = its first payload entry, for A tile t in column k.
 __global__ void __launch_bounds__(256)
 k_ab_link(int64_t numtileA, const int32_t *__restrict__ colidxA, const int64_t *__restrict__ tile_ptrB,
-          const int64_t *__restrict__ rnpB, int4 *__restrict__ link, int64_t *__restrict__ lpay) {
+          const int64_t *__restrict__ rnpB, int4 *__restrict__ link, int64_t *__restrict__ lpay,
+          int32_t *__restrict__ zero, int nzero) {
     const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < nzero) zero[t] = 0;  // the plan's flags and counters (no launch of their own)
     if (t >= numtileA) return;
     const int32_t k = colidxA[t];
     const int64_t bs = tile_ptrB[k], be = tile_ptrB[k + 1];
@@ -140,19 +142,13 @@ __device__ __forceinline__ void row_result(int32_t bi, int64_t na, int64_t pa, i
 // at most 64 B tiles count distinct columns with a 2048-column LDS bitmap when
 // the columns span no more, else with one wave-wide sort; larger rows use an
 // LDS bitmap over successive column windows.
-__global__ void __launch_bounds__(RS_WAVES *WAVE)
-k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
-            const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
-            int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats,
-            int32_t *__restrict__ hlist, int32_t *__restrict__ hcount) {
-    __shared__ uint32_t s_bm[RS_WAVES][RS_WIN_WORDS];
-    __shared__ int32_t s_bs[RS_WAVES][WAVE];
-    __shared__ int32_t s_pre[RS_WAVES][WAVE + 1];
-    const int w = threadIdx.x / WAVE, lane = lane_id();
-    const int32_t bi = blockIdx.x * RS_WAVES + w;
-    // the tile-rows k_row_light left (marked in bin)
-    if (bi >= tilemA || bin[bi] != BIN_REST) return;
-    uint32_t *bm = s_bm[w];
+__device__ __forceinline__ void
+row_stats_one(int32_t bi, const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA,
+              const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, const BinTable &bt,
+              int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats,
+              int32_t *__restrict__ hlist, int32_t *__restrict__ hcount, uint32_t *bm, int32_t *s_bsw,
+              int32_t *s_prew) {
+    const int lane = lane_id();
     const int64_t as = tile_ptrA[bi], ae = tile_ptrA[bi + 1];
     const int64_t na = ae - as;
     int32_t cmin = INT_MAX, cmax = -1;
@@ -166,19 +162,19 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
         pb = last_lane(dpp_incl_sum(l.z));
         if (nb <= WAVE) {
             if (lane < na) {
-                s_pre[w][lane + 1] = incl;
-                s_bs[w][lane] = l.x;
+                s_prew[lane + 1] = incl;
+                s_bsw[lane] = l.x;
             }
-            if (lane == 0) s_pre[w][0] = 0;
+            if (lane == 0) s_prew[0] = 0;
             wave_sync();
             int32_t j = 0;
             bool seg_first = false, seg_last = false;  // first / last B tile of its A tile's B tile-row
             if (lane < nb) {
-                const int t = owner_of(s_pre[w], (int)na, lane);
-                const int pos = lane - s_pre[w][t];
-                j = colidxB[s_bs[w][t] + pos];
+                const int t = owner_of(s_prew, (int)na, lane);
+                const int pos = lane - s_prew[t];
+                j = colidxB[s_bsw[t] + pos];
                 seg_first = pos == 0;
-                seg_last = lane == s_pre[w][t + 1] - 1;
+                seg_last = lane == s_prew[t + 1] - 1;
             }
             if (stats) {  // learned-scheduler features from the columns already loaded
                 // each A tile's span = last - first + 1 of its B tile-row's columns
@@ -249,17 +245,17 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
                 }
                 const int32_t incl = wave_incl_sum(len);
                 const int32_t total = __shfl(incl, WAVE - 1, WAVE);
-                s_pre[w][lane] = incl;
-                s_bs[w][lane] = bs;
+                s_prew[lane] = incl;
+                s_bsw[lane] = bs;
                 wave_sync();
                 for (int32_t t = lane; t < total; t += WAVE) {
                     int lo = 0, hi = WAVE - 1;
                     while (lo < hi) {
                         int mid = (lo + hi) >> 1;
-                        if (s_pre[w][mid] > t) hi = mid; else lo = mid + 1;
+                        if (s_prew[mid] > t) hi = mid; else lo = mid + 1;
                     }
-                    const int32_t excl = lo ? s_pre[w][lo - 1] : 0;
-                    const int64_t off = (int64_t)colidxB[s_bs[w][lo] + (t - excl)] - w0;
+                    const int32_t excl = lo ? s_prew[lo - 1] : 0;
+                    const int64_t off = (int64_t)colidxB[s_bsw[lo] + (t - excl)] - w0;
                     if (off >= 0 && off < wbits) atomicOr(&bm[off >> 5], 1u << (off & 31));
                 }
                 wave_sync();
@@ -273,6 +269,31 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     if (lane == 0)
         row_result(bi, na, tile_nnzA[ae] - tile_nnzA[as], nb, pb, count, cmin, cmax, f_maxlen, f_sumspan,
                    f_nonempty, bt, tile_ptrC, bin, stats);
+}
+
+// One wave per chunk of RS_CHUNK consecutive tile-rows: the rows k_row_light
+// left (marked BIN_REST), taken one after the other.  A grid of chunks, not of
+// rows, so a product with no such row costs a short launch.
+constexpr int RS_CHUNK = 8;
+__global__ void __launch_bounds__(RS_WAVES *WAVE)
+k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
+            const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
+            int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats,
+            int32_t *__restrict__ hlist, int32_t *__restrict__ hcount) {
+    __shared__ uint32_t s_bm[RS_WAVES][RS_WIN_WORDS];
+    __shared__ int32_t s_bs[RS_WAVES][WAVE];
+    __shared__ int32_t s_pre[RS_WAVES][WAVE + 1];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int64_t c0 = ((int64_t)blockIdx.x * RS_WAVES + w) * RS_CHUNK;
+    const bool marked = lane < RS_CHUNK && c0 + lane < tilemA && bin[c0 + lane] == BIN_REST;
+    uint64_t mk = __ballot(marked);
+    while (mk) {  // wave-uniform
+        const int32_t bi = (int32_t)(c0 + __ffsll((unsigned long long)mk) - 1);
+        mk &= mk - 1;
+        row_stats_one(bi, tile_ptrA, tile_nnzA, link, colidxB, bt, tile_ptrC, bin, stats, hlist, hcount, s_bm[w],
+                      s_bs[w], s_pre[w]);
+        wave_sync();
+    }
 }
 
 // Step-1 count of light tile-rows (at most 16 A tiles, 32 B tiles reached,
@@ -438,6 +459,190 @@ __global__ void k_bin_header(const int64_t *__restrict__ counts, int nbins, int6
     if (t == 0) hdr[0] = total0 ? *total0 : 0;
     if (t == 1) hdr[1] = total1 ? *total1 : 0;
     if (t <= nbins) hdr[2 + t] = counts[(int64_t)t * nblocks];
+}
+
+// ---------------------------------------------------------------- Step-1 tail in two launches
+// After the Step-1 count: the exclusive scan of the tile-rows' C tile counts
+// (tile_ptrC, the reference's first ExclusiveSum, src/spgemm_raw_v2.cu:131-138),
+// the stable partition of the tile-rows by bin, and the header the host reads
+// (numtileC, bin offsets) -- two kernels instead of a scan (2), the bin count,
+// its scan, the scatter and the publish.  A block takes RT_TILE consecutive
+// tile-rows; the apply kernel's blocks add up the earlier blocks' partials
+// themselves (at most RT_MAX_BLOCKS of them).
+constexpr int RT_THREADS = 256;
+constexpr int RT_ITEMS = 8;
+constexpr int RT_TILE = RT_THREADS * RT_ITEMS;
+constexpr int64_t RT_MAX_BLOCKS = 2048;
+
+// partial[b] = C tiles of block b's rows; bc[bin * nb + b] = its rows in each bin.
+__global__ void __launch_bounds__(RT_THREADS)
+k_rows_reduce(const int64_t *__restrict__ tile_ptrC, const uint8_t *__restrict__ bin, int64_t n, int nbins,
+              int64_t nb, int64_t *__restrict__ partial, int32_t *__restrict__ bc) {
+    __shared__ int64_t s_w[RT_THREADS / WAVE];
+    __shared__ int32_t s_c[RT_THREADS / WAVE][MAX_BINS];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int64_t base = (int64_t)blockIdx.x * RT_TILE;
+    int64_t sum = 0;
+    int32_t c[MAX_BINS] = {};
+#pragma unroll
+    for (int i = 0; i < RT_ITEMS; i++) {
+        const int64_t idx = base + i * RT_THREADS + threadIdx.x;
+        const int b = idx < n ? bin[idx] : 0xFF;
+        sum += idx < n ? tile_ptrC[idx + 1] : 0;
+#pragma unroll
+        for (int bb = 0; bb < MAX_BINS; bb++) c[bb] += __popcll(__ballot(b == bb));
+    }
+    sum = wave_sum(sum);
+    if (lane == 0) {
+        s_w[w] = sum;
+#pragma unroll
+        for (int bb = 0; bb < MAX_BINS; bb++) s_c[w][bb] = c[bb];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t t = 0;
+        for (int q = 0; q < RT_THREADS / WAVE; q++) t += s_w[q];
+        partial[blockIdx.x] = t;
+    }
+    if (threadIdx.x < nbins) {
+        int32_t t = 0;
+        for (int q = 0; q < RT_THREADS / WAVE; q++) t += s_c[q][threadIdx.x];
+        bc[(int64_t)threadIdx.x * nb + blockIdx.x] = t;
+    }
+}
+
+// Block b: its rows' tile_ptrC (inclusive scan of the counts plus the earlier
+// blocks' tiles), its rows' places in the bin-ordered row list, and (block 0)
+// the header hdr[0] = numtileC, hdr[1] = *total1 (0 when null), hdr[2 + bin] = the bin's first
+// slot (bin <= nbins), also published to the host's pinned words (pub, then
+// the sequence number seq; pub null: not published).
+__global__ void __launch_bounds__(RT_THREADS)
+k_rows_apply(int64_t *__restrict__ tile_ptrC, const uint8_t *__restrict__ bin, int64_t n, int nbins, int64_t nb,
+             const int64_t *__restrict__ partial, const int32_t *__restrict__ bc, int32_t *__restrict__ rows,
+             const int64_t *__restrict__ total1, int64_t *__restrict__ hdr, int32_t *pub, int64_t seq) {
+    __shared__ int64_t s_d[RT_TILE + RT_TILE / 32];
+    __shared__ int64_t s_w[RT_THREADS / WAVE];
+    __shared__ int64_t s_red[RT_THREADS / WAVE][2 * MAX_BINS + 2];
+    __shared__ int64_t s_base[MAX_BINS];
+    __shared__ int64_t s_hdr[MAX_BINS + 3];
+    __shared__ int s_wc[RT_THREADS / WAVE][MAX_BINS];
+    const int w = threadIdx.x / WAVE, lane = lane_id();
+    const int64_t blk = blockIdx.x;
+    // the earlier blocks' tiles, every block's tiles, each bin's rows (all
+    // blocks) and its rows in the earlier blocks
+    int64_t carry = 0, total = 0, tot[MAX_BINS] = {}, pre[MAX_BINS] = {};
+    for (int64_t j = threadIdx.x; j < nb; j += RT_THREADS) {
+        const int64_t pj = partial[j];
+        total += pj;
+        carry += j < blk ? pj : 0;
+#pragma unroll
+        for (int bb = 0; bb < MAX_BINS; bb++) {
+            const int64_t v = bb < nbins ? bc[(int64_t)bb * nb + j] : 0;
+            tot[bb] += v;
+            pre[bb] += j < blk ? v : 0;
+        }
+    }
+    carry = wave_sum(carry);
+    total = wave_sum(total);
+#pragma unroll
+    for (int bb = 0; bb < MAX_BINS; bb++) {
+        tot[bb] = wave_sum(tot[bb]);
+        pre[bb] = wave_sum(pre[bb]);
+    }
+    if (lane == 0) {
+        s_red[w][0] = carry;
+        s_red[w][1] = total;
+#pragma unroll
+        for (int bb = 0; bb < MAX_BINS; bb++) {
+            s_red[w][2 + bb] = tot[bb];
+            s_red[w][2 + MAX_BINS + bb] = pre[bb];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t r[2 * MAX_BINS + 2] = {};
+        for (int q = 0; q < RT_THREADS / WAVE; q++)
+            for (int k = 0; k < 2 * MAX_BINS + 2; k++) r[k] += s_red[q][k];
+        s_red[0][0] = r[0];  // carry
+        int64_t first = 0;  // first slot of bin bb = rows in the bins before it
+        s_hdr[0] = r[1];
+        s_hdr[1] = total1 ? *total1 : 0;
+        for (int bb = 0; bb <= nbins; bb++) {
+            s_hdr[2 + bb] = first;
+            if (bb < nbins) {
+                s_base[bb] = first + r[2 + MAX_BINS + bb];
+                first += r[2 + bb];
+            }
+        }
+    }
+    __syncthreads();
+    if (blk == 0) {  // the header: device copy, then the host's words and sequence number
+        const int nw = 2 * (nbins + 3);
+        if ((int)threadIdx.x < nbins + 3) hdr[threadIdx.x] = s_hdr[threadIdx.x];
+        if (pub) {
+            if ((int)threadIdx.x < nw) pub[threadIdx.x] = reinterpret_cast<const int32_t *>(s_hdr)[threadIdx.x];
+            __threadfence_system();
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(reinterpret_cast<int64_t *>(pub + PUB_WORDS), seq, __ATOMIC_RELEASE,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+    // tile_ptrC of this block's rows: counts at [base + 1, base + RT_TILE]
+    const int64_t base = blk * RT_TILE;
+    auto pad = [](int i) { return i + (i >> 5); };
+#pragma unroll
+    for (int i = 0; i < RT_ITEMS; i++) {
+        const int li = i * RT_THREADS + threadIdx.x;
+        const int64_t idx = base + li;
+        s_d[pad(li)] = idx < n ? tile_ptrC[idx + 1] : 0;
+    }
+    __syncthreads();
+    int64_t v[RT_ITEMS];
+    int64_t run = 0;
+#pragma unroll
+    for (int i = 0; i < RT_ITEMS; i++) {
+        run += s_d[pad(threadIdx.x * RT_ITEMS + i)];
+        v[i] = run;
+    }
+    const int64_t incl = wave_incl_sum(run);
+    if (lane == WAVE - 1) s_w[w] = incl;
+    __syncthreads();
+    int64_t off = s_red[0][0] + incl - run;
+    for (int q = 0; q < w; q++) off += s_w[q];
+#pragma unroll
+    for (int i = 0; i < RT_ITEMS; i++) s_d[pad(threadIdx.x * RT_ITEMS + i)] = v[i] + off;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RT_ITEMS; i++) {
+        const int li = i * RT_THREADS + threadIdx.x;
+        const int64_t idx = base + li;
+        if (idx < n) tile_ptrC[idx + 1] = s_d[pad(li)];
+    }
+    // stable partition: rows of bin b ascending from s_base[b]
+    for (int round = 0; round < RT_ITEMS; round++) {
+        const int64_t idx = base + round * RT_THREADS + threadIdx.x;
+        const int b = idx < n ? bin[idx] : 0xFF;
+        int rank = 0;
+        for (int bb = 0; bb < nbins; bb++) {
+            const uint64_t m = __ballot(b == bb);
+            if (b == bb) rank = mbcnt(m);
+            if (lane == 0) s_wc[w][bb] = __popcll(m);
+        }
+        __syncthreads();
+        if (b < nbins) {
+            int64_t o = s_base[b];
+            for (int q = 0; q < w; q++) o += s_wc[q][b];
+            rows[o + rank] = (int32_t)idx;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < nbins) {
+            int t = 0;
+            for (int q = 0; q < RT_THREADS / WAVE; q++) t += s_wc[q][threadIdx.x];
+            s_base[threadIdx.x] += t;
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------- wave helpers
@@ -932,7 +1137,8 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
     if (zero_flags) TSG_CHECK(zero_small(ctx, p.overflow(), PLAN_FLAG_WORDS, nullptr, 0));
     if (A->numtile > 0) {
         k_ab_link<<<nblk(A->numtile, 256), 256, 0, ctx->stream>>>(A->numtile, A->tile_columnidx, B->tile_ptr,
-                                                                   ok ? B->row_nnz_ptr : nullptr, p.link, p.lpay);
+                                                                   ok ? B->row_nnz_ptr : nullptr, p.link, p.lpay,
+                                                                   p.overflow(), PLAN_FLAG_WORDS);
         TSG_HIP(hipGetLastError());
     }
     return TSG_OK;
@@ -965,7 +1171,7 @@ int plan_read(Context *ctx, Plan &p, int64_t *total0, int64_t *total1) {
 
 // A->B links, Step 1 count, statistics, exclusive scan of tile_ptrC.
 int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-               const tsg_options *opts, Plan &p, int32_t *stats_out) {
+               const tsg_options *opts, Plan &p, int32_t *stats_out, bool scan) {
     const BinTable bt = bin_table_for(ctx, A, B, opts);
     const int32_t tilem = A->tilem;
     TSG_CHECK(plan_setup(ctx, A, B, bt, p, tilem == 0));
@@ -988,7 +1194,10 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
             if (!rc) rc = palloc(ctx, reinterpret_cast<int64_t **>(&p.hb.top), 1);
             if (!rc && (B->flags & 2)) rc = build_brow_bitmaps(ctx, B, p.hb);  // long B tile-rows exist
         }
-        if (!rc) rc = zero_small(ctx, p.overflow(), PLAN_FLAG_WORDS, reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
+        // k_ab_link cleared the flags (it runs when A has tiles)
+        if (!rc)
+            rc = zero_small(ctx, A->numtile > 0 ? nullptr : p.overflow(), PLAN_FLAG_WORDS,
+                            reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
         // light tile-rows at 16 lanes each; the rest (marked) a wave each
         if (!rc) {
             k_row_light<<<(unsigned)((tilem + 3) / 4), WAVE, 0, ctx->stream>>>(
@@ -996,7 +1205,7 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_light launch");
         }
         if (!rc) {
-            k_row_stats<<<nblk(tilem, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
+            k_row_stats<<<nblk((tilem + RS_CHUNK - 1) / RS_CHUNK, RS_WAVES), RS_WAVES * WAVE, 0, ctx->stream>>>(
                 A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out,
                 hlist, hc);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_stats launch");
@@ -1004,17 +1213,54 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
         if (!rc && hlist) rc = launch_heavy_count(ctx, A, B, C->tile_ptr, hlist, hc, stats_out, p.hb);
         pool_free(ctx, hlist);
         TSG_CHECK(rc);
-        TSG_CHECK(scan_inclusive_i64(ctx, C->tile_ptr, (int64_t)tilem + 1));
+        if (scan) TSG_CHECK(scan_inclusive_i64(ctx, C->tile_ptr, (int64_t)tilem + 1));
     }
+    return TSG_OK;
+}
+
+bool plan_tail_fused(int32_t tilem) {
+    return tilem > 0 && ((int64_t)tilem + RT_TILE - 1) / RT_TILE <= RT_MAX_BLOCKS;
+}
+
+int plan_tail(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const int64_t *total1, int64_t *numtile,
+              int64_t *t1) {
+    if (!plan_tail_fused(tilem)) {
+        TSG_CHECK(scan_inclusive_i64(ctx, tile_ptrC, (int64_t)tilem + 1));
+        TSG_CHECK(plan_bin(ctx, p, tilem, tile_ptrC + tilem, total1));
+        return plan_read(ctx, p, numtile, t1);
+    }
+    hipStream_t s = ctx->stream;
+    const int64_t nb = ((int64_t)tilem + RT_TILE - 1) / RT_TILE;
+    int64_t *partial = nullptr;
+    int32_t *bc = nullptr;
+    int rc = palloc(ctx, &partial, (size_t)nb);
+    if (!rc) rc = palloc(ctx, &bc, (size_t)(nb * p.nbins));
+    if (!rc) {
+        k_rows_reduce<<<(unsigned)nb, RT_THREADS, 0, s>>>(tile_ptrC, p.bin, tilem, p.nbins, nb, partial, bc);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_rows_reduce launch");
+    }
+    const int nw = 2 * (p.nbins + 3);
+    if (!rc) {
+        const int64_t seq = publish_seq(ctx);
+        k_rows_apply<<<(unsigned)nb, RT_THREADS, 0, s>>>(tile_ptrC, p.bin, tilem, p.nbins, nb, partial, bc, p.rows,
+                                                          total1, p.hdr, ctx->pub_dev, seq);
+        if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_rows_apply launch");
+        if (!rc) rc = publish_wait(ctx, seq, nw, ctx->pinned);
+    }
+    pool_free(ctx, partial);
+    pool_free(ctx, bc);
+    TSG_CHECK(rc);
+    if (numtile) *numtile = ctx->pinned[0];
+    if (t1) *t1 = ctx->pinned[1];
+    for (int b = 0; b <= p.nbins; b++) p.bin_off[b] = ctx->pinned[2 + b];
     return TSG_OK;
 }
 
 // plan_count, binning and one host read of (numtileC, bin offsets).
 static int plan_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                      const tsg_options *opts, Plan &p) {
-    TSG_CHECK(plan_count(ctx, A, B, C, opts, p, nullptr));
-    TSG_CHECK(plan_bin(ctx, p, A->tilem, C->tile_ptr + A->tilem, nullptr));
-    return plan_read(ctx, p, &C->numtile, nullptr);
+    TSG_CHECK(plan_count(ctx, A, B, C, opts, p, nullptr, !plan_tail_fused(A->tilem)));
+    return plan_tail(ctx, p, A->tilem, C->tile_ptr, nullptr, &C->numtile, nullptr);
 }
 
 // Staged kernels over the rows of bins [bin, bin_end) (adjacent in the row list).
@@ -1197,12 +1443,14 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
     exact.overflow = p.overflow();
     if (!rc) rc = run_symbolic(ctx, A, B, C, p, exact);
     free_heavy_bitmaps(ctx, p);  // the fill has used them
-    // tile_nnz[0] = 0 by the scan; nnzC also lands next to the capacity flag
-    if (!rc) rc = scan_inclusive_i64(ctx, C->tile_nnz, C->numtile + 1, true, &p.flags->nnz);
+    // tile_nnz[0] = 0 by the scan; nnzC also lands next to the capacity flag,
+    // and the scan publishes both for the host
+    int64_t seq = 0;
+    if (!rc) rc = scan_publish_i64(ctx, C->tile_nnz, C->numtile + 1, true, &p.flags->nnz, &p.flags->cap, &seq);
     if (!rc) rc = prepare_numeric(ctx, A, C, p, exact);
     if (!rc) {
         // the capacity flag and nnzC in one read
-        rc = read_words(ctx, p.flags, 4, ctx->pinned);
+        rc = publish_wait(ctx, seq, 4, ctx->pinned);
         syncs++;
         C->nnz = ctx->pinned[1];
         if (!rc && *reinterpret_cast<int32_t *>(ctx->pinned))

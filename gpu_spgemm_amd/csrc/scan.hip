@@ -53,8 +53,11 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict
                                                               const int64_t *__restrict__ dlast,
                                                               const int64_t *__restrict__ partial_incl,
                                                               const int64_t *__restrict__ partial_raw,
-                                                              bool zero_first, int64_t *__restrict__ total_out) {
+                                                              bool zero_first, int64_t *__restrict__ total_out,
+                                                              const int32_t *__restrict__ pub_word, int32_t *pub,
+                                                              int64_t seq) {
     __shared__ int64_t s_d[SCAN_TILE + SCAN_TILE / 32];
+    __shared__ int64_t s_tot;
     __shared__ int64_t s_w[SCAN_THREADS / WAVE];
     __shared__ int64_t s_c[SCAN_THREADS / WAVE];
     n = scan_len(n, dlast);
@@ -99,6 +102,19 @@ __global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(int64_t *__restrict
         if (idx < n) {
             data[idx] = s_d[pad(li)];
             if (total_out && idx == n - 1) *total_out = s_d[pad(li)];
+            if (idx == n - 1) s_tot = s_d[pad(li)];
+        }
+    }
+    if (pub && blockIdx.x == gridDim.x - 1) {  // the block holding the total publishes it (dlast unused)
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            pub[0] = *pub_word;
+            pub[1] = 0;
+            pub[2] = (int32_t)(uint32_t)(uint64_t)s_tot;
+            pub[3] = (int32_t)(uint32_t)((uint64_t)s_tot >> 32);
+            __threadfence_system();
+            __hip_atomic_store(reinterpret_cast<int64_t *>(pub + PUB_WORDS), seq, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
 }
@@ -119,13 +135,17 @@ int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb) {
     return TSG_OK;
 }
 
-int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, int64_t *total_out,
-                       const int64_t *dlast) {
+// pub_word: the block holding the total also publishes (*pub_word, 0, total)
+// to the host's words with sequence number seq.
+static int scan_impl(Context *ctx, int64_t *data, int64_t n, bool zero_first, int64_t *total_out,
+                     const int64_t *dlast, const int32_t *pub_word, int64_t seq) {
     if (n <= 0) return TSG_OK;
+    int32_t *const pub = pub_word ? ctx->pub_dev : nullptr;
     int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nb > INT32_MAX) return fail(TSG_ERR_OVERFLOW, "scan of %lld items exceeds grid limit", (long long)n);
     if (nb == 1) {
-        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, dlast, nullptr, nullptr, zero_first, total_out);
+        k_scan_apply<<<1, SCAN_THREADS, 0, ctx->stream>>>(data, n, dlast, nullptr, nullptr, zero_first, total_out,
+                                                          pub_word, pub, seq);
         TSG_HIP(hipGetLastError());
         return TSG_OK;
     }
@@ -140,11 +160,24 @@ int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, 
     if (rc == TSG_OK && !raw) rc = scan_inclusive_i64(ctx, partial, nb);
     if (rc == TSG_OK) {
         k_scan_apply<<<(unsigned)nb, SCAN_THREADS, 0, ctx->stream>>>(data, n, dlast, raw ? nullptr : partial,
-                                                                     raw ? partial : nullptr, zero_first, total_out);
+                                                                     raw ? partial : nullptr, zero_first, total_out,
+                                                                     pub_word, pub, seq);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_scan_apply launch failed");
     }
     pool_free(ctx, partial);
     return rc;
+}
+
+int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, int64_t *total_out,
+                       const int64_t *dlast) {
+    return scan_impl(ctx, data, n, zero_first, total_out, dlast, nullptr, 0);
+}
+
+int scan_publish_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, int64_t *total_out,
+                     const int32_t *word, int64_t *seq) {
+    if (n <= 0) return fail(TSG_ERR_ARG, "scan_publish_i64: empty scan");
+    *seq = publish_seq(ctx);
+    return scan_impl(ctx, data, n, zero_first, total_out, nullptr, word, *seq);
 }
 
 // The words, then (after a system-scope fence by every writer and a barrier)
@@ -159,11 +192,9 @@ __global__ void __launch_bounds__(PUB_WORDS) k_publish(const int32_t *__restrict
                                    __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-int read_words(Context *ctx, const void *dsrc, int n, void *out) {
-    if (n < 0 || n > PUB_WORDS) return fail(TSG_ERR_ARG, "read_words: at most %d words", PUB_WORDS);
-    const int64_t seq = ++ctx->pub_seq;
-    k_publish<<<1, PUB_WORDS, 0, ctx->stream>>>(static_cast<const int32_t *>(dsrc), n, ctx->pub_dev, seq);
-    TSG_HIP(hipGetLastError());
+int64_t publish_seq(Context *ctx) { return ++ctx->pub_seq; }
+
+int publish_wait(Context *ctx, int64_t seq, int n, void *out) {
     const int64_t *flag = reinterpret_cast<const int64_t *>(ctx->pub + PUB_WORDS);
     for (uint64_t it = 1;; it++) {
         if (__atomic_load_n(flag, __ATOMIC_ACQUIRE) == seq) break;
@@ -178,6 +209,14 @@ int read_words(Context *ctx, const void *dsrc, int n, void *out) {
     }
     memcpy(out, ctx->pub, sizeof(int32_t) * (size_t)n);
     return TSG_OK;
+}
+
+int read_words(Context *ctx, const void *dsrc, int n, void *out) {
+    if (n < 0 || n > PUB_WORDS) return fail(TSG_ERR_ARG, "read_words: at most %d words", PUB_WORDS);
+    const int64_t seq = publish_seq(ctx);
+    k_publish<<<1, PUB_WORDS, 0, ctx->stream>>>(static_cast<const int32_t *>(dsrc), n, ctx->pub_dev, seq);
+    TSG_HIP(hipGetLastError());
+    return publish_wait(ctx, seq, n, out);
 }
 
 int read_i64(Context *ctx, const int64_t *dptr, int64_t *out) { return read_words(ctx, dptr, 2, out); }

@@ -136,6 +136,11 @@ constexpr int64_t ROW_CHUNK = int64_t(1) << 23;
 // sums of the blocks past it are zero.
 int scan_inclusive_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first = false,
                        int64_t *total_out = nullptr, const int64_t *dlast = nullptr);
+// The same scan (n >= 1), whose last block also publishes the words
+// (*word, 0, total as int64) for publish_wait(ctx, *seq, 4, ...): the read of a
+// total without a launch of its own.
+int scan_publish_i64(Context *ctx, int64_t *data, int64_t n, bool zero_first, int64_t *total_out,
+                     const int32_t *word, int64_t *seq);
 // Zeroes a[0..na) and b[0..nb) (at most 64 each) with one small kernel on the context's stream.
 int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb);
 // Reads n 32-bit words (n <= PUB_WORDS) of device memory into out after the
@@ -144,6 +149,11 @@ int zero_small(Context *ctx, int32_t *a, int na, int64_t *b, int nb);
 // counts as a host sync).  Replaces a device-to-host copy and a stream sync.
 constexpr int PUB_WORDS = 64;
 int read_words(Context *ctx, const void *dsrc, int n, void *out);
+// The same protocol for a kernel that publishes by itself: it writes n words to
+// ctx->pub_dev, a system-scope fence, then publish_seq()'s number at word
+// PUB_WORDS; publish_wait spins for that number and copies the words to out.
+int64_t publish_seq(Context *ctx);
+int publish_wait(Context *ctx, int64_t seq, int n, void *out);
 // Reads one int64 from device memory (blocking; counts as a host sync).
 int read_i64(Context *ctx, const int64_t *dptr, int64_t *out);
 int read_i32(Context *ctx, const int32_t *dptr, int32_t *out);
