@@ -21,7 +21,7 @@ namespace tsg {
 
 constexpr int RN_GROUPS = 4;   // tile-rows per wave; lane = (tile-row, scalar row)
 constexpr int RN_NC = 16;      // C tiles a tile-row may hold (the small staged variants' cap)
-constexpr int RN_VCAP = 256;   // C values of one tile-row held in LDS at a time
+constexpr int RN_VCAP = 208;   // C values of one tile-row held in LDS at a time (a 2-D 13-point row block: 208)
 constexpr int RN_EB = 6;       // A entries whose loads issue together (a 2-D 5-point row: 5)
 constexpr int RN_BB = 6;       // B row entries whose loads issue together (a 2-D 5-point row: 5)
 

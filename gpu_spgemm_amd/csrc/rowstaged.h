@@ -37,7 +37,7 @@ __device__ __forceinline__ void store_mask16(uint16_t *dst, const uint4 &lo, con
     reinterpret_cast<uint4 *>(dst)[1] = hi;
 }
 
-constexpr int MAX_VAR = 4;               // staged variants
+constexpr int MAX_VAR = 4;               // staged variants (bins R, S, W, M)
 constexpr int MAX_BINS = MAX_VAR + 2;    // + C-tile kernels + row-by-row kernels
 
 // LDS capacities of one staged kernel variant; a tile-row is staged by the

@@ -1347,7 +1347,7 @@ int numeric_kernel_mask(const Plan &p) {
     const int fb = p.nvar;
     if (p.nvar > 0) {
         const bool rn = rows_own(p);
-        if (rn) m |= 1;
+        if (rn && rows(0, 1) > 0) m |= 1;
         if (rows(rn ? 1 : 0, 2) > 0) m |= 2;
         if (rows(2, 3) > 0) m |= 4;
         if (rows(3, 4) > 0) m |= 8;
