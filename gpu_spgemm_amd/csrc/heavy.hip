@@ -59,6 +59,7 @@ constexpr int RGN_COLS = 4096;         // tile-column span of a numeric sub-wind
 constexpr int RGN_BYTES = 36864;       // LDS bytes of a numeric sub-window: RGN_TILE_BYTES a tile + its values
 constexpr int RGN_CUR = 512;           // A entries of a tile-row whose B positions carry over sub-windows in LDS
 constexpr int64_t RGN_GCUR = 262144;   // and up to this many more, in a global area per workgroup
+constexpr int64_t RGS_GCUR = 65536;    // A entries of a tile-row whose B positions carry over symbolic windows
 constexpr int RGN_TILE_BYTES = 52;     // 16 row masks, 16 row offsets and the tile's value offset
 constexpr int RGW_LONG = 64;           // B segments longer than this are walked by the workgroup
 constexpr int RGW_ILP = 4;             // A entries per thread per symbolic pass (independent loads)
@@ -172,7 +173,10 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
                const int32_t *__restrict__ colA, const int64_t *__restrict__ rpB, const int32_t *__restrict__ colB,
                const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ row_nc,
                const int32_t *__restrict__ colidxC, uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC,
-               int64_t *__restrict__ tile_nnzC) {
+               int64_t *__restrict__ tile_nnzC, int64_t *__restrict__ gcur, int64_t gcap) {
+    // this workgroup's cursors: positions [0, gcap), then (entries left, column) pairs
+    int64_t *const gpos = gcur + (int64_t)blockIdx.x * gcap * 2;
+    int2 *const gcn = reinterpret_cast<int2 *>(gpos + gcap);
     __shared__ uint32_t s_m[RGW_COLS * 8];  // two 16-bit row masks per word
     __shared__ RgwSeg s_q[RGW_THREADS];     // long segments of one pass
     __shared__ int64_t s_rp[17];
@@ -194,71 +198,120 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
         if (item >= total) break;  // uniform: every thread read the same item
         rgw_setup(item, rows, row_begin, nr, woff, m, rpA, tile_ptrC, row_nc, colidxC, s_it, s_rp, s_pb);
         const int64_t e0 = s_rp[0], e1 = s_rp[16];
+        bool fresh = true;  // no window of this item processed yet: cursors not yet set
         for (int w = 0; w < s_it.nw; w++) {
             const int32_t c0 = s_it.c0 + w * RGW_COLS;
             const int32_t lo_col = c0 * 16;
             const int64_t hi_col = (int64_t)(c0 + RGW_COLS) * 16;
             const int64_t p0 = s_pb[w], np = s_pb[w + 1] - p0;
-            if (np == 0) continue;  // no C tile in this window (uniform)
+            // no C tile in this window (uniform): no B entry of the tile-row lies
+            // in it either, so the cursors stay valid
+            if (np == 0) continue;
             for (int i = tid; i < RGW_COLS * 8; i += RGW_THREADS) s_m[i] = 0u;
             __syncthreads();
             for (int64_t base = e0; base < e1; base += RGW_THREADS * RGW_ILP) {  // uniform trip count
-                // thread t takes entries base + t + 256u (u < RGW_ILP), loads of all u issued together
-                int32_t k[RGW_ILP], jf[RGW_ILP], jl[RGW_ILP];
+                // thread t takes entries base + t + 256u (u < RGW_ILP).  Each entry's
+                // cursor -- its next B position, the entries left in its B row and
+                // the column there -- carries from one window to the next within the
+                // item (the windows follow in column order), so only the item's
+                // first window looks B up; an entry whose next column lies past the
+                // window touches no B memory.
+                int64_t pos[RGW_ILP];
+                int32_t rem[RGW_ILP], nx[RGW_ILP], k[RGW_ILP];
+                bool dirty[RGW_ILP];
+#pragma unroll
+                for (int u = 0; u < RGW_ILP; u++) {
+                    const int64_t e = base + tid + RGW_THREADS * u, ci = e - e0;
+                    pos[u] = 0;
+                    rem[u] = 0;
+                    nx[u] = INT_MAX;
+                    dirty[u] = false;
+                    k[u] = -1;
+                    if (e < e1) {
+                        if (!fresh && ci < gcap) {
+                            pos[u] = gpos[ci];
+                            const int2 c = gcn[ci];
+                            rem[u] = c.x;
+                            nx[u] = c.y;
+                        } else {
+                            k[u] = colA[e];
+                        }
+                    }
+                }
                 int64_t b0[RGW_ILP], b1[RGW_ILP];
 #pragma unroll
                 for (int u = 0; u < RGW_ILP; u++) {
+                    b0[u] = k[u] >= 0 ? rpB[k[u]] : 0;
+                    b1[u] = k[u] >= 0 ? rpB[k[u] + 1] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < RGW_ILP; u++) {
+                    if (k[u] < 0) continue;
                     const int64_t e = base + tid + RGW_THREADS * u;
-                    k[u] = colA[e < e1 ? e : e0];  // clamped: the load stays in A
+                    pos[u] = rgw_seek(colB, b0[u], b1[u], lo_col);
+                    rem[u] = (int32_t)(b1[u] - pos[u]);
+                    nx[u] = rem[u] > 0 ? colB[pos[u]] : INT_MAX;
+                    dirty[u] = e - e0 < gcap;
                 }
 #pragma unroll
                 for (int u = 0; u < RGW_ILP; u++) {
-                    const int64_t e = base + tid + RGW_THREADS * u;
-                    b0[u] = rpB[k[u]];
-                    b1[u] = e < e1 ? rpB[k[u] + 1] : b0[u];  // an entry past the tile-row: empty segment
-                }
-#pragma unroll
-                for (int u = 0; u < RGW_ILP; u++) {
-                    jf[u] = b1[u] > b0[u] ? colB[b0[u]] : INT_MAX;
-                    jl[u] = b1[u] > b0[u] ? colB[b1[u] - 1] : INT_MIN;
-                }
-#pragma unroll
-                for (int u = 0; u < RGW_ILP; u++) {
-                    if (jl[u] < lo_col || jf[u] >= hi_col) continue;  // no entry in the window
+                    if (rem[u] <= 0 || nx[u] >= hi_col) continue;  // no entry in the window
                     const int64_t e = base + tid + RGW_THREADS * u;
                     int r = 0;
 #pragma unroll
                     for (int st = 8; st > 0; st >>= 1)
                         if (r + st <= 15 && s_rp[r + st] <= e) r += st;
-                    const int64_t lo = jf[u] >= lo_col ? b0[u] : lower_bound<int64_t, int32_t>(colB, b0[u], b1[u], lo_col);
-                    int qi = RGW_THREADS;  // long: more than RGW_LONG entries inside the window
-                    if (b1[u] - lo > RGW_LONG && colB[lo + RGW_LONG] < hi_col) qi = atomicAdd(&s_nq, 1);
+                    const int64_t lo = pos[u], hi = lo + rem[u];
+                    int64_t next = hi;      // the first position at or past hi_col
+                    int32_t nxt = INT_MAX;  // and its column
+                    int qi = RGW_THREADS;   // long: more than RGW_LONG entries inside the window
+                    if (hi - lo > RGW_LONG && colB[lo + RGW_LONG] < hi_col) qi = atomicAdd(&s_nq, 1);
                     if (qi < RGW_THREADS) {
-                        s_q[qi] = RgwSeg{lo, b1[u], r};
+                        next = lower_bound<int64_t, int32_t>(colB, lo + RGW_LONG, hi, (int32_t)min<int64_t>(hi_col, INT_MAX));
+                        s_q[qi] = RgwSeg{lo, next, r};
+                        nxt = next < hi ? colB[next] : INT_MAX;
                     } else {
-                        for (int64_t f0 = lo; f0 < b1[u]; f0 += RGW_PF) {  // up to RGW_PF loads issued together
+                        for (int64_t f0 = lo; f0 < hi; f0 += RGW_PF) {  // up to RGW_PF loads issued together
                             int32_t jj[RGW_PF];
 #pragma unroll
                             for (int t = 0; t < RGW_PF; t++) jj[t] = INT_MAX;
 #pragma unroll
                             for (int t = 0; t < RGW_PF0; t++)
-                                if (f0 + t < b1[u]) jj[t] = colB[f0 + t];
-                            if (f0 + RGW_PF0 < b1[u] && jj[RGW_PF0 - 1] < hi_col) {
+                                if (f0 + t < hi) jj[t] = colB[f0 + t];
+                            if (f0 + RGW_PF0 < hi && jj[RGW_PF0 - 1] < hi_col) {
 #pragma unroll
                                 for (int t = RGW_PF0; t < RGW_PF; t++)
-                                    if (f0 + t < b1[u]) jj[t] = colB[f0 + t];
+                                    if (f0 + t < hi) jj[t] = colB[f0 + t];
                             }
-                            bool past = false;
+                            int in = 0;
 #pragma unroll
                             for (int t = 0; t < RGW_PF; t++) {
                                 const int tc = (jj[t] >> 4) - c0;
-                                if (f0 + t < b1[u] && jj[t] < hi_col && (unsigned)tc < (unsigned)RGW_COLS)
+                                const bool inw = f0 + t < hi && jj[t] < hi_col;
+                                if (inw && (unsigned)tc < (unsigned)RGW_COLS)
                                     atomicOr(&s_m[tc * 8 + (r >> 1)], (0x8000u >> (jj[t] & 15)) << ((r & 1) * 16));
-                                past = past || jj[t] >= hi_col;
+                                in += inw ? 1 : 0;
                             }
-                            if (past) break;
+                            if (in < RGW_PF && f0 + in < hi) {  // left the window at f0 + in (sorted columns)
+                                next = f0 + in;
+#pragma unroll
+                                for (int t = 0; t < RGW_PF; t++)
+                                    if (t == in) nxt = jj[t];
+                                break;
+                            }
                         }
                     }
+                    pos[u] = next;
+                    rem[u] = (int32_t)(hi - next);
+                    nx[u] = nxt;
+                    dirty[u] = e - e0 < gcap;
+                }
+#pragma unroll
+                for (int u = 0; u < RGW_ILP; u++) {
+                    const int64_t ci = base + tid + RGW_THREADS * u - e0;
+                    if (!dirty[u]) continue;
+                    gpos[ci] = pos[u];
+                    gcn[ci] = make_int2(rem[u], nx[u]);
                 }
                 __syncthreads();
                 const int nq = min(s_nq, RGW_THREADS);
@@ -289,6 +342,7 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
                 if (rr == 15) tile_nnzC[t + 1] = inc;
             }
             __syncthreads();
+            fresh = false;
         }
     }
 }
@@ -679,10 +733,22 @@ static unsigned rgw_grid(Context *ctx, int per_cu) {
     return (unsigned)std::max(1, ctx->prop.multiProcessorCount * per_cu);
 }
 
+static int64_t rgn_gcap(const tsg_tile_matrix *A) { return std::min<int64_t>(RGN_GCUR, A->nnz); }
+static int64_t rgs_gcap(const tsg_tile_matrix *A) { return std::min<int64_t>(RGS_GCUR, A->nnz); }
+
+// The cursor area both row-by-row kernels use (allocated once per product,
+// at the size of the larger of their grids' areas).
+static int alloc_rg_cursors(Context *ctx, const tsg_tile_matrix *A, int64_t **gcur) {
+    if (*gcur) return TSG_OK;
+    const int64_t need = std::max((int64_t)rgw_grid(ctx, 3) * rgn_gcap(A), (int64_t)rgw_grid(ctx, 4) * rgs_gcap(A));
+    return need > 0 ? palloc(ctx, gcur, (size_t)need * 2) : TSG_OK;
+}
+
 int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                       const int32_t *rows, const BinRange &br, int64_t *woff, const SlotLayout &sl) {
+                       const int32_t *rows, const BinRange &br, int64_t *woff, int64_t **gcur, const SlotLayout &sl) {
     const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
+    TSG_CHECK(alloc_rg_cursors(ctx, A, gcur));
     unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
     k_rgw_count<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, br.lo, br.hi,
                                                                          C->tile_ptr,
@@ -692,19 +758,16 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
     k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);
     k_rgw_symbolic<<<rgw_grid(ctx, 4), RGW_THREADS, 0, ctx->stream>>>(
         rows, begin, count, br.lo, br.hi, woff, ctr, A->m, A->row_ptr, A->row_col, B->row_ptr, B->row_col, C->tile_ptr,
-        sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz);
+        sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, *gcur, *gcur ? rgs_gcap(A) : 0);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
-
-static int64_t rgn_gcap(const tsg_tile_matrix *A) { return std::min<int64_t>(RGN_GCUR, A->nnz); }
 
 int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, const int32_t *rows,
                        const BinRange &br, int64_t *woffn, int64_t **gcur, const SlotLayout &sl) {
     const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
-    if (!*gcur && rgn_gcap(A) > 0)
-        TSG_CHECK(palloc(ctx, gcur, (size_t)rgw_grid(ctx, 3) * (size_t)rgn_gcap(A) * 2));
+    TSG_CHECK(alloc_rg_cursors(ctx, A, gcur));
     const int64_t h = C->dtype == TSG_F32 ? rgw_item_values<float>() : rgw_item_values<double>();
     k_rgw_count_num<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, br.lo, br.hi,
                                                                              C->tile_ptr,

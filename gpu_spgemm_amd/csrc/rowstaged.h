@@ -157,7 +157,7 @@ bool plan_tail_fused(int32_t tilem);
 int plan_tail(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const int64_t *total1, int64_t *numtile,
               int64_t *t1);
 int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                 const Plan &p, const SlotLayout &sl);
+                 Plan &p, const SlotLayout &sl);
 // Bits of tsg_stats.numeric_kernels for the plan's host-known bins.
 int numeric_kernel_mask(const Plan &p);
 int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,

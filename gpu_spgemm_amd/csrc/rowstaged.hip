@@ -1319,7 +1319,7 @@ BinRange bin_range(const Plan &p, int b, int e) {
 // Structure fill + symbolic for every bin.
 
 int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                 const Plan &p, const SlotLayout &sl) {
+                 Plan &p, const SlotLayout &sl) {
     const int fb = p.nvar;  // C-tile kernels
     if (p.nvar > 0) {
         const bool rn = rows_own(p);
@@ -1334,7 +1334,7 @@ int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *
     if (p.rg) {  // row-by-row kernels: Step 1 fill as the C-tile rows, then masks from the products
         const BinRange r = bin_range(p, fb + 1, fb + 2);
         TSG_CHECK(launch_structure_fill_list(ctx, A, B, C, p.rows, r, sl, p.hb));
-        TSG_CHECK(launch_rg_symbolic(ctx, A, B, C, p.rows, r, p.rgw, sl));
+        TSG_CHECK(launch_rg_symbolic(ctx, A, B, C, p.rows, r, p.rgw, &p.rgc, sl));
     }
     return TSG_OK;
 }

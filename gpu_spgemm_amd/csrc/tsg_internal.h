@@ -235,7 +235,7 @@ void tuner_release(Context *ctx);    // autotune.hip
 // count + 3 int64: the rows' work-item offsets (written by the symbolic
 // launch, read by the numeric one) and two work counters.
 int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
-                       const int32_t *rows, const BinRange &br, int64_t *woff, const SlotLayout &sl);
+                       const int32_t *rows, const BinRange &br, int64_t *woff, int64_t **gcur, const SlotLayout &sl);
 // After C's value offsets are scanned: the rows' numeric work-item offsets
 // (count + 1 int64 in woffn).
 // *gcur: the numeric kernel's per-workgroup cursor area, allocated on first use.
