@@ -53,7 +53,7 @@ namespace tsg {
 
 constexpr int RGW_THREADS = 256;       // 16 scalar rows x 16 lanes
 constexpr int RGW_COLS = 1024;         // C tile columns per window
-constexpr int RGW_ITEM_TILES = 4096;   // C tiles per work item (aim)
+constexpr int RGW_ITEM_TILES = 4096;   // C tiles per work item (aim, at most; fewer when C is small)
 constexpr int RGW_GMAX = 64;           // windows per work item (cap)
 constexpr int RGN_COLS = 4096;         // tile-column span of a numeric sub-window (LDS map)
 constexpr int RGN_BYTES = 36864;       // LDS bytes of a numeric sub-window: RGN_TILE_BYTES a tile + its values
@@ -80,10 +80,10 @@ template <>
 __device__ __forceinline__ float rgw_fma<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
 // Windows of a row with C tiles [cs, ce) and how many go in one work item.
-__device__ __forceinline__ void rgw_split(const int32_t *__restrict__ colidxC, int64_t cs, int64_t ce,
+__device__ __forceinline__ void rgw_split(const int32_t *__restrict__ colidxC, int64_t cs, int64_t ce, int item_tiles,
                                           int64_t &nwin, int64_t &per_item) {
     nwin = ce > cs ? (int64_t)(colidxC[ce - 1] - colidxC[cs]) / RGW_COLS + 1 : 0;
-    const int64_t items = max<int64_t>(max<int64_t>(1, (ce - cs + RGW_ITEM_TILES - 1) / RGW_ITEM_TILES),
+    const int64_t items = max<int64_t>(max<int64_t>(1, (ce - cs + item_tiles - 1) / item_tiles),
                                        (nwin + RGW_GMAX - 1) / RGW_GMAX);
     per_item = nwin > 0 ? (nwin + min(items, nwin) - 1) / min(items, nwin) : 1;
 }
@@ -93,7 +93,8 @@ __device__ __forceinline__ void rgw_split(const int32_t *__restrict__ colidxC, i
 __global__ void __launch_bounds__(256)
 k_rgw_count(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ dlo,
             const int64_t *__restrict__ dhi, const int64_t *__restrict__ tile_ptrC,
-            const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC, int64_t *__restrict__ woff) {
+            const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC, int item_tiles,
+            int64_t *__restrict__ woff) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t cover = nr;
     bin_rows(dlo, dhi, row_begin, nr);
@@ -104,7 +105,7 @@ k_rgw_count(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, con
     }
     int64_t cs, ce, nwin, per;
     c_row_range(tile_ptrC, row_nc, rows[row_begin + q], cs, ce);
-    rgw_split(colidxC, cs, ce, nwin, per);
+    rgw_split(colidxC, cs, ce, item_tiles, nwin, per);
     woff[q + 1] = (nwin + per - 1) / per;
 }
 
@@ -125,7 +126,7 @@ __device__ __forceinline__ void rgw_setup(int64_t item, const int32_t *__restric
                                           int64_t nr, const int64_t *__restrict__ woff, int32_t m,
                                           const int64_t *__restrict__ rpA, const int64_t *__restrict__ tile_ptrC,
                                           const int32_t *__restrict__ row_nc, const int32_t *__restrict__ colidxC,
-                                          RgwItem &s_it, int64_t *s_rp, int64_t *s_pb) {
+                                          int item_tiles, RgwItem &s_it, int64_t *s_rp, int64_t *s_pb) {
     if (threadIdx.x == 0) {
         int64_t lo = 0, hi = nr - 1;  // largest q with woff[q] <= item
         while (lo < hi) {
@@ -135,7 +136,7 @@ __device__ __forceinline__ void rgw_setup(int64_t item, const int32_t *__restric
         const int32_t bi = rows[row_begin + lo];
         int64_t cs, ce, nwin, per;
         c_row_range(tile_ptrC, row_nc, bi, cs, ce);
-        rgw_split(colidxC, cs, ce, nwin, per);
+        rgw_split(colidxC, cs, ce, item_tiles, nwin, per);
         const int64_t w0 = (item - woff[lo]) * per;
         s_it.bi = bi;
         s_it.c0 = colidxC[cs] + (int32_t)w0 * RGW_COLS;
@@ -172,8 +173,9 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
                unsigned long long *__restrict__ ctr, int32_t m, const int64_t *__restrict__ rpA,
                const int32_t *__restrict__ colA, const int64_t *__restrict__ rpB, const int32_t *__restrict__ colB,
                const int64_t *__restrict__ tile_ptrC, const int32_t *__restrict__ row_nc,
-               const int32_t *__restrict__ colidxC, uint16_t *__restrict__ maskC, uint8_t *__restrict__ ptrRowC,
-               int64_t *__restrict__ tile_nnzC, int64_t *__restrict__ gcur, int64_t gcap) {
+               const int32_t *__restrict__ colidxC, int item_tiles, uint16_t *__restrict__ maskC,
+               uint8_t *__restrict__ ptrRowC, int64_t *__restrict__ tile_nnzC, int64_t *__restrict__ gcur,
+               int64_t gcap) {
     // this workgroup's cursors: positions [0, gcap), then (entries left, column) pairs
     int64_t *const gpos = gcur + (int64_t)blockIdx.x * gcap * 2;
     int2 *const gcn = reinterpret_cast<int2 *>(gpos + gcap);
@@ -196,7 +198,7 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
         __syncthreads();
         const int64_t item = s_item;
         if (item >= total) break;  // uniform: every thread read the same item
-        rgw_setup(item, rows, row_begin, nr, woff, m, rpA, tile_ptrC, row_nc, colidxC, s_it, s_rp, s_pb);
+        rgw_setup(item, rows, row_begin, nr, woff, m, rpA, tile_ptrC, row_nc, colidxC, item_tiles, s_it, s_rp, s_pb);
         const int64_t e0 = s_rp[0], e1 = s_rp[16];
         bool fresh = true;  // no window of this item processed yet: cursors not yet set
         for (int w = 0; w < s_it.nw; w++) {
@@ -750,15 +752,21 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
     if (count <= 0) return TSG_OK;
     TSG_CHECK(alloc_rg_cursors(ctx, A, gcur));
     unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
+    // about 128 work items a workgroup over C's tiles, RGW_ITEM_TILES at most
+    // (each item searches B once per A entry of its tile-row; smaller C need
+    // smaller items for the grid's tail to stay even)
+    const unsigned grid = rgw_grid(ctx, 4);
+    const int item_tiles = (int)std::max<int64_t>(1024, std::min<int64_t>(RGW_ITEM_TILES, C->numtile / ((int64_t)grid * 128)));
     k_rgw_count<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, br.lo, br.hi,
-                                                                         C->tile_ptr,
-                                                                         sl.row_nc, C->tile_columnidx, woff);
+                                                                         C->tile_ptr, sl.row_nc, C->tile_columnidx,
+                                                                         item_tiles, woff);
     TSG_HIP(hipGetLastError());
     TSG_CHECK(scan_inclusive_i64(ctx, woff, count + 1));
     k_rgw_zero<<<1, 64, 0, ctx->stream>>>(ctr);
-    k_rgw_symbolic<<<rgw_grid(ctx, 4), RGW_THREADS, 0, ctx->stream>>>(
+    k_rgw_symbolic<<<grid, RGW_THREADS, 0, ctx->stream>>>(
         rows, begin, count, br.lo, br.hi, woff, ctr, A->m, A->row_ptr, A->row_col, B->row_ptr, B->row_col, C->tile_ptr,
-        sl.row_nc, C->tile_columnidx, C->mask, C->tile_csr_Ptr, C->tile_nnz, *gcur, *gcur ? rgs_gcap(A) : 0);
+        sl.row_nc, C->tile_columnidx, item_tiles, C->mask, C->tile_csr_Ptr, C->tile_nnz, *gcur,
+        *gcur ? rgs_gcap(A) : 0);
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
