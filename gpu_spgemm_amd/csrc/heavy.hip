@@ -361,22 +361,31 @@ struct RgwNumLDS {
 __host__ __device__ constexpr int rgn_val_offset(int nt) { return (nt * RGN_TILE_BYTES + 15) & ~15; }
 
 // Values of one numeric work item: the tiles of one tile-row whose value
-// offsets start in one block of this many values (about eight sub-windows'
-// worth: each item searches B once per A entry of its tile-row, so items much
-// smaller cost searches; much larger ones leave the grid's tail unbalanced).
+// offsets start in one block of this many values.  Each item searches B once
+// per A entry of its tile-row, so small items cost searches, and large ones
+// leave the grid's tail unbalanced: about 128 items a workgroup over C's
+// values, between two and eight sub-windows' worth (k_rgw_count_num decides
+// from nnz(C) on the device and leaves the figure in ctr[2] for the kernel).
 template <typename V>
-constexpr int rgw_item_values() { return 8 * RGN_BYTES / (int)sizeof(V); }
+constexpr int64_t rgw_item_min() { return 2 * RGN_BYTES / (int)sizeof(V); }
+template <typename V>
+constexpr int64_t rgw_item_max() { return 8 * RGN_BYTES / (int)sizeof(V); }
 
 // Numeric work items of each listed row: woffn[q + 1] = ceil(row values / item values).
 __global__ void __launch_bounds__(256)
 k_rgw_count_num(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, const int64_t *__restrict__ dlo,
                 const int64_t *__restrict__ dhi, const int64_t *__restrict__ tile_ptrC,
-                const int32_t *__restrict__ row_nc, const int64_t *__restrict__ tile_nnzC, int64_t item_values,
-                int64_t *__restrict__ woffn) {
+                const int32_t *__restrict__ row_nc, const int64_t *__restrict__ tile_nnzC,
+                const int64_t *__restrict__ nnzC, int64_t hmin, int64_t hmax, int64_t per,
+                int64_t *__restrict__ woffn, unsigned long long *__restrict__ ctr) {
     const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t cover = nr;
+    const int64_t item_values = min(hmax, max(hmin, nnzC ? *nnzC / per : hmin));
     bin_rows(dlo, dhi, row_begin, nr);
-    if (q == 0) woffn[0] = 0;
+    if (q == 0) {
+        woffn[0] = 0;
+        ctr[2] = (unsigned long long)item_values;
+    }
     if (q >= nr) {
         if (q < cover) woffn[q + 1] = 0;  // past a device-sized bin's rows: no items
         return;
@@ -457,7 +466,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
               const int64_t *__restrict__ tile_nnzC, V *__restrict__ valC, uint8_t *__restrict__ colC,
               int32_t *__restrict__ overflow, int64_t val_cap, int64_t val_done, int64_t *__restrict__ gcur,
               int64_t gcap) {
-    constexpr int64_t H = rgw_item_values<V>();
+    const int64_t H = (int64_t)ctr[2];  // values per work item (k_rgw_count_num)
     // this workgroup's global cursor area: positions [0, gcap), row ends [gcap, 2 gcap)
     int64_t *const gb = gcur + (int64_t)blockIdx.x * gcap * 2;
     __shared__ RgwNumLDS s;
@@ -772,14 +781,18 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
 }
 
 int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, const int32_t *rows,
-                       const BinRange &br, int64_t *woffn, int64_t **gcur, const SlotLayout &sl) {
+                       const BinRange &br, int64_t *woff, int64_t *woffn, int64_t **gcur, const int64_t *nnzC,
+                       const SlotLayout &sl) {
     const int64_t begin = br.begin, count = br.count;
     if (count <= 0) return TSG_OK;
     TSG_CHECK(alloc_rg_cursors(ctx, A, gcur));
-    const int64_t h = C->dtype == TSG_F32 ? rgw_item_values<float>() : rgw_item_values<double>();
-    k_rgw_count_num<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(rows, begin, count, br.lo, br.hi,
-                                                                             C->tile_ptr,
-                                                                             sl.row_nc, C->tile_nnz, h, woffn);
+    const bool f32 = C->dtype == TSG_F32;
+    const int64_t hmin = f32 ? rgw_item_min<float>() : rgw_item_min<double>();
+    const int64_t hmax = f32 ? rgw_item_max<float>() : rgw_item_max<double>();
+    unsigned long long *ctr = reinterpret_cast<unsigned long long *>(woff + count + 1);
+    k_rgw_count_num<<<(unsigned)((count + 256) / 256), 256, 0, ctx->stream>>>(
+        rows, begin, count, br.lo, br.hi, C->tile_ptr, sl.row_nc, C->tile_nnz, nnzC, hmin, hmax,
+        (int64_t)rgw_grid(ctx, 3) * 128, woffn, ctr);
     TSG_HIP(hipGetLastError());
     return scan_inclusive_i64(ctx, woffn, count + 1);
 }

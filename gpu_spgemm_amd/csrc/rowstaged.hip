@@ -1131,7 +1131,7 @@ int plan_setup(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
     TSG_CHECK(palloc(ctx, &p.link, (size_t)A->numtile));
     TSG_CHECK(palloc(ctx, &p.lpay, (size_t)A->numtile));
     if (p.rg) {
-        TSG_CHECK(palloc(ctx, &p.rgw, (size_t)tilem + 3));
+        TSG_CHECK(palloc(ctx, &p.rgw, (size_t)tilem + 4));  // offsets, 2 work counters, numeric item size
         TSG_CHECK(palloc(ctx, &p.rgn, (size_t)tilem + 1));
     }
     if (zero_flags) TSG_CHECK(zero_small(ctx, p.overflow(), PLAN_FLAG_WORDS, nullptr, 0));
@@ -1387,7 +1387,8 @@ int run_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B
 int prepare_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, Plan &p, const SlotLayout &sl) {
     if (!p.rg) return TSG_OK;
     const int fb = p.nvar;
-    return prepare_rg_numeric(ctx, A, C, p.rows, bin_range(p, fb + 1, fb + 2), p.rgn, &p.rgc, sl);
+    return prepare_rg_numeric(ctx, A, C, p.rows, bin_range(p, fb + 1, fb + 2), p.rgw, p.rgn, &p.rgc,
+                              p.flags ? &p.flags->nnz : nullptr, sl);
 }
 
 void free_c(Context *ctx, tsg_tile_matrix *C) {

@@ -240,7 +240,8 @@ int launch_rg_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
 // (count + 1 int64 in woffn).
 // *gcur: the numeric kernel's per-workgroup cursor area, allocated on first use.
 int prepare_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *C, const int32_t *rows,
-                       const BinRange &br, int64_t *woffn, int64_t **gcur, const SlotLayout &sl);
+                       const BinRange &br, int64_t *woff, int64_t *woffn, int64_t **gcur, const int64_t *nnzC,
+                       const SlotLayout &sl);
 // any_order: k_rgw_numeric adds the staged products of a batch of A entries with LDS
 // atomics, all lanes at once, instead of entry by entry in ascending k (sum_order 1).
 int launch_rg_numeric(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
