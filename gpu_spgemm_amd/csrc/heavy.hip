@@ -355,7 +355,7 @@ k_rgw_symbolic(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, 
 struct RgwNumLDS {
     alignas(16) unsigned char buf[RGN_BYTES];
     int64_t cur[RGN_CUR];  // per A entry (first RGN_CUR of the tile-row): next B position
-    int64_t hi[RGN_CUR];   // and the end of its B row
+    int2 cn[RGN_CUR];      // and (entries left in its B row, the column there)
     uint16_t map[RGN_COLS];
 };
 __host__ __device__ constexpr int rgn_val_offset(int nt) { return (nt * RGN_TILE_BYTES + 15) & ~15; }
@@ -402,15 +402,18 @@ struct RgwNumItem {
 };
 
 // Head of a batch of 64 A entries (lane l: entry eh + l, if below eb): its B
-// segment in the sub-window -- the carried position, or a search in the item's
-// first sub-window -- and its first RGW_PF0 products' columns and values.
+// segment in the sub-window -- the carried cursor (position, entries left in
+// the B row, the column there), or a search in the item's first sub-window --
+// and its first RGW_PF0 products' columns and values.  skip: the carried
+// column lies past the sub-window, so the entry has no product in it and
+// nothing of B is loaded.
 template <typename V>
 __device__ __forceinline__ void rgn_head(int64_t eh, int64_t eb, int64_t rp0, bool first, const int64_t *cur,
-                                         const int64_t *chi, const int64_t *gb, int64_t gcap,
+                                         const int2 *ccn, const int64_t *gb, const int2 *gcn, int64_t gcap,
                                          const V *__restrict__ valA, const int32_t *__restrict__ colA,
                                          const int64_t *__restrict__ rpB, const int32_t *__restrict__ colB,
-                                         const V *__restrict__ valB, int32_t lo_col, int64_t &lo, int64_t &hi, V &a,
-                                         int32_t *jj, V *bv) {
+                                         const V *__restrict__ valB, int32_t lo_col, int64_t hi_col, int64_t &lo,
+                                         int64_t &hi, V &a, bool &skip, int32_t *jj, V *bv) {
     const int64_t e = eh + lane_id();
     const int64_t ci = e - rp0;
     const bool in_lds = ci < RGN_CUR;
@@ -418,22 +421,25 @@ __device__ __forceinline__ void rgn_head(int64_t eh, int64_t eb, int64_t rp0, bo
     lo = 0;
     hi = 0;
     a = V(0);
+    skip = false;
     if (e < eb) {
-        a = valA[e];
         if (cached && !first) {
             lo = in_lds ? cur[ci] : gb[ci - RGN_CUR];
-            hi = in_lds ? chi[ci] : gb[gcap + ci - RGN_CUR];
+            const int2 c = in_lds ? ccn[ci] : gcn[ci - RGN_CUR];
+            hi = lo + c.x;
+            skip = c.y >= hi_col;
         } else {
             const int32_t k = colA[e];
             hi = rpB[k + 1];
             lo = rgw_seek(colB, rpB[k], hi, lo_col);
         }
+        if (!skip) a = valA[e];
     }
 #pragma unroll
     for (int t = 0; t < RGW_PF0; t++) {
         jj[t] = INT_MAX;
         bv[t] = V(0);
-        if (lo + t < hi) {
+        if (!skip && lo + t < hi) {
             jj[t] = colB[lo + t];
             bv[t] = valB[lo + t];
         }
@@ -467,8 +473,9 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
               int32_t *__restrict__ overflow, int64_t val_cap, int64_t val_done, int64_t *__restrict__ gcur,
               int64_t gcap) {
     const int64_t H = (int64_t)ctr[2];  // values per work item (k_rgw_count_num)
-    // this workgroup's global cursor area: positions [0, gcap), row ends [gcap, 2 gcap)
+    // this workgroup's global cursor area: positions [0, gcap), then (entries left, column) pairs
     int64_t *const gb = gcur + (int64_t)blockIdx.x * gcap * 2;
+    int2 *const gcn = reinterpret_cast<int2 *>(gb + gcap);
     __shared__ RgwNumLDS s;
     __shared__ int64_t s_rp[17];
     __shared__ RgwNumItem s_it;
@@ -593,10 +600,11 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                 // the next batch's head is loaded while this batch's products are added
                 int64_t plo, phi;
                 V pa;
+                bool pskip;
                 int32_t pjj[RGW_PF0];
                 V pbv[RGW_PF0];
-                rgn_head<V>(ea, eb, s_rp[0], first, s.cur, s.hi, gb, gcap, valA, colA, rpB, colB, valB, lo_col, plo,
-                            phi, pa, pjj, pbv);
+                rgn_head<V>(ea, eb, s_rp[0], first, s.cur, s.cn, gb, gcn, gcap, valA, colA, rpB, colB, valB, lo_col,
+                            hi_col, plo, phi, pa, pskip, pjj, pbv);
                 for (int64_t e0 = ea; e0 < eb; e0 += WAVE) {  // 64 A entries of row r at a time
                     // lane l owns entry e0 + l: its B segment in the sub-window,
                     // the first RGN_PF products' B values and output slots (each
@@ -612,7 +620,9 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                     const bool cached = in_lds || ci - RGN_CUR < gcap;
                     const int64_t lo = plo, hi = phi;
                     const V a = pa;
-                    int64_t next = lo;  // where the segment ends (the next sub-window's start)
+                    const bool skip = pskip;  // the cursor stays as it is
+                    int64_t next = lo;        // where the segment ends (the next sub-window's start)
+                    int32_t nxt = INT_MAX;    // and the column there
                     int sl[RGN_PF];
                     V bv[RGN_PF];
                     bool more = false, any = false;
@@ -668,10 +678,14 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                         more = lo + RGN_PF < hi && jj[RGN_PF - 1] < hi_col;
 #pragma unroll
                         for (int t = 0; t < RGN_PF; t++) next += (lo + t < hi && jj[t] < hi_col) ? 1 : 0;
+                        // the first loaded column past the sub-window (entries are in column order)
+#pragma unroll
+                        for (int t = RGN_PF - 1; t >= 0; t--)
+                            if (lo + t < hi && jj[t] >= hi_col) nxt = jj[t];
                     }
                     if (e0 + WAVE < eb)  // wave-uniform
-                        rgn_head<V>(e0 + WAVE, eb, s_rp[0], first, s.cur, s.hi, gb, gcap, valA, colA, rpB, colB, valB,
-                                    lo_col, plo, phi, pa, pjj, pbv);
+                        rgn_head<V>(e0 + WAVE, eb, s_rp[0], first, s.cur, s.cn, gb, gcn, gcap, valA, colA, rpB, colB,
+                                    valB, lo_col, hi_col, plo, phi, pa, pskip, pjj, pbv);
                     if (ANY) {  // every lane adds its staged products at once, in any order
 #pragma unroll
                         for (int t = 0; t < RGN_PF; t++)
@@ -694,6 +708,7 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                             const V sa = __shfl(a, sgi, WAVE);
                             const int64_t slo = __shfl(lo, sgi, WAVE) + RGN_PF, shi = __shfl(hi, sgi, WAVE);
                             int64_t f0 = slo;
+                            int32_t pj = INT_MAX;  // the column where the segment left the sub-window
                             for (; f0 < shi; f0 += WAVE) {  // wave-uniform trip
                                 const int64_t f = f0 + l;
                                 const int32_t j = f < shi ? colB[f] : INT_MAX;
@@ -702,20 +717,26 @@ k_rgw_numeric(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nr, c
                                 if (off >= 0) vals[off] = rgw_fma<V>(sa, b, vals[off]);
                                 const uint64_t past = __ballot(j >= hi_col);
                                 if (past) {  // the segment left the sub-window
-                                    f0 += __ffsll((unsigned long long)past) - 1;
+                                    const int q = __ffsll((unsigned long long)past) - 1;
+                                    pj = __shfl(j, q, WAVE);
+                                    f0 += q;
                                     break;
                                 }
                             }
-                            if (l == sgi) next = min(f0, shi);
+                            if (l == sgi) {
+                                next = min(f0, shi);
+                                nxt = next < shi ? pj : INT_MAX;
+                            }
                         }
                     }
-                    if (e < eb && cached) {
+                    if (e < eb && cached && !skip) {
+                        const int2 c = make_int2((int32_t)(hi - next), nxt);
                         if (in_lds) {
                             s.cur[ci] = next;
-                            s.hi[ci] = hi;
+                            s.cn[ci] = c;
                         } else {
                             gb[ci - RGN_CUR] = next;
-                            gb[gcap + ci - RGN_CUR] = hi;
+                            gcn[ci - RGN_CUR] = c;
                         }
                     }
                 }
