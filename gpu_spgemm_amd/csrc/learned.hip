@@ -59,6 +59,10 @@ struct Learned {
         const void *a = nullptr, *b = nullptr;
         int64_t key[8] = {};
         int64_t pred_tiles = 0, pred_nnz = 0;
+        // capacities for the next call of this pair (0 tiles: none yet): the
+        // larger of the predictions and the sizes the last call found
+        DeviceCaps caps;
+        int nbins = 0;
         uint64_t used = 0;  // 0 = empty
     };
     static constexpr int NREUSE = 4;
@@ -342,6 +346,64 @@ static int learned_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
     return TSG_OK;
 }
 
+// Capacities of a repeated product from a finished call: the larger of the
+// predicted and the exact sizes, 2 % more, and each bin's rows (a bin that was
+// empty stays closed: a row in it sends the call back to the exact pipeline).
+static void remember_caps(Learned::Reuse &ru, const Plan &p, int64_t numtile, int64_t nnz, int64_t pred_tiles,
+                          int64_t cap_nnz) {
+    auto grow = [](int64_t v, int64_t extra) { return v + v / 50 + extra; };
+    ru.caps.tiles = grow(std::max(numtile, pred_tiles), 64);
+    ru.caps.nnz = grow(std::max(nnz, cap_nnz), 4096);
+    ru.nbins = p.nbins;
+    for (int b = 0; b < MAX_BINS; b++) {
+        const int64_t rows = b < p.nbins ? p.bin_off[b + 1] - p.bin_off[b] : 0;
+        ru.caps.bins[b] = rows > 0 ? grow(rows, 64) : 0;
+    }
+}
+
+// A repeated product of a cached pair: C's arrays sized up front from the
+// remembered capacities, every step queued with device-side sizes
+// (spgemm_binned_device) and one host read at the end -- where the first call
+// (and the exact pipeline) read numtileC and nnz(C) between the steps.  Past
+// any capacity (a device flag) the result is discarded: *redo is set and the
+// caller runs the exact pipeline.
+static int learned_repeat(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
+                          const tsg_options *opts, const Learned::Reuse &ru, double scale, hipEvent_t *ev,
+                          bool *redo, int *syncs) {
+    *redo = false;
+    DeviceCaps caps = ru.caps;  // scale < 1: the test hook that forces the exact re-run
+    if (scale < 1.0) {
+        caps.tiles = std::max<int64_t>(1, (int64_t)((double)caps.tiles * scale));
+        caps.nnz = std::max<int64_t>(1, (int64_t)((double)caps.nnz * scale));
+        for (auto &b : caps.bins) b = (int64_t)((double)b * scale);
+    }
+    Plan p;
+    int64_t seq = 0;
+    ctx->device_sizes = true;
+    int rc = spgemm_binned_device(ctx, A, B, C, opts, caps, p, ev ? ev + 1 : nullptr, &seq);
+    ctx->device_sizes = false;
+    PlanFlags fl{};
+    if (!rc) {  // the flags, published before the numeric step (which runs meanwhile)
+        rc = publish_wait(ctx, seq, PLAN_FLAG_WORDS, ctx->pinned);
+        (*syncs)++;
+        memcpy(&fl, ctx->pinned, sizeof fl);
+    }
+    if (!rc && p.nbins != ru.nbins) rc = fail(TSG_ERR_ARG, "bin table changed between calls");
+    free_plan(ctx, p);  // stream-ordered: the queued numeric kernels run before any reuse
+    if (rc) {
+        free_c(ctx, C);
+        return rc;
+    }
+    if (fl.cap || fl.nnz > caps.nnz) {  // some capacity was short
+        free_c(ctx, C);
+        *redo = true;
+        return TSG_OK;
+    }
+    C->numtile = (int64_t)fl.pred_tiles;
+    C->nnz = fl.nnz;
+    return TSG_OK;
+}
+
 int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
                    const tsg_options *opts, tsg_stats *st) {
     Learned *L = state(ctx);
@@ -375,6 +437,47 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
     Learned::Reuse &ru = reuse ? *hit : *slot;
     ru.used = ++L->tick;
     if (timing && !rc) (void)hipEventRecord(ev[0], s);
+    if (!rc && reuse && ru.caps.tiles > 0 && tilem > 0 && (!opts || opts->numeric_variant == 0) &&
+        staged_ok(A, B, opts) && ru.caps.tiles <= INT32_MAX) {
+        bool redo = false;
+        rc = learned_repeat(ctx, A, B, C, opts, ru, pscale, timing ? ev : nullptr, &redo, &syncs);
+        if (!rc && !redo) {
+            if (timing) (void)hipEventRecord(ev[3], s);
+            if (st) {
+                st->numtileC = C->numtile;
+                st->nnzC = C->nnz;
+                st->host_syncs = syncs;
+                st->fallback_used = 0;
+                st->pred_numtileC = ru.caps.tiles;
+                st->pred_nnzC = ru.caps.nnz;
+                st->pred_reused = 1;
+                st->numeric_kernels = 0;
+            }
+            if (timing) {
+                if (st) {
+                    float a = 0, b = 0, c3 = 0;
+                    (void)hipEventSynchronize(ev[3]);
+                    (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+                    (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+                    (void)hipEventElapsedTime(&c3, ev[2], ev[3]);
+                    st->ms_step1 = a; st->ms_step2 = b; st->ms_step3 = c3; st->ms_total = a + b + c3;
+                }
+                for (auto &e : ev) (void)hipEventDestroy(e);
+            }
+            return TSG_OK;
+        }
+        if (!rc) {  // capacities were short: the exact pipeline, and its sizes remembered
+            fallback = 1;
+            ru.caps = DeviceCaps{};
+            C->m = A->m; C->n = B->n; C->tilem = A->tilem; C->tilen = B->tilen;
+            C->dtype = A->dtype; C->location = TSG_DEVICE;
+        }
+    }
+    if (rc) {
+        if (timing)
+            for (auto &e : ev) if (e) (void)hipEventDestroy(e);
+        return rc;
+    }
     if (!rc) rc = palloc(ctx, &C->tile_ptr, (size_t)tilem + 1);
     if (reuse) {  // the plain Step-1 count; the predictions are this operand pair's last ones
         if (!rc) rc = plan_count(ctx, A, B, C, opts, p, nullptr, !plan_tail_fused(tilem));
@@ -455,6 +558,9 @@ int spgemm_learned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix
         sl.val_done = cap;
         if (!rc && C->nnz > 0) rc = run_numeric(ctx, A, B, C, p, sl);
     }
+    if (!rc && fallback != 2)
+        remember_caps(ru, p, C->numtile, C->nnz,
+                      (int64_t)std::ceil((double)fl.pred_tiles * std::exp((double)L->off_nc)), cap);
     if (timing && !rc) (void)hipEventRecord(ev[3], s);
     if (st) {
         st->numtileC = C->numtile;

@@ -22,7 +22,7 @@ __device__ __forceinline__ int owner_of(const int32_t *seg, int n, int32_t i) {
 // take item idx as tile-row idx (nullptr list), which covers the same rows
 // once each and skips a dependent load.
 inline const int32_t *row_list(const int32_t *rows, const BinRange &br, int32_t tilem) {
-    return (!br.lo && br.begin == 0 && br.count == tilem) ? nullptr : rows;
+    return (br.all || (!br.lo && br.begin == 0 && br.count == tilem)) ? nullptr : rows;
 }
 
 __device__ __forceinline__ uint4 load_ptr16(const uint8_t *p, int64_t tile) {
@@ -102,6 +102,10 @@ struct Plan {
     const int64_t *d_bins = nullptr;
     int64_t bin_cap[MAX_BINS] = {};
     bool rl_any = false;  // BinTable::rl_any
+    // a device-sized product whose capacities leave one bin open with room for
+    // every tile-row: that bin holds them all (else the capacity check empties
+    // every bin), so its row list is the identity
+    int all_bin = -1;
     bool any_order = false;  // BinTable::any_order
     // device address of the capacity flag (kernels take it as int32_t *)
     int32_t *overflow() const { return flags ? &flags->cap : nullptr; }
@@ -154,6 +158,16 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
 // two launches and plan_count must have left tile_ptrC unscanned (scan =
 // false); else the separate scan, binning and read (plan_count scanned).
 bool plan_tail_fused(int32_t tilem);
+// Capacities of a device-sized product's bins.
+struct BinCaps {
+    int64_t v[MAX_BINS];
+};
+// The same without the host read (the header stays in p.hdr on the device);
+// with caps, the capacity check of the device-sized pipeline (numtileC against
+// cap_tiles, each bin's rows against its cover: past any of them the flag is
+// set and every bin emptied) and numtileC also written to *numtile_out.
+int plan_tail_device(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const int64_t *total1,
+                     const BinCaps *caps = nullptr, int64_t cap_tiles = 0, int64_t *numtile_out = nullptr);
 int plan_tail(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const int64_t *total1, int64_t *numtile,
               int64_t *t1);
 int run_symbolic(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, const tsg_tile_matrix *C,
@@ -183,8 +197,13 @@ struct DeviceCaps {
 // at the capacities (C->numtile, C->nnz hold them), and a product past any
 // capacity sets p.overflow() and leaves C undefined.  numtileC ends in
 // p.hdr[0] and nnzC in p.flags->nnz.  The plan and C stay allocated.
+// ev (optional): two events recorded after C's arrays are allocated and after
+// the symbolic step.  flags_seq (optional): the plan's flags (capacity flag,
+// nnz(C), numtileC in pred_tiles) published before the numeric step is queued;
+// wait with publish_wait(ctx, *flags_seq, PLAN_FLAG_WORDS, ...).
 int spgemm_binned_device(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                         const tsg_options *opts, const DeviceCaps &caps, Plan &p);
+                         const tsg_options *opts, const DeviceCaps &caps, Plan &p, hipEvent_t *ev = nullptr,
+                         int64_t *flags_seq = nullptr);
 
 // keep != nullptr: the plan (row bins, links) is handed over instead of freed,
 // for re-running the numeric step.

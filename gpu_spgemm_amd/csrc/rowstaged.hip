@@ -519,7 +519,8 @@ k_rows_reduce(const int64_t *__restrict__ tile_ptrC, const uint8_t *__restrict__
 __global__ void __launch_bounds__(RT_THREADS)
 k_rows_apply(int64_t *__restrict__ tile_ptrC, const uint8_t *__restrict__ bin, int64_t n, int nbins, int64_t nb,
              const int64_t *__restrict__ partial, const int32_t *__restrict__ bc, int32_t *__restrict__ rows,
-             const int64_t *__restrict__ total1, int64_t *__restrict__ hdr, int32_t *pub, int64_t seq) {
+             const int64_t *__restrict__ total1, int64_t *__restrict__ hdr, int32_t *pub, int64_t seq, bool guard,
+             BinCaps caps, int64_t cap_tiles, int32_t *__restrict__ overflow, int64_t *__restrict__ numtile_out) {
     __shared__ int64_t s_d[RT_TILE + RT_TILE / 32];
     __shared__ int64_t s_w[RT_THREADS / WAVE];
     __shared__ int64_t s_red[RT_THREADS / WAVE][2 * MAX_BINS + 2];
@@ -578,6 +579,15 @@ k_rows_apply(int64_t *__restrict__ tile_ptrC, const uint8_t *__restrict__ bin, i
     __syncthreads();
     if (blk == 0) {  // the header: device copy, then the host's words and sequence number
         const int nw = 2 * (nbins + 3);
+        if (guard) {  // past a capacity: the flag, and every bin emptied (no later kernel writes C)
+            bool over = s_hdr[0] > cap_tiles;
+            for (int b = 0; b < nbins; b++) over = over || s_hdr[3 + b] - s_hdr[2 + b] > caps.v[b];
+            if (over && threadIdx.x == 0) atomicOr(overflow, 1);
+            if (threadIdx.x == 0 && numtile_out) *numtile_out = s_hdr[0];
+            __syncthreads();
+            if (over && (int)threadIdx.x <= nbins) s_hdr[2 + threadIdx.x] = 0;
+            __syncthreads();
+        }
         if ((int)threadIdx.x < nbins + 3) hdr[threadIdx.x] = s_hdr[threadIdx.x];
         if (pub) {
             if ((int)threadIdx.x < nw) pub[threadIdx.x] = reinterpret_cast<const int32_t *>(s_hdr)[threadIdx.x];
@@ -1222,13 +1232,11 @@ bool plan_tail_fused(int32_t tilem) {
     return tilem > 0 && ((int64_t)tilem + RT_TILE - 1) / RT_TILE <= RT_MAX_BLOCKS;
 }
 
-int plan_tail(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const int64_t *total1, int64_t *numtile,
-              int64_t *t1) {
-    if (!plan_tail_fused(tilem)) {
-        TSG_CHECK(scan_inclusive_i64(ctx, tile_ptrC, (int64_t)tilem + 1));
-        TSG_CHECK(plan_bin(ctx, p, tilem, tile_ptrC + tilem, total1));
-        return plan_read(ctx, p, numtile, t1);
-    }
+// The fused tail's two launches; with publish, block 0 of the second also
+// sends the header to the host (seq: its number).
+static int rows_tail_launch(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const int64_t *total1,
+                            bool publish, int64_t *seq, const BinCaps *caps = nullptr, int64_t cap_tiles = 0,
+                            int64_t *numtile_out = nullptr) {
     hipStream_t s = ctx->stream;
     const int64_t nb = ((int64_t)tilem + RT_TILE - 1) / RT_TILE;
     int64_t *partial = nullptr;
@@ -1239,17 +1247,50 @@ int plan_tail(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const in
         k_rows_reduce<<<(unsigned)nb, RT_THREADS, 0, s>>>(tile_ptrC, p.bin, tilem, p.nbins, nb, partial, bc);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_rows_reduce launch");
     }
-    const int nw = 2 * (p.nbins + 3);
     if (!rc) {
-        const int64_t seq = publish_seq(ctx);
+        *seq = publish ? publish_seq(ctx) : 0;
         k_rows_apply<<<(unsigned)nb, RT_THREADS, 0, s>>>(tile_ptrC, p.bin, tilem, p.nbins, nb, partial, bc, p.rows,
-                                                          total1, p.hdr, ctx->pub_dev, seq);
+                                                          total1, p.hdr, publish ? ctx->pub_dev : nullptr, *seq,
+                                                          caps != nullptr, caps ? *caps : BinCaps{}, cap_tiles,
+                                                          p.overflow(), numtile_out);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_rows_apply launch");
-        if (!rc) rc = publish_wait(ctx, seq, nw, ctx->pinned);
     }
+    // stream-ordered reuse: later work on this stream runs after the kernels above
     pool_free(ctx, partial);
     pool_free(ctx, bc);
-    TSG_CHECK(rc);
+    return rc;
+}
+
+__global__ void k_device_guard(int64_t *__restrict__ hdr, int nbins, int64_t cap_tiles, BinCaps caps,
+                               int32_t *__restrict__ overflow);
+
+int plan_tail_device(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const int64_t *total1,
+                     const BinCaps *caps, int64_t cap_tiles, int64_t *numtile_out) {
+    if (!plan_tail_fused(tilem)) {
+        TSG_CHECK(scan_inclusive_i64(ctx, tile_ptrC, (int64_t)tilem + 1));
+        TSG_CHECK(plan_bin(ctx, p, tilem, tile_ptrC + tilem, total1));
+        if (caps) {
+            k_device_guard<<<1, 64, 0, ctx->stream>>>(p.hdr, p.nbins, cap_tiles, *caps, p.overflow());
+            TSG_HIP(hipGetLastError());
+            if (numtile_out)
+                TSG_HIP(hipMemcpyAsync(numtile_out, p.hdr, sizeof(int64_t), hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        return TSG_OK;
+    }
+    int64_t seq = 0;
+    return rows_tail_launch(ctx, p, tilem, tile_ptrC, total1, false, &seq, caps, cap_tiles, numtile_out);
+}
+
+int plan_tail(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_ptrC, const int64_t *total1, int64_t *numtile,
+              int64_t *t1) {
+    if (!plan_tail_fused(tilem)) {
+        TSG_CHECK(scan_inclusive_i64(ctx, tile_ptrC, (int64_t)tilem + 1));
+        TSG_CHECK(plan_bin(ctx, p, tilem, tile_ptrC + tilem, total1));
+        return plan_read(ctx, p, numtile, t1);
+    }
+    int64_t seq = 0;
+    TSG_CHECK(rows_tail_launch(ctx, p, tilem, tile_ptrC, total1, true, &seq));
+    TSG_CHECK(publish_wait(ctx, seq, 2 * (p.nbins + 3), ctx->pinned));
     if (numtile) *numtile = ctx->pinned[0];
     if (t1) *t1 = ctx->pinned[1];
     for (int b = 0; b <= p.nbins; b++) p.bin_off[b] = ctx->pinned[2 + b];
@@ -1309,6 +1350,7 @@ BinRange bin_range(const Plan &p, int b, int e) {
         for (int q = b; q < e; q++) r.count += p.bin_cap[q];
         r.lo = p.d_bins + b;
         r.hi = p.d_bins + e;
+        r.all = p.all_bin >= b && p.all_bin < e;
     } else {
         r.begin = p.bin_off[b];
         r.count = p.bin_off[e] - p.bin_off[b];
@@ -1502,9 +1544,6 @@ int spgemm_binned(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix 
 // (hdr[0]) against C's tile arrays and every bin's row count against the rows
 // its launches cover.  Past any of them the flag is set and every bin emptied,
 // so no later kernel writes C (the caller then computes the batch exactly).
-struct BinCaps {
-    int64_t v[MAX_BINS];
-};
 __global__ void k_device_guard(int64_t *__restrict__ hdr, int nbins, int64_t cap_tiles, BinCaps caps,
                                int32_t *__restrict__ overflow) {
     if (threadIdx.x != 0) return;
@@ -1516,7 +1555,8 @@ __global__ void k_device_guard(int64_t *__restrict__ hdr, int nbins, int64_t cap
 }
 
 int spgemm_binned_device(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B, tsg_tile_matrix *C,
-                         const tsg_options *opts, const DeviceCaps &caps, Plan &p) {
+                         const tsg_options *opts, const DeviceCaps &caps, Plan &p, hipEvent_t *ev,
+                         int64_t *flags_seq) {
     if (!ctx->device_sizes) return fail(TSG_ERR_ARG, "device-sized pipeline outside a graph build");
     if (A->tilem <= 0) return fail(TSG_ERR_ARG, "a device-sized product needs tile-rows");
     if (!binned_variant(opts) || !staged_ok(A, B, opts))
@@ -1528,16 +1568,21 @@ int spgemm_binned_device(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
     p = Plan{};
     const int32_t tilem = A->tilem;
     TSG_CHECK(palloc(ctx, &C->tile_ptr, (size_t)tilem + 1));
-    TSG_CHECK(plan_count(ctx, A, B, C, opts, p, nullptr));
-    TSG_CHECK(plan_bin(ctx, p, tilem, C->tile_ptr + tilem, nullptr));
+    TSG_CHECK(plan_count(ctx, A, B, C, opts, p, nullptr, !plan_tail_fused(tilem)));
     if (p.nbins > MAX_BINS) return fail(TSG_ERR_ARG, "bin table");
     BinCaps bc{};
+    int open = 0, last_open = -1;
     for (int b = 0; b < p.nbins; b++) {
         bc.v[b] = std::min<int64_t>(caps.bins[b], tilem);
         p.bin_cap[b] = bc.v[b];
+        if (bc.v[b] > 0) {
+            open++;
+            last_open = b;
+        }
     }
-    k_device_guard<<<1, 64, 0, ctx->stream>>>(p.hdr, p.nbins, caps.tiles, bc, p.overflow());
-    TSG_HIP(hipGetLastError());
+    if (open == 1 && bc.v[last_open] >= tilem) p.all_bin = last_open;
+    TSG_CHECK(plan_tail_device(ctx, p, tilem, C->tile_ptr, nullptr, &bc, caps.tiles,
+                               reinterpret_cast<int64_t *>(&p.flags->pred_tiles)));
     p.d_bins = p.hdr + 2;
     C->numtile = caps.tiles;
     TSG_CHECK(palloc(ctx, &C->tile_rowidx, (size_t)caps.tiles));
@@ -1545,6 +1590,7 @@ int spgemm_binned_device(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
     TSG_CHECK(palloc(ctx, &C->mask, (size_t)caps.tiles * 16));
     TSG_CHECK(palloc(ctx, &C->tile_csr_Ptr, (size_t)caps.tiles * 16));
     TSG_CHECK(palloc(ctx, &C->tile_nnz, (size_t)caps.tiles + 1));
+    if (ev) (void)hipEventRecord(ev[0], ctx->stream);
     SlotLayout sl;
     sl.overflow = p.overflow();
     sl.val_cap = caps.nnz;
@@ -1552,6 +1598,11 @@ int spgemm_binned_device(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_
     // C's value offsets over its numtileC tiles (hdr[0] = tile_ptrC[tilem])
     TSG_CHECK(scan_inclusive_i64(ctx, C->tile_nnz, caps.tiles + 1, true, &p.flags->nnz, C->tile_ptr + tilem));
     TSG_CHECK(prepare_numeric(ctx, A, C, p, sl));
+    if (ev) (void)hipEventRecord(ev[1], ctx->stream);
+    // every capacity check is done by now (numeric kernels write no value past
+    // caps.nnz, and nnz(C) is known): the flags can go to the host while the
+    // numeric step runs
+    if (flags_seq) TSG_CHECK(publish_start(ctx, p.flags, PLAN_FLAG_WORDS, flags_seq));
     C->nnz = caps.nnz;
     TSG_CHECK(pool_alloc(ctx, &C->tile_csr_Val, (size_t)std::max<int64_t>(caps.nnz, 1) * vsize(C->dtype)));
     TSG_CHECK(palloc(ctx, &C->tile_csr_Col, (size_t)std::max<int64_t>(caps.nnz, 1)));

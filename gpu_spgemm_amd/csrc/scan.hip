@@ -211,11 +211,17 @@ int publish_wait(Context *ctx, int64_t seq, int n, void *out) {
     return TSG_OK;
 }
 
-int read_words(Context *ctx, const void *dsrc, int n, void *out) {
+int publish_start(Context *ctx, const void *dsrc, int n, int64_t *seq) {
     if (n < 0 || n > PUB_WORDS) return fail(TSG_ERR_ARG, "read_words: at most %d words", PUB_WORDS);
-    const int64_t seq = publish_seq(ctx);
-    k_publish<<<1, PUB_WORDS, 0, ctx->stream>>>(static_cast<const int32_t *>(dsrc), n, ctx->pub_dev, seq);
+    *seq = publish_seq(ctx);
+    k_publish<<<1, PUB_WORDS, 0, ctx->stream>>>(static_cast<const int32_t *>(dsrc), n, ctx->pub_dev, *seq);
     TSG_HIP(hipGetLastError());
+    return TSG_OK;
+}
+
+int read_words(Context *ctx, const void *dsrc, int n, void *out) {
+    int64_t seq = 0;
+    TSG_CHECK(publish_start(ctx, dsrc, n, &seq));
     return publish_wait(ctx, seq, n, out);
 }
 

@@ -154,6 +154,9 @@ int read_words(Context *ctx, const void *dsrc, int n, void *out);
 // PUB_WORDS; publish_wait spins for that number and copies the words to out.
 int64_t publish_seq(Context *ctx);
 int publish_wait(Context *ctx, int64_t seq, int n, void *out);
+// read_words in two halves: the publishing kernel queued now (its number in
+// *seq), the wait (publish_wait) later, after more work has been queued.
+int publish_start(Context *ctx, const void *dsrc, int n, int64_t *seq);
 // Reads one int64 from device memory (blocking; counts as a host sync).
 int read_i64(Context *ctx, const int64_t *dptr, int64_t *out);
 int read_i32(Context *ctx, const int32_t *dptr, int32_t *out);
@@ -215,6 +218,7 @@ int launch_heavy_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_ma
 struct BinRange {
     int64_t begin = 0, count = 0;
     const int64_t *lo = nullptr, *hi = nullptr;
+    bool all = false;  // the range's rows are every tile-row in order (device range known to hold them all)
 };
 inline BinRange host_range(int64_t begin, int64_t count) {
     BinRange r;

@@ -275,3 +275,33 @@ def test_calibration_matches_oracle(trained):
     got = trained.learned_calibration()
     print(f"    calibration GPU {got} oracle {off}", flush=True)
     assert np.allclose(got, off, atol=1e-4)
+
+
+@pytest.mark.parametrize("name", ["lap2d_300x200", "er4000x10", "rmat12"])
+def test_learned_repeat_has_one_read(trained, name):
+    """A repeated product of the same operands: C is sized from the remembered
+    capacities (the larger of the predictions and the sizes the first call
+    found), every step is queued with device-side sizes and the call makes one
+    host read, where the first makes two; both equal the exact pipeline's C.
+    pred_scale 0.5 cuts those capacities: the repeat overflows on the device
+    and the call re-runs exactly (fallback 1), with the same result."""
+    A = {"lap2d_300x200": lambda: g.gen_laplace2d(300, 200), "er4000x10": lambda: g.gen_er(4000, 10, seed=31),
+         "rmat12": lambda: g.gen_rmat(12, 16, seed=6, dtype=g.F64)}[name]()
+    tA, tB = _pair(trained, A)
+    Ce, se = trained.spgemm(tA, tB)
+    e = Ce.download()
+    C1, s1 = trained.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
+    C2, s2 = trained.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
+    C3, s3 = trained.spgemm(tA, tB, scheduler=g.SCHED_LEARNED)
+    print(f"[case] learned repeat {name}: syncs {s1['host_syncs']} {s2['host_syncs']} {s3['host_syncs']} "
+          f"fallback {s1['fallback_used']} {s2['fallback_used']} {s3['fallback_used']}", flush=True)
+    for C, s in ((C1, s1), (C2, s2), (C3, s3)):
+        assert s["numtileC"] == se["numtileC"] and s["nnzC"] == se["nnzC"]
+        assert_tiles_equal(C.download(), e, keys=CKEYS, label=f"{name} ")
+    assert s2["host_syncs"] == 1 and s3["host_syncs"] == 1 and s2["fallback_used"] == 0
+    # capacities cut to half: the device flags the overflow, the call re-runs exactly
+    Ca, sa = trained.spgemm(tA, tB, scheduler=g.SCHED_LEARNED, pred_scale=0.5)
+    Cb, sb = trained.spgemm(tA, tB, scheduler=g.SCHED_LEARNED, pred_scale=0.5)
+    assert sb["fallback_used"] == 1 and sb["host_syncs"] >= 3
+    for C in (Ca, Cb):
+        assert_tiles_equal(C.download(), e, keys=CKEYS, label=f"{name} cut ")
