@@ -14,6 +14,7 @@
 // tile-row whose values exceed the LDS window runs in windows of whole C
 // tiles, the products walked once per window.
 #include <climits>
+#include <type_traits>
 
 #include "rowstaged.h"
 
@@ -22,16 +23,17 @@ namespace tsg {
 constexpr int RN_GROUPS = 4;   // tile-rows per wave; lane = (tile-row, scalar row)
 constexpr int RN_NC = 16;      // C tiles a tile-row may hold (the small staged variants' cap)
 constexpr int RN_VCAP = 208;   // C values of one tile-row held in LDS at a time (a 2-D 13-point row block: 208)
-constexpr int RN_EB = 6;       // A entries whose loads issue together (a 2-D 5-point row: 5)
-constexpr int RN_BB = 6;       // B row entries whose loads issue together (a 2-D 5-point row: 5)
+constexpr int RN_EB = 5;       // A entries whose loads issue together (a 2-D 5-point row: 5)
+constexpr int RN_BB = 5;       // B row entries whose loads issue together (a 2-D 5-point row: 5)
 
 template <typename V>
 struct RowNumLDS {
-    V val[RN_GROUPS][RN_VCAP + 16];  // + a slot per lane that takes the products outside the window
+    V val[RN_GROUPS][RN_VCAP + 1];  // + one slot for the products outside the window (never read)
     alignas(16) uint16_t cmask[RN_GROUPS][RN_NC * 16];  // row masks of the tile-row's C tiles
     alignas(16) uint8_t cptr[RN_GROUPS][RN_NC * 16];    // row offsets inside each C tile
     int32_t coff[RN_GROUPS][RN_NC + 1];                 // each C tile's first value, from the tile-row's first
     int32_t cj[RN_GROUPS][RN_NC];                       // C tile columns, ascending
+    uint8_t colb[RN_GROUPS][RN_VCAP + 1];               // col byte of each value
 };
 
 // Bits of a 16-bit row mask that stand for columns below c (bit 15-c is column c).
@@ -44,7 +46,21 @@ __device__ __forceinline__ double fma_rn<double>(double a, double b, double c) {
 template <>
 __device__ __forceinline__ float fma_rn<float>(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
-template <typename V>
+// A load at a 32-bit element index from a kernel-argument base: the address
+// is the base plus a 32-bit byte offset (no 64-bit address arithmetic).
+// Valid while the array holds less than 4 GB.
+template <typename T>
+__device__ __forceinline__ T ld_narrow(const T *__restrict__ base, uint32_t i) {
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T ld_idx(const T *__restrict__ base, uint32_t i) { return ld_narrow(base, i); }
+template <typename T>
+__device__ __forceinline__ T ld_idx(const T *__restrict__ base, int64_t i) { return base[i]; }
+
+// NARROW: every index into A's and B's scalar CSR arrays fits a 32-bit byte
+// offset (nnz and rows below 2^29), so positions are 32-bit.
+template <typename V, bool NARROW>
 __global__ void __launch_bounds__(WAVE)
 k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, const int64_t *__restrict__ dlo,
            const int64_t *__restrict__ dhi, int32_t m,
@@ -54,6 +70,7 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
            const int64_t *__restrict__ tile_nnzC, const uint16_t *__restrict__ maskC,
            const uint8_t *__restrict__ ptrC, V *__restrict__ valC, uint8_t *__restrict__ colC,
            int32_t *__restrict__ overflow, int64_t val_cap, int64_t val_done) {
+    using IX = typename std::conditional<NARROW, uint32_t, int64_t>::type;
     __shared__ RowNumLDS<V> s;
     const int lane = lane_id();
     const int g = lane >> 4, r = lane & 15;
@@ -63,10 +80,10 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
     // this lane's scalar row of A, its first RN_EB entries and their B row
     // ranges: issued before the C headers, so the two load chains overlap
     const int64_t gr = (int64_t)(bi < 0 ? 0 : bi) * 16 + r;
-    int64_t e0 = 0, e1 = 0;
+    IX e0 = 0, e1 = 0;
     if (bi >= 0 && gr < m) {
-        e0 = rpA[gr];
-        e1 = rpA[gr + 1];
+        e0 = (IX)ld_idx(rpA, (IX)gr);
+        e1 = (IX)ld_idx(rpA, (IX)(gr + 1));
     }
     int64_t cs = 0, ce = 0;
     if (bi >= 0) {
@@ -75,22 +92,22 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
     }
     int32_t kk[RN_EB];
     V aa[RN_EB];
-    int64_t j0[RN_EB];
+    IX j0[RN_EB];
     int32_t jn[RN_EB];  // B row lengths, 0 past the A row's end
     // A entries [eb, eb + RN_EB) and their B rows' ranges; loads past the
     // row's end repeat its last entry (in bounds, no branch) and are then
     // given an empty B row (eb < e1)
-    auto load_a = [&](int64_t eb) {
-        const int ne = (int)min<int64_t>(RN_EB, e1 - eb);
+    auto load_a = [&](IX eb) {
+        const int ne = (int)min<int64_t>(RN_EB, (int64_t)(e1 - eb));
 #pragma unroll
         for (int u = 0; u < RN_EB; u++) {
-            const int64_t e = eb + min(u, ne - 1);
-            kk[u] = colA[e];
-            aa[u] = valA[e];
+            const IX e = eb + (IX)min(u, ne - 1);
+            kk[u] = ld_idx(colA, e);
+            aa[u] = ld_idx(valA, e);
         }
 #pragma unroll
         for (int u = 0; u < RN_EB; u++) {
-            const int64_t b0 = rpB[kk[u]], b1 = rpB[kk[u] + 1];
+            const IX b0 = (IX)ld_idx(rpB, (IX)kk[u]), b1 = (IX)ld_idx(rpB, (IX)(kk[u] + 1));
             j0[u] = b0;
             jn[u] = u < ne ? (int32_t)(b1 - b0) : 0;
         }
@@ -155,7 +172,7 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
             const int32_t jlo = s.cj[g][w0], jhi = s.cj[g][w1 - 1];
             // A entries RN_EB at a time, their B rows RN_BB entries at a time:
             // each batch's loads are issued together (one latency a batch)
-            for (int64_t eb = e0; eb < e1; eb += RN_EB) {
+            for (IX eb = e0; eb < e1; eb += RN_EB) {
                 if (!pre) load_a(eb);
                 pre = false;
                 // RN_BB entries of B row u from its entry o (o < jn[u]); past
@@ -164,17 +181,19 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
                     const int n = jn[u] - o;
 #pragma unroll
                     for (int q = 0; q < RN_BB; q++) {
-                        const int64_t j = j0[u] + o + min(q, n - 1);
-                        c[q] = colB[j];
-                        b[q] = valB[j];
+                        const IX j = j0[u] + (IX)(o + min(q, n - 1));
+                        c[q] = ld_idx(colB, j);
+                        b[q] = ld_idx(valB, j);
                         if (q >= n) c[q] = INT_MAX;
                     }
                 };
                 // a batch's products into the window, in order; the current C
                 // tile's column, this row's first value in it and its row mask
-                // stay in registers while the B row stays in that tile
+                // stay in registers while the B row stays in that tile.  The
+                // output's col byte goes beside its value.
                 auto apply = [&](V a, const int32_t(&c)[RN_BB], const V(&b)[RN_BB], int &ct, int32_t &jc,
                                  int32_t &base, uint32_t &mk) {
+                    int pos[RN_BB];
 #pragma unroll
                     for (int q = 0; q < RN_BB; q++) {
                         const int32_t J = c[q] >> 4;
@@ -186,9 +205,20 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
                             base = s.coff[g][ct] - vb + s.cptr[g][t];
                             mk = s.cmask[g][t];
                         }
-                        // outside the window: the lane's own spare slot, never read
-                        const int pos = in ? base + __popc(mk & mask_below(c[q] & 15)) : RN_VCAP + r;
-                        s.val[g][pos] = fma_rn<V>(a, b[q], s.val[g][pos]);
+                        // outside the window: the group's spare slot, whose
+                        // contents (lanes race on it) are never read
+                        pos[q] = in ? base + __popc(mk & mask_below(c[q] & 15)) : RN_VCAP;
+                    }
+                    // a batch's columns are distinct (no row holds a column
+                    // twice on this path), so are its positions inside the
+                    // window: its reads issue together, then its writes
+                    V v[RN_BB];
+#pragma unroll
+                    for (int q = 0; q < RN_BB; q++) v[q] = s.val[g][pos[q]];
+#pragma unroll
+                    for (int q = 0; q < RN_BB; q++) {
+                        s.val[g][pos[q]] = fma_rn<V>(a, b[q], v[q]);
+                        s.colb[g][pos[q]] = (uint8_t)((r << 4) | (c[q] & 15));
                     }
                 };
                 // the first batch of B row u + 1 is loaded before row u's
@@ -220,21 +250,15 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
             }
         }
         wave_sync();
-        // the window's col bytes: lane r writes its row's run in each tile;
-        // its values: the group copies them 16 consecutive a step (coalesced)
+        // the window's values and col bytes (every position took at least
+        // one product): the group copies them 16 consecutive a step
         if (act) {
-            for (int ct = w0; ct < w1; ct++) {
-                const int q = ct * 16 + r;
-                uint32_t mr = s.cmask[g][q];
-                int o = s.coff[g][ct] + s.cptr[g][q];
-                while (mr) {
-                    const int c = __clz(mr) - 16;
-                    mr ^= 0x8000u >> c;
-                    colC[vs + o] = (uint8_t)((r << 4) | c);
-                    o++;
-                }
+            V *vo = valC + (vs + vb);
+            uint8_t *co8 = colC + (vs + vb);
+            for (int i = r; i < vn; i += 16) {
+                vo[i] = s.val[g][i];
+                co8[i] = s.colb[g][i];
             }
-            for (int i = r; i < vn; i += 16) valC[vs + vb + i] = s.val[g][i];
         }
         wave_sync();
         w0 = w1;
@@ -483,16 +507,27 @@ int launch_num_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matri
     if (count <= 0) return TSG_OK;
     if (sl.row_nc) return fail(TSG_ERR_ARG, "the row-lane numeric kernel runs on the exact C layout only");
     const unsigned grid = (unsigned)((count + RN_GROUPS - 1) / RN_GROUPS);
-    if (A->dtype == TSG_F32)
-        k_num_rows<float><<<grid, WAVE, 0, ctx->stream>>>(
-            row_list(rows, br, C->tilem), br.begin, count, br.lo, br.hi, A->m, A->row_ptr, A->row_col, (const float *)A->row_val, B->row_ptr, B->row_col,
-            (const float *)B->row_val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr,
-            (float *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done);
-    else
-        k_num_rows<double><<<grid, WAVE, 0, ctx->stream>>>(
-            row_list(rows, br, C->tilem), br.begin, count, br.lo, br.hi, A->m, A->row_ptr, A->row_col, (const double *)A->row_val, B->row_ptr, B->row_col,
-            (const double *)B->row_val, C->tile_ptr, C->tile_columnidx, C->tile_nnz, C->mask, C->tile_csr_Ptr,
-            (double *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, sl.val_done);
+    // 32-bit byte offsets reach every scalar CSR entry of A and B
+    const int64_t lim = (int64_t)1 << 29;
+    const bool narrow = A->nnz < lim && B->nnz < lim && A->m < lim && B->m < lim;
+#define TSG_NUM_ROWS(V, N)                                                                                        \
+    k_num_rows<V, N><<<grid, WAVE, 0, ctx->stream>>>(                                                             \
+        row_list(rows, br, C->tilem), br.begin, count, br.lo, br.hi, A->m, A->row_ptr, A->row_col,               \
+        (const V *)A->row_val, B->row_ptr, B->row_col, (const V *)B->row_val, C->tile_ptr, C->tile_columnidx,    \
+        C->tile_nnz, C->mask, C->tile_csr_Ptr, (V *)C->tile_csr_Val, C->tile_csr_Col, sl.overflow, sl.val_cap, \
+        sl.val_done)
+    if (A->dtype == TSG_F32) {
+        if (narrow)
+            TSG_NUM_ROWS(float, true);
+        else
+            TSG_NUM_ROWS(float, false);
+    } else {
+        if (narrow)
+            TSG_NUM_ROWS(double, true);
+        else
+            TSG_NUM_ROWS(double, false);
+    }
+#undef TSG_NUM_ROWS
     TSG_HIP(hipGetLastError());
     return TSG_OK;
 }
