@@ -283,7 +283,7 @@ constexpr int SR_WORDS = 32;  // bitmap: 1024 tile columns (a 2-D 5-point row sp
 // (a group takes one of the two ways).
 struct SymRowsGroup {
     alignas(16) uint16_t bmask[SR_NB * 16];
-    uint16_t cmask[SR_NC * 16];
+    alignas(16) uint16_t cmask[SR_NC * 16];
     union {
         uint32_t bm[SR_WORDS];  // bitmap way
         int32_t bj[SR_NB];      // fallback: the B tiles' columns
@@ -471,15 +471,25 @@ k_sym_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
             }
         }
     }
-    // masks, row offsets (a 16-lane scan per C tile) and tile counts
-    if (!ok) nc = 0;
-    for (int ct = 0; ct < nc; ct++) {  // uniform in the group
-        const uint32_t rm = G.cmask[ct * 16 + l];
-        maskC[(cs + ct) * 16 + l] = (uint16_t)rm;
-        const int p = __popc(rm);
-        const int inc = group16_incl_sum(p);
-        ptrRowC[(cs + ct) * 16 + l] = (uint8_t)(inc - p);
-        if (l == 15) tile_nnzC[cs + ct + 1] = inc;
+    wave_sync();
+    // lane l writes C tile l whole: its 16 row masks, their exclusive row
+    // offsets (step2_build_ptrRowC_kernel's scan) and its count
+    if (ok && l < nc) {
+        const uint4 m0 = reinterpret_cast<const uint4 *>(&G.cmask[l * 16])[0];
+        const uint4 m1 = reinterpret_cast<const uint4 *>(&G.cmask[l * 16])[1];
+        const uint32_t w[8] = {m0.x, m0.y, m0.z, m0.w, m1.x, m1.y, m1.z, m1.w};  // rows 2i (low), 2i+1 (high)
+        uint32_t pw[4] = {0u, 0u, 0u, 0u};
+        uint32_t acc = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            pw[i >> 2] |= acc << (8 * (i & 3));
+            acc += __popc((w[i >> 1] >> (16 * (i & 1))) & 0xFFFFu);
+        }
+        uint4 *mo = reinterpret_cast<uint4 *>(maskC + (cs + l) * 16);
+        mo[0] = m0;
+        mo[1] = m1;
+        reinterpret_cast<uint4 *>(ptrRowC + (cs + l) * 16)[0] = make_uint4(pw[0], pw[1], pw[2], pw[3]);
+        tile_nnzC[cs + l + 1] = acc;
     }
 }
 
