@@ -660,7 +660,7 @@ k_rows_apply(int64_t *__restrict__ tile_ptrC, const uint8_t *__restrict__ bin, i
 // ---------------------------------------------------------------- staged symbolic (fill + masks)
 template <typename Var>
 struct SymLDS {
-    uint32_t cmask[Var::NC * 16];
+    alignas(16) uint32_t cmask[Var::NC * 16];
     alignas(16) uint16_t amask[Var::NA * 16];  // 16-byte stores
     alignas(16) uint16_t bmask[Var::NB * 16];
     int32_t abs_[Var::NA];
@@ -799,15 +799,26 @@ k_sym_staged(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows,
         if (rm) atomicOr(&s.cmask[s.bct[b] * 16 + r], rm);
     }
     wave_sync();
-    // write: groups of 16 lanes = one C tile
-    for (int it = lane; it < nc * 16; it += WAVE) {
-        const int ct = it >> 4, r = it & 15;
-        const uint32_t rm = s.cmask[it];
-        maskC[(cs + ct) * 16 + r] = (uint16_t)rm;
-        const int pc = __popc(rm);
-        const int inc = group16_incl_sum(pc);
-        ptrRowC[(cs + ct) * 16 + r] = (uint8_t)(inc - pc);
-        if (r == 15) tile_nnzC[cs + ct + 1] = inc;
+    // write: lane ct writes C tile ct whole -- its 16 row masks (two 16-byte
+    // stores), their exclusive row offsets (one) and its count
+    for (int ct = lane; ct < nc; ct += WAVE) {
+        uint32_t mw[8], pw[4] = {0u, 0u, 0u, 0u}, acc = 0;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint4 m = reinterpret_cast<const uint4 *>(&s.cmask[ct * 16])[q];
+            mw[2 * q] = (m.x & 0xFFFFu) | (m.y << 16);
+            mw[2 * q + 1] = (m.z & 0xFFFFu) | (m.w << 16);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            pw[i >> 2] |= acc << (8 * (i & 3));
+            acc += __popc((mw[i >> 1] >> (16 * (i & 1))) & 0xFFFFu);
+        }
+        uint4 *mo = reinterpret_cast<uint4 *>(maskC + (cs + ct) * 16);
+        mo[0] = make_uint4(mw[0], mw[1], mw[2], mw[3]);
+        mo[1] = make_uint4(mw[4], mw[5], mw[6], mw[7]);
+        reinterpret_cast<uint4 *>(ptrRowC + (cs + ct) * 16)[0] = make_uint4(pw[0], pw[1], pw[2], pw[3]);
+        tile_nnzC[cs + ct + 1] = acc;
     }
 }
 
