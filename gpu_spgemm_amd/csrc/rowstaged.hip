@@ -296,13 +296,15 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     }
 }
 
-// Step-1 count of light tile-rows (at most 16 A tiles, 32 B tiles reached,
+// Step-1 count of light tile-rows (at most 32 A tiles, 64 B tiles reached,
 // C tile columns spanning less than 1024) at 16 lanes each, 4 a wave: the
-// wave-per-row kernel above leaves most of its lanes idle on them.  Each other
-// row's bin is set to BIN_REST for that kernel (a mark, not a list: appending
-// to a list through one counter serialised on rows that are all heavy).
-// Results and statistics are the ones the wave-per-row kernel computes.
-constexpr int RL_NA = 16, RL_NB = 32, RL_WORDS = 32;
+// wave-per-row kernel above leaves most of its lanes idle on them.  Lane l
+// takes A tiles l and l + 16 and B tiles l + 16 h.  Each other row's bin is
+// set to BIN_REST for that kernel (a mark, not a list: appending to a list
+// through one counter serialised on rows that are all heavy).  Results and
+// statistics are the ones the wave-per-row kernel computes.
+constexpr int RL_NA = 32, RL_NB = 64, RL_WORDS = 32;
+constexpr int RL_APL = RL_NA / 16, RL_BPL = RL_NB / 16;  // A and B tiles a lane
 __global__ void __launch_bounds__(WAVE)
 k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
@@ -322,29 +324,51 @@ k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     bool light = row && na <= RL_NA;
     // every lane of the group goes through the group operations; `light`
     // (uniform in the group) gates memory traffic
-    int4 lk = make_int4(0, 0, 0, 0);
-    if (light && l < na) lk = link[as + l];
+    // A and B tiles a lane that some group of the wave needs (uniform)
+    const int apl = __any(light && na > 16) ? RL_APL : 1;
+    int4 lk[RL_APL];
+#pragma unroll
+    for (int h = 0; h < RL_APL; h++) {
+        lk[h] = make_int4(0, 0, 0, 0);
+        if (h < apl && light && l + 16 * h < na) lk[h] = link[as + l + 16 * h];
+    }
     // the row's A entries (its bin needs them), loaded with the links
     int64_t nz = 0;
     if (light && l < 2) nz = tile_nnzA[l == 0 ? as : ae];
     const int64_t pa = __shfl(nz, 1, 16) - __shfl(nz, 0, 16);
-    const int incl = group16_incl_sum(lk.y);
-    const int nb = __shfl(incl, 15, 16);
-    const int64_t pbi = group16_incl_sum<int64_t>((int64_t)lk.z);
-    const int64_t pb = __shfl(pbi, 15, 16);
+    // B tiles before each A tile (inclusive prefix in A tile order l, then l + 16)
+    int incl[RL_APL];
+    int nb = 0;
+    int64_t pb = 0;
+#pragma unroll
+    for (int h = 0; h < RL_APL; h++) {
+        incl[h] = nb;
+        if (h < apl) {
+            incl[h] = group16_incl_sum(lk[h].y) + nb;
+            nb = __shfl(incl[h], 15, 16);
+            pb += __shfl(group16_incl_sum<int64_t>((int64_t)lk[h].z), 15, 16);
+        }
+    }
     light = light && nb <= RL_NB;
-    if (light && l < na) {
-        s_pre[g][l + 1] = incl;
-        s_bs[g][l] = lk.x;
+    if (light) {
+#pragma unroll
+        for (int h = 0; h < RL_APL; h++) {
+            const int t = l + 16 * h;
+            if (t < na) {
+                s_pre[g][t + 1] = incl[h];
+                s_bs[g][t] = lk[h].x;
+            }
+        }
     }
     if (light && l == 0) s_pre[g][0] = 0;
     wave_sync();
-    int32_t j[2] = {0, 0};
+    int32_t j[RL_BPL] = {};
     int64_t sumspan = 0;
     int32_t cmn = -INT_MAX, cmx = -1;  // cmn: the negated minimum
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
+    for (int h = 0; h < RL_BPL; h++) {
         const int t = l + 16 * h;
+        if (h > 0 && !__any(light && t < nb)) break;  // no group has B tiles this far
         if (light && t < nb) {
             const int a = owner_of(s_pre[g], (int)na, t);
             const int pos = t - s_pre[g][a];
@@ -363,7 +387,7 @@ k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     wave_sync();
     if (light)
 #pragma unroll
-        for (int h = 0; h < 2; h++)
+        for (int h = 0; h < RL_BPL; h++)
             if (l + 16 * h < nb) atomicOr(&s_bm[g][(j[h] - cmin) >> 5], 1u << ((j[h] - cmin) & 31));
     wave_sync();
     int pc = 0;
@@ -375,8 +399,15 @@ k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     if (stats) {
         const int64_t ss = group16_incl_sum<int64_t>(sumspan);
         f_sumspan = __shfl(ss, 15, 16);
-        f_maxlen = group16_max(l < na ? lk.y : 0);
-        f_nonempty = __shfl(group16_incl_sum((light && l < na && lk.y > 0) ? 1 : 0), 15, 16);
+        int ml = 0, ne = 0;
+#pragma unroll
+        for (int h = 0; h < RL_APL; h++) {
+            const bool in = l + 16 * h < na;
+            ml = max(ml, in ? lk[h].y : 0);
+            ne += (light && in && lk[h].y > 0) ? 1 : 0;
+        }
+        f_maxlen = group16_max(ml);
+        f_nonempty = __shfl(group16_incl_sum(ne), 15, 16);
     }
     if (l == 0 && row) {
         if (light)
@@ -470,7 +501,7 @@ __global__ void k_bin_header(const int64_t *__restrict__ counts, int nbins, int6
 // tile-rows; the apply kernel's blocks add up the earlier blocks' partials
 // themselves (at most RT_MAX_BLOCKS of them).
 constexpr int RT_THREADS = 256;
-constexpr int RT_ITEMS = 8;
+constexpr int RT_ITEMS = 4;
 constexpr int RT_TILE = RT_THREADS * RT_ITEMS;
 constexpr int64_t RT_MAX_BLOCKS = 2048;
 
