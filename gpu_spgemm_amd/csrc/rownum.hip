@@ -251,13 +251,34 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
         }
         wave_sync();
         // the window's values and col bytes (every position took at least
-        // one product): the group copies them 16 consecutive a step
+        // one product): the group copies them out in 16-byte words of values
+        // and 4-byte words of col bytes, the unaligned ends one at a time
         if (act) {
             V *vo = valC + (vs + vb);
             uint8_t *co8 = colC + (vs + vb);
-            for (int i = r; i < vn; i += 16) {
-                vo[i] = s.val[g][i];
-                co8[i] = s.colb[g][i];
+            constexpr int K = 16 / (int)sizeof(V);
+            const int h = min(vn, (int)(((16 - ((uintptr_t)vo & 15)) & 15) / sizeof(V)));
+            const int nv = (vn - h) / K, tv = h + nv * K;
+            if (r < h) vo[r] = s.val[g][r];
+            if (tv + r < vn) vo[tv + r] = s.val[g][tv + r];
+            for (int w = r; w < nv; w += 16) {
+                const int i = h + w * K;
+                V t[K];
+#pragma unroll
+                for (int k = 0; k < K; k++) t[k] = s.val[g][i + k];
+                uint4 u;
+                __builtin_memcpy(&u, t, 16);
+                *reinterpret_cast<uint4 *>(vo + i) = u;
+            }
+            const int hb = min(vn, (int)((4 - ((uintptr_t)co8 & 3)) & 3));
+            const int nw = (vn - hb) / 4, tb = hb + nw * 4;
+            if (r < hb) co8[r] = s.colb[g][r];
+            if (tb + r < vn) co8[tb + r] = s.colb[g][tb + r];
+            for (int w = r; w < nw; w += 16) {
+                const int i = hb + w * 4;
+                const uint32_t word = (uint32_t)s.colb[g][i] | ((uint32_t)s.colb[g][i + 1] << 8) |
+                                      ((uint32_t)s.colb[g][i + 2] << 16) | ((uint32_t)s.colb[g][i + 3] << 24);
+                *reinterpret_cast<uint32_t *>(co8 + i) = word;
             }
         }
         wave_sync();
