@@ -58,6 +58,34 @@ __device__ __forceinline__ T ld_idx(const T *__restrict__ base, uint32_t i) { re
 template <typename T>
 __device__ __forceinline__ T ld_idx(const T *__restrict__ base, int64_t i) { return base[i]; }
 
+// NQ consecutive entries from element j of a 16-byte aligned array, read as
+// whole aligned 16-byte words: only words holding one of the first ne (>= 1)
+// entries are read (the others repeat the first word's address), so every
+// read lies in a word the array occupies.  Entries past ne are the words'
+// other contents and must not be used.
+template <typename T, int NQ, typename IX>
+__device__ __forceinline__ void ld_run(const T *__restrict__ base, IX j, int ne, T (&out)[NQ]) {
+    constexpr int PER = 16 / (int)sizeof(T);
+    constexpr int NCH = (PER - 1 + NQ + PER - 1) / PER;
+    const int sh = (int)(j & (IX)(PER - 1));
+    const IX a0 = j - (IX)sh;
+    T w[NCH * PER];
+#pragma unroll
+    for (int k = 0; k < NCH; k++) {
+        const IX ak = PER * k < sh + ne ? a0 + (IX)(PER * k) : a0;
+        const uint4 u = ld_idx(reinterpret_cast<const uint4 *>(base), (IX)(ak / (IX)PER));
+        __builtin_memcpy(&w[k * PER], &u, 16);
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        T v = w[q];
+#pragma unroll
+        for (int t = 1; t < PER; t++)
+            if (sh == t) v = w[q + t];
+        out[q] = v;
+    }
+}
+
 // NARROW: every index into A's and B's scalar CSR arrays fits a 32-bit byte
 // offset (nnz and rows below 2^29), so positions are 32-bit.
 template <typename V, bool NARROW>
@@ -94,17 +122,16 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
     V aa[RN_EB];
     IX j0[RN_EB];
     int32_t jn[RN_EB];  // B row lengths, 0 past the A row's end
-    // A entries [eb, eb + RN_EB) and their B rows' ranges; loads past the
-    // row's end repeat its last entry (in bounds, no branch) and are then
-    // given an empty B row (eb < e1)
+    // A entries [eb, eb + RN_EB) (read as aligned 16-byte words) and their B
+    // rows' ranges; entries past the row's end are given an empty B row
+    // (eb < e1)
     auto load_a = [&](IX eb) {
         const int ne = (int)min<int64_t>(RN_EB, (int64_t)(e1 - eb));
+        ld_run<int32_t, RN_EB>(colA, eb, ne, kk);
+        ld_run<V, RN_EB>(valA, eb, ne, aa);
 #pragma unroll
-        for (int u = 0; u < RN_EB; u++) {
-            const IX e = eb + (IX)min(u, ne - 1);
-            kk[u] = ld_idx(colA, e);
-            aa[u] = ld_idx(valA, e);
-        }
+        for (int u = 0; u < RN_EB; u++)
+            if (u >= ne) kk[u] = 0;  // past the row's end: B row 0, taken as empty
 #pragma unroll
         for (int u = 0; u < RN_EB; u++) {
             const IX b0 = (IX)ld_idx(rpB, (IX)kk[u]), b1 = (IX)ld_idx(rpB, (IX)(kk[u] + 1));
@@ -175,17 +202,16 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
             for (IX eb = e0; eb < e1; eb += RN_EB) {
                 if (!pre) load_a(eb);
                 pre = false;
-                // RN_BB entries of B row u from its entry o (o < jn[u]); past
-                // the row's end its last entry again, column INT_MAX
+                // RN_BB entries of B row u from its entry o (o < jn[u]), read as
+                // aligned 16-byte words; past the row's end column INT_MAX
                 auto load_b = [&](int u, int o, int32_t(&c)[RN_BB], V(&b)[RN_BB]) {
                     const int n = jn[u] - o;
+                    const IX j = j0[u] + (IX)o;
+                    ld_run<int32_t, RN_BB>(colB, j, min(n, RN_BB), c);
+                    ld_run<V, RN_BB>(valB, j, min(n, RN_BB), b);
 #pragma unroll
-                    for (int q = 0; q < RN_BB; q++) {
-                        const IX j = j0[u] + (IX)(o + min(q, n - 1));
-                        c[q] = ld_idx(colB, j);
-                        b[q] = ld_idx(valB, j);
+                    for (int q = 0; q < RN_BB; q++)
                         if (q >= n) c[q] = INT_MAX;
-                    }
                 };
                 // a batch's products into the window, in order; the current C
                 // tile's column, this row's first value in it and its row mask
@@ -538,6 +564,11 @@ int launch_num_rows(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matri
     if (count <= 0) return TSG_OK;
     if (sl.row_nc) return fail(TSG_ERR_ARG, "the row-lane numeric kernel runs on the exact C layout only");
     const unsigned grid = (unsigned)((count + RN_GROUPS - 1) / RN_GROUPS);
+    // the kernel reads A's and B's scalar CSR entries as aligned 16-byte words
+    // (the library allocates them on 256-byte boundaries)
+    auto al16 = [](const void *q) { return ((uintptr_t)q & 15) == 0; };
+    if (!al16(A->row_col) || !al16(A->row_val) || !al16(B->row_col) || !al16(B->row_val))
+        return fail(TSG_ERR_ARG, "row-lane numeric: scalar CSR arrays must start on 16-byte boundaries");
     // 32-bit byte offsets reach every scalar CSR entry of A and B
     const int64_t lim = (int64_t)1 << 29;
     const bool narrow = A->nnz < lim && B->nnz < lim && A->m < lim && B->m < lim;
