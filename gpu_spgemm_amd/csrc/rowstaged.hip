@@ -297,23 +297,42 @@ k_row_stats(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
 }
 
 // Step-1 count of light tile-rows (at most 32 A tiles, 64 B tiles reached,
-// C tile columns spanning less than 1024) at 16 lanes each, 4 a wave: the
-// wave-per-row kernel above leaves most of its lanes idle on them.  Lane l
-// takes A tiles l and l + 16 and B tiles l + 16 h.  Each other row's bin is
-// set to BIN_REST for that kernel (a mark, not a list: appending to a list
+// C tile columns spanning less than 1024) at RL_GL = 8 lanes each, 8 a wave:
+// the wave-per-row kernel above leaves most of its lanes idle on them.  Lane
+// l takes A tiles l + 8 h and B tiles l + 8 h; the passes past the first are
+// skipped when no group of the wave needs them.  Each other row's bin is set
+// to BIN_REST for that kernel (a mark, not a list: appending to a list
 // through one counter serialised on rows that are all heavy).  Results and
 // statistics are the ones the wave-per-row kernel computes.
 constexpr int RL_NA = 32, RL_NB = 64, RL_WORDS = 32;
-constexpr int RL_APL = RL_NA / 16, RL_BPL = RL_NB / 16;  // A and B tiles a lane
+constexpr int RL_GL = 8, RL_GROUPS = WAVE / RL_GL;           // lanes a tile-row, tile-rows a wave
+constexpr int RL_APL = RL_NA / RL_GL, RL_BPL = RL_NB / RL_GL;  // A and B tiles a lane
+
+template <typename T>
+__device__ __forceinline__ T group8_incl_sum(T v) {
+    const int l = lane_id() & (RL_GL - 1);
+#pragma unroll
+    for (int d = 1; d < RL_GL; d <<= 1) {
+        const T o = __shfl_up(v, d, RL_GL);
+        if (l >= d) v += o;
+    }
+    return v;
+}
+__device__ __forceinline__ int group8_max(int v) {
+#pragma unroll
+    for (int d = RL_GL / 2; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d, RL_GL));
+    return v;
+}
+
 __global__ void __launch_bounds__(WAVE)
 k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ tile_nnzA, int32_t tilemA,
             const int4 *__restrict__ link, const int32_t *__restrict__ colidxB, BinTable bt,
             int64_t *__restrict__ tile_ptrC, uint8_t *__restrict__ bin, int32_t *__restrict__ stats) {
-    __shared__ uint32_t s_bm[4][RL_WORDS];
-    __shared__ int32_t s_bs[4][RL_NA];
-    __shared__ int32_t s_pre[4][RL_NA + 1];
-    const int lane = lane_id(), g = lane >> 4, l = lane & 15;
-    const int32_t bi = blockIdx.x * 4 + g;
+    __shared__ uint32_t s_bm[RL_GROUPS][RL_WORDS];
+    __shared__ int32_t s_bs[RL_GROUPS][RL_NA];
+    __shared__ int32_t s_pre[RL_GROUPS][RL_NA + 1];
+    const int lane = lane_id(), g = lane / RL_GL, l = lane % RL_GL;
+    const int32_t bi = blockIdx.x * RL_GROUPS + g;
     const bool row = bi < tilemA;
     int64_t as = 0, ae = 0;
     if (row) {
@@ -324,36 +343,34 @@ k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     bool light = row && na <= RL_NA;
     // every lane of the group goes through the group operations; `light`
     // (uniform in the group) gates memory traffic
-    // A and B tiles a lane that some group of the wave needs (uniform)
-    const int apl = __any(light && na > 16) ? RL_APL : 1;
     int4 lk[RL_APL];
 #pragma unroll
     for (int h = 0; h < RL_APL; h++) {
         lk[h] = make_int4(0, 0, 0, 0);
-        if (h < apl && light && l + 16 * h < na) lk[h] = link[as + l + 16 * h];
+        if (light && l + RL_GL * h < na) lk[h] = link[as + l + RL_GL * h];
     }
     // the row's A entries (its bin needs them), loaded with the links
     int64_t nz = 0;
     if (light && l < 2) nz = tile_nnzA[l == 0 ? as : ae];
-    const int64_t pa = __shfl(nz, 1, 16) - __shfl(nz, 0, 16);
-    // B tiles before each A tile (inclusive prefix in A tile order l, then l + 16)
+    const int64_t pa = __shfl(nz, 1, RL_GL) - __shfl(nz, 0, RL_GL);
+    // B tiles before each A tile's end (A tile order l, l + 8, ...)
     int incl[RL_APL];
     int nb = 0;
     int64_t pb = 0;
 #pragma unroll
     for (int h = 0; h < RL_APL; h++) {
         incl[h] = nb;
-        if (h < apl) {
-            incl[h] = group16_incl_sum(lk[h].y) + nb;
-            nb = __shfl(incl[h], 15, 16);
-            pb += __shfl(group16_incl_sum<int64_t>((int64_t)lk[h].z), 15, 16);
+        if (h == 0 || __any(light && na > RL_GL * h)) {  // wave-uniform
+            incl[h] = group8_incl_sum(lk[h].y) + nb;
+            nb = __shfl(incl[h], RL_GL - 1, RL_GL);
+            pb += __shfl(group8_incl_sum<int64_t>((int64_t)lk[h].z), RL_GL - 1, RL_GL);
         }
     }
     light = light && nb <= RL_NB;
     if (light) {
 #pragma unroll
         for (int h = 0; h < RL_APL; h++) {
-            const int t = l + 16 * h;
+            const int t = l + RL_GL * h;
             if (t < na) {
                 s_pre[g][t + 1] = incl[h];
                 s_bs[g][t] = lk[h].x;
@@ -367,7 +384,7 @@ k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
     int32_t cmn = -INT_MAX, cmx = -1;  // cmn: the negated minimum
 #pragma unroll
     for (int h = 0; h < RL_BPL; h++) {
-        const int t = l + 16 * h;
+        const int t = l + RL_GL * h;
         if (h > 0 && !__any(light && t < nb)) break;  // no group has B tiles this far
         if (light && t < nb) {
             const int a = owner_of(s_pre[g], (int)na, t);
@@ -380,34 +397,34 @@ k_row_light(const int64_t *__restrict__ tile_ptrA, const int64_t *__restrict__ t
             cmx = max(cmx, j[h]);
         }
     }
-    const int32_t cmin = -group16_max(cmn), cmax = group16_max(cmx);
+    const int32_t cmin = -group8_max(cmn), cmax = group8_max(cmx);
     light = light && (nb == 0 || cmax - cmin < RL_WORDS * 32);
     if (light)
-        for (int i = l; i < RL_WORDS; i += 16) s_bm[g][i] = 0u;
+        for (int i = l; i < RL_WORDS; i += RL_GL) s_bm[g][i] = 0u;
     wave_sync();
     if (light)
 #pragma unroll
         for (int h = 0; h < RL_BPL; h++)
-            if (l + 16 * h < nb) atomicOr(&s_bm[g][(j[h] - cmin) >> 5], 1u << ((j[h] - cmin) & 31));
+            if (l + RL_GL * h < nb) atomicOr(&s_bm[g][(j[h] - cmin) >> 5], 1u << ((j[h] - cmin) & 31));
     wave_sync();
     int pc = 0;
     if (light)
 #pragma unroll
-        for (int i = 0; i < RL_WORDS / 16; i++) pc += __popc(s_bm[g][l * (RL_WORDS / 16) + i]);
-    const int cnt = __shfl(group16_incl_sum(pc), 15, 16);
+        for (int i = 0; i < RL_WORDS / RL_GL; i++) pc += __popc(s_bm[g][l * (RL_WORDS / RL_GL) + i]);
+    const int cnt = __shfl(group8_incl_sum(pc), RL_GL - 1, RL_GL);
     int64_t f_sumspan = 0, f_maxlen = 0, f_nonempty = 0;
     if (stats) {
-        const int64_t ss = group16_incl_sum<int64_t>(sumspan);
-        f_sumspan = __shfl(ss, 15, 16);
+        const int64_t ss = group8_incl_sum<int64_t>(sumspan);
+        f_sumspan = __shfl(ss, RL_GL - 1, RL_GL);
         int ml = 0, ne = 0;
 #pragma unroll
         for (int h = 0; h < RL_APL; h++) {
-            const bool in = l + 16 * h < na;
+            const bool in = l + RL_GL * h < na;
             ml = max(ml, in ? lk[h].y : 0);
             ne += (light && in && lk[h].y > 0) ? 1 : 0;
         }
-        f_maxlen = group16_max(ml);
-        f_nonempty = __shfl(group16_incl_sum(ne), 15, 16);
+        f_maxlen = group8_max(ml);
+        f_nonempty = __shfl(group8_incl_sum(ne), RL_GL - 1, RL_GL);
     }
     if (l == 0 && row) {
         if (light)
@@ -1250,9 +1267,9 @@ int plan_count(Context *ctx, const tsg_tile_matrix *A, const tsg_tile_matrix *B,
         if (!rc)
             rc = zero_small(ctx, A->numtile > 0 ? nullptr : p.overflow(), PLAN_FLAG_WORDS,
                             reinterpret_cast<int64_t *>(p.hb.top), p.hb.top ? 1 : 0);
-        // light tile-rows at 16 lanes each; the rest (marked) a wave each
+        // light tile-rows at 8 lanes each; the rest (marked) a wave each
         if (!rc) {
-            k_row_light<<<(unsigned)((tilem + 3) / 4), WAVE, 0, ctx->stream>>>(
+            k_row_light<<<(unsigned)((tilem + RL_GROUPS - 1) / RL_GROUPS), WAVE, 0, ctx->stream>>>(
                 A->tile_ptr, A->tile_nnz, A->tilem, p.link, B->tile_columnidx, bt, C->tile_ptr, p.bin, stats_out);
             if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_row_light launch");
         }
