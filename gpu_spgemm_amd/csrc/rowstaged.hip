@@ -568,7 +568,8 @@ __global__ void __launch_bounds__(RT_THREADS)
 k_rows_apply(int64_t *__restrict__ tile_ptrC, const uint8_t *__restrict__ bin, int64_t n, int nbins, int64_t nb,
              const int64_t *__restrict__ partial, const int32_t *__restrict__ bc, int32_t *__restrict__ rows,
              const int64_t *__restrict__ total1, int64_t *__restrict__ hdr, int32_t *pub, int64_t seq, bool guard,
-             BinCaps caps, int64_t cap_tiles, int32_t *__restrict__ overflow, int64_t *__restrict__ numtile_out) {
+             BinCaps caps, int64_t cap_tiles, int32_t *__restrict__ overflow, int64_t *__restrict__ numtile_out,
+             int nstaged) {
     __shared__ int64_t s_d[RT_TILE + RT_TILE / 32];
     __shared__ int64_t s_w[RT_THREADS / WAVE];
     __shared__ int64_t s_red[RT_THREADS / WAVE][2 * MAX_BINS + 2];
@@ -677,7 +678,16 @@ k_rows_apply(int64_t *__restrict__ tile_ptrC, const uint8_t *__restrict__ bin, i
         const int64_t idx = base + li;
         if (idx < n) tile_ptrC[idx + 1] = s_d[pad(li)];
     }
-    // stable partition: rows of bin b ascending from s_base[b]
+    // stable partition: rows of bin b ascending from s_base[b].  With host
+    // sizes, a staged bin (b < nstaged: their launches go through row_list)
+    // holding every tile-row needs no list: its kernels take item idx as
+    // tile-row idx.  The C-tile and row-by-row bins, and device sizes, always
+    // read the list.
+    if (!guard) {
+        bool whole = false;
+        for (int bb = 0; bb < nbins && bb < nstaged; bb++) whole = whole || s_hdr[3 + bb] - s_hdr[2 + bb] == n;
+        if (whole) return;
+    }
     for (int round = 0; round < RT_ITEMS; round++) {
         const int64_t idx = base + round * RT_THREADS + threadIdx.x;
         const int b = idx < n ? bin[idx] : 0xFF;
@@ -1311,7 +1321,7 @@ static int rows_tail_launch(Context *ctx, Plan &p, int32_t tilem, int64_t *tile_
         k_rows_apply<<<(unsigned)nb, RT_THREADS, 0, s>>>(tile_ptrC, p.bin, tilem, p.nbins, nb, partial, bc, p.rows,
                                                           total1, p.hdr, publish ? ctx->pub_dev : nullptr, *seq,
                                                           caps != nullptr, caps ? *caps : BinCaps{}, cap_tiles,
-                                                          p.overflow(), numtile_out);
+                                                          p.overflow(), numtile_out, p.nvar);
         if (hipGetLastError() != hipSuccess) rc = fail(TSG_ERR_HIP, "k_rows_apply launch");
     }
     // stream-ordered reuse: later work on this stream runs after the kernels above
