@@ -319,9 +319,9 @@ k_num_rows(const int32_t *__restrict__ rows, int64_t row_begin, int64_t nrows, c
 // two B tiles a lane for the C tile columns (a 1024-column LDS bitmap, ranks
 // by popcount prefix), then lane r builds row r of every C tile -- the OR of
 // B rows k selected by A's row mask r, as step2_symbolic_kernel
-// (src/spgemm_raw_kernels.cuh:105-163) -- with no atomics, and the per-tile
-// row offsets and counts by a 16-lane scan (step2_build_ptrRowC_kernel,
-// :166-215).
+// (src/spgemm_raw_kernels.cuh:105-163) -- with no atomics; then lane t
+// writes C tile t whole: its row masks, its exclusive row offsets
+// (step2_build_ptrRowC_kernel, :166-215) and its count.
 constexpr int SR_NA = 8, SR_NB = 32, SR_NC = 16;  // R's caps
 constexpr int SR_BPL = SR_NB / 16;                 // B tiles a lane
 constexpr int SR_WORDS = 32;  // bitmap: 1024 tile columns (a 2-D 5-point row spans 513)
